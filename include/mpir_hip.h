@@ -1,0 +1,173 @@
+/*
+ * mpir_hip.h -- C ABI of the MI355X (gfx950) implementation of the
+ * yex33/MPIR solve path for quasi-definite systems K = [H A^T; A -D]:
+ * fill-reducing ordering, QDLDL elimination tree, LDL^T factorization and
+ * GMRES-IR with LDL^T as the preconditioner.
+ *
+ * Plain C types only. Index arrays are int32 ("_i32") or int64 ("_i64");
+ * int64 entry points narrow to int32 internally and fail with
+ * MPIR_ERR_INVALID when a value does not fit. FP16/BF16 values travel as
+ * their 16-bit patterns (uint16_t).
+ *
+ * Every function is synchronous with respect to the host unless its name
+ * ends in "_dev" (device pointers, work ordered on the solver's stream and
+ * complete on return). Errors: negative status; mpir_last_error() returns a
+ * message for the calling thread.
+ */
+#ifndef MPIR_HIP_H
+#define MPIR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision tags */
+#define MPIR_F64 0
+#define MPIR_F32 1
+#define MPIR_F16 2
+#define MPIR_BF16 3
+
+/* status codes (QDLDL's own -1 / -2 keep their meaning where they apply) */
+#define MPIR_OK 0
+#define MPIR_ERR_INVALID (-100)
+#define MPIR_ERR_HIP (-101)
+#define MPIR_ERR_NODEVICE (-102)
+#define MPIR_ERR_STATE (-103)
+#define MPIR_ERR_NUMERIC (-104)
+#define MPIR_ERR_INTERNAL (-105)
+
+/* orderings */
+#define MPIR_ORDER_NATURAL 0 /* the matrix as given (QDLDL's behaviour) */
+#define MPIR_ORDER_AMD 1     /* approximate minimum degree + etree postorder */
+
+const char* mpir_last_error(void);
+const char* mpir_version(void);
+/* 1 if a gfx950 device is usable by this process, 0 otherwise */
+int mpir_device_ok(void);
+
+/* ------------------------------------------------------------------ QDLDL
+ * Replaces QDLDL_etree<INT> (yex33/MPIR include/MPIR/qdldl.hpp:48-105).
+ * Same arguments, same outputs, same return: sum(Lnz), -1 for an empty
+ * column or an entry below the diagonal, -2 when sum(Lnz) overflows INT.
+ * Host computation (symbolic, pointer chasing). */
+int32_t mpir_qdldl_etree_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* work, int32_t* Lnz,
+                             int32_t* etree);
+int64_t mpir_qdldl_etree_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, int64_t* work, int64_t* Lnz,
+                             int64_t* etree);
+
+/* Replaces QDLDL_factor<INT,PREC> (include/MPIR/qdldl.hpp:172-345). Same
+ * arguments and outputs (Lp, Li, Lx in CSC with rows ascending, D, Dinv);
+ * returns the number of positive entries of D, or -1 on a zero pivot.
+ * The factorization runs on the GPU; bwork/iwork/fwork are accepted for
+ * signature compatibility and may be NULL. */
+int32_t mpir_qdldl_factor_f64_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, int32_t* Lp,
+                                  int32_t* Li, double* Lx, double* D, double* Dinv, const int32_t* Lnz,
+                                  const int32_t* etree, unsigned char* bwork, int32_t* iwork, double* fwork);
+int32_t mpir_qdldl_factor_f32_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const float* Ax, int32_t* Lp,
+                                  int32_t* Li, float* Lx, float* D, float* Dinv, const int32_t* Lnz,
+                                  const int32_t* etree, unsigned char* bwork, int32_t* iwork, float* fwork);
+int64_t mpir_qdldl_factor_f64_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const double* Ax, int64_t* Lp,
+                                  int64_t* Li, double* Lx, double* D, double* Dinv, const int64_t* Lnz,
+                                  const int64_t* etree, unsigned char* bwork, int64_t* iwork, double* fwork);
+int64_t mpir_qdldl_factor_f32_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const float* Ax, int64_t* Lp,
+                                  int64_t* Li, float* Lx, float* D, float* Dinv, const int64_t* Lnz,
+                                  const int64_t* etree, unsigned char* bwork, int64_t* iwork, float* fwork);
+
+/* Replaces QDLDL_solve<INT,FACTPREC,XPREC> (include/MPIR/qdldl.hpp:417-432):
+ * x := (L D L^T)^{-1} x with L in FACTPREC and arithmetic in XPREC. The
+ * solve runs on the GPU. Returns 0, or a negative status (the reference's
+ * solve is void and unchecked). */
+int32_t mpir_qdldl_solve_f64_f64_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const double* Lx,
+                                     const double* Dinv, double* x);
+int32_t mpir_qdldl_solve_f32_f64_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const float* Lx,
+                                     const float* Dinv, double* x);
+int32_t mpir_qdldl_solve_f32_f32_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const float* Lx,
+                                     const float* Dinv, float* x);
+int64_t mpir_qdldl_solve_f64_f64_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const double* Lx,
+                                     const double* Dinv, double* x);
+int64_t mpir_qdldl_solve_f32_f64_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const float* Lx,
+                                     const float* Dinv, double* x);
+
+/* ------------------------------------------------------------------ GMRES-IR
+ * Replaces GmresLDLIR<UF,UW,UR> (include/MPIR/gmres_ir.hpp:90-315) in its
+ * LDL^T form (docs/Design/SoftDetailedDes/MIS.tex:505-568). */
+typedef struct mpir_gmres_ldl* mpir_gmres_ldl_t;
+
+typedef struct mpir_gmres_stats {
+  int32_t n;
+  int64_t nnz;                 /* upper-triangular input entries */
+  int64_t nnz_l;               /* strictly-lower entries of L */
+  int32_t nsupernodes;
+  int32_t nlevels;             /* levels of the supernodal tree */
+  int32_t max_front;
+  double factor_flops;
+  int32_t positive_pivots;     /* QDLDL_factor's return value */
+  int32_t ir_iterations;       /* last solve */
+  int32_t gmres_iterations;    /* last solve, summed over PrecondGmres calls */
+  int32_t precond_applications;
+  int32_t stop_reason;         /* 0 max iterations, 1 small update, 2 stagnation, 3 zero residual */
+  double final_rel_residual;   /* ||b_s - K_s x||_inf / ||b_s||_inf, scaled system */
+  double analyze_ms, factor_ms, solve_ms;
+  int64_t device_bytes;
+} mpir_gmres_stats;
+
+/* uf: MPIR_F64 or MPIR_F32; uw, ur: MPIR_F64 */
+int mpir_gmres_ldl_create(int uf, int uw, int ur, mpir_gmres_ldl_t* out);
+void mpir_gmres_ldl_destroy(mpir_gmres_ldl_t h);
+int mpir_gmres_ldl_set_max_ir(mpir_gmres_ldl_t h, int64_t n);        /* SetMaxIRIterations */
+int mpir_gmres_ldl_set_max_gmres(mpir_gmres_ldl_t h, int64_t n);     /* SetMaxGmresIterations */
+int mpir_gmres_ldl_set_tol(mpir_gmres_ldl_t h, double tol);          /* SetTolerance */
+int mpir_gmres_ldl_set_ordering(mpir_gmres_ldl_t h, int ordering);  /* before analyze */
+/* symbolic phase on the pattern of the upper CSC (reused by later factors) */
+int mpir_gmres_ldl_analyze(mpir_gmres_ldl_t h, int32_t n, const int32_t* Ap, const int32_t* Ai);
+/* numeric phase; Ax in the analyzed pattern's entry order (host / device) */
+int mpir_gmres_ldl_factor(mpir_gmres_ldl_t h, const double* Ax);
+int mpir_gmres_ldl_factor_dev(mpir_gmres_ldl_t h, const double* Ax);
+/* Compute(Ap, Ai, Ax): analyze when the pattern differs, then factor */
+int mpir_gmres_ldl_compute(mpir_gmres_ldl_t h, int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax);
+/* Solve(b) for nrhs columns of length n with leading dimension ld */
+int mpir_gmres_ldl_solve(mpir_gmres_ldl_t h, const double* b, double* x, int32_t nrhs, int64_t ld);
+int mpir_gmres_ldl_solve_dev(mpir_gmres_ldl_t h, const double* b, double* x, int32_t nrhs, int64_t ld);
+int mpir_gmres_ldl_stats(mpir_gmres_ldl_t h, mpir_gmres_stats* out);
+/* the fill-reducing permutation chosen by analyze: perm[k] = original index */
+int mpir_gmres_ldl_perm(mpir_gmres_ldl_t h, int32_t* perm);
+/* the factor in QDLDL's CSC layout, permuted order (sizes n+1 / nnz_l / n) */
+int mpir_gmres_ldl_export_factor(mpir_gmres_ldl_t h, int64_t* Lp, int32_t* Li, double* Lx, double* D, double* Dinv);
+/* kernel-level entry points on the handle's device vectors (permuted,
+ * scaled system), for parity tests and benchmarks */
+int mpir_gmres_ldl_precond_dev(mpir_gmres_ldl_t h, double* x);         /* x := M^{-1} x */
+int mpir_gmres_ldl_spmv_dev(mpir_gmres_ldl_t h, const double* x, double* y); /* y = K_s x */
+/* average ms of `reps` runs measured with HIP events on the handle's stream:
+ * what = 0 SpMV, 1 M^{-1} application, 2 numeric factorization */
+int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);
+/* HIP stream of the handle (hipStream_t as void*) */
+void* mpir_gmres_ldl_stream(mpir_gmres_ldl_t h);
+
+/* ------------------------------------------------------------------ utilities
+ * FP64 symmetric SpMV from the upper CSC with the reference's rounding
+ * (MatrixMultiply, include/MPIR/ops.hpp:25-45), on the GPU. */
+int mpir_spmv_upper_f64(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* x, double* y);
+/* r = b - K x in FP64 (ResidualCSCSymmetric, include/qdldl-luir/residual.hpp:7-44) */
+int mpir_residual_upper_f64(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* x,
+                            const double* b, double* r);
+
+/* Synthetic quasi-definite KKT systems (deterministic, SplitMix64):
+ * dim 2/3 grid Laplacian of side k plus h_shift*I, m constraint rows
+ * (a_mode 0: cnt_lo..cnt_hi distinct uniform columns, a_mode 1: a uniform
+ * column and its +x,+y,+z grid neighbours), D = d_value*I, x_true ~ U(-1,1),
+ * b = K x_true. Query sizes with NULL arrays, then fill. */
+int mpir_gen_kkt(int dim, int k, double h_shift, int64_t m, int a_mode, int cnt_lo, int cnt_hi, double d_value,
+                 uint64_t seed, int32_t* n_out, int64_t* nnz_out, int32_t* n1_out, int32_t* Ap, int32_t* Ai,
+                 double* Ax, double* x_true, double* b);
+
+/* AMD ordering of an upper CSC pattern (host): perm[k] = original index */
+int mpir_amd_order(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* perm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPIR_HIP_H */

@@ -1,0 +1,100 @@
+"""Build libmpir_hip.so (HIP for gfx950 + host C++) in-tree.
+
+    python -m mpir_amd.build            # compile everything, always
+    python -m mpir_amd.build --jobs 8
+
+Objects go to _build/ at the repository root; the shared library is written
+to mpir_amd/libmpir_hip.so.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = ROOT / "_build" / "mpir_hip"
+LIB = PKG / "libmpir_hip.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+ARCH = "gfx950"
+
+# Packed FP32 instructions are disabled for all device code.
+HIP_FLAGS = [
+    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-munsafe-fp-atomics",
+]
+# Host code: no contraction so host sums round exactly as written.
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-sign-compare"]
+
+HOST_SOURCES = [
+    CSRC / "host" / "generators.cpp",
+    CSRC / "host" / "ordering.cpp",
+    CSRC / "host" / "symbolic.cpp",
+    CSRC / "plan.cpp",
+]
+HIP_SOURCES = [
+    CSRC / "kernels" / "factor.hip",
+    CSRC / "kernels" / "solve.hip",
+    CSRC / "kernels" / "spmv.hip",
+    CSRC / "gmres.hip",
+    CSRC / "capi.cpp",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(line for line in proc.stdout.splitlines() if "packed-fp32-ops" not in line)
+    if proc.returncode != 0:
+        raise RuntimeError(f"command failed ({proc.returncode}): {' '.join(cmd)}\n{out}")
+    if out.strip():
+        print(out)
+
+
+def _obj(src: Path) -> Path:
+    rel = src.relative_to(CSRC)
+    return BUILD / (str(rel).replace("/", "__") + ".o")
+
+
+def build(jobs: int | None = None, verbose: bool = False) -> Path:
+    """Compile every source and link the shared library; returns its path."""
+    if not Path(HIPCC).exists() and shutil.which(HIPCC) is None:
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    BUILD.mkdir(parents=True, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    cmds = []
+    for src in HOST_SOURCES:
+        cmds.append([CXX, *CXX_FLAGS, "-c", str(src), "-o", str(_obj(src))])
+    for src in HIP_SOURCES:
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(_obj(src))])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        futs = [ex.submit(_run, c) for c in cmds]
+        for f in futs:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    objs = [str(_obj(s)) for s in HOST_SOURCES + HIP_SOURCES]
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--jobs", type=int, default=None)
+    args = ap.parse_args(argv)
+    build(args.jobs, verbose=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,497 @@
+// extern "C" entry points of libmpir_hip.so (declared in include/mpir_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mpir_hip.h"
+#include "common.hpp"
+#include "host/sparse.hpp"
+#include "solver.hpp"
+
+namespace mpir {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("host allocation failed");
+    return kErrInternal;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return kErrInternal;
+  }
+}
+
+// The QDLDL drop-ins keep one engine per factor precision, reused while the
+// pattern stays the same (hash of Ap/Ai).
+struct QdldlEngine {
+  std::unique_ptr<GmresLdl> solver;
+  uint64_t pattern_hash = 0;
+  int32_t n = -1;
+};
+std::mutex g_engine_mu;
+QdldlEngine g_factor_engine[2];
+QdldlEngine g_solve_engine[2];
+
+uint64_t fnv1a(const void* data, size_t bytes, uint64_t h = 1469598103934665603ull) {
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  for (size_t i = 0; i < bytes; ++i) {
+    h ^= p[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+bool narrow(int64_t v, int32_t* out) {
+  if (v < std::numeric_limits<int32_t>::min() || v > std::numeric_limits<int32_t>::max()) return false;
+  *out = static_cast<int32_t>(v);
+  return true;
+}
+
+std::vector<int32_t> narrow_vec(const int64_t* p, int64_t n) {
+  std::vector<int32_t> v(n);
+  for (int64_t i = 0; i < n; ++i)
+    if (!narrow(p[i], &v[i])) throw Error(kErrInvalid, "index does not fit in int32");
+  return v;
+}
+
+int32_t etree_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* work, int32_t* Lnz, int32_t* etree,
+                   bool check_i32, int64_t* total) {
+  // the reference reports entries below the diagonal on stdout
+  for (int32_t j = 0; j < n; ++j)
+    for (int32_t q = Ap[j]; q < Ap[j + 1]; ++q)
+      if (Ai[q] > j) {
+        bool empty_before = false;
+        for (int32_t c = 0; c < n; ++c)
+          if (Ap[c] == Ap[c + 1]) { empty_before = true; break; }
+        if (!empty_before) {
+          std::printf("!!! Matrix is not upper triangular\n");
+          std::fflush(stdout);
+        }
+        j = n;
+        break;
+      }
+  *total = etree_counts(n, Ap, Ai, work, Lnz, etree, check_i32);
+  return 0;
+}
+
+// Factor the given (already ordered) upper CSC on the GPU with QDLDL's
+// semantics and write QDLDL's CSC output.
+template <class T>
+int64_t factor_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const T* Ax, int64_t* Lp64, int32_t* Li, T* Lx,
+                    T* D, T* Dinv, const int32_t* Lnz, const int32_t* etree) {
+  const int prec = std::is_same<T, double>::value ? kF64 : kF32;
+  std::lock_guard<std::mutex> lock(g_engine_mu);
+  QdldlEngine& E = g_factor_engine[prec];
+  if (!E.solver) {
+    E.solver = std::make_unique<GmresLdl>(prec, kF64, kF64);
+    E.solver->set_ordering(kOrderNatural);
+    E.solver->set_scaling(false);
+  }
+  const int64_t nnz = Ap[n];
+  uint64_t h = fnv1a(Ap, sizeof(int32_t) * (n + 1));
+  h = fnv1a(Ai, sizeof(int32_t) * nnz, h);
+  if (E.n != n || E.pattern_hash != h || !E.solver->analyzed()) {
+    E.n = -1;
+    E.solver->analyze(n, Ap, Ai);
+    E.n = n;
+    E.pattern_hash = h;
+  }
+  // the caller's etree / counts must describe this pattern
+  std::vector<int32_t> work(n), lnz(n), par(n);
+  etree_counts(n, Ap, Ai, work.data(), lnz.data(), par.data(), false);
+  for (int32_t j = 0; j < n; ++j)
+    if (lnz[j] != Lnz[j] || par[j] != etree[j]) throw Error(kErrInvalid, "Lnz/etree do not match the pattern");
+  std::vector<double> ax(Ax, Ax + nnz);
+  try {
+    E.solver->factor(ax.data(), false);
+  } catch (const Error& e) {
+    if (e.code == kErrNumeric) return -1;  // QDLDL: exact zero pivot
+    throw;
+  }
+  E.solver->export_factor(Lp64, Li, Lx, D, Dinv, prec);
+  return E.solver->stats().positive_pivots;
+}
+
+template <class TF, class TX>
+int solve_impl(int32_t n, const int64_t* Lp, const int32_t* Li, const TF* Lx, const TF* Dinv, TX* x) {
+  static_assert(std::is_same<TX, double>::value || std::is_same<TX, float>::value, "x precision");
+  const int prec = std::is_same<TF, double>::value ? kF64 : kF32;
+  std::lock_guard<std::mutex> lock(g_engine_mu);
+  QdldlEngine& E = g_solve_engine[prec];
+  if (!E.solver) E.solver = std::make_unique<GmresLdl>(prec, kF64, kF64);
+  E.solver->load_factor_csc(n, Lp, Li, Lx, Dinv);
+  DevBuf xd(sizeof(double) * n);
+  std::vector<double> xh(x, x + n);
+  hipStream_t st = E.solver->stream();
+  MPIR_HIP(hipMemcpyAsync(xd.get(), xh.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  E.solver->precond_apply_dev(xd.as<double>());
+  MPIR_HIP(hipMemcpyAsync(xh.data(), xd.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  MPIR_HIP(hipStreamSynchronize(st));
+  for (int32_t i = 0; i < n; ++i) x[i] = static_cast<TX>(xh[i]);
+  return 0;
+}
+
+int spmv_upper_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* x,
+                    const double* b, double* y) {
+  MPIR_CHECK(n >= 1 && Ap && Ai && Ax && x && y, "bad SpMV arguments");
+  CscUpper A;
+  A.n = n;
+  A.p.assign(Ap, Ap + n + 1);
+  A.i.assign(Ai, Ai + Ap[n]);
+  A.x.assign(Ax, Ax + Ap[n]);
+  for (int32_t j = 0; j < n; ++j)
+    for (int32_t q = Ap[j]; q < Ap[j + 1]; ++q) MPIR_CHECK(Ai[q] >= 0 && Ai[q] <= j, "input must be upper triangular");
+  CsrFull F = csr_from_upper(A);
+  hipStream_t st;
+  MPIR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{st};
+  DevBuf dp(sizeof(int32_t) * (n + 1)), dj(sizeof(int32_t) * std::max<size_t>(1, F.j.size())),
+      dx(sizeof(double) * std::max<size_t>(1, F.x.size())), dv(sizeof(double) * n), dy(sizeof(double) * n),
+      db(sizeof(double) * n);
+  MPIR_HIP(hipMemcpyAsync(dp.get(), F.p.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+  MPIR_HIP(hipMemcpyAsync(dj.get(), F.j.data(), sizeof(int32_t) * F.j.size(), hipMemcpyHostToDevice, st));
+  MPIR_HIP(hipMemcpyAsync(dx.get(), F.x.data(), sizeof(double) * F.x.size(), hipMemcpyHostToDevice, st));
+  MPIR_HIP(hipMemcpyAsync(dv.get(), x, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  CsrDev D{n, dp.as<int32_t>(), dj.as<int32_t>(), dx.as<double>()};
+  if (b) {
+    MPIR_HIP(hipMemcpyAsync(db.get(), b, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    residual_f64(D, dv.as<double>(), db.as<double>(), dy.as<double>(), st);
+  } else {
+    spmv_f64(D, dv.as<double>(), dy.as<double>(), st);
+  }
+  MPIR_HIP(hipMemcpyAsync(y, dy.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  MPIR_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+}  // namespace
+}  // namespace mpir
+
+using namespace mpir;
+
+struct mpir_gmres_ldl {
+  std::unique_ptr<GmresLdl> s;
+  std::vector<int32_t> Ap, Ai;
+};
+
+extern "C" {
+
+const char* mpir_last_error(void) { return g_last_error.c_str(); }
+const char* mpir_version(void) { return "mpir_hip 0.1 (gfx950)"; }
+
+int mpir_device_ok(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+  return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+int32_t mpir_qdldl_etree_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* work, int32_t* Lnz,
+                             int32_t* etree) {
+  int64_t total = -1;
+  const int rc = guarded([&] { return etree_impl(n, Ap, Ai, work, Lnz, etree, true, &total); });
+  return rc != 0 ? rc : static_cast<int32_t>(total);
+}
+
+int64_t mpir_qdldl_etree_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, int64_t* work, int64_t* Lnz,
+                             int64_t* etree) {
+  int64_t total = -1;
+  const int rc = guarded([&] {
+    int32_t n32;
+    MPIR_CHECK(narrow(n, &n32) && n32 >= 0, "n does not fit in int32");
+    std::vector<int32_t> p = narrow_vec(Ap, n + 1);
+    std::vector<int32_t> i = narrow_vec(Ai, Ap[n]);
+    std::vector<int32_t> w(n32), l(n32), e(n32);
+    etree_impl(n32, p.data(), i.data(), w.data(), l.data(), e.data(), false, &total);
+    for (int64_t k = 0; k < n; ++k) {
+      work[k] = w[k];
+      Lnz[k] = l[k];
+      etree[k] = e[k];
+    }
+    return 0;
+  });
+  return rc != 0 ? rc : total;
+}
+
+#define MPIR_FACTOR_I32(SUF, T)                                                                                       \
+  int32_t mpir_qdldl_factor_##SUF##_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const T* Ax, int32_t* Lp,   \
+                                        int32_t* Li, T* Lx, T* D, T* Dinv, const int32_t* Lnz, const int32_t* etree, \
+                                        unsigned char*, int32_t*, T*) {                                                \
+    int64_t ret = 0;                                                                                                   \
+    const int rc = guarded([&] {                                                                                       \
+      std::vector<int64_t> lp(static_cast<size_t>(n) + 1);                                                           \
+      ret = factor_impl<T>(n, Ap, Ai, Ax, lp.data(), Li, Lx, D, Dinv, Lnz, etree);                                     \
+      for (int32_t k = 0; k <= n; ++k) Lp[k] = static_cast<int32_t>(lp[k]);                                          \
+      return 0;                                                                                                        \
+    });                                                                                                                \
+    return rc != 0 ? rc : static_cast<int32_t>(ret);                                                                  \
+  }
+
+#define MPIR_FACTOR_I64(SUF, T)                                                                                       \
+  int64_t mpir_qdldl_factor_##SUF##_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const T* Ax, int64_t* Lp,   \
+                                        int64_t* Li, T* Lx, T* D, T* Dinv, const int64_t* Lnz, const int64_t* etree, \
+                                        unsigned char*, int64_t*, T*) {                                                \
+    int64_t ret = 0;                                                                                                   \
+    const int rc = guarded([&] {                                                                                       \
+      int32_t n32;                                                                                                     \
+      MPIR_CHECK(narrow(n, &n32) && n32 >= 1, "n does not fit in int32");                                             \
+      std::vector<int32_t> p = narrow_vec(Ap, n + 1), i = narrow_vec(Ai, Ap[n]);                                     \
+      std::vector<int32_t> l = narrow_vec(Lnz, n), e = narrow_vec(etree, n);                                         \
+      int64_t lnz = 0;                                                                                                 \
+      for (int64_t k = 0; k < n; ++k) lnz += Lnz[k];                                                                 \
+      std::vector<int32_t> li(static_cast<size_t>(std::max<int64_t>(lnz, 1)));                                       \
+      ret = factor_impl<T>(n32, p.data(), i.data(), Ax, Lp, li.data(), Lx, D, Dinv, l.data(), e.data());              \
+      for (int64_t k = 0; k < lnz; ++k) Li[k] = li[k];                                                               \
+      return 0;                                                                                                        \
+    });                                                                                                                \
+    return rc != 0 ? rc : ret;                                                                                         \
+  }
+
+MPIR_FACTOR_I32(f64, double)
+MPIR_FACTOR_I32(f32, float)
+MPIR_FACTOR_I64(f64, double)
+MPIR_FACTOR_I64(f32, float)
+
+#define MPIR_SOLVE_I32(SUF, TF, TX)                                                                                  \
+  int32_t mpir_qdldl_solve_##SUF##_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const TF* Lx,               \
+                                       const TF* Dinv, TX* x) {                                                      \
+    return guarded([&] {                                                                                             \
+      MPIR_CHECK(n >= 1 && Lp && Li && Lx && Dinv && x, "bad solve arguments");                                     \
+      std::vector<int64_t> lp(Lp, Lp + n + 1);                                                                       \
+      return solve_impl<TF, TX>(n, lp.data(), Li, Lx, Dinv, x);                                                      \
+    });                                                                                                              \
+  }
+#define MPIR_SOLVE_I64(SUF, TF, TX)                                                                                  \
+  int64_t mpir_qdldl_solve_##SUF##_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const TF* Lx,               \
+                                       const TF* Dinv, TX* x) {                                                      \
+    return guarded([&] {                                                                                             \
+      int32_t n32;                                                                                                   \
+      MPIR_CHECK(narrow(n, &n32) && n32 >= 1 && Lp && Li && Lx && Dinv && x, "bad solve arguments");                \
+      std::vector<int32_t> li = narrow_vec(Li, Lp[n]);                                                              \
+      return solve_impl<TF, TX>(n32, Lp, li.data(), Lx, Dinv, x);                                                    \
+    });                                                                                                              \
+  }
+MPIR_SOLVE_I32(f64_f64, double, double)
+MPIR_SOLVE_I32(f32_f64, float, double)
+MPIR_SOLVE_I32(f32_f32, float, float)
+MPIR_SOLVE_I64(f64_f64, double, double)
+MPIR_SOLVE_I64(f32_f64, float, double)
+
+int mpir_gmres_ldl_create(int uf, int uw, int ur, mpir_gmres_ldl_t* out) {
+  return guarded([&] {
+    MPIR_CHECK(out, "null output handle");
+    *out = nullptr;
+    auto h = std::make_unique<mpir_gmres_ldl>();
+    h->s = std::make_unique<GmresLdl>(uf, uw, ur);
+    *out = h.release();
+    return 0;
+  });
+}
+
+void mpir_gmres_ldl_destroy(mpir_gmres_ldl_t h) { delete h; }
+
+#define MPIR_HANDLE_CHECK \
+  MPIR_CHECK(h && h->s, "null solver handle")
+
+int mpir_gmres_ldl_set_max_ir(mpir_gmres_ldl_t h, int64_t n) {
+  return guarded([&] { MPIR_HANDLE_CHECK; MPIR_CHECK(n >= 0, "negative iteration count"); h->s->set_max_ir(n); return 0; });
+}
+int mpir_gmres_ldl_set_max_gmres(mpir_gmres_ldl_t h, int64_t n) {
+  return guarded([&] { MPIR_HANDLE_CHECK; MPIR_CHECK(n >= 0, "negative iteration count"); h->s->set_max_gmres(n); return 0; });
+}
+int mpir_gmres_ldl_set_tol(mpir_gmres_ldl_t h, double tol) {
+  return guarded([&] { MPIR_HANDLE_CHECK; h->s->set_tol(tol); return 0; });
+}
+int mpir_gmres_ldl_set_ordering(mpir_gmres_ldl_t h, int ordering) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(ordering == MPIR_ORDER_NATURAL || ordering == MPIR_ORDER_AMD, "unknown ordering");
+    h->s->set_ordering(ordering);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_analyze(mpir_gmres_ldl_t h, int32_t n, const int32_t* Ap, const int32_t* Ai) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->analyze(n, Ap, Ai);
+    h->Ap.assign(Ap, Ap + n + 1);
+    h->Ai.assign(Ai, Ai + Ap[n]);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_factor(mpir_gmres_ldl_t h, const double* Ax) {
+  return guarded([&] { MPIR_HANDLE_CHECK; MPIR_CHECK(Ax, "null values"); h->s->factor(Ax, false); return 0; });
+}
+int mpir_gmres_ldl_factor_dev(mpir_gmres_ldl_t h, const double* Ax) {
+  return guarded([&] { MPIR_HANDLE_CHECK; MPIR_CHECK(Ax, "null values"); h->s->factor(Ax, true); return 0; });
+}
+int mpir_gmres_ldl_compute(mpir_gmres_ldl_t h, int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(n >= 1 && Ap && Ai && Ax, "bad compute arguments");
+    const bool same = h->s->analyzed() && static_cast<int32_t>(h->Ap.size()) == n + 1 &&
+                      std::equal(h->Ap.begin(), h->Ap.end(), Ap) && static_cast<int64_t>(h->Ai.size()) == Ap[n] &&
+                      std::equal(h->Ai.begin(), h->Ai.end(), Ai);
+    if (!same) {
+      h->s->analyze(n, Ap, Ai);
+      h->Ap.assign(Ap, Ap + n + 1);
+      h->Ai.assign(Ai, Ai + Ap[n]);
+    }
+    h->s->factor(Ax, false);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_solve(mpir_gmres_ldl_t h, const double* b, double* x, int32_t nrhs, int64_t ld) {
+  return guarded([&] { MPIR_HANDLE_CHECK; MPIR_CHECK(b && x, "null vectors"); h->s->solve(b, x, nrhs, ld, false); return 0; });
+}
+int mpir_gmres_ldl_solve_dev(mpir_gmres_ldl_t h, const double* b, double* x, int32_t nrhs, int64_t ld) {
+  return guarded([&] { MPIR_HANDLE_CHECK; MPIR_CHECK(b && x, "null vectors"); h->s->solve(b, x, nrhs, ld, true); return 0; });
+}
+int mpir_gmres_ldl_stats(mpir_gmres_ldl_t h, mpir_gmres_stats* out) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(out, "null stats");
+    const SolveStats& s = h->s->stats();
+    out->n = s.n;
+    out->nnz = s.nnz;
+    out->nnz_l = s.nnz_l;
+    out->nsupernodes = s.nsupernodes;
+    out->nlevels = s.nlevels;
+    out->max_front = s.max_front;
+    out->factor_flops = s.factor_flops;
+    out->positive_pivots = s.positive_pivots;
+    out->ir_iterations = s.ir_iterations;
+    out->gmres_iterations = s.gmres_iterations;
+    out->precond_applications = s.precond_applications;
+    out->stop_reason = s.stop_reason;
+    out->final_rel_residual = s.final_rel_residual;
+    out->analyze_ms = s.analyze_ms;
+    out->factor_ms = s.factor_ms;
+    out->solve_ms = s.solve_ms;
+    out->device_bytes = s.device_bytes;
+    return 0;
+  });
+}
+int mpir_gmres_ldl_perm(mpir_gmres_ldl_t h, int32_t* perm) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(h->s->analyzed(), "no pattern analyzed");
+    std::memcpy(perm, h->s->perm().data(), sizeof(int32_t) * h->s->n());
+    return 0;
+  });
+}
+int mpir_gmres_ldl_export_factor(mpir_gmres_ldl_t h, int64_t* Lp, int32_t* Li, double* Lx, double* D, double* Dinv) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(Lp, "null Lp");
+    h->s->export_factor(Lp, Li, Lx, D, Dinv, kF64);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_precond_dev(mpir_gmres_ldl_t h, double* x) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(h->s->factored(), "no factor");
+    h->s->precond_apply_dev(x);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_spmv_dev(mpir_gmres_ldl_t h, const double* x, double* y) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(h->s->factored(), "no factor");
+    h->s->spmv_dev(x, y);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(avg_ms, "null output");
+    *avg_ms = h->s->time_kernels(what, reps);
+    return 0;
+  });
+}
+void* mpir_gmres_ldl_stream(mpir_gmres_ldl_t h) { return h && h->s ? static_cast<void*>(h->s->stream()) : nullptr; }
+
+int mpir_spmv_upper_f64(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* x, double* y) {
+  return guarded([&] { return spmv_upper_impl(n, Ap, Ai, Ax, x, nullptr, y); });
+}
+int mpir_residual_upper_f64(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* x,
+                            const double* b, double* r) {
+  return guarded([&] {
+    MPIR_CHECK(b, "null b");
+    return spmv_upper_impl(n, Ap, Ai, Ax, x, b, r);
+  });
+}
+
+int mpir_gen_kkt(int dim, int k, double h_shift, int64_t m, int a_mode, int cnt_lo, int cnt_hi, double d_value,
+                 uint64_t seed, int32_t* n_out, int64_t* nnz_out, int32_t* n1_out, int32_t* Ap, int32_t* Ai,
+                 double* Ax, double* x_true, double* b) {
+  return guarded([&] {
+    KktSpec s;
+    s.dim = dim;
+    s.k = k;
+    s.h_shift = h_shift;
+    s.m = m;
+    s.a_mode = a_mode;
+    s.cnt_lo = cnt_lo;
+    s.cnt_hi = cnt_hi;
+    s.d_value = d_value;
+    s.seed = seed;
+    KktProblem P;
+    try {
+      P = generate_kkt(s);
+    } catch (const std::invalid_argument& e) {
+      throw Error(kErrInvalid, e.what());
+    }
+    if (n_out) *n_out = P.K.n;
+    if (nnz_out) *nnz_out = P.K.nnz();
+    if (n1_out) *n1_out = P.n1;
+    if (Ap) std::memcpy(Ap, P.K.p.data(), sizeof(int32_t) * P.K.p.size());
+    if (Ai) std::memcpy(Ai, P.K.i.data(), sizeof(int32_t) * P.K.i.size());
+    if (Ax) std::memcpy(Ax, P.K.x.data(), sizeof(double) * P.K.x.size());
+    if (x_true) std::memcpy(x_true, P.x_true.data(), sizeof(double) * P.x_true.size());
+    if (b) std::memcpy(b, P.b.data(), sizeof(double) * P.b.size());
+    return 0;
+  });
+}
+
+int mpir_amd_order(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* perm) {
+  return guarded([&] {
+    MPIR_CHECK(n >= 1 && Ap && Ai && perm, "bad ordering arguments");
+    CscUpper A;
+    A.n = n;
+    A.p.assign(Ap, Ap + n + 1);
+    A.i.assign(Ai, Ai + Ap[n]);
+    A.x.assign(Ap[n], 0.0);
+    std::vector<int32_t> p = amd_order(A);
+    std::memcpy(perm, p.data(), sizeof(int32_t) * n);
+    return 0;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,744 @@
+// GMRES-IR on the device: vector kernels and the driver.
+//
+// Algorithm (yex33/MPIR include/MPIR/gmres_ir.hpp, LDL^T variant of the MIS,
+// docs/Design/SoftDetailedDes/MIS.tex:505-568):
+//   Compute: AD_i = 1/sqrt|a_ii| when |a_ii| > eps(UF) (gmres_ir.hpp:347-363),
+//            K_s = AD K AD, fill-reducing order, LDL^T of K_s in UF.
+//   Solve  : b_s = b o AD; x = PrecondGmres(b_s); r = b_s - K_s x (FP64);
+//            repeat: scale = ||r||_inf; d = PrecondGmres(r * (1/scale)) * scale;
+//            x += d; r = b_s - K_s x; stop when ||d|| <= 10 eps ||x_old|| or
+//            | ||r||/||b_s|| - previous | < tol (gmres_ir.hpp:534-634);
+//            return x o AD.
+//   PrecondGmres: left-preconditioned GMRES without restart, M^{-1} =
+//            (L D L^T)^{-1}, Givens rotations c = h_kk/nu, s = -h_k+1,k/nu,
+//            back substitution, x = V y (gmres_ir.hpp:636-741). The basis is
+//            orthogonalised by classical Gram-Schmidt applied twice (CGS2)
+//            instead of modified Gram-Schmidt with a conditional second pass.
+// Vectors stay on the device in the fill-reducing order; only the Hessenberg
+// column and the norms that drive the control flow come back to the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <numeric>
+
+#include "host/sparse.hpp"
+#include "solver.hpp"
+
+namespace mpir {
+
+namespace {
+
+constexpr int kRedBlocks = 512;
+constexpr int kRedThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) r += sh[i];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) r = fmax(r, sh[i]);
+  __syncthreads();
+  return r;
+}
+
+// op 0: sum x_i y_i, op 1: sum x_i^2, op 2: max |x_i|
+__global__ void __launch_bounds__(kRedThreads) k_reduce_partial(int n, const double* __restrict__ x,
+                                                                const double* __restrict__ y, int op,
+                                                                double* __restrict__ part) {
+  __shared__ double sh[kRedThreads / 64];
+  double acc = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double v = x[i];
+    if (op == 0) acc += v * y[i];
+    else if (op == 1) acc += v * v;
+    else acc = fmax(acc, fabs(v));
+  }
+  const double r = op == 2 ? block_max(acc, sh) : block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// out[slot] = reduction of the partials (post 1: sqrt)
+__global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* __restrict__ part, int np, int op, int post,
+                                                              double* __restrict__ out) {
+  __shared__ double sh[kRedThreads / 64];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc = op == 2 ? fmax(acc, part[i]) : acc + part[i];
+  const double r = op == 2 ? block_max(acc, sh) : block_sum(acc, sh);
+  if (threadIdx.x == 0) *out = post ? sqrt(r) : r;
+}
+
+// partial dots of w against k1 basis vectors: part[j * gridDim.x + block]
+__global__ void __launch_bounds__(kRedThreads) k_mdot_partial(int n, const double* __restrict__ V, int64_t ldv, int k1,
+                                                              const double* __restrict__ w, double* __restrict__ part) {
+  __shared__ double sh[kRedThreads / 64];
+  for (int j = 0; j < k1; ++j) {
+    const double* vj = V + ldv * j;
+    double acc = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += vj[i] * w[i];
+    const double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[static_cast<int64_t>(j) * gridDim.x + blockIdx.x] = r;
+  }
+}
+
+// h[j] (+)= sum of the partials of vector j, one block per j
+__global__ void __launch_bounds__(kRedThreads) k_mdot_final(const double* __restrict__ part, int np, double* __restrict__ h,
+                                                            int accumulate) {
+  __shared__ double sh[kRedThreads / 64];
+  const int j = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[static_cast<int64_t>(j) * np + i];
+  const double r = block_sum(acc, sh);
+  if (threadIdx.x == 0) h[j] = accumulate ? h[j] + r : r;
+}
+
+// w -= sum_j V_j h_j
+__global__ void k_sub_basis(int n, const double* __restrict__ V, int64_t ldv, int k1, const double* __restrict__ h,
+                            double* __restrict__ w) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int j = 0; j < k1; ++j) acc += V[ldv * j + i] * h[j];
+  w[i] -= acc;
+}
+
+// res = sum_j V_j y_j, j ascending from zero (gmres_ir.hpp:735-739)
+__global__ void k_combine(int n, const double* __restrict__ V, int64_t ldv, int k, const double* __restrict__ y,
+                          double* __restrict__ res) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int j = 0; j < k; ++j) acc += V[ldv * j + i] * y[j];
+  res[i] = acc;
+}
+
+// out = x * s where s = scalars[slot] (recip: s = 1 / scalars[slot])
+__global__ void k_scale_dev(int n, const double* __restrict__ x, const double* __restrict__ sp, int recip,
+                            double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = recip ? 1.0 / *sp : *sp;
+  out[i] = x[i] * s;
+}
+
+__global__ void k_scale_host(int n, const double* __restrict__ x, double s, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = x[i] * s;
+}
+
+__global__ void k_add(int n, double* __restrict__ x, const double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] + d[i];
+}
+
+__global__ void k_compute_ad(int n, const double* __restrict__ ax, const int32_t* __restrict__ diag_pos, double eps,
+                             double* __restrict__ ad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = 1.0;
+  const int q = diag_pos[i];
+  if (q >= 0) {
+    const double a = fabs(ax[q]);
+    if (a > eps) v = 1.0 / sqrt(a);
+  }
+  ad[i] = v;
+}
+
+__global__ void k_scale_entries(int64_t nnz, const double* __restrict__ ax, const int32_t* __restrict__ row,
+                                const int32_t* __restrict__ col, const double* __restrict__ ad, double* __restrict__ axs) {
+  const int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (q < nnz) axs[q] = ax[q] * (ad[row[q]] * ad[col[q]]);
+}
+
+__global__ void k_gather(int64_t nnz, const double* __restrict__ src, const int32_t* __restrict__ idx,
+                         double* __restrict__ out) {
+  const int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (q < nnz) out[q] = src[idx[q]];
+}
+
+// out[k] = b[perm[k]] * ad[perm[k]]  (b o AD, then into the fill-reducing order)
+__global__ void k_perm_in(int n, const double* __restrict__ b, const int32_t* __restrict__ perm,
+                          const double* __restrict__ ad, double* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = b[perm[k]] * ad[perm[k]];
+}
+
+// x[perm[k]] = xp[k] * ad[perm[k]]
+__global__ void k_perm_out(int n, const double* __restrict__ xp, const int32_t* __restrict__ perm,
+                           const double* __restrict__ ad, double* __restrict__ x) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[perm[k]] = xp[k] * ad[perm[k]];
+}
+
+inline unsigned nblk(int64_t n, int bs = 256) { return static_cast<unsigned>((n + bs - 1) / bs); }
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T>
+void upload_vec(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+  b.alloc(std::max<size_t>(v.size(), 1) * sizeof(T));
+  if (!v.empty()) MPIR_HIP(hipMemcpyAsync(b.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+// small-vector slots
+enum Slot { kSlotNorm = 0, kSlotH = 8 };
+
+}  // namespace
+
+GmresLdl::GmresLdl(int uf, int uw, int ur) : uf_(uf), uw_(uw), ur_(ur) {
+  MPIR_CHECK(uf == kF64 || uf == kF32, "factor precision must be f64 or f32");
+  MPIR_CHECK(uw == kF64 && ur == kF64, "working and residual precision must be f64");
+  MPIR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  MPIR_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096 * sizeof(double), hipHostMallocDefault));
+}
+
+GmresLdl::~GmresLdl() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (h_pinned_) (void)hipHostFree(h_pinned_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void GmresLdl::release_numeric() { factored_ = false; }
+
+void GmresLdl::analyze(int32_t n, const int32_t* Ap, const int32_t* Ai) {
+  const double t0 = now_ms();
+  MPIR_CHECK(n >= 1, "matrix dimension must be positive");
+  MPIR_CHECK(Ap && Ai, "null pattern pointer");
+  MPIR_CHECK(Ap[0] == 0, "column pointers must start at 0");
+  for (int32_t j = 0; j < n; ++j) {
+    MPIR_CHECK(Ap[j + 1] > Ap[j], "empty column in the upper-triangular input");
+    for (int32_t q = Ap[j]; q < Ap[j + 1]; ++q) {
+      MPIR_CHECK(Ai[q] >= 0 && Ai[q] <= j, "entry below the diagonal or out of range");
+      if (q > Ap[j]) MPIR_CHECK(Ai[q] > Ai[q - 1], "row indices must be strictly ascending");
+    }
+  }
+  analyzed_ = factored_ = false;
+  n_ = n;
+  nnz_ = Ap[n];
+  Ap_.assign(Ap, Ap + n + 1);
+  Ai_.assign(Ai, Ai + nnz_);
+  CscUpper A;
+  A.n = n;
+  A.p = Ap_;
+  A.i = Ai_;
+  A.x.assign(nnz_, 0.0);
+
+  // ordering: AMD followed by a postorder of the elimination tree
+  std::vector<int32_t> work(n), Lnz(n), parent(n);
+  if (ordering_ == kOrderAmd) {
+    std::vector<int32_t> p0 = amd_order(A);
+    std::vector<int32_t> vm;
+    CscUpper C0 = permute_upper(A, p0, &vm);
+    if (etree_counts(n, C0.p.data(), C0.i.data(), work.data(), Lnz.data(), parent.data()) < 0)
+      throw Error(kErrInvalid, "etree failed on the ordered matrix");
+    std::vector<int32_t> post = tree_postorder(n, parent.data());
+    perm_.resize(n);
+    for (int32_t k = 0; k < n; ++k) perm_[k] = p0[post[k]];
+  } else {
+    perm_.resize(n);
+    std::iota(perm_.begin(), perm_.end(), 0);
+  }
+  std::vector<int32_t> val_map;
+  CscUpper C = permute_upper(A, perm_, &val_map);
+  const int64_t lnz = etree_counts(n, C.p.data(), C.i.data(), work.data(), Lnz.data(), parent.data());
+  if (lnz == -2) throw Error(kErrInvalid, "nonzeros of L overflow int32");
+  if (lnz < 0) throw Error(kErrInvalid, "etree failed");
+  plan_ = std::make_unique<FactorPlan>(build_factor_plan(C, val_map, parent.data(), Lnz.data()));
+  hipStream_t st = stream_;
+  upload_plan();
+
+  // FP64 CSR of the permuted, scaled matrix for the SpMV: entry -> original
+  CsrFull F = csr_from_upper(C);
+  std::vector<int32_t> src(F.src.size());
+  for (size_t q = 0; q < src.size(); ++q) src[q] = val_map[F.src[q]];
+  upload_vec(d_csr_p_, F.p, st);
+  upload_vec(d_csr_j_, F.j, st);
+  upload_vec(d_csr_src_, src, st);
+  d_csr_x_.alloc(std::max<size_t>(1, F.j.size()) * sizeof(double));
+  // entry rows / columns and diagonal positions of the original pattern
+  std::vector<int32_t> ecol(nnz_), diag_pos(n, -1);
+  for (int32_t j = 0; j < n; ++j)
+    for (int32_t q = Ap_[j]; q < Ap_[j + 1]; ++q) {
+      ecol[q] = j;
+      if (Ai_[q] == j) diag_pos[j] = q;
+    }
+  upload_vec(d_entry_col_, ecol, st);
+  upload_vec(d_entry_row_, Ai_, st);
+  upload_vec(d_diag_pos_, diag_pos, st);
+  upload_vec(d_perm_, perm_, st);
+
+  alloc_numeric();
+  MPIR_HIP(hipStreamSynchronize(st));
+
+  const Supernodes& S = plan_->sn;
+  stats_ = SolveStats();
+  stats_.n = n;
+  stats_.nnz = nnz_;
+  stats_.nnz_l = S.nnz_l;
+  stats_.nsupernodes = S.ns;
+  stats_.nlevels = S.nlevels;
+  stats_.max_front = S.max_front;
+  stats_.factor_flops = S.flops;
+  stats_.device_bytes = static_cast<int64_t>(d_panel_.bytes() + d_upool_.bytes());
+  analyzed_ = true;
+  precond_only_ = false;
+  stats_.analyze_ms = now_ms() - t0;
+}
+
+void GmresLdl::upload_plan() {
+  const FactorPlan& P = *plan_;
+  const Supernodes& S = P.sn;
+  hipStream_t st = stream_;
+  upload_vec(d_first_, S.first, st);
+  upload_vec(d_nrow_, S.nrow, st);
+  upload_vec(d_rows_off_, S.rows_off, st);
+  upload_vec(d_rows_, S.rows, st);
+  upload_vec(d_relidx_, S.relidx, st);
+  upload_vec(d_panel_off_, S.panel_off, st);
+  upload_vec(d_upd_off_, P.upd_off, st);
+  upload_vec(d_child_ptr_, P.child_ptr, st);
+  upload_vec(d_child_list_, P.child_list, st);
+  keep_.clear();
+  keep_.resize(8);
+  upload_vec(keep_[0], P.small_list, st);
+  upload_vec(keep_[1], P.big_list, st);
+  upload_vec(keep_[2], P.asm_task, st);
+  upload_vec(keep_[3], P.pan_task, st);
+  upload_vec(keep_[4], P.upd_task, st);
+  upload_vec(keep_[5], P.val_dst, st);
+  upload_vec(keep_[6], P.diag_task, st);
+  dfp_ = DeviceFactorPlan();
+  dfp_.nlevels = S.nlevels;
+  dfp_.panel_entries = P.panel_entries;
+  dfp_.small_list = keep_[0].as<int32_t>();
+  dfp_.big_list = keep_[1].as<int32_t>();
+  dfp_.asm_task = keep_[2].as<Task4>();
+  dfp_.pan_task = keep_[3].as<Task4>();
+  dfp_.upd_task = keep_[4].as<Task4>();
+  dfp_.val_dst = keep_[5].as<int64_t>();
+  dfp_.diag_task = keep_[6].as<Task4>();
+  dfp_.h_small_ptr = P.small_ptr;
+  dfp_.h_big_ptr = P.big_ptr;
+  dfp_.h_asm_ptr = P.asm_ptr;
+  dfp_.h_level_step_ptr = P.level_step_ptr;
+  dfp_.h_pan_ptr = P.pan_ptr;
+  dfp_.h_upd_ptr = P.upd_ptr;
+  dfp_.h_diag_ptr = P.diag_ptr;
+  dsp_ = DeviceSolvePlan();
+  dsp_.nlevels = S.nlevels;
+  dsp_.small_list = dfp_.small_list;
+  dsp_.big_list = dfp_.big_list;
+  dsp_.diag_task = dfp_.diag_task;
+  dsp_.gemv_task = dfp_.pan_task;
+  dsp_.h_small_ptr = P.small_ptr;
+  dsp_.h_big_ptr = P.big_ptr;
+  dsp_.h_level_step_ptr = P.level_step_ptr;
+  dsp_.h_diag_ptr = P.diag_ptr;
+  dsp_.h_task_ptr = P.pan_ptr;
+  d_scratch_.alloc(std::max<int64_t>(1, static_cast<int64_t>(P.max_step_tasks) * kPanelNB) * sizeof(double));
+  dsp_.scratch = d_scratch_.get();
+  d_uvec_.alloc(std::max<int64_t>(1, static_cast<int64_t>(S.rows.size())) * sizeof(double));
+}
+
+void GmresLdl::alloc_numeric() {
+  const FactorPlan& P = *plan_;
+  const int32_t n = n_;
+  const size_t tf = uf_ == kF64 ? sizeof(double) : sizeof(float);
+  d_ax_.alloc(std::max<int64_t>(1, nnz_) * sizeof(double));
+  d_axs_.alloc(std::max<int64_t>(1, nnz_) * sizeof(double));
+  d_ad_.alloc(n * sizeof(double));
+  d_panel_.alloc(std::max<int64_t>(1, P.panel_entries) * tf);
+  d_upool_.alloc(std::max<int64_t>(1, P.upd_pool) * tf);
+  d_D_.alloc(n * tf);
+  d_Dinv_.alloc(n * tf);
+  d_err_.alloc(sizeof(int));
+  d_cnt_.alloc(sizeof(int));
+  d_b_.alloc(n * sizeof(double));
+  d_x_.alloc(n * sizeof(double));
+  d_r_.alloc(n * sizeof(double));
+  d_d_.alloc(n * sizeof(double));
+  d_w_.alloc(n * sizeof(double));
+  d_red_.alloc(static_cast<size_t>(kRedBlocks) * 64 * sizeof(double));
+  d_small_.alloc(256 * sizeof(double));
+}
+
+void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, const void* Lx, const void* Dinv) {
+  MPIR_CHECK(n >= 1, "matrix dimension must be positive");
+  analyzed_ = factored_ = false;
+  Supernodes S;
+  try {
+    S = supernodes_from_l(n, Lp, Li);
+  } catch (const std::exception& e) {
+    throw Error(kErrInvalid, std::string("factor pattern rejected: ") + e.what());
+  }
+  n_ = n;
+  nnz_ = 0;
+  perm_.resize(n);
+  std::iota(perm_.begin(), perm_.end(), 0);
+  plan_ = std::make_unique<FactorPlan>(build_plan_from_supernodes(std::move(S)));
+  upload_plan();
+  alloc_numeric();
+  const Supernodes& SS = plan_->sn;
+  const size_t tf = uf_ == kF64 ? sizeof(double) : sizeof(float);
+  std::vector<char> panel(static_cast<size_t>(plan_->panel_entries) * tf, 0);
+  for (int32_t s = 0; s < SS.ns; ++s) {
+    const int32_t f = SS.first[s], l = SS.first[s + 1];
+    const int64_t m = (l - f) + SS.nrow[s];
+    for (int32_t j = f; j < l; ++j) {
+      const int64_t cj = j - f;
+      const int64_t len = Lp[j + 1] - Lp[j];
+      std::memcpy(panel.data() + (SS.panel_off[s] + cj + 1 + cj * m) * tf,
+                  static_cast<const char*>(Lx) + Lp[j] * tf, len * tf);
+    }
+  }
+  MPIR_HIP(hipMemcpyAsync(d_panel_.get(), panel.data(), panel.size(), hipMemcpyHostToDevice, stream_));
+  MPIR_HIP(hipMemcpyAsync(d_Dinv_.get(), Dinv, n * tf, hipMemcpyHostToDevice, stream_));
+  MPIR_HIP(hipStreamSynchronize(stream_));
+  analyzed_ = factored_ = true;
+  precond_only_ = true;
+}
+
+void GmresLdl::factor(const double* Ax, bool device_ptr) {
+  if (!analyzed_) throw Error(kErrState, "factor called before analyze");
+  const double t0 = now_ms();
+  hipStream_t st = stream_;
+  factored_ = false;
+  if (nnz_ > 0)
+    MPIR_HIP(hipMemcpyAsync(d_ax_.get(), Ax, nnz_ * sizeof(double),
+                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  // |a_ii| > eps(UF) selects the diagonals that scale; +inf disables scaling
+  const double eps = !scaling_ ? std::numeric_limits<double>::infinity()
+                     : uf_ == kF64 ? std::numeric_limits<double>::epsilon()
+                                   : static_cast<double>(std::numeric_limits<float>::epsilon());
+  hipLaunchKernelGGL(k_compute_ad, dim3(nblk(n_)), dim3(256), 0, st, n_, d_ax_.as<double>(), d_diag_pos_.as<int32_t>(),
+                     eps, d_ad_.as<double>());
+  hipLaunchKernelGGL(k_scale_entries, dim3(nblk(nnz_)), dim3(256), 0, st, nnz_, d_ax_.as<double>(),
+                     d_entry_row_.as<int32_t>(), d_entry_col_.as<int32_t>(), d_ad_.as<double>(), d_axs_.as<double>());
+  const int64_t ncsr = 2 * nnz_ - n_;
+  hipLaunchKernelGGL(k_gather, dim3(nblk(ncsr)), dim3(256), 0, st, ncsr, d_axs_.as<double>(), d_csr_src_.as<int32_t>(),
+                     d_csr_x_.as<double>());
+  MPIR_HIP(hipGetLastError());
+  int err = 0, pos = 0;
+  if (uf_ == kF64) {
+    FactorArgs<double> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
+                         d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_upd_off_.as<int64_t>(),
+                         d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<double>(),
+                         d_upool_.as<double>(), d_D_.as<double>(), d_Dinv_.as<double>(), d_err_.as<int>()};
+    factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, st);
+    count_positive<double>(n_, d_D_.as<double>(), d_cnt_.as<int>(), st);
+  } else {
+    FactorArgs<float> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
+                        d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_upd_off_.as<int64_t>(),
+                        d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<float>(),
+                        d_upool_.as<float>(), d_D_.as<float>(), d_Dinv_.as<float>(), d_err_.as<int>()};
+    factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, st);
+    count_positive<float>(n_, d_D_.as<float>(), d_cnt_.as<int>(), st);
+  }
+  MPIR_HIP(hipMemcpyAsync(&h_pinned_[0], d_err_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+  MPIR_HIP(hipMemcpyAsync(&h_pinned_[1], d_cnt_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+  MPIR_HIP(hipStreamSynchronize(st));
+  std::memcpy(&err, &h_pinned_[0], sizeof(int));
+  std::memcpy(&pos, &h_pinned_[1], sizeof(int));
+  stats_.positive_pivots = pos;
+  stats_.factor_ms = now_ms() - t0;
+  if (err) throw Error(kErrNumeric, "zero pivot in the LDL^T factorization");
+  factored_ = true;
+}
+
+double GmresLdl::dnrm2(const double* v) {
+  hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, 1,
+                     d_red_.as<double>());
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 1, 1,
+                     d_small_.as<double>() + kSlotNorm);
+  MPIR_HIP(hipMemcpyAsync(h_pinned_, d_small_.as<double>() + kSlotNorm, sizeof(double), hipMemcpyDeviceToHost, stream_));
+  MPIR_HIP(hipStreamSynchronize(stream_));
+  return h_pinned_[0];
+}
+
+double GmresLdl::infnrm(const double* v) {
+  hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, 2,
+                     d_red_.as<double>());
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 2, 0,
+                     d_small_.as<double>() + kSlotNorm);
+  MPIR_HIP(hipMemcpyAsync(h_pinned_, d_small_.as<double>() + kSlotNorm, sizeof(double), hipMemcpyDeviceToHost, stream_));
+  MPIR_HIP(hipStreamSynchronize(stream_));
+  return h_pinned_[0];
+}
+
+void GmresLdl::precond_apply_dev(double* x) {
+  stats_.precond_applications++;
+  if (uf_ == kF64) {
+    SolveArgs<double> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
+                        d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(),
+                        d_child_list_.as<int32_t>(), d_panel_.as<double>(), d_Dinv_.as<double>()};
+    ldl_solve<double, double>(a, dsp_, x, d_uvec_.as<double>(), stream_);
+  } else {
+    SolveArgs<float> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
+                       d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(),
+                       d_child_list_.as<int32_t>(), d_panel_.as<float>(), d_Dinv_.as<float>()};
+    ldl_solve<float, double>(a, dsp_, x, d_uvec_.as<double>(), stream_);
+  }
+}
+
+void GmresLdl::spmv_dev(const double* x, double* y) {
+  CsrDev A{n_, d_csr_p_.as<int32_t>(), d_csr_j_.as<int32_t>(), d_csr_x_.as<double>()};
+  spmv_f64(A, x, y, stream_);
+}
+
+void GmresLdl::precond_gmres(const double* b, double* res) {
+  hipStream_t st = stream_;
+  const int n = n_;
+  max_gmres_ = std::min<int64_t>(max_gmres_, n);  // persists, as gmres_ir.hpp:643
+  const int m = static_cast<int>(max_gmres_);
+  const int64_t ldv = n;
+  MPIR_CHECK(m + 2 <= 4096, "GMRES iteration limit above 4094");
+  d_V_.ensure(static_cast<size_t>(m + 1) * ldv * sizeof(double));
+  d_hcol_.ensure(static_cast<size_t>(m + 2) * sizeof(double));
+  d_red_.ensure(static_cast<size_t>(m + 1) * kRedBlocks * sizeof(double));
+  d_small_.ensure(static_cast<size_t>(kSlotH + m + 2) * sizeof(double));
+  double* V = d_V_.as<double>();
+  MPIR_HIP(hipMemsetAsync(res, 0, n * sizeof(double), st));
+  double rho = dnrm2(b);
+  if (rho < tol_) return;
+  MPIR_HIP(hipMemcpyAsync(V, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  precond_apply_dev(V);
+  rho = dnrm2(V);
+  hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, V, 1.0 / rho, V);
+  std::vector<std::vector<double>> h(m + 1, std::vector<double>(m, 0.0));
+  std::vector<double> g(m + 1, 0.0), c(m, 0.0), s(m, 0.0);
+  g[0] = rho;
+  const unsigned red_blocks = static_cast<unsigned>(std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + 255) / 256)));
+  double* part = d_red_.as<double>();
+  double* hc = d_hcol_.as<double>();
+  int k;
+  for (k = 0; k < m && rho > tol_; ++k) {
+    double* w = V + ldv * (k + 1);
+    spmv_dev(V + ldv * k, w);
+    precond_apply_dev(w);
+    // CGS2: two passes of h = V^T w, w -= V h
+    const int k1 = k + 1;
+    for (int pass = 0; pass < 2; ++pass) {
+      double* hp = pass == 0 ? hc : d_small_.as<double>() + kSlotH;
+      hipLaunchKernelGGL(k_mdot_partial, dim3(red_blocks), dim3(kRedThreads), 0, st, n, V, ldv, k1, w, part);
+      hipLaunchKernelGGL(k_mdot_final, dim3(k1), dim3(kRedThreads), 0, st, part, static_cast<int>(red_blocks), hp, 0);
+      hipLaunchKernelGGL(k_sub_basis, dim3(nblk(n)), dim3(256), 0, st, n, V, ldv, k1, hp, w);
+      if (pass == 1) hipLaunchKernelGGL(k_add, dim3(nblk(k1)), dim3(256), 0, st, k1, hc, hp);
+    }
+    hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, w, nullptr, 1, part);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, part, kRedBlocks, 1, 1, hc + k1);
+    MPIR_HIP(hipMemcpyAsync(h_pinned_, hc, (k1 + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    MPIR_HIP(hipStreamSynchronize(st));
+    for (int j = 0; j <= k; ++j) h[j][k] = h_pinned_[j];
+    h[k + 1][k] = h_pinned_[k1];
+    if (h[k + 1][k] != 0) hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, w, 1.0 / h[k + 1][k], w);
+    for (int j = 0; j < k; ++j) {
+      const double w1 = c[j] * h[j][k] - s[j] * h[j + 1][k];
+      const double w2 = s[j] * h[j][k] + c[j] * h[j + 1][k];
+      h[j][k] = w1;
+      h[j + 1][k] = w2;
+    }
+    const double nu = std::hypot(h[k][k], h[k + 1][k]);
+    if (nu != 0) {
+      c[k] = h[k][k] / nu;
+      s[k] = -h[k + 1][k] / nu;
+      h[k][k] = c[k] * h[k][k] - s[k] * h[k + 1][k];
+      h[k + 1][k] = 0.0;
+      const double w1 = c[k] * g[k] - s[k] * g[k + 1];
+      const double w2 = s[k] * g[k] + c[k] * g[k + 1];
+      g[k] = w1;
+      g[k + 1] = w2;
+    }
+    rho = std::fabs(g[k + 1]);
+  }
+  stats_.gmres_iterations += k;
+  std::vector<double> y(std::max(k, 1), 0.0);
+  for (int i = k; i-- > 0;) {
+    y[i] = g[i];
+    for (int j = i + 1; j < k; ++j) y[i] -= h[i][j] * y[j];
+    y[i] /= h[i][i];
+  }
+  if (k > 0) {
+    std::memcpy(h_pinned_, y.data(), k * sizeof(double));
+    MPIR_HIP(hipMemcpyAsync(hc, h_pinned_, k * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_combine, dim3(nblk(n)), dim3(256), 0, st, n, V, ldv, k, hc, res);
+    MPIR_HIP(hipStreamSynchronize(st));
+  }
+}
+
+void GmresLdl::solve_one(const double* b, double* x) {
+  hipStream_t st = stream_;
+  const int n = n_;
+  CsrDev A{n_, d_csr_p_.as<int32_t>(), d_csr_j_.as<int32_t>(), d_csr_x_.as<double>()};
+  double* r = d_r_.as<double>();
+  double* d = d_d_.as<double>();
+  double* tmp = d_w_.as<double>();
+  precond_gmres(b, x);
+  residual_f64(A, x, b, r, st);
+  const double bnorm = infnrm(b);
+  double r_prev = std::numeric_limits<double>::max();
+  stats_.stop_reason = 0;
+  int64_t iter;
+  double r_norm = 0.0;
+  for (iter = 1; iter <= max_ir_; ++iter) {
+    const double scale = infnrm(r);
+    // An exactly zero residual would make the reference scale by 1/0
+    // (gmres_ir.hpp:562-563) and carry NaN forward; x is exact, so stop.
+    if (scale == 0.0) {
+      stats_.stop_reason = 3;
+      r_norm = 0.0;
+      break;
+    }
+    hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, r, 1.0 / scale, tmp);
+    precond_gmres(tmp, d);
+    hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, d, scale, d);
+    const double d_norm = infnrm(d);
+    const double x_norm = infnrm(x);
+    hipLaunchKernelGGL(k_add, dim3(nblk(n)), dim3(256), 0, st, n, x, d);
+    residual_f64(A, x, b, r, st);
+    r_norm = infnrm(r) / bnorm;
+    stats_.ir_iterations = static_cast<int32_t>(iter);
+    if (d_norm <= x_norm * (10 * std::numeric_limits<double>::epsilon())) {
+      stats_.stop_reason = 1;
+      break;
+    }
+    if (std::fabs(r_norm - r_prev) < tol_) {
+      stats_.stop_reason = 2;
+      break;
+    }
+    r_prev = r_norm;
+  }
+  if (max_ir_ == 0) r_norm = infnrm(r) / bnorm;
+  stats_.final_rel_residual = r_norm;
+}
+
+void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool device_ptr) {
+  if (!factored_) throw Error(kErrState, "solve called before a successful factorization");
+  if (precond_only_) throw Error(kErrState, "handle holds a loaded factor only; call compute first");
+  MPIR_CHECK(nrhs >= 1 && ld >= n_, "bad right-hand-side layout");
+  const double t0 = now_ms();
+  hipStream_t st = stream_;
+  const int n = n_;
+  stats_.ir_iterations = 0;
+  stats_.gmres_iterations = 0;
+  stats_.precond_applications = 0;
+  for (int c = 0; c < nrhs; ++c) {
+    const double* bc = b + ld * c;
+    double* xc = x + ld * c;
+    const double* bin;
+    if (device_ptr) {
+      bin = bc;
+    } else {
+      MPIR_HIP(hipMemcpyAsync(d_r_.get(), bc, n * sizeof(double), hipMemcpyHostToDevice, st));
+      bin = d_r_.as<double>();
+    }
+    hipLaunchKernelGGL(k_perm_in, dim3(nblk(n)), dim3(256), 0, st, n, bin, d_perm_.as<int32_t>(), d_ad_.as<double>(),
+                       d_b_.as<double>());
+    solve_one(d_b_.as<double>(), d_x_.as<double>());
+    double* xout = device_ptr ? xc : d_d_.as<double>();
+    hipLaunchKernelGGL(k_perm_out, dim3(nblk(n)), dim3(256), 0, st, n, d_x_.as<double>(), d_perm_.as<int32_t>(),
+                       d_ad_.as<double>(), xout);
+    if (!device_ptr) MPIR_HIP(hipMemcpyAsync(xc, xout, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    MPIR_HIP(hipStreamSynchronize(st));
+  }
+  stats_.solve_ms = now_ms() - t0;
+}
+
+double GmresLdl::time_kernels(int what, int reps) {
+  if (!factored_) throw Error(kErrState, "time_kernels needs a factorization");
+  MPIR_CHECK(reps >= 1 && what >= 0 && what <= 2, "bad timing request");
+  if (what == 2) MPIR_CHECK(!precond_only_, "no values to refactor");
+  hipStream_t st = stream_;
+  double* x = d_w_.as<double>();
+  double* y = d_d_.as<double>();
+  // a deterministic nonzero input: the scaling vector
+  MPIR_HIP(hipMemcpyAsync(x, d_ad_.get(), n_ * sizeof(double), hipMemcpyDeviceToDevice, st));
+  hipEvent_t e0, e1;
+  MPIR_HIP(hipEventCreate(&e0));
+  MPIR_HIP(hipEventCreate(&e1));
+  auto run = [&]() {
+    if (what == 0) {
+      spmv_dev(x, y);
+    } else if (what == 1) {
+      precond_apply_dev(x);
+    } else if (uf_ == kF64) {
+      FactorArgs<double> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
+                           d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_upd_off_.as<int64_t>(),
+                           d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<double>(),
+                           d_upool_.as<double>(), d_D_.as<double>(), d_Dinv_.as<double>(), d_err_.as<int>()};
+      factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, st);
+    } else {
+      FactorArgs<float> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
+                          d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_upd_off_.as<int64_t>(),
+                          d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<float>(),
+                          d_upool_.as<float>(), d_D_.as<float>(), d_Dinv_.as<float>(), d_err_.as<int>()};
+      factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, st);
+    }
+  };
+  run();  // warm
+  MPIR_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) run();
+  MPIR_HIP(hipEventRecord(e1, st));
+  MPIR_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  MPIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return static_cast<double>(ms) / reps;
+}
+
+void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void* Dinv, int prec) {
+  if (!factored_ && stats_.positive_pivots < 0) throw Error(kErrState, "no factor to export");
+  const Supernodes& S = plan_->sn;
+  const size_t tf = uf_ == kF64 ? 8 : 4;
+  std::vector<char> panel(plan_->panel_entries * tf), D(n_ * tf), Di(n_ * tf);
+  MPIR_HIP(hipMemcpyAsync(panel.data(), d_panel_.get(), panel.size(), hipMemcpyDeviceToHost, stream_));
+  MPIR_HIP(hipMemcpyAsync(D.data(), d_D_.get(), D.size(), hipMemcpyDeviceToHost, stream_));
+  MPIR_HIP(hipMemcpyAsync(Di.data(), d_Dinv_.get(), Di.size(), hipMemcpyDeviceToHost, stream_));
+  MPIR_HIP(hipStreamSynchronize(stream_));
+  auto get = [&](const std::vector<char>& buf, int64_t i) -> double {
+    if (tf == 8) { double v; std::memcpy(&v, buf.data() + i * 8, 8); return v; }
+    float v; std::memcpy(&v, buf.data() + i * 4, 4); return v;
+  };
+  auto put = [&](void* dst, int64_t i, double v) {
+    if (prec == kF64) static_cast<double*>(dst)[i] = v;
+    else static_cast<float*>(dst)[i] = static_cast<float>(v);
+  };
+  int64_t q = 0;
+  Lp[0] = 0;
+  for (int32_t s = 0; s < S.ns; ++s) {
+    const int32_t f = S.first[s], l = S.first[s + 1];
+    const int64_t m = (l - f) + S.nrow[s];
+    for (int32_t j = f; j < l; ++j) {
+      const int64_t cj = j - f;
+      for (int64_t r = cj + 1; r < m; ++r) {
+        if (Li) Li[q] = r < (l - f) ? static_cast<int32_t>(f + r) : S.rows[S.rows_off[s] + (r - (l - f))];
+        if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * m));
+        ++q;
+      }
+      Lp[j + 1] = q;
+      if (Dv) put(Dv, j, get(D, j));
+      if (Dinv) put(Dinv, j, get(Di, j));
+    }
+  }
+}
+
+}  // namespace mpir

@@ -1,0 +1,99 @@
+// Launch interfaces of the HIP kernels (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../plan.hpp"
+
+namespace mpir {
+
+// Device view of a factorization: pattern arrays of the plan plus the
+// numeric storage in factor precision T.
+template <class T>
+struct FactorArgs {
+  const int32_t* first;      // ns+1
+  const int32_t* nrow;       // ns
+  const int64_t* rows_off;   // ns+1
+  const int32_t* rows;       // sum nrow
+  const int32_t* relidx;     // sum nrow
+  const int64_t* panel_off;  // ns+1
+  const int64_t* upd_off;    // ns
+  const int32_t* child_ptr;  // ns+1
+  const int32_t* child_list;
+  T* panel;
+  T* upool;
+  T* D;
+  T* Dinv;
+  int* err;
+};
+
+// Device copies of the plan's task lists, with the host-side offsets the
+// launch loop needs.
+struct DeviceFactorPlan {
+  int nlevels = 0;
+  int64_t panel_entries = 0;
+  const int32_t* small_list = nullptr;
+  const int32_t* big_list = nullptr;
+  const Task4* asm_task = nullptr;
+  const Task4* pan_task = nullptr;
+  const Task4* upd_task = nullptr;
+  const Task4* diag_task = nullptr;
+  const int64_t* val_dst = nullptr;
+  std::vector<int32_t> h_small_ptr, h_big_ptr, h_asm_ptr, h_level_step_ptr, h_pan_ptr, h_upd_ptr, h_diag_ptr;
+};
+
+template <class T>
+void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const double* ax_scaled, int64_t nnz,
+                    hipStream_t st);
+template <class T>
+void count_positive(int n, const T* D, int* out, hipStream_t st);
+
+// ---------------------------------------------------------------- solves
+// Device view of the factor for the triangular solves (factor storage TF,
+// arithmetic in TX).
+template <class TF>
+struct SolveArgs {
+  const int32_t* first;
+  const int32_t* nrow;
+  const int64_t* rows_off;
+  const int32_t* rows;
+  const int32_t* relidx;
+  const int64_t* panel_off;
+  const int32_t* child_ptr;
+  const int32_t* child_list;
+  const TF* panel;
+  const TF* Dinv;
+};
+
+struct DeviceSolvePlan {
+  int nlevels = 0;
+  const int32_t* small_list = nullptr;  // small fronts, grouped by level
+  const int32_t* big_list = nullptr;    // big fronts, grouped by level
+  const Task4* diag_task = nullptr;     // per step: (s, task0, task1, step)
+  const Task4* gemv_task = nullptr;     // per step: (s, row0, row1, step)
+  void* scratch = nullptr;              // partial sums of the backward GEMV
+  std::vector<int32_t> h_small_ptr, h_big_ptr, h_level_step_ptr, h_diag_ptr, h_task_ptr;
+};
+
+// x (length n, permuted order) is overwritten by (L D L^T)^{-1} x.
+// uvec: scratch of sum(nrow) entries of TX.
+template <class TF, class TX>
+void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st);
+
+// ---------------------------------------------------------------- vectors / SpMV
+struct CsrDev {
+  int32_t n = 0;
+  const int32_t* p = nullptr;
+  const int32_t* j = nullptr;
+  const double* x = nullptr;
+};
+
+// y = A x (FP64, rows summed left to right, no fused multiply-add)
+void spmv_f64(const CsrDev& A, const double* x, double* y, hipStream_t st);
+// r = b - A x
+void residual_f64(const CsrDev& A, const double* x, const double* b, double* r, hipStream_t st);
+
+}  // namespace mpir

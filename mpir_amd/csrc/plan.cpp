@@ -1,0 +1,191 @@
+#include "plan.hpp"
+
+#include <algorithm>
+#include <map>
+#include <set>
+#include <stdexcept>
+#include <utility>
+
+namespace mpir {
+
+namespace {
+
+// Best-fit allocator over one linear pool; free blocks are coalesced.
+class PoolAlloc {
+ public:
+  int64_t alloc(int64_t size) {
+    size = (size + 63) & ~int64_t(63);
+    auto it = by_size_.lower_bound({size, -1});
+    if (it != by_size_.end()) {
+      const int64_t bsize = it->first, off = it->second;
+      by_size_.erase(it);
+      by_off_.erase(off);
+      if (bsize > size) insert(off + size, bsize - size);
+      return off;
+    }
+    const int64_t off = top_;
+    top_ += size;
+    return off;
+  }
+  void release(int64_t off, int64_t size) {
+    size = (size + 63) & ~int64_t(63);
+    auto nx = by_off_.lower_bound(off);
+    if (nx != by_off_.end() && off + size == nx->first) {
+      size += nx->second;
+      by_size_.erase({nx->second, nx->first});
+      nx = by_off_.erase(nx);
+    }
+    if (nx != by_off_.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        off = pv->first;
+        size += pv->second;
+        by_size_.erase({pv->second, pv->first});
+        by_off_.erase(pv);
+      }
+    }
+    insert(off, size);
+  }
+  int64_t peak() const { return top_; }
+
+ private:
+  void insert(int64_t off, int64_t size) {
+    by_off_[off] = size;
+    by_size_.insert({size, off});
+  }
+  std::map<int64_t, int64_t> by_off_;
+  std::set<std::pair<int64_t, int64_t>> by_size_;
+  int64_t top_ = 0;
+};
+
+}  // namespace
+
+FactorPlan build_plan_from_supernodes(Supernodes sn) {
+  FactorPlan P;
+  P.sn = std::move(sn);
+  Supernodes& S = P.sn;
+  const int32_t ns = S.ns;
+  P.panel_entries = S.panel_off[ns];
+
+  // children lists
+  P.child_ptr.assign(static_cast<size_t>(ns) + 1, 0);
+  for (int32_t s = 0; s < ns; ++s)
+    if (S.sparent[s] >= 0) P.child_ptr[S.sparent[s] + 1]++;
+  for (int32_t s = 0; s < ns; ++s) P.child_ptr[s + 1] += P.child_ptr[s];
+  P.child_list.resize(P.child_ptr[ns]);
+  {
+    std::vector<int32_t> next(P.child_ptr.begin(), P.child_ptr.end() - 1);
+    for (int32_t s = 0; s < ns; ++s)
+      if (S.sparent[s] >= 0) P.child_list[next[S.sparent[s]]++] = s;
+  }
+
+  // update-matrix pool with level lifetimes
+  P.upd_off.assign(ns, -1);
+  PoolAlloc pool;
+  for (int32_t l = 0; l < S.nlevels; ++l) {
+    for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int32_t s = S.level_sn[q];
+      const int64_t nr = S.nrow[s];
+      if (nr > 0) P.upd_off[s] = pool.alloc(nr * nr);
+    }
+    for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int32_t s = S.level_sn[q];
+      for (int32_t t = P.child_ptr[s]; t < P.child_ptr[s + 1]; ++t) {
+        const int32_t c = P.child_list[t];
+        const int64_t nr = S.nrow[c];
+        if (nr > 0) pool.release(P.upd_off[c], nr * nr);
+      }
+    }
+  }
+  P.upd_pool = pool.peak();
+
+  // front classes, assembly tasks, blocked-factor tasks
+  P.small_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.big_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.asm_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.level_step_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.pan_ptr.assign(1, 0);
+  P.upd_ptr.assign(1, 0);
+  P.diag_ptr.assign(1, 0);
+  for (int32_t l = 0; l < S.nlevels; ++l) {
+    int32_t max_steps = 0;
+    for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int32_t s = S.level_sn[q];
+      const int32_t nc = S.first[s + 1] - S.first[s];
+      const int32_t m = nc + S.nrow[s];
+      if (m <= kSmallFront) {
+        P.small_list.push_back(s);
+        continue;
+      }
+      P.big_list.push_back(s);
+      max_steps = std::max(max_steps, (nc + kPanelNB - 1) / kPanelNB);
+      const bool has_children = P.child_ptr[s + 1] > P.child_ptr[s];
+      for (int32_t c0 = 0; c0 < m; c0 += kAsmCols) {
+        const int32_t c1 = std::min(m, c0 + kAsmCols);
+        // blocks made only of panel columns need work only with children
+        if (c1 <= nc && !has_children) continue;
+        P.asm_task.push_back({s, c0, c1, 0});
+      }
+    }
+    P.small_ptr[l + 1] = static_cast<int32_t>(P.small_list.size());
+    P.big_ptr[l + 1] = static_cast<int32_t>(P.big_list.size());
+    P.asm_ptr[l + 1] = static_cast<int32_t>(P.asm_task.size());
+    for (int32_t k = 0; k < max_steps; ++k) {
+      for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
+        const int32_t s = P.big_list[q];
+        const int32_t nc = S.first[s + 1] - S.first[s];
+        const int32_t m = nc + S.nrow[s];
+        const int32_t k0 = k * kPanelNB;
+        if (k0 >= nc) continue;
+        const int32_t nbk = std::min(kPanelNB, nc - k0);
+        // rows strictly below the diagonal block, in chunks of 256
+        const int32_t pan0 = static_cast<int32_t>(P.pan_task.size());
+        for (int32_t r0 = k0 + nbk; r0 < m; r0 += 256)
+          P.pan_task.push_back({s, r0, std::min(m, r0 + 256), k});
+        P.diag_task.push_back({s, pan0, static_cast<int32_t>(P.pan_task.size()), k});
+        const int32_t t0 = k0 + nbk;
+        for (int32_t c0 = t0; c0 < m; c0 += kTile)
+          for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, k});
+      }
+      P.pan_ptr.push_back(static_cast<int32_t>(P.pan_task.size()));
+      P.upd_ptr.push_back(static_cast<int32_t>(P.upd_task.size()));
+      P.diag_ptr.push_back(static_cast<int32_t>(P.diag_task.size()));
+      P.max_step_tasks = std::max(P.max_step_tasks, P.pan_ptr.back() - P.pan_ptr[P.pan_ptr.size() - 2]);
+    }
+    P.level_step_ptr[l + 1] = P.level_step_ptr[l] + max_steps;
+  }
+  P.nsmall = static_cast<int64_t>(P.small_list.size());
+  P.nbig = static_cast<int64_t>(P.big_list.size());
+  return P;
+}
+
+FactorPlan build_factor_plan(const CscUpper& C, const std::vector<int32_t>& val_map,
+                             const int32_t* parent, const int32_t* Lnz) {
+  FactorPlan P = build_plan_from_supernodes(build_supernodes(C, parent, Lnz));
+  const Supernodes& S = P.sn;
+  const int32_t n = C.n;
+  // value destinations: C entry (i, j), i <= j, is L-view entry (j, i) of
+  // the panel of the supernode that owns column i
+  P.val_dst.assign(C.nnz(), -1);
+  for (int32_t j = 0; j < n; ++j) {
+    for (int32_t q = C.p[j]; q < C.p[j + 1]; ++q) {
+      const int32_t i = C.i[q];
+      const int32_t s = S.col2sn[i];
+      const int32_t f = S.first[s], l = S.first[s + 1];
+      const int64_t m = (l - f) + S.nrow[s];
+      int64_t rp;
+      if (j < l) {
+        rp = j - f;
+      } else {
+        const int32_t* R = S.rows.data() + S.rows_off[s];
+        const int32_t* it = std::lower_bound(R, R + S.nrow[s], j);
+        if (it == R + S.nrow[s] || *it != j) throw std::logic_error("entry outside the L pattern");
+        rp = (l - f) + (it - R);
+      }
+      P.val_dst[val_map[q]] = S.panel_off[s] + static_cast<int64_t>(i - f) * m + rp;
+    }
+  }
+  return P;
+}
+
+}  // namespace mpir

@@ -1,0 +1,66 @@
+// Per-pattern numeric plan for the multifrontal LDL^T and the supernodal
+// triangular solves: everything here depends only on the sparsity pattern,
+// so it is built once on the host and reused for every set of values.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "host/sparse.hpp"
+
+namespace mpir {
+
+// Fronts with at most this many rows are factored by one wavefront in LDS.
+constexpr int kSmallFront = 64;
+// Column-block width of the blocked factorization of large fronts.
+constexpr int kPanelNB = 64;
+// Tile edge of the trailing-update tasks.
+constexpr int kTile = 64;
+// Column-block width of the extend-add assembly tasks.
+constexpr int kAsmCols = 64;
+
+struct Task4 {
+  int32_t a, b, c, d;
+};
+
+struct FactorPlan {
+  Supernodes sn;
+  // children of each supernode, ascending
+  std::vector<int32_t> child_ptr, child_list;
+  // update-matrix pool: U_s is nrow_s x nrow_s column-major at upd_off[s]
+  std::vector<int64_t> upd_off;
+  int64_t upd_pool = 0;
+  // per level: small fronts, big fronts
+  std::vector<int32_t> small_ptr, small_list;
+  std::vector<int32_t> big_ptr, big_list;
+  // per level: assembly tasks (s, c0, c1, 0) over big fronts
+  std::vector<int32_t> asm_ptr;
+  std::vector<Task4> asm_task;
+  // per (level, step): panel tasks (s, row0, row1, step) and update tiles
+  // (s, tile_row0, tile_col0, step); indexed through step_ptr
+  std::vector<int32_t> level_step_ptr;  // nlevels+1 into the step arrays
+  std::vector<int32_t> pan_ptr;          // nsteps_total+1
+  std::vector<Task4> pan_task;
+  std::vector<int32_t> upd_ptr;          // nsteps_total+1
+  std::vector<Task4> upd_task;
+  // per step: the big fronts active in it, (s, pan0, pan1, step) where
+  // [pan0, pan1) are that front's entries of pan_task
+  std::vector<int32_t> diag_ptr;         // nsteps_total+1
+  std::vector<Task4> diag_task;
+  int32_t max_step_tasks = 0;            // max pan tasks in one step
+  // for each entry of the ORIGINAL upper CSC: destination in the panel pool
+  std::vector<int64_t> val_dst;
+  // statistics
+  int64_t panel_entries = 0;
+  int64_t nsmall = 0, nbig = 0;
+};
+
+// Schedule and storage layout of a supernodal partition (no value map).
+FactorPlan build_plan_from_supernodes(Supernodes sn);
+
+// perm: new -> old. C: permuted upper CSC with val_map (C entry -> original
+// entry). parent/Lnz: etree of C.
+FactorPlan build_factor_plan(const CscUpper& C, const std::vector<int32_t>& val_map,
+                             const int32_t* parent, const int32_t* Lnz);
+
+}  // namespace mpir

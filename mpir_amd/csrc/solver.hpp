@@ -1,0 +1,120 @@
+// GMRES-IR solver with an LDL^T preconditioner, resident on one gfx950 GPU.
+// Host-side mirror of GmresLDLIR<UF,UW,UR> (include/MPIR/gmres_ir.hpp:90-315
+// of yex33/MPIR) in its LDL^T form (docs/Design/SoftDetailedDes/MIS.tex:505-568).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels/kernels.hpp"
+#include "plan.hpp"
+
+namespace mpir {
+
+enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1 };
+
+struct SolveStats {
+  int32_t n = 0;
+  int64_t nnz = 0;
+  int64_t nnz_l = 0;
+  int32_t nsupernodes = 0;
+  int32_t nlevels = 0;
+  int32_t max_front = 0;
+  double factor_flops = 0.0;
+  int32_t positive_pivots = -1;
+  int32_t ir_iterations = 0;
+  int32_t gmres_iterations = 0;  // summed over all PrecondGmres calls
+  int32_t precond_applications = 0;
+  int32_t stop_reason = 0;        // 0 max iterations, 1 small update, 2 stagnation, 3 zero residual
+  double final_rel_residual = 0;  // ||r||_inf / ||b_s||_inf in the scaled system
+  double analyze_ms = 0, factor_ms = 0, solve_ms = 0;
+  int64_t device_bytes = 0;
+};
+
+class GmresLdl {
+ public:
+  GmresLdl(int uf, int uw, int ur);
+  ~GmresLdl();
+  GmresLdl(const GmresLdl&) = delete;
+  GmresLdl& operator=(const GmresLdl&) = delete;
+
+  void set_max_ir(int64_t v) { max_ir_ = v; }
+  void set_max_gmres(int64_t v) { max_gmres_ = v; }
+  void set_tol(double v) { tol_ = v; }
+  void set_ordering(int o) { ordering_ = o; }
+  // diagonal scaling of Compute (gmres_ir.hpp:347-363); off for the bare
+  // QDLDL_factor drop-in
+  void set_scaling(bool on) { scaling_ = on; }
+
+  // Symbolic phase on the pattern of the upper CSC (reusable across values).
+  void analyze(int32_t n, const int32_t* Ap, const int32_t* Ai);
+  // Numeric phase: scaling, conversion, factorization. Ax in the original
+  // entry order of the analyzed pattern; host or device pointer.
+  void factor(const double* Ax, bool device_ptr);
+  // x = K^{-1} b for nrhs right-hand sides with leading dimension ld.
+  void solve(const double* b, double* x, int nrhs, int64_t ld, bool device_ptr);
+
+  const SolveStats& stats() const { return stats_; }
+  hipStream_t stream() const { return stream_; }
+  bool analyzed() const { return analyzed_; }
+  bool factored() const { return factored_; }
+  int32_t n() const { return n_; }
+  const std::vector<int32_t>& perm() const { return perm_; }
+
+  // building blocks exposed for tests and the kernel-level C entry points
+  void precond_apply_dev(double* x);  // x := (L D L^T)^{-1} x, permuted order
+  void spmv_dev(const double* x, double* y);  // y = K_s x, permuted, scaled
+  // factor (permuted order) copied out in QDLDL's CSC layout
+  void export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* D, void* Dinv, int prec);
+  // Average time (ms) of `reps` back-to-back runs of one kernel group on the
+  // handle's stream, measured with HIP events: 0 SpMV, 1 M^{-1} application,
+  // 2 numeric factorization (values already loaded by factor()).
+  double time_kernels(int what, int reps);
+  // Load a factor given in QDLDL's CSC layout (host arrays, Lx/Dinv in the
+  // handle's factor precision) for use as M^{-1} only.
+  void load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, const void* Lx, const void* Dinv);
+
+ private:
+  void solve_one(const double* b_dev, double* x_dev);
+  void precond_gmres(const double* b, double* res);  // device vectors (permuted)
+  double dnrm2(const double* v);
+  double infnrm(const double* v);
+  void release_numeric();
+  void upload_plan();
+  void alloc_numeric();
+
+  int uf_, uw_, ur_;
+  int64_t max_ir_ = 10, max_gmres_ = 10;
+  double tol_ = 1e-10;
+  int ordering_ = kOrderAmd;
+  bool scaling_ = true;
+  bool analyzed_ = false, factored_ = false, precond_only_ = false;
+  hipStream_t stream_ = nullptr;
+  int32_t n_ = 0;
+  int64_t nnz_ = 0;
+  SolveStats stats_;
+
+  // pattern
+  std::vector<int32_t> Ap_, Ai_, perm_;
+  std::unique_ptr<FactorPlan> plan_;
+  DeviceFactorPlan dfp_;
+  DeviceSolvePlan dsp_;
+  std::vector<DevBuf> keep_;  // device copies of plan arrays
+  DevBuf d_first_, d_nrow_, d_rows_off_, d_rows_, d_relidx_, d_panel_off_, d_upd_off_, d_child_ptr_, d_child_list_;
+  DevBuf d_csr_p_, d_csr_j_, d_csr_src_, d_csr_x_;
+  DevBuf d_entry_col_, d_entry_row_, d_diag_pos_, d_perm_;
+  // numeric
+  DevBuf d_ax_, d_axs_, d_ad_, d_adp_, d_panel_, d_upool_, d_D_, d_Dinv_, d_err_, d_cnt_;
+  DevBuf d_uvec_, d_scratch_;
+  // GMRES workspace
+  DevBuf d_V_, d_w_, d_b_, d_x_, d_r_, d_d_, d_red_, d_small_;
+  DevBuf d_hcol_, d_givens_;
+  std::vector<double> h_small_;
+  double* h_pinned_ = nullptr;
+};
+
+}  // namespace mpir

@@ -1,0 +1,162 @@
+"""GPU path vs the oracle on the same seeded inputs (through the C ABI).
+
+Bars: SpMV / residual bit-exact (integer and index work and the FP64 sums,
+which keep the reference's order); factor pattern (Lp, Li) and inertia
+bit-exact; factor values and solves within the stated relative tolerances
+(the GPU eliminates in a different order than the up-looking loop);
+GMRES-IR backward error within the requested tolerance.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mpir_amd
+from tests.oracle_lib import permute_upper
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+QDLDL_CASES = ["qdldl_small_natural", "qdldl_small_amd", "qdldl_k6_amd"]
+# relative tolerance of factor values: FP64 / FP32 elimination in another order
+FACTOR_RTOL = {np.float64: 1e-10, np.float32: 2e-3}
+
+
+def golden(name):
+    return dict(np.load(GOLDEN / f"{name}.npz", allow_pickle=False))
+
+
+def backward_error(oracle, Ap, Ai, Ax, x, b):
+    r = b - oracle.matmul(Ap, Ai, Ax, x)
+    knorm = np.abs(oracle.matmul(Ap, Ai, np.abs(Ax), np.ones(len(b)))).max()
+    return np.abs(r).max() / (knorm * np.abs(x).max() + np.abs(b).max())
+
+
+@pytest.mark.parametrize("cfg,k", [(0, None), (1, 12), (1, 24)])
+def test_spmv_and_residual_bit_exact(gpu, oracle, cfg, k):
+    P = mpir_amd.kkt_config(cfg, k=k)
+    x = np.random.default_rng(1).uniform(-1, 1, P.n)
+    y = mpir_amd.spmv_upper(P.Ap, P.Ai, P.Ax, x)
+    np.testing.assert_array_equal(y, oracle.matmul(P.Ap, P.Ai, P.Ax, x))
+    r = mpir_amd.residual_upper(P.Ap, P.Ai, P.Ax, x, P.b)
+    np.testing.assert_array_equal(r, P.b - oracle.matmul(P.Ap, P.Ai, P.Ax, x))
+
+
+def test_spmv_integer_inputs_exact(gpu):
+    P = mpir_amd.kkt_config(0)
+    Ax = np.round(P.Ax * 8)
+    x = np.random.default_rng(2).integers(-50, 50, P.n).astype(np.float64)
+    U = np.zeros((P.n, P.n))
+    for j in range(P.n):
+        U[P.Ai[P.Ap[j]:P.Ap[j + 1]], j] = Ax[P.Ap[j]:P.Ap[j + 1]]
+    K = U + np.triu(U, 1).T
+    np.testing.assert_array_equal(mpir_amd.spmv_upper(P.Ap, P.Ai, Ax, x), K @ x)
+
+
+@pytest.mark.parametrize("case", QDLDL_CASES)
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_qdldl_factor_dropin(gpu, case, dt):
+    g = golden(case)
+    tag = "f64" if dt == np.float64 else "f32"
+    n = len(g["Ap"]) - 1
+    lnz = int(g["etree_sum"])
+    Lp = np.zeros(n + 1, np.int32)
+    Li = np.zeros(lnz, np.int32)
+    Lx = np.zeros(lnz, dt)
+    D = np.zeros(n, dt)
+    Dinv = np.zeros(n, dt)
+    ret = mpir_amd.QDLDL_factor(n, g["Ap"], g["Ai"], g["Ax"].astype(dt), Lp, Li, Lx, D, Dinv, g["Lnz"], g["etree"])
+    assert ret == int(g[f"{tag}_ret"])
+    np.testing.assert_array_equal(Lp, g[f"{tag}_Lp"])
+    np.testing.assert_array_equal(Li, g[f"{tag}_Li"])
+    rtol = FACTOR_RTOL[dt]
+    scale = np.abs(g[f"{tag}_Lx"]).max()
+    np.testing.assert_allclose(Lx, g[f"{tag}_Lx"], rtol=rtol, atol=rtol * scale)
+    np.testing.assert_allclose(D, g[f"{tag}_D"], rtol=rtol)
+    np.testing.assert_allclose(Dinv, g[f"{tag}_Dinv"], rtol=rtol)
+
+
+@pytest.mark.parametrize("case", QDLDL_CASES)
+def test_qdldl_solve_dropin(gpu, case):
+    g = golden(case)
+    n = len(g["Ap"]) - 1
+    # FP64 factor, FP64 x: the golden factor itself, solved on the GPU
+    x = g["b"].copy()
+    mpir_amd.QDLDL_solve(n, g["f64_Lp"], g["f64_Li"], g["f64_Lx"], g["f64_Dinv"], x)
+    np.testing.assert_allclose(x, g["f64_solve_f64"], rtol=1e-11, atol=1e-12 * np.abs(x).max())
+    # FP32 factor, FP64 x
+    x = g["b"].copy()
+    mpir_amd.QDLDL_solve(n, g["f32_Lp"], g["f32_Li"], g["f32_Lx"], g["f32_Dinv"], x)
+    np.testing.assert_allclose(x, g["f32_solve_f64"], rtol=1e-11, atol=1e-12 * np.abs(x).max())
+
+
+def test_qdldl_zero_pivot_dropin(gpu):
+    Ap = np.array([0, 1, 3], np.int32)
+    Ai = np.array([0, 0, 1], np.int32)
+    Ax = np.array([1.0, 1.0, 1.0])
+    Lp, Li, Lx = np.zeros(3, np.int32), np.zeros(1, np.int32), np.zeros(1)
+    D, Dinv = np.zeros(2), np.zeros(2)
+    assert mpir_amd.QDLDL_factor(2, Ap, Ai, Ax, Lp, Li, Lx, D, Dinv, np.array([1, 0], np.int32),
+                                 np.array([1, -1], np.int32)) == -1
+
+
+@pytest.mark.parametrize("cfg,k,uf", [(0, None, np.float32), (0, None, np.float64), (1, 12, np.float32),
+                                      (1, 24, np.float32), (1, 24, np.float64)])
+def test_gmres_ir_matches_oracle_outcome(gpu, oracle, cfg, k, uf):
+    P = mpir_amd.kkt_config(cfg, k=k)
+    tol = 1e-12
+    s = mpir_amd.GmresLDLIR(uf)
+    s.SetTolerance(tol)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    st = s.stats()
+    assert st["positive_pivots"] == P.n1  # inertia of K: n1 positive, m negative
+    x = s.Solve(P.b)
+    eta = backward_error(oracle, P.Ap, P.Ai, P.Ax, x, P.b)
+    perm = s.perm()
+    rc, xo, info = oracle.gmres_ir(P.Ap, P.Ai, P.Ax, P.b, perm=perm, uf=uf, tol=tol)
+    eta_o = backward_error(oracle, P.Ap, P.Ai, P.Ax, xo, P.b)
+    assert rc == 0
+    assert eta < 1e-13, (eta, eta_o, s.stats())
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-8 * np.abs(xo).max())
+
+
+def test_gmres_ir_factor_pattern_matches_oracle(gpu, oracle):
+    # the GPU factor of P K_s P^T has exactly the oracle's pattern and inertia
+    P = mpir_amd.kkt_config(1, k=10)
+    s = mpir_amd.GmresLDLIR(np.float64)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    Lp, Li, Lx, D, Dinv = s.export_factor()
+    perm = s.perm()
+    # same scaling as the reference's Compute (gmres_ir.hpp:347-363)
+    ad = np.ones(P.n)
+    for j in range(P.n):
+        q = P.Ap[j + 1] - 1
+        if P.Ai[q] == j and abs(P.Ax[q]) > np.finfo(np.float64).eps:
+            ad[j] = 1.0 / np.sqrt(abs(P.Ax[q]))
+    cols = np.repeat(np.arange(P.n), np.diff(P.Ap))
+    Axs = P.Ax * (ad[P.Ai] * ad[cols])
+    Cp, Ci, Cx = permute_upper(P.Ap, P.Ai, Axs, perm)
+    ret, F = oracle.factor(Cp, Ci, Cx, np.float64)
+    assert ret == P.n1 == s.stats()["positive_pivots"]
+    np.testing.assert_array_equal(Lp, F["Lp"])
+    np.testing.assert_array_equal(Li, F["Li"])
+    np.testing.assert_allclose(Lx, F["Lx"], rtol=1e-9, atol=1e-11 * np.abs(F["Lx"]).max())
+    np.testing.assert_allclose(D, F["D"], rtol=1e-10)
+
+
+def test_gmres_ir_multiple_rhs_and_reuse(gpu, oracle):
+    P = mpir_amd.kkt_config(0)
+    s = mpir_amd.GmresLDLIR(np.float32)
+    s.SetTolerance(1e-12)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-1, 1, (P.n, 3))
+    B = np.stack([oracle.matmul(P.Ap, P.Ai, P.Ax, X[:, c]) for c in range(3)], axis=1)
+    Xs = s.Solve(B)
+    for c in range(3):
+        assert backward_error(oracle, P.Ap, P.Ai, P.Ax, Xs[:, c], B[:, c]) < 1e-13
+        np.testing.assert_allclose(s.Solve(B[:, c]), Xs[:, c], rtol=0, atol=0)  # deterministic
+    # fresh values on the same pattern reuse the symbolic analysis
+    Ax2 = P.Ax * (1.0 + 0.01 * rng.uniform(-1, 1, len(P.Ax)))
+    s.Compute(P.Ap, P.Ai, Ax2)
+    x2 = s.Solve(P.b)
+    assert backward_error(oracle, P.Ap, P.Ai, Ax2, x2, P.b) < 1e-13

@@ -1,0 +1,118 @@
+"""Host-side pieces of libmpir_hip.so (no GPU work): the C ABI exports, the
+workload generator, the AMD ordering and the QDLDL_etree drop-in."""
+import json
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import mpir_amd
+from mpir_amd import _lib
+from tests.oracle_lib import permute_upper
+
+ROOT = Path(__file__).resolve().parent.parent
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "mpir_hip.h").read_text()
+    return sorted(set(re.findall(r"\b(mpir_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 30
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_generator_is_deterministic_and_pinned(oracle):
+    import hashlib
+    P = mpir_amd.kkt_config(0)
+    man = json.loads((GOLDEN / "manifest.json").read_text())
+    h = hashlib.sha256()
+    for a in (P.Ap, P.Ai, P.Ax, P.x_true, P.b):
+        h.update(np.ascontiguousarray(a).tobytes())
+    assert h.hexdigest() == man["cfg0_generator_sha256"]
+    assert (P.n, P.n1, int(P.Ap[-1])) == (1280, 1024, 4032)
+    # b = K x_true exactly as the reference's MatrixMultiply sums it
+    np.testing.assert_array_equal(oracle.matmul(P.Ap, P.Ai, P.Ax, P.x_true), P.b)
+
+
+def test_generator_workload_sizes():
+    P = mpir_amd.kkt_config(1, k=16)
+    n1 = 16 ** 3
+    assert P.n1 == n1 and P.n == n1 + n1 // 4
+    # every constraint row has 4 distinct grid columns plus its -D diagonal
+    cols = np.diff(P.Ap)[n1:]
+    assert np.all(cols == 5)
+    assert np.all(P.Ax[P.Ap[n1 + 1:] - 1] == -1e-4)
+
+
+def test_amd_valid_deterministic_and_reduces_fill(oracle):
+    P = mpir_amd.kkt_config(0)
+    p1 = mpir_amd.amd_order(P.Ap, P.Ai)
+    p2 = mpir_amd.amd_order(P.Ap, P.Ai)
+    np.testing.assert_array_equal(p1, p2)
+    assert sorted(p1.tolist()) == list(range(P.n))
+    g = dict(np.load(GOLDEN / "cfg0_symbolic.npz", allow_pickle=False))
+    np.testing.assert_array_equal(p1, g["perm"])  # own-path regression pin
+    Cp, Ci, _ = permute_upper(P.Ap, P.Ai, P.Ax, p1)
+    fill_amd = oracle.etree(Cp, Ci)[0]
+    fill_nat = oracle.etree(P.Ap, P.Ai)[0]
+    assert fill_amd < 0.25 * fill_nat
+
+
+def test_amd_fill_close_to_scipy_minimum_degree(oracle):
+    # SuperLU's multiple minimum degree on A^T + A as an independent yardstick
+    sp = pytest.importorskip("scipy.sparse")
+    spl = pytest.importorskip("scipy.sparse.linalg")
+    P = mpir_amd.kkt_config(1, k=12)
+    p = mpir_amd.amd_order(P.Ap, P.Ai)
+    Cp, Ci, _ = permute_upper(P.Ap, P.Ai, P.Ax, p)
+    ours = oracle.etree(Cp, Ci)[0]
+    U = sp.csc_matrix((P.Ax, P.Ai, P.Ap), shape=(P.n, P.n))
+    K = (U + sp.triu(U, 1).T).tocsc()
+    lu = spl.splu(K, permc_spec="MMD_AT_PLUS_A", diag_pivot_thresh=0.0, options=dict(SymmetricMode=True))
+    mmd = lu.L.nnz - P.n
+    assert ours < 1.15 * mmd
+
+
+@pytest.mark.parametrize("case", ["qdldl_small_natural", "qdldl_small_amd", "qdldl_k6_amd"])
+def test_qdldl_etree_dropin_bit_exact(case):
+    g = dict(np.load(GOLDEN / f"{case}.npz", allow_pickle=False))
+    n = len(g["Ap"]) - 1
+    for dt in (np.int32, np.int64):
+        Ap, Ai = g["Ap"].astype(dt), g["Ai"].astype(dt)
+        work, Lnz, et = (np.zeros(n, dt) for _ in range(3))
+        tot = mpir_amd.QDLDL_etree(n, Ap, Ai, work, Lnz, et)
+        assert tot == int(g["etree_sum"])
+        np.testing.assert_array_equal(Lnz, g["Lnz"])
+        np.testing.assert_array_equal(et, g["etree"])
+
+
+def test_qdldl_etree_error_codes(capfd):
+    w, l, e = (np.zeros(2, np.int32) for _ in range(3))
+    # empty column -> -1
+    assert mpir_amd.QDLDL_etree(2, np.array([0, 1, 1], np.int32), np.array([0], np.int32), w, l, e) == -1
+    # entry below the diagonal -> -1 and the reference's message
+    assert mpir_amd.QDLDL_etree(2, np.array([0, 2, 3], np.int32), np.array([0, 1, 1], np.int32), w, l, e) == -1
+    assert "not upper triangular" in capfd.readouterr().out
+
+
+def test_qdldl_etree_overflow_code():
+    # row 0 coupled to every column: L is dense, sum(Lnz) = n (n - 1) / 2
+    n = 65600
+    Ap = np.arange(0, 2 * n + 1, 2, dtype=np.int64)
+    Ap[1:] -= 1
+    Ai = np.empty(2 * n - 1, np.int64)
+    Ai[0] = 0
+    Ai[1::2] = 0
+    Ai[2::2] = np.arange(1, n)
+    work, Lnz, et = (np.zeros(n, np.int32) for _ in range(3))
+    assert mpir_amd.QDLDL_etree(n, Ap.astype(np.int32), Ai.astype(np.int32), work, Lnz, et) == -2
+    w64, l64, e64 = (np.zeros(n, np.int64) for _ in range(3))
+    assert mpir_amd.QDLDL_etree(n, Ap, Ai, w64, l64, e64) == n * (n - 1) // 2

@@ -1,0 +1,235 @@
+"""The oracle (oracle/mpir_oracle.c) pinned before anything is compared with it.
+
+* golden vectors written by tests/golden/make_golden.py from the reference's
+  own qdldl.hpp / ops.hpp (bit-exact);
+* the compiled reference headers on fresh random inputs (bit-exact);
+* ops.hpp's doctest known answers (include/MPIR/ops.hpp:63-351);
+* brute-force symbolic elimination, dense reconstruction and dense solves;
+* GMRES-IR closed forms: K = I, and an exact FP64 factor (one Krylov step).
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.oracle_lib import permute_upper
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+QDLDL_CASES = ["qdldl_small_natural", "qdldl_small_amd", "qdldl_k6_amd"]
+
+
+def load(name):
+    return dict(np.load(GOLDEN / f"{name}.npz", allow_pickle=False))
+
+
+def test_fixture_manifest():
+    import hashlib
+    man = json.loads((GOLDEN / "manifest.json").read_text())
+    for fname, digest in man["files"].items():
+        assert hashlib.sha256((GOLDEN / fname).read_bytes()).hexdigest() == digest, fname
+
+
+@pytest.mark.parametrize("case", QDLDL_CASES)
+def test_etree_matches_reference_golden(oracle, case):
+    g = load(case)
+    tot, Lnz, et = oracle.etree(g["Ap"], g["Ai"])
+    assert tot == int(g["etree_sum"])
+    np.testing.assert_array_equal(Lnz, g["Lnz"])
+    np.testing.assert_array_equal(et, g["etree"])
+
+
+@pytest.mark.parametrize("case", QDLDL_CASES)
+@pytest.mark.parametrize("tag,dt", [("f64", np.float64), ("f32", np.float32)])
+def test_factor_and_solve_match_reference_golden(oracle, case, tag, dt):
+    g = load(case)
+    ret, F = oracle.factor(g["Ap"], g["Ai"], g["Ax"], dt)
+    assert ret == int(g[f"{tag}_ret"])
+    for key in ("Lp", "Li", "Lx", "D", "Dinv"):
+        np.testing.assert_array_equal(F[key], g[f"{tag}_{key}"], err_msg=key)
+    x = oracle.solve(F["Lp"], F["Li"], F["Lx"], F["Dinv"], g["b"], np.float64)
+    np.testing.assert_array_equal(x, g[f"{tag}_solve_f64"])
+    if dt == np.float32:
+        x32 = oracle.solve(F["Lp"], F["Li"], F["Lx"], F["Dinv"], g["b"], np.float32)
+        np.testing.assert_array_equal(x32, g[f"{tag}_solve_f32"])
+
+
+@pytest.mark.parametrize("case", QDLDL_CASES)
+def test_matmul_matches_reference_golden(oracle, case):
+    g = load(case)
+    np.testing.assert_array_equal(oracle.matmul(g["Ap"], g["Ai"], g["Ax"], g["b"]), g["matmul"])
+
+
+def test_ops_match_reference_golden(oracle):
+    g = load("ops_vectors")
+    for key in ("v_rand", "w_rand", "v_odd", "w_odd", "v_big", "v_tiny", "v_mixed"):
+        assert oracle.dnrm2(g[key]) == g[f"dnrm2_{key}"], key
+        assert oracle.infnrm(g[key]) == g[f"infnrm_{key}"], key
+    assert oracle.dot(g["v_rand"], g["w_rand"]) == g["dot_rand"]
+    assert oracle.dot(g["v_odd"], g["w_odd"]) == g["dot_odd"]
+
+
+def test_ops_doctest_known_answers(oracle):
+    # include/MPIR/ops.hpp:202-223 (VectorDot), 289-316 (Dnrm2), 331-351 (InfNrm)
+    assert oracle.dot([1.1, 2.2, 3.3], [4.4, 5.5, 6.6]) == pytest.approx(38.72)
+    assert oracle.dot([1.5, 2.5, 3.5], [4.5, 5.5, 6.5]) == pytest.approx(43.25)
+    assert oracle.dot([], []) == 0.0
+    assert oracle.dnrm2([3.0, 4.0]) == pytest.approx(5.0)
+    assert oracle.dnrm2([1.0, 2.0, 2.0]) == pytest.approx(3.0)
+    assert oracle.dnrm2([0.0, 0.0, 0.0]) == 0.0
+    assert oracle.dnrm2([5.5]) == pytest.approx(5.5)
+    v = [3.14, 2.71, 1.41]
+    assert oracle.dnrm2(v) == pytest.approx(np.sqrt(np.dot(v, v)))
+    assert oracle.dnrm2([]) == 0.0
+    assert oracle.infnrm([3.0, -4.0, 5.5, -6.1]) == pytest.approx(6.1)
+    assert oracle.infnrm([-1.1, -2.2, -3.3]) == pytest.approx(3.3)
+    assert oracle.infnrm([0.0, 0.0, 0.0]) == 0.0
+    assert oracle.infnrm([7.5]) == pytest.approx(7.5)
+    assert oracle.infnrm([-3.14, 2.71, -1.41, 5.89]) == pytest.approx(5.89)
+    # scaled sum of squares survives values whose squares overflow / underflow
+    assert oracle.dnrm2([3e200, 4e200]) == pytest.approx(5e200)
+    assert oracle.dnrm2([3e-200, 4e-200]) == pytest.approx(5e-200)
+
+
+def random_qd(n1, m, density, seed):
+    """Random quasi-definite upper CSC: SPD H (diagonally dominant), random A, -D."""
+    rng = np.random.default_rng(seed)
+    n = n1 + m
+    K = np.zeros((n, n))
+    H = (rng.random((n1, n1)) < density) * rng.standard_normal((n1, n1))
+    H = np.triu(H, 1)
+    H = H + H.T
+    H += np.diag(np.abs(H).sum(1) + 1.0)
+    K[:n1, :n1] = H
+    A = (rng.random((m, n1)) < density) * rng.standard_normal((m, n1))
+    K[n1:, :n1] = A
+    K[:n1, n1:] = A.T
+    K[n1:, n1:] = -np.diag(rng.uniform(0.1, 1.0, m))
+    U = np.triu(K)
+    Ap = np.zeros(n + 1, np.int32)
+    rows, vals = [], []
+    for j in range(n):
+        r = np.nonzero(U[:, j])[0]
+        rows.extend(r)
+        vals.extend(U[r, j])
+        Ap[j + 1] = len(rows)
+    return K, Ap, np.array(rows, np.int32), np.array(vals)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_matches_compiled_reference_random(oracle, ref, seed):
+    K, Ap, Ai, Ax = random_qd(20 + 3 * seed, 7 + seed, 0.15, seed)
+    assert oracle.etree(Ap, Ai)[0] == ref.etree(Ap, Ai)[0]
+    for dt in (np.float64, np.float32):
+        r1, F1 = oracle.factor(Ap, Ai, Ax, dt)
+        r2, F2 = ref.factor(Ap, Ai, Ax, dt)
+        assert r1 == r2
+        for key in ("Lp", "Li", "Lx", "D", "Dinv"):
+            np.testing.assert_array_equal(F1[key], F2[key])
+        b = np.random.default_rng(seed + 100).uniform(-1, 1, len(Ap) - 1)
+        np.testing.assert_array_equal(oracle.solve(F1["Lp"], F1["Li"], F1["Lx"], F1["Dinv"], b),
+                                      ref.solve(F2["Lp"], F2["Li"], F2["Lx"], F2["Dinv"], b))
+        np.testing.assert_array_equal(oracle.matmul(Ap, Ai, Ax, b), ref.matmul(Ap, Ai, Ax, b))
+    v = np.random.default_rng(seed).standard_normal(37 + seed)
+    w = np.random.default_rng(seed + 1).standard_normal(37 + seed)
+    assert oracle.dot(v, w) == ref.dot(v, w)
+    assert oracle.dnrm2(v) == ref.dnrm2(v)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_etree_brute_force(oracle, seed):
+    # boolean elimination on the dense pattern: L pattern, counts, parents
+    K, Ap, Ai, Ax = random_qd(12, 5, 0.2, 50 + seed)
+    n = K.shape[0]
+    pat = K != 0
+    for j in range(n):
+        rows = [i for i in range(j + 1, n) if pat[i, j]]
+        for a in rows:
+            for b in rows:
+                pat[a, b] = True
+    Lnz = np.array([np.count_nonzero(pat[j + 1:, j]) for j in range(n)])
+    parent = np.array([(j + 1 + np.argmax(pat[j + 1:, j])) if Lnz[j] else -1 for j in range(n)])
+    tot, oLnz, oet = oracle.etree(Ap, Ai)
+    np.testing.assert_array_equal(oLnz, Lnz)
+    np.testing.assert_array_equal(oet, parent)
+    assert tot == Lnz.sum()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_factor_reconstructs_and_solves(oracle, seed):
+    K, Ap, Ai, Ax = random_qd(25, 9, 0.2, 80 + seed)
+    n = K.shape[0]
+    ret, F = oracle.factor(Ap, Ai, Ax, np.float64)
+    assert ret == 25  # inertia of a quasi-definite matrix: n1 positive pivots
+    L = np.eye(n)
+    for j in range(n):
+        for q in range(F["Lp"][j], F["Lp"][j + 1]):
+            L[F["Li"][q], j] = F["Lx"][q]
+    err = np.linalg.norm(K - L @ np.diag(F["D"]) @ L.T) / np.linalg.norm(K)
+    assert err < 1e-14
+    b = np.random.default_rng(seed).uniform(-1, 1, n)
+    x = oracle.solve(F["Lp"], F["Li"], F["Lx"], F["Dinv"], b)
+    np.testing.assert_allclose(x, np.linalg.solve(K, b), rtol=1e-11, atol=1e-12)
+
+
+def test_zero_pivot_and_bad_input(oracle):
+    # [[1, 1], [1, 1]] has an exact zero pivot
+    ret, _ = oracle.factor(np.array([0, 1, 3], np.int32), np.array([0, 0, 1], np.int32), np.array([1.0, 1.0, 1.0]))
+    assert ret == -1
+    # empty column / entry below the diagonal
+    assert oracle.etree(np.array([0, 1, 1], np.int32), np.array([0], np.int32))[0] == -1
+    assert oracle.etree(np.array([0, 2, 3], np.int32), np.array([0, 1, 1], np.int32))[0] == -1
+
+
+def identity_upper(n):
+    return np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32), np.ones(n)
+
+
+def test_gmres_identity_closed_form(oracle):
+    Ap, Ai, Ax = identity_upper(9)
+    b = np.linspace(-1, 1, 9)
+    rc, x, info = oracle.gmres_ir(Ap, Ai, Ax, b)
+    assert rc == 0
+    np.testing.assert_array_equal(x, b)
+    # every PrecondGmres call ends after one Arnoldi step (happy breakdown):
+    # applications of M^{-1} = one initial solve + one per step, per call
+    assert info["gmres_iterations"] == info["precond"] / 2
+    assert info["gmres_iterations"] <= 2
+    assert info["stop_reason"] in (1, 3)  # small update or exactly zero residual
+
+
+def test_gmres_exact_factor_one_step(oracle):
+    # FP64 factor of K: M^{-1} K = I to rounding, every PrecondGmres call stops after one step
+    K, Ap, Ai, Ax = random_qd(30, 10, 0.15, 7)
+    b = K @ np.random.default_rng(1).uniform(-1, 1, K.shape[0])
+    rc, x, info = oracle.gmres_ir(Ap, Ai, Ax, b, uf=np.float64, tol=1e-10)
+    assert rc == 0
+    calls = info["precond"]
+    # calls = 1 initial solve per PrecondGmres + 1 per Arnoldi step
+    assert info["gmres_iterations"] == calls / 2
+    eta = np.abs(b - K @ x).max() / (np.abs(K).sum(1).max() * np.abs(x).max() + np.abs(b).max())
+    assert eta < 1e-15
+
+
+def test_gmres_ir_cfg0_fp32_factor(oracle):
+    import mpir_amd
+    P = mpir_amd.kkt_config(0)
+    perm = mpir_amd.amd_order(P.Ap, P.Ai)
+    rc, x, info = oracle.gmres_ir(P.Ap, P.Ai, P.Ax, P.b, perm=perm, uf=np.float32, tol=1e-12)
+    assert rc == 0 and info["factor_ret"] == P.n1
+    y = oracle.matmul(P.Ap, P.Ai, P.Ax, x)
+    knorm = np.abs(oracle.matmul(P.Ap, P.Ai, np.abs(P.Ax), np.ones(P.n))).max()
+    eta = np.abs(P.b - y).max() / (knorm * np.abs(x).max() + np.abs(P.b).max())
+    assert eta < 1e-14
+    assert np.abs(x - P.x_true).max() < 1e-9
+
+
+def test_permute_helper_matches_reference_pattern(oracle):
+    import mpir_amd
+    P = mpir_amd.kkt_config(0)
+    g = load("cfg0_symbolic")
+    Cp, Ci, Cx = permute_upper(P.Ap, P.Ai, P.Ax, g["perm"])
+    tot, Lnz, et = oracle.etree(Cp, Ci)
+    assert tot == int(g["etree_sum"])
+    np.testing.assert_array_equal(Lnz, g["Lnz"])
+    np.testing.assert_array_equal(et, g["etree"])
