@@ -343,19 +343,35 @@ void GmresLdl::upload_plan() {
   dfp_.h_pan_ptr = P.pan_ptr;
   dfp_.h_upd_ptr = P.upd_ptr;
   dfp_.h_diag_ptr = P.diag_ptr;
+  keep_.resize(16);
+  upload_vec(keep_[12], P.rpart_task, st);
+  upload_vec(keep_[13], P.part_off, st);
+  keep_[14].alloc(static_cast<size_t>(std::max<int64_t>(P.npart, 1)) * sizeof(double));
+  upload_vec(keep_[7], P.sfwd_task, st);
+  upload_vec(keep_[8], P.sbwd_task, st);
+  upload_vec(keep_[9], P.flag_off, st);
+  keep_[10].alloc(static_cast<size_t>(std::max(P.nflags, 1)) * sizeof(int));
+  MPIR_HIP(hipMemsetAsync(keep_[10].get(), 0, keep_[10].bytes(), st));
+  keep_[11].alloc(sizeof(int));
+  MPIR_HIP(hipMemsetAsync(keep_[11].get(), 0, sizeof(int), st));
+  solve_epoch_ = 0;
   dsp_ = DeviceSolvePlan();
   dsp_.nlevels = S.nlevels;
   dsp_.small_list = dfp_.small_list;
-  dsp_.big_list = dfp_.big_list;
-  dsp_.diag_task = dfp_.diag_task;
-  dsp_.gemv_task = dfp_.pan_task;
+  dsp_.fwd_task = keep_[7].as<int32_t>();
+  dsp_.bwd_task = keep_[8].as<int32_t>();
+  dsp_.flag_off = keep_[9].as<int32_t>();
+  dsp_.flags = keep_[10].as<int>();
+  dsp_.err = keep_[11].as<int>();
   dsp_.h_small_ptr = P.small_ptr;
-  dsp_.h_big_ptr = P.big_ptr;
-  dsp_.h_level_step_ptr = P.level_step_ptr;
-  dsp_.h_diag_ptr = P.diag_ptr;
-  dsp_.h_task_ptr = P.pan_ptr;
-  d_scratch_.alloc(std::max<int64_t>(1, static_cast<int64_t>(P.max_step_tasks) * kPanelNB) * sizeof(double));
-  dsp_.scratch = d_scratch_.get();
+  dsp_.h_fwd_ptr = P.sfwd_ptr;
+  dsp_.h_bwd_ptr = P.sbwd_ptr;
+  dsp_.h_rpart_ptr = P.rpart_ptr;
+  dsp_.rpart_task = keep_[12].as<int32_t>();
+  dsp_.part_off = keep_[13].as<int64_t>();
+  dsp_.part = keep_[14].get();
+  d_scratch_.alloc(std::max<int32_t>(S.n, 1) * sizeof(double));
+  dsp_.ybuf = d_scratch_.get();
   d_uvec_.alloc(std::max<int64_t>(1, static_cast<int64_t>(S.rows.size())) * sizeof(double));
 }
 
@@ -490,12 +506,12 @@ void GmresLdl::precond_apply_dev(double* x) {
     SolveArgs<double> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
                         d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(),
                         d_child_list_.as<int32_t>(), d_panel_.as<double>(), d_Dinv_.as<double>()};
-    ldl_solve<double, double>(a, dsp_, x, d_uvec_.as<double>(), stream_);
+    ldl_solve<double, double>(a, dsp_, x, d_uvec_.as<double>(), ++solve_epoch_, stream_);
   } else {
     SolveArgs<float> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
                        d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(),
                        d_child_list_.as<int32_t>(), d_panel_.as<float>(), d_Dinv_.as<float>()};
-    ldl_solve<float, double>(a, dsp_, x, d_uvec_.as<double>(), stream_);
+    ldl_solve<float, double>(a, dsp_, x, d_uvec_.as<double>(), ++solve_epoch_, stream_);
   }
 }
 
@@ -659,7 +675,19 @@ void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool devi
     if (!device_ptr) MPIR_HIP(hipMemcpyAsync(xc, xout, n * sizeof(double), hipMemcpyDeviceToHost, st));
     MPIR_HIP(hipStreamSynchronize(st));
   }
+  check_solve_flags();
   stats_.solve_ms = now_ms() - t0;
+}
+
+void GmresLdl::check_solve_flags() {
+  int err = 0;
+  MPIR_HIP(hipMemcpyAsync(&h_pinned_[2], dsp_.err, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  MPIR_HIP(hipStreamSynchronize(stream_));
+  std::memcpy(&err, &h_pinned_[2], sizeof(int));
+  if (err) {
+    MPIR_HIP(hipMemsetAsync(dsp_.err, 0, sizeof(int), stream_));
+    throw Error(kErrInternal, "triangular solve: a dependency wait expired");
+  }
 }
 
 double GmresLdl::time_kernels(int what, int reps) {
@@ -700,6 +728,7 @@ double GmresLdl::time_kernels(int what, int reps) {
   MPIR_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   MPIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+  check_solve_flags();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return static_cast<double>(ms) / reps;

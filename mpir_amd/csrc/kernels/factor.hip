@@ -94,23 +94,27 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
 }
 
 // ---------------------------------------------------------------- assembly
-// Column-block ownership: the task (s, c0, c1) zeroes the update-matrix
-// columns of [c0, c1) and adds every child's contribution that lands in those
-// columns, children in fixed order, so no two blocks write the same entry.
+// Block ownership: the task (s, c0, c1, t0) owns the entries of the front
+// in columns [c0, c1) and rows [t0, t0 + kAsmRows): it zeroes them in the
+// update matrix and adds every child's contribution that lands there,
+// children in fixed order, so no two tasks write the same entry. Child
+// columns (rows) landing in a range of the parent are contiguous because the
+// relative indices increase.
 template <class T>
 __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   const Task4 tk = tasks[blockIdx.x];
-  const int s = tk.a, c0 = tk.b, c1 = tk.c;
+  const int s = tk.a, c0 = tk.b, c1 = tk.c, t0 = tk.d;
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
   const int nr = a.nrow[s];
   const int m = nc + nr;
+  const int t1 = min(m, t0 + kAsmRows);
   T* P = a.panel + a.panel_off[s];
   T* U = nr > 0 ? a.upool + a.upd_off[s] : nullptr;
   const int tid = threadIdx.x;
   for (int c = c0 < nc ? nc : c0; c < c1; ++c) {
     const int cb = c - nc;
-    for (int r = cb + tid; r < nr; r += blockDim.x) U[r + static_cast<int64_t>(cb) * nr] = T(0);
+    for (int r = max(c, t0) + tid; r < t1; r += blockDim.x) U[(r - nc) + static_cast<int64_t>(cb) * nr] = T(0);
   }
   __syncthreads();
   for (int t = a.child_ptr[s]; t < a.child_ptr[s + 1]; ++t) {
@@ -119,20 +123,20 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
     if (nrc == 0) continue;
     const T* Uc = a.upool + a.upd_off[ch];
     const int32_t* rel = a.relidx + a.rows_off[ch];
-    // child columns b with rel[b] in [c0, c1): a contiguous range
-    int lo = 0, hi = nrc;
-    while (lo < hi) { const int mid = (lo + hi) >> 1; if (rel[mid] < c0) lo = mid + 1; else hi = mid; }
-    const int b0 = lo;
-    hi = nrc;
-    while (lo < hi) { const int mid = (lo + hi) >> 1; if (rel[mid] < c1) lo = mid + 1; else hi = mid; }
-    const int b1 = lo;
+    auto lower = [&](int v) {
+      int lo = 0, hi = nrc;
+      while (lo < hi) { const int mid = (lo + hi) >> 1; if (rel[mid] < v) lo = mid + 1; else hi = mid; }
+      return lo;
+    };
+    const int b0 = lower(c0), b1 = lower(c1);
+    const int a0 = lower(t0), a1 = lower(t1);
     for (int b = b0; b < b1; ++b) {
       const int rb = rel[b];
       const T* src = Uc + static_cast<int64_t>(b) * nrc;
-      for (int r = b + tid; r < nrc; r += blockDim.x) {
-        const int ra = rel[r];
-        if (rb < nc) P[ra + static_cast<int64_t>(rb) * m] += src[r];
-        else U[(ra - nc) + static_cast<int64_t>(rb - nc) * nr] += src[r];
+      for (int q = max(b, a0) + tid; q < a1; q += blockDim.x) {
+        const int ra = rel[q];
+        if (rb < nc) P[ra + static_cast<int64_t>(rb) * m] += src[q];
+        else U[(ra - nc) + static_cast<int64_t>(rb - nc) * nr] += src[q];
       }
     }
     __syncthreads();
@@ -140,10 +144,11 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
 }
 
 // ---------------------------------------------------------------- big fronts
-// Diagonal block of step k: LDL^T of an nbk x nbk block in LDS.
+// Diagonal block of step k: LDL^T of an nbk x nbk block by one wavefront,
+// lane r holding row r of the block in registers; column j's multipliers are
+// broadcast through LDS.
 template <class T>
-__global__ void __launch_bounds__(256) k_diag_block(FactorArgs<T> a, const Task4* __restrict__ fronts) {
-  __shared__ T B[kPanelNB][kPanelNB + 1];
+__global__ void __launch_bounds__(64) k_diag_block(FactorArgs<T> a, const Task4* __restrict__ fronts) {
   __shared__ T lb[kPanelNB];
   const Task4 tk = fronts[blockIdx.x];
   const int s = tk.a, k = tk.d;
@@ -153,40 +158,43 @@ __global__ void __launch_bounds__(256) k_diag_block(FactorArgs<T> a, const Task4
   const int nbk = min(kPanelNB, nc - k0);
   const int m = nc + a.nrow[s];
   T* P = a.panel + a.panel_off[s] + k0 + static_cast<int64_t>(k0) * m;
-  const int tid = threadIdx.x;
-  for (int e = tid; e < nbk * nbk; e += blockDim.x) {
-    const int r = e % nbk, c = e / nbk;
-    if (r >= c) B[r][c] = P[r + static_cast<int64_t>(c) * m];
-  }
-  __syncthreads();
-  for (int j = 0; j < nbk; ++j) {
-    const T d = B[j][j];
-    const T dinv = T(1) / d;
-    for (int r = j + 1 + tid; r < nbk; r += blockDim.x) lb[r] = B[r][j] * dinv;
-    __syncthreads();
-    const int w = nbk - j - 1;  // trailing size
-    for (int e = tid; e < w * w; e += blockDim.x) {
-      const int r = j + 1 + e % w, c = j + 1 + e / w;
-      if (r >= c) B[r][c] -= B[r][j] * lb[c];
+  const int r = threadIdx.x;
+  T b[kPanelNB];
+#pragma unroll
+  for (int c = 0; c < kPanelNB; ++c) b[c] = (c < nbk && r < nbk) ? P[r + static_cast<int64_t>(c) * m] : T(0);
+#pragma unroll
+  for (int j = 0; j < kPanelNB; ++j) {
+    if (j < nbk) {
+      const T d = __shfl(b[j], j);
+      const T dinv = T(1) / d;
+      const T y = b[j];
+      const T l = y * dinv;
+      lb[r] = l;
+      __syncthreads();
+      if (r > j) {
+#pragma unroll
+        for (int c = j + 1; c < kPanelNB; ++c)
+          if (c <= r) b[c] -= y * lb[c];
+        b[j] = l;
+      }
+      if (r == j) {
+        if (d == T(0)) flag_zero_pivot(a.err);
+        a.D[f + k0 + j] = d;
+        a.Dinv[f + k0 + j] = dinv;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    for (int r = j + 1 + tid; r < nbk; r += blockDim.x) B[r][j] = lb[r];
-    if (tid == 0) {
-      if (d == T(0)) flag_zero_pivot(a.err);
-      a.D[f + k0 + j] = d;
-      a.Dinv[f + k0 + j] = dinv;
-      B[j][j] = d;
-    }
-    __syncthreads();
   }
-  for (int e = tid; e < nbk * nbk; e += blockDim.x) {
-    const int r = e % nbk, c = e / nbk;
-    if (r >= c) P[r + static_cast<int64_t>(c) * m] = B[r][c];
+  if (r < nbk) {
+#pragma unroll
+    for (int c = 0; c < kPanelNB; ++c)
+      if (c <= r && c < nbk) P[r + static_cast<int64_t>(c) * m] = b[c];
   }
 }
 
 // Rows below the diagonal block: X L_kk^T = A_rows, L_rows = X Dinv.
-// One thread per row, the row kept in registers.
+// One thread per row, the row in registers, eliminated right-looking so the
+// updates of later columns are independent of each other.
 template <class T>
 __global__ void __launch_bounds__(256) k_panel_trsm(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   __shared__ T Lk[kPanelNB][kPanelNB + 1];
@@ -200,26 +208,22 @@ __global__ void __launch_bounds__(256) k_panel_trsm(FactorArgs<T> a, const Task4
   const int nbk = min(kPanelNB, nc - k0);
   T* P = a.panel + a.panel_off[s];
   const int tid = threadIdx.x;
-  for (int e = tid; e < nbk * nbk; e += blockDim.x) {
-    const int r = e % nbk, c = e / nbk;
-    if (r > c) Lk[r][c] = P[(k0 + r) + static_cast<int64_t>(k0 + c) * m];
-  }
-  for (int c = tid; c < nbk; c += blockDim.x) dv[c] = a.Dinv[f + k0 + c];
+  const int lane = tid & 63, w = tid >> 6;
+  for (int c = w; c < nbk; c += 4)
+    if (lane < nbk && lane > c) Lk[lane][c] = P[(k0 + lane) + static_cast<int64_t>(k0 + c) * m];
+  if (tid < nbk) dv[tid] = a.Dinv[f + k0 + tid];
   __syncthreads();
   const int r = r0 + tid;
   if (r >= r1) return;
   T x[kPanelNB];
 #pragma unroll
-  for (int c = 0; c < kPanelNB; ++c)
-    x[c] = c < nbk ? P[r + static_cast<int64_t>(k0 + c) * m] : T(0);
+  for (int c = 0; c < kPanelNB; ++c) x[c] = c < nbk ? P[r + static_cast<int64_t>(k0 + c) * m] : T(0);
 #pragma unroll
-  for (int c = 0; c < kPanelNB; ++c) {
-    if (c < nbk) {
-      T v = x[c];
+  for (int t = 0; t < kPanelNB; ++t) {
+    if (t < nbk) {
+      const T xt = x[t];
 #pragma unroll
-      for (int t = 0; t < kPanelNB; ++t)
-        if (t < c) v -= x[t] * Lk[c][t];
-      x[c] = v;
+      for (int c = t + 1; c < kPanelNB; ++c) x[c] -= xt * Lk[c][t];
     }
   }
 #pragma unroll
@@ -312,7 +316,7 @@ void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const do
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
     for (int q = st0; q < st1; ++q) {
       const int d0 = dp.h_diag_ptr[q], d1 = dp.h_diag_ptr[q + 1];
-      hipLaunchKernelGGL(k_diag_block<T>, dim3(d1 - d0), dim3(256), 0, st, a, dp.diag_task + d0);
+      hipLaunchKernelGGL(k_diag_block<T>, dim3(d1 - d0), dim3(64), 0, st, a, dp.diag_task + d0);
       const int p0 = dp.h_pan_ptr[q], p1 = dp.h_pan_ptr[q + 1];
       if (p1 > p0) hipLaunchKernelGGL(k_panel_trsm<T>, dim3(p1 - p0), dim3(256), 0, st, a, dp.pan_task + p0);
       const int u0 = dp.h_upd_ptr[q], u1 = dp.h_upd_ptr[q + 1];

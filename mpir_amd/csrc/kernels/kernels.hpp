@@ -71,17 +71,23 @@ struct SolveArgs {
 struct DeviceSolvePlan {
   int nlevels = 0;
   const int32_t* small_list = nullptr;  // small fronts, grouped by level
-  const int32_t* big_list = nullptr;    // big fronts, grouped by level
-  const Task4* diag_task = nullptr;     // per step: (s, task0, task1, step)
-  const Task4* gemv_task = nullptr;     // per step: (s, row0, row1, step)
-  void* scratch = nullptr;              // partial sums of the backward GEMV
-  std::vector<int32_t> h_small_ptr, h_big_ptr, h_level_step_ptr, h_diag_ptr, h_task_ptr;
+  const int32_t* fwd_task = nullptr;    // (s, row block) pairs, grouped by level
+  const int32_t* bwd_task = nullptr;    // (s, column block) pairs, grouped by level
+  const int32_t* flag_off = nullptr;    // per supernode: first flag of a big front
+  int* flags = nullptr;                 // one per column block of each big front
+  int* err = nullptr;                   // bit 1: a bounded wait expired
+  void* ybuf = nullptr;                 // n entries of the solve precision
+  const int32_t* rpart_task = nullptr;  // (s, column block, row chunk) triples
+  const int64_t* part_off = nullptr;    // per supernode
+  void* part = nullptr;                 // partial sums of the solve precision
+  std::vector<int32_t> h_small_ptr, h_fwd_ptr, h_bwd_ptr, h_rpart_ptr;
 };
 
 // x (length n, permuted order) is overwritten by (L D L^T)^{-1} x.
-// uvec: scratch of sum(nrow) entries of TX.
+// uvec: scratch of sum(nrow) entries of TX. epoch: a counter that increases
+// by one per call on the same plan (flag values of the two passes).
 template <class TF, class TX>
-void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st);
+void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, int epoch, hipStream_t st);
 
 // ---------------------------------------------------------------- vectors / SpMV
 struct CsrDev {

@@ -124,7 +124,8 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
         const int32_t c1 = std::min(m, c0 + kAsmCols);
         // blocks made only of panel columns need work only with children
         if (c1 <= nc && !has_children) continue;
-        P.asm_task.push_back({s, c0, c1, 0});
+        // rows at or below the block's first column (lower part only)
+        for (int32_t t0 = (c0 / kAsmRows) * kAsmRows; t0 < m; t0 += kAsmRows) P.asm_task.push_back({s, c0, c1, t0});
       }
     }
     P.small_ptr[l + 1] = static_cast<int32_t>(P.small_list.size());
@@ -156,6 +157,44 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
   }
   P.nsmall = static_cast<int64_t>(P.small_list.size());
   P.nbig = static_cast<int64_t>(P.big_list.size());
+
+  // sync-free solve tasks and per-front flag ranges
+  P.flag_off.assign(S.ns, -1);
+  P.part_off.assign(S.ns, -1);
+  P.rpart_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.sfwd_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.sbwd_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  for (int32_t l = 0; l < S.nlevels; ++l) {
+    for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
+      const int32_t s = P.big_list[q];
+      const int32_t nc = S.first[s + 1] - S.first[s];
+      const int32_t m = nc + S.nrow[s];
+      const int32_t ncb = (nc + kSolveRB - 1) / kSolveRB;
+      const int32_t nrb = (m + kSolveRB - 1) / kSolveRB;
+      P.flag_off[s] = P.nflags;
+      P.nflags += ncb;
+      for (int32_t i = 0; i < nrb; ++i) {
+        P.sfwd_task.push_back(s);
+        P.sfwd_task.push_back(i);
+      }
+      for (int32_t j = ncb - 1; j >= 0; --j) {
+        P.sbwd_task.push_back(s);
+        P.sbwd_task.push_back(j);
+      }
+      const int32_t nch = (S.nrow[s] + kRpartRows - 1) / kRpartRows;
+      P.part_off[s] = P.npart;
+      P.npart += static_cast<int64_t>(ncb) * nch * kSolveRB;
+      for (int32_t j = 0; j < ncb; ++j)
+        for (int32_t c = 0; c < nch; ++c) {
+          P.rpart_task.push_back(s);
+          P.rpart_task.push_back(j);
+          P.rpart_task.push_back(c);
+        }
+    }
+    P.rpart_ptr[l + 1] = static_cast<int32_t>(P.rpart_task.size() / 3);
+    P.sfwd_ptr[l + 1] = static_cast<int32_t>(P.sfwd_task.size() / 2);
+    P.sbwd_ptr[l + 1] = static_cast<int32_t>(P.sbwd_task.size() / 2);
+  }
   return P;
 }
 

@@ -14,10 +14,15 @@ namespace mpir {
 constexpr int kSmallFront = 64;
 // Column-block width of the blocked factorization of large fronts.
 constexpr int kPanelNB = 64;
+// Row / column block of the sync-free big-front triangular solves.
+constexpr int kSolveRB = 64;
+// R rows per backward partial-sum task.
+constexpr int kRpartRows = 4096;
 // Tile edge of the trailing-update tasks.
 constexpr int kTile = 64;
-// Column-block width of the extend-add assembly tasks.
+// Column-block width and row range of the extend-add assembly tasks.
 constexpr int kAsmCols = 64;
+constexpr int kAsmRows = 512;
 
 struct Task4 {
   int32_t a, b, c, d;
@@ -48,6 +53,20 @@ struct FactorPlan {
   std::vector<int32_t> diag_ptr;         // nsteps_total+1
   std::vector<Task4> diag_task;
   int32_t max_step_tasks = 0;            // max pan tasks in one step
+  // sync-free solves of big fronts: per level, forward tasks (s, row block)
+  // in ascending block order and backward tasks (s, column block) in
+  // descending order, so every wait targets an earlier workgroup
+  std::vector<int32_t> sfwd_ptr, sbwd_ptr;  // nlevels+1
+  std::vector<int32_t> sfwd_task, sbwd_task;  // pairs (s, block)
+  std::vector<int32_t> flag_off;              // per supernode, -1 if small
+  int32_t nflags = 0;
+  // backward contribution of the rows below the columns (R rows), computed
+  // before the chain: tasks (s, column block, row chunk) per level, partial
+  // sums at part_off[s] + (block * nchunk + chunk) * kSolveRB
+  std::vector<int32_t> rpart_ptr;             // nlevels+1
+  std::vector<int32_t> rpart_task;            // triples
+  std::vector<int64_t> part_off;              // per supernode
+  int64_t npart = 0;
   // for each entry of the ORIGINAL upper CSC: destination in the panel pool
   std::vector<int64_t> val_dst;
   // statistics

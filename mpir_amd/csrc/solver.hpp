@@ -67,6 +67,7 @@ class GmresLdl {
 
   // building blocks exposed for tests and the kernel-level C entry points
   void precond_apply_dev(double* x);  // x := (L D L^T)^{-1} x, permuted order
+  void check_flags() { check_solve_flags(); }
   void spmv_dev(const double* x, double* y);  // y = K_s x, permuted, scaled
   // factor (permuted order) copied out in QDLDL's CSC layout
   void export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* D, void* Dinv, int prec);
@@ -85,6 +86,7 @@ class GmresLdl {
   double infnrm(const double* v);
   void release_numeric();
   void upload_plan();
+  void check_solve_flags();  // throws if a bounded wait in a solve expired
   void alloc_numeric();
 
   int uf_, uw_, ur_;
@@ -94,6 +96,7 @@ class GmresLdl {
   bool scaling_ = true;
   bool analyzed_ = false, factored_ = false, precond_only_ = false;
   hipStream_t stream_ = nullptr;
+  int solve_epoch_ = 0;
   int32_t n_ = 0;
   int64_t nnz_ = 0;
   SolveStats stats_;

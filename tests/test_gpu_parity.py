@@ -160,3 +160,39 @@ def test_gmres_ir_multiple_rhs_and_reuse(gpu, oracle):
     s.Compute(P.Ap, P.Ai, Ax2)
     x2 = s.Solve(P.b)
     assert backward_error(oracle, P.Ap, P.Ai, Ax2, x2, P.b) < 1e-13
+
+
+@pytest.mark.parametrize("k", [16, 24, 32])
+def test_solve_big_fronts_matches_oracle(gpu, oracle, k):
+    # a factor with large dense fronts (sync-free block solves), exported in
+    # QDLDL's CSC layout and solved both by the GPU drop-in and the oracle
+    P = mpir_amd.kkt_config(1, k=k)
+    s = mpir_amd.GmresLDLIR(np.float64)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    assert s.stats()["max_front"] > 256
+    Lp, Li, Lx, D, Dinv = s.export_factor()
+    n = P.n
+    b = np.random.default_rng(k).uniform(-1, 1, n)
+    x = b.copy()
+    mpir_amd.QDLDL_solve(n, Lp, Li.astype(np.int64), Lx, Dinv, x)
+    xo = oracle.solve(Lp.astype(np.int32), Li, Lx, Dinv, b)
+    np.testing.assert_allclose(x, xo, rtol=1e-10, atol=1e-10 * np.abs(xo).max())
+    # repeated applications agree bit for bit (deterministic kernels)
+    x2 = b.copy()
+    mpir_amd.QDLDL_solve(n, Lp, Li.astype(np.int64), Lx, Dinv, x2)
+    np.testing.assert_array_equal(x, x2)
+
+
+@pytest.mark.parametrize("k", [24, 32])
+def test_exact_factor_gives_one_step_per_gmres_call(gpu, oracle, k):
+    # with an FP64 factor M^{-1} K = I to rounding, so every PrecondGmres
+    # call stops after about one Arnoldi step
+    P = mpir_amd.kkt_config(1, k=k)
+    s = mpir_amd.GmresLDLIR(np.float64)
+    s.SetTolerance(1e-10)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    x = s.Solve(P.b)
+    st = s.stats()
+    calls = st["ir_iterations"] + 1
+    assert st["gmres_iterations"] <= 2 * calls, st
+    assert backward_error(oracle, P.Ap, P.Ai, P.Ax, x, P.b) < 1e-14
