@@ -52,8 +52,8 @@ def backward_error(P, x) -> float:
     return float(np.abs(r).max() / (knorm * np.abs(x).max() + np.abs(P.b).max()))
 
 
-def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x):
-    s = mpir.GmresLDLIR(uf)
+def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x, ordering):
+    s = mpir.GmresLDLIR(uf, ordering=ordering)
     s.SetTolerance(tol)
     t0 = time.perf_counter()
     s.Analyze(P.Ap, P.Ai)
@@ -95,7 +95,7 @@ def cpu_baseline(mpir, full_flops: float, seconds_budget: float = 20.0) -> dict:
     from tests.oracle_lib import Oracle
     orc = Oracle()
     S = mpir.kkt_config(1, k=24)
-    s = mpir.GmresLDLIR(np.float32)
+    s = mpir.GmresLDLIR(np.float32, ordering=mpir.ORDER_ND)
     s.Analyze(S.Ap, S.Ai)
     perm = s.perm()
     sample_flops = s.stats()["factor_flops"]
@@ -129,6 +129,8 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--k", type=int, default=64, help="grid side (64 = the benchmark workload)")
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--ordering", choices=["amd", "nd"], default="nd",
+                    help="fill-reducing ordering (nd = nested dissection, amd = approximate minimum degree)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-factor comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = ap.parse_args(argv)
@@ -149,7 +151,8 @@ def main(argv=None) -> int:
     dev_b = torch.from_numpy(P.b).to(dev)
     dev_x = torch.zeros(P.n, dtype=torch.float64, device=dev)
 
-    s32, r32 = run_path(mpir, torch, P, np.float32, args.tol, args.steps, args.warmup, dev_ax, dev_b, dev_x)
+    order = {"amd": mpir.ORDER_AMD, "nd": mpir.ORDER_ND}[args.ordering]
+    s32, r32 = run_path(mpir, torch, P, np.float32, args.tol, args.steps, args.warmup, dev_ax, dev_b, dev_x, order)
     eta32 = backward_error(P, r32["x"])
     st = r32["stats"]
     log(f"f32: {st}")
@@ -171,7 +174,7 @@ def main(argv=None) -> int:
     r64 = None
     if not args.no_fp64:
         s64, r64 = run_path(mpir, torch, P, np.float64, args.tol, max(1, args.steps // 2), args.warmup, dev_ax, dev_b,
-                            dev_x)
+                            dev_x, order)
         r64["eta"] = backward_error(P, r64["x"])
         del s64
 
@@ -190,7 +193,7 @@ def main(argv=None) -> int:
         "dtype": "f32-factor/f64",
         "data": "synthetic",
         "config": {"workload": f"KKT 3D {args.k}^3 Laplacian + {P.n - P.n1} constraint rows (n={P.n}), "
-                               "AMD, FP32 LDL^T + FP64 GMRES-IR, tol 1e-10, 1 RHS",
+                               f"{args.ordering.upper()} ordering, FP32 LDL^T + FP64 GMRES-IR, tol 1e-10, 1 RHS",
                    "n": P.n, "nnz_upper": nnz_upper, "nnz_l": st["nnz_l"], "factor_flops": st["factor_flops"]},
         "bwd_err": eta32,
         "p50_ms": round(pct(r32["lat_ms"], 0.5), 3),

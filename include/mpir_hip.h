@@ -42,6 +42,7 @@ extern "C" {
 /* orderings */
 #define MPIR_ORDER_NATURAL 0 /* the matrix as given (QDLDL's behaviour) */
 #define MPIR_ORDER_AMD 1     /* approximate minimum degree + etree postorder */
+#define MPIR_ORDER_ND 2      /* nested dissection (AMD on small parts) + postorder */
 
 const char* mpir_last_error(void);
 const char* mpir_version(void);

@@ -25,7 +25,7 @@ from . import _lib
 from ._lib import GmresStats, last_error, load
 
 F64, F32, F16, BF16 = 0, 1, 2, 3
-ORDER_NATURAL, ORDER_AMD = 0, 1
+ORDER_NATURAL, ORDER_AMD, ORDER_ND = 0, 1, 2
 
 _PREC_OF = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 
@@ -305,5 +305,5 @@ def residual_upper(Ap, Ai, Ax, x, b) -> np.ndarray:
 __all__ = [
     "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
     "amd_order", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
-    "ORDER_AMD",
+    "ORDER_AMD", "ORDER_ND",
 ]

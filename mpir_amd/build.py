@@ -28,16 +28,17 @@ ARCH = "gfx950"
 
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = [
-    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-g", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-munsafe-fp-atomics",
 ]
 # Host code: no contraction so host sums round exactly as written.
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-sign-compare"]
+CXX_FLAGS = ["-O2", "-g", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-sign-compare"]
 
 HOST_SOURCES = [
     CSRC / "host" / "generators.cpp",
     CSRC / "host" / "ordering.cpp",
+    CSRC / "host" / "ordering_nd.cpp",
     CSRC / "host" / "symbolic.cpp",
     CSRC / "plan.cpp",
 ]

@@ -329,7 +329,8 @@ int mpir_gmres_ldl_set_tol(mpir_gmres_ldl_t h, double tol) {
 int mpir_gmres_ldl_set_ordering(mpir_gmres_ldl_t h, int ordering) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;
-    MPIR_CHECK(ordering == MPIR_ORDER_NATURAL || ordering == MPIR_ORDER_AMD, "unknown ordering");
+    MPIR_CHECK(ordering == MPIR_ORDER_NATURAL || ordering == MPIR_ORDER_AMD || ordering == MPIR_ORDER_ND,
+               "unknown ordering");
     h->s->set_ordering(ordering);
     return 0;
   });

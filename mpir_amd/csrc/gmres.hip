@@ -236,33 +236,25 @@ void GmresLdl::analyze(int32_t n, const int32_t* Ap, const int32_t* Ai) {
   nnz_ = Ap[n];
   Ap_.assign(Ap, Ap + n + 1);
   Ai_.assign(Ai, Ai + nnz_);
-  CscUpper A;
-  A.n = n;
-  A.p = Ap_;
-  A.i = Ai_;
-  A.x.assign(nnz_, 0.0);
-
-  // ordering: AMD followed by a postorder of the elimination tree
-  std::vector<int32_t> work(n), Lnz(n), parent(n);
-  if (ordering_ == kOrderAmd) {
-    std::vector<int32_t> p0 = amd_order(A);
-    std::vector<int32_t> vm;
-    CscUpper C0 = permute_upper(A, p0, &vm);
-    if (etree_counts(n, C0.p.data(), C0.i.data(), work.data(), Lnz.data(), parent.data()) < 0)
-      throw Error(kErrInvalid, "etree failed on the ordered matrix");
-    std::vector<int32_t> post = tree_postorder(n, parent.data());
-    perm_.resize(n);
-    for (int32_t k = 0; k < n; ++k) perm_[k] = p0[post[k]];
-  } else {
-    perm_.resize(n);
-    std::iota(perm_.begin(), perm_.end(), 0);
+  Analysis an;
+  try {
+    an = analyze_pattern(n, Ap, Ai, ordering_, relax_);
+  } catch (const std::invalid_argument& e) {
+    throw Error(kErrInvalid, e.what());
+  } catch (const std::overflow_error& e) {
+    throw Error(kErrInvalid, e.what());
   }
-  std::vector<int32_t> val_map;
-  CscUpper C = permute_upper(A, perm_, &val_map);
-  const int64_t lnz = etree_counts(n, C.p.data(), C.i.data(), work.data(), Lnz.data(), parent.data());
-  if (lnz == -2) throw Error(kErrInvalid, "nonzeros of L overflow int32");
-  if (lnz < 0) throw Error(kErrInvalid, "etree failed");
-  plan_ = std::make_unique<FactorPlan>(build_factor_plan(C, val_map, parent.data(), Lnz.data()));
+  perm_ = std::move(an.perm);
+  plan_ = std::make_unique<FactorPlan>(std::move(an.plan));
+  relaxed_ = an.relaxed;
+  const CscUpper& C = an.C;
+  const std::vector<int32_t>& val_map = an.val_map;
+  Cp_ = C.p;
+  Ci_ = C.i;
+  parent_ = an.parent;
+  Lnz_ = an.Lnz;
+  true_nnz_l_ = an.nnz_l;
+  true_flops_ = an.flops;
   hipStream_t st = stream_;
   upload_plan();
 
@@ -293,11 +285,11 @@ void GmresLdl::analyze(int32_t n, const int32_t* Ap, const int32_t* Ai) {
   stats_ = SolveStats();
   stats_.n = n;
   stats_.nnz = nnz_;
-  stats_.nnz_l = S.nnz_l;
+  stats_.nnz_l = true_nnz_l_;
   stats_.nsupernodes = S.ns;
   stats_.nlevels = S.nlevels;
   stats_.max_front = S.max_front;
-  stats_.factor_flops = S.flops;
+  stats_.factor_flops = true_flops_;
   stats_.device_bytes = static_cast<int64_t>(d_panel_.bytes() + d_upool_.bytes());
   analyzed_ = true;
   precond_only_ = false;
@@ -343,7 +335,11 @@ void GmresLdl::upload_plan() {
   dfp_.h_pan_ptr = P.pan_ptr;
   dfp_.h_upd_ptr = P.upd_ptr;
   dfp_.h_diag_ptr = P.diag_ptr;
-  keep_.resize(16);
+  keep_.resize(20);
+  upload_vec(keep_[15], P.asm_child_ptr, st);
+  upload_vec(keep_[16], P.asm_child, st);
+  upload_vec(keep_[17], P.sfwd_child_ptr, st);
+  upload_vec(keep_[18], P.sfwd_child, st);
   upload_vec(keep_[12], P.rpart_task, st);
   upload_vec(keep_[13], P.part_off, st);
   keep_[14].alloc(static_cast<size_t>(std::max<int64_t>(P.npart, 1)) * sizeof(double));
@@ -367,6 +363,10 @@ void GmresLdl::upload_plan() {
   dsp_.h_fwd_ptr = P.sfwd_ptr;
   dsp_.h_bwd_ptr = P.sbwd_ptr;
   dsp_.h_rpart_ptr = P.rpart_ptr;
+  dsp_.fwd_child_ptr = keep_[17].as<int32_t>();
+  dsp_.fwd_child = keep_[18].as<int32_t>();
+  dfp_.asm_child_ptr = keep_[15].as<int32_t>();
+  dfp_.asm_child = keep_[16].as<int32_t>();
   dsp_.rpart_task = keep_[12].as<int32_t>();
   dsp_.part_off = keep_[13].as<int64_t>();
   dsp_.part = keep_[14].get();
@@ -751,17 +751,36 @@ void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void*
     if (prec == kF64) static_cast<double*>(dst)[i] = v;
     else static_cast<float*>(dst)[i] = static_cast<float>(v);
   };
+  // true pattern of each column (relaxed blocks also store explicit zeros)
+  std::vector<int64_t> TLp;
+  std::vector<int32_t> TLi;
+  const bool use_true = relaxed_ && !Cp_.empty();
+  if (use_true) l_pattern(n_, Cp_.data(), Ci_.data(), parent_.data(), Lnz_.data(), TLp, TLi);
   int64_t q = 0;
   Lp[0] = 0;
   for (int32_t s = 0; s < S.ns; ++s) {
     const int32_t f = S.first[s], l = S.first[s + 1];
     const int64_t m = (l - f) + S.nrow[s];
+    const int32_t* R = S.rows.data() + S.rows_off[s];
     for (int32_t j = f; j < l; ++j) {
       const int64_t cj = j - f;
-      for (int64_t r = cj + 1; r < m; ++r) {
-        if (Li) Li[q] = r < (l - f) ? static_cast<int32_t>(f + r) : S.rows[S.rows_off[s] + (r - (l - f))];
-        if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * m));
-        ++q;
+      auto row_of = [&](int64_t r) { return r < (l - f) ? static_cast<int32_t>(f + r) : R[r - (l - f)]; };
+      if (use_true) {
+        int64_t r = cj + 1;
+        for (int64_t t = TLp[j]; t < TLp[j + 1]; ++t) {
+          const int32_t row = TLi[t];
+          while (r < m && row_of(r) < row) ++r;
+          if (r >= m || row_of(r) != row) throw Error(kErrInternal, "factor pattern outside its supernode");
+          if (Li) Li[q] = row;
+          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * m));
+          ++q;
+        }
+      } else {
+        for (int64_t r = cj + 1; r < m; ++r) {
+          if (Li) Li[q] = row_of(r);
+          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * m));
+          ++q;
+        }
       }
       Lp[j + 1] = q;
       if (Dv) put(Dv, j, get(D, j));

@@ -65,6 +65,10 @@ void spmv_full(const CsrFull& A, const double* x, double* y);
 // CSC, diagonal ignored). perm[k] = original index of the k-th pivot.
 std::vector<int32_t> amd_order(const CscUpper& A, int64_t* stats_nnz_est = nullptr);
 
+// Nested dissection (multilevel bisection, vertex separators, AMD on
+// subgraphs of at most leaf_size vertices). perm[k] = original index.
+std::vector<int32_t> nd_order(const CscUpper& A, int32_t leaf_size = 200);
+
 // C = triu(P A P^T) with perm[k] = old index of new index k. val_map[q] = the
 // index into A's arrays that entry q of C takes its value from.
 CscUpper permute_upper(const CscUpper& A, const std::vector<int32_t>& perm,
@@ -116,5 +120,17 @@ Supernodes build_supernodes(const CscUpper& C, const int32_t* parent, const int3
 // std::logic_error if the pattern is not closed under its elimination tree.
 Supernodes supernodes_from_l(int32_t n, const int64_t* Lp, const int32_t* Li);
 void finish_supernodes(Supernodes& S);
+
+// Relaxed supernode partition of a postordered pattern: order[k] = column of
+// C placed at position k (a topological reordering of the elimination tree),
+// first = block starts in the new order.
+struct RelaxedPartition {
+  std::vector<int32_t> order;
+  std::vector<int32_t> first;
+  std::vector<std::vector<int32_t>> pending;  // work space of the traversal
+};
+RelaxedPartition relax_supernodes(const CscUpper& C, const int32_t* parent, const int32_t* Lnz);
+// Supernodes for a given partition; R_s is the union structure.
+Supernodes build_supernodes_forced(const CscUpper& C, const int32_t* parent, const std::vector<int32_t>& first);
 
 }  // namespace mpir

@@ -286,3 +286,216 @@ Supernodes supernodes_from_l(int32_t n, const int64_t* Lp, const int32_t* Li) {
 }
 
 }  // namespace mpir
+
+namespace mpir {
+
+// Relaxed supernodes. Starting from the fundamental supernodes of the
+// (postordered) factor pattern, a child is merged into its parent when the
+// merged block stays small or gains few explicit zeros (thresholds after
+// Ashcraft & Grimes, "The influence of relaxed supernode partitions on the
+// multifrontal method", ACM TOMS 15(4), 1989). Merging moves the child's
+// columns to just before the parent's, which is a topological reordering of
+// the elimination tree: the fill (and every L value) is unchanged.
+// Returns new -> old column order (of C's columns) and the block starts.
+RelaxedPartition relax_supernodes(const CscUpper& C, const int32_t* parent, const int32_t* Lnz) {
+  const int32_t n = C.n;
+  // fundamental partition
+  std::vector<int32_t> first{0};
+  for (int32_t j = 0; j + 1 < n; ++j)
+    if (!(parent[j] == j + 1 && Lnz[j] == Lnz[j + 1] + 1)) first.push_back(j + 1);
+  first.push_back(n);
+  const int32_t ns = static_cast<int32_t>(first.size()) - 1;
+  std::vector<int32_t> col2sn(n);
+  for (int32_t s = 0; s < ns; ++s)
+    for (int32_t j = first[s]; j < first[s + 1]; ++j) col2sn[j] = s;
+  std::vector<int64_t> nc(ns), nr(ns);
+  std::vector<double> zeros(ns, 0.0);
+  std::vector<int32_t> sp(ns, -1);
+  for (int32_t s = 0; s < ns; ++s) {
+    nc[s] = first[s + 1] - first[s];
+    nr[s] = Lnz[first[s + 1] - 1];
+    const int32_t pc = parent[first[s + 1] - 1];
+    sp[s] = pc >= 0 ? col2sn[pc] : -1;
+  }
+  // children lists (ascending)
+  std::vector<int32_t> cptr(static_cast<size_t>(ns) + 1, 0), cl(ns);
+  for (int32_t s = 0; s < ns; ++s)
+    if (sp[s] >= 0) cptr[sp[s] + 1]++;
+  for (int32_t s = 0; s < ns; ++s) cptr[s + 1] += cptr[s];
+  {
+    std::vector<int32_t> next(cptr.begin(), cptr.end() - 1);
+    for (int32_t s = 0; s < ns; ++s)
+      if (sp[s] >= 0) cl[next[sp[s]]++] = s;
+  }
+  // bottom-up merging; merged[c] = true when c joined its parent's group
+  std::vector<uint8_t> merged(ns, 0);
+  auto entries = [](double c, double r) { return c * (c - 1) / 2 + c * r; };
+  for (int32_t p = 0; p < ns; ++p) {
+    // children by descending column count: big children first
+    std::vector<int32_t> kids(cl.begin() + cptr[p], cl.begin() + cptr[p + 1]);
+    std::sort(kids.begin(), kids.end(), [&](int32_t a, int32_t b) { return nc[a] > nc[b] || (nc[a] == nc[b] && a < b); });
+    for (int32_t c : kids) {
+      const double ncm = static_cast<double>(nc[p] + nc[c]);
+      const double z = zeros[p] + zeros[c] + static_cast<double>(nc[c]) * static_cast<double>(nc[p] + nr[p] - nr[c]);
+      const double frac = z / std::max(1.0, entries(ncm, static_cast<double>(nr[p])));
+      const bool take = ncm <= 4 || (ncm <= 16 && frac < 0.8) || (ncm <= 48 && frac < 0.1) || frac < 0.05;
+      if (!take) continue;
+      merged[c] = 1;
+      zeros[p] = z;
+      nc[p] += nc[c];
+    }
+  }
+  // new order: postorder of the merged tree; a group lists the columns of its
+  // merged members (recursively, each member after its own merged children)
+  // after all non-merged subtrees hanging below the group
+  RelaxedPartition out;
+  out.order.reserve(n);
+  out.first.push_back(0);
+  std::vector<int32_t> gmem;  // members of the current group
+  // iterative DFS over groups: a group is identified by its top supernode
+  struct Frame { int32_t top; int stage; std::vector<int32_t> subs; size_t next; };
+  std::vector<Frame> stack;
+  auto collect = [&](int32_t top, std::vector<int32_t>& members, std::vector<int32_t>& subs) {
+    // members in column order (merged children first, recursively), subs =
+    // non-merged children of any member (roots of other groups)
+    std::vector<std::pair<int32_t, int>> st{{top, 0}};
+    while (!st.empty()) {
+      const int32_t v = st.back().first;
+      if (st.back().second == 0) {
+        st.back().second = 1;
+        for (int32_t t = cptr[v]; t < cptr[v + 1]; ++t) {
+          const int32_t c = cl[t];
+          if (merged[c]) st.push_back({c, 0});
+          else subs.push_back(c);
+        }
+      } else {
+        members.push_back(v);
+        st.pop_back();
+      }
+    }
+  };
+  for (int32_t root = 0; root < ns; ++root) {
+    if (sp[root] >= 0) continue;
+    stack.push_back({root, 0, {}, 0});
+    while (!stack.empty()) {
+      Frame& fr = stack.back();
+      if (fr.stage == 0) {
+        std::vector<int32_t> members;
+        collect(fr.top, members, fr.subs);
+        std::sort(fr.subs.begin(), fr.subs.end());
+        fr.stage = 1;
+        gmem = std::move(members);
+        stack.back().next = 0;
+        // remember members in the frame via a parallel vector
+        out.pending.push_back(std::move(gmem));
+        continue;
+      }
+      if (fr.next < fr.subs.size()) {
+        const int32_t child = fr.subs[fr.next++];
+        stack.push_back({child, 0, {}, 0});
+        continue;
+      }
+      // all sub-groups emitted: emit this group's columns
+      std::vector<int32_t> members = std::move(out.pending.back());
+      out.pending.pop_back();
+      for (int32_t v : members)
+        for (int32_t j = first[v]; j < first[v + 1]; ++j) out.order.push_back(j);
+      out.first.push_back(static_cast<int32_t>(out.order.size()));
+      stack.pop_back();
+    }
+  }
+  if (static_cast<int32_t>(out.order.size()) != n) throw std::logic_error("relaxed supernodes lost columns");
+  return out;
+}
+
+Supernodes build_supernodes_forced(const CscUpper& C, const int32_t* parent, const std::vector<int32_t>& first_in) {
+  Supernodes S;
+  const int32_t n = C.n;
+  S.n = n;
+  S.first = first_in;
+  S.ns = static_cast<int32_t>(S.first.size()) - 1;
+  const int32_t ns = S.ns;
+  S.col2sn.assign(n, -1);
+  S.sparent.assign(ns, -1);
+  for (int32_t s = 0; s < ns; ++s)
+    for (int32_t j = S.first[s]; j < S.first[s + 1]; ++j) S.col2sn[j] = s;
+  for (int32_t s = 0; s < ns; ++s) {
+    const int32_t pc = parent[S.first[s + 1] - 1];
+    S.sparent[s] = pc >= 0 ? S.col2sn[pc] : -1;
+    if (S.sparent[s] >= 0 && S.sparent[s] <= s) throw std::logic_error("relaxed supernodal tree not topological");
+  }
+  std::vector<int32_t> tp(static_cast<size_t>(n) + 1, 0), ti(C.nnz());
+  for (int32_t j = 0; j < n; ++j)
+    for (int32_t q = C.p[j]; q < C.p[j + 1]; ++q)
+      if (C.i[q] != j) tp[C.i[q] + 1]++;
+  for (int32_t i = 0; i < n; ++i) tp[i + 1] += tp[i];
+  {
+    std::vector<int32_t> next(tp.begin(), tp.end() - 1);
+    for (int32_t j = 0; j < n; ++j)
+      for (int32_t q = C.p[j]; q < C.p[j + 1]; ++q)
+        if (C.i[q] != j) ti[next[C.i[q]]++] = j;
+  }
+  std::vector<int32_t> cptr(static_cast<size_t>(ns) + 1, 0), clist(ns);
+  for (int32_t s = 0; s < ns; ++s)
+    if (S.sparent[s] >= 0) cptr[S.sparent[s] + 1]++;
+  for (int32_t s = 0; s < ns; ++s) cptr[s + 1] += cptr[s];
+  {
+    std::vector<int32_t> next(cptr.begin(), cptr.end() - 1);
+    for (int32_t s = 0; s < ns; ++s)
+      if (S.sparent[s] >= 0) clist[next[S.sparent[s]]++] = s;
+  }
+  // R_s by union (no count check: relaxed blocks hold explicit zeros)
+  S.nrow.assign(ns, 0);
+  S.rows_off.assign(static_cast<size_t>(ns) + 1, 0);
+  std::vector<int32_t> mark(n, -1), buf;
+  std::vector<std::vector<int32_t>> R(ns);
+  for (int32_t s = 0; s < ns; ++s) {
+    const int32_t f = S.first[s], l = S.first[s + 1];
+    buf.clear();
+    auto add = [&](int32_t r) {
+      if (r >= l && mark[r] != s) {
+        mark[r] = s;
+        buf.push_back(r);
+      }
+    };
+    for (int32_t j = f; j < l; ++j)
+      for (int32_t q = tp[j]; q < tp[j + 1]; ++q) add(ti[q]);
+    for (int32_t t = cptr[s]; t < cptr[s + 1]; ++t) {
+      for (int32_t r : R[clist[t]]) add(r);
+    }
+    std::sort(buf.begin(), buf.end());
+    R[s] = buf;
+    S.nrow[s] = static_cast<int32_t>(buf.size());
+    S.rows_off[s + 1] = S.rows_off[s] + S.nrow[s];
+    for (int32_t t = cptr[s]; t < cptr[s + 1]; ++t) std::vector<int32_t>().swap(R[clist[t]]);
+  }
+  // rows of the remaining (root) lists were kept; rebuild the flat array
+  S.rows.resize(S.rows_off[ns]);
+  {
+    // recompute R in a second pass to fill the flat array (children lists
+    // were released as soon as their parent absorbed them)
+    std::fill(mark.begin(), mark.end(), -1);
+    for (int32_t s = 0; s < ns; ++s) {
+      const int32_t f = S.first[s], l = S.first[s + 1];
+      int32_t* Rs = S.rows.data() + S.rows_off[s];
+      int64_t cnt = 0;
+      auto add = [&](int32_t r) {
+        if (r >= l && mark[r] != s) {
+          mark[r] = s;
+          Rs[cnt++] = r;
+        }
+      };
+      for (int32_t j = f; j < l; ++j)
+        for (int32_t q = tp[j]; q < tp[j + 1]; ++q) add(ti[q]);
+      for (int32_t t = cptr[s]; t < cptr[s + 1]; ++t) {
+        const int32_t c = clist[t];
+        for (int64_t q = S.rows_off[c]; q < S.rows_off[c + 1]; ++q) add(S.rows[q]);
+      }
+      std::sort(Rs, Rs + cnt);
+    }
+  }
+  finish_supernodes(S);
+  return S;
+}
+
+}  // namespace mpir

@@ -101,7 +101,8 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
 // columns (rows) landing in a range of the parent are contiguous because the
 // relative indices increase.
 template <class T>
-__global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* __restrict__ tasks) {
+__global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* __restrict__ tasks,
+                                                  const int32_t* __restrict__ cptr, const int32_t* __restrict__ cl) {
   const Task4 tk = tasks[blockIdx.x];
   const int s = tk.a, c0 = tk.b, c1 = tk.c, t0 = tk.d;
   const int f = a.first[s];
@@ -117,19 +118,11 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
     for (int r = max(c, t0) + tid; r < t1; r += blockDim.x) U[(r - nc) + static_cast<int64_t>(cb) * nr] = T(0);
   }
   __syncthreads();
-  for (int t = a.child_ptr[s]; t < a.child_ptr[s + 1]; ++t) {
-    const int ch = a.child_list[t];
+  for (int e = cptr[blockIdx.x]; e < cptr[blockIdx.x + 1]; ++e) {
+    const int ch = cl[5 * e], b0 = cl[5 * e + 1], b1 = cl[5 * e + 2], a0 = cl[5 * e + 3], a1 = cl[5 * e + 4];
     const int nrc = a.nrow[ch];
-    if (nrc == 0) continue;
     const T* Uc = a.upool + a.upd_off[ch];
     const int32_t* rel = a.relidx + a.rows_off[ch];
-    auto lower = [&](int v) {
-      int lo = 0, hi = nrc;
-      while (lo < hi) { const int mid = (lo + hi) >> 1; if (rel[mid] < v) lo = mid + 1; else hi = mid; }
-      return lo;
-    };
-    const int b0 = lower(c0), b1 = lower(c1);
-    const int a0 = lower(t0), a1 = lower(t1);
     for (int b = b0; b < b1; ++b) {
       const int rb = rel[b];
       const T* src = Uc + static_cast<int64_t>(b) * nrc;
@@ -312,7 +305,9 @@ void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const do
     if (s1 > s0)
       hipLaunchKernelGGL(k_small_fronts<T>, dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0);
     const int a0 = dp.h_asm_ptr[l], a1 = dp.h_asm_ptr[l + 1];
-    if (a1 > a0) hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(256), 0, st, a, dp.asm_task + a0);
+    if (a1 > a0)
+      hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(256), 0, st, a, dp.asm_task + a0, dp.asm_child_ptr + a0,
+                         dp.asm_child);
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
     for (int q = st0; q < st1; ++q) {
       const int d0 = dp.h_diag_ptr[q], d1 = dp.h_diag_ptr[q + 1];

@@ -41,6 +41,8 @@ struct DeviceFactorPlan {
   const Task4* pan_task = nullptr;
   const Task4* upd_task = nullptr;
   const Task4* diag_task = nullptr;
+  const int32_t* asm_child_ptr = nullptr;
+  const int32_t* asm_child = nullptr;
   const int64_t* val_dst = nullptr;
   std::vector<int32_t> h_small_ptr, h_big_ptr, h_asm_ptr, h_level_step_ptr, h_pan_ptr, h_upd_ptr, h_diag_ptr;
 };
@@ -78,6 +80,8 @@ struct DeviceSolvePlan {
   int* err = nullptr;                   // bit 1: a bounded wait expired
   void* ybuf = nullptr;                 // n entries of the solve precision
   const int32_t* rpart_task = nullptr;  // (s, column block, row chunk) triples
+  const int32_t* fwd_child_ptr = nullptr;  // per forward task + 1
+  const int32_t* fwd_child = nullptr;      // (child, q0, q1) triples
   const int64_t* part_off = nullptr;    // per supernode
   void* part = nullptr;                 // partial sums of the solve precision
   std::vector<int32_t> h_small_ptr, h_fwd_ptr, h_bwd_ptr, h_rpart_ptr;

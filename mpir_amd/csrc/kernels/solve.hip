@@ -133,6 +133,7 @@ __device__ __forceinline__ void publish_flag(int* flag, int epoch) {
 // tile of the last dependency) is loaded before the waits.
 template <class TF, class TX>
 __global__ void __launch_bounds__(256) k_fwd_front(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+                                                   const int32_t* __restrict__ cptr, const int32_t* __restrict__ cl,
                                                    const int32_t* __restrict__ flag_off, int* flags, int epoch,
                                                    TX* x, TX* __restrict__ uvec, TX* ybuf, int* err) {
   __shared__ TX vsh[kRB];
@@ -158,16 +159,9 @@ __global__ void __launch_bounds__(256) k_fwd_front(SolveArgs<TF> a, const int32_
     vsh[tid] = (tid < rows && r < nc) ? x[f + r] : TX(0);
   }
   __syncthreads();
-  for (int t = a.child_ptr[s]; t < a.child_ptr[s + 1]; ++t) {
-    const int c = a.child_list[t];
-    const int nrc = a.nrow[c];
+  for (int e = cptr[blockIdx.x]; e < cptr[blockIdx.x + 1]; ++e) {
+    const int c = cl[3 * e], q0 = cl[3 * e + 1], q1 = cl[3 * e + 2];
     const int32_t* rel = a.relidx + a.rows_off[c];
-    int lo = 0, hi = nrc;
-    while (lo < hi) { const int mid = (lo + hi) >> 1; if (rel[mid] < r0) lo = mid + 1; else hi = mid; }
-    const int q0 = lo;
-    hi = nrc;
-    while (lo < hi) { const int mid = (lo + hi) >> 1; if (rel[mid] < r0 + rows) lo = mid + 1; else hi = mid; }
-    const int q1 = lo;
     const TX* uc = uvec + a.rows_off[c];
     for (int q = q0 + tid; q < q1; q += blockDim.x) vsh[rel[q] - r0] += uc[q];
     __syncthreads();
@@ -215,12 +209,13 @@ __global__ void __launch_bounds__(256) k_fwd_front(SolveArgs<TF> a, const int32_
 
 // Backward, rows below the columns of a front (R rows, all final): partial
 // sums part[(j * nch + c) * 64 + t] = sum over the chunk's rows r of
-// L[nc + r, 64 j + t] x[R_r]. Independent tasks, no waits.
+// L[nc + r, 64 j + t] x[R_r]. Columns are contiguous in the panel, so each
+// wave walks whole columns (coalesced) against x_R staged in LDS.
 template <class TF, class TX>
 __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
                                                    const int64_t* __restrict__ part_off, const TX* __restrict__ x,
                                                    TX* __restrict__ part) {
-  __shared__ TX red[4][kRB];
+  __shared__ TX xs[kRpartRows];
   const int s = tasks[3 * blockIdx.x], j = tasks[3 * blockIdx.x + 1], ch = tasks[3 * blockIdx.x + 2];
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
@@ -232,29 +227,29 @@ __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const TF* P = a.panel + a.panel_off[s];
   const int32_t* R = a.rows + a.rows_off[s];
-  TX acc[kRB];
-#pragma unroll
-  for (int t = 0; t < kRB; ++t) acc[t] = TX(0);
-  const int q0 = ch * kRpartRows, q1 = min(nr, q0 + kRpartRows);
-  for (int q = q0 + w * kRB + lane; q - lane < q1; q += 4 * kRB) {
-    const bool ok = q < q1;
-    const int r = nc + (ok ? q : q0);
-    const TX xr = ok ? x[R[r - nc]] : TX(0);
-    const TF* row = P + r + static_cast<int64_t>(c0) * m;
-#pragma unroll
-    for (int t = 0; t < kRB; ++t)
-      if (t < bw) acc[t] += static_cast<TX>(row[static_cast<int64_t>(t) * m]) * xr;
-  }
-#pragma unroll
-  for (int t = 0; t < kRB; ++t) {
-    TX v = acc[t];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (lane == t) red[w][t] = v;
-  }
+  const int q0 = ch * kRpartRows, len = min(kRpartRows, nr - q0);
+  for (int q = tid; q < len; q += blockDim.x) xs[q] = x[R[q0 + q]];
   __syncthreads();
-  if (tid < kRB)
-    part[part_off[s] + (static_cast<int64_t>(j) * nch + ch) * kRB + tid] =
-        (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+  TX* out = part + part_off[s] + static_cast<int64_t>(j) * nch * kRB + static_cast<int64_t>(ch) * kRB;
+  // wave w: columns w, w + 4, ... of the block, four at a time
+  for (int t0 = w * 4; t0 < bw; t0 += 16) {
+    TX acc[4] = {TX(0), TX(0), TX(0), TX(0)};
+    const TF* col[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      col[u] = P + (nc + q0) + static_cast<int64_t>(c0 + min(t0 + u, bw - 1)) * m;
+    for (int q = lane; q < len; q += 64) {
+      const TX xv = xs[q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] += static_cast<TX>(col[u][q]) * xv;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      TX v = acc[u];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0 && t0 + u < bw) out[t0 + u] = v;
+    }
+  }
 }
 
 // Backward, column block j of front s: z = x(block) - sum over the rows below
@@ -360,8 +355,9 @@ void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uve
       hipLaunchKernelGGL((k_fwd_small<TF, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x, uvec);
     const int t0 = dp.h_fwd_ptr[l], t1 = dp.h_fwd_ptr[l + 1];
     if (t1 > t0)
-      hipLaunchKernelGGL((k_fwd_front<TF, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.fwd_task + 2 * t0, dp.flag_off,
-                         dp.flags, 2 * epoch + 1, x, uvec, ybuf, dp.err);
+      hipLaunchKernelGGL((k_fwd_front<TF, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.fwd_task + 2 * t0,
+                         dp.fwd_child_ptr + t0, dp.fwd_child, dp.flag_off, dp.flags, 2 * epoch + 1, x, uvec, ybuf,
+                         dp.err);
   }
   // backward, root to leaves
   TX* part = reinterpret_cast<TX*>(dp.part);

@@ -2,6 +2,7 @@
 
 #include <algorithm>
 #include <map>
+#include <numeric>
 #include <set>
 #include <stdexcept>
 #include <utility>
@@ -158,8 +159,30 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
   P.nsmall = static_cast<int64_t>(P.small_list.size());
   P.nbig = static_cast<int64_t>(P.big_list.size());
 
+  auto lower = [&](int32_t c, int32_t v) {
+    const int32_t* rel = S.relidx.data() + S.rows_off[c];
+    return static_cast<int32_t>(std::lower_bound(rel, rel + S.nrow[c], v) - rel);
+  };
+  // child ranges of every assembly task
+  P.asm_child_ptr.assign(P.asm_task.size() + 1, 0);
+  for (size_t t = 0; t < P.asm_task.size(); ++t) {
+    const Task4 tk = P.asm_task[t];
+    const int32_t s = tk.a, c0 = tk.b, c1 = tk.c, t0 = tk.d;
+    const int32_t m = (S.first[s + 1] - S.first[s]) + S.nrow[s];
+    const int32_t t1 = std::min(m, t0 + kAsmRows);
+    for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+      const int32_t c = P.child_list[q];
+      if (S.nrow[c] == 0) continue;
+      const int32_t b0 = lower(c, c0), b1 = lower(c, c1), a0 = lower(c, t0), a1 = lower(c, t1);
+      if (b0 >= b1 || a1 <= std::max(b0, a0)) continue;
+      P.asm_child.insert(P.asm_child.end(), {c, b0, b1, a0, a1});
+    }
+    P.asm_child_ptr[t + 1] = static_cast<int32_t>(P.asm_child.size() / 5);
+  }
+
   // sync-free solve tasks and per-front flag ranges
   P.flag_off.assign(S.ns, -1);
+  P.sfwd_child_ptr.assign(1, 0);
   P.part_off.assign(S.ns, -1);
   P.rpart_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   P.sfwd_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
@@ -176,6 +199,13 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
       for (int32_t i = 0; i < nrb; ++i) {
         P.sfwd_task.push_back(s);
         P.sfwd_task.push_back(i);
+        const int32_t r0 = i * kSolveRB, r1 = std::min(m, r0 + kSolveRB);
+        for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+          const int32_t c = P.child_list[q];
+          const int32_t q0 = lower(c, r0), q1 = lower(c, r1);
+          if (q1 > q0) P.sfwd_child.insert(P.sfwd_child.end(), {c, q0, q1});
+        }
+        P.sfwd_child_ptr.push_back(static_cast<int32_t>(P.sfwd_child.size() / 3));
       }
       for (int32_t j = ncb - 1; j >= 0; --j) {
         P.sbwd_task.push_back(s);
@@ -198,9 +228,73 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
   return P;
 }
 
+namespace {
+void fill_value_map(FactorPlan& P, const CscUpper& C, const std::vector<int32_t>& val_map);
+}
+
 FactorPlan build_factor_plan(const CscUpper& C, const std::vector<int32_t>& val_map,
                              const int32_t* parent, const int32_t* Lnz) {
   FactorPlan P = build_plan_from_supernodes(build_supernodes(C, parent, Lnz));
+  fill_value_map(P, C, val_map);
+  return P;
+}
+
+FactorPlan build_factor_plan_forced(const CscUpper& C, const std::vector<int32_t>& val_map,
+                                    const int32_t* parent, const std::vector<int32_t>& first) {
+  FactorPlan P = build_plan_from_supernodes(build_supernodes_forced(C, parent, first));
+  fill_value_map(P, C, val_map);
+  return P;
+}
+
+Analysis analyze_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax) {
+  Analysis R;
+  CscUpper A;
+  A.n = n;
+  A.p.assign(Ap, Ap + n + 1);
+  A.i.assign(Ai, Ai + Ap[n]);
+  A.x.assign(A.i.size(), 0.0);
+  std::vector<int32_t> work(n), Lnz(n), parent(n);
+  if (ordering == 1 || ordering == 2) {
+    std::vector<int32_t> p0 = ordering == 1 ? amd_order(A) : nd_order(A);
+    std::vector<int32_t> vm;
+    CscUpper C0 = permute_upper(A, p0, &vm);
+    if (etree_counts(n, C0.p.data(), C0.i.data(), work.data(), Lnz.data(), parent.data()) < 0)
+      throw std::invalid_argument("etree failed on the ordered matrix");
+    std::vector<int32_t> post = tree_postorder(n, parent.data());
+    R.perm.resize(n);
+    for (int32_t k = 0; k < n; ++k) R.perm[k] = p0[post[k]];
+  } else {
+    R.perm.resize(n);
+    std::iota(R.perm.begin(), R.perm.end(), 0);
+  }
+  R.C = permute_upper(A, R.perm, &R.val_map);
+  int64_t lnz = etree_counts(n, R.C.p.data(), R.C.i.data(), work.data(), Lnz.data(), parent.data());
+  if (lnz == -2) throw std::overflow_error("nonzeros of L overflow int32");
+  if (lnz < 0) throw std::invalid_argument("etree failed");
+  if (ordering != 0 && relax) {
+    RelaxedPartition rp = relax_supernodes(R.C, parent.data(), Lnz.data());
+    std::vector<int32_t> perm2(n);
+    for (int32_t k = 0; k < n; ++k) perm2[k] = R.perm[rp.order[k]];
+    R.perm.swap(perm2);
+    R.C = permute_upper(A, R.perm, &R.val_map);
+    lnz = etree_counts(n, R.C.p.data(), R.C.i.data(), work.data(), Lnz.data(), parent.data());
+    if (lnz < 0) throw std::logic_error("etree failed after relaxation");
+    R.plan = build_factor_plan_forced(R.C, R.val_map, parent.data(), rp.first);
+    R.relaxed = true;
+  } else {
+    R.plan = build_factor_plan(R.C, R.val_map, parent.data(), Lnz.data());
+  }
+  R.nnz_l = lnz;
+  R.flops = static_cast<double>(n);
+  for (int32_t j = 0; j < n; ++j) R.flops += static_cast<double>(Lnz[j]) * (static_cast<double>(Lnz[j]) + 2.0);
+  R.parent = std::move(parent);
+  R.Lnz = std::move(Lnz);
+  R.C.x.clear();
+  return R;
+}
+
+namespace {
+void fill_value_map(FactorPlan& P, const CscUpper& C, const std::vector<int32_t>& val_map) {
   const Supernodes& S = P.sn;
   const int32_t n = C.n;
   // value destinations: C entry (i, j), i <= j, is L-view entry (j, i) of
@@ -224,7 +318,7 @@ FactorPlan build_factor_plan(const CscUpper& C, const std::vector<int32_t>& val_
       P.val_dst[val_map[q]] = S.panel_off[s] + static_cast<int64_t>(i - f) * m + rp;
     }
   }
-  return P;
 }
+}  // namespace
 
 }  // namespace mpir

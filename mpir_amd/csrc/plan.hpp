@@ -17,7 +17,7 @@ constexpr int kPanelNB = 64;
 // Row / column block of the sync-free big-front triangular solves.
 constexpr int kSolveRB = 64;
 // R rows per backward partial-sum task.
-constexpr int kRpartRows = 4096;
+constexpr int kRpartRows = 2048;
 // Tile edge of the trailing-update tasks.
 constexpr int kTile = 64;
 // Column-block width and row range of the extend-add assembly tasks.
@@ -41,6 +41,10 @@ struct FactorPlan {
   // per level: assembly tasks (s, c0, c1, 0) over big fronts
   std::vector<int32_t> asm_ptr;
   std::vector<Task4> asm_task;
+  // per assembly task, the children that land in it: (child, b0, b1, a0, a1)
+  // = child columns [b0, b1) and child rows [a0, a1) mapping into the task
+  std::vector<int32_t> asm_child_ptr;  // per task + 1
+  std::vector<int32_t> asm_child;      // 5-tuples
   // per (level, step): panel tasks (s, row0, row1, step) and update tiles
   // (s, tile_row0, tile_col0, step); indexed through step_ptr
   std::vector<int32_t> level_step_ptr;  // nlevels+1 into the step arrays
@@ -58,6 +62,10 @@ struct FactorPlan {
   // descending order, so every wait targets an earlier workgroup
   std::vector<int32_t> sfwd_ptr, sbwd_ptr;  // nlevels+1
   std::vector<int32_t> sfwd_task, sbwd_task;  // pairs (s, block)
+  // per forward task, the children whose update vectors land in its rows:
+  // (child, q0, q1) = child rows [q0, q1)
+  std::vector<int32_t> sfwd_child_ptr;        // per forward task + 1
+  std::vector<int32_t> sfwd_child;            // triples
   std::vector<int32_t> flag_off;              // per supernode, -1 if small
   int32_t nflags = 0;
   // backward contribution of the rows below the columns (R rows), computed
@@ -81,5 +89,23 @@ FactorPlan build_plan_from_supernodes(Supernodes sn);
 // entry). parent/Lnz: etree of C.
 FactorPlan build_factor_plan(const CscUpper& C, const std::vector<int32_t>& val_map,
                              const int32_t* parent, const int32_t* Lnz);
+// Same with a given (relaxed) supernode partition.
+FactorPlan build_factor_plan_forced(const CscUpper& C, const std::vector<int32_t>& val_map,
+                                    const int32_t* parent, const std::vector<int32_t>& first);
+
+// The host-side symbolic phase of the solver: ordering (0 natural, 1 AMD,
+// 2 nested dissection; AMD/ND followed by an elimination-tree postorder),
+// optional relaxed supernodes, the permuted upper CSC and the plan.
+struct Analysis {
+  std::vector<int32_t> perm;     // new -> old
+  CscUpper C;                    // triu(P A P^T), pattern only
+  std::vector<int32_t> val_map;  // C entry -> A entry
+  std::vector<int32_t> parent, Lnz;
+  int64_t nnz_l = 0;
+  double flops = 0.0;            // sum Lnz_j (Lnz_j + 2) + n
+  bool relaxed = false;
+  FactorPlan plan;
+};
+Analysis analyze_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax);
 
 }  // namespace mpir

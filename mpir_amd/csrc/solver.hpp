@@ -15,7 +15,7 @@
 
 namespace mpir {
 
-enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1 };
+enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1, kOrderNd = 2 };
 
 struct SolveStats {
   int32_t n = 0;
@@ -49,6 +49,7 @@ class GmresLdl {
   // diagonal scaling of Compute (gmres_ir.hpp:347-363); off for the bare
   // QDLDL_factor drop-in
   void set_scaling(bool on) { scaling_ = on; }
+  void set_relax(bool on) { relax_ = on; }
 
   // Symbolic phase on the pattern of the upper CSC (reusable across values).
   void analyze(int32_t n, const int32_t* Ap, const int32_t* Ai);
@@ -94,6 +95,11 @@ class GmresLdl {
   double tol_ = 1e-10;
   int ordering_ = kOrderAmd;
   bool scaling_ = true;
+  bool relax_ = true;     // relaxed supernodes (AMD / ND orderings only)
+  bool relaxed_ = false;  // the current plan uses them
+  std::vector<int32_t> Cp_, Ci_, parent_, Lnz_;  // pattern of P K P^T, etree
+  int64_t true_nnz_l_ = 0;
+  double true_flops_ = 0.0;
   bool analyzed_ = false, factored_ = false, precond_only_ = false;
   hipStream_t stream_ = nullptr;
   int solve_epoch_ = 0;

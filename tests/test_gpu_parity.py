@@ -99,12 +99,13 @@ def test_qdldl_zero_pivot_dropin(gpu):
                                  np.array([1, -1], np.int32)) == -1
 
 
-@pytest.mark.parametrize("cfg,k,uf", [(0, None, np.float32), (0, None, np.float64), (1, 12, np.float32),
-                                      (1, 24, np.float32), (1, 24, np.float64)])
-def test_gmres_ir_matches_oracle_outcome(gpu, oracle, cfg, k, uf):
+@pytest.mark.parametrize("cfg,k,uf,order", [(0, None, np.float32, 1), (0, None, np.float64, 1), (1, 12, np.float32, 1),
+                                            (1, 24, np.float32, 1), (1, 24, np.float64, 1), (1, 24, np.float32, 2),
+                                            (0, None, np.float32, 2)])
+def test_gmres_ir_matches_oracle_outcome(gpu, oracle, cfg, k, uf, order):
     P = mpir_amd.kkt_config(cfg, k=k)
     tol = 1e-12
-    s = mpir_amd.GmresLDLIR(uf)
+    s = mpir_amd.GmresLDLIR(uf, ordering=order)
     s.SetTolerance(tol)
     s.Compute(P.Ap, P.Ai, P.Ax)
     st = s.stats()
