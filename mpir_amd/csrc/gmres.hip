@@ -335,6 +335,8 @@ void GmresLdl::upload_plan() {
   dfp_.h_pan_ptr = P.pan_ptr;
   dfp_.h_upd_ptr = P.upd_ptr;
   dfp_.h_diag_ptr = P.diag_ptr;
+  dfp_.h_pan_aug_ptr = P.pan_aug_ptr;
+  dfp_.h_upd_aug_ptr = P.upd_aug_ptr;
   keep_.resize(20);
   upload_vec(keep_[15], P.asm_child_ptr, st);
   upload_vec(keep_[16], P.asm_child, st);
@@ -345,33 +347,30 @@ void GmresLdl::upload_plan() {
   keep_[14].alloc(static_cast<size_t>(std::max<int64_t>(P.npart, 1)) * sizeof(double));
   upload_vec(keep_[7], P.sfwd_task, st);
   upload_vec(keep_[8], P.sbwd_task, st);
-  upload_vec(keep_[9], P.flag_off, st);
-  keep_[10].alloc(static_cast<size_t>(std::max(P.nflags, 1)) * sizeof(int));
-  MPIR_HIP(hipMemsetAsync(keep_[10].get(), 0, keep_[10].bytes(), st));
-  keep_[11].alloc(sizeof(int));
-  MPIR_HIP(hipMemsetAsync(keep_[11].get(), 0, sizeof(int), st));
-  solve_epoch_ = 0;
+  upload_vec(keep_[9], P.sxc_task, st);
+  upload_vec(keep_[10], P.su_task, st);
+  dfp_.asm_child_ptr = keep_[15].as<int32_t>();
+  dfp_.asm_child = keep_[16].as<int32_t>();
   dsp_ = DeviceSolvePlan();
   dsp_.nlevels = S.nlevels;
   dsp_.small_list = dfp_.small_list;
-  dsp_.fwd_task = keep_[7].as<int32_t>();
+  dsp_.gather_task = keep_[7].as<int32_t>();
+  dsp_.gather_child_ptr = keep_[17].as<int32_t>();
+  dsp_.gather_child = keep_[18].as<int32_t>();
+  dsp_.xc_task = keep_[9].as<int32_t>();
+  dsp_.u_task = keep_[10].as<int32_t>();
   dsp_.bwd_task = keep_[8].as<int32_t>();
-  dsp_.flag_off = keep_[9].as<int32_t>();
-  dsp_.flags = keep_[10].as<int>();
-  dsp_.err = keep_[11].as<int>();
-  dsp_.h_small_ptr = P.small_ptr;
-  dsp_.h_fwd_ptr = P.sfwd_ptr;
-  dsp_.h_bwd_ptr = P.sbwd_ptr;
-  dsp_.h_rpart_ptr = P.rpart_ptr;
-  dsp_.fwd_child_ptr = keep_[17].as<int32_t>();
-  dsp_.fwd_child = keep_[18].as<int32_t>();
-  dfp_.asm_child_ptr = keep_[15].as<int32_t>();
-  dfp_.asm_child = keep_[16].as<int32_t>();
   dsp_.rpart_task = keep_[12].as<int32_t>();
   dsp_.part_off = keep_[13].as<int64_t>();
   dsp_.part = keep_[14].get();
+  dsp_.h_small_ptr = P.small_ptr;
+  dsp_.h_gather_ptr = P.sfwd_ptr;
+  dsp_.h_xc_ptr = P.sxc_ptr;
+  dsp_.h_u_ptr = P.su_ptr;
+  dsp_.h_bwd_ptr = P.sbwd_ptr;
+  dsp_.h_rpart_ptr = P.rpart_ptr;
   d_scratch_.alloc(std::max<int32_t>(S.n, 1) * sizeof(double));
-  dsp_.ybuf = d_scratch_.get();
+  dsp_.xf = d_scratch_.get();
   d_uvec_.alloc(std::max<int64_t>(1, static_cast<int64_t>(S.rows.size())) * sizeof(double));
 }
 
@@ -418,16 +417,24 @@ void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, 
   std::vector<char> panel(static_cast<size_t>(plan_->panel_entries) * tf, 0);
   for (int32_t s = 0; s < SS.ns; ++s) {
     const int32_t f = SS.first[s], l = SS.first[s + 1];
-    const int64_t m = (l - f) + SS.nrow[s];
+    const int64_t ld = panel_ld(l - f, SS.nrow[s]);
     for (int32_t j = f; j < l; ++j) {
       const int64_t cj = j - f;
       const int64_t len = Lp[j + 1] - Lp[j];
-      std::memcpy(panel.data() + (SS.panel_off[s] + cj + 1 + cj * m) * tf,
+      std::memcpy(panel.data() + (SS.panel_off[s] + cj + 1 + cj * ld) * tf,
                   static_cast<const char*>(Lx) + Lp[j] * tf, len * tf);
     }
   }
   MPIR_HIP(hipMemcpyAsync(d_panel_.get(), panel.data(), panel.size(), hipMemcpyHostToDevice, stream_));
   MPIR_HIP(hipMemcpyAsync(d_Dinv_.get(), Dinv, n * tf, hipMemcpyHostToDevice, stream_));
+  // the inverses of the diagonal blocks of big fronts, from L and D = 1/Dinv
+  if (uf_ == kF64) {
+    reciprocal<double>(n, d_Dinv_.as<double>(), d_D_.as<double>(), stream_);
+    invert_diag_blocks<double>(factor_args<double>(), dfp_, stream_);
+  } else {
+    reciprocal<float>(n, d_Dinv_.as<float>(), d_D_.as<float>(), stream_);
+    invert_diag_blocks<float>(factor_args<float>(), dfp_, stream_);
+  }
   MPIR_HIP(hipStreamSynchronize(stream_));
   analyzed_ = factored_ = true;
   precond_only_ = true;
@@ -455,17 +462,11 @@ void GmresLdl::factor(const double* Ax, bool device_ptr) {
   MPIR_HIP(hipGetLastError());
   int err = 0, pos = 0;
   if (uf_ == kF64) {
-    FactorArgs<double> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
-                         d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_upd_off_.as<int64_t>(),
-                         d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<double>(),
-                         d_upool_.as<double>(), d_D_.as<double>(), d_Dinv_.as<double>(), d_err_.as<int>()};
+    const FactorArgs<double> a = factor_args<double>();
     factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, st);
     count_positive<double>(n_, d_D_.as<double>(), d_cnt_.as<int>(), st);
   } else {
-    FactorArgs<float> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
-                        d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_upd_off_.as<int64_t>(),
-                        d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<float>(),
-                        d_upool_.as<float>(), d_D_.as<float>(), d_Dinv_.as<float>(), d_err_.as<int>()};
+    const FactorArgs<float> a = factor_args<float>();
     factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, st);
     count_positive<float>(n_, d_D_.as<float>(), d_cnt_.as<int>(), st);
   }
@@ -502,17 +503,8 @@ double GmresLdl::infnrm(const double* v) {
 
 void GmresLdl::precond_apply_dev(double* x) {
   stats_.precond_applications++;
-  if (uf_ == kF64) {
-    SolveArgs<double> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
-                        d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(),
-                        d_child_list_.as<int32_t>(), d_panel_.as<double>(), d_Dinv_.as<double>()};
-    ldl_solve<double, double>(a, dsp_, x, d_uvec_.as<double>(), ++solve_epoch_, stream_);
-  } else {
-    SolveArgs<float> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
-                       d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(),
-                       d_child_list_.as<int32_t>(), d_panel_.as<float>(), d_Dinv_.as<float>()};
-    ldl_solve<float, double>(a, dsp_, x, d_uvec_.as<double>(), ++solve_epoch_, stream_);
-  }
+  if (uf_ == kF64) ldl_solve<double, double>(solve_args<double>(), dsp_, x, d_uvec_.as<double>(), stream_);
+  else ldl_solve<float, double>(solve_args<float>(), dsp_, x, d_uvec_.as<double>(), stream_);
 }
 
 void GmresLdl::spmv_dev(const double* x, double* y) {
@@ -675,19 +667,7 @@ void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool devi
     if (!device_ptr) MPIR_HIP(hipMemcpyAsync(xc, xout, n * sizeof(double), hipMemcpyDeviceToHost, st));
     MPIR_HIP(hipStreamSynchronize(st));
   }
-  check_solve_flags();
   stats_.solve_ms = now_ms() - t0;
-}
-
-void GmresLdl::check_solve_flags() {
-  int err = 0;
-  MPIR_HIP(hipMemcpyAsync(&h_pinned_[2], dsp_.err, sizeof(int), hipMemcpyDeviceToHost, stream_));
-  MPIR_HIP(hipStreamSynchronize(stream_));
-  std::memcpy(&err, &h_pinned_[2], sizeof(int));
-  if (err) {
-    MPIR_HIP(hipMemsetAsync(dsp_.err, 0, sizeof(int), stream_));
-    throw Error(kErrInternal, "triangular solve: a dependency wait expired");
-  }
 }
 
 double GmresLdl::time_kernels(int what, int reps) {
@@ -708,16 +688,10 @@ double GmresLdl::time_kernels(int what, int reps) {
     } else if (what == 1) {
       precond_apply_dev(x);
     } else if (uf_ == kF64) {
-      FactorArgs<double> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
-                           d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_upd_off_.as<int64_t>(),
-                           d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<double>(),
-                           d_upool_.as<double>(), d_D_.as<double>(), d_Dinv_.as<double>(), d_err_.as<int>()};
+      const FactorArgs<double> a = factor_args<double>();
       factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, st);
     } else {
-      FactorArgs<float> a{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(), d_rows_.as<int32_t>(),
-                          d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(), d_upd_off_.as<int64_t>(),
-                          d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<float>(),
-                          d_upool_.as<float>(), d_D_.as<float>(), d_Dinv_.as<float>(), d_err_.as<int>()};
+      const FactorArgs<float> a = factor_args<float>();
       factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, st);
     }
   };
@@ -728,7 +702,6 @@ double GmresLdl::time_kernels(int what, int reps) {
   MPIR_HIP(hipEventSynchronize(e1));
   float ms = 0.f;
   MPIR_HIP(hipEventElapsedTime(&ms, e0, e1));
-  check_solve_flags();
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return static_cast<double>(ms) / reps;
@@ -761,6 +734,7 @@ void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void*
   for (int32_t s = 0; s < S.ns; ++s) {
     const int32_t f = S.first[s], l = S.first[s + 1];
     const int64_t m = (l - f) + S.nrow[s];
+    const int64_t ld = panel_ld(l - f, S.nrow[s]);
     const int32_t* R = S.rows.data() + S.rows_off[s];
     for (int32_t j = f; j < l; ++j) {
       const int64_t cj = j - f;
@@ -772,13 +746,13 @@ void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void*
           while (r < m && row_of(r) < row) ++r;
           if (r >= m || row_of(r) != row) throw Error(kErrInternal, "factor pattern outside its supernode");
           if (Li) Li[q] = row;
-          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * m));
+          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * ld));
           ++q;
         }
       } else {
         for (int64_t r = cj + 1; r < m; ++r) {
           if (Li) Li[q] = row_of(r);
-          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * m));
+          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * ld));
           ++q;
         }
       }

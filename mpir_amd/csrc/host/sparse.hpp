@@ -92,6 +92,16 @@ void l_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai,
                std::vector<int64_t>& Lp, std::vector<int32_t>& Li);
 
 // Supernodal partition and the multifrontal schedule built on it.
+// Fronts with at most this many rows are factored and solved by one
+// wavefront in LDS; larger ("big") fronts use the blocked kernels.
+constexpr int kSmallFront = 64;
+
+// Leading dimension of a supernode's panel. A big front (m = nc + nr rows)
+// carries nc extra rows below R_s, initialised to the identity, which the
+// blocked factorization turns into L_11^{-T} D^{-1} of its diagonal block:
+// the triangular solves then become plain matrix-vector products.
+constexpr int64_t panel_ld(int64_t nc, int64_t nr) { return nc + nr > kSmallFront ? 2 * nc + nr : nc + nr; }
+
 struct Supernodes {
   int32_t n = 0;
   int32_t ns = 0;                   // number of supernodes
@@ -105,7 +115,7 @@ struct Supernodes {
   int32_t nlevels = 0;
   std::vector<int32_t> level_ptr;   // nlevels+1 into level_sn
   std::vector<int32_t> level_sn;    // supernodes grouped by level
-  std::vector<int64_t> panel_off;   // ns+1, L panel storage (m_s * ncol_s)
+  std::vector<int64_t> panel_off;   // ns+1, panel storage (panel_ld * ncol_s)
   std::vector<int32_t> relidx;      // per supernode, position of each R_s row
                                     // inside the parent's front index list
   // statistics

@@ -119,7 +119,7 @@ void finish_supernodes(Supernodes& S) {
   for (int32_t s = 0; s < ns; ++s) {
     const int64_t nc = S.first[s + 1] - S.first[s];
     const int64_t m = nc + S.nrow[s];
-    S.panel_off[s + 1] = S.panel_off[s] + m * nc;
+    S.panel_off[s + 1] = S.panel_off[s] + panel_ld(nc, S.nrow[s]) * nc;
     S.max_front = std::max<int32_t>(S.max_front, static_cast<int32_t>(m));
   }
   for (int32_t s = 0; s < ns; ++s) {

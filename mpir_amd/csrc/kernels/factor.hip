@@ -8,9 +8,15 @@
 // level of the supernodal tree, which is what exposes parallelism.
 //
 // Storage (per supernode s with nc columns and nr rows below the block,
-// m = nc + nr):
-//   panel  m x nc column-major (ld = m): rows [cols of s, R_s]; on exit the
-//          strictly-lower part holds L and the diagonal holds D
+// m = nc + nr, ld = panel_ld(nc, nr)):
+//   panel  ld x nc column-major: rows [cols of s, R_s] and, for big fronts
+//          (m > 64), nc more rows that start as the identity. On exit the
+//          strictly-lower part of the first m rows holds L, the diagonal D,
+//          and the extra rows Z = L_11^{-T} D^{-1} (upper triangular): the
+//          blocked elimination applied to [F; I] computes them with the same
+//          panel solves and updates, since eliminating the first nc columns
+//          of [[F_11, F_21^T, I], [F_21, F_22, 0], [I, 0, 0]] leaves
+//          I * L_11^{-T} D^{-1} in the rows of the identity
 //   U_s    nr x nr column-major (ld = nr), lower part: the update matrix
 //          handed to the parent (extend-add through relidx)
 #include <hip/hip_runtime.h>
@@ -109,6 +115,7 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
   const int nc = a.first[s + 1] - f;
   const int nr = a.nrow[s];
   const int m = nc + nr;
+  const int64_t ld = panel_ld(nc, nr);
   const int t1 = min(m, t0 + kAsmRows);
   T* P = a.panel + a.panel_off[s];
   T* U = nr > 0 ? a.upool + a.upd_off[s] : nullptr;
@@ -128,7 +135,7 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
       const T* src = Uc + static_cast<int64_t>(b) * nrc;
       for (int q = max(b, a0) + tid; q < a1; q += blockDim.x) {
         const int ra = rel[q];
-        if (rb < nc) P[ra + static_cast<int64_t>(rb) * m] += src[q];
+        if (rb < nc) P[ra + rb * ld] += src[q];
         else U[(ra - nc) + static_cast<int64_t>(rb - nc) * nr] += src[q];
       }
     }
@@ -149,12 +156,12 @@ __global__ void __launch_bounds__(64) k_diag_block(FactorArgs<T> a, const Task4*
   const int nc = a.first[s + 1] - f;
   const int k0 = k * kPanelNB;
   const int nbk = min(kPanelNB, nc - k0);
-  const int m = nc + a.nrow[s];
-  T* P = a.panel + a.panel_off[s] + k0 + static_cast<int64_t>(k0) * m;
+  const int64_t ld = panel_ld(nc, a.nrow[s]);
+  T* P = a.panel + a.panel_off[s] + k0 + k0 * ld;
   const int r = threadIdx.x;
   T b[kPanelNB];
 #pragma unroll
-  for (int c = 0; c < kPanelNB; ++c) b[c] = (c < nbk && r < nbk) ? P[r + static_cast<int64_t>(c) * m] : T(0);
+  for (int c = 0; c < kPanelNB; ++c) b[c] = (c < nbk && r < nbk) ? P[r + c * ld] : T(0);
 #pragma unroll
   for (int j = 0; j < kPanelNB; ++j) {
     if (j < nbk) {
@@ -181,13 +188,15 @@ __global__ void __launch_bounds__(64) k_diag_block(FactorArgs<T> a, const Task4*
   if (r < nbk) {
 #pragma unroll
     for (int c = 0; c < kPanelNB; ++c)
-      if (c <= r && c < nbk) P[r + static_cast<int64_t>(c) * m] = b[c];
+      if (c <= r && c < nbk) P[r + c * ld] = b[c];
   }
 }
 
 // Rows below the diagonal block: X L_kk^T = A_rows, L_rows = X Dinv.
 // One thread per row, the row in registers, eliminated right-looking so the
-// updates of later columns are independent of each other.
+// updates of later columns are independent of each other. Rows r >= m are
+// the identity rows: row m + i enters at the step of its own column i with
+// e_i as its value (it is zero in every earlier column).
 template <class T>
 __global__ void __launch_bounds__(256) k_panel_trsm(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   __shared__ T Lk[kPanelNB][kPanelNB + 1];
@@ -197,20 +206,23 @@ __global__ void __launch_bounds__(256) k_panel_trsm(FactorArgs<T> a, const Task4
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
   const int m = nc + a.nrow[s];
+  const int64_t ld = panel_ld(nc, a.nrow[s]);
   const int k0 = k * kPanelNB;
   const int nbk = min(kPanelNB, nc - k0);
   T* P = a.panel + a.panel_off[s];
   const int tid = threadIdx.x;
   const int lane = tid & 63, w = tid >> 6;
   for (int c = w; c < nbk; c += 4)
-    if (lane < nbk && lane > c) Lk[lane][c] = P[(k0 + lane) + static_cast<int64_t>(k0 + c) * m];
+    if (lane < nbk && lane > c) Lk[lane][c] = P[(k0 + lane) + (k0 + c) * ld];
   if (tid < nbk) dv[tid] = a.Dinv[f + k0 + tid];
   __syncthreads();
   const int r = r0 + tid;
   if (r >= r1) return;
+  const int unit = r >= m + k0 ? r - m - k0 : -1;  // identity row of this block
   T x[kPanelNB];
 #pragma unroll
-  for (int c = 0; c < kPanelNB; ++c) x[c] = c < nbk ? P[r + static_cast<int64_t>(k0 + c) * m] : T(0);
+  for (int c = 0; c < kPanelNB; ++c)
+    x[c] = c >= nbk ? T(0) : unit >= 0 ? (c == unit ? T(1) : T(0)) : P[r + (k0 + c) * ld];
 #pragma unroll
   for (int t = 0; t < kPanelNB; ++t) {
     if (t < nbk) {
@@ -221,12 +233,13 @@ __global__ void __launch_bounds__(256) k_panel_trsm(FactorArgs<T> a, const Task4
   }
 #pragma unroll
   for (int c = 0; c < kPanelNB; ++c)
-    if (c < nbk) P[r + static_cast<int64_t>(k0 + c) * m] = x[c] * dv[c];
+    if (c < nbk) P[r + (k0 + c) * ld] = x[c] * dv[c];
 }
 
 // Trailing update tile: C[r0:r0+64, c0:c0+64] -= L_r D L_c^T over the nbk
 // columns of step k, lower part only. Columns < nc live in the panel,
-// columns >= nc in the update matrix.
+// columns >= nc in the update matrix. Tiles with r0 >= m cover identity rows
+// (rows m .. m+nc) and only panel columns.
 template <class T>
 __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   __shared__ T As[kPanelNB][kTile + 1];
@@ -237,6 +250,8 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
   const int nc = a.first[s + 1] - f;
   const int nr = a.nrow[s];
   const int m = nc + nr;
+  const int64_t ld = panel_ld(nc, nr);
+  const int rlim = r0 >= m ? m + nc : m;
   const int k0 = k * kPanelNB;
   const int nbk = min(kPanelNB, nc - k0);
   const T* P = a.panel + a.panel_off[s];
@@ -246,8 +261,8 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
   for (int e = tid; e < nbk * kTile; e += blockDim.x) {
     const int i = e % kTile, t = e / kTile;
     const int ra = r0 + i, rb = c0 + i;
-    As[t][i] = ra < m ? P[ra + static_cast<int64_t>(k0 + t) * m] : T(0);
-    Bs[t][i] = rb < m ? P[rb + static_cast<int64_t>(k0 + t) * m] * a.D[f + k0 + t] : T(0);
+    As[t][i] = ra < rlim ? P[ra + (k0 + t) * ld] : T(0);
+    Bs[t][i] = rb < m ? P[rb + (k0 + t) * ld] * a.D[f + k0 + t] : T(0);
   }
   __syncthreads();
   const int ti = (tid % 16) * 4, tj = (tid / 16) * 4;
@@ -272,9 +287,9 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int r = r0 + ti + i;
-      if (r >= m || r < c) continue;
-      if (c < nc) Pw[r + static_cast<int64_t>(c) * m] -= acc[i][j];
-      else U[(r - nc) + static_cast<int64_t>(c - nc) * nr] -= acc[i][j];
+      if (r >= rlim || r < c) continue;
+      if (c < nc) Pw[r + c * ld] -= acc[i][j];
+      else if (r < m) U[(r - nc) + static_cast<int64_t>(c - nc) * nr] -= acc[i][j];
     }
   }
 }
@@ -322,6 +337,31 @@ void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const do
 }
 
 template <class T>
+void invert_diag_blocks(const FactorArgs<T>& a, const DeviceFactorPlan& dp, hipStream_t st) {
+  for (int l = 0; l < dp.nlevels; ++l) {
+    const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
+    for (int q = st0; q < st1; ++q) {
+      const int p0 = dp.h_pan_aug_ptr[q], p1 = dp.h_pan_ptr[q + 1];
+      if (p1 > p0) hipLaunchKernelGGL(k_panel_trsm<T>, dim3(p1 - p0), dim3(256), 0, st, a, dp.pan_task + p0);
+      const int u0 = dp.h_upd_aug_ptr[q], u1 = dp.h_upd_ptr[q + 1];
+      if (u1 > u0) hipLaunchKernelGGL(k_update_tiles<T>, dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
+    }
+  }
+  MPIR_HIP(hipGetLastError());
+}
+
+template <class T>
+__global__ void k_reciprocal(int n, const T* __restrict__ x, T* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = T(1) / x[i];
+}
+
+template <class T>
+void reciprocal(int n, const T* x, T* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_reciprocal<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, x, y);
+}
+
+template <class T>
 void count_positive(int n, const T* D, int* out, hipStream_t st) {
   MPIR_HIP(hipMemsetAsync(out, 0, sizeof(int), st));
   hipLaunchKernelGGL(k_count_positive<T>, dim3(std::min<unsigned>(1024, blocks_for(n, 256))), dim3(256), 0, st, n, D,
@@ -332,6 +372,10 @@ template void factor_numeric<double>(const FactorArgs<double>&, const DeviceFact
                                      hipStream_t);
 template void factor_numeric<float>(const FactorArgs<float>&, const DeviceFactorPlan&, const double*, int64_t,
                                     hipStream_t);
+template void invert_diag_blocks<double>(const FactorArgs<double>&, const DeviceFactorPlan&, hipStream_t);
+template void invert_diag_blocks<float>(const FactorArgs<float>&, const DeviceFactorPlan&, hipStream_t);
+template void reciprocal<double>(int, const double*, double*, hipStream_t);
+template void reciprocal<float>(int, const float*, float*, hipStream_t);
 template void count_positive<double>(int, const double*, int*, hipStream_t);
 template void count_positive<float>(int, const float*, int*, hipStream_t);
 

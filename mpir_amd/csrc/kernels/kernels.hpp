@@ -45,11 +45,19 @@ struct DeviceFactorPlan {
   const int32_t* asm_child = nullptr;
   const int64_t* val_dst = nullptr;
   std::vector<int32_t> h_small_ptr, h_big_ptr, h_asm_ptr, h_level_step_ptr, h_pan_ptr, h_upd_ptr, h_diag_ptr;
+  std::vector<int32_t> h_pan_aug_ptr, h_upd_aug_ptr;
 };
 
 template <class T>
 void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const double* ax_scaled, int64_t nnz,
                     hipStream_t st);
+// Recomputes only the rows L_11^{-T} D^{-1} below each big front's panel from
+// a factor already in place (L in the panel, D and Dinv set).
+template <class T>
+void invert_diag_blocks(const FactorArgs<T>& a, const DeviceFactorPlan& dp, hipStream_t st);
+// y = 1 / x elementwise
+template <class T>
+void reciprocal(int n, const T* x, T* y, hipStream_t st);
 template <class T>
 void count_positive(int n, const T* D, int* out, hipStream_t st);
 
@@ -67,31 +75,30 @@ struct SolveArgs {
   const int32_t* child_ptr;
   const int32_t* child_list;
   const TF* panel;
+  const TF* D;
   const TF* Dinv;
 };
 
 struct DeviceSolvePlan {
   int nlevels = 0;
-  const int32_t* small_list = nullptr;  // small fronts, grouped by level
-  const int32_t* fwd_task = nullptr;    // (s, row block) pairs, grouped by level
-  const int32_t* bwd_task = nullptr;    // (s, column block) pairs, grouped by level
-  const int32_t* flag_off = nullptr;    // per supernode: first flag of a big front
-  int* flags = nullptr;                 // one per column block of each big front
-  int* err = nullptr;                   // bit 1: a bounded wait expired
-  void* ybuf = nullptr;                 // n entries of the solve precision
-  const int32_t* rpart_task = nullptr;  // (s, column block, row chunk) triples
-  const int32_t* fwd_child_ptr = nullptr;  // per forward task + 1
-  const int32_t* fwd_child = nullptr;      // (child, q0, q1) triples
-  const int64_t* part_off = nullptr;    // per supernode
-  void* part = nullptr;                 // partial sums of the solve precision
-  std::vector<int32_t> h_small_ptr, h_fwd_ptr, h_bwd_ptr, h_rpart_ptr;
+  const int32_t* small_list = nullptr;    // small fronts, grouped by level
+  const int32_t* gather_task = nullptr;   // (s, row block) pairs, grouped by level
+  const int32_t* gather_child_ptr = nullptr;  // per gather task + 1
+  const int32_t* gather_child = nullptr;      // (child, q0, q1) triples
+  const int32_t* xc_task = nullptr;       // (s, column block) pairs
+  const int32_t* u_task = nullptr;        // (s, R row block) pairs
+  const int32_t* bwd_task = nullptr;      // (s, row block) pairs
+  const int32_t* rpart_task = nullptr;    // (s, column block, row chunk) triples
+  const int64_t* part_off = nullptr;      // per supernode
+  void* part = nullptr;                   // partial sums of the solve precision
+  void* xf = nullptr;                     // n entries: forward result of big fronts
+  std::vector<int32_t> h_small_ptr, h_gather_ptr, h_xc_ptr, h_u_ptr, h_bwd_ptr, h_rpart_ptr;
 };
 
 // x (length n, permuted order) is overwritten by (L D L^T)^{-1} x.
-// uvec: scratch of sum(nrow) entries of TX. epoch: a counter that increases
-// by one per call on the same plan (flag values of the two passes).
+// uvec: scratch of sum(nrow) entries of TX.
 template <class TF, class TX>
-void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, int epoch, hipStream_t st);
+void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st);
 
 // ---------------------------------------------------------------- vectors / SpMV
 struct CsrDev {

@@ -10,6 +10,12 @@
 // u_s = v_R - L_21 y on its rows below for the parent. Backward: a front
 // gathers x on R_s (all ancestors, already final) and solves its block
 // transposed. Fronts of one level are independent.
+//
+// Big fronts hold Z = L_11^{-T} D^{-1} (upper triangular) below their panel,
+// so their solves are matrix-vector products without a dependency chain:
+//   forward   x_c = Z^T v_c  (= D^{-1} L_11^{-1} v_c),  u = v_R - L_21 D x_c
+//   backward  x_c = Z D (x_c - L_21^T x_R)          (= L_11^{-T} (...))
+// Every sum runs in a fixed order, so repeated solves agree bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -80,131 +86,158 @@ __global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF> a, const int32_t
 }
 
 // ---------------------------------------------------------------- big fronts
-// One launch per level handles every big front of the level. A front is cut
-// into 64-row blocks; its diagonal blocks form a chain (block i needs y of
-// every column block j < i), carried by per-block flags that hold the epoch
-// of the current pass, so flags never need resetting. Workgroups are
-// launched in an order where every wait targets a workgroup with a smaller
-// index, which the in-order dispatch has already started. Waits are bounded:
-// on expiry the kernel records an error and finishes.
 constexpr int kRB = kSolveRB;
-constexpr int kSpinLimit = 1 << 24;
+constexpr int kChunk = kRpartRows;  // vector entries staged in LDS at a time
 
-// Values passed between workgroups of one launch use relaxed device-scope
-// atomic loads and stores: on gfx950 these are plain accesses with the sc1
-// (device-coherent) bit, so no cache writeback or invalidation is needed. A
-// writer waits for its stores to complete (s_waitcnt) before a barrier and
-// the flag store; a reader spins on the flag, then loads the data.
-template <class T>
-__device__ __forceinline__ T ld_dev(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class T>
-__device__ __forceinline__ void st_dev(T* p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void wait_flag(const int* flag, int epoch, int* err) {
-  int it = 0;
-  while (ld_dev(flag) != epoch) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++it > kSpinLimit) {
-      atomicOr(err, 2);
-      break;
-    }
-  }
-  // keeps the compiler from moving the data loads above the wait (no cache
-  // maintenance is emitted at workgroup scope)
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
-// every thread of the block calls this after its last data store
-__device__ __forceinline__ void publish_flag(int* flag, int epoch) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) st_dev(flag, epoch);
-}
-
-// Forward, row block i of front s: v = [b on columns; 0 on R] + children's
-// update vectors - sum_j L[block i, block j] y_j; diagonal blocks then solve
-// their unit-lower block, write y (ybuf) and D^{-1} y (x) and publish.
-// Everything that does not depend on another block (the diagonal block, the
-// tile of the last dependency) is loaded before the waits.
-template <class TF, class TX>
-__global__ void __launch_bounds__(256) k_fwd_front(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+// Forward gather, 64-row block i of front s: v = [b on columns; 0 on R] +
+// the children's update vectors, children in fixed order. The column part
+// is written back into x, the R part into the front's update vector.
+template <class TX>
+__global__ void __launch_bounds__(64) k_fwd_gather(const int32_t* __restrict__ first, const int32_t* __restrict__ nrow,
+                                                   const int64_t* __restrict__ rows_off,
+                                                   const int32_t* __restrict__ relidx, const int32_t* __restrict__ tasks,
                                                    const int32_t* __restrict__ cptr, const int32_t* __restrict__ cl,
-                                                   const int32_t* __restrict__ flag_off, int* flags, int epoch,
-                                                   TX* x, TX* __restrict__ uvec, TX* ybuf, int* err) {
+                                                   TX* x, TX* uvec) {
   __shared__ TX vsh[kRB];
-  __shared__ TX part[4][kRB];
-  __shared__ TF Ld[kRB][kRB + 1];
   const int s = tasks[2 * blockIdx.x], i = tasks[2 * blockIdx.x + 1];
+  const int f = first[s];
+  const int nc = first[s + 1] - f;
+  const int m = nc + nrow[s];
+  const int r0 = i * kRB;
+  const int lane = threadIdx.x;
+  const int r = r0 + lane;
+  vsh[lane] = r < nc ? x[f + r] : TX(0);
+  __syncthreads();
+  for (int e = cptr[blockIdx.x]; e < cptr[blockIdx.x + 1]; ++e) {
+    const int c = cl[3 * e], q0 = cl[3 * e + 1], q1 = cl[3 * e + 2];
+    const int q = q0 + lane;
+    if (q < q1) vsh[relidx[rows_off[c] + q] - r0] += uvec[rows_off[c] + q];
+    __syncthreads();
+  }
+  if (r < nc) x[f + r] = vsh[lane];
+  else if (r < m) uvec[rows_off[s] + (r - nc)] = vsh[lane];
+}
+
+// Forward, column block j of front s: xf[col] = sum_{i <= col} Z[i, col] v_i
+// over the block's columns (Z is zero below its diagonal, so whole 64-row
+// tiles are summed). Columns of Z are contiguous: each wave walks four
+// columns at a time down the rows, v staged in LDS.
+template <class TF, class TX>
+__global__ void __launch_bounds__(256) k_fwd_xc(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+                                                const TX* __restrict__ x, TX* __restrict__ xf) {
+  __shared__ TX vs[kChunk];
+  const int s = tasks[2 * blockIdx.x], j = tasks[2 * blockIdx.x + 1];
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
   const int nr = a.nrow[s];
   const int m = nc + nr;
-  const int r0 = i * kRB;
-  const int rows = min(kRB, m - r0);
+  const int64_t ld = panel_ld(nc, nr);
+  const int c0 = j * kRB;
+  const int bw = min(kRB, nc - c0);
+  const int rend = c0 + bw;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const TF* P = a.panel + a.panel_off[s];
-  TX* us = uvec + a.rows_off[s];
-  const bool diag = r0 < nc;
-  const int bwd = diag ? min(kRB, nc - r0) : 0;
-  // diagonal block into LDS
-  for (int t = w; t < bwd; t += 4)
-    if (lane < rows) Ld[lane][t] = P[(r0 + lane) + static_cast<int64_t>(r0 + t) * m];
-  if (tid < kRB) {
-    const int r = r0 + tid;
-    vsh[tid] = (tid < rows && r < nc) ? x[f + r] : TX(0);
-  }
-  __syncthreads();
-  for (int e = cptr[blockIdx.x]; e < cptr[blockIdx.x + 1]; ++e) {
-    const int c = cl[3 * e], q0 = cl[3 * e + 1], q1 = cl[3 * e + 2];
-    const int32_t* rel = a.relidx + a.rows_off[c];
-    const TX* uc = uvec + a.rows_off[c];
-    for (int q = q0 + tid; q < q1; q += blockDim.x) vsh[rel[q] - r0] += uc[q];
+  const TF* Z = a.panel + a.panel_off[s] + m;
+  TX acc[4][4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[g][u] = TX(0);
+  for (int q0 = 0; q0 < rend; q0 += kChunk) {
+    const int len = min(kChunk, rend - q0);
     __syncthreads();
-  }
-  const int ncb = (nc + kRB - 1) / kRB;
-  const int jend = min(i, ncb);
-  const int rr = r0 + (lane < rows ? lane : 0);
-  TX acc = TX(0);
-  // column blocks oldest first: the newest dependency (j = i - 1) is the last
-  // item of its wave, so after its flag only one 64-column update remains;
-  // each tile is loaded before the wait
-  for (int j = w; j < jend; j += 4) {
-    const int c0 = j * kRB;
-    const int bw = min(kRB, nc - c0);
-    const TF* col = P + rr + static_cast<int64_t>(c0) * m;
-    TF lv[kRB];
+    for (int q = tid; q < len; q += blockDim.x) vs[q] = x[f + q0 + q];
+    __syncthreads();
 #pragma unroll
-    for (int t = 0; t < kRB; ++t) lv[t] = t < bw ? col[static_cast<int64_t>(t) * m] : TF(0);
-    if (lane == 0) wait_flag(flags + flag_off[s] + j, epoch, err);
-    const TX ylane = lane < bw ? ld_dev(ybuf + f + c0 + lane) : TX(0);
+    for (int g = 0; g < 4; ++g) {
+      const int t0 = g * 16 + w * 4;
+      if (t0 >= bw) break;
+      const TF* col[4];
 #pragma unroll
-    for (int t = 0; t < kRB; ++t) acc += static_cast<TX>(lv[t]) * __shfl(ylane, t);
-  }
-  part[w][lane] = acc;
-  __syncthreads();
-  if (w == 0) {
-    TX va = lane < rows ? vsh[lane] - ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane])) : TX(0);
-    if (diag) {
-      for (int t = 0; t < bwd; ++t) {
-        const TX yt = __shfl(va, t);
-        if (lane > t && lane < rows) va -= static_cast<TX>(Ld[lane][t]) * yt;
+      for (int u = 0; u < 4; ++u) col[u] = Z + q0 + (c0 + min(t0 + u, bw - 1)) * ld;
+      for (int q = lane; q < len; q += 64) {
+        const TX v = vs[q];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[g][u] += static_cast<TX>(col[u][q]) * v;
       }
-      if (lane < bwd) {
-        st_dev(ybuf + f + r0 + lane, va);
-        x[f + r0 + lane] = va * static_cast<TX>(a.Dinv[f + r0 + lane]);
-      } else if (lane < rows) {
-        us[r0 + lane - nc] = va;
-      }
-    } else if (lane < rows) {
-      us[r0 + lane - nc] = va;
     }
   }
-  if (diag) publish_flag(flags + flag_off[s] + i, epoch);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int t0 = g * 16 + w * 4;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      TX v = acc[g][u];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0 && t0 + u < bw) xf[f + c0 + t0 + u] = v;
+    }
+  }
+}
+
+// Row-block GEMV over a panel: out[r] = sum_{c in [cb, ce)} M[r, c] y_c for
+// the 64 rows r0.. of the block, y_c = scale[c] * (yv[c] - sum of nch
+// partial sums) staged in LDS. Each lane owns a row, the four waves split
+// the columns, and the wave results are added in a fixed order.
+template <class TF, class TX>
+__device__ __forceinline__ TX rowblock_gemv(const TF* __restrict__ M, int64_t ld, int rows, int cb, int ce,
+                                            const TF* __restrict__ scale, const TX* __restrict__ yv,
+                                            const TX* __restrict__ part, int nch, TX* ys, TX (*red)[kRB]) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int rr = lane < rows ? lane : 0;
+  TX acc = TX(0);
+  for (int q0 = cb; q0 < ce; q0 += kChunk) {
+    const int len = min(kChunk, ce - q0);
+    __syncthreads();
+    for (int q = tid; q < len; q += blockDim.x) {
+      const int c = q0 + q;
+      TX z = yv[c];
+      if (part != nullptr) {
+        const TX* pp = part + static_cast<int64_t>(c / kRB) * nch * kRB + (c % kRB);
+        TX ps = TX(0);
+        for (int h = 0; h < nch; ++h) ps += pp[static_cast<int64_t>(h) * kRB];
+        z -= ps;
+      }
+      ys[q] = static_cast<TX>(scale[c]) * z;
+    }
+    __syncthreads();
+    // wave w: columns [w * per, (w + 1) * per) of the chunk, eight at a time
+    const int per = (len + 3) / 4;
+    const int cs = min(len, w * per), cend = min(len, cs + per);
+    const TF* mp = M + rr + (q0 + cs) * ld;
+    int c = cs;
+    for (; c + 8 <= cend; c += 8, mp += 8 * ld) {
+      TF lv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) lv[u] = mp[u * ld];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += static_cast<TX>(lv[u]) * ys[c + u];
+    }
+    for (; c < cend; ++c, mp += ld) acc += static_cast<TX>(*mp) * ys[c];
+  }
+  red[w][lane] = acc;
+  __syncthreads();
+  return (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+// Forward, 64-row block b of R_s: u = v_R - L_21 D xf.
+template <class TF, class TX>
+__global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+                                               const TX* __restrict__ xf, TX* __restrict__ uvec) {
+  __shared__ TX ys[kChunk];
+  __shared__ TX red[4][kRB];
+  const int s = tasks[2 * blockIdx.x], b = tasks[2 * blockIdx.x + 1];
+  const int f = a.first[s];
+  const int nc = a.first[s + 1] - f;
+  const int nr = a.nrow[s];
+  const int64_t ld = panel_ld(nc, nr);
+  const int rb0 = b * kRB;
+  const int rows = min(kRB, nr - rb0);
+  const TF* M = a.panel + a.panel_off[s] + nc + rb0;
+  const TX sum = rowblock_gemv<TF, TX>(M, ld, rows, 0, nc, a.D + f, xf + f, static_cast<const TX*>(nullptr), 0, ys, red);
+  const int lane = threadIdx.x;
+  if (lane < rows) {
+    TX* u = uvec + a.rows_off[s] + rb0;
+    u[lane] = u[lane] - sum;
+  }
 }
 
 // Backward, rows below the columns of a front (R rows, all final): partial
@@ -220,7 +253,7 @@ __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
   const int nr = a.nrow[s];
-  const int m = nc + nr;
+  const int64_t ld = panel_ld(nc, nr);
   const int c0 = j * kRB;
   const int bw = min(kRB, nc - c0);
   const int nch = (nr + kRpartRows - 1) / kRpartRows;
@@ -231,13 +264,12 @@ __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_
   for (int q = tid; q < len; q += blockDim.x) xs[q] = x[R[q0 + q]];
   __syncthreads();
   TX* out = part + part_off[s] + static_cast<int64_t>(j) * nch * kRB + static_cast<int64_t>(ch) * kRB;
-  // wave w: columns w, w + 4, ... of the block, four at a time
+  // wave w: columns w * 4, w * 4 + 16, ... of the block, four at a time
   for (int t0 = w * 4; t0 < bw; t0 += 16) {
     TX acc[4] = {TX(0), TX(0), TX(0), TX(0)};
     const TF* col[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      col[u] = P + (nc + q0) + static_cast<int64_t>(c0 + min(t0 + u, bw - 1)) * m;
+    for (int u = 0; u < 4; ++u) col[u] = P + (nc + q0) + (c0 + min(t0 + u, bw - 1)) * ld;
     for (int q = lane; q < len; q += 64) {
       const TX xv = xs[q];
 #pragma unroll
@@ -252,115 +284,55 @@ __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_
   }
 }
 
-// Backward, column block j of front s: z = x(block) - sum over the rows below
-// of L[r, block]^T x_r (R rows are final; rows of later column blocks wait
-// for those blocks' flags), then the unit-upper solve with L_jj^T. The tile
-// of the last dependency (block j+1) is staged transposed in LDS before the
-// wait, so after its flag only 64 LDS dot products, the diagonal solve and
-// the publish remain.
+// Backward, 64-row block i of the columns of front s:
+// x_r = sum_{c >= 64 i} Z[r, c] D_c z_c, z = xf - (partial sums of L_21^T x_R).
 template <class TF, class TX>
-__global__ void __launch_bounds__(256) k_bwd_front(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
-                                                   const int32_t* __restrict__ flag_off, int* flags, int epoch,
-                                                   TX* x, const int64_t* __restrict__ part_off,
-                                                   const TX* __restrict__ part, int* err) {
+__global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+                                                const int64_t* __restrict__ part_off, const TX* __restrict__ part,
+                                                const TX* __restrict__ xf, TX* __restrict__ x) {
+  __shared__ TX ys[kChunk];
   __shared__ TX red[4][kRB];
-  __shared__ TX xs[kRB];
-  __shared__ TF Ld[kRB][kRB + 1];
-  __shared__ TF Tt[kRB][kRB + 1];  // critical tile, Tt[t][r] = L[rb0 + r, c0 + t]
-  const int s = tasks[2 * blockIdx.x], j = tasks[2 * blockIdx.x + 1];
+  const int s = tasks[2 * blockIdx.x], i = tasks[2 * blockIdx.x + 1];
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
   const int nr = a.nrow[s];
   const int m = nc + nr;
-  const int c0 = j * kRB;
-  const int bw = min(kRB, nc - c0);
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const TF* P = a.panel + a.panel_off[s];
-  const int32_t* R = a.rows + a.rows_off[s];
-  const int ncb = (nc + kRB - 1) / kRB;
-  const int rb0 = c0 + bw;                 // first row below the block
-  const bool crit = j + 1 < ncb;           // tile 0 = rows [rb0, rb0 + 64) holds block j+1
-  const int crow = crit ? min(kRB, nc - rb0) : 0;  // its rows (columns of s only)
-  for (int t = w; t < bw; t += 4) {
-    if (lane < bw) Ld[lane][t] = P[(c0 + lane) + static_cast<int64_t>(c0 + t) * m];
-    if (lane < crow) Tt[t][lane] = P[(rb0 + lane) + static_cast<int64_t>(c0 + t) * m];
-  }
-  TX acc[kRB];
-#pragma unroll
-  for (int t = 0; t < kRB; ++t) acc[t] = TX(0);
-  // rows of later column blocks in 64-row tiles, oldest block first; the
-  // critical tile (q = 0 when crit) is left to the LDS path, and the R rows
-  // after the last column block come from the partial sums
-  const int q_first = crit ? 1 : 0;
-  const int ntiles = crit ? (nc - rb0 + kRB - 1) / kRB : 0;
-  for (int q = ntiles - 1 - w; q >= q_first; q -= 4) {
-    const int rs = rb0 + q * kRB;
-    const int r = rs + lane;
-    const int rc = r < nc ? r : rs;
-    const TF* row = P + rc + static_cast<int64_t>(c0) * m;
-    TF lv[kRB];
-#pragma unroll
-    for (int t = 0; t < kRB; ++t) lv[t] = t < bw ? row[static_cast<int64_t>(t) * m] : TF(0);
-    if (lane == 0) wait_flag(flags + flag_off[s] + rs / kRB, epoch, err);
-    const TX xr = r < nc ? ld_dev(x + f + r) : TX(0);
-#pragma unroll
-    for (int t = 0; t < kRB; ++t) acc[t] += static_cast<TX>(lv[t]) * xr;
-  }
-#pragma unroll
-  for (int t = 0; t < kRB; ++t) {
-    TX v = acc[t];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (lane == t) red[w][t] = v;
-  }
-  __syncthreads();
-  if (w == 0) {
-    TX z = lane < bw ? x[f + c0 + lane] - ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane])) : TX(0);
-    // R rows beyond the last column block's tile: partial sums in fixed order
-    const int nch = (nr + kRpartRows - 1) / kRpartRows;
-    if (lane < bw) {
-      const TX* pp = part + part_off[s] + static_cast<int64_t>(j) * nch * kRB + lane;
-      TX rs = TX(0);
-      for (int c = 0; c < nch; ++c) rs += pp[static_cast<int64_t>(c) * kRB];
-      z -= rs;
-    }
-    if (crit) {
-      if (lane == 0) wait_flag(flags + flag_off[s] + j + 1, epoch, err);
-      xs[lane] = lane < crow ? ld_dev(x + f + rb0 + lane) : TX(0);
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-      if (lane < bw) {
-        TX cs = TX(0);
-        for (int r = 0; r < crow; ++r) cs += static_cast<TX>(Tt[lane][r]) * xs[r];
-        z -= cs;
-      }
-    }
-    for (int t = bw - 1; t >= 0; --t) {
-      const TX xt = __shfl(z, t);
-      if (lane < t) z -= static_cast<TX>(Ld[t][lane]) * xt;
-    }
-    if (lane < bw) st_dev(x + f + c0 + lane, z);
-  }
-  publish_flag(flags + flag_off[s] + j, epoch);
+  const int64_t ld = panel_ld(nc, nr);
+  const int r0 = i * kRB;
+  const int rows = min(kRB, nc - r0);
+  const int nch = (nr + kRpartRows - 1) / kRpartRows;
+  const TF* M = a.panel + a.panel_off[s] + m + r0;
+  // the partial sums are laid out by column block: pass them shifted so that
+  // column c (absolute within the front) finds block c / 64
+  const TX* pp = nch > 0 ? part + part_off[s] : static_cast<const TX*>(nullptr);
+  const TX sum = rowblock_gemv<TF, TX>(M, ld, rows, r0, nc, a.D + f, xf + f, pp, nch, ys, red);
+  const int lane = threadIdx.x;
+  if (lane < rows) x[f + r0 + lane] = sum;
 }
 
 }  // namespace
 
 template <class TF, class TX>
-void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, int epoch, hipStream_t st) {
-  TX* ybuf = reinterpret_cast<TX*>(dp.ybuf);
+void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st) {
+  TX* xf = reinterpret_cast<TX*>(dp.xf);
+  TX* part = reinterpret_cast<TX*>(dp.part);
   // forward, leaves to root; D^{-1} fused
   for (int l = 0; l < dp.nlevels; ++l) {
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
       hipLaunchKernelGGL((k_fwd_small<TF, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x, uvec);
-    const int t0 = dp.h_fwd_ptr[l], t1 = dp.h_fwd_ptr[l + 1];
-    if (t1 > t0)
-      hipLaunchKernelGGL((k_fwd_front<TF, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.fwd_task + 2 * t0,
-                         dp.fwd_child_ptr + t0, dp.fwd_child, dp.flag_off, dp.flags, 2 * epoch + 1, x, uvec, ybuf,
-                         dp.err);
+    const int g0 = dp.h_gather_ptr[l], g1 = dp.h_gather_ptr[l + 1];
+    if (g1 > g0)
+      hipLaunchKernelGGL((k_fwd_gather<TX>), dim3(g1 - g0), dim3(64), 0, st, a.first, a.nrow, a.rows_off, a.relidx,
+                         dp.gather_task + 2 * g0, dp.gather_child_ptr + g0, dp.gather_child, x, uvec);
+    const int c0 = dp.h_xc_ptr[l], c1 = dp.h_xc_ptr[l + 1];
+    if (c1 > c0)
+      hipLaunchKernelGGL((k_fwd_xc<TF, TX>), dim3(c1 - c0), dim3(256), 0, st, a, dp.xc_task + 2 * c0, x, xf);
+    const int u0 = dp.h_u_ptr[l], u1 = dp.h_u_ptr[l + 1];
+    if (u1 > u0)
+      hipLaunchKernelGGL((k_fwd_u<TF, TX>), dim3(u1 - u0), dim3(256), 0, st, a, dp.u_task + 2 * u0, xf, uvec);
   }
   // backward, root to leaves
-  TX* part = reinterpret_cast<TX*>(dp.part);
   for (int l = dp.nlevels - 1; l >= 0; --l) {
     const int p0 = dp.h_rpart_ptr[l], p1 = dp.h_rpart_ptr[l + 1];
     if (p1 > p0)
@@ -368,16 +340,16 @@ void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uve
                          dp.part_off, x, part);
     const int t0 = dp.h_bwd_ptr[l], t1 = dp.h_bwd_ptr[l + 1];
     if (t1 > t0)
-      hipLaunchKernelGGL((k_bwd_front<TF, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.bwd_task + 2 * t0, dp.flag_off,
-                         dp.flags, 2 * epoch + 2, x, dp.part_off, part, dp.err);
+      hipLaunchKernelGGL((k_bwd_xc<TF, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.bwd_task + 2 * t0, dp.part_off,
+                         part, xf, x);
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0) hipLaunchKernelGGL((k_bwd_small<TF, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x);
   }
   MPIR_HIP(hipGetLastError());
 }
 
-template void ldl_solve<double, double>(const SolveArgs<double>&, const DeviceSolvePlan&, double*, double*, int, hipStream_t);
-template void ldl_solve<float, double>(const SolveArgs<float>&, const DeviceSolvePlan&, double*, double*, int, hipStream_t);
-template void ldl_solve<float, float>(const SolveArgs<float>&, const DeviceSolvePlan&, float*, float*, int, hipStream_t);
+template void ldl_solve<double, double>(const SolveArgs<double>&, const DeviceSolvePlan&, double*, double*, hipStream_t);
+template void ldl_solve<float, double>(const SolveArgs<float>&, const DeviceSolvePlan&, double*, double*, hipStream_t);
+template void ldl_solve<float, float>(const SolveArgs<float>&, const DeviceSolvePlan&, float*, float*, hipStream_t);
 
 }  // namespace mpir

@@ -107,6 +107,8 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
   P.level_step_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   P.pan_ptr.assign(1, 0);
   P.upd_ptr.assign(1, 0);
+  P.pan_aug_ptr.clear();
+  P.upd_aug_ptr.clear();
   P.diag_ptr.assign(1, 0);
   for (int32_t l = 0; l < S.nlevels; ++l) {
     int32_t max_steps = 0;
@@ -133,21 +135,35 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     P.big_ptr[l + 1] = static_cast<int32_t>(P.big_list.size());
     P.asm_ptr[l + 1] = static_cast<int32_t>(P.asm_task.size());
     for (int32_t k = 0; k < max_steps; ++k) {
-      for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
-        const int32_t s = P.big_list[q];
-        const int32_t nc = S.first[s + 1] - S.first[s];
-        const int32_t m = nc + S.nrow[s];
-        const int32_t k0 = k * kPanelNB;
-        if (k0 >= nc) continue;
-        const int32_t nbk = std::min(kPanelNB, nc - k0);
-        // rows strictly below the diagonal block, in chunks of 256
-        const int32_t pan0 = static_cast<int32_t>(P.pan_task.size());
-        for (int32_t r0 = k0 + nbk; r0 < m; r0 += 256)
-          P.pan_task.push_back({s, r0, std::min(m, r0 + 256), k});
-        P.diag_task.push_back({s, pan0, static_cast<int32_t>(P.pan_task.size()), k});
-        const int32_t t0 = k0 + nbk;
-        for (int32_t c0 = t0; c0 < m; c0 += kTile)
-          for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, k});
+      // real rows first, then the identity rows below R_s (aug rows), so a
+      // run that only rebuilds the inverses can launch the tail of a step
+      for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+          P.pan_aug_ptr.push_back(static_cast<int32_t>(P.pan_task.size()));
+          P.upd_aug_ptr.push_back(static_cast<int32_t>(P.upd_task.size()));
+        }
+        for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
+          const int32_t s = P.big_list[q];
+          const int32_t nc = S.first[s + 1] - S.first[s];
+          const int32_t m = nc + S.nrow[s];
+          const int32_t k0 = k * kPanelNB;
+          if (k0 >= nc) continue;
+          const int32_t nbk = std::min(kPanelNB, nc - k0);
+          const int32_t t0 = k0 + nbk;
+          if (pass == 0) {
+            // rows strictly below the diagonal block, in chunks of 256
+            const int32_t pan0 = static_cast<int32_t>(P.pan_task.size());
+            for (int32_t r0 = t0; r0 < m; r0 += 256) P.pan_task.push_back({s, r0, std::min(m, r0 + 256), k});
+            P.diag_task.push_back({s, pan0, static_cast<int32_t>(P.pan_task.size()), k});
+            for (int32_t c0 = t0; c0 < m; c0 += kTile)
+              for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, k});
+          } else {
+            // aug rows i < t0 are the only ones nonzero in this step's columns
+            for (int32_t r0 = 0; r0 < t0; r0 += 256) P.pan_task.push_back({s, m + r0, m + std::min(t0, r0 + 256), k});
+            for (int32_t c0 = t0; c0 < nc; c0 += kTile)
+              for (int32_t r0 = 0; r0 < t0; r0 += kTile) P.upd_task.push_back({s, m + r0, c0, k});
+          }
+        }
       }
       P.pan_ptr.push_back(static_cast<int32_t>(P.pan_task.size()));
       P.upd_ptr.push_back(static_cast<int32_t>(P.upd_task.size()));
@@ -180,38 +196,47 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     P.asm_child_ptr[t + 1] = static_cast<int32_t>(P.asm_child.size() / 5);
   }
 
-  // sync-free solve tasks and per-front flag ranges
-  P.flag_off.assign(S.ns, -1);
+  // solve tasks of the big fronts, per level
   P.sfwd_child_ptr.assign(1, 0);
   P.part_off.assign(S.ns, -1);
   P.rpart_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   P.sfwd_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.sxc_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.su_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   P.sbwd_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   for (int32_t l = 0; l < S.nlevels; ++l) {
     for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
       const int32_t s = P.big_list[q];
       const int32_t nc = S.first[s + 1] - S.first[s];
-      const int32_t m = nc + S.nrow[s];
+      const int32_t nr = S.nrow[s];
+      const int32_t m = nc + nr;
       const int32_t ncb = (nc + kSolveRB - 1) / kSolveRB;
       const int32_t nrb = (m + kSolveRB - 1) / kSolveRB;
-      P.flag_off[s] = P.nflags;
-      P.nflags += ncb;
       for (int32_t i = 0; i < nrb; ++i) {
         P.sfwd_task.push_back(s);
         P.sfwd_task.push_back(i);
         const int32_t r0 = i * kSolveRB, r1 = std::min(m, r0 + kSolveRB);
-        for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
-          const int32_t c = P.child_list[q];
+        for (int32_t q2 = P.child_ptr[s]; q2 < P.child_ptr[s + 1]; ++q2) {
+          const int32_t c = P.child_list[q2];
           const int32_t q0 = lower(c, r0), q1 = lower(c, r1);
           if (q1 > q0) P.sfwd_child.insert(P.sfwd_child.end(), {c, q0, q1});
         }
         P.sfwd_child_ptr.push_back(static_cast<int32_t>(P.sfwd_child.size() / 3));
       }
+      // forward column blocks, longest first; backward row blocks likewise
       for (int32_t j = ncb - 1; j >= 0; --j) {
-        P.sbwd_task.push_back(s);
-        P.sbwd_task.push_back(j);
+        P.sxc_task.push_back(s);
+        P.sxc_task.push_back(j);
       }
-      const int32_t nch = (S.nrow[s] + kRpartRows - 1) / kRpartRows;
+      for (int32_t b = 0; b * kSolveRB < nr; ++b) {
+        P.su_task.push_back(s);
+        P.su_task.push_back(b);
+      }
+      for (int32_t i = 0; i < ncb; ++i) {
+        P.sbwd_task.push_back(s);
+        P.sbwd_task.push_back(i);
+      }
+      const int32_t nch = (nr + kRpartRows - 1) / kRpartRows;
       P.part_off[s] = P.npart;
       P.npart += static_cast<int64_t>(ncb) * nch * kSolveRB;
       for (int32_t j = 0; j < ncb; ++j)
@@ -223,6 +248,8 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     }
     P.rpart_ptr[l + 1] = static_cast<int32_t>(P.rpart_task.size() / 3);
     P.sfwd_ptr[l + 1] = static_cast<int32_t>(P.sfwd_task.size() / 2);
+    P.sxc_ptr[l + 1] = static_cast<int32_t>(P.sxc_task.size() / 2);
+    P.su_ptr[l + 1] = static_cast<int32_t>(P.su_task.size() / 2);
     P.sbwd_ptr[l + 1] = static_cast<int32_t>(P.sbwd_task.size() / 2);
   }
   return P;
@@ -305,7 +332,7 @@ void fill_value_map(FactorPlan& P, const CscUpper& C, const std::vector<int32_t>
       const int32_t i = C.i[q];
       const int32_t s = S.col2sn[i];
       const int32_t f = S.first[s], l = S.first[s + 1];
-      const int64_t m = (l - f) + S.nrow[s];
+      const int64_t ld = panel_ld(l - f, S.nrow[s]);
       int64_t rp;
       if (j < l) {
         rp = j - f;
@@ -315,7 +342,7 @@ void fill_value_map(FactorPlan& P, const CscUpper& C, const std::vector<int32_t>
         if (it == R + S.nrow[s] || *it != j) throw std::logic_error("entry outside the L pattern");
         rp = (l - f) + (it - R);
       }
-      P.val_dst[val_map[q]] = S.panel_off[s] + static_cast<int64_t>(i - f) * m + rp;
+      P.val_dst[val_map[q]] = S.panel_off[s] + static_cast<int64_t>(i - f) * ld + rp;
     }
   }
 }

@@ -10,11 +10,9 @@
 
 namespace mpir {
 
-// Fronts with at most this many rows are factored by one wavefront in LDS.
-constexpr int kSmallFront = 64;
 // Column-block width of the blocked factorization of large fronts.
 constexpr int kPanelNB = 64;
-// Row / column block of the sync-free big-front triangular solves.
+// Row / column block of the big-front triangular solves.
 constexpr int kSolveRB = 64;
 // R rows per backward partial-sum task.
 constexpr int kRpartRows = 2048;
@@ -57,20 +55,23 @@ struct FactorPlan {
   std::vector<int32_t> diag_ptr;         // nsteps_total+1
   std::vector<Task4> diag_task;
   int32_t max_step_tasks = 0;            // max pan tasks in one step
-  // sync-free solves of big fronts: per level, forward tasks (s, row block)
-  // in ascending block order and backward tasks (s, column block) in
-  // descending order, so every wait targets an earlier workgroup
-  std::vector<int32_t> sfwd_ptr, sbwd_ptr;  // nlevels+1
-  std::vector<int32_t> sfwd_task, sbwd_task;  // pairs (s, block)
-  // per forward task, the children whose update vectors land in its rows:
-  // (child, q0, q1) = child rows [q0, q1)
+  // per step, the first panel / update task on the identity rows below R_s
+  // (the inverse of the diagonal block); those tasks end the step's range
+  std::vector<int32_t> pan_aug_ptr, upd_aug_ptr;  // nsteps_total
+  // triangular solves of big fronts, per level (pairs (s, block)):
+  //   sfwd: 64-row blocks of the front, gathering b and the children's
+  //         update vectors; sxc: column blocks, x_c = (L_11^{-T} D^{-1})^T v_c;
+  //   su:   64-row blocks of R_s, u = v_R - L_21 D x_c;
+  //   sbwd: row blocks, x_c = L_11^{-T} (x_c - L_21^T x_R)
+  std::vector<int32_t> sfwd_ptr, sxc_ptr, su_ptr, sbwd_ptr;  // nlevels+1
+  std::vector<int32_t> sfwd_task, sxc_task, su_task, sbwd_task;
+  // per forward gather task, the children whose update vectors land in its
+  // rows: (child, q0, q1) = child rows [q0, q1)
   std::vector<int32_t> sfwd_child_ptr;        // per forward task + 1
   std::vector<int32_t> sfwd_child;            // triples
-  std::vector<int32_t> flag_off;              // per supernode, -1 if small
-  int32_t nflags = 0;
-  // backward contribution of the rows below the columns (R rows), computed
-  // before the chain: tasks (s, column block, row chunk) per level, partial
-  // sums at part_off[s] + (block * nchunk + chunk) * kSolveRB
+  // backward contribution of the rows below the columns (R rows): tasks
+  // (s, column block, row chunk) per level, partial sums at
+  // part_off[s] + (block * nchunk + chunk) * kSolveRB
   std::vector<int32_t> rpart_ptr;             // nlevels+1
   std::vector<int32_t> rpart_task;            // triples
   std::vector<int64_t> part_off;              // per supernode

@@ -68,7 +68,6 @@ class GmresLdl {
 
   // building blocks exposed for tests and the kernel-level C entry points
   void precond_apply_dev(double* x);  // x := (L D L^T)^{-1} x, permuted order
-  void check_flags() { check_solve_flags(); }
   void spmv_dev(const double* x, double* y);  // y = K_s x, permuted, scaled
   // factor (permuted order) copied out in QDLDL's CSC layout
   void export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* D, void* Dinv, int prec);
@@ -87,7 +86,20 @@ class GmresLdl {
   double infnrm(const double* v);
   void release_numeric();
   void upload_plan();
-  void check_solve_flags();  // throws if a bounded wait in a solve expired
+  template <class T>
+  FactorArgs<T> factor_args() {
+    return FactorArgs<T>{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(),
+                         d_rows_.as<int32_t>(), d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(),
+                         d_upd_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(),
+                         d_panel_.as<T>(), d_upool_.as<T>(), d_D_.as<T>(), d_Dinv_.as<T>(), d_err_.as<int>()};
+  }
+  template <class T>
+  SolveArgs<T> solve_args() {
+    return SolveArgs<T>{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(),
+                        d_rows_.as<int32_t>(), d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(),
+                        d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<T>(), d_D_.as<T>(),
+                        d_Dinv_.as<T>()};
+  }
   void alloc_numeric();
 
   int uf_, uw_, ur_;
@@ -102,7 +114,6 @@ class GmresLdl {
   double true_flops_ = 0.0;
   bool analyzed_ = false, factored_ = false, precond_only_ = false;
   hipStream_t stream_ = nullptr;
-  int solve_epoch_ = 0;
   int32_t n_ = 0;
   int64_t nnz_ = 0;
   SolveStats stats_;
