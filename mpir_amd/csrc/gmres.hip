@@ -317,7 +317,6 @@ void GmresLdl::upload_plan() {
   upload_vec(keep_[3], P.pan_task, st);
   upload_vec(keep_[4], P.upd_task, st);
   upload_vec(keep_[5], P.val_dst, st);
-  upload_vec(keep_[6], P.diag_task, st);
   dfp_ = DeviceFactorPlan();
   dfp_.nlevels = S.nlevels;
   dfp_.panel_entries = P.panel_entries;
@@ -327,14 +326,12 @@ void GmresLdl::upload_plan() {
   dfp_.pan_task = keep_[3].as<Task4>();
   dfp_.upd_task = keep_[4].as<Task4>();
   dfp_.val_dst = keep_[5].as<int64_t>();
-  dfp_.diag_task = keep_[6].as<Task4>();
   dfp_.h_small_ptr = P.small_ptr;
   dfp_.h_big_ptr = P.big_ptr;
   dfp_.h_asm_ptr = P.asm_ptr;
   dfp_.h_level_step_ptr = P.level_step_ptr;
   dfp_.h_pan_ptr = P.pan_ptr;
   dfp_.h_upd_ptr = P.upd_ptr;
-  dfp_.h_diag_ptr = P.diag_ptr;
   dfp_.h_pan_aug_ptr = P.pan_aug_ptr;
   dfp_.h_upd_aug_ptr = P.upd_aug_ptr;
   keep_.resize(20);
@@ -369,8 +366,10 @@ void GmresLdl::upload_plan() {
   dsp_.h_u_ptr = P.su_ptr;
   dsp_.h_bwd_ptr = P.sbwd_ptr;
   dsp_.h_rpart_ptr = P.rpart_ptr;
-  d_scratch_.alloc(std::max<int32_t>(S.n, 1) * sizeof(double));
-  dsp_.xf = d_scratch_.get();
+  upload_vec(keep_[11], P.xfp_off, st);
+  dsp_.xfp_off = keep_[11].as<int64_t>();
+  d_scratch_.alloc(static_cast<size_t>(std::max<int64_t>(P.nxfp, 1)) * sizeof(double));
+  dsp_.xfp = d_scratch_.get();
   d_uvec_.alloc(std::max<int64_t>(1, static_cast<int64_t>(S.rows.size())) * sizeof(double));
 }
 
@@ -417,12 +416,15 @@ void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, 
   std::vector<char> panel(static_cast<size_t>(plan_->panel_entries) * tf, 0);
   for (int32_t s = 0; s < SS.ns; ++s) {
     const int32_t f = SS.first[s], l = SS.first[s + 1];
-    const int64_t ld = panel_ld(l - f, SS.nrow[s]);
+    const int64_t nc = l - f;
     for (int32_t j = f; j < l; ++j) {
       const int64_t cj = j - f;
       const int64_t len = Lp[j + 1] - Lp[j];
-      std::memcpy(panel.data() + (SS.panel_off[s] + cj + 1 + cj * ld) * tf,
-                  static_cast<const char*>(Lx) + Lp[j] * tf, len * tf);
+      for (int64_t t = 0; t < len; ++t) {
+        const int64_t r = cj + 1 + t;  // dense front row: the pattern fills the supernode
+        std::memcpy(panel.data() + (SS.panel_off[s] + panel_l_index(nc, SS.nrow[s], r, cj)) * tf,
+                    static_cast<const char*>(Lx) + (Lp[j] + t) * tf, tf);
+      }
     }
   }
   MPIR_HIP(hipMemcpyAsync(d_panel_.get(), panel.data(), panel.size(), hipMemcpyHostToDevice, stream_));
@@ -734,7 +736,6 @@ void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void*
   for (int32_t s = 0; s < S.ns; ++s) {
     const int32_t f = S.first[s], l = S.first[s + 1];
     const int64_t m = (l - f) + S.nrow[s];
-    const int64_t ld = panel_ld(l - f, S.nrow[s]);
     const int32_t* R = S.rows.data() + S.rows_off[s];
     for (int32_t j = f; j < l; ++j) {
       const int64_t cj = j - f;
@@ -746,13 +747,13 @@ void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void*
           while (r < m && row_of(r) < row) ++r;
           if (r >= m || row_of(r) != row) throw Error(kErrInternal, "factor pattern outside its supernode");
           if (Li) Li[q] = row;
-          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * ld));
+          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + panel_l_index(l - f, S.nrow[s], r, cj)));
           ++q;
         }
       } else {
         for (int64_t r = cj + 1; r < m; ++r) {
           if (Li) Li[q] = row_of(r);
-          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + r + cj * ld));
+          if (Lx) put(Lx, q, get(panel, S.panel_off[s] + panel_l_index(l - f, S.nrow[s], r, cj)));
           ++q;
         }
       }

@@ -95,12 +95,23 @@ void l_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai,
 // Fronts with at most this many rows are factored and solved by one
 // wavefront in LDS; larger ("big") fronts use the blocked kernels.
 constexpr int kSmallFront = 64;
+// Column-block width of the blocked factorization of big fronts.
+constexpr int kPanelNB = 64;
 
 // Leading dimension of a supernode's panel. A big front (m = nc + nr rows)
 // carries nc extra rows below R_s, initialised to the identity, which the
 // blocked factorization turns into L_11^{-T} D^{-1} of its diagonal block:
 // the triangular solves then become plain matrix-vector products.
 constexpr int64_t panel_ld(int64_t nc, int64_t nr) { return nc + nr > kSmallFront ? 2 * nc + nr : nc + nr; }
+
+// Position of L[r, c] (front-local row r > column c) in a panel. Big fronts
+// keep the strictly-lower part of each kPanelNB-wide diagonal block transposed in
+// the block's upper triangle (their factorization writes it there while the
+// lower triangle is still being read).
+constexpr int64_t panel_l_index(int64_t nc, int64_t nr, int64_t r, int64_t c) {
+  return (nc + nr > kSmallFront && r < nc && r / kPanelNB == c / kPanelNB) ? c + r * panel_ld(nc, nr)
+                                                                  : r + c * panel_ld(nc, nr);
+}
 
 struct Supernodes {
   int32_t n = 0;

@@ -11,8 +11,11 @@
 // m = nc + nr, ld = panel_ld(nc, nr)):
 //   panel  ld x nc column-major: rows [cols of s, R_s] and, for big fronts
 //          (m > 64), nc more rows that start as the identity. On exit the
-//          strictly-lower part of the first m rows holds L, the diagonal D,
-//          and the extra rows Z = L_11^{-T} D^{-1} (upper triangular): the
+//          strictly-lower part of the first m rows holds L (small fronts
+//          also keep D on the diagonal; big fronts keep the strictly-lower
+//          part of each 64 x 64 diagonal block transposed, in the block's
+//          upper triangle), and the extra rows Z = L_11^{-T} D^{-1} (upper
+//          triangular): the
 //          blocked elimination applied to [F; I] computes them with the same
 //          panel solves and updates, since eliminating the first nc columns
 //          of [[F_11, F_21^T, I], [F_21, F_22, 0], [I, 0, 0]] leaves
@@ -144,63 +147,20 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
 }
 
 // ---------------------------------------------------------------- big fronts
-// Diagonal block of step k: LDL^T of an nbk x nbk block by one wavefront,
-// lane r holding row r of the block in registers; column j's multipliers are
-// broadcast through LDS.
+// Panel step k of big front s, task (s, r0, r1, k). Every workgroup factors
+// the nbk x nbk diagonal block in LDS (right-looking LDL^T) and carries 64
+// identity rows through the same elimination, which leaves
+// Z = L_kk^{-T} D_k^{-1} (upper triangular) in them. The task with
+// r0 = k0 owns the diagonal block and writes L_kk, D and D^{-1}; every other
+// task forms its rows of L as X Z, X = its rows of the current panel
+// (identity rows of the front, r >= m, enter at their own column as e_i).
+// refactor = false: L_kk and D are already final (a factor loaded from
+// outside); the block is rebuilt from them and only Z is formed.
 template <class T>
-__global__ void __launch_bounds__(64) k_diag_block(FactorArgs<T> a, const Task4* __restrict__ fronts) {
-  __shared__ T lb[kPanelNB];
-  const Task4 tk = fronts[blockIdx.x];
-  const int s = tk.a, k = tk.d;
-  const int f = a.first[s];
-  const int nc = a.first[s + 1] - f;
-  const int k0 = k * kPanelNB;
-  const int nbk = min(kPanelNB, nc - k0);
-  const int64_t ld = panel_ld(nc, a.nrow[s]);
-  T* P = a.panel + a.panel_off[s] + k0 + k0 * ld;
-  const int r = threadIdx.x;
-  T b[kPanelNB];
-#pragma unroll
-  for (int c = 0; c < kPanelNB; ++c) b[c] = (c < nbk && r < nbk) ? P[r + c * ld] : T(0);
-#pragma unroll
-  for (int j = 0; j < kPanelNB; ++j) {
-    if (j < nbk) {
-      const T d = __shfl(b[j], j);
-      const T dinv = T(1) / d;
-      const T y = b[j];
-      const T l = y * dinv;
-      lb[r] = l;
-      __syncthreads();
-      if (r > j) {
-#pragma unroll
-        for (int c = j + 1; c < kPanelNB; ++c)
-          if (c <= r) b[c] -= y * lb[c];
-        b[j] = l;
-      }
-      if (r == j) {
-        if (d == T(0)) flag_zero_pivot(a.err);
-        a.D[f + k0 + j] = d;
-        a.Dinv[f + k0 + j] = dinv;
-      }
-      __syncthreads();
-    }
-  }
-  if (r < nbk) {
-#pragma unroll
-    for (int c = 0; c < kPanelNB; ++c)
-      if (c <= r && c < nbk) P[r + c * ld] = b[c];
-  }
-}
-
-// Rows below the diagonal block: X L_kk^T = A_rows, L_rows = X Dinv.
-// One thread per row, the row in registers, eliminated right-looking so the
-// updates of later columns are independent of each other. Rows r >= m are
-// the identity rows: row m + i enters at the step of its own column i with
-// e_i as its value (it is zero in every earlier column).
-template <class T>
-__global__ void __launch_bounds__(256) k_panel_trsm(FactorArgs<T> a, const Task4* __restrict__ tasks) {
-  __shared__ T Lk[kPanelNB][kPanelNB + 1];
-  __shared__ T dv[kPanelNB];
+__global__ void __launch_bounds__(256) k_panel_step(FactorArgs<T> a, const Task4* __restrict__ tasks, bool refactor) {
+  __shared__ T F[kPanelNB][kPanelNB + 1];  // diagonal block, then the task's rows
+  __shared__ T G[kPanelNB][kPanelNB + 1];  // identity rows -> Z
+  __shared__ T lc[kPanelNB], lg[kPanelNB];
   const Task4 tk = tasks[blockIdx.x];
   const int s = tk.a, r0 = tk.b, r1 = tk.c, k = tk.d;
   const int f = a.first[s];
@@ -210,30 +170,82 @@ __global__ void __launch_bounds__(256) k_panel_trsm(FactorArgs<T> a, const Task4
   const int k0 = k * kPanelNB;
   const int nbk = min(kPanelNB, nc - k0);
   T* P = a.panel + a.panel_off[s];
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, w = tid >> 6;
-  for (int c = w; c < nbk; c += 4)
-    if (lane < nbk && lane > c) Lk[lane][c] = P[(k0 + lane) + (k0 + c) * ld];
-  if (tid < nbk) dv[tid] = a.Dinv[f + k0 + tid];
-  __syncthreads();
-  const int r = r0 + tid;
-  if (r >= r1) return;
-  const int unit = r >= m + k0 ? r - m - k0 : -1;  // identity row of this block
-  T x[kPanelNB];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const bool owner = r0 == k0;
+  const int rows = r1 - r0;
+  // this thread's part of its row (columns 16w .. 16w + 15), loaded early
+  const int r = r0 + lane;
+  const int unit = r >= m + k0 ? r - m - k0 : -1;
+  T xv[16];
 #pragma unroll
-  for (int c = 0; c < kPanelNB; ++c)
-    x[c] = c >= nbk ? T(0) : unit >= 0 ? (c == unit ? T(1) : T(0)) : P[r + (k0 + c) * ld];
-#pragma unroll
-  for (int t = 0; t < kPanelNB; ++t) {
-    if (t < nbk) {
-      const T xt = x[t];
-#pragma unroll
-      for (int c = t + 1; c < kPanelNB; ++c) x[c] -= xt * Lk[c][t];
+  for (int u = 0; u < 16; ++u) {
+    const int c = 16 * w + u;
+    xv[u] = (owner || lane >= rows || c >= nbk) ? T(0)
+            : unit >= 0                         ? (c == unit ? T(1) : T(0))
+                                                : P[r + (k0 + c) * ld];
+  }
+  const T* Dk = a.D + f + k0;
+  for (int c = w; c < kPanelNB; c += 4) {
+    T v = T(0);
+    if (lane < nbk && c < nbk && lane >= c) {
+      if (refactor) v = P[(k0 + lane) + (k0 + c) * ld];
+      else v = lane == c ? Dk[c] : P[(k0 + c) + (k0 + lane) * ld] * Dk[c];
     }
+    F[lane][c] = v;
+    G[lane][c] = lane == c ? T(1) : T(0);
+  }
+  __syncthreads();
+  for (int j = 0; j < nbk; ++j) {
+    const T d = F[j][j];
+    const T dinv = T(1) / d;
+    if (w == 0) {
+      if (lane > j && lane < nbk) {
+        const T l = F[lane][j] * dinv;
+        lc[lane] = l;
+        // L_kk goes to the block's upper triangle (transposed): the other
+        // workgroups of this launch are still reading the lower triangle
+        if (owner && refactor) P[(k0 + j) + (k0 + lane) * ld] = l;
+      }
+      if (lane == j && owner && refactor) {
+        if (d == T(0)) flag_zero_pivot(a.err);
+        a.D[f + k0 + j] = d;
+        a.Dinv[f + k0 + j] = dinv;
+      }
+    } else if (w == 1) {
+      if (lane <= j) lg[lane] = G[lane][j] * dinv;
+    }
+    __syncthreads();
+    // trailing block rows (lower part) and active identity rows i <= j
+    if (refactor && lane > j && lane < nbk) {
+      const T l = lc[lane];
+      for (int c = j + 1 + w; c <= lane; c += 4) F[lane][c] -= l * F[c][j];
+    }
+    if (lane <= j) {
+      const T g = lg[lane];
+      for (int c = j + 1 + w; c < nbk; c += 4) G[lane][c] -= g * F[c][j];
+      if (w == 0) G[lane][j] = g;
+    }
+    __syncthreads();
+  }
+  if (owner) return;
+  // rows of L: X Z, thread (row lane, columns 16w ..)
+#pragma unroll
+  for (int u = 0; u < 16; ++u) F[lane][16 * w + u] = xv[u];
+  __syncthreads();
+  if (lane >= rows) return;
+  T acc[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) acc[u] = T(0);
+  for (int t = 0; t < nbk; ++t) {
+    const T xt = F[lane][t];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) acc[u] += xt * G[t][16 * w + u];
   }
 #pragma unroll
-  for (int c = 0; c < kPanelNB; ++c)
-    if (c < nbk) P[r + (k0 + c) * ld] = x[c] * dv[c];
+  for (int u = 0; u < 16; ++u) {
+    const int c = 16 * w + u;
+    if (c < nbk) P[r + (k0 + c) * ld] = acc[u];
+  }
 }
 
 // Trailing update tile: C[r0:r0+64, c0:c0+64] -= L_r D L_c^T over the nbk
@@ -325,10 +337,8 @@ void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const do
                          dp.asm_child);
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
     for (int q = st0; q < st1; ++q) {
-      const int d0 = dp.h_diag_ptr[q], d1 = dp.h_diag_ptr[q + 1];
-      hipLaunchKernelGGL(k_diag_block<T>, dim3(d1 - d0), dim3(64), 0, st, a, dp.diag_task + d0);
       const int p0 = dp.h_pan_ptr[q], p1 = dp.h_pan_ptr[q + 1];
-      if (p1 > p0) hipLaunchKernelGGL(k_panel_trsm<T>, dim3(p1 - p0), dim3(256), 0, st, a, dp.pan_task + p0);
+      if (p1 > p0) hipLaunchKernelGGL(k_panel_step<T>, dim3(p1 - p0), dim3(256), 0, st, a, dp.pan_task + p0, true);
       const int u0 = dp.h_upd_ptr[q], u1 = dp.h_upd_ptr[q + 1];
       if (u1 > u0) hipLaunchKernelGGL(k_update_tiles<T>, dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
     }
@@ -342,7 +352,7 @@ void invert_diag_blocks(const FactorArgs<T>& a, const DeviceFactorPlan& dp, hipS
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_aug_ptr[q], p1 = dp.h_pan_ptr[q + 1];
-      if (p1 > p0) hipLaunchKernelGGL(k_panel_trsm<T>, dim3(p1 - p0), dim3(256), 0, st, a, dp.pan_task + p0);
+      if (p1 > p0) hipLaunchKernelGGL(k_panel_step<T>, dim3(p1 - p0), dim3(256), 0, st, a, dp.pan_task + p0, false);
       const int u0 = dp.h_upd_aug_ptr[q], u1 = dp.h_upd_ptr[q + 1];
       if (u1 > u0) hipLaunchKernelGGL(k_update_tiles<T>, dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
     }

@@ -40,11 +40,10 @@ struct DeviceFactorPlan {
   const Task4* asm_task = nullptr;
   const Task4* pan_task = nullptr;
   const Task4* upd_task = nullptr;
-  const Task4* diag_task = nullptr;
   const int32_t* asm_child_ptr = nullptr;
   const int32_t* asm_child = nullptr;
   const int64_t* val_dst = nullptr;
-  std::vector<int32_t> h_small_ptr, h_big_ptr, h_asm_ptr, h_level_step_ptr, h_pan_ptr, h_upd_ptr, h_diag_ptr;
+  std::vector<int32_t> h_small_ptr, h_big_ptr, h_asm_ptr, h_level_step_ptr, h_pan_ptr, h_upd_ptr;
   std::vector<int32_t> h_pan_aug_ptr, h_upd_aug_ptr;
 };
 
@@ -85,13 +84,14 @@ struct DeviceSolvePlan {
   const int32_t* gather_task = nullptr;   // (s, row block) pairs, grouped by level
   const int32_t* gather_child_ptr = nullptr;  // per gather task + 1
   const int32_t* gather_child = nullptr;      // (child, q0, q1) triples
-  const int32_t* xc_task = nullptr;       // (s, column block) pairs
+  const int32_t* xc_task = nullptr;       // (s, column block, row chunk) triples
   const int32_t* u_task = nullptr;        // (s, R row block) pairs
   const int32_t* bwd_task = nullptr;      // (s, row block) pairs
   const int32_t* rpart_task = nullptr;    // (s, column block, row chunk) triples
   const int64_t* part_off = nullptr;      // per supernode
   void* part = nullptr;                   // partial sums of the solve precision
-  void* xf = nullptr;                     // n entries: forward result of big fronts
+  const int64_t* xfp_off = nullptr;       // per supernode
+  void* xfp = nullptr;                    // forward partial sums of big fronts
   std::vector<int32_t> h_small_ptr, h_gather_ptr, h_xc_ptr, h_u_ptr, h_bwd_ptr, h_rpart_ptr;
 };
 

@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF> a, const int32_t
 
 // ---------------------------------------------------------------- big fronts
 constexpr int kRB = kSolveRB;
-constexpr int kChunk = kRpartRows;  // vector entries staged in LDS at a time
+constexpr int kChunk = 2048;  // vector entries staged in LDS at a time (row-block GEMV)
 
 // Forward gather, 64-row block i of front s: v = [b on columns; 0 on R] +
 // the children's update vectors, children in fixed order. The column part
@@ -118,15 +118,57 @@ __global__ void __launch_bounds__(64) k_fwd_gather(const int32_t* __restrict__ f
   else if (r < m) uvec[rows_off[s] + (r - nc)] = vsh[lane];
 }
 
-// Forward, column block j of front s: xf[col] = sum_{i <= col} Z[i, col] v_i
-// over the block's columns (Z is zero below its diagonal, so whole 64-row
-// tiles are summed). Columns of Z are contiguous: each wave walks four
-// columns at a time down the rows, v staged in LDS.
+// Dot products of the bw (<= 64) columns base + t * ld, rows [0, len), with
+// vs (LDS): out[t] = sum_q base[q + t * ld] vs[q]. Wave w takes columns
+// 4w .. 4w + 3, then 16 further, lanes stride the rows (coalesced); four
+// row strides are loaded before use so each wave keeps 16 loads in flight.
+// Partial sums are combined across lanes in a fixed order.
+template <class TF, class TX>
+__device__ __forceinline__ void column_dots(const TF* __restrict__ base, int64_t ld, int bw, int len,
+                                            const TX* __restrict__ vs, TX* __restrict__ out) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  for (int t0 = w * 4; t0 < bw; t0 += 16) {
+    const TF* col[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) col[u] = base + min(t0 + u, bw - 1) * ld;
+    TX acc[4] = {TX(0), TX(0), TX(0), TX(0)};
+    int q = lane;
+    for (; q + 192 < len; q += 256) {
+      TF lv[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) lv[r][u] = col[u][q + 64 * r];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const TX v = vs[q + 64 * r];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] += static_cast<TX>(lv[r][u]) * v;
+      }
+    }
+    for (; q < len; q += 64) {
+      const TX v = vs[q];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] += static_cast<TX>(col[u][q]) * v;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      TX v = acc[u];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0 && t0 + u < bw) out[t0 + u] = v;
+    }
+  }
+}
+
+// Forward, rows [h * kXcRows, ...) of column block j of front s: partial sums
+// of xf[col] = sum_{i <= col} Z[i, col] v_i over the block's columns (Z is
+// zero below its diagonal, so whole 64-row tiles are summed).
 template <class TF, class TX>
 __global__ void __launch_bounds__(256) k_fwd_xc(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
-                                                const TX* __restrict__ x, TX* __restrict__ xf) {
-  __shared__ TX vs[kChunk];
-  const int s = tasks[2 * blockIdx.x], j = tasks[2 * blockIdx.x + 1];
+                                                const int64_t* __restrict__ xfp_off, const TX* __restrict__ x,
+                                                TX* __restrict__ xfp) {
+  __shared__ TX vs[kXcRows];
+  const int s = tasks[3 * blockIdx.x], j = tasks[3 * blockIdx.x + 1], h = tasks[3 * blockIdx.x + 2];
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
   const int nr = a.nrow[s];
@@ -134,82 +176,53 @@ __global__ void __launch_bounds__(256) k_fwd_xc(SolveArgs<TF> a, const int32_t* 
   const int64_t ld = panel_ld(nc, nr);
   const int c0 = j * kRB;
   const int bw = min(kRB, nc - c0);
-  const int rend = c0 + bw;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const TF* Z = a.panel + a.panel_off[s] + m;
-  TX acc[4][4];
-#pragma unroll
-  for (int g = 0; g < 4; ++g)
-#pragma unroll
-    for (int u = 0; u < 4; ++u) acc[g][u] = TX(0);
-  for (int q0 = 0; q0 < rend; q0 += kChunk) {
-    const int len = min(kChunk, rend - q0);
-    __syncthreads();
-    for (int q = tid; q < len; q += blockDim.x) vs[q] = x[f + q0 + q];
-    __syncthreads();
-#pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int t0 = g * 16 + w * 4;
-      if (t0 >= bw) break;
-      const TF* col[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) col[u] = Z + q0 + (c0 + min(t0 + u, bw - 1)) * ld;
-      for (int q = lane; q < len; q += 64) {
-        const TX v = vs[q];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc[g][u] += static_cast<TX>(col[u][q]) * v;
-      }
-    }
-  }
-#pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const int t0 = g * 16 + w * 4;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      TX v = acc[g][u];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0 && t0 + u < bw) xf[f + c0 + t0 + u] = v;
-    }
-  }
+  const int q0 = h * kXcRows;
+  const int len = min(kXcRows, c0 + bw - q0);
+  const int nhx = (nc + kXcRows - 1) / kXcRows;
+  for (int q = threadIdx.x; q < len; q += blockDim.x) vs[q] = x[f + q0 + q];
+  __syncthreads();
+  const TF* base = a.panel + a.panel_off[s] + m + q0 + c0 * ld;
+  column_dots<TF, TX>(base, ld, bw, len, vs, xfp + xfp_off[s] + (static_cast<int64_t>(j) * nhx + h) * kRB);
 }
 
-// Row-block GEMV over a panel: out[r] = sum_{c in [cb, ce)} M[r, c] y_c for
-// the 64 rows r0.. of the block, y_c = scale[c] * (yv[c] - sum of nch
-// partial sums) staged in LDS. Each lane owns a row, the four waves split
-// the columns, and the wave results are added in a fixed order.
-template <class TF, class TX>
+// Column c of a big front after the forward pass: the sum of its partials.
+template <class TX>
+__device__ __forceinline__ TX forward_value(const TX* __restrict__ xfp_s, int nc, int c) {
+  const int nhx = (nc + kXcRows - 1) / kXcRows;
+  const int j = c / kRB;
+  const int nv = (min(nc, (j + 1) * kRB) + kXcRows - 1) / kXcRows;
+  const TX* pp = xfp_s + static_cast<int64_t>(j) * nhx * kRB + (c % kRB);
+  TX v = TX(0);
+  for (int h = 0; h < nv; ++h) v += pp[static_cast<int64_t>(h) * kRB];
+  return v;
+}
+
+// Row-block GEMV over a panel: returns sum_{c in [cb, ce)} M[r, c] y_c for the
+// row r = lane (< rows) of the 64-row block at M, with y_c = stage(c) staged
+// in LDS. Each lane owns a row, the four waves split the columns (16 loads
+// in flight each), and the wave results are added in a fixed order.
+template <class TF, class TX, class Stage>
 __device__ __forceinline__ TX rowblock_gemv(const TF* __restrict__ M, int64_t ld, int rows, int cb, int ce,
-                                            const TF* __restrict__ scale, const TX* __restrict__ yv,
-                                            const TX* __restrict__ part, int nch, TX* ys, TX (*red)[kRB]) {
+                                            Stage stage, TX* ys, TX (*red)[kRB]) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int rr = lane < rows ? lane : 0;
   TX acc = TX(0);
   for (int q0 = cb; q0 < ce; q0 += kChunk) {
     const int len = min(kChunk, ce - q0);
     __syncthreads();
-    for (int q = tid; q < len; q += blockDim.x) {
-      const int c = q0 + q;
-      TX z = yv[c];
-      if (part != nullptr) {
-        const TX* pp = part + static_cast<int64_t>(c / kRB) * nch * kRB + (c % kRB);
-        TX ps = TX(0);
-        for (int h = 0; h < nch; ++h) ps += pp[static_cast<int64_t>(h) * kRB];
-        z -= ps;
-      }
-      ys[q] = static_cast<TX>(scale[c]) * z;
-    }
+    for (int q = tid; q < len; q += blockDim.x) ys[q] = stage(q0 + q);
     __syncthreads();
-    // wave w: columns [w * per, (w + 1) * per) of the chunk, eight at a time
+    // wave w: columns [w * per, (w + 1) * per) of the chunk, 16 at a time
     const int per = (len + 3) / 4;
     const int cs = min(len, w * per), cend = min(len, cs + per);
     const TF* mp = M + rr + (q0 + cs) * ld;
     int c = cs;
-    for (; c + 8 <= cend; c += 8, mp += 8 * ld) {
-      TF lv[8];
+    for (; c + 16 <= cend; c += 16, mp += 16 * ld) {
+      TF lv[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) lv[u] = mp[u * ld];
+      for (int u = 0; u < 16; ++u) lv[u] = mp[u * ld];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) acc += static_cast<TX>(lv[u]) * ys[c + u];
+      for (int u = 0; u < 16; ++u) acc += static_cast<TX>(lv[u]) * ys[c + u];
     }
     for (; c < cend; ++c, mp += ld) acc += static_cast<TX>(*mp) * ys[c];
   }
@@ -221,7 +234,8 @@ __device__ __forceinline__ TX rowblock_gemv(const TF* __restrict__ M, int64_t ld
 // Forward, 64-row block b of R_s: u = v_R - L_21 D xf.
 template <class TF, class TX>
 __global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
-                                               const TX* __restrict__ xf, TX* __restrict__ uvec) {
+                                               const int64_t* __restrict__ xfp_off, const TX* __restrict__ xfp,
+                                               TX* __restrict__ uvec) {
   __shared__ TX ys[kChunk];
   __shared__ TX red[4][kRB];
   const int s = tasks[2 * blockIdx.x], b = tasks[2 * blockIdx.x + 1];
@@ -232,7 +246,10 @@ __global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF> a, const int32_t* _
   const int rb0 = b * kRB;
   const int rows = min(kRB, nr - rb0);
   const TF* M = a.panel + a.panel_off[s] + nc + rb0;
-  const TX sum = rowblock_gemv<TF, TX>(M, ld, rows, 0, nc, a.D + f, xf + f, static_cast<const TX*>(nullptr), 0, ys, red);
+  const TX* xs = xfp + xfp_off[s];
+  const TF* D = a.D + f;
+  const TX sum = rowblock_gemv<TF, TX>(M, ld, rows, 0, nc,
+                                       [=](int c) { return static_cast<TX>(D[c]) * forward_value(xs, nc, c); }, ys, red);
   const int lane = threadIdx.x;
   if (lane < rows) {
     TX* u = uvec + a.rows_off[s] + rb0;
@@ -242,8 +259,7 @@ __global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF> a, const int32_t* _
 
 // Backward, rows below the columns of a front (R rows, all final): partial
 // sums part[(j * nch + c) * 64 + t] = sum over the chunk's rows r of
-// L[nc + r, 64 j + t] x[R_r]. Columns are contiguous in the panel, so each
-// wave walks whole columns (coalesced) against x_R staged in LDS.
+// L[nc + r, 64 j + t] x[R_r].
 template <class TF, class TX>
 __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
                                                    const int64_t* __restrict__ part_off, const TX* __restrict__ x,
@@ -257,31 +273,13 @@ __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_
   const int c0 = j * kRB;
   const int bw = min(kRB, nc - c0);
   const int nch = (nr + kRpartRows - 1) / kRpartRows;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const TF* P = a.panel + a.panel_off[s];
   const int32_t* R = a.rows + a.rows_off[s];
   const int q0 = ch * kRpartRows, len = min(kRpartRows, nr - q0);
-  for (int q = tid; q < len; q += blockDim.x) xs[q] = x[R[q0 + q]];
+  for (int q = threadIdx.x; q < len; q += blockDim.x) xs[q] = x[R[q0 + q]];
   __syncthreads();
-  TX* out = part + part_off[s] + static_cast<int64_t>(j) * nch * kRB + static_cast<int64_t>(ch) * kRB;
-  // wave w: columns w * 4, w * 4 + 16, ... of the block, four at a time
-  for (int t0 = w * 4; t0 < bw; t0 += 16) {
-    TX acc[4] = {TX(0), TX(0), TX(0), TX(0)};
-    const TF* col[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) col[u] = P + (nc + q0) + (c0 + min(t0 + u, bw - 1)) * ld;
-    for (int q = lane; q < len; q += 64) {
-      const TX xv = xs[q];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] += static_cast<TX>(col[u][q]) * xv;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      TX v = acc[u];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0 && t0 + u < bw) out[t0 + u] = v;
-    }
-  }
+  const TF* base = a.panel + a.panel_off[s] + nc + q0 + c0 * ld;
+  column_dots<TF, TX>(base, ld, bw, len, xs,
+                      part + part_off[s] + static_cast<int64_t>(j) * nch * kRB + static_cast<int64_t>(ch) * kRB);
 }
 
 // Backward, 64-row block i of the columns of front s:
@@ -289,7 +287,8 @@ __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_
 template <class TF, class TX>
 __global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
                                                 const int64_t* __restrict__ part_off, const TX* __restrict__ part,
-                                                const TX* __restrict__ xf, TX* __restrict__ x) {
+                                                const int64_t* __restrict__ xfp_off, const TX* __restrict__ xfp,
+                                                TX* __restrict__ x) {
   __shared__ TX ys[kChunk];
   __shared__ TX red[4][kRB];
   const int s = tasks[2 * blockIdx.x], i = tasks[2 * blockIdx.x + 1];
@@ -302,10 +301,19 @@ __global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF> a, const int32_t* 
   const int rows = min(kRB, nc - r0);
   const int nch = (nr + kRpartRows - 1) / kRpartRows;
   const TF* M = a.panel + a.panel_off[s] + m + r0;
-  // the partial sums are laid out by column block: pass them shifted so that
-  // column c (absolute within the front) finds block c / 64
-  const TX* pp = nch > 0 ? part + part_off[s] : static_cast<const TX*>(nullptr);
-  const TX sum = rowblock_gemv<TF, TX>(M, ld, rows, r0, nc, a.D + f, xf + f, pp, nch, ys, red);
+  const TX* xs = xfp + xfp_off[s];
+  const TX* ps = part + (nch > 0 ? part_off[s] : 0);
+  const TF* D = a.D + f;
+  const TX sum = rowblock_gemv<TF, TX>(
+      M, ld, rows, r0, nc,
+      [=](int c) {
+        TX z = forward_value(xs, nc, c);
+        const TX* pp = ps + static_cast<int64_t>(c / kRB) * nch * kRB + (c % kRB);
+        TX rs = TX(0);
+        for (int h = 0; h < nch; ++h) rs += pp[static_cast<int64_t>(h) * kRB];
+        return static_cast<TX>(D[c]) * (z - rs);
+      },
+      ys, red);
   const int lane = threadIdx.x;
   if (lane < rows) x[f + r0 + lane] = sum;
 }
@@ -314,7 +322,7 @@ __global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF> a, const int32_t* 
 
 template <class TF, class TX>
 void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st) {
-  TX* xf = reinterpret_cast<TX*>(dp.xf);
+  TX* xfp = reinterpret_cast<TX*>(dp.xfp);
   TX* part = reinterpret_cast<TX*>(dp.part);
   // forward, leaves to root; D^{-1} fused
   for (int l = 0; l < dp.nlevels; ++l) {
@@ -327,10 +335,12 @@ void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uve
                          dp.gather_task + 2 * g0, dp.gather_child_ptr + g0, dp.gather_child, x, uvec);
     const int c0 = dp.h_xc_ptr[l], c1 = dp.h_xc_ptr[l + 1];
     if (c1 > c0)
-      hipLaunchKernelGGL((k_fwd_xc<TF, TX>), dim3(c1 - c0), dim3(256), 0, st, a, dp.xc_task + 2 * c0, x, xf);
+      hipLaunchKernelGGL((k_fwd_xc<TF, TX>), dim3(c1 - c0), dim3(256), 0, st, a, dp.xc_task + 3 * c0, dp.xfp_off, x,
+                         xfp);
     const int u0 = dp.h_u_ptr[l], u1 = dp.h_u_ptr[l + 1];
     if (u1 > u0)
-      hipLaunchKernelGGL((k_fwd_u<TF, TX>), dim3(u1 - u0), dim3(256), 0, st, a, dp.u_task + 2 * u0, xf, uvec);
+      hipLaunchKernelGGL((k_fwd_u<TF, TX>), dim3(u1 - u0), dim3(256), 0, st, a, dp.u_task + 2 * u0, dp.xfp_off, xfp,
+                         uvec);
   }
   // backward, root to leaves
   for (int l = dp.nlevels - 1; l >= 0; --l) {
@@ -341,7 +351,7 @@ void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uve
     const int t0 = dp.h_bwd_ptr[l], t1 = dp.h_bwd_ptr[l + 1];
     if (t1 > t0)
       hipLaunchKernelGGL((k_bwd_xc<TF, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.bwd_task + 2 * t0, dp.part_off,
-                         part, xf, x);
+                         part, dp.xfp_off, xfp, x);
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0) hipLaunchKernelGGL((k_bwd_small<TF, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x);
   }

@@ -109,7 +109,6 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
   P.upd_ptr.assign(1, 0);
   P.pan_aug_ptr.clear();
   P.upd_aug_ptr.clear();
-  P.diag_ptr.assign(1, 0);
   for (int32_t l = 0; l < S.nlevels; ++l) {
     int32_t max_steps = 0;
     for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
@@ -151,15 +150,15 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
           const int32_t nbk = std::min(kPanelNB, nc - k0);
           const int32_t t0 = k0 + nbk;
           if (pass == 0) {
-            // rows strictly below the diagonal block, in chunks of 256
-            const int32_t pan0 = static_cast<int32_t>(P.pan_task.size());
-            for (int32_t r0 = t0; r0 < m; r0 += 256) P.pan_task.push_back({s, r0, std::min(m, r0 + 256), k});
-            P.diag_task.push_back({s, pan0, static_cast<int32_t>(P.pan_task.size()), k});
+            // the diagonal block's own task, then rows below it in chunks of 64
+            P.pan_task.push_back({s, k0, t0, k});
+            for (int32_t r0 = t0; r0 < m; r0 += kPanelRows) P.pan_task.push_back({s, r0, std::min(m, r0 + kPanelRows), k});
             for (int32_t c0 = t0; c0 < m; c0 += kTile)
               for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, k});
           } else {
             // aug rows i < t0 are the only ones nonzero in this step's columns
-            for (int32_t r0 = 0; r0 < t0; r0 += 256) P.pan_task.push_back({s, m + r0, m + std::min(t0, r0 + 256), k});
+            for (int32_t r0 = 0; r0 < t0; r0 += kPanelRows)
+              P.pan_task.push_back({s, m + r0, m + std::min(t0, r0 + kPanelRows), k});
             for (int32_t c0 = t0; c0 < nc; c0 += kTile)
               for (int32_t r0 = 0; r0 < t0; r0 += kTile) P.upd_task.push_back({s, m + r0, c0, k});
           }
@@ -167,8 +166,6 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
       }
       P.pan_ptr.push_back(static_cast<int32_t>(P.pan_task.size()));
       P.upd_ptr.push_back(static_cast<int32_t>(P.upd_task.size()));
-      P.diag_ptr.push_back(static_cast<int32_t>(P.diag_task.size()));
-      P.max_step_tasks = std::max(P.max_step_tasks, P.pan_ptr.back() - P.pan_ptr[P.pan_ptr.size() - 2]);
     }
     P.level_step_ptr[l + 1] = P.level_step_ptr[l] + max_steps;
   }
@@ -199,6 +196,7 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
   // solve tasks of the big fronts, per level
   P.sfwd_child_ptr.assign(1, 0);
   P.part_off.assign(S.ns, -1);
+  P.xfp_off.assign(S.ns, -1);
   P.rpart_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   P.sfwd_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   P.sxc_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
@@ -223,10 +221,17 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
         }
         P.sfwd_child_ptr.push_back(static_cast<int32_t>(P.sfwd_child.size() / 3));
       }
-      // forward column blocks, longest first; backward row blocks likewise
-      for (int32_t j = ncb - 1; j >= 0; --j) {
-        P.sxc_task.push_back(s);
-        P.sxc_task.push_back(j);
+      // forward column blocks by row chunk; backward row blocks, longest first
+      const int32_t nhx = (nc + kXcRows - 1) / kXcRows;
+      P.xfp_off[s] = P.nxfp;
+      P.nxfp += static_cast<int64_t>(ncb) * nhx * kSolveRB;
+      for (int32_t j = 0; j < ncb; ++j) {
+        const int32_t rend = std::min(nc, (j + 1) * kSolveRB);
+        for (int32_t h = 0; h * kXcRows < rend; ++h) {
+          P.sxc_task.push_back(s);
+          P.sxc_task.push_back(j);
+          P.sxc_task.push_back(h);
+        }
       }
       for (int32_t b = 0; b * kSolveRB < nr; ++b) {
         P.su_task.push_back(s);
@@ -248,7 +253,7 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     }
     P.rpart_ptr[l + 1] = static_cast<int32_t>(P.rpart_task.size() / 3);
     P.sfwd_ptr[l + 1] = static_cast<int32_t>(P.sfwd_task.size() / 2);
-    P.sxc_ptr[l + 1] = static_cast<int32_t>(P.sxc_task.size() / 2);
+    P.sxc_ptr[l + 1] = static_cast<int32_t>(P.sxc_task.size() / 3);
     P.su_ptr[l + 1] = static_cast<int32_t>(P.su_task.size() / 2);
     P.sbwd_ptr[l + 1] = static_cast<int32_t>(P.sbwd_task.size() / 2);
   }

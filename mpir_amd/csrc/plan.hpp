@@ -10,12 +10,14 @@
 
 namespace mpir {
 
-// Column-block width of the blocked factorization of large fronts.
-constexpr int kPanelNB = 64;
+// Rows per panel-step task (at most 64).
+constexpr int kPanelRows = 64;
 // Row / column block of the big-front triangular solves.
 constexpr int kSolveRB = 64;
 // R rows per backward partial-sum task.
-constexpr int kRpartRows = 2048;
+constexpr int kRpartRows = 512;
+// Rows per forward task of a big front's inverse block (partial sums).
+constexpr int kXcRows = 512;
 // Tile edge of the trailing-update tasks.
 constexpr int kTile = 64;
 // Column-block width and row range of the extend-add assembly tasks.
@@ -44,23 +46,24 @@ struct FactorPlan {
   std::vector<int32_t> asm_child_ptr;  // per task + 1
   std::vector<int32_t> asm_child;      // 5-tuples
   // per (level, step): panel tasks (s, row0, row1, step) and update tiles
-  // (s, tile_row0, tile_col0, step); indexed through step_ptr
+  // (s, tile_row0, tile_col0, step); indexed through step_ptr. A front's
+  // first panel task of a step has row0 = the step's first column: it owns
+  // the diagonal block (writes L_kk, D, D^{-1}); the others cover at most
+  // kPanelRows rows below it (or identity rows m ..)
   std::vector<int32_t> level_step_ptr;  // nlevels+1 into the step arrays
   std::vector<int32_t> pan_ptr;          // nsteps_total+1
   std::vector<Task4> pan_task;
   std::vector<int32_t> upd_ptr;          // nsteps_total+1
   std::vector<Task4> upd_task;
-  // per step: the big fronts active in it, (s, pan0, pan1, step) where
-  // [pan0, pan1) are that front's entries of pan_task
-  std::vector<int32_t> diag_ptr;         // nsteps_total+1
-  std::vector<Task4> diag_task;
-  int32_t max_step_tasks = 0;            // max pan tasks in one step
   // per step, the first panel / update task on the identity rows below R_s
   // (the inverse of the diagonal block); those tasks end the step's range
   std::vector<int32_t> pan_aug_ptr, upd_aug_ptr;  // nsteps_total
   // triangular solves of big fronts, per level (pairs (s, block)):
   //   sfwd: 64-row blocks of the front, gathering b and the children's
-  //         update vectors; sxc: column blocks, x_c = (L_11^{-T} D^{-1})^T v_c;
+  //         update vectors; sxc: triples (s, column block, row chunk) of
+  //         x_c = (L_11^{-T} D^{-1})^T v_c, partial sums over kXcRows rows at
+  //         xfp_off[s] + (block * nchunk + chunk) * kSolveRB,
+  //         nchunk = ceil(nc / kXcRows);
   //   su:   64-row blocks of R_s, u = v_R - L_21 D x_c;
   //   sbwd: row blocks, x_c = L_11^{-T} (x_c - L_21^T x_R)
   std::vector<int32_t> sfwd_ptr, sxc_ptr, su_ptr, sbwd_ptr;  // nlevels+1
@@ -76,6 +79,8 @@ struct FactorPlan {
   std::vector<int32_t> rpart_task;            // triples
   std::vector<int64_t> part_off;              // per supernode
   int64_t npart = 0;
+  std::vector<int64_t> xfp_off;               // per supernode
+  int64_t nxfp = 0;
   // for each entry of the ORIGINAL upper CSC: destination in the panel pool
   std::vector<int64_t> val_dst;
   // statistics

@@ -120,10 +120,12 @@ def test_gmres_ir_matches_oracle_outcome(gpu, oracle, cfg, k, uf, order):
     np.testing.assert_allclose(x, xo, rtol=0, atol=1e-8 * np.abs(xo).max())
 
 
-def test_gmres_ir_factor_pattern_matches_oracle(gpu, oracle):
+@pytest.mark.parametrize("k,order", [(10, 1), (24, 2)])
+def test_gmres_ir_factor_pattern_matches_oracle(gpu, oracle, k, order):
     # the GPU factor of P K_s P^T has exactly the oracle's pattern and inertia
-    P = mpir_amd.kkt_config(1, k=10)
-    s = mpir_amd.GmresLDLIR(np.float64)
+    # (k = 24 with nested dissection: multi-block fronts of several hundred rows)
+    P = mpir_amd.kkt_config(1, k=k)
+    s = mpir_amd.GmresLDLIR(np.float64, ordering=order)
     s.Compute(P.Ap, P.Ai, P.Ax)
     Lp, Li, Lx, D, Dinv = s.export_factor()
     perm = s.perm()
