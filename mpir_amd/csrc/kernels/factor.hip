@@ -59,9 +59,15 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
   T* P = a.panel + a.panel_off[s];
   T* U = nr > 0 ? a.upool + a.upd_off[s] : nullptr;
 
-  // load the panel columns, zero the update block
-  for (int c = 0; c < m; ++c) {
-    if (lane < m) F[lane][c] = (c < nc) ? P[lane + static_cast<int64_t>(c) * m] : T(0);
+  // load the panel columns, zero the update block (loads unconditional from
+  // clamped addresses so that they are all in flight together)
+  {
+    const int ls = min(lane, m - 1);
+#pragma unroll 16
+    for (int c = 0; c < kSmallFront; ++c) {
+      const T v = P[ls + static_cast<int64_t>(min(c, nc - 1)) * m];
+      F[lane][c] = (lane < m && c < nc) ? v : T(0);
+    }
   }
   __syncthreads();
   // extend-add the children's update matrices, one child at a time
@@ -71,10 +77,22 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
     if (nrc == 0) continue;
     const T* Uc = a.upool + a.upd_off[c];
     const int32_t* rel = a.relidx + a.rows_off[c];
-    const int ra = lane < nrc ? rel[lane] : 0;
-    for (int b = 0; b < nrc; ++b) {
-      const int rb = rel[b];
-      if (lane >= b && lane < nrc) F[ra][rb] += Uc[lane + static_cast<int64_t>(b) * nrc];
+    const int ls = min(lane, nrc - 1);
+    const int ra = rel[ls];
+    for (int b0 = 0; b0 < nrc; b0 += 8) {
+      T uv[8];
+      int rb[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int b = min(b0 + q, nrc - 1);
+        rb[q] = rel[b];
+        uv[q] = Uc[ls + static_cast<int64_t>(b) * nrc];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int b = b0 + q;
+        if (b < nrc && lane >= b && lane < nrc) F[ra][rb[q]] += uv[q];
+      }
     }
     __syncthreads();
   }
@@ -90,11 +108,12 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
       for (int c = j + 1; c <= lane; ++c) F[lane][c] -= y * lb[c];
       F[lane][j] = lb[lane];
     }
-    if (lane == 0) {
-      a.D[f + j] = d;
-      a.Dinv[f + j] = dinv;
-    }
     __syncthreads();
+  }
+  if (lane < nc) {
+    const T d = F[lane][lane];
+    a.D[f + lane] = d;
+    a.Dinv[f + lane] = T(1) / d;
   }
   for (int c = 0; c < nc; ++c)
     if (lane < m && lane >= c) P[lane + static_cast<int64_t>(c) * m] = F[lane][c];
@@ -156,94 +175,126 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
 // (identity rows of the front, r >= m, enter at their own column as e_i).
 // refactor = false: L_kk and D are already final (a factor loaded from
 // outside); the block is rebuilt from them and only Z is formed.
+constexpr int kPanelThreads = 1024;  // 16 waves, each owning 4 block columns
+
 template <class T>
-__global__ void __launch_bounds__(256) k_panel_step(FactorArgs<T> a, const Task4* __restrict__ tasks, bool refactor) {
-  __shared__ T F[kPanelNB][kPanelNB + 1];  // diagonal block, then the task's rows
-  __shared__ T G[kPanelNB][kPanelNB + 1];  // identity rows -> Z
-  __shared__ T lc[kPanelNB], lg[kPanelNB];
+__global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, const Task4* __restrict__ tasks,
+                                                              bool refactor) {
+  constexpr int NB = kPanelNB;
+  constexpr int CW = NB / (kPanelThreads / 64);  // columns per wave (4)
+  __shared__ __align__(16) T col[2][NB];   // column j of the block, double-buffered
+  __shared__ __align__(16) T gcol[2][NB];  // column j of the identity rows
+  __shared__ __align__(16) T Zs[NB][NB];   // Z, row-major
+  __shared__ __align__(16) T Xs[NB][NB];   // the task's rows, transposed: Xs[t][row]
   const Task4 tk = tasks[blockIdx.x];
   const int s = tk.a, r0 = tk.b, r1 = tk.c, k = tk.d;
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
   const int m = nc + a.nrow[s];
   const int64_t ld = panel_ld(nc, a.nrow[s]);
-  const int k0 = k * kPanelNB;
-  const int nbk = min(kPanelNB, nc - k0);
+  const int k0 = k * NB;
+  const int nbk = min(NB, nc - k0);
   T* P = a.panel + a.panel_off[s];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c0w = CW * w;  // this thread's columns: c0w .. c0w + CW - 1
   const bool owner = r0 == k0;
   const int rows = r1 - r0;
-  // this thread's part of its row (columns 16w .. 16w + 15), loaded early
+  // Loads are unconditional from clamped addresses, then selected: a load
+  // under a branch is waited for before the branch joins.
   const int r = r0 + lane;
   const int unit = r >= m + k0 ? r - m - k0 : -1;
-  T xv[16];
+  const bool live = !owner && lane < rows;
+  const int64_t rsafe = (live && unit < 0) ? r : k0;
+  T xv[CW];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int c = 16 * w + u;
-    xv[u] = (owner || lane >= rows || c >= nbk) ? T(0)
-            : unit >= 0                         ? (c == unit ? T(1) : T(0))
-                                                : P[r + (k0 + c) * ld];
+  for (int u = 0; u < CW; ++u) {
+    const int c = c0w + u;
+    const T v = P[rsafe + (k0 + min(c, nbk - 1)) * ld];
+    xv[u] = (!live || c >= nbk) ? T(0) : unit >= 0 ? (c == unit ? T(1) : T(0)) : v;
   }
+  // entries (lane, c0w + u) of the block's lower triangle (fa) and of the
+  // identity rows (ga), in registers through the elimination
   const T* Dk = a.D + f + k0;
-  for (int c = w; c < kPanelNB; c += 4) {
-    T v = T(0);
-    if (lane < nbk && c < nbk && lane >= c) {
-      if (refactor) v = P[(k0 + lane) + (k0 + c) * ld];
-      else v = lane == c ? Dk[c] : P[(k0 + c) + (k0 + lane) * ld] * Dk[c];
-    }
-    F[lane][c] = v;
-    G[lane][c] = lane == c ? T(1) : T(0);
-  }
-  __syncthreads();
-  for (int j = 0; j < nbk; ++j) {
-    const T d = F[j][j];
-    const T dinv = T(1) / d;
-    if (w == 0) {
-      if (lane > j && lane < nbk) {
-        const T l = F[lane][j] * dinv;
-        lc[lane] = l;
-        // L_kk goes to the block's upper triangle (transposed): the other
-        // workgroups of this launch are still reading the lower triangle
-        if (owner && refactor) P[(k0 + j) + (k0 + lane) * ld] = l;
-      }
-      if (lane == j && owner && refactor) {
-        if (d == T(0)) flag_zero_pivot(a.err);
-        a.D[f + k0 + j] = d;
-        a.Dinv[f + k0 + j] = dinv;
-      }
-    } else if (w == 1) {
-      if (lane <= j) lg[lane] = G[lane][j] * dinv;
-    }
-    __syncthreads();
-    // trailing block rows (lower part) and active identity rows i <= j
-    if (refactor && lane > j && lane < nbk) {
-      const T l = lc[lane];
-      for (int c = j + 1 + w; c <= lane; c += 4) F[lane][c] -= l * F[c][j];
-    }
-    if (lane <= j) {
-      const T g = lg[lane];
-      for (int c = j + 1 + w; c < nbk; c += 4) G[lane][c] -= g * F[c][j];
-      if (w == 0) G[lane][j] = g;
-    }
-    __syncthreads();
-  }
-  if (owner) return;
-  // rows of L: X Z, thread (row lane, columns 16w ..)
+  const int lsafe = min(lane, nbk - 1);
+  T fa[CW], ga[CW];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) F[lane][16 * w + u] = xv[u];
+  for (int u = 0; u < CW; ++u) {
+    const int c = c0w + u;
+    const int cs = min(c, nbk - 1);
+    const T v = refactor ? P[(k0 + lsafe) + (k0 + cs) * ld] : P[(k0 + cs) + (k0 + lsafe) * ld];
+    const T dc = refactor ? T(1) : Dk[cs];
+    const bool in = lane < nbk && c < nbk && lane >= c;
+    fa[u] = !in ? T(0) : refactor ? v : lane == c ? dc : v * dc;
+    ga[u] = (lane == c && c < nbk) ? T(1) : T(0);
+  }
+  for (int j = 0; j < nbk; ++j) {
+    const int b = j & 1;
+    if (w == j / CW) {
+      const int uj = j % CW;
+      T fv = fa[0], gv = ga[0];
+#pragma unroll
+      for (int u = 1; u < CW; ++u) {
+        fv = u == uj ? fa[u] : fv;
+        gv = u == uj ? ga[u] : gv;
+      }
+      col[b][lane] = fv;
+      gcol[b][lane] = gv;
+    }
+    __syncthreads();
+    T cv[CW];
+#pragma unroll
+    for (int u = 0; u < CW; ++u) cv[u] = col[b][c0w + u];
+    const T d = col[b][j];
+    const T dinv = T(1) / d;
+    const T lr = lane > j ? col[b][lane] * dinv : T(0);
+    const T gi = lane <= j ? gcol[b][lane] * dinv : T(0);
+    // the owner keeps column j of L_kk (and D on the diagonal) in LDS; its
+    // stores are issued after the loop (a barrier waits for pending stores)
+    if (owner && w == 0) Xs[j][lane] = lane > j ? lr : lane == j ? d : T(0);
+#pragma unroll
+    for (int u = 0; u < CW; ++u) {
+      const int c = c0w + u;
+      const bool upd = c > j && c < nbk;
+      const T fnew = fa[u] - lr * cv[u];
+      const T gnew = ga[u] - gi * cv[u];
+      fa[u] = (refactor && upd && lane >= c) ? fnew : fa[u];
+      ga[u] = upd ? gnew : c == j ? gi : ga[u];
+    }
+  }
+  if (owner) {
+    if (!refactor) return;
+    __syncthreads();
+    // L_kk goes to the block's upper triangle, transposed (entry (j, r) holds
+    // L[r, j]): the other workgroups of this launch read the lower triangle
+    for (int rr = w; rr < nbk; rr += kPanelThreads / 64)
+      if (lane < rr) P[(k0 + lane) + (k0 + rr) * ld] = Xs[lane][rr];
+    if (tid < nbk) {
+      const T d = Xs[tid][tid];
+      if (d == T(0)) flag_zero_pivot(a.err);
+      a.D[f + k0 + tid] = d;
+      a.Dinv[f + k0 + tid] = T(1) / d;
+    }
+    return;
+  }
+  // rows of L: X Z, thread (row lane, columns c0w ..)
+#pragma unroll
+  for (int u = 0; u < CW; ++u) {
+    Zs[lane][c0w + u] = ga[u];
+    Xs[c0w + u][lane] = xv[u];
+  }
   __syncthreads();
   if (lane >= rows) return;
-  T acc[16];
+  T acc[CW];
 #pragma unroll
-  for (int u = 0; u < 16; ++u) acc[u] = T(0);
+  for (int u = 0; u < CW; ++u) acc[u] = T(0);
   for (int t = 0; t < nbk; ++t) {
-    const T xt = F[lane][t];
+    const T xt = Xs[t][lane];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) acc[u] += xt * G[t][16 * w + u];
+    for (int u = 0; u < CW; ++u) acc[u] += xt * Zs[t][c0w + u];
   }
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int c = 16 * w + u;
+  for (int u = 0; u < CW; ++u) {
+    const int c = c0w + u;
     if (c < nbk) P[r + (k0 + c) * ld] = acc[u];
   }
 }
@@ -270,11 +321,26 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
   T* Pw = a.panel + a.panel_off[s];
   T* U = nr > 0 ? a.upool + a.upd_off[s] : nullptr;
   const int tid = threadIdx.x;
-  for (int e = tid; e < nbk * kTile; e += blockDim.x) {
-    const int i = e % kTile, t = e / kTile;
+  {
+    // thread (row i = tid % 64, columns t = tid / 64 + 4q): all loads issued
+    // before any is used (clamped addresses, values selected afterwards)
+    const int i = tid & 63, t0 = tid >> 6;
     const int ra = r0 + i, rb = c0 + i;
-    As[t][i] = ra < rlim ? P[ra + (k0 + t) * ld] : T(0);
-    Bs[t][i] = rb < m ? P[rb + (k0 + t) * ld] * a.D[f + k0 + t] : T(0);
+    const int64_t ras = ra < rlim ? ra : r0, rbs = rb < m ? rb : c0;
+    T av[16], bv[16], dv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int t = min(t0 + 4 * q, nbk - 1);
+      av[q] = P[ras + (k0 + t) * ld];
+      bv[q] = P[rbs + (k0 + t) * ld];
+      dv[q] = a.D[f + k0 + t];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int t = t0 + 4 * q;
+      As[t][i] = (ra < rlim && t < nbk) ? av[q] : T(0);
+      Bs[t][i] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
+    }
   }
   __syncthreads();
   const int ti = (tid % 16) * 4, tj = (tid / 16) * 4;
@@ -292,18 +358,26 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
   }
+  // read-modify-write of the target: every load first, then the stores
+  T* dst[4][4];
+  T cur[4][4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int c = c0 + tj + j;
-    if (c >= m) continue;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int r = r0 + ti + i;
-      if (r >= rlim || r < c) continue;
-      if (c < nc) Pw[r + c * ld] -= acc[i][j];
-      else if (r < m) U[(r - nc) + static_cast<int64_t>(c - nc) * nr] -= acc[i][j];
+      const bool ok = c < m && r < rlim && r >= c && (c < nc || r < m);
+      dst[i][j] = !ok ? Pw : c < nc ? Pw + r + c * ld : U + (r - nc) + static_cast<int64_t>(c - nc) * nr;
+      cur[i][j] = *dst[i][j];
+      if (!ok) dst[i][j] = nullptr;
     }
   }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (dst[i][j] != nullptr) *dst[i][j] = cur[i][j] - acc[i][j];
 }
 
 template <class T>
@@ -338,7 +412,7 @@ void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const do
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_ptr[q], p1 = dp.h_pan_ptr[q + 1];
-      if (p1 > p0) hipLaunchKernelGGL(k_panel_step<T>, dim3(p1 - p0), dim3(256), 0, st, a, dp.pan_task + p0, true);
+      if (p1 > p0) hipLaunchKernelGGL(k_panel_step<T>, dim3(p1 - p0), dim3(kPanelThreads), 0, st, a, dp.pan_task + p0, true);
       const int u0 = dp.h_upd_ptr[q], u1 = dp.h_upd_ptr[q + 1];
       if (u1 > u0) hipLaunchKernelGGL(k_update_tiles<T>, dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
     }
@@ -352,7 +426,7 @@ void invert_diag_blocks(const FactorArgs<T>& a, const DeviceFactorPlan& dp, hipS
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_aug_ptr[q], p1 = dp.h_pan_ptr[q + 1];
-      if (p1 > p0) hipLaunchKernelGGL(k_panel_step<T>, dim3(p1 - p0), dim3(256), 0, st, a, dp.pan_task + p0, false);
+      if (p1 > p0) hipLaunchKernelGGL(k_panel_step<T>, dim3(p1 - p0), dim3(kPanelThreads), 0, st, a, dp.pan_task + p0, false);
       const int u0 = dp.h_upd_aug_ptr[q], u1 = dp.h_upd_ptr[q + 1];
       if (u1 > u0) hipLaunchKernelGGL(k_update_tiles<T>, dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
     }
