@@ -122,15 +122,17 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
 }
 
 // ---------------------------------------------------------------- assembly
-// Block ownership: the task (s, c0, c1, t0) owns the entries of the front
-// in columns [c0, c1) and rows [t0, t0 + kAsmRows): it zeroes them in the
-// update matrix and adds every child's contribution that lands there,
-// children in fixed order, so no two tasks write the same entry. Child
-// columns (rows) landing in a range of the parent are contiguous because the
-// relative indices increase.
+// Tile (s, c0, c1, t0): columns [c0, c1) and rows [t0, t0 + 64) of front s
+// (lower part). The tile is accumulated in LDS: panel columns start from the
+// scattered matrix entries, update-matrix columns from zero; the children's
+// update matrices are added in fixed order (a barrier between children, no
+// two entries of one child land on the same place), then the tile is
+// written back once.
 template <class T>
 __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* __restrict__ tasks,
                                                   const int32_t* __restrict__ cptr, const int32_t* __restrict__ cl) {
+  constexpr int E = kAsmTile;
+  __shared__ T tile[E][E + 1];  // tile[row - t0][col - c0]
   const Task4 tk = tasks[blockIdx.x];
   const int s = tk.a, c0 = tk.b, c1 = tk.c, t0 = tk.d;
   const int f = a.first[s];
@@ -138,13 +140,20 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
   const int nr = a.nrow[s];
   const int m = nc + nr;
   const int64_t ld = panel_ld(nc, nr);
-  const int t1 = min(m, t0 + kAsmRows);
+  const int t1 = min(m, t0 + E);
   T* P = a.panel + a.panel_off[s];
-  T* U = nr > 0 ? a.upool + a.upd_off[s] : nullptr;
-  const int tid = threadIdx.x;
-  for (int c = c0 < nc ? nc : c0; c < c1; ++c) {
-    const int cb = c - nc;
-    for (int r = max(c, t0) + tid; r < t1; r += blockDim.x) U[(r - nc) + static_cast<int64_t>(cb) * nr] = T(0);
+  T* U = nr > 0 ? a.upool + a.upd_off[s] : P;
+  const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
+  const int r = t0 + i;
+  {
+    T v[E / 4];
+#pragma unroll
+    for (int q = 0; q < E / 4; ++q) {
+      const int c = c0 + w + 4 * q;
+      const bool in = c < c1 && r < t1 && r >= c && c < nc;
+      v[q] = *(in ? P + r + c * ld : P);
+      tile[i][w + 4 * q] = in ? v[q] : T(0);
+    }
   }
   __syncthreads();
   for (int e = cptr[blockIdx.x]; e < cptr[blockIdx.x + 1]; ++e) {
@@ -152,16 +161,30 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
     const int nrc = a.nrow[ch];
     const T* Uc = a.upool + a.upd_off[ch];
     const int32_t* rel = a.relidx + a.rows_off[ch];
-    for (int b = b0; b < b1; ++b) {
-      const int rb = rel[b];
-      const T* src = Uc + static_cast<int64_t>(b) * nrc;
-      for (int q = max(b, a0) + tid; q < a1; q += blockDim.x) {
-        const int ra = rel[q];
-        if (rb < nc) P[ra + rb * ld] += src[q];
-        else U[(ra - nc) + static_cast<int64_t>(rb - nc) * nr] += src[q];
-      }
+    const int na = a1 - a0, npair = (b1 - b0) * na;
+    T v[E * E / 256];
+    int dr[E * E / 256], dc[E * E / 256];
+#pragma unroll
+    for (int k = 0; k < E * E / 256; ++k) {
+      const int p = tid + 256 * k;
+      const int b = b0 + (p < npair ? p / na : 0);
+      const int q = a0 + (p < npair ? p % na : 0);
+      v[k] = Uc[q + static_cast<int64_t>(b) * nrc];
+      dr[k] = (p < npair && q >= b) ? rel[q] - t0 : -1;
+      dc[k] = rel[b] - c0;
     }
+#pragma unroll
+    for (int k = 0; k < E * E / 256; ++k)
+      if (dr[k] >= 0) tile[dr[k]][dc[k]] += v[k];
     __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < E / 4; ++q) {
+    const int c = c0 + w + 4 * q;
+    if (c < c1 && r < t1 && r >= c) {
+      if (c < nc) P[r + c * ld] = tile[i][w + 4 * q];
+      else U[(r - nc) + static_cast<int64_t>(c - nc) * nr] = tile[i][w + 4 * q];
+    }
   }
 }
 
@@ -299,14 +322,42 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
   }
 }
 
+// Matrix-core product for the trailing updates: v_mfma 16x16x4 in the
+// precision of the factor. Lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; output v (0..3) of lane l is D[row(l / 16, v)][l % 16]
+// with row = 4 (l / 16) + v for fp32 and (l / 16) + 4 v for fp64 (measured
+// on gfx950).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <class T>
+struct Mfma16;
+template <>
+struct Mfma16<float> {
+  using V = f32x4;
+  static __device__ __forceinline__ V op(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+template <>
+struct Mfma16<double> {
+  using V = f64x4;
+  static __device__ __forceinline__ V op(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+
 // Trailing update tile: C[r0:r0+64, c0:c0+64] -= L_r D L_c^T over the nbk
 // columns of step k, lower part only. Columns < nc live in the panel,
 // columns >= nc in the update matrix. Tiles with r0 >= m cover identity rows
-// (rows m .. m+nc) and only panel columns.
+// (rows m .. m+nc) and only panel columns. Each wave computes a 32 x 32
+// quarter with 2 x 2 MFMA blocks.
 template <class T>
 __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
-  __shared__ T As[kPanelNB][kTile + 1];
-  __shared__ T Bs[kPanelNB][kTile + 1];
+  using V = typename Mfma16<T>::V;
+  __shared__ T As[kPanelNB][kTile + 1];  // As[t][i] = L[r0 + i, k0 + t]
+  __shared__ T Bs[kPanelNB][kTile + 1];  // Bs[t][j] = L[c0 + j, k0 + t] D[k0 + t]
   const Task4 tk = tasks[blockIdx.x];
   const int s = tk.a, r0 = tk.b, c0 = tk.c, k = tk.d;
   const int f = a.first[s];
@@ -319,65 +370,74 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
   const int nbk = min(kPanelNB, nc - k0);
   const T* P = a.panel + a.panel_off[s];
   T* Pw = a.panel + a.panel_off[s];
-  T* U = nr > 0 ? a.upool + a.upd_off[s] : nullptr;
-  const int tid = threadIdx.x;
+  T* U = nr > 0 ? a.upool + a.upd_off[s] : Pw;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   {
-    // thread (row i = tid % 64, columns t = tid / 64 + 4q): all loads issued
-    // before any is used (clamped addresses, values selected afterwards)
-    const int i = tid & 63, t0 = tid >> 6;
+    // thread (row i = lane, columns t = w + 4q): all loads issued before any
+    // is used (clamped addresses, values selected afterwards)
+    const int i = lane;
     const int ra = r0 + i, rb = c0 + i;
     const int64_t ras = ra < rlim ? ra : r0, rbs = rb < m ? rb : c0;
     T av[16], bv[16], dv[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int t = min(t0 + 4 * q, nbk - 1);
+      const int t = min(w + 4 * q, nbk - 1);
       av[q] = P[ras + (k0 + t) * ld];
       bv[q] = P[rbs + (k0 + t) * ld];
       dv[q] = a.D[f + k0 + t];
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int t = t0 + 4 * q;
+      const int t = w + 4 * q;
       As[t][i] = (ra < rlim && t < nbk) ? av[q] : T(0);
       Bs[t][i] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
     }
   }
   __syncthreads();
-  const int ti = (tid % 16) * 4, tj = (tid / 16) * 4;
-  T acc[4][4];
+  const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  V acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
-  for (int t = 0; t < nbk; ++t) {
-    T av[4], bv[4];
+    for (int y = 0; y < 2; ++y)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) { av[i] = As[t][ti + i]; bv[i] = Bs[t][tj + i]; }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+      for (int v = 0; v < 4; ++v) acc[x][y][v] = T(0);
+  for (int kk = 0; kk < nbk; kk += 4) {
+    const int t = kk + lk;  // rows t >= nbk of As / Bs are zero
+    const T a0 = As[t][wr + li], a1 = As[t][wr + 16 + li];
+    const T b0 = Bs[t][wc + li], b1 = Bs[t][wc + 16 + li];
+    acc[0][0] = Mfma16<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma16<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma16<T>::op(a1, b1, acc[1][1]);
   }
-  // read-modify-write of the target: every load first, then the stores
-  T* dst[4][4];
-  T cur[4][4];
+  // the tile goes through LDS (As is free after this barrier) so that the
+  // read-modify-write of the target is coalesced: element (x, y, v) is row
+  // wr + 16x + row(lk, v), column wc + 16y + li of the tile
+  __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int c = c0 + tj + j;
+  for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = r0 + ti + i;
-      const bool ok = c < m && r < rlim && r >= c && (c < nc || r < m);
-      dst[i][j] = !ok ? Pw : c < nc ? Pw + r + c * ld : U + (r - nc) + static_cast<int64_t>(c - nc) * nr;
-      cur[i][j] = *dst[i][j];
-      if (!ok) dst[i][j] = nullptr;
-    }
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) As[wc + 16 * y + li][wr + 16 * x + Mfma16<T>::row(lk, v)] = acc[x][y][v];
+  __syncthreads();
+  // thread (row lane, columns w + 4q): every load first, then the stores
+  const int r = r0 + lane;
+  T* dst[16];
+  T cur[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = c0 + w + 4 * q;
+    const bool ok = c < m && r < rlim && r >= c && (c < nc || r < m);
+    T* p = !ok ? Pw : c < nc ? Pw + r + c * ld : U + (r - nc) + static_cast<int64_t>(c - nc) * nr;
+    cur[q] = *p;
+    dst[q] = ok ? p : nullptr;
   }
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-      if (dst[i][j] != nullptr) *dst[i][j] = cur[i][j] - acc[i][j];
+  for (int q = 0; q < 16; ++q)
+    if (dst[q] != nullptr) *dst[q] = cur[q] - As[w + 4 * q][lane];
 }
 
 template <class T>

@@ -121,13 +121,11 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
       }
       P.big_list.push_back(s);
       max_steps = std::max(max_steps, (nc + kPanelNB - 1) / kPanelNB);
-      const bool has_children = P.child_ptr[s + 1] > P.child_ptr[s];
-      for (int32_t c0 = 0; c0 < m; c0 += kAsmCols) {
-        const int32_t c1 = std::min(m, c0 + kAsmCols);
-        // blocks made only of panel columns need work only with children
-        if (c1 <= nc && !has_children) continue;
-        // rows at or below the block's first column (lower part only)
-        for (int32_t t0 = (c0 / kAsmRows) * kAsmRows; t0 < m; t0 += kAsmRows) P.asm_task.push_back({s, c0, c1, t0});
+      // 64 x 64 tiles of the lower part; tiles made only of panel columns
+      // are kept below only if a child's update lands in them
+      for (int32_t c0 = 0; c0 < m; c0 += kAsmTile) {
+        const int32_t c1 = std::min(m, c0 + kAsmTile);
+        for (int32_t t0 = c0; t0 < m; t0 += kAsmTile) P.asm_task.push_back({s, c0, c1, t0});
       }
     }
     P.small_ptr[l + 1] = static_cast<int32_t>(P.small_list.size());
@@ -176,21 +174,36 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     const int32_t* rel = S.relidx.data() + S.rows_off[c];
     return static_cast<int32_t>(std::lower_bound(rel, rel + S.nrow[c], v) - rel);
   };
-  // child ranges of every assembly task
-  P.asm_child_ptr.assign(P.asm_task.size() + 1, 0);
-  for (size_t t = 0; t < P.asm_task.size(); ++t) {
-    const Task4 tk = P.asm_task[t];
-    const int32_t s = tk.a, c0 = tk.b, c1 = tk.c, t0 = tk.d;
-    const int32_t m = (S.first[s + 1] - S.first[s]) + S.nrow[s];
-    const int32_t t1 = std::min(m, t0 + kAsmRows);
-    for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
-      const int32_t c = P.child_list[q];
-      if (S.nrow[c] == 0) continue;
-      const int32_t b0 = lower(c, c0), b1 = lower(c, c1), a0 = lower(c, t0), a1 = lower(c, t1);
-      if (b0 >= b1 || a1 <= std::max(b0, a0)) continue;
-      P.asm_child.insert(P.asm_child.end(), {c, b0, b1, a0, a1});
+  // child ranges of every assembly tile; panel-only tiles without any
+  // child contribution are dropped (their entries are final after the
+  // scatter of the matrix values)
+  {
+    std::vector<Task4> kept;
+    std::vector<int32_t> kept_level(static_cast<size_t>(S.nlevels) + 1, 0);
+    P.asm_child_ptr.assign(1, 0);
+    for (int32_t l = 0; l < S.nlevels; ++l) {
+      for (int32_t t = P.asm_ptr[l]; t < P.asm_ptr[l + 1]; ++t) {
+        const Task4 tk = P.asm_task[t];
+        const int32_t s = tk.a, c0 = tk.b, c1 = tk.c, t0 = tk.d;
+        const int32_t nc = S.first[s + 1] - S.first[s];
+        const int32_t m = nc + S.nrow[s];
+        const int32_t t1 = std::min(m, t0 + kAsmTile);
+        const size_t before = P.asm_child.size();
+        for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+          const int32_t c = P.child_list[q];
+          if (S.nrow[c] == 0) continue;
+          const int32_t b0 = lower(c, c0), b1 = lower(c, c1), a0 = lower(c, t0), a1 = lower(c, t1);
+          if (b0 >= b1 || a1 <= std::max(b0, a0)) continue;
+          P.asm_child.insert(P.asm_child.end(), {c, b0, b1, a0, a1});
+        }
+        if (c1 <= nc && P.asm_child.size() == before) continue;
+        kept.push_back(tk);
+        P.asm_child_ptr.push_back(static_cast<int32_t>(P.asm_child.size() / 5));
+      }
+      kept_level[l + 1] = static_cast<int32_t>(kept.size());
     }
-    P.asm_child_ptr[t + 1] = static_cast<int32_t>(P.asm_child.size() / 5);
+    P.asm_task.swap(kept);
+    P.asm_ptr.swap(kept_level);
   }
 
   // solve tasks of the big fronts, per level

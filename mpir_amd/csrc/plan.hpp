@@ -20,9 +20,8 @@ constexpr int kRpartRows = 512;
 constexpr int kXcRows = 512;
 // Tile edge of the trailing-update tasks.
 constexpr int kTile = 64;
-// Column-block width and row range of the extend-add assembly tasks.
-constexpr int kAsmCols = 64;
-constexpr int kAsmRows = 512;
+// Edge of the extend-add assembly tiles.
+constexpr int kAsmTile = 64;
 
 struct Task4 {
   int32_t a, b, c, d;
