@@ -6,8 +6,7 @@
  *
  * Plain C types only. Index arrays are int32 ("_i32") or int64 ("_i64");
  * int64 entry points narrow to int32 internally and fail with
- * MPIR_ERR_INVALID when a value does not fit. FP16/BF16 values travel as
- * their 16-bit patterns (uint16_t).
+ * MPIR_ERR_INVALID when a value does not fit.
  *
  * Every function is synchronous with respect to the host unless its name
  * ends in "_dev" (device pointers, work ordered on the solver's stream and
@@ -91,6 +90,8 @@ int64_t mpir_qdldl_solve_f64_f64_i64(int64_t n, const int64_t* Lp, const int64_t
                                      const double* Dinv, double* x);
 int64_t mpir_qdldl_solve_f32_f64_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const float* Lx,
                                      const float* Dinv, double* x);
+int64_t mpir_qdldl_solve_f32_f32_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const float* Lx,
+                                     const float* Dinv, float* x);
 
 /* ------------------------------------------------------------------ GMRES-IR
  * Replaces GmresLDLIR<UF,UW,UR> (include/MPIR/gmres_ir.hpp:90-315) in its

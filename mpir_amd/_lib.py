@@ -40,6 +40,7 @@ SIGNATURES = {
     "mpir_qdldl_solve_f32_f32_i32": (i32, [i32, P, P, P, P, P]),
     "mpir_qdldl_solve_f64_f64_i64": (i64, [i64, P, P, P, P, P]),
     "mpir_qdldl_solve_f32_f64_i64": (i64, [i64, P, P, P, P, P]),
+    "mpir_qdldl_solve_f32_f32_i64": (i64, [i64, P, P, P, P, P]),
     "mpir_gmres_ldl_create": (C.c_int, [C.c_int, C.c_int, C.c_int, P]),
     "mpir_gmres_ldl_destroy": (None, [P]),
     "mpir_gmres_ldl_set_max_ir": (C.c_int, [P, i64]),

@@ -21,6 +21,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build" / "mpir_hip"
 LIB = PKG / "libmpir_hip.so"
+DROPIN_SRC = ROOT / "tests" / "cpp" / "dropin.cpp"
+DROPIN_BIN = ROOT / "tests" / "cpp" / "dropin_test"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -84,6 +86,9 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     objs = [str(_obj(s)) for s in HOST_SOURCES + HIP_SOURCES]
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
     os.replace(tmp, LIB)
+    # the C++ drop-in test program (a reference-style caller of include/mpir/*.hpp)
+    _run([CXX, "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(DROPIN_SRC), f"-L{PKG}", "-lmpir_hip",
+          "-Wl,-rpath,$ORIGIN/../../mpir_amd", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(DROPIN_BIN)])
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -299,6 +299,7 @@ MPIR_SOLVE_I32(f32_f64, float, double)
 MPIR_SOLVE_I32(f32_f32, float, float)
 MPIR_SOLVE_I64(f64_f64, double, double)
 MPIR_SOLVE_I64(f32_f64, float, double)
+MPIR_SOLVE_I64(f32_f32, float, float)
 
 int mpir_gmres_ldl_create(int uf, int uw, int ur, mpir_gmres_ldl_t* out) {
   return guarded([&] {

@@ -205,8 +205,17 @@ enum Slot { kSlotNorm = 0, kSlotH = 8 };
 }  // namespace
 
 GmresLdl::GmresLdl(int uf, int uw, int ur) : uf_(uf), uw_(uw), ur_(ur) {
+  // Refinable<UF,UW,UR> (fp_concepts.hpp:112-113): eps(UF) >= eps(UW) >= eps(UR)
   MPIR_CHECK(uf == kF64 || uf == kF32, "factor precision must be f64 or f32");
   MPIR_CHECK(uw == kF64 && ur == kF64, "working and residual precision must be f64");
+  // the solver runs only on a gfx950 device; there is no host fallback
+  {
+    int count = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1 || hipGetDeviceProperties(&prop, 0) != hipSuccess ||
+        std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      throw Error(kErrNoDevice, "no gfx950 device: this library runs the solve path on the GPU only");
+  }
   MPIR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   MPIR_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096 * sizeof(double), hipHostMallocDefault));
 }

@@ -1,0 +1,164 @@
+// A reference-style caller of QDLDL_etree / QDLDL_factor / QDLDL_solve and
+// GmresLDLIR<UF,UW,UR>, compiled against include/mpir/*.hpp: the calls are
+// written exactly as a user of yex33/MPIR writes them (qdldl.hpp,
+// gmres_ir.hpp, luir.hpp:79-178 for the QDLDL call sequence).
+//
+//   dropin_test          full run on a gfx950 device (exit 0 on success)
+//   dropin_test --host   no device: etree must work, the GPU entry points
+//                        must fail with MPIR_ERR_NODEVICE (exit 0 if so)
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mpir/gmres_ir.hpp"
+#include "mpir/qdldl.hpp"
+
+namespace {
+
+struct Csc {
+  int n = 0;
+  std::vector<int> p, i;
+  std::vector<double> x;
+};
+
+// small quasi-definite K = [H A^T; A -D]: H = 2D Laplacian + shift, A random
+Csc make_kkt(int k, int m) {
+  std::vector<double> h;
+  const int n1 = k * k, n = n1 + m;
+  std::vector<std::vector<std::pair<int, double>>> cols(n);
+  for (int c = 0; c < n1; ++c) {
+    const int gx = c % k, gy = c / k;
+    if (gy > 0) cols[c].push_back({c - k, -1.0});
+    if (gx > 0) cols[c].push_back({c - 1, -1.0});
+    cols[c].push_back({c, 4.01});
+  }
+  unsigned s = 12345u;
+  auto rnd = [&]() { s = s * 1664525u + 1013904223u; return (s >> 8) / double(1u << 24); };
+  for (int r = 0; r < m; ++r) {
+    const int c = n1 + r;
+    const int a = static_cast<int>(rnd() * n1), b = (a + 1 + static_cast<int>(rnd() * (n1 - 1))) % n1;
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    cols[c].push_back({lo, rnd() - 0.5});
+    cols[c].push_back({hi, rnd() - 0.5});
+    cols[c].push_back({c, -1e-2});
+  }
+  Csc K;
+  K.n = n;
+  K.p.push_back(0);
+  for (int c = 0; c < n; ++c) {
+    for (auto& e : cols[c]) {
+      K.i.push_back(e.first);
+      K.x.push_back(e.second);
+    }
+    K.p.push_back(static_cast<int>(K.i.size()));
+  }
+  return K;
+}
+
+std::vector<double> matvec(const Csc& K, const std::vector<double>& v) {
+  std::vector<double> y(K.n, 0.0);
+  for (int c = 0; c < K.n; ++c)
+    for (int q = K.p[c]; q < K.p[c + 1]; ++q) {
+      const int r = K.i[q];
+      y[c] += K.x[q] * v[r];
+      if (r != c) y[r] += K.x[q] * v[c];
+    }
+  return y;
+}
+
+double rel_residual(const Csc& K, const std::vector<double>& x, const std::vector<double>& b) {
+  const std::vector<double> y = matvec(K, x);
+  double r = 0, bn = 0;
+  for (int i = 0; i < K.n; ++i) {
+    r = std::fmax(r, std::fabs(b[i] - y[i]));
+    bn = std::fmax(bn, std::fabs(b[i]));
+  }
+  return r / bn;
+}
+
+int failures = 0;
+void expect(bool ok, const char* what) {
+  std::printf("%s %s\n", ok ? "ok  " : "FAIL", what);
+  if (!ok) ++failures;
+}
+
+template <typename INT, typename PREC, typename XPREC>
+void qdldl_sequence(const Csc& K, double tol, const char* label) {
+  // the call sequence of luir.hpp:79-178
+  const INT n = K.n;
+  std::vector<INT> Ap(K.p.begin(), K.p.end()), Ai(K.i.begin(), K.i.end());
+  std::vector<PREC> Ax(K.x.begin(), K.x.end());
+  std::vector<INT> work(n), Lnz(n), etree(n);
+  const INT sumLnz = QDLDL_etree(n, Ap.data(), Ai.data(), work.data(), Lnz.data(), etree.data());
+  expect(sumLnz > 0, "QDLDL_etree returns sum(Lnz)");
+  std::vector<INT> Lp(n + 1), Li(sumLnz), iwork(3 * n);
+  std::vector<PREC> Lx(sumLnz), D(n), Dinv(n), fwork(n);
+  std::vector<unsigned char> bw(n);
+  const INT pos = QDLDL_factor(n, Ap.data(), Ai.data(), Ax.data(), Lp.data(), Li.data(), Lx.data(), D.data(),
+                               Dinv.data(), Lnz.data(), etree.data(), reinterpret_cast<bool*>(bw.data()),
+                               iwork.data(), fwork.data());
+  char msg[160];
+  std::snprintf(msg, sizeof msg, "%s: QDLDL_factor inertia %lld == %d", label, static_cast<long long>(pos), K.n - 40);
+  expect(pos == K.n - 40, msg);
+  std::vector<double> xt(K.n);
+  for (int i = 0; i < K.n; ++i) xt[i] = std::sin(0.37 * i);
+  const std::vector<double> b = matvec(K, xt);
+  std::vector<XPREC> x(b.begin(), b.end());
+  QDLDL_solve(n, Lp.data(), Li.data(), Lx.data(), Dinv.data(), x.data());
+  const double res = rel_residual(K, std::vector<double>(x.begin(), x.end()), b);
+  std::snprintf(msg, sizeof msg, "%s: QDLDL_solve relative residual %.2e < %.0e", label, res, tol);
+  expect(res < tol, msg);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const bool host_only = argc > 1 && std::strcmp(argv[1], "--host") == 0;
+  const Csc K = make_kkt(12, 40);
+  if (host_only) {
+    std::vector<int> work(K.n), Lnz(K.n), etree(K.n);
+    const int s = QDLDL_etree(K.n, K.p.data(), K.i.data(), work.data(), Lnz.data(), etree.data());
+    expect(s > 0, "QDLDL_etree runs on the host");
+    std::vector<int> Lp(K.n + 1), Li(s);
+    std::vector<double> Lx(s), D(K.n), Dinv(K.n);
+    const int rc = QDLDL_factor(K.n, K.p.data(), K.i.data(), K.x.data(), Lp.data(), Li.data(), Lx.data(), D.data(),
+                                Dinv.data(), Lnz.data(), etree.data(), static_cast<bool*>(nullptr),
+                                static_cast<int*>(nullptr), static_cast<double*>(nullptr));
+    expect(rc == MPIR_ERR_NODEVICE, "QDLDL_factor without a gfx950 device fails with MPIR_ERR_NODEVICE");
+    bool threw = false;
+    try {
+      GmresLDLIR<float, double, double> solver;
+    } catch (const std::runtime_error&) {
+      threw = true;
+    }
+    expect(threw, "GmresLDLIR without a gfx950 device throws");
+    return failures == 0 ? 0 : 1;
+  }
+  qdldl_sequence<int, double, double>(K, 1e-12, "int/double/double");
+  qdldl_sequence<long, double, double>(K, 1e-12, "long/double/double");
+  qdldl_sequence<int, float, double>(K, 1e-5, "int/float/double");
+  qdldl_sequence<long long, float, float>(K, 1e-4, "long long/float/float");
+
+  std::vector<std::size_t> Ap(K.p.begin(), K.p.end()), Ai(K.i.begin(), K.i.end());
+  std::vector<double> xt(K.n);
+  for (int i = 0; i < K.n; ++i) xt[i] = std::cos(0.21 * i);
+  const std::vector<double> b = matvec(K, xt);
+  GmresLDLIR<float, double, double> solver;
+  solver.SetTolerance(1e-12);
+  solver.SetMaxIRIterations(10);
+  solver.SetMaxGmresIterations(10);
+  solver.Compute<double>(Ap, Ai, K.x, 0);
+  const std::vector<double> x = solver.Solve(b);
+  const double res = rel_residual(K, x, b);
+  char msg[160];
+  std::snprintf(msg, sizeof msg, "GmresLDLIR<float,double,double>: relative residual %.2e < 1e-13", res);
+  expect(res < 1e-13, msg);
+  expect(solver.Solve(std::vector<double>(3)).empty(), "Solve on a size mismatch returns an empty vector");
+  GmresLDLIR<double, double, double> solver64;
+  solver64.Compute<double>(Ap, Ai, K.x, 0);
+  const double res64 = rel_residual(K, solver64.Solve(b), b);
+  std::snprintf(msg, sizeof msg, "GmresLDLIR<double,double,double>: relative residual %.2e < 1e-13", res64);
+  expect(res64 < 1e-13, msg);
+  return failures == 0 ? 0 : 1;
+}

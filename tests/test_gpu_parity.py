@@ -199,3 +199,15 @@ def test_exact_factor_gives_one_step_per_gmres_call(gpu, oracle, k):
     calls = st["ir_iterations"] + 1
     assert st["gmres_iterations"] <= 2 * calls, st
     assert backward_error(oracle, P.Ap, P.Ai, P.Ax, x, P.b) < 1e-14
+
+
+def test_cpp_dropin_program(gpu):
+    # a C++ caller written against the reference's names (QDLDL_etree /
+    # QDLDL_factor / QDLDL_solve with several INT/PREC/XPREC combinations,
+    # GmresLDLIR<float|double, double, double>) through include/mpir/*.hpp
+    import subprocess
+    exe = Path(__file__).resolve().parent / "cpp" / "dropin_test"
+    assert exe.exists(), "tests/cpp/dropin_test is built by mpir_amd.build"
+    proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    print(proc.stdout)
+    assert proc.returncode == 0, proc.stdout + proc.stderr

@@ -116,3 +116,18 @@ def test_qdldl_etree_overflow_code():
     assert mpir_amd.QDLDL_etree(n, Ap.astype(np.int32), Ai.astype(np.int32), work, Lnz, et) == -2
     w64, l64, e64 = (np.zeros(n, np.int64) for _ in range(3))
     assert mpir_amd.QDLDL_etree(n, Ap, Ai, w64, l64, e64) == n * (n - 1) // 2
+
+
+DROPIN = Path(__file__).resolve().parent / "cpp" / "dropin_test"
+
+
+def test_cpp_dropin_headers_without_a_device():
+    # include/mpir/*.hpp compiled into a reference-style caller: the host-side
+    # QDLDL_etree works, every GPU entry point fails loudly without a device
+    if mpir_amd.device_ok():
+        pytest.skip("a gfx950 device is present; the GPU test runs the full program")
+    import subprocess
+    assert DROPIN.exists(), "tests/cpp/dropin_test is built by mpir_amd.build"
+    proc = subprocess.run([str(DROPIN), "--host"], capture_output=True, text=True, timeout=60)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert "MPIR_ERR_NODEVICE" in proc.stdout
