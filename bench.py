@@ -123,6 +123,41 @@ def cpu_baseline(mpir, full_flops: float, seconds_budget: float = 20.0) -> dict:
     }
 
 
+def precision_sweep(mpir, k: int, tol: float) -> int:
+    """The precision form of the workload: factor precision FP64 / FP32 / FP16
+    / BF16 (16-bit: FP32 accumulation, D^{-1} in FP32), the reference's
+    diagonal scaling and Ruiz equilibration, D in {1e-2, 1e-4, 1e-6}. One
+    JSON line per case: iterations, backward error, forward error against
+    the generator's x_true and against the FP64-factor solution."""
+    cases = [(np.float64, "f64"), (np.float32, "f32"), (np.float16, "f16"), ("bf16", "bf16")]
+    for d in (1e-2, 1e-4, 1e-6):
+        P = mpir.kkt_config(1, k=k, d_value=d)
+        x64 = None
+        for uf, name in cases:
+            for scaling, sname in ((mpir.SCALE_DIAG, "diag"), (mpir.SCALE_RUIZ, "ruiz")):
+                s = mpir.GmresLDLIR(uf, ordering=mpir.ORDER_ND, scaling=scaling)
+                s.SetTolerance(tol)
+                row = {"workload": f"KKT 3D {k}^3", "D": d, "factor": name, "scaling": sname}
+                try:
+                    s.Compute(P.Ap, P.Ai, P.Ax)
+                    x = s.Solve(P.b)
+                except mpir.MpirError as e:
+                    row["error"] = str(e)
+                    print(json.dumps(row), flush=True)
+                    continue
+                st = s.stats()
+                if x64 is None and name == "f64":
+                    x64 = x
+                row.update({"ir_iterations": st["ir_iterations"], "gmres_iterations": st["gmres_iterations"],
+                            "stop_reason": st["stop_reason"], "ruiz_sweeps": st["ruiz_sweeps"],
+                            "inertia_ok": st["positive_pivots"] == P.n1, "bwd_err": backward_error(P, x),
+                            "fwd_err_true": float(np.abs(x - P.x_true).max() / np.abs(P.x_true).max()),
+                            "fwd_err_fp64": float(np.abs(x - x64).max() / np.abs(x64).max()),
+                            "factor_ms": round(st["factor_ms"], 3), "solve_ms": round(st["solve_ms"], 3)})
+                print(json.dumps(row), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +169,8 @@ def main(argv=None) -> int:
                     help="fill-reducing ordering (nd = nested dissection, amd = approximate minimum degree)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-factor comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--sweep", action="store_true",
+                    help="precision form instead of the bench line: f64/f32/f16/bf16 factors x scaling x D scale")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve path runs on a single device "
@@ -145,6 +182,8 @@ def main(argv=None) -> int:
     if not mpir.device_ok():
         print("no gfx950 device", file=sys.stderr)
         return 1
+    if args.sweep:
+        return precision_sweep(mpir, args.k, args.tol)
     log(f"generating the {args.k}^3 KKT workload")
     P = mpir.kkt_config(1, k=args.k)
     dev = torch.device("cuda:0")

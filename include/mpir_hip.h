@@ -38,6 +38,11 @@ extern "C" {
 #define MPIR_ERR_NUMERIC (-104)
 #define MPIR_ERR_INTERNAL (-105)
 
+/* scalings */
+#define MPIR_SCALE_NONE 0
+#define MPIR_SCALE_DIAG 1
+#define MPIR_SCALE_RUIZ 2
+
 /* orderings */
 #define MPIR_ORDER_NATURAL 0 /* the matrix as given (QDLDL's behaviour) */
 #define MPIR_ORDER_AMD 1     /* approximate minimum degree + etree postorder */
@@ -114,15 +119,24 @@ typedef struct mpir_gmres_stats {
   double final_rel_residual;   /* ||b_s - K_s x||_inf / ||b_s||_inf, scaled system */
   double analyze_ms, factor_ms, solve_ms;
   int64_t device_bytes;
+  int32_t ruiz_sweeps;         /* equilibration sweeps of the last factorization */
 } mpir_gmres_stats;
 
-/* uf: MPIR_F64 or MPIR_F32; uw, ur: MPIR_F64 */
+/* uf: MPIR_F64, MPIR_F32, or MPIR_F16 / MPIR_BF16 (factor computed with
+ * FP32 arithmetic and its working values stored rounded to the 16-bit
+ * format; L and the triangular-solve operands held in that format, D^{-1}
+ * in FP32); uw, ur: MPIR_F64 */
 int mpir_gmres_ldl_create(int uf, int uw, int ur, mpir_gmres_ldl_t* out);
 void mpir_gmres_ldl_destroy(mpir_gmres_ldl_t h);
 int mpir_gmres_ldl_set_max_ir(mpir_gmres_ldl_t h, int64_t n);        /* SetMaxIRIterations */
 int mpir_gmres_ldl_set_max_gmres(mpir_gmres_ldl_t h, int64_t n);     /* SetMaxGmresIterations */
 int mpir_gmres_ldl_set_tol(mpir_gmres_ldl_t h, double tol);          /* SetTolerance */
 int mpir_gmres_ldl_set_ordering(mpir_gmres_ldl_t h, int ordering);  /* before analyze */
+/* scaling of K before factoring: MPIR_SCALE_NONE, MPIR_SCALE_DIAG (the
+ * reference's s_i = 1/sqrt|k_ii| where |k_ii| > eps(UF), gmres_ir.hpp:347-363;
+ * the default) or MPIR_SCALE_RUIZ (symmetric Ruiz equilibration: up to 10
+ * sweeps, until every row's max-norm is within 1e-2 of one) */
+int mpir_gmres_ldl_set_scaling(mpir_gmres_ldl_t h, int mode);
 /* symbolic phase on the pattern of the upper CSC (reused by later factors) */
 int mpir_gmres_ldl_analyze(mpir_gmres_ldl_t h, int32_t n, const int32_t* Ap, const int32_t* Ai);
 /* numeric phase; Ax in the analyzed pattern's entry order (host / device) */

@@ -28,6 +28,8 @@ F64, F32, F16, BF16 = 0, 1, 2, 3
 ORDER_NATURAL, ORDER_AMD, ORDER_ND = 0, 1, 2
 
 _PREC_OF = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
+_FACTOR_PREC_OF = {**_PREC_OF, np.dtype(np.float16): F16}
+SCALE_NONE, SCALE_DIAG, SCALE_RUIZ = 0, 1, 2
 
 
 class MpirError(RuntimeError):
@@ -117,18 +119,22 @@ class GmresLDLIR:
     reference: the upper triangle of K in CSC (Ap, Ai, Ax), rows ascending.
     """
 
-    def __init__(self, UF=np.float64, UW=np.float64, UR=np.float64, ordering: int = ORDER_AMD):
-        uf = _PREC_OF.get(np.dtype(UF))
+    def __init__(self, UF=np.float64, UW=np.float64, UR=np.float64, ordering: int = ORDER_AMD,
+                 scaling: int = SCALE_DIAG):
+        # UF may also be np.float16 or "bf16": a factor computed with FP32
+        # arithmetic and stored (and applied) in the 16-bit format
+        uf = BF16 if isinstance(UF, str) and UF.lower() == "bf16" else _FACTOR_PREC_OF.get(np.dtype(UF))
         if uf is None:
-            raise TypeError("UF must be float64 or float32")
+            raise TypeError("UF must be float64, float32, float16 or 'bf16'")
         if np.dtype(UW) != np.float64 or np.dtype(UR) != np.float64:
             raise TypeError("UW and UR must be float64")
         h = C.c_void_p()
         _check(load().mpir_gmres_ldl_create(uf, F64, F64, C.byref(h)), "GmresLDLIR()")
         self._h = h
         self._n = 0
-        self.UF = np.dtype(UF)
+        self.UF = UF if uf == BF16 else np.dtype(UF)
         _check(load().mpir_gmres_ldl_set_ordering(self._h, ordering), "set_ordering")
+        _check(load().mpir_gmres_ldl_set_scaling(self._h, scaling), "set_scaling")
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -148,6 +154,10 @@ class GmresLDLIR:
 
     def SetOrdering(self, ordering: int) -> None:
         _check(load().mpir_gmres_ldl_set_ordering(self._h, int(ordering)), "SetOrdering")
+
+    def SetScaling(self, mode: int) -> None:
+        """SCALE_NONE, SCALE_DIAG (the reference's, default) or SCALE_RUIZ."""
+        _check(load().mpir_gmres_ldl_set_scaling(self._h, int(mode)), "SetScaling")
 
     # phases
     def Analyze(self, Ap, Ai) -> None:
@@ -305,5 +315,5 @@ def residual_upper(Ap, Ai, Ax, x, b) -> np.ndarray:
 __all__ = [
     "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
     "amd_order", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
-    "ORDER_AMD", "ORDER_ND",
+    "ORDER_AMD", "ORDER_ND", "SCALE_NONE", "SCALE_DIAG", "SCALE_RUIZ",
 ]

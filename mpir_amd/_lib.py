@@ -18,6 +18,7 @@ class GmresStats(C.Structure):
         ("gmres_iterations", i32), ("precond_applications", i32), ("stop_reason", i32),
         ("final_rel_residual", f64), ("analyze_ms", f64), ("factor_ms", f64), ("solve_ms", f64),
         ("device_bytes", i64),
+        ("ruiz_sweeps", i32),
     ]
 
     def as_dict(self) -> dict:
@@ -47,6 +48,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_set_max_gmres": (C.c_int, [P, i64]),
     "mpir_gmres_ldl_set_tol": (C.c_int, [P, f64]),
     "mpir_gmres_ldl_set_ordering": (C.c_int, [P, C.c_int]),
+    "mpir_gmres_ldl_set_scaling": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_analyze": (C.c_int, [P, i32, P, P]),
     "mpir_gmres_ldl_factor": (C.c_int, [P, P]),
     "mpir_gmres_ldl_factor_dev": (C.c_int, [P, P]),

@@ -104,7 +104,7 @@ int64_t factor_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const T* Ax
   if (!E.solver) {
     E.solver = std::make_unique<GmresLdl>(prec, kF64, kF64);
     E.solver->set_ordering(kOrderNatural);
-    E.solver->set_scaling(false);
+    E.solver->set_scaling(kScaleNone);
   }
   const int64_t nnz = Ap[n];
   uint64_t h = fnv1a(Ap, sizeof(int32_t) * (n + 1));
@@ -326,6 +326,14 @@ int mpir_gmres_ldl_set_max_gmres(mpir_gmres_ldl_t h, int64_t n) {
 int mpir_gmres_ldl_set_tol(mpir_gmres_ldl_t h, double tol) {
   return guarded([&] { MPIR_HANDLE_CHECK; h->s->set_tol(tol); return 0; });
 }
+int mpir_gmres_ldl_set_scaling(mpir_gmres_ldl_t h, int mode) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->set_scaling(mode);
+    return 0;
+  });
+}
+
 int mpir_gmres_ldl_set_ordering(mpir_gmres_ldl_t h, int ordering) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;
@@ -394,6 +402,7 @@ int mpir_gmres_ldl_stats(mpir_gmres_ldl_t h, mpir_gmres_stats* out) {
     out->factor_ms = s.factor_ms;
     out->solve_ms = s.solve_ms;
     out->device_bytes = s.device_bytes;
+    out->ruiz_sweeps = s.ruiz_sweeps;
     return 0;
   });
 }

@@ -16,6 +16,8 @@
 //            instead of modified Gram-Schmidt with a conditional second pass.
 // Vectors stay on the device in the fill-reducing order; only the Hessenberg
 // column and the norms that drive the control flow come back to the host.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -167,6 +169,28 @@ __global__ void k_scale_entries(int64_t nnz, const double* __restrict__ ax, cons
   if (q < nnz) axs[q] = ax[q] * (ad[row[q]] * ad[col[q]]);
 }
 
+// Symmetric Ruiz equilibration of the full CSR (permuted rows, original
+// scale vector ad): rmax[i] = max_j |ad_i k_ij ad_j|; dev[i] = |1 - rmax[i]|.
+__global__ void k_ruiz_rowmax(int n, const int32_t* __restrict__ p, const int32_t* __restrict__ j,
+                              const double* __restrict__ x, const int32_t* __restrict__ perm,
+                              const double* __restrict__ ad, double* __restrict__ rmax, double* __restrict__ dev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double ai = ad[perm[i]];
+  double mx = 0.0;
+  for (int q = p[i]; q < p[i + 1]; ++q) mx = fmax(mx, fabs(ai * x[q] * ad[perm[j[q]]]));
+  rmax[i] = mx;
+  dev[i] = fabs(1.0 - mx);
+}
+__global__ void k_ruiz_update(int n, const int32_t* __restrict__ perm, const double* __restrict__ rmax,
+                              double* __restrict__ ad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && rmax[i] > 0.0) ad[perm[i]] /= sqrt(rmax[i]);
+}
+__global__ void k_fill(int n, double v, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
 __global__ void k_gather(int64_t nnz, const double* __restrict__ src, const int32_t* __restrict__ idx,
                          double* __restrict__ out) {
   const int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -206,7 +230,7 @@ enum Slot { kSlotNorm = 0, kSlotH = 8 };
 
 GmresLdl::GmresLdl(int uf, int uw, int ur) : uf_(uf), uw_(uw), ur_(ur) {
   // Refinable<UF,UW,UR> (fp_concepts.hpp:112-113): eps(UF) >= eps(UW) >= eps(UR)
-  MPIR_CHECK(uf == kF64 || uf == kF32, "factor precision must be f64 or f32");
+  MPIR_CHECK(uf == kF64 || uf == kF32 || uf == kF16 || uf == kBF16, "factor precision must be f64, f32, f16 or bf16");
   MPIR_CHECK(uw == kF64 && ur == kF64, "working and residual precision must be f64");
   // the solver runs only on a gfx950 device; there is no host fallback
   {
@@ -342,6 +366,7 @@ void GmresLdl::upload_plan() {
   dfp_.h_pan_ptr = P.pan_ptr;
   dfp_.h_upd_ptr = P.upd_ptr;
   dfp_.h_pan_aug_ptr = P.pan_aug_ptr;
+  dfp_.n_inv_task = static_cast<int>(P.sxc_task.size() / 3);
   dfp_.h_upd_aug_ptr = P.upd_aug_ptr;
   keep_.resize(20);
   upload_vec(keep_[15], P.asm_child_ptr, st);
@@ -364,6 +389,7 @@ void GmresLdl::upload_plan() {
   dsp_.gather_child_ptr = keep_[17].as<int32_t>();
   dsp_.gather_child = keep_[18].as<int32_t>();
   dsp_.xc_task = keep_[9].as<int32_t>();
+  dfp_.inv_task = keep_[9].as<int32_t>();
   dsp_.u_task = keep_[10].as<int32_t>();
   dsp_.bwd_task = keep_[8].as<int32_t>();
   dsp_.rpart_task = keep_[12].as<int32_t>();
@@ -390,6 +416,8 @@ void GmresLdl::alloc_numeric() {
   d_axs_.alloc(std::max<int64_t>(1, nnz_) * sizeof(double));
   d_ad_.alloc(n * sizeof(double));
   d_panel_.alloc(std::max<int64_t>(1, P.panel_entries) * tf);
+  if (uf_ == kF16 || uf_ == kBF16) d_panel16_.alloc(std::max<int64_t>(1, P.panel_entries) * 2);
+  d_nonfinite_.alloc(sizeof(int));
   d_upool_.alloc(std::max<int64_t>(1, P.upd_pool) * tf);
   d_D_.alloc(n * tf);
   d_Dinv_.alloc(n * tf);
@@ -439,6 +467,7 @@ void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, 
   MPIR_HIP(hipMemcpyAsync(d_panel_.get(), panel.data(), panel.size(), hipMemcpyHostToDevice, stream_));
   MPIR_HIP(hipMemcpyAsync(d_Dinv_.get(), Dinv, n * tf, hipMemcpyHostToDevice, stream_));
   // the inverses of the diagonal blocks of big fronts, from L and D = 1/Dinv
+  MPIR_CHECK(uf_ == kF64 || uf_ == kF32, "a loaded factor must be f64 or f32");
   if (uf_ == kF64) {
     reciprocal<double>(n, d_Dinv_.as<double>(), d_D_.as<double>(), stream_);
     invert_diag_blocks<double>(factor_args<double>(), dfp_, stream_);
@@ -451,6 +480,15 @@ void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, 
   precond_only_ = true;
 }
 
+void GmresLdl::round_factor16() {
+  if (uf_ == kF16)
+    round_panel<__half>(plan_->panel_entries, d_panel_.as<float>(), d_panel16_.as<__half>(), d_nonfinite_.as<int>(),
+                        stream_);
+  else if (uf_ == kBF16)
+    round_panel<__hip_bfloat16>(plan_->panel_entries, d_panel_.as<float>(), d_panel16_.as<__hip_bfloat16>(),
+                                d_nonfinite_.as<int>(), stream_);
+}
+
 void GmresLdl::factor(const double* Ax, bool device_ptr) {
   if (!analyzed_) throw Error(kErrState, "factor called before analyze");
   const double t0 = now_ms();
@@ -459,36 +497,64 @@ void GmresLdl::factor(const double* Ax, bool device_ptr) {
   if (nnz_ > 0)
     MPIR_HIP(hipMemcpyAsync(d_ax_.get(), Ax, nnz_ * sizeof(double),
                             device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-  // |a_ii| > eps(UF) selects the diagonals that scale; +inf disables scaling
-  const double eps = !scaling_ ? std::numeric_limits<double>::infinity()
-                     : uf_ == kF64 ? std::numeric_limits<double>::epsilon()
-                                   : static_cast<double>(std::numeric_limits<float>::epsilon());
-  hipLaunchKernelGGL(k_compute_ad, dim3(nblk(n_)), dim3(256), 0, st, n_, d_ax_.as<double>(), d_diag_pos_.as<int32_t>(),
-                     eps, d_ad_.as<double>());
+  const int64_t ncsr = 2 * nnz_ - n_;
+  if (scaling_ == kScaleRuiz) {
+    // Ruiz: at most kRuizSweeps sweeps, stopping once every row's max-norm is
+    // within kRuizTol of one (the tolerance is checked before each update)
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n_)), dim3(256), 0, st, n_, 1.0, d_ad_.as<double>());
+    hipLaunchKernelGGL(k_gather, dim3(nblk(ncsr)), dim3(256), 0, st, ncsr, d_ax_.as<double>(),
+                       d_csr_src_.as<int32_t>(), d_csr_x_.as<double>());
+    stats_.ruiz_sweeps = 0;
+    for (int sweep = 0; sweep <= kRuizSweeps; ++sweep) {
+      hipLaunchKernelGGL(k_ruiz_rowmax, dim3(nblk(n_)), dim3(256), 0, st, n_, d_csr_p_.as<int32_t>(),
+                         d_csr_j_.as<int32_t>(), d_csr_x_.as<double>(), d_perm_.as<int32_t>(), d_ad_.as<double>(),
+                         d_r_.as<double>(), d_w_.as<double>());
+      if (infnrm(d_w_.as<double>()) <= kRuizTol || sweep == kRuizSweeps) break;
+      hipLaunchKernelGGL(k_ruiz_update, dim3(nblk(n_)), dim3(256), 0, st, n_, d_perm_.as<int32_t>(),
+                         d_r_.as<double>(), d_ad_.as<double>());
+      stats_.ruiz_sweeps = sweep + 1;
+    }
+  } else {
+    // |a_ii| > eps(UF) selects the diagonals that scale (gmres_ir.hpp:347-363);
+    // +inf disables scaling
+    const double eps = scaling_ == kScaleNone ? std::numeric_limits<double>::infinity()
+                       : uf_ == kF64         ? std::numeric_limits<double>::epsilon()
+                       : uf_ == kF32         ? static_cast<double>(std::numeric_limits<float>::epsilon())
+                       : uf_ == kF16         ? 0.0009765625    // 2^-10
+                                             : 0.0078125;      // 2^-7
+    hipLaunchKernelGGL(k_compute_ad, dim3(nblk(n_)), dim3(256), 0, st, n_, d_ax_.as<double>(),
+                       d_diag_pos_.as<int32_t>(), eps, d_ad_.as<double>());
+  }
   hipLaunchKernelGGL(k_scale_entries, dim3(nblk(nnz_)), dim3(256), 0, st, nnz_, d_ax_.as<double>(),
                      d_entry_row_.as<int32_t>(), d_entry_col_.as<int32_t>(), d_ad_.as<double>(), d_axs_.as<double>());
-  const int64_t ncsr = 2 * nnz_ - n_;
   hipLaunchKernelGGL(k_gather, dim3(nblk(ncsr)), dim3(256), 0, st, ncsr, d_axs_.as<double>(), d_csr_src_.as<int32_t>(),
                      d_csr_x_.as<double>());
   MPIR_HIP(hipGetLastError());
   int err = 0, pos = 0;
   if (uf_ == kF64) {
     const FactorArgs<double> a = factor_args<double>();
-    factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, st);
+    factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, 0, st);
     count_positive<double>(n_, d_D_.as<double>(), d_cnt_.as<int>(), st);
   } else {
     const FactorArgs<float> a = factor_args<float>();
-    factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, st);
+    factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, rounding(), st);
     count_positive<float>(n_, d_D_.as<float>(), d_cnt_.as<int>(), st);
+    round_factor16();
   }
   MPIR_HIP(hipMemcpyAsync(&h_pinned_[0], d_err_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
   MPIR_HIP(hipMemcpyAsync(&h_pinned_[1], d_cnt_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+  int nonfinite = 0;
+  if (uf_ == kF16 || uf_ == kBF16)
+    MPIR_HIP(hipMemcpyAsync(&h_pinned_[2], d_nonfinite_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
   MPIR_HIP(hipStreamSynchronize(st));
   std::memcpy(&err, &h_pinned_[0], sizeof(int));
   std::memcpy(&pos, &h_pinned_[1], sizeof(int));
+  if (uf_ == kF16 || uf_ == kBF16) std::memcpy(&nonfinite, &h_pinned_[2], sizeof(int));
   stats_.positive_pivots = pos;
   stats_.factor_ms = now_ms() - t0;
   if (err) throw Error(kErrNumeric, "zero pivot in the LDL^T factorization");
+  if (nonfinite)
+    throw Error(kErrNumeric, std::to_string(nonfinite) + " factor entries overflow the " + prec_name(uf_) + " range");
   factored_ = true;
 }
 
@@ -514,8 +580,11 @@ double GmresLdl::infnrm(const double* v) {
 
 void GmresLdl::precond_apply_dev(double* x) {
   stats_.precond_applications++;
-  if (uf_ == kF64) ldl_solve<double, double>(solve_args<double>(), dsp_, x, d_uvec_.as<double>(), stream_);
-  else ldl_solve<float, double>(solve_args<float>(), dsp_, x, d_uvec_.as<double>(), stream_);
+  double* u = d_uvec_.as<double>();
+  if (uf_ == kF64) ldl_solve<double, double, double>(solve_args<double, double>(d_panel_.get()), dsp_, x, u, stream_);
+  else if (uf_ == kF32) ldl_solve<float, float, double>(solve_args<float, float>(d_panel_.get()), dsp_, x, u, stream_);
+  else if (uf_ == kF16) ldl_solve<__half, float, double>(solve_args<__half, float>(d_panel16_.get()), dsp_, x, u, stream_);
+  else ldl_solve<__hip_bfloat16, float, double>(solve_args<__hip_bfloat16, float>(d_panel16_.get()), dsp_, x, u, stream_);
 }
 
 void GmresLdl::spmv_dev(const double* x, double* y) {
@@ -700,10 +769,11 @@ double GmresLdl::time_kernels(int what, int reps) {
       precond_apply_dev(x);
     } else if (uf_ == kF64) {
       const FactorArgs<double> a = factor_args<double>();
-      factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, st);
+      factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, 0, st);
     } else {
       const FactorArgs<float> a = factor_args<float>();
-      factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, st);
+      factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, rounding(), st);
+      round_factor16();
     }
   };
   run();  // warm

@@ -22,9 +22,12 @@
 //          I * L_11^{-T} D^{-1} in the rows of the identity
 //   U_s    nr x nr column-major (ld = nr), lower part: the update matrix
 //          handed to the parent (extend-add through relidx)
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "../common.hpp"
 #include "../plan.hpp"
@@ -36,17 +39,30 @@ namespace {
 
 __device__ __forceinline__ void flag_zero_pivot(int* err) { atomicOr(err, 1); }
 
+// Rounding of the factor's entries when the factor is held in a 16-bit
+// format: R = 0 none, 1 to FP16, 2 to BF16 (round to nearest even). The
+// matrix values entering the factorization and every entry of L are rounded
+// as they are stored, so the trailing updates multiply 16-bit operands;
+// products, sums and the update matrices (Schur complements) stay in T
+// (FP32 accumulation). D, D^{-1} and the inverse rows are not rounded.
+template <int R, class T>
+__device__ __forceinline__ T rnd(T v) {
+  if constexpr (R == 1) return static_cast<T>(__half2float(__float2half_rn(static_cast<float>(v))));
+  else if constexpr (R == 2) return static_cast<T>(__bfloat162float(__float2bfloat16(static_cast<float>(v))));
+  else return v;
+}
+
 // ---------------------------------------------------------------- scatter
-template <class T>
+template <class T, int R>
 __global__ void k_scatter_values(int64_t nnz, const double* __restrict__ ax,
                                  const int64_t* __restrict__ dst, T* __restrict__ panel) {
   const int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  if (q < nnz) panel[dst[q]] = static_cast<T>(ax[q]);
+  if (q < nnz) panel[dst[q]] = rnd<R>(static_cast<T>(ax[q]));
 }
 
 // ---------------------------------------------------------------- small fronts
 // One wavefront per front (m <= 64), the whole front in LDS.
-template <class T>
+template <class T, int R>
 __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int32_t* __restrict__ list) {
   __shared__ T F[kSmallFront][kSmallFront + 1];
   __shared__ T lb[kSmallFront];
@@ -116,7 +132,7 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
     a.Dinv[f + lane] = T(1) / d;
   }
   for (int c = 0; c < nc; ++c)
-    if (lane < m && lane >= c) P[lane + static_cast<int64_t>(c) * m] = F[lane][c];
+    if (lane < m && lane >= c) P[lane + static_cast<int64_t>(c) * m] = lane == c ? F[lane][c] : rnd<R>(F[lane][c]);
   for (int b = 0; b < nr; ++b)
     if (lane >= b && lane < nr) U[lane + static_cast<int64_t>(b) * nr] = F[nc + lane][nc + b];
 }
@@ -128,7 +144,7 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
 // update matrices are added in fixed order (a barrier between children, no
 // two entries of one child land on the same place), then the tile is
 // written back once.
-template <class T>
+template <class T, int R>
 __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* __restrict__ tasks,
                                                   const int32_t* __restrict__ cptr, const int32_t* __restrict__ cl) {
   constexpr int E = kAsmTile;
@@ -200,7 +216,7 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
 // outside); the block is rebuilt from them and only Z is formed.
 constexpr int kPanelThreads = 1024;  // 16 waves, each owning 4 block columns
 
-template <class T>
+template <class T, int R>
 __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, const Task4* __restrict__ tasks,
                                                               bool refactor) {
   constexpr int NB = kPanelNB;
@@ -290,7 +306,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
     // L_kk goes to the block's upper triangle, transposed (entry (j, r) holds
     // L[r, j]): the other workgroups of this launch read the lower triangle
     for (int rr = w; rr < nbk; rr += kPanelThreads / 64)
-      if (lane < rr) P[(k0 + lane) + (k0 + rr) * ld] = Xs[lane][rr];
+      if (lane < rr) P[(k0 + lane) + (k0 + rr) * ld] = rnd<R>(Xs[lane][rr]);
     if (tid < nbk) {
       const T d = Xs[tid][tid];
       if (d == T(0)) flag_zero_pivot(a.err);
@@ -318,7 +334,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
 #pragma unroll
   for (int u = 0; u < CW; ++u) {
     const int c = c0w + u;
-    if (c < nbk) P[r + (k0 + c) * ld] = acc[u];
+    if (c < nbk) P[r + (k0 + c) * ld] = r < m ? rnd<R>(acc[u]) : acc[u];
   }
 }
 
@@ -353,7 +369,7 @@ struct Mfma16<double> {
 // columns >= nc in the update matrix. Tiles with r0 >= m cover identity rows
 // (rows m .. m+nc) and only panel columns. Each wave computes a 32 x 32
 // quarter with 2 x 2 MFMA blocks.
-template <class T>
+template <class T, int R>
 __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   using V = typename Mfma16<T>::V;
   __shared__ T As[kPanelNB][kTile + 1];  // As[t][i] = L[r0 + i, k0 + t]
@@ -453,30 +469,69 @@ inline unsigned blocks_for(int64_t n, int bs) { return static_cast<unsigned>((n 
 
 }  // namespace
 
-template <class T>
-void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const double* ax_scaled,
-                    int64_t nnz, hipStream_t st) {
+template <class T, int R>
+void factor_levels(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const double* ax_scaled, int64_t nnz,
+                   hipStream_t st) {
   MPIR_HIP(hipMemsetAsync(a.panel, 0, sizeof(T) * dp.panel_entries, st));
   MPIR_HIP(hipMemsetAsync(a.err, 0, sizeof(int), st));
   if (nnz > 0)
-    hipLaunchKernelGGL(k_scatter_values<T>, dim3(blocks_for(nnz, 256)), dim3(256), 0, st, nnz, ax_scaled,
+    hipLaunchKernelGGL((k_scatter_values<T, R>), dim3(blocks_for(nnz, 256)), dim3(256), 0, st, nnz, ax_scaled,
                        dp.val_dst, a.panel);
   for (int l = 0; l < dp.nlevels; ++l) {
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
-      hipLaunchKernelGGL(k_small_fronts<T>, dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0);
+      hipLaunchKernelGGL((k_small_fronts<T, R>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0);
     const int a0 = dp.h_asm_ptr[l], a1 = dp.h_asm_ptr[l + 1];
     if (a1 > a0)
-      hipLaunchKernelGGL(k_assemble<T>, dim3(a1 - a0), dim3(256), 0, st, a, dp.asm_task + a0, dp.asm_child_ptr + a0,
-                         dp.asm_child);
+      hipLaunchKernelGGL((k_assemble<T, R>), dim3(a1 - a0), dim3(256), 0, st, a, dp.asm_task + a0,
+                         dp.asm_child_ptr + a0, dp.asm_child);
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_ptr[q], p1 = dp.h_pan_ptr[q + 1];
-      if (p1 > p0) hipLaunchKernelGGL(k_panel_step<T>, dim3(p1 - p0), dim3(kPanelThreads), 0, st, a, dp.pan_task + p0, true);
+      if (p1 > p0)
+        hipLaunchKernelGGL((k_panel_step<T, R>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a, dp.pan_task + p0,
+                           true);
       const int u0 = dp.h_upd_ptr[q], u1 = dp.h_upd_ptr[q + 1];
-      if (u1 > u0) hipLaunchKernelGGL(k_update_tiles<T>, dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
+      if (u1 > u0) hipLaunchKernelGGL((k_update_tiles<T, R>), dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
     }
   }
+}
+
+// W = Z D: column c of the inverse rows of every big front times D_c, which
+// turns Z = L_11^{-T} D^{-1} into the unit upper triangular L_11^{-T}.
+// Tasks (s, column block j, row chunk h) cover the rows at or above the
+// block's last column.
+template <class T>
+__global__ void __launch_bounds__(256) k_inverse_to_unit(FactorArgs<T> a, const int32_t* __restrict__ tasks) {
+  const int s = tasks[3 * blockIdx.x], j = tasks[3 * blockIdx.x + 1], h = tasks[3 * blockIdx.x + 2];
+  const int f = a.first[s];
+  const int nc = a.first[s + 1] - f;
+  const int nr = a.nrow[s];
+  const int m = nc + nr;
+  const int64_t ld = panel_ld(nc, nr);
+  const int c0 = j * kPanelNB;
+  const int bw = min(kPanelNB, nc - c0);
+  const int q0 = h * kXcRows;
+  const int len = min(kXcRows, c0 + bw - q0);
+  T* W = a.panel + a.panel_off[s] + m + q0;
+  for (int e = threadIdx.x; e < len * bw; e += blockDim.x) {
+    const int q = e % len, c = c0 + e / len;
+    W[q + c * ld] *= a.D[f + c];
+  }
+}
+
+template <class T>
+void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const double* ax_scaled, int64_t nnz,
+                    int rounding, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (rounding == 1) factor_levels<T, 1>(a, dp, ax_scaled, nnz, st);
+    else if (rounding == 2) factor_levels<T, 2>(a, dp, ax_scaled, nnz, st);
+    else factor_levels<T, 0>(a, dp, ax_scaled, nnz, st);
+  } else {
+    factor_levels<T, 0>(a, dp, ax_scaled, nnz, st);
+  }
+  if (dp.n_inv_task > 0)
+    hipLaunchKernelGGL(k_inverse_to_unit<T>, dim3(dp.n_inv_task), dim3(256), 0, st, a, dp.inv_task);
   MPIR_HIP(hipGetLastError());
 }
 
@@ -486,12 +541,47 @@ void invert_diag_blocks(const FactorArgs<T>& a, const DeviceFactorPlan& dp, hipS
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_aug_ptr[q], p1 = dp.h_pan_ptr[q + 1];
-      if (p1 > p0) hipLaunchKernelGGL(k_panel_step<T>, dim3(p1 - p0), dim3(kPanelThreads), 0, st, a, dp.pan_task + p0, false);
+      if (p1 > p0)
+        hipLaunchKernelGGL((k_panel_step<T, 0>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a, dp.pan_task + p0,
+                           false);
       const int u0 = dp.h_upd_aug_ptr[q], u1 = dp.h_upd_ptr[q + 1];
-      if (u1 > u0) hipLaunchKernelGGL(k_update_tiles<T>, dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
+      if (u1 > u0) hipLaunchKernelGGL((k_update_tiles<T, 0>), dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
     }
   }
+  if (dp.n_inv_task > 0)
+    hipLaunchKernelGGL(k_inverse_to_unit<T>, dim3(dp.n_inv_task), dim3(256), 0, st, a, dp.inv_task);
   MPIR_HIP(hipGetLastError());
+}
+
+// The factor in a 16-bit format: dst[q] = round(src[q]) over the whole
+// panel pool (FP16 or BF16, round to nearest even).
+template <class H>
+__global__ void k_round_panel(int64_t n, const float* __restrict__ src, H* __restrict__ dst) {
+  const int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (q < n) {
+    if constexpr (std::is_same<H, __half>::value) dst[q] = __float2half_rn(src[q]);
+    else dst[q] = __float2bfloat16(src[q]);
+  }
+}
+
+// counts entries of the rounded panel that are not finite (out of range)
+template <class H>
+__global__ void k_count_nonfinite(int64_t n, const H* __restrict__ v, int* __restrict__ out) {
+  int cnt = 0;
+  for (int64_t q = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; q < n;
+       q += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    cnt += isfinite(static_cast<float>(v[q])) ? 0 : 1;
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(out, cnt);
+}
+
+template <class H>
+void round_panel(int64_t n, const float* src, H* dst, int* nonfinite, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_round_panel<H>, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, src, dst);
+  MPIR_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_count_nonfinite<H>, dim3(std::min<unsigned>(2048, blocks_for(n, 256))), dim3(256), 0, st, n,
+                     dst, nonfinite);
 }
 
 template <class T>
@@ -512,12 +602,14 @@ void count_positive(int n, const T* D, int* out, hipStream_t st) {
                      out);
 }
 
-template void factor_numeric<double>(const FactorArgs<double>&, const DeviceFactorPlan&, const double*, int64_t,
+template void factor_numeric<double>(const FactorArgs<double>&, const DeviceFactorPlan&, const double*, int64_t, int,
                                      hipStream_t);
-template void factor_numeric<float>(const FactorArgs<float>&, const DeviceFactorPlan&, const double*, int64_t,
+template void factor_numeric<float>(const FactorArgs<float>&, const DeviceFactorPlan&, const double*, int64_t, int,
                                     hipStream_t);
 template void invert_diag_blocks<double>(const FactorArgs<double>&, const DeviceFactorPlan&, hipStream_t);
 template void invert_diag_blocks<float>(const FactorArgs<float>&, const DeviceFactorPlan&, hipStream_t);
+template void round_panel<__half>(int64_t, const float*, __half*, int*, hipStream_t);
+template void round_panel<__hip_bfloat16>(int64_t, const float*, __hip_bfloat16*, int*, hipStream_t);
 template void reciprocal<double>(int, const double*, double*, hipStream_t);
 template void reciprocal<float>(int, const float*, float*, hipStream_t);
 template void count_positive<double>(int, const double*, int*, hipStream_t);

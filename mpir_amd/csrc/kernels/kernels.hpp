@@ -1,6 +1,8 @@
 // Launch interfaces of the HIP kernels (gfx950).
 #pragma once
 
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -45,15 +47,24 @@ struct DeviceFactorPlan {
   const int64_t* val_dst = nullptr;
   std::vector<int32_t> h_small_ptr, h_big_ptr, h_asm_ptr, h_level_step_ptr, h_pan_ptr, h_upd_ptr;
   std::vector<int32_t> h_pan_aug_ptr, h_upd_aug_ptr;
+  // (s, column block, row chunk) tasks over the inverse rows of big fronts
+  const int32_t* inv_task = nullptr;
+  int n_inv_task = 0;
 };
 
+// rounding: storage format of the factor's working values (0 as T, 1 FP16,
+// 2 BF16; only for T = float). Leaves W = L_11^{-T} below big panels.
 template <class T>
 void factor_numeric(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const double* ax_scaled, int64_t nnz,
-                    hipStream_t st);
-// Recomputes only the rows L_11^{-T} D^{-1} below each big front's panel from
-// a factor already in place (L in the panel, D and Dinv set).
+                    int rounding, hipStream_t st);
+// Recomputes only the rows W = L_11^{-T} below each big front's panel from a
+// factor already in place (L in the panel, D and Dinv set).
 template <class T>
 void invert_diag_blocks(const FactorArgs<T>& a, const DeviceFactorPlan& dp, hipStream_t st);
+// The whole panel pool rounded to a 16-bit factor format; *nonfinite
+// receives the number of entries outside the format's range.
+template <class H>
+void round_panel(int64_t n, const float* src, H* dst, int* nonfinite, hipStream_t st);
 // y = 1 / x elementwise
 template <class T>
 void reciprocal(int n, const T* x, T* y, hipStream_t st);
@@ -62,8 +73,8 @@ void count_positive(int n, const T* D, int* out, hipStream_t st);
 
 // ---------------------------------------------------------------- solves
 // Device view of the factor for the triangular solves (factor storage TF,
-// arithmetic in TX).
-template <class TF>
+// D^{-1} in TD, arithmetic in TX).
+template <class TF, class TD>
 struct SolveArgs {
   const int32_t* first;
   const int32_t* nrow;
@@ -73,9 +84,8 @@ struct SolveArgs {
   const int64_t* panel_off;
   const int32_t* child_ptr;
   const int32_t* child_list;
-  const TF* panel;
-  const TF* D;
-  const TF* Dinv;
+  const TF* panel;  // L, and W = L_11^{-T} below the panels of big fronts
+  const TD* Dinv;
 };
 
 struct DeviceSolvePlan {
@@ -97,8 +107,8 @@ struct DeviceSolvePlan {
 
 // x (length n, permuted order) is overwritten by (L D L^T)^{-1} x.
 // uvec: scratch of sum(nrow) entries of TX.
-template <class TF, class TX>
-void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st);
+template <class TF, class TD, class TX>
+void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st);
 
 // ---------------------------------------------------------------- vectors / SpMV
 struct CsrDev {

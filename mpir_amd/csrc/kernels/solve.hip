@@ -11,11 +11,15 @@
 // gathers x on R_s (all ancestors, already final) and solves its block
 // transposed. Fronts of one level are independent.
 //
-// Big fronts hold Z = L_11^{-T} D^{-1} (upper triangular) below their panel,
+// Big fronts hold W = L_11^{-T} (unit upper triangular) below their panel,
 // so their solves are matrix-vector products without a dependency chain:
-//   forward   x_c = Z^T v_c  (= D^{-1} L_11^{-1} v_c),  u = v_R - L_21 D x_c
-//   backward  x_c = Z D (x_c - L_21^T x_R)          (= L_11^{-T} (...))
+//   forward   y_c = W^T v_c  (= L_11^{-1} v_c),  u = v_R - L_21 y_c
+//   backward  x_c = W (D^{-1} y_c - L_21^T x_R)  (= L_11^{-T} (...))
 // Every sum runs in a fixed order, so repeated solves agree bit for bit.
+// The factor is stored in TF (double, float, half or bfloat16), D^{-1} in
+// TD (TF, or float for the 16-bit factors); products and sums run in TX.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -28,8 +32,8 @@ namespace mpir {
 namespace {
 
 // ---------------------------------------------------------------- small fronts
-template <class TF, class TX>
-__global__ void __launch_bounds__(64) k_fwd_small(SolveArgs<TF> a, const int32_t* __restrict__ list, TX* __restrict__ x,
+template <class TF, class TD, class TX>
+__global__ void __launch_bounds__(64) k_fwd_small(SolveArgs<TF, TD> a, const int32_t* __restrict__ list, TX* __restrict__ x,
                                                   TX* __restrict__ uvec) {
   __shared__ TX v[kSmallFront];
   const int s = list[blockIdx.x];
@@ -56,8 +60,8 @@ __global__ void __launch_bounds__(64) k_fwd_small(SolveArgs<TF> a, const int32_t
   else if (lane < m) uvec[a.rows_off[s] + (lane - nc)] = v[lane];
 }
 
-template <class TF, class TX>
-__global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF> a, const int32_t* __restrict__ list, TX* __restrict__ x) {
+template <class TF, class TD, class TX>
+__global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF, TD> a, const int32_t* __restrict__ list, TX* __restrict__ x) {
   __shared__ TX w[kSmallFront];
   const int s = list[blockIdx.x];
   const int lane = threadIdx.x;
@@ -161,10 +165,10 @@ __device__ __forceinline__ void column_dots(const TF* __restrict__ base, int64_t
 }
 
 // Forward, rows [h * kXcRows, ...) of column block j of front s: partial sums
-// of xf[col] = sum_{i <= col} Z[i, col] v_i over the block's columns (Z is
+// of y[col] = sum_{i <= col} W[i, col] v_i over the block's columns (W is
 // zero below its diagonal, so whole 64-row tiles are summed).
-template <class TF, class TX>
-__global__ void __launch_bounds__(256) k_fwd_xc(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+template <class TF, class TD, class TX>
+__global__ void __launch_bounds__(256) k_fwd_xc(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
                                                 const int64_t* __restrict__ xfp_off, const TX* __restrict__ x,
                                                 TX* __restrict__ xfp) {
   __shared__ TX vs[kXcRows];
@@ -185,7 +189,8 @@ __global__ void __launch_bounds__(256) k_fwd_xc(SolveArgs<TF> a, const int32_t* 
   column_dots<TF, TX>(base, ld, bw, len, vs, xfp + xfp_off[s] + (static_cast<int64_t>(j) * nhx + h) * kRB);
 }
 
-// Column c of a big front after the forward pass: the sum of its partials.
+// y_c of column c of a big front after the forward pass: the sum of its
+// partials.
 template <class TX>
 __device__ __forceinline__ TX forward_value(const TX* __restrict__ xfp_s, int nc, int c) {
   const int nhx = (nc + kXcRows - 1) / kXcRows;
@@ -231,9 +236,9 @@ __device__ __forceinline__ TX rowblock_gemv(const TF* __restrict__ M, int64_t ld
   return (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
-// Forward, 64-row block b of R_s: u = v_R - L_21 D xf.
-template <class TF, class TX>
-__global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+// Forward, 64-row block b of R_s: u = v_R - L_21 y.
+template <class TF, class TD, class TX>
+__global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
                                                const int64_t* __restrict__ xfp_off, const TX* __restrict__ xfp,
                                                TX* __restrict__ uvec) {
   __shared__ TX ys[kChunk];
@@ -247,9 +252,7 @@ __global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF> a, const int32_t* _
   const int rows = min(kRB, nr - rb0);
   const TF* M = a.panel + a.panel_off[s] + nc + rb0;
   const TX* xs = xfp + xfp_off[s];
-  const TF* D = a.D + f;
-  const TX sum = rowblock_gemv<TF, TX>(M, ld, rows, 0, nc,
-                                       [=](int c) { return static_cast<TX>(D[c]) * forward_value(xs, nc, c); }, ys, red);
+  const TX sum = rowblock_gemv<TF, TX>(M, ld, rows, 0, nc, [=](int c) { return forward_value(xs, nc, c); }, ys, red);
   const int lane = threadIdx.x;
   if (lane < rows) {
     TX* u = uvec + a.rows_off[s] + rb0;
@@ -260,8 +263,8 @@ __global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF> a, const int32_t* _
 // Backward, rows below the columns of a front (R rows, all final): partial
 // sums part[(j * nch + c) * 64 + t] = sum over the chunk's rows r of
 // L[nc + r, 64 j + t] x[R_r].
-template <class TF, class TX>
-__global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+template <class TF, class TD, class TX>
+__global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
                                                    const int64_t* __restrict__ part_off, const TX* __restrict__ x,
                                                    TX* __restrict__ part) {
   __shared__ TX xs[kRpartRows];
@@ -283,9 +286,9 @@ __global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF> a, const int32_
 }
 
 // Backward, 64-row block i of the columns of front s:
-// x_r = sum_{c >= 64 i} Z[r, c] D_c z_c, z = xf - (partial sums of L_21^T x_R).
-template <class TF, class TX>
-__global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF> a, const int32_t* __restrict__ tasks,
+// x_r = sum_{c >= 64 i} W[r, c] z_c, z = D^{-1} y - (partial sums of L_21^T x_R).
+template <class TF, class TD, class TX>
+__global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
                                                 const int64_t* __restrict__ part_off, const TX* __restrict__ part,
                                                 const int64_t* __restrict__ xfp_off, const TX* __restrict__ xfp,
                                                 TX* __restrict__ x) {
@@ -303,15 +306,15 @@ __global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF> a, const int32_t* 
   const TF* M = a.panel + a.panel_off[s] + m + r0;
   const TX* xs = xfp + xfp_off[s];
   const TX* ps = part + (nch > 0 ? part_off[s] : 0);
-  const TF* D = a.D + f;
+  const TD* Dinv = a.Dinv + f;
   const TX sum = rowblock_gemv<TF, TX>(
       M, ld, rows, r0, nc,
       [=](int c) {
-        TX z = forward_value(xs, nc, c);
+        const TX z = static_cast<TX>(Dinv[c]) * forward_value(xs, nc, c);
         const TX* pp = ps + static_cast<int64_t>(c / kRB) * nch * kRB + (c % kRB);
         TX rs = TX(0);
         for (int h = 0; h < nch; ++h) rs += pp[static_cast<int64_t>(h) * kRB];
-        return static_cast<TX>(D[c]) * (z - rs);
+        return z - rs;
       },
       ys, red);
   const int lane = threadIdx.x;
@@ -320,46 +323,53 @@ __global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF> a, const int32_t* 
 
 }  // namespace
 
-template <class TF, class TX>
-void ldl_solve(const SolveArgs<TF>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st) {
+template <class TF, class TD, class TX>
+void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st) {
   TX* xfp = reinterpret_cast<TX*>(dp.xfp);
   TX* part = reinterpret_cast<TX*>(dp.part);
   // forward, leaves to root; D^{-1} fused
   for (int l = 0; l < dp.nlevels; ++l) {
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
-      hipLaunchKernelGGL((k_fwd_small<TF, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x, uvec);
+      hipLaunchKernelGGL((k_fwd_small<TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x, uvec);
     const int g0 = dp.h_gather_ptr[l], g1 = dp.h_gather_ptr[l + 1];
     if (g1 > g0)
       hipLaunchKernelGGL((k_fwd_gather<TX>), dim3(g1 - g0), dim3(64), 0, st, a.first, a.nrow, a.rows_off, a.relidx,
                          dp.gather_task + 2 * g0, dp.gather_child_ptr + g0, dp.gather_child, x, uvec);
     const int c0 = dp.h_xc_ptr[l], c1 = dp.h_xc_ptr[l + 1];
     if (c1 > c0)
-      hipLaunchKernelGGL((k_fwd_xc<TF, TX>), dim3(c1 - c0), dim3(256), 0, st, a, dp.xc_task + 3 * c0, dp.xfp_off, x,
+      hipLaunchKernelGGL((k_fwd_xc<TF, TD, TX>), dim3(c1 - c0), dim3(256), 0, st, a, dp.xc_task + 3 * c0, dp.xfp_off, x,
                          xfp);
     const int u0 = dp.h_u_ptr[l], u1 = dp.h_u_ptr[l + 1];
     if (u1 > u0)
-      hipLaunchKernelGGL((k_fwd_u<TF, TX>), dim3(u1 - u0), dim3(256), 0, st, a, dp.u_task + 2 * u0, dp.xfp_off, xfp,
+      hipLaunchKernelGGL((k_fwd_u<TF, TD, TX>), dim3(u1 - u0), dim3(256), 0, st, a, dp.u_task + 2 * u0, dp.xfp_off, xfp,
                          uvec);
   }
   // backward, root to leaves
   for (int l = dp.nlevels - 1; l >= 0; --l) {
     const int p0 = dp.h_rpart_ptr[l], p1 = dp.h_rpart_ptr[l + 1];
     if (p1 > p0)
-      hipLaunchKernelGGL((k_bwd_rpart<TF, TX>), dim3(p1 - p0), dim3(256), 0, st, a, dp.rpart_task + 3 * p0,
+      hipLaunchKernelGGL((k_bwd_rpart<TF, TD, TX>), dim3(p1 - p0), dim3(256), 0, st, a, dp.rpart_task + 3 * p0,
                          dp.part_off, x, part);
     const int t0 = dp.h_bwd_ptr[l], t1 = dp.h_bwd_ptr[l + 1];
     if (t1 > t0)
-      hipLaunchKernelGGL((k_bwd_xc<TF, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.bwd_task + 2 * t0, dp.part_off,
+      hipLaunchKernelGGL((k_bwd_xc<TF, TD, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.bwd_task + 2 * t0, dp.part_off,
                          part, dp.xfp_off, xfp, x);
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
-    if (s1 > s0) hipLaunchKernelGGL((k_bwd_small<TF, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x);
+    if (s1 > s0) hipLaunchKernelGGL((k_bwd_small<TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x);
   }
   MPIR_HIP(hipGetLastError());
 }
 
-template void ldl_solve<double, double>(const SolveArgs<double>&, const DeviceSolvePlan&, double*, double*, hipStream_t);
-template void ldl_solve<float, double>(const SolveArgs<float>&, const DeviceSolvePlan&, double*, double*, hipStream_t);
-template void ldl_solve<float, float>(const SolveArgs<float>&, const DeviceSolvePlan&, float*, float*, hipStream_t);
+template void ldl_solve<double, double, double>(const SolveArgs<double, double>&, const DeviceSolvePlan&, double*,
+                                                 double*, hipStream_t);
+template void ldl_solve<float, float, double>(const SolveArgs<float, float>&, const DeviceSolvePlan&, double*, double*,
+                                              hipStream_t);
+template void ldl_solve<float, float, float>(const SolveArgs<float, float>&, const DeviceSolvePlan&, float*, float*,
+                                             hipStream_t);
+template void ldl_solve<__half, float, double>(const SolveArgs<__half, float>&, const DeviceSolvePlan&, double*,
+                                               double*, hipStream_t);
+template void ldl_solve<__hip_bfloat16, float, double>(const SolveArgs<__hip_bfloat16, float>&,
+                                                       const DeviceSolvePlan&, double*, double*, hipStream_t);
 
 }  // namespace mpir

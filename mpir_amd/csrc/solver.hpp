@@ -17,6 +17,13 @@ namespace mpir {
 
 enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1, kOrderNd = 2 };
 
+// scaling of K before the factorization
+constexpr int kScaleNone = 0;
+constexpr int kScaleDiag = 1;  // the reference's s_i = 1/sqrt|k_ii| (gmres_ir.hpp:347-363)
+constexpr int kScaleRuiz = 2;  // symmetric Ruiz equilibration of the max-norms
+constexpr int kRuizSweeps = 10;
+constexpr double kRuizTol = 1e-2;
+
 struct SolveStats {
   int32_t n = 0;
   int64_t nnz = 0;
@@ -33,6 +40,7 @@ struct SolveStats {
   double final_rel_residual = 0;  // ||r||_inf / ||b_s||_inf in the scaled system
   double analyze_ms = 0, factor_ms = 0, solve_ms = 0;
   int64_t device_bytes = 0;
+  int32_t ruiz_sweeps = 0;  // Ruiz sweeps of the last factorization
 };
 
 class GmresLdl {
@@ -48,7 +56,11 @@ class GmresLdl {
   void set_ordering(int o) { ordering_ = o; }
   // diagonal scaling of Compute (gmres_ir.hpp:347-363); off for the bare
   // QDLDL_factor drop-in
-  void set_scaling(bool on) { scaling_ = on; }
+  // 0 none, 1 the reference's diagonal scaling (default), 2 Ruiz
+  void set_scaling(int mode) {
+    MPIR_CHECK(mode >= kScaleNone && mode <= kScaleRuiz, "scaling mode must be 0, 1 or 2");
+    scaling_ = mode;
+  }
   void set_relax(bool on) { relax_ = on; }
 
   // Symbolic phase on the pattern of the upper CSC (reusable across values).
@@ -93,20 +105,23 @@ class GmresLdl {
                          d_upd_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(),
                          d_panel_.as<T>(), d_upool_.as<T>(), d_D_.as<T>(), d_Dinv_.as<T>(), d_err_.as<int>()};
   }
-  template <class T>
-  SolveArgs<T> solve_args() {
-    return SolveArgs<T>{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(),
-                        d_rows_.as<int32_t>(), d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(),
-                        d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), d_panel_.as<T>(), d_D_.as<T>(),
-                        d_Dinv_.as<T>()};
+  template <class TF, class TD>
+  SolveArgs<TF, TD> solve_args(void* panel) {
+    return SolveArgs<TF, TD>{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(),
+                             d_rows_.as<int32_t>(), d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(),
+                             d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(), static_cast<const TF*>(panel),
+                             d_Dinv_.as<TD>()};
   }
+  // storage rounding of the factor's working values (see factor_numeric)
+  int rounding() const { return uf_ == kF16 ? 1 : uf_ == kBF16 ? 2 : 0; }
+  void round_factor16();  // the 16-bit copy of the factor (and its range check)
   void alloc_numeric();
 
   int uf_, uw_, ur_;
   int64_t max_ir_ = 10, max_gmres_ = 10;
   double tol_ = 1e-10;
   int ordering_ = kOrderAmd;
-  bool scaling_ = true;
+  int scaling_ = kScaleDiag;
   bool relax_ = true;     // relaxed supernodes (AMD / ND orderings only)
   bool relaxed_ = false;  // the current plan uses them
   std::vector<int32_t> Cp_, Ci_, parent_, Lnz_;  // pattern of P K P^T, etree
@@ -128,7 +143,7 @@ class GmresLdl {
   DevBuf d_csr_p_, d_csr_j_, d_csr_src_, d_csr_x_;
   DevBuf d_entry_col_, d_entry_row_, d_diag_pos_, d_perm_;
   // numeric
-  DevBuf d_ax_, d_axs_, d_ad_, d_adp_, d_panel_, d_upool_, d_D_, d_Dinv_, d_err_, d_cnt_;
+  DevBuf d_ax_, d_axs_, d_ad_, d_adp_, d_panel_, d_panel16_, d_upool_, d_D_, d_Dinv_, d_err_, d_cnt_, d_nonfinite_;
   DevBuf d_uvec_, d_scratch_;
   // GMRES workspace
   DevBuf d_V_, d_w_, d_b_, d_x_, d_r_, d_d_, d_red_, d_small_;

@@ -211,3 +211,54 @@ def test_cpp_dropin_program(gpu):
     proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     print(proc.stdout)
     assert proc.returncode == 0, proc.stdout + proc.stderr
+
+
+@pytest.mark.parametrize("uf", [np.float16, "bf16"])
+@pytest.mark.parametrize("scaling", [mpir_amd.SCALE_DIAG, mpir_amd.SCALE_RUIZ])
+def test_16bit_factor_gmres_ir(gpu, oracle, uf, scaling):
+    # the precision form: L (and the solve operands) held in FP16 / BF16,
+    # FP32 accumulation, FP64 residual; on the D = 1e-2 scale GMRES-IR
+    # reaches the FP64 answer and the factor keeps the inertia of K
+    P = mpir_amd.kkt_config(1, k=16, d_value=1e-2)
+    s = mpir_amd.GmresLDLIR(uf, ordering=mpir_amd.ORDER_ND, scaling=scaling)
+    s.SetTolerance(1e-12)
+    s.SetMaxGmresIterations(50)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    st = s.stats()
+    assert st["positive_pivots"] == P.n1
+    if scaling == mpir_amd.SCALE_RUIZ:
+        assert 1 <= st["ruiz_sweeps"] <= 10
+    x = s.Solve(P.b)
+    assert backward_error(oracle, P.Ap, P.Ai, P.Ax, x, P.b) < 1e-13, s.stats()
+    s64 = mpir_amd.GmresLDLIR(np.float64, ordering=mpir_amd.ORDER_ND)
+    s64.Compute(P.Ap, P.Ai, P.Ax)
+    x64 = s64.Solve(P.b)
+    np.testing.assert_allclose(x, x64, rtol=0, atol=1e-10 * np.abs(x64).max())
+    # a 16-bit factor is coarser than FP32: more Arnoldi steps, same answer
+    s32 = mpir_amd.GmresLDLIR(np.float32, ordering=mpir_amd.ORDER_ND, scaling=scaling)
+    s32.SetTolerance(1e-12)
+    s32.Compute(P.Ap, P.Ai, P.Ax)
+    s32.Solve(P.b)
+    assert s.stats()["gmres_iterations"] >= s32.stats()["gmres_iterations"]
+
+
+def test_fp16_factor_overflow_is_reported(gpu):
+    # D = 1e-6: the factor leaves the FP16 range; the library says so
+    P = mpir_amd.kkt_config(1, k=12, d_value=1e-6)
+    s = mpir_amd.GmresLDLIR(np.float16, ordering=mpir_amd.ORDER_ND)
+    with pytest.raises(mpir_amd.MpirError, match="overflow the f16 range"):
+        s.Compute(P.Ap, P.Ai, P.Ax)
+
+
+def test_ruiz_scaling_keeps_the_solution(gpu, oracle):
+    # equilibration changes the scaled system, not the answer
+    P = mpir_amd.kkt_config(1, k=12)
+    xs = []
+    for scaling in (mpir_amd.SCALE_NONE, mpir_amd.SCALE_DIAG, mpir_amd.SCALE_RUIZ):
+        s = mpir_amd.GmresLDLIR(np.float32, ordering=mpir_amd.ORDER_ND, scaling=scaling)
+        s.SetTolerance(1e-12)
+        s.Compute(P.Ap, P.Ai, P.Ax)
+        xs.append(s.Solve(P.b))
+        assert backward_error(oracle, P.Ap, P.Ai, P.Ax, xs[-1], P.b) < 1e-13
+    for x in xs[1:]:
+        np.testing.assert_allclose(x, xs[0], rtol=0, atol=1e-9 * np.abs(xs[0]).max())
