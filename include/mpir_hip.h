@@ -98,6 +98,33 @@ int64_t mpir_qdldl_solve_f32_f64_i64(int64_t n, const int64_t* Lp, const int64_t
 int64_t mpir_qdldl_solve_f32_f32_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const float* Lx,
                                      const float* Dinv, float* x);
 
+/* ------------------------------------------------------------------ LUIR
+ * Replaces LUIR_QDLDL<FACTPREC, double, double, double, INT> and its Stats
+ * (include/qdldl-luir/luir.hpp:33-241, stats.hpp:6-114): QDLDL etree and
+ * factor of the matrix as given (upper CSC, values rounded to FACTPREC),
+ * x0 = a FACTPREC solve, then classic iterative refinement with the residual
+ * in double, each correction solved in FACTPREC, stopping when
+ * ||d||/||x|| <= 10 eps(double) or ||d||/||d_prev|| >= 0.5, at most maxiter
+ * steps (maxiter = 0: one solve in double). Factor and refinement run on the
+ * GPU. Returns 0, -1 / -2 where QDLDL_etree or QDLDL_factor would, or a
+ * negative status; the reference exits the process instead. */
+typedef struct mpir_luir_stats {
+  int32_t n;
+  int64_t nnz;                  /* entries of triu(A) */
+  int64_t lnz;                  /* entries of L below the diagonal */
+  int32_t num_refinement_steps;
+  double relative_residual;     /* ||b - A x||_inf / ||b||_inf, residual rounded to FACTPREC */
+  double time_sym, time_num_fact, time_solve, time_ref;  /* seconds */
+} mpir_luir_stats;
+int32_t mpir_luir_qdldl_f64_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* b,
+                                double* x, mpir_luir_stats* stats, int32_t maxiter);
+int32_t mpir_luir_qdldl_f32_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* b,
+                                double* x, mpir_luir_stats* stats, int32_t maxiter);
+int64_t mpir_luir_qdldl_f64_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const double* Ax, const double* b,
+                                double* x, mpir_luir_stats* stats, int32_t maxiter);
+int64_t mpir_luir_qdldl_f32_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const double* Ax, const double* b,
+                                double* x, mpir_luir_stats* stats, int32_t maxiter);
+
 /* ------------------------------------------------------------------ GMRES-IR
  * Replaces GmresLDLIR<UF,UW,UR> (include/MPIR/gmres_ir.hpp:90-315) in its
  * LDL^T form (docs/Design/SoftDetailedDes/MIS.tex:505-568). */

@@ -111,6 +111,30 @@ def QDLDL_solve(n, Lp, Li, Lx, Dinv, x) -> None:
     _check(int(fn(int(n), _ptr(Lp_), _ptr(Li_), _ptr(Lx_), _ptr(Di_), _ptr(x_))), "QDLDL_solve")
 
 
+# ---------------------------------------------------------------------- LUIR
+def LUIR_QDLDL(Ap, Ai, Ax, b, FACTPREC=np.float32, maxiter: int = 10):
+    """LUIR_QDLDL<FACTPREC, double, double, double, INT> (include/qdldl-luir/
+    luir.hpp:33-241) on the GPU: QDLDL factor of the matrix as given (upper
+    CSC), classic iterative refinement with a double residual. Returns
+    (x, stats) with stats the fields of the reference's Stats (stats.hpp)."""
+    it = np.dtype(np.asarray(Ap).dtype)
+    it = np.dtype(np.int64) if it == np.int64 else np.dtype(np.int32)
+    ft = {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32"}.get(np.dtype(FACTPREC))
+    if ft is None:
+        raise TypeError("FACTPREC must be float64 or float32")
+    Ap_, Ai_ = _req(Ap, it, "Ap"), _req(Ai, it, "Ai")
+    Ax_ = np.ascontiguousarray(Ax, dtype=np.float64)
+    b_ = np.ascontiguousarray(b, dtype=np.float64)
+    n = len(Ap_) - 1
+    x = np.zeros(n)
+    st = _lib.LuirStats()
+    fn = getattr(load(), f"mpir_luir_qdldl_{ft}_{'i64' if it == np.int64 else 'i32'}")
+    rc = int(fn(n, _ptr(Ap_), _ptr(Ai_), _ptr(Ax_), _ptr(b_), _ptr(x), C.byref(st), int(maxiter)))
+    if rc < 0:
+        raise MpirError(rc, "LUIR_QDLDL")
+    return x, st.as_dict()
+
+
 # ---------------------------------------------------------------------- GMRES-IR
 class GmresLDLIR:
     """GmresLDLIR<UF,UW,UR> (gmres_ir.hpp:90-315), LDL^T preconditioner, on the GPU.
@@ -313,7 +337,7 @@ def residual_upper(Ap, Ai, Ax, x, b) -> np.ndarray:
 
 
 __all__ = [
-    "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
+    "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "LUIR_QDLDL", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
     "amd_order", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
     "ORDER_AMD", "ORDER_ND", "SCALE_NONE", "SCALE_DIAG", "SCALE_RUIZ",
 ]

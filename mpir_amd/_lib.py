@@ -25,6 +25,16 @@ class GmresStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class LuirStats(C.Structure):
+    _fields_ = [
+        ("n", i32), ("nnz", i64), ("lnz", i64), ("num_refinement_steps", i32), ("relative_residual", f64),
+        ("time_sym", f64), ("time_num_fact", f64), ("time_solve", f64), ("time_ref", f64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "mpir_last_error": (C.c_char_p, []),
@@ -42,6 +52,10 @@ SIGNATURES = {
     "mpir_qdldl_solve_f64_f64_i64": (i64, [i64, P, P, P, P, P]),
     "mpir_qdldl_solve_f32_f64_i64": (i64, [i64, P, P, P, P, P]),
     "mpir_qdldl_solve_f32_f32_i64": (i64, [i64, P, P, P, P, P]),
+    "mpir_luir_qdldl_f64_i32": (i32, [i32, P, P, P, P, P, P, i32]),
+    "mpir_luir_qdldl_f32_i32": (i32, [i32, P, P, P, P, P, P, i32]),
+    "mpir_luir_qdldl_f64_i64": (i64, [i64, P, P, P, P, P, P, i32]),
+    "mpir_luir_qdldl_f32_i64": (i64, [i64, P, P, P, P, P, P, i32]),
     "mpir_gmres_ldl_create": (C.c_int, [C.c_int, C.c_int, C.c_int, P]),
     "mpir_gmres_ldl_destroy": (None, [P]),
     "mpir_gmres_ldl_set_max_ir": (C.c_int, [P, i64]),

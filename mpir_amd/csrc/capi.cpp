@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <chrono>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -49,6 +50,7 @@ struct QdldlEngine {
 };
 std::mutex g_engine_mu;
 QdldlEngine g_factor_engine[2];
+QdldlEngine g_luir_engine[2];
 QdldlEngine g_solve_engine[2];
 
 uint64_t fnv1a(const void* data, size_t bytes, uint64_t h = 1469598103934665603ull) {
@@ -147,6 +149,67 @@ int solve_impl(int32_t n, const int64_t* Lp, const int32_t* Li, const TF* Lx, co
   MPIR_HIP(hipMemcpyAsync(xh.data(), xd.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
   MPIR_HIP(hipStreamSynchronize(st));
   for (int32_t i = 0; i < n; ++i) x[i] = static_cast<TX>(xh[i]);
+  return 0;
+}
+
+// LUIR_QDLDL<FACTPREC, double, double, double, INT> (luir.hpp:33-241): the
+// matrix as given (natural order, no scaling), factored on the GPU, then
+// classic iterative refinement on the GPU.
+template <class T>
+int luir_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* b, double* x,
+              mpir_luir_stats* stats, int maxiter) {
+  MPIR_CHECK(n >= 1 && Ap && Ai && Ax && b && x, "bad LUIR arguments");
+  const int prec = std::is_same<T, double>::value ? kF64 : kF32;
+  std::lock_guard<std::mutex> lock(g_engine_mu);
+  QdldlEngine& E = g_luir_engine[prec];
+  if (!E.solver) {
+    E.solver = std::make_unique<GmresLdl>(prec, kF64, kF64);
+    E.solver->set_ordering(kOrderNatural);
+    E.solver->set_scaling(kScaleNone);
+  }
+  const auto ts = std::chrono::steady_clock::now();
+  // QDLDL_etree's checks and messages (luir.hpp:86-102 report and exit)
+  std::vector<int32_t> work(n), lnz(n), par(n);
+  const int64_t sum = etree_counts(n, Ap, Ai, work.data(), lnz.data(), par.data(), true);
+  if (sum == -1) {
+    std::fprintf(stderr, "!!! The input matrix is not in upper triangular form\n");
+    return -1;
+  }
+  if (sum == -2) {
+    std::fprintf(stderr, "!!! total nonzeros in L below diagonal overflows in INT\n");
+    return -2;
+  }
+  const int64_t nnz = Ap[n];
+  uint64_t h = fnv1a(Ap, sizeof(int32_t) * (n + 1));
+  h = fnv1a(Ai, sizeof(int32_t) * nnz, h);
+  if (E.n != n || E.pattern_hash != h || !E.solver->analyzed()) {
+    E.n = -1;
+    E.solver->analyze(n, Ap, Ai);
+    E.n = n;
+    E.pattern_hash = h;
+  }
+  const double t_sym = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+  const auto tf = std::chrono::steady_clock::now();
+  try {
+    E.solver->factor(Ax, false);
+  } catch (const Error& e) {
+    if (e.code == kErrNumeric) return -1;  // QDLDL: exact zero pivot
+    throw;
+  }
+  const double t_fact = std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
+  LuirResult r;
+  E.solver->luir(b, x, maxiter, &r);
+  if (stats) {
+    stats->n = n;
+    stats->nnz = nnz;
+    stats->lnz = sum;
+    stats->num_refinement_steps = maxiter == 0 ? 0 : r.steps;
+    stats->relative_residual = r.rel_residual;
+    stats->time_sym = t_sym;
+    stats->time_num_fact = t_fact;
+    stats->time_solve = r.time_solve * 1e-3;
+    stats->time_ref = r.time_ref * 1e-3;
+  }
   return 0;
 }
 
@@ -294,6 +357,24 @@ MPIR_FACTOR_I64(f32, float)
       return solve_impl<TF, TX>(n32, Lp, li.data(), Lx, Dinv, x);                                                    \
     });                                                                                                              \
   }
+#define MPIR_LUIR(SUF, TF)                                                                                     \
+  int32_t mpir_luir_qdldl_##SUF##_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax,          \
+                                      const double* b, double* x, mpir_luir_stats* stats, int32_t maxiter) {      \
+    return guarded([&] { return luir_impl<TF>(n, Ap, Ai, Ax, b, x, stats, maxiter); });                           \
+  }                                                                                                              \
+  int64_t mpir_luir_qdldl_##SUF##_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const double* Ax,          \
+                                      const double* b, double* x, mpir_luir_stats* stats, int32_t maxiter) {      \
+    return guarded([&] {                                                                                         \
+      int32_t n32;                                                                                               \
+      MPIR_CHECK(narrow(n, &n32) && n32 >= 1 && Ap && Ai, "bad LUIR arguments");                                 \
+      std::vector<int32_t> p = narrow_vec(Ap, n + 1);                                                            \
+      std::vector<int32_t> i = narrow_vec(Ai, Ap[n]);                                                            \
+      return luir_impl<TF>(n32, p.data(), i.data(), Ax, b, x, stats, maxiter);                                    \
+    });                                                                                                          \
+  }
+MPIR_LUIR(f64, double)
+MPIR_LUIR(f32, float)
+
 MPIR_SOLVE_I32(f64_f64, double, double)
 MPIR_SOLVE_I32(f32_f64, float, double)
 MPIR_SOLVE_I32(f32_f32, float, float)

@@ -43,6 +43,13 @@ struct SolveStats {
   int32_t ruiz_sweeps = 0;  // Ruiz sweeps of the last factorization
 };
 
+// Outcome of LUIR_QDLDL (include/qdldl-luir/luir.hpp:33-241, stats.hpp)
+struct LuirResult {
+  int32_t steps = 0;          // num_refinement_steps_
+  double rel_residual = 0.0;  // relative_residual_
+  double time_solve = 0.0, time_ref = 0.0;  // ms
+};
+
 class GmresLdl {
  public:
   GmresLdl(int uf, int uw, int ur);
@@ -78,6 +85,9 @@ class GmresLdl {
   int32_t n() const { return n_; }
   const std::vector<int32_t>& perm() const { return perm_; }
 
+  // classic iterative refinement of LUIR_QDLDL on the current factor
+  // (natural ordering, no scaling); b, x host arrays
+  void luir(const double* b, double* x, int maxiter, LuirResult* out);
   // building blocks exposed for tests and the kernel-level C entry points
   void precond_apply_dev(double* x);  // x := (L D L^T)^{-1} x, permuted order
   void spmv_dev(const double* x, double* y);  // y = K_s x, permuted, scaled
@@ -115,6 +125,8 @@ class GmresLdl {
   // storage rounding of the factor's working values (see factor_numeric)
   int rounding() const { return uf_ == kF16 ? 1 : uf_ == kBF16 ? 2 : 0; }
   void round_factor16();  // the 16-bit copy of the factor (and its range check)
+  template <class TF>
+  void luir_impl(const double* b, double* x, int maxiter, LuirResult* out);
   void alloc_numeric();
 
   int uf_, uw_, ur_;

@@ -14,6 +14,12 @@
  *   or_solve_*        - oracle/_ref bit-exact; dense solve on tiny inputs
  *   or_matmul_upper   - oracle/_ref bit-exact; exact on integer inputs
  *   or_dnrm2/infnrm   - ops.hpp doctest known answers; oracle/_ref
+ *   or_luir_*         - LUIR_QDLDL (include/qdldl-luir/luir.hpp:33-241) on the
+ *                       pinned factor / solve / residual above; the loop
+ *                       itself is pinned by closed forms (an exact FP64
+ *                       factor stops after one correction; K = I) and by
+ *                       outcome (luir.hpp does not build here: it includes
+ *                       the absent QD library)
  *   or_gmres_ir       - closed forms (K = I, exact FP64 factor: one step with
  *                       happy breakdown), Arnoldi invariants, backward error.
  *                       Iteration-level parity with the reference's LDL^T
@@ -368,6 +374,101 @@ static void or_permute(int32_t n, const int32_t* Ap, const int32_t* Ai, const do
 int32_t or_gmres_ir(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const int32_t* perm, int32_t uf,
                     int64_t ir_iter, int64_t gmres_iter, double tol, const double* b, double* x, double* info);
 
+
+/* LUIR_QDLDL<FACTPREC, double, double, double, int32_t>,
+ * include/qdldl-luir/luir.hpp:33-241: QDLDL_etree + QDLDL_factor on the
+ * matrix as given (values rounded to FACTPREC, :120-148); x0 = solve in
+ * FACTPREC (:173-178); refinement (:196-225): correction = FACTPREC(b - A x)
+ * with the residual in double (ResidualCSCSymmetric, residual.hpp:7-44),
+ * scaled by 1/||.||_inf (tblas.hpp scale: x[i] *= s), solved in FACTPREC,
+ * scaled back, x += correction in double; stop when
+ * ||d||/||x|| <= 10 eps(double) or ||d||/||d_prev|| >= 0.5. maxiter = 0 is a
+ * single solve in double (:162-170). One deviation: an exactly zero residual
+ * ends the loop (x is exact) where the reference divides by its zero norm
+ * (:203) and turns x into NaN. info: [0] refinement steps,
+ * [1] ||b - A x||_inf / ||b||_inf (residual rounded to FACTPREC first),
+ * [2] sum(Lnz). Returns QDLDL_factor's value (< 0 on failure). */
+static void or_residual_upper(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* x,
+                              const double* b, double* r) {
+  or_matmul_upper(n, Ap, Ai, Ax, x, r);
+  for (int32_t i = 0; i < n; i++) r[i] = b[i] - r[i];
+}
+
+#define OR_LUIR(NAME, TF, FACTOR, SOLVE_TF, SOLVE_TX)                                                         \
+  int32_t NAME(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* b, double* x, \
+               int32_t maxiter, double* info) {                                                              \
+    int32_t* work = (int32_t*)malloc(sizeof(int32_t) * n);                                                    \
+    int32_t* Lnz = (int32_t*)malloc(sizeof(int32_t) * n);                                                     \
+    int32_t* etree = (int32_t*)malloc(sizeof(int32_t) * n);                                                   \
+    int64_t sum = or_etree(n, Ap, Ai, work, Lnz, etree);                                                      \
+    if (sum < 0) { free(work); free(Lnz); free(etree); return (int32_t)sum; }                                 \
+    const int32_t nnz = Ap[n];                                                                                \
+    int32_t* Lp = (int32_t*)malloc(sizeof(int32_t) * (n + 1));                                                \
+    int32_t* Li = (int32_t*)malloc(sizeof(int32_t) * (sum > 0 ? sum : 1));                                   \
+    TF* Lx = (TF*)malloc(sizeof(TF) * (sum > 0 ? sum : 1));                                                   \
+    TF* D = (TF*)malloc(sizeof(TF) * n);                                                                      \
+    TF* Dinv = (TF*)malloc(sizeof(TF) * n);                                                                   \
+    TF* Af = (TF*)malloc(sizeof(TF) * (nnz > 0 ? nnz : 1));                                                   \
+    for (int32_t q = 0; q < nnz; q++) Af[q] = (TF)Ax[q];                                                      \
+    int32_t ret = FACTOR(n, Ap, Ai, Af, Lp, Li, Lx, D, Dinv, Lnz, etree);                                     \
+    info[2] = (double)sum;                                                                                    \
+    double* r = (double*)malloc(sizeof(double) * n);                                                          \
+    TF* c = (TF*)malloc(sizeof(TF) * n);                                                                      \
+    info[0] = 0;                                                                                              \
+    if (ret >= 0) {                                                                                           \
+      double bn = 0;                                                                                          \
+      for (int32_t i = 0; i < n; i++) bn = fabs(b[i]) > bn ? fabs(b[i]) : bn;                                 \
+      if (maxiter == 0) {                                                                                     \
+        for (int32_t i = 0; i < n; i++) x[i] = b[i];                                                          \
+        SOLVE_TX(n, Lp, Li, Lx, Dinv, x);                                                                     \
+      } else {                                                                                                \
+        for (int32_t i = 0; i < n; i++) c[i] = (TF)b[i];                                                      \
+        SOLVE_TF(n, Lp, Li, Lx, Dinv, c);                                                                     \
+        for (int32_t i = 0; i < n; i++) x[i] = (double)c[i];                                                  \
+        double d0 = DBL_MAX;                                                                                  \
+        for (int32_t it = 1; it <= maxiter; it++) {                                                           \
+          or_residual_upper(n, Ap, Ai, Ax, x, b, r);                                                          \
+          TF cm = 0;                                                                                          \
+          for (int32_t i = 0; i < n; i++) {                                                                   \
+            c[i] = (TF)r[i];                                                                                  \
+            TF a = c[i] < 0 ? -c[i] : c[i];                                                                   \
+            if (a > cm) cm = a;                                                                               \
+          }                                                                                                   \
+          const double nc = (double)cm;                                                                       \
+          if (nc == 0.0) { info[0] = it; break; } /* deviation: the reference divides by zero here */         \
+          for (int32_t i = 0; i < n; i++) c[i] = (TF)((double)c[i] * (1. / nc));                              \
+          SOLVE_TF(n, Lp, Li, Lx, Dinv, c);                                                                   \
+          TF dm = 0;                                                                                          \
+          for (int32_t i = 0; i < n; i++) {                                                                   \
+            c[i] = (TF)((double)c[i] * nc);                                                                   \
+            TF a = c[i] < 0 ? -c[i] : c[i];                                                                   \
+            if (a > dm) dm = a;                                                                               \
+          }                                                                                                   \
+          double xm = 0;                                                                                      \
+          for (int32_t i = 0; i < n; i++) xm = fabs(x[i]) > xm ? fabs(x[i]) : xm;                             \
+          const double dn = (double)dm;                                                                       \
+          const int stop = dn / xm <= 10 * DBL_EPSILON || dn / d0 >= 0.5;                                     \
+          info[0] = it;                                                                                       \
+          for (int32_t i = 0; i < n; i++) x[i] = (double)c[i] + x[i];                                         \
+          if (stop) break;                                                                                    \
+          d0 = dn;                                                                                            \
+        }                                                                                                     \
+      }                                                                                                       \
+      or_residual_upper(n, Ap, Ai, Ax, x, b, r);                                                              \
+      TF rm = 0;                                                                                              \
+      for (int32_t i = 0; i < n; i++) {                                                                       \
+        TF a = (TF)r[i];                                                                                      \
+        a = a < 0 ? -a : a;                                                                                   \
+        if (a > rm) rm = a;                                                                                   \
+      }                                                                                                       \
+      info[1] = (double)rm / bn;                                                                              \
+    }                                                                                                         \
+    free(work); free(Lnz); free(etree); free(Lp); free(Li); free(Lx); free(D); free(Dinv); free(Af);          \
+    free(r); free(c);                                                                                         \
+    return ret;                                                                                               \
+  }
+OR_LUIR(or_luir_f64, double, or_factor_f64, or_solve_f64_f64, or_solve_f64_f64)
+OR_LUIR(or_luir_f32, float, or_factor_f32, or_solve_f32_f32, or_solve_f32_f64)
 
 static double or_now(void) {
   struct timespec ts;

@@ -1,5 +1,6 @@
-// A reference-style caller of QDLDL_etree / QDLDL_factor / QDLDL_solve and
-// GmresLDLIR<UF,UW,UR>, compiled against include/mpir/*.hpp: the calls are
+// A reference-style caller of QDLDL_etree / QDLDL_factor / QDLDL_solve,
+// LUIR_QDLDL and GmresLDLIR<UF,UW,UR>, compiled against include/mpir/*.hpp:
+// the calls are
 // written exactly as a user of yex33/MPIR writes them (qdldl.hpp,
 // gmres_ir.hpp, luir.hpp:79-178 for the QDLDL call sequence).
 //
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "mpir/gmres_ir.hpp"
+#include "mpir/luir.hpp"
 #include "mpir/qdldl.hpp"
 
 namespace {
@@ -139,6 +141,29 @@ int main(int argc, char** argv) {
   qdldl_sequence<long, double, double>(K, 1e-12, "long/double/double");
   qdldl_sequence<int, float, double>(K, 1e-5, "int/float/double");
   qdldl_sequence<long long, float, float>(K, 1e-4, "long long/float/float");
+
+  {
+    // LUIR_QDLDL<FACTPREC, RESPREC, WORKPREC, XPREC, INT> with its Stats
+    std::vector<double> xt(K.n), x(K.n);
+    for (int i = 0; i < K.n; ++i) xt[i] = std::sin(0.11 * i);
+    const std::vector<double> b = matvec(K, xt);
+    Stats stats;
+    LUIR_QDLDL<float, double, double, double, int>(K.n, K.p.data(), K.i.data(), K.x.data(), b.data(), x.data(),
+                                                   &stats, 10);
+    char msg[160];
+    std::snprintf(msg, sizeof msg, "LUIR_QDLDL<float,...>: %d steps, relative residual %.2e < 1e-14",
+                  stats.num_refinement_steps_, stats.relative_residual_);
+    expect(stats.relative_residual_ < 1e-14 && rel_residual(K, x, b) < 1e-14 && stats.num_refinement_steps_ >= 1,
+           msg);
+    expect(stats.n_ == K.n && stats.nnz_ == static_cast<long>(K.i.size()) && stats.get_total_time() > 0,
+           "LUIR_QDLDL fills Stats");
+    std::vector<long> lp(K.p.begin(), K.p.end()), li(K.i.begin(), K.i.end());
+    std::vector<double> x2(K.n);
+    Stats s2;
+    LUIR_QDLDL<double, double, double, double, long>(K.n, lp.data(), li.data(), K.x.data(), b.data(), x2.data(),
+                                                     &s2, 0);
+    expect(s2.num_refinement_steps_ == 0 && rel_residual(K, x2, b) < 1e-12, "LUIR_QDLDL<double,...> maxiter 0");
+  }
 
   std::vector<std::size_t> Ap(K.p.begin(), K.p.end()), Ai(K.i.begin(), K.i.end());
   std::vector<double> xt(K.n);

@@ -42,6 +42,8 @@ class Oracle:
             "or_dot": (f64, [i64, P, P]),
             "or_infnrm": (f64, [i64, P]),
             "or_gmres_ir": (i32, [i32, P, P, P, P, i32, i64, i64, f64, P, P, P]),
+            "or_luir_f64": (i32, [i32, P, P, P, P, P, i32, P]),
+            "or_luir_f32": (i32, [i32, P, P, P, P, P, i32, P]),
         })
 
     def etree(self, Ap, Ai):
@@ -94,6 +96,16 @@ class Oracle:
     def infnrm(self, x):
         x = np.ascontiguousarray(x, np.float64)
         return self.lib.or_infnrm(len(x), ptr(x))
+
+    def luir(self, Ap, Ai, Ax, b, factprec=np.float32, maxiter=10):
+        """LUIR_QDLDL restated (luir.hpp:33-241): (ret, x, steps, rel_residual)."""
+        n = len(Ap) - 1
+        x = np.zeros(n)
+        info = np.zeros(3)
+        f = self.lib.or_luir_f64 if factprec == np.float64 else self.lib.or_luir_f32
+        ret = f(n, ptr(Ap), ptr(Ai), ptr(np.ascontiguousarray(Ax, np.float64)),
+                ptr(np.ascontiguousarray(b, np.float64)), ptr(x), int(maxiter), ptr(info))
+        return ret, x, int(info[0]), float(info[1])
 
     def gmres_ir(self, Ap, Ai, Ax, b, perm=None, uf=np.float64, ir_iter=10, gmres_iter=10, tol=1e-10):
         n = len(Ap) - 1

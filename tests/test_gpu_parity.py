@@ -262,3 +262,41 @@ def test_ruiz_scaling_keeps_the_solution(gpu, oracle):
         assert backward_error(oracle, P.Ap, P.Ai, P.Ax, xs[-1], P.b) < 1e-13
     for x in xs[1:]:
         np.testing.assert_allclose(x, xs[0], rtol=0, atol=1e-9 * np.abs(xs[0]).max())
+
+
+@pytest.mark.parametrize("fp", [np.float32, np.float64])
+@pytest.mark.parametrize("cfg,k", [(0, None), (1, 12)])
+def test_luir_matches_oracle(gpu, oracle, fp, cfg, k):
+    # LUIR_QDLDL on the matrix as given (pre-ordered by AMD here, as a caller
+    # of the reference would): same refinement outcome as the restatement
+    P = mpir_amd.kkt_config(cfg, k=k)
+    Cp, Ci, Cx = permute_upper(P.Ap, P.Ai, P.Ax, mpir_amd.amd_order(P.Ap, P.Ai))
+    y = np.random.default_rng(5).uniform(-1, 1, P.n)
+    b = oracle.matmul(Cp, Ci, Cx, y)
+    x, st = mpir_amd.LUIR_QDLDL(Cp, Ci, Cx, b, fp, maxiter=20)
+    ret, xo, steps_o, rr_o = oracle.luir(Cp, Ci, Cx, b, fp, maxiter=20)
+    assert ret == P.n1
+    assert st["n"] == P.n and st["nnz"] == Cp[-1] and st["lnz"] == oracle.etree(Cp, Ci)[0]
+    assert abs(st["num_refinement_steps"] - steps_o) <= 1, (st, steps_o)
+    assert st["relative_residual"] < 1e-13 and rr_o < 1e-13
+    np.testing.assert_allclose(x, xo, rtol=0, atol=1e-10 * np.abs(xo).max())
+    # maxiter = 0: one solve in double
+    x0, st0 = mpir_amd.LUIR_QDLDL(Cp, Ci, Cx, b, fp, maxiter=0)
+    ret, xo0, _, _ = oracle.luir(Cp, Ci, Cx, b, fp, maxiter=0)
+    assert st0["num_refinement_steps"] == 0
+    # without refinement both carry the factor's error (a different
+    # elimination order, so not the same digits): compare against y
+    err_gpu, err_orc = np.abs(x0 - y).max(), np.abs(xo0 - y).max()
+    assert err_gpu <= 10 * err_orc + 1e-12, (err_gpu, err_orc)
+
+
+def test_luir_int64_and_bad_input(gpu):
+    P = mpir_amd.kkt_config(0)
+    Cp, Ci, Cx = permute_upper(P.Ap, P.Ai, P.Ax, mpir_amd.amd_order(P.Ap, P.Ai))
+    x32, _ = mpir_amd.LUIR_QDLDL(Cp, Ci, Cx, P.b, np.float32)
+    x64, _ = mpir_amd.LUIR_QDLDL(Cp.astype(np.int64), Ci.astype(np.int64), Cx, P.b, np.float32)
+    np.testing.assert_array_equal(x32, x64)
+    bad_i = Ci.copy()
+    bad_i[Cp[5]] = 7  # an entry below the diagonal of column 5
+    with pytest.raises(mpir_amd.MpirError):
+        mpir_amd.LUIR_QDLDL(Cp, bad_i, Cx, P.b, np.float32)

@@ -233,3 +233,47 @@ def test_permute_helper_matches_reference_pattern(oracle):
     assert tot == int(g["etree_sum"])
     np.testing.assert_array_equal(Lnz, g["Lnz"])
     np.testing.assert_array_equal(et, g["etree"])
+
+
+def test_luir_identity_closed_form(oracle):
+    # K = I: x0 = FACTPREC(b); each correction is the residual rounded to FP32,
+    # so x reaches b exactly after a few steps and the zero residual ends
+    # the loop (the reference would divide by it)
+    Ap, Ai, Ax = identity_upper(7)
+    b = np.linspace(-1, 1, 7) / 3.0
+    ret, x, steps, rr = oracle.luir(Ap, Ai, Ax, b, np.float32)
+    assert ret == 7
+    np.testing.assert_array_equal(x, b)
+    assert 2 <= steps <= 4 and rr == 0.0
+    # FP64 factor: x0 is already exact, the first residual is zero
+    ret, x, steps, rr = oracle.luir(Ap, Ai, Ax, b, np.float64)
+    np.testing.assert_array_equal(x, b)
+    assert steps == 1 and rr == 0.0
+
+
+def test_luir_exact_factor_and_maxiter_zero(oracle):
+    # an FP64 factor: corrections are at the rounding level of x, so the
+    # ||d||/||x|| <= 10 eps test (or the 0.5 ratio) ends the loop within two
+    # steps; maxiter = 0 is a single solve in double
+    K, Ap, Ai, Ax = random_qd(30, 10, 0.15, 11)
+    b = K @ np.random.default_rng(2).uniform(-1, 1, K.shape[0])
+    ret, x, steps, rr = oracle.luir(Ap, Ai, Ax, b, np.float64)
+    assert ret == 30 and 1 <= steps <= 2 and rr < 1e-14
+    ret, x0, steps0, rr0 = oracle.luir(Ap, Ai, Ax, b, np.float64, maxiter=0)
+    assert steps0 == 0 and rr0 < 1e-13
+    F = oracle.factor(Ap, Ai, Ax, np.float64)[1]
+    np.testing.assert_array_equal(x0, oracle.solve(F["Lp"], F["Li"], F["Lx"], F["Dinv"], b))
+
+
+def test_luir_fp32_factor_refines_to_fp64(oracle):
+    # the stopping rules on a system whose FP32 factor is a contraction: the
+    # corrections shrink geometrically until the 0.5 ratio or 10 eps test
+    import mpir_amd
+    P = mpir_amd.kkt_config(0)
+    Cp, Ci, Cx = permute_upper(P.Ap, P.Ai, P.Ax, mpir_amd.amd_order(P.Ap, P.Ai))
+    y = np.random.default_rng(3).uniform(-1, 1, P.n)
+    b = oracle.matmul(Cp, Ci, Cx, y)
+    ret, x, steps, rr = oracle.luir(Cp, Ci, Cx, b, np.float32, maxiter=20)
+    assert ret == P.n1
+    assert 2 <= steps < 20 and rr < 1e-12
+    np.testing.assert_allclose(x, y, rtol=0, atol=1e-9)
