@@ -186,6 +186,14 @@ int mpir_gmres_ldl_spmv_dev(mpir_gmres_ldl_t h, const double* x, double* y); /* 
 /* average ms of `reps` runs measured with HIP events on the handle's stream:
  * what = 0 SpMV, 1 M^{-1} application, 2 numeric factorization */
 int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);
+/* The log of the last solve in the JSON schema of the reference's VERBOSE
+ * build (gmres_ir.hpp:119-124, 305-314, 539-631): max_outer_iterations,
+ * max_inner_iterations, tolerance and an "events" array with one
+ * "gmres_ir" event (initial_residual; per iteration: iteration, residual,
+ * d_norm, timestamp_ms, inner_time_ms; stop_reason, stop_reason_detail,
+ * total_iterations, total_time_ms). Writes at most cap bytes including the
+ * terminating NUL and returns the full length (or a negative status). */
+int64_t mpir_gmres_ldl_log(mpir_gmres_ldl_t h, char* buf, int64_t cap);
 /* HIP stream of the handle (hipStream_t as void*) */
 void* mpir_gmres_ldl_stream(mpir_gmres_ldl_t h);
 
@@ -208,6 +216,13 @@ int mpir_gen_kkt(int dim, int k, double h_shift, int64_t m, int a_mode, int cnt_
 
 /* AMD ordering of an upper CSC pattern (host): perm[k] = original index */
 int mpir_amd_order(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* perm);
+/* A Matrix Market coordinate file read as examples/example.cpp:87-138 reads
+ * it: triplets transposed, sorted by (column, row), compressed to CSC (the
+ * upper triangle of a symmetric file stored by its lower triangle). Call
+ * with Ap = Ai = Ax = NULL for n / nnz, then with arrays of n+1 / nnz / nnz.
+ * Host only. */
+int mpir_read_mtx_upper(const char* path, int32_t* n, int64_t* nnz, int32_t* symmetric, int32_t* Ap, int32_t* Ai,
+                        double* Ax);
 
 #ifdef __cplusplus
 }

@@ -225,6 +225,16 @@ class GmresLDLIR:
         _check(load().mpir_gmres_ldl_stats(self._h, C.byref(st)), "stats")
         return st.as_dict()
 
+    def log(self) -> dict:
+        """The last solve's log in the schema of the reference's VERBOSE build."""
+        import json
+        lib = load()
+        n = int(lib.mpir_gmres_ldl_log(self._h, None, 0))
+        _check(n, "log")
+        buf = C.create_string_buffer(n + 1)
+        _check(int(lib.mpir_gmres_ldl_log(self._h, buf, n + 1)), "log")
+        return json.loads(buf.value.decode())
+
     def perm(self) -> np.ndarray:
         p = np.empty(self._n, dtype=np.int32)
         _check(load().mpir_gmres_ldl_perm(self._h, _ptr(p)), "perm")
@@ -304,6 +314,22 @@ def kkt_config(cfg: int, seed: int = 0, d_value: float | None = None, k: int | N
     raise ValueError("unknown configuration")
 
 
+def read_mtx(path):
+    """A Matrix Market file read as examples/example.cpp:87-138 reads it:
+    (n, Ap, Ai, Ax, symmetric) of the transposed triplets in CSC, i.e. the
+    upper triangle of a symmetric matrix stored by its lower triangle."""
+    lib = load()
+    n, nnz, sym = C.c_int32(), C.c_int64(), C.c_int32()
+    p = str(path).encode()
+    _check(lib.mpir_read_mtx_upper(p, C.byref(n), C.byref(nnz), C.byref(sym), None, None, None), "read_mtx")
+    Ap = np.empty(n.value + 1, np.int32)
+    Ai = np.empty(nnz.value, np.int32)
+    Ax = np.empty(nnz.value)
+    _check(lib.mpir_read_mtx_upper(p, C.byref(n), C.byref(nnz), C.byref(sym), _ptr(Ap), _ptr(Ai), _ptr(Ax)),
+           "read_mtx")
+    return n.value, Ap, Ai, Ax, bool(sym.value)
+
+
 def amd_order(Ap, Ai) -> np.ndarray:
     Ap = np.ascontiguousarray(Ap, dtype=np.int32)
     Ai = np.ascontiguousarray(Ai, dtype=np.int32)
@@ -338,6 +364,6 @@ def residual_upper(Ap, Ai, Ax, x, b) -> np.ndarray:
 
 __all__ = [
     "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "LUIR_QDLDL", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
-    "amd_order", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
+    "amd_order", "read_mtx", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
     "ORDER_AMD", "ORDER_ND", "SCALE_NONE", "SCALE_DIAG", "SCALE_RUIZ",
 ]

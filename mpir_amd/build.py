@@ -23,6 +23,8 @@ BUILD = ROOT / "_build" / "mpir_hip"
 LIB = PKG / "libmpir_hip.so"
 DROPIN_SRC = ROOT / "tests" / "cpp" / "dropin.cpp"
 DROPIN_BIN = ROOT / "tests" / "cpp" / "dropin_test"
+DRIVER_SRC = CSRC / "driver" / "mtx_solve.cpp"
+DRIVER_BIN = PKG / "mtx_solve"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -42,6 +44,7 @@ HOST_SOURCES = [
     CSRC / "host" / "ordering.cpp",
     CSRC / "host" / "ordering_nd.cpp",
     CSRC / "host" / "symbolic.cpp",
+    CSRC / "host" / "mtx.cpp",
     CSRC / "plan.cpp",
 ]
 HIP_SOURCES = [
@@ -89,6 +92,9 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     # the C++ drop-in test program (a reference-style caller of include/mpir/*.hpp)
     _run([CXX, "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(DROPIN_SRC), f"-L{PKG}", "-lmpir_hip",
           "-Wl,-rpath,$ORIGIN/../../mpir_amd", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(DROPIN_BIN)])
+    # the Matrix Market driver (examples/example.cpp of the reference)
+    _run([CXX, "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(DRIVER_SRC), f"-L{PKG}", "-lmpir_hip",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(DRIVER_BIN)])
     if verbose:
         print(f"built {LIB}")
     return LIB

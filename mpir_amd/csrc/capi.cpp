@@ -264,6 +264,42 @@ extern "C" {
 const char* mpir_last_error(void) { return g_last_error.c_str(); }
 const char* mpir_version(void) { return "mpir_hip 0.1 (gfx950)"; }
 
+int mpir_read_mtx_upper(const char* path, int32_t* n, int64_t* nnz, int32_t* symmetric, int32_t* Ap, int32_t* Ai,
+                        double* Ax) {
+  return guarded([&] {
+    MPIR_CHECK(path && n && nnz, "bad Matrix Market arguments");
+    MtxUpper M;
+    try {
+      M = read_mtx_upper(path);
+    } catch (const std::invalid_argument& e) {
+      throw Error(kErrInvalid, e.what());
+    }
+    *n = M.n;
+    *nnz = static_cast<int64_t>(M.i.size());
+    if (symmetric) *symmetric = M.symmetric ? 1 : 0;
+    if (Ap) std::copy(M.p.begin(), M.p.end(), Ap);
+    if (Ai) std::copy(M.i.begin(), M.i.end(), Ai);
+    if (Ax) std::copy(M.x.begin(), M.x.end(), Ax);
+    return 0;
+  });
+}
+
+int64_t mpir_gmres_ldl_log(mpir_gmres_ldl_t h, char* buf, int64_t cap) {
+  int64_t len = 0;
+  const int rc = guarded([&] {
+    MPIR_CHECK(h && h->s, "null solver handle");
+    const std::string j = h->s->log_json();
+    len = static_cast<int64_t>(j.size());
+    if (buf && cap > 0) {
+      const int64_t k = std::min<int64_t>(cap - 1, len);
+      std::memcpy(buf, j.data(), static_cast<size_t>(k));
+      buf[k] = 0;
+    }
+    return 0;
+  });
+  return rc < 0 ? rc : len;
+}
+
 int mpir_device_ok(void) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return 0;

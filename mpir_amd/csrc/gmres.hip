@@ -26,6 +26,7 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <string>
 
 #include "host/sparse.hpp"
 #include "solver.hpp"
@@ -696,11 +697,14 @@ void GmresLdl::solve_one(const double* b, double* x) {
   double* r = d_r_.as<double>();
   double* d = d_d_.as<double>();
   double* tmp = d_w_.as<double>();
+  const double t_start = now_ms();
   precond_gmres(b, x);
   residual_f64(A, x, b, r, st);
   const double bnorm = infnrm(b);
   double r_prev = std::numeric_limits<double>::max();
   stats_.stop_reason = 0;
+  log_ = SolveLog();
+  log_.initial_residual = infnrm(r);
   int64_t iter;
   double r_norm = 0.0;
   for (iter = 1; iter <= max_ir_; ++iter) {
@@ -713,7 +717,9 @@ void GmresLdl::solve_one(const double* b, double* x) {
       break;
     }
     hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, r, 1.0 / scale, tmp);
+    const double t_inner = now_ms();
     precond_gmres(tmp, d);
+    const double inner_ms = now_ms() - t_inner;
     hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, d, scale, d);
     const double d_norm = infnrm(d);
     const double x_norm = infnrm(x);
@@ -721,6 +727,8 @@ void GmresLdl::solve_one(const double* b, double* x) {
     residual_f64(A, x, b, r, st);
     r_norm = infnrm(r) / bnorm;
     stats_.ir_iterations = static_cast<int32_t>(iter);
+    // the VERBOSE log's record of the iteration (gmres_ir.hpp:583-596)
+    log_.iterations.push_back({static_cast<int32_t>(iter), r_norm, d_norm, now_ms() - t_start, inner_ms});
     if (d_norm <= x_norm * (10 * std::numeric_limits<double>::epsilon())) {
       stats_.stop_reason = 1;
       break;
@@ -733,6 +741,41 @@ void GmresLdl::solve_one(const double* b, double* x) {
   }
   if (max_ir_ == 0) r_norm = infnrm(r) / bnorm;
   stats_.final_rel_residual = r_norm;
+  log_.stop_reason = stats_.stop_reason;
+  log_.total_iterations = static_cast<int32_t>(std::min<int64_t>(iter, max_ir_));
+  log_.total_time_ms = now_ms() - t_start;
+}
+
+// The solver log in the schema of the reference's VERBOSE build
+// (gmres_ir.hpp:119-124, 305-314, 539-631): max iterations, tolerance and one
+// "gmres_ir" event with its per-iteration records for the last solve.
+std::string GmresLdl::log_json() const {
+  auto num = [](double v) {
+    if (!std::isfinite(v)) return std::string("null");
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.17g", v);
+    return std::string(buf);
+  };
+  static const char* reasons[] = {"max_iterations_reached", "converged_small_update",
+                                  "converged_residual_stagnation", "converged_zero_residual"};
+  static const char* details[] = {"", "d_norm <= x_norm * (10*eps)", "abs(r_norm - r_norm_prev) < tol",
+                                  "residual exactly zero"};
+  std::string o = "{\n  \"max_outer_iterations\": " + std::to_string(max_ir_) +
+                  ",\n  \"max_inner_iterations\": " + std::to_string(max_gmres_) + ",\n  \"tolerance\": " +
+                  num(tol_) + ",\n  \"events\": [\n    {\n      \"name\": \"gmres_ir\",\n      \"initial_residual\": " +
+                  num(log_.initial_residual) + ",\n      \"iterations\": [";
+  for (size_t i = 0; i < log_.iterations.size(); ++i) {
+    const auto& e = log_.iterations[i];
+    o += std::string(i ? "," : "") + "\n        {\"iteration\": " + std::to_string(e.iteration) +
+         ", \"residual\": " + num(e.residual) + ", \"d_norm\": " + num(e.d_norm) +
+         ", \"timestamp_ms\": " + num(e.timestamp_ms) + ", \"inner_time_ms\": " + num(e.inner_time_ms) + "}";
+  }
+  const int sr = std::max(0, std::min(3, log_.stop_reason));
+  o += std::string(log_.iterations.empty() ? "]" : "\n      ]") + ",\n      \"stop_reason\": \"" + reasons[sr] + "\"";
+  if (sr) o += ",\n      \"stop_reason_detail\": \"" + std::string(details[sr]) + "\"";
+  o += ",\n      \"total_iterations\": " + std::to_string(log_.total_iterations) + ",\n      \"total_time_ms\": " +
+       num(log_.total_time_ms) + "\n    }\n  ]\n}\n";
+  return o;
 }
 
 void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool device_ptr) {

@@ -92,6 +92,18 @@ void l_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai,
                std::vector<int64_t>& Lp, std::vector<int32_t>& Li);
 
 // Supernodal partition and the multifrontal schedule built on it.
+// A Matrix Market file read as the reference's driver reads it
+// (examples/example.cpp:87-138): triplets transposed, sorted by (column,
+// row), compressed to CSC; for a symmetric file stored by its lower
+// triangle this is the upper-triangular CSC.
+struct MtxUpper {
+  int32_t n = 0;
+  bool symmetric = false;
+  std::vector<int32_t> p, i;
+  std::vector<double> x;
+};
+MtxUpper read_mtx_upper(const std::string& path);
+
 // Fronts with at most this many rows are factored and solved by one
 // wavefront in LDS; larger ("big") fronts use the blocked kernels.
 constexpr int kSmallFront = 64;

@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -41,6 +42,19 @@ struct SolveStats {
   double analyze_ms = 0, factor_ms = 0, solve_ms = 0;
   int64_t device_bytes = 0;
   int32_t ruiz_sweeps = 0;  // Ruiz sweeps of the last factorization
+};
+
+// Record of the last GMRES-IR solve, in the terms of the reference's
+// VERBOSE log (gmres_ir.hpp:539-631)
+struct SolveLog {
+  struct Iter {
+    int32_t iteration;
+    double residual, d_norm, timestamp_ms, inner_time_ms;
+  };
+  double initial_residual = 0.0;
+  std::vector<Iter> iterations;
+  int32_t stop_reason = 0, total_iterations = 0;
+  double total_time_ms = 0.0;
 };
 
 // Outcome of LUIR_QDLDL (include/qdldl-luir/luir.hpp:33-241, stats.hpp)
@@ -79,6 +93,7 @@ class GmresLdl {
   void solve(const double* b, double* x, int nrhs, int64_t ld, bool device_ptr);
 
   const SolveStats& stats() const { return stats_; }
+  std::string log_json() const;  // the log of the last solve (JSON text)
   hipStream_t stream() const { return stream_; }
   bool analyzed() const { return analyzed_; }
   bool factored() const { return factored_; }
@@ -144,6 +159,7 @@ class GmresLdl {
   int32_t n_ = 0;
   int64_t nnz_ = 0;
   SolveStats stats_;
+  SolveLog log_;
 
   // pattern
   std::vector<int32_t> Ap_, Ai_, perm_;

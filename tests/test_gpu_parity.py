@@ -300,3 +300,45 @@ def test_luir_int64_and_bad_input(gpu):
     bad_i[Cp[5]] = 7  # an entry below the diagonal of column 5
     with pytest.raises(mpir_amd.MpirError):
         mpir_amd.LUIR_QDLDL(Cp, bad_i, Cx, P.b, np.float32)
+
+
+def test_mtx_driver_and_verbose_log(gpu, tmp_path):
+    # the reference's example driver on the Matrix Market fixture: x = K^{-1} K 1
+    import json
+    import shutil
+    import subprocess
+    src = GOLDEN / "kkt_cfg0.mtx"
+    mtx = tmp_path / "kkt_cfg0.mtx"
+    shutil.copy(src, mtx)
+    exe = Path(mpir_amd.__file__).resolve().parent / "mtx_solve"
+    proc = subprocess.run([str(exe), str(mtx), "--uf", "f32", "--log"], capture_output=True, text=True, timeout=120)
+    print(proc.stdout)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    out = dict(l.split(":", 1) for l in proc.stdout.splitlines() if ":" in l)
+    assert float(out["||x - xref|| / ||xref||".ljust(30)]) < 1e-12
+    assert float(out["||b - Ax|| / ||b||".ljust(30)]) < 1e-14
+    assert "relative residual @ iter 1 =" in proc.stdout
+    logs = list(tmp_path.glob("kkt_cfg0_*.json"))
+    assert len(logs) == 1
+    log = json.loads(logs[0].read_text())
+    for key in ("matrix", "factorization_precision", "relative_error", "relative_residual", "ilu_k",
+                "max_outer_iterations", "max_inner_iterations", "tolerance", "events"):
+        assert key in log, key
+    ev = log["events"][0]
+    assert ev["name"] == "gmres_ir" and ev["total_iterations"] == len(ev["iterations"]) >= 1
+    assert ev["stop_reason"] in ("converged_small_update", "converged_residual_stagnation",
+                                 "converged_zero_residual", "max_iterations_reached")
+    assert set(ev["iterations"][0]) == {"iteration", "residual", "d_norm", "timestamp_ms", "inner_time_ms"}
+
+
+def test_solver_log_matches_stats(gpu):
+    P = mpir_amd.kkt_config(0)
+    s = mpir_amd.GmresLDLIR(np.float32)
+    s.SetTolerance(1e-12)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    s.Solve(P.b)
+    st, log = s.stats(), s.log()
+    ev = log["events"][0]
+    assert ev["total_iterations"] == st["ir_iterations"] == len(ev["iterations"])
+    assert ev["iterations"][-1]["residual"] == st["final_rel_residual"]
+    assert log["tolerance"] == 1e-12 and log["max_outer_iterations"] == 10

@@ -131,3 +131,39 @@ def test_cpp_dropin_headers_without_a_device():
     proc = subprocess.run([str(DROPIN), "--host"], capture_output=True, text=True, timeout=60)
     assert proc.returncode == 0, proc.stdout + proc.stderr
     assert "MPIR_ERR_NODEVICE" in proc.stdout
+
+
+MTX = Path(__file__).resolve().parent / "golden" / "kkt_cfg0.mtx"
+
+
+def test_matrix_market_reader_matches_the_generator():
+    # the reference driver's read (transposed triplets, sorted by column then
+    # row) of a lower-stored symmetric file gives back the upper CSC exactly
+    n, Ap, Ai, Ax, sym = mpir_amd.read_mtx(MTX)
+    P = mpir_amd.kkt_config(0)
+    assert sym and n == P.n
+    np.testing.assert_array_equal(Ap, P.Ap)
+    np.testing.assert_array_equal(Ai, P.Ai)
+    np.testing.assert_array_equal(Ax, P.Ax)
+
+
+def test_matrix_market_reader_rejects_bad_files(tmp_path):
+    bad = tmp_path / "bad.mtx"
+    bad.write_text("%%MatrixMarket matrix array real general\n2 2\n1\n2\n3\n4\n")
+    with pytest.raises(mpir_amd.MpirError, match="coordinate"):
+        mpir_amd.read_mtx(bad)
+    bad.write_text("%%MatrixMarket matrix coordinate real symmetric\n3 3 2\n1 1 1.0\n4 1 2.0\n")
+    with pytest.raises(mpir_amd.MpirError, match="out of range"):
+        mpir_amd.read_mtx(bad)
+    with pytest.raises(mpir_amd.MpirError, match="cannot open"):
+        mpir_amd.read_mtx(tmp_path / "missing.mtx")
+
+
+def test_mtx_driver_without_a_device():
+    if mpir_amd.device_ok():
+        pytest.skip("a gfx950 device is present; the GPU test runs the driver")
+    import subprocess
+    exe = Path(mpir_amd.__file__).resolve().parent / "mtx_solve"
+    assert exe.exists(), "mpir_amd/mtx_solve is built by mpir_amd.build"
+    proc = subprocess.run([str(exe), str(MTX)], capture_output=True, text=True, timeout=60)
+    assert proc.returncode != 0 and "no gfx950 device" in proc.stderr
