@@ -221,8 +221,8 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
                                                               bool refactor) {
   constexpr int NB = kPanelNB;
   constexpr int CW = NB / (kPanelThreads / 64);  // columns per wave (4)
-  __shared__ __align__(16) T col[2][NB];   // column j of the block, double-buffered
-  __shared__ __align__(16) T gcol[2][NB];  // column j of the identity rows
+  __shared__ T colb[2][CW][NB];   // the pivot block's columns, double-buffered
+  __shared__ T gcolb[2][CW][NB];  // the same columns of the identity rows
   __shared__ __align__(16) T Zs[NB][NB];   // Z, row-major
   __shared__ __align__(16) T Xs[NB][NB];   // the task's rows, transposed: Xs[t][row]
   const Task4 tk = tasks[blockIdx.x];
@@ -251,8 +251,10 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
     const T v = P[rsafe + (k0 + min(c, nbk - 1)) * ld];
     xv[u] = (!live || c >= nbk) ? T(0) : unit >= 0 ? (c == unit ? T(1) : T(0)) : v;
   }
-  // entries (lane, c0w + u) of the block's lower triangle (fa) and of the
-  // identity rows (ga), in registers through the elimination
+  // Entries (lane, c0w + u) of the block's lower triangle (fa) and of the
+  // identity rows (ga), in registers through the elimination. With refactor
+  // fa holds the block as eliminated so far; otherwise it holds the given
+  // factor (L below the diagonal, from its transposed slot, and D on it).
   const T* Dk = a.D + f + k0;
   const int lsafe = min(lane, nbk - 1);
   T fa[CW], ga[CW];
@@ -261,43 +263,108 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
     const int c = c0w + u;
     const int cs = min(c, nbk - 1);
     const T v = refactor ? P[(k0 + lsafe) + (k0 + cs) * ld] : P[(k0 + cs) + (k0 + lsafe) * ld];
-    const T dc = refactor ? T(1) : Dk[cs];
+    const T dc = Dk[cs];
     const bool in = lane < nbk && c < nbk && lane >= c;
-    fa[u] = !in ? T(0) : refactor ? v : lane == c ? dc : v * dc;
+    fa[u] = !in ? T(0) : (refactor || lane != c) ? v : dc;
     ga[u] = (lane == c && c < nbk) ? T(1) : T(0);
   }
-  for (int j = 0; j < nbk; ++j) {
-    const int b = j & 1;
-    if (w == j / CW) {
-      const int uj = j % CW;
-      T fv = fa[0], gv = ga[0];
+  // Blocked right-looking elimination, CW columns per barrier: wave J owns
+  // columns CW J .. CW J + CW - 1 and publishes them; every thread then
+  // factors the CW x CW pivot block itself, forms the multipliers of its
+  // row and of its columns by substitution, and applies the rank-CW update.
+  const int nstep = (nbk + CW - 1) / CW;
+  for (int J = 0; J < nstep; ++J) {
+    const int b = J & 1;
+    const int j0 = CW * J;
+    if (w == J) {
 #pragma unroll
-      for (int u = 1; u < CW; ++u) {
-        fv = u == uj ? fa[u] : fv;
-        gv = u == uj ? ga[u] : gv;
+      for (int t = 0; t < CW; ++t) {
+        colb[b][t][lane] = fa[t];
+        gcolb[b][t][lane] = ga[t];
       }
-      col[b][lane] = fv;
-      gcol[b][lane] = gv;
     }
     __syncthreads();
-    T cv[CW];
+    bool valid[CW];
 #pragma unroll
-    for (int u = 0; u < CW; ++u) cv[u] = col[b][c0w + u];
-    const T d = col[b][j];
-    const T dinv = T(1) / d;
-    const T lr = lane > j ? col[b][lane] * dinv : T(0);
-    const T gi = lane <= j ? gcol[b][lane] * dinv : T(0);
-    // the owner keeps column j of L_kk (and D on the diagonal) in LDS; its
+    for (int t = 0; t < CW; ++t) valid[t] = j0 + t < nbk;
+    // pivot block L_B D_B L_B^T; dl[t][q] = d_q l_tq, dinv = 1 / d
+    T d[CW], dinv[CW], l[CW][CW], dl[CW][CW];
+#pragma unroll
+    for (int t = 0; t < CW; ++t) {
+      T dt = colb[b][t][j0 + t];
+      if (refactor) {
+#pragma unroll
+        for (int q = 0; q < t; ++q) dt -= l[t][q] * dl[t][q];
+      }
+      d[t] = valid[t] ? dt : T(1);
+      dinv[t] = T(1) / d[t];
+#pragma unroll
+      for (int u = t + 1; u < CW; ++u) {
+        T v = colb[b][t][j0 + u];
+        if (refactor) {
+#pragma unroll
+          for (int q = 0; q < t; ++q) v -= l[u][q] * dl[t][q];
+          v *= dinv[t];
+        }
+        l[u][t] = (valid[u] && valid[t]) ? v : T(0);
+        dl[u][t] = d[t] * l[u][t];
+      }
+    }
+    // multipliers of this thread's row and identity row
+    T mr[CW], gm[CW];
+#pragma unroll
+    for (int t = 0; t < CW; ++t) {
+      T v = colb[b][t][lane];
+      T g = gcolb[b][t][lane];
+#pragma unroll
+      for (int q = 0; q < t; ++q) {
+        if (refactor) v -= mr[q] * dl[t][q];
+        g -= gm[q] * dl[t][q];
+      }
+      mr[t] = refactor ? v * dinv[t] : v;
+      gm[t] = g * dinv[t];
+    }
+    // the owner keeps its columns of L_kk (and D on the diagonal) in LDS; its
     // stores are issued after the loop (a barrier waits for pending stores)
-    if (owner && w == 0) Xs[j][lane] = lane > j ? lr : lane == j ? d : T(0);
+    if (owner && w == 0) {
+      const int s0 = lane - j0;
+#pragma unroll
+      for (int t = 0; t < CW; ++t) {
+        T lv = T(0);
+#pragma unroll
+        for (int u = t + 1; u < CW; ++u) lv = s0 == u ? l[u][t] : lv;
+        Xs[j0 + t][lane] = s0 < t ? T(0) : s0 == t ? d[t] : s0 < CW ? lv : mr[t];
+      }
+    }
+    // rank-CW update of this thread's entries right of the block
 #pragma unroll
     for (int u = 0; u < CW; ++u) {
       const int c = c0w + u;
-      const bool upd = c > j && c < nbk;
-      const T fnew = fa[u] - lr * cv[u];
-      const T gnew = ga[u] - gi * cv[u];
-      fa[u] = (refactor && upd && lane >= c) ? fnew : fa[u];
-      ga[u] = upd ? gnew : c == j ? gi : ga[u];
+      if (c >= j0 + CW && c < nbk) {
+        T mc[CW];
+        T fu = T(0), gu = T(0);
+#pragma unroll
+        for (int t = 0; t < CW; ++t) {
+          T v = colb[b][t][c];
+          if (refactor) {
+#pragma unroll
+            for (int q = 0; q < t; ++q) v -= mc[q] * dl[t][q];
+            v *= dinv[t];
+          }
+          mc[t] = v;
+          const T dm = d[t] * mc[t];
+          fu += mr[t] * dm;
+          gu += gm[t] * dm;
+        }
+        if (refactor && lane >= c) fa[u] -= fu;
+        ga[u] -= gu;
+      }
+    }
+    // the identity rows' entries in the block columns are final: the multipliers
+    if (w == J) {
+#pragma unroll
+      for (int t = 0; t < CW; ++t)
+        if (valid[t]) ga[t] = gm[t];
     }
   }
   if (owner) {
