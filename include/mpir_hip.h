@@ -164,6 +164,14 @@ int mpir_gmres_ldl_set_ordering(mpir_gmres_ldl_t h, int ordering);  /* before an
  * the default) or MPIR_SCALE_RUIZ (symmetric Ruiz equilibration: up to 10
  * sweeps, until every row's max-norm is within 1e-2 of one) */
 int mpir_gmres_ldl_set_scaling(mpir_gmres_ldl_t h, int mode);
+/* schedule of the triangular solves inside M^{-1} (no counterpart in the
+ * reference, whose QDLDL_solve is a sequential loop): MPIR_SCHED_LEVELS, one
+ * launch per task kind and tree level, or MPIR_SCHED_DAG (default), one
+ * persistent launch per direction driven by per-front dependency counters.
+ * Both give bit-identical results. */
+#define MPIR_SCHED_LEVELS 0
+#define MPIR_SCHED_DAG 1
+int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule);
 /* symbolic phase on the pattern of the upper CSC (reused by later factors) */
 int mpir_gmres_ldl_analyze(mpir_gmres_ldl_t h, int32_t n, const int32_t* Ap, const int32_t* Ai);
 /* numeric phase; Ax in the analyzed pattern's entry order (host / device) */

@@ -30,6 +30,7 @@ ORDER_NATURAL, ORDER_AMD, ORDER_ND = 0, 1, 2
 _PREC_OF = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 _FACTOR_PREC_OF = {**_PREC_OF, np.dtype(np.float16): F16}
 SCALE_NONE, SCALE_DIAG, SCALE_RUIZ = 0, 1, 2
+SCHED_LEVELS, SCHED_DAG = 0, 1
 
 
 class MpirError(RuntimeError):
@@ -182,6 +183,10 @@ class GmresLDLIR:
     def SetScaling(self, mode: int) -> None:
         """SCALE_NONE, SCALE_DIAG (the reference's, default) or SCALE_RUIZ."""
         _check(load().mpir_gmres_ldl_set_scaling(self._h, int(mode)), "SetScaling")
+
+    def SetSolveSchedule(self, schedule: int) -> None:
+        """SCHED_DAG (default) or SCHED_LEVELS for the triangular solves of M^{-1}."""
+        _check(load().mpir_gmres_ldl_set_schedule(self._h, int(schedule)), "SetSolveSchedule")
 
     # phases
     def Analyze(self, Ap, Ai) -> None:
@@ -365,5 +370,5 @@ def residual_upper(Ap, Ai, Ax, x, b) -> np.ndarray:
 __all__ = [
     "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "LUIR_QDLDL", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
     "amd_order", "read_mtx", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
-    "ORDER_AMD", "ORDER_ND", "SCALE_NONE", "SCALE_DIAG", "SCALE_RUIZ",
+    "ORDER_AMD", "ORDER_ND", "SCALE_NONE", "SCALE_DIAG", "SCALE_RUIZ", "SCHED_LEVELS", "SCHED_DAG",
 ]

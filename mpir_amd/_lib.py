@@ -63,6 +63,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_set_tol": (C.c_int, [P, f64]),
     "mpir_gmres_ldl_set_ordering": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_scaling": (C.c_int, [P, C.c_int]),
+    "mpir_gmres_ldl_set_schedule": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_log": (i64, [P, C.c_char_p, i64]),
     "mpir_read_mtx_upper": (C.c_int, [C.c_char_p, P, P, P, P, P, P]),
     "mpir_gmres_ldl_analyze": (C.c_int, [P, i32, P, P]),

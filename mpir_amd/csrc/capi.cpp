@@ -147,7 +147,7 @@ int solve_impl(int32_t n, const int64_t* Lp, const int32_t* Li, const TF* Lx, co
   MPIR_HIP(hipMemcpyAsync(xd.get(), xh.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
   E.solver->precond_apply_dev(xd.as<double>());
   MPIR_HIP(hipMemcpyAsync(xh.data(), xd.get(), sizeof(double) * n, hipMemcpyDeviceToHost, st));
-  MPIR_HIP(hipStreamSynchronize(st));
+  E.solver->sync_check();
   for (int32_t i = 0; i < n; ++i) x[i] = static_cast<TX>(xh[i]);
   return 0;
 }
@@ -450,6 +450,13 @@ int mpir_gmres_ldl_set_scaling(mpir_gmres_ldl_t h, int mode) {
     return 0;
   });
 }
+int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->set_schedule(schedule);
+    return 0;
+  });
+}
 
 int mpir_gmres_ldl_set_ordering(mpir_gmres_ldl_t h, int ordering) {
   return guarded([&] {
@@ -544,6 +551,7 @@ int mpir_gmres_ldl_precond_dev(mpir_gmres_ldl_t h, double* x) {
     MPIR_HANDLE_CHECK;
     MPIR_CHECK(h->s->factored(), "no factor");
     h->s->precond_apply_dev(x);
+    h->s->sync_check();
     return 0;
   });
 }

@@ -423,6 +423,33 @@ void GmresLdl::upload_plan() {
   dsp_.xfp_off = keep_[11].as<int64_t>();
   d_scratch_.alloc(static_cast<size_t>(std::max<int64_t>(P.nxfp, 1)) * sizeof(double));
   dsp_.xfp = d_scratch_.get();
+  keep_.resize(24);
+  upload_vec(keep_[20], P.dag_fwd, st);
+  upload_vec(keep_[21], P.dag_bwd, st);
+  upload_vec(keep_[22], P.dag_meta, st);
+  keep_[23].alloc(static_cast<size_t>(dag_counter_count(S.ns)) * sizeof(int));
+  d_serr_.alloc(sizeof(int));
+  MPIR_HIP(hipMemsetAsync(d_serr_.get(), 0, sizeof(int), st));
+  dsp_.err = d_serr_.as<int>();
+  dsp_.schedule = schedule_;
+  DagArgs& g = dsp_.dag;
+  g.ns = S.ns;
+  g.nfwd = static_cast<int>(P.dag_fwd.size());
+  g.nbwd = static_cast<int>(P.dag_bwd.size());
+  g.fwd = keep_[20].as<Task4>();
+  g.bwd = keep_[21].as<Task4>();
+  g.meta = keep_[22].as<int32_t>();
+  g.small_list = dsp_.small_list;
+  g.gather_task = dsp_.gather_task;
+  g.gather_child_ptr = dsp_.gather_child_ptr;
+  g.gather_child = dsp_.gather_child;
+  g.xc_task = dsp_.xc_task;
+  g.u_task = dsp_.u_task;
+  g.rpart_task = dsp_.rpart_task;
+  g.bwd_task = dsp_.bwd_task;
+  g.xfp_off = dsp_.xfp_off;
+  g.part_off = dsp_.part_off;
+  g.counters = keep_[23].as<int>();
   d_uvec_.alloc(std::max<int64_t>(1, static_cast<int64_t>(S.rows.size())) * sizeof(double));
 }
 
@@ -603,6 +630,17 @@ void GmresLdl::precond_apply_dev(double* x) {
   else if (uf_ == kF32) ldl_solve<float, float, double>(solve_args<float, float>(d_panel_.get()), dsp_, x, u, stream_);
   else if (uf_ == kF16) ldl_solve<__half, float, double>(solve_args<__half, float>(d_panel16_.get()), dsp_, x, u, stream_);
   else ldl_solve<__hip_bfloat16, float, double>(solve_args<__hip_bfloat16, float>(d_panel16_.get()), dsp_, x, u, stream_);
+}
+
+void GmresLdl::sync_check() {
+  MPIR_HIP(hipStreamSynchronize(stream_));
+  if (!d_serr_.get()) return;
+  int e = 0;
+  MPIR_HIP(hipMemcpy(&e, d_serr_.get(), sizeof(int), hipMemcpyDeviceToHost));
+  if (e) {
+    MPIR_HIP(hipMemset(d_serr_.get(), 0, sizeof(int)));
+    throw Error(kErrInternal, "a triangular-solve dependency wait expired");
+  }
 }
 
 void GmresLdl::spmv_dev(const double* x, double* y) {
@@ -805,7 +843,7 @@ void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool devi
     hipLaunchKernelGGL(k_perm_out, dim3(nblk(n)), dim3(256), 0, st, n, d_x_.as<double>(), d_perm_.as<int32_t>(),
                        d_ad_.as<double>(), xout);
     if (!device_ptr) MPIR_HIP(hipMemcpyAsync(xc, xout, n * sizeof(double), hipMemcpyDeviceToHost, st));
-    MPIR_HIP(hipStreamSynchronize(st));
+    sync_check();
   }
   stats_.solve_ms = now_ms() - t0;
 }
@@ -845,6 +883,7 @@ double GmresLdl::time_kernels(int what, int reps) {
   MPIR_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  sync_check();
   return static_cast<double>(ms) / reps;
 }
 
@@ -911,7 +950,7 @@ void GmresLdl::luir_impl(const double* b, double* x, int maxiter, LuirResult* ou
   hipLaunchKernelGGL((k_convert<double, TF>), dim3(nblk(n)), dim3(256), 0, st, n, dr, c);
   out->rel_residual = absmax(c) / bn;
   MPIR_HIP(hipMemcpyAsync(x, dx, n * sizeof(double), hipMemcpyDeviceToHost, st));
-  MPIR_HIP(hipStreamSynchronize(st));
+  sync_check();
 }
 
 void GmresLdl::luir(const double* b, double* x, int maxiter, LuirResult* out) {

@@ -88,7 +88,42 @@ struct SolveArgs {
   const TD* Dinv;
 };
 
+// Schedules of the triangular solves: one launch per task kind and level,
+// or one persistent launch per direction driven by dependency counters.
+constexpr int kScheduleLevels = 0;
+constexpr int kScheduleDag = 1;
+
+// Dependency-driven schedule: task lists in topological order (kDag* kinds)
+// and per-front counts; counters: kNumSolveCounters * ns completion counts
+// followed by kDispensers task dispensers per direction, each on its own
+// 128-byte line; all zeroed before every solve.
+constexpr int kNumSolveCounters = 6;
+constexpr int kDispensers = 8;
+constexpr int kDispenserStride = 32;
+constexpr int dispenser_base(int ns) { return (kNumSolveCounters * ns + kDispenserStride - 1) / kDispenserStride * kDispenserStride; }
+constexpr int dag_counter_count(int ns) { return dispenser_base(ns) + 2 * kDispensers * kDispenserStride; }
+struct DagArgs {
+  int ns = 0, nfwd = 0, nbwd = 0;
+  const Task4* fwd = nullptr;
+  const Task4* bwd = nullptr;
+  const int32_t* meta = nullptr;  // kMetaStride per supernode
+  const int32_t* small_list = nullptr;
+  const int32_t* gather_task = nullptr;
+  const int32_t* gather_child_ptr = nullptr;
+  const int32_t* gather_child = nullptr;
+  const int32_t* xc_task = nullptr;
+  const int32_t* u_task = nullptr;
+  const int32_t* rpart_task = nullptr;
+  const int32_t* bwd_task = nullptr;
+  const int64_t* xfp_off = nullptr;
+  const int64_t* part_off = nullptr;
+  int* counters = nullptr;
+};
+
 struct DeviceSolvePlan {
+  int schedule = kScheduleDag;
+  DagArgs dag;
+  int* err = nullptr;  // bit 1: a dependency wait expired
   int nlevels = 0;
   const int32_t* small_list = nullptr;    // small fronts, grouped by level
   const int32_t* gather_task = nullptr;   // (s, row block) pairs, grouped by level

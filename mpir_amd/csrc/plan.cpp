@@ -59,6 +59,54 @@ class PoolAlloc {
   int64_t top_ = 0;
 };
 
+// Task lists of the dependency-driven solves from the per-level lists. A
+// forward task of a front waits for its children (small fronts, gathers),
+// for its gathers (column blocks) or for its column blocks (R blocks); a
+// backward task waits for its parent (R partials, small fronts) or for its
+// R partials (column blocks). Children sit on lower levels and a front's
+// own task kinds follow each other within its level, so every wait is on
+// tasks earlier in the list.
+void build_dag_schedule(FactorPlan& P) {
+  const Supernodes& S = P.sn;
+  const int32_t ns = S.ns;
+  P.dag_meta.assign(static_cast<size_t>(ns) * kMetaStride, 0);
+  for (int32_t s = 0; s < ns; ++s) {
+    int32_t* m = &P.dag_meta[static_cast<size_t>(s) * kMetaStride];
+    m[kMetaParent] = S.sparent[s];
+    m[kMetaNChild] = P.child_ptr[s + 1] - P.child_ptr[s];
+    m[kMetaNBFinal] = 1;  // small fronts: one backward task
+  }
+  auto count = [&](const std::vector<int32_t>& tasks, int stride, int field) {
+    for (size_t q = 0; q < tasks.size(); q += stride) P.dag_meta[static_cast<size_t>(tasks[q]) * kMetaStride + field]++;
+  };
+  for (int32_t q = 0; q < static_cast<int32_t>(P.big_list.size()); ++q)
+    P.dag_meta[static_cast<size_t>(P.big_list[q]) * kMetaStride + kMetaNBFinal] = 0;
+  count(P.sfwd_task, 2, kMetaNGather);
+  count(P.sxc_task, 3, kMetaNXc);
+  count(P.su_task, 2, kMetaNU);
+  count(P.rpart_task, 3, kMetaNRpart);
+  count(P.sbwd_task, 2, kMetaNBFinal);
+  auto grouped = [](std::vector<Task4>& out, int32_t kind, int32_t b, int32_t e) {
+    for (int32_t q = b; q < e; q += kDagGroup) out.push_back({kind, q, std::min(kDagGroup, e - q), 0});
+  };
+  auto single = [](std::vector<Task4>& out, int32_t kind, int32_t b, int32_t e) {
+    for (int32_t q = b; q < e; ++q) out.push_back({kind, q, 1, 0});
+  };
+  P.dag_fwd.clear();
+  P.dag_bwd.clear();
+  for (int32_t l = 0; l < S.nlevels; ++l) {
+    grouped(P.dag_fwd, kDagSmall, P.small_ptr[l], P.small_ptr[l + 1]);
+    grouped(P.dag_fwd, kDagGather, P.sfwd_ptr[l], P.sfwd_ptr[l + 1]);
+    single(P.dag_fwd, kDagXc, P.sxc_ptr[l], P.sxc_ptr[l + 1]);
+    single(P.dag_fwd, kDagU, P.su_ptr[l], P.su_ptr[l + 1]);
+  }
+  for (int32_t l = S.nlevels - 1; l >= 0; --l) {
+    single(P.dag_bwd, kDagRpart, P.rpart_ptr[l], P.rpart_ptr[l + 1]);
+    single(P.dag_bwd, kDagBxc, P.sbwd_ptr[l], P.sbwd_ptr[l + 1]);
+    grouped(P.dag_bwd, kDagSmall, P.small_ptr[l], P.small_ptr[l + 1]);
+  }
+}
+
 }  // namespace
 
 FactorPlan build_plan_from_supernodes(Supernodes sn) {
@@ -270,6 +318,7 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     P.su_ptr[l + 1] = static_cast<int32_t>(P.su_task.size() / 2);
     P.sbwd_ptr[l + 1] = static_cast<int32_t>(P.sbwd_task.size() / 2);
   }
+  build_dag_schedule(P);
   return P;
 }
 

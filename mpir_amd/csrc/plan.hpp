@@ -27,6 +27,15 @@ struct Task4 {
   int32_t a, b, c, d;
 };
 
+// Kinds of the dependency-driven solve tasks (Task4.a); b = first index into
+// the kind's own task list, c = number of entries (up to kDagGroup for the
+// one-wavefront kinds, else 1).
+enum : int32_t { kDagSmall = 0, kDagGather, kDagXc, kDagU, kDagRpart, kDagBxc };
+constexpr int kDagGroup = 4;
+// Per-supernode entries of the solve dependency counts.
+enum : int32_t { kMetaParent = 0, kMetaNChild, kMetaNGather, kMetaNXc, kMetaNU, kMetaNRpart, kMetaNBFinal,
+                 kMetaStride = 8 };
+
 struct FactorPlan {
   Supernodes sn;
   // children of each supernode, ascending
@@ -80,6 +89,13 @@ struct FactorPlan {
   int64_t npart = 0;
   std::vector<int64_t> xfp_off;               // per supernode
   int64_t nxfp = 0;
+  // dependency-driven solves: every task above, forward in level order
+  // (small groups, gathers, column blocks, R blocks per level) and backward
+  // in reverse level order (R partials, column blocks, small groups); each
+  // task depends only on tasks before it. dag_meta: kMetaStride entries per
+  // supernode (parent, children, and the task counts the waits compare with)
+  std::vector<Task4> dag_fwd, dag_bwd;
+  std::vector<int32_t> dag_meta;
   // for each entry of the ORIGINAL upper CSC: destination in the panel pool
   std::vector<int64_t> val_dst;
   // statistics

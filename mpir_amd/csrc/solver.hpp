@@ -83,6 +83,15 @@ class GmresLdl {
     scaling_ = mode;
   }
   void set_relax(bool on) { relax_ = on; }
+  // schedule of the triangular solves: kScheduleLevels or kScheduleDag
+  void set_schedule(int sched) {
+    MPIR_CHECK(sched == kScheduleLevels || sched == kScheduleDag, "solve schedule must be 0 (levels) or 1 (dag)");
+    schedule_ = sched;
+    dsp_.schedule = sched;
+  }
+  // Waits for the handle's stream and reports a triangular solve whose
+  // dependency wait expired.
+  void sync_check();
 
   // Symbolic phase on the pattern of the upper CSC (reusable across values).
   void analyze(int32_t n, const int32_t* Ap, const int32_t* Ai);
@@ -149,6 +158,7 @@ class GmresLdl {
   double tol_ = 1e-10;
   int ordering_ = kOrderAmd;
   int scaling_ = kScaleDiag;
+  int schedule_ = kScheduleDag;
   bool relax_ = true;     // relaxed supernodes (AMD / ND orderings only)
   bool relaxed_ = false;  // the current plan uses them
   std::vector<int32_t> Cp_, Ci_, parent_, Lnz_;  // pattern of P K P^T, etree
@@ -172,7 +182,7 @@ class GmresLdl {
   DevBuf d_entry_col_, d_entry_row_, d_diag_pos_, d_perm_;
   // numeric
   DevBuf d_ax_, d_axs_, d_ad_, d_adp_, d_panel_, d_panel16_, d_upool_, d_D_, d_Dinv_, d_err_, d_cnt_, d_nonfinite_;
-  DevBuf d_uvec_, d_scratch_;
+  DevBuf d_uvec_, d_scratch_, d_serr_;
   // GMRES workspace
   DevBuf d_V_, d_w_, d_b_, d_x_, d_r_, d_d_, d_red_, d_small_;
   DevBuf d_hcol_, d_givens_;

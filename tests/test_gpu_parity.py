@@ -186,6 +186,35 @@ def test_solve_big_fronts_matches_oracle(gpu, oracle, k):
     np.testing.assert_array_equal(x, x2)
 
 
+@pytest.mark.parametrize("uf,k,order", [(np.float64, 16, mpir_amd.ORDER_ND), (np.float32, 24, mpir_amd.ORDER_ND),
+                                        (np.float32, 10, mpir_amd.ORDER_NATURAL), ("bf16", 16, mpir_amd.ORDER_AMD)])
+def test_dag_schedule_matches_level_schedule_bitwise(gpu, oracle, uf, k, order):
+    # the dependency-driven solves run the same task bodies with the same
+    # sums as the per-level launches, so whole GMRES-IR solves agree bit for
+    # bit; natural ordering gives a chain of big fronts (one per level)
+    P = mpir_amd.kkt_config(1, k=k)
+    s = mpir_amd.GmresLDLIR(uf, ordering=order)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    x_dag = s.Solve(P.b)
+    st_dag = s.stats()
+    s.SetSolveSchedule(mpir_amd.SCHED_LEVELS)
+    x_lev = s.Solve(P.b)
+    st_lev = s.stats()
+    np.testing.assert_array_equal(x_dag, x_lev)
+    assert st_dag["ir_iterations"] == st_lev["ir_iterations"]
+    assert st_dag["gmres_iterations"] == st_lev["gmres_iterations"]
+    s.SetSolveSchedule(mpir_amd.SCHED_DAG)
+    np.testing.assert_array_equal(s.Solve(P.b), x_dag)
+    if uf in (np.float64, np.float32):
+        assert backward_error(oracle, P.Ap, P.Ai, P.Ax, x_dag, P.b) < 1e-13
+
+
+def test_bad_schedule_rejected(gpu):
+    s = mpir_amd.GmresLDLIR(np.float64)
+    with pytest.raises(mpir_amd.MpirError):
+        s.SetSolveSchedule(2)
+
+
 @pytest.mark.parametrize("k", [24, 32])
 def test_exact_factor_gives_one_step_per_gmres_call(gpu, oracle, k):
     # with an FP64 factor M^{-1} K = I to rounding, so every PrecondGmres
