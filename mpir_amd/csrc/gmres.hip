@@ -384,72 +384,66 @@ void GmresLdl::upload_plan() {
   dfp_.h_pan_ptr = P.pan_ptr;
   dfp_.h_upd_ptr = P.upd_ptr;
   dfp_.h_pan_aug_ptr = P.pan_aug_ptr;
-  dfp_.n_inv_task = static_cast<int>(P.sxc_task.size() / 3);
+  dfp_.n_inv_task = static_cast<int>(P.fx_task.size() / 3);
   dfp_.h_upd_aug_ptr = P.upd_aug_ptr;
-  keep_.resize(20);
+  keep_.resize(40);
   upload_vec(keep_[15], P.asm_child_ptr, st);
   upload_vec(keep_[16], P.asm_child, st);
-  upload_vec(keep_[17], P.sfwd_child_ptr, st);
-  upload_vec(keep_[18], P.sfwd_child, st);
-  upload_vec(keep_[12], P.rpart_task, st);
-  upload_vec(keep_[13], P.part_off, st);
-  keep_[14].alloc(static_cast<size_t>(std::max<int64_t>(P.npart, 1)) * sizeof(double));
-  upload_vec(keep_[7], P.sfwd_task, st);
-  upload_vec(keep_[8], P.sbwd_task, st);
-  upload_vec(keep_[9], P.sxc_task, st);
-  upload_vec(keep_[10], P.su_task, st);
   dfp_.asm_child_ptr = keep_[15].as<int32_t>();
   dfp_.asm_child = keep_[16].as<int32_t>();
+  // triangular solves
   dsp_ = DeviceSolvePlan();
-  dsp_.nlevels = S.nlevels;
-  dsp_.small_list = dfp_.small_list;
-  dsp_.gather_task = keep_[7].as<int32_t>();
-  dsp_.gather_child_ptr = keep_[17].as<int32_t>();
-  dsp_.gather_child = keep_[18].as<int32_t>();
-  dsp_.xc_task = keep_[9].as<int32_t>();
-  dfp_.inv_task = keep_[9].as<int32_t>();
-  dsp_.u_task = keep_[10].as<int32_t>();
-  dsp_.bwd_task = keep_[8].as<int32_t>();
-  dsp_.rpart_task = keep_[12].as<int32_t>();
-  dsp_.part_off = keep_[13].as<int64_t>();
-  dsp_.part = keep_[14].get();
-  dsp_.h_small_ptr = P.small_ptr;
-  dsp_.h_gather_ptr = P.sfwd_ptr;
-  dsp_.h_xc_ptr = P.sxc_ptr;
-  dsp_.h_u_ptr = P.su_ptr;
-  dsp_.h_bwd_ptr = P.sbwd_ptr;
-  dsp_.h_rpart_ptr = P.rpart_ptr;
-  upload_vec(keep_[11], P.xfp_off, st);
-  dsp_.xfp_off = keep_[11].as<int64_t>();
-  d_scratch_.alloc(static_cast<size_t>(std::max<int64_t>(P.nxfp, 1)) * sizeof(double));
-  dsp_.xfp = d_scratch_.get();
-  keep_.resize(24);
-  upload_vec(keep_[20], P.dag_fwd, st);
-  upload_vec(keep_[21], P.dag_bwd, st);
-  upload_vec(keep_[22], P.dag_meta, st);
-  keep_[23].alloc(static_cast<size_t>(dag_counter_count(S.ns)) * sizeof(int));
-  d_serr_.alloc(sizeof(int));
-  MPIR_HIP(hipMemsetAsync(d_serr_.get(), 0, sizeof(int), st));
-  dsp_.err = d_serr_.as<int>();
   dsp_.schedule = schedule_;
-  DagArgs& g = dsp_.dag;
+  dsp_.nlevels = S.nlevels;
+  dsp_.h_small_ptr = P.small_ptr;
+  dsp_.h_fx_ptr = P.fx_ptr;
+  dsp_.h_fu_ptr = P.fu_ptr;
+  dsp_.h_br_ptr = P.br_ptr;
+  dsp_.h_bx_ptr = P.bx_ptr;
+  SolveDev& g = dsp_.d;
   g.ns = S.ns;
   g.nfwd = static_cast<int>(P.dag_fwd.size());
   g.nbwd = static_cast<int>(P.dag_bwd.size());
-  g.fwd = keep_[20].as<Task4>();
-  g.bwd = keep_[21].as<Task4>();
-  g.meta = keep_[22].as<int32_t>();
-  g.small_list = dsp_.small_list;
-  g.gather_task = dsp_.gather_task;
-  g.gather_child_ptr = dsp_.gather_child_ptr;
-  g.gather_child = dsp_.gather_child;
-  g.xc_task = dsp_.xc_task;
-  g.u_task = dsp_.u_task;
-  g.rpart_task = dsp_.rpart_task;
-  g.bwd_task = dsp_.bwd_task;
-  g.xfp_off = dsp_.xfp_off;
-  g.part_off = dsp_.part_off;
-  g.counters = keep_[23].as<int>();
+  int k = 17;
+  auto up = [&](const auto& v) {
+    upload_vec(keep_[k], v, st);
+    return keep_[k++].get();
+  };
+  g.fwd = static_cast<const Task4*>(up(P.dag_fwd));
+  g.bwd = static_cast<const Task4*>(up(P.dag_bwd));
+  g.meta = static_cast<const int32_t*>(up(P.dag_meta));
+  g.small_list = dfp_.small_list;
+  g.fx_task = static_cast<const int32_t*>(up(P.fx_task));
+  dfp_.inv_task = g.fx_task;
+  g.fu_task = static_cast<const int32_t*>(up(P.fu_task));
+  g.br_task = static_cast<const int32_t*>(up(P.br_task));
+  g.bx_task = static_cast<const int32_t*>(up(P.bx_task));
+  g.frow_off = static_cast<const int32_t*>(up(P.frow_off));
+  g.grow_ptr = static_cast<const int32_t*>(up(P.grow_ptr));
+  g.gent = static_cast<const Task4*>(up(P.gent));
+  g.cb_base = static_cast<const int32_t*>(up(P.cb_base));
+  g.ypart_off = static_cast<const int64_t*>(up(P.ypart_off));
+  g.upart_off = static_cast<const int64_t*>(up(P.upart_off));
+  g.rpart_off = static_cast<const int64_t*>(up(P.rpart_off));
+  g.xpart_off = static_cast<const int64_t*>(up(P.xpart_off));
+  MPIR_CHECK(k <= 40, "plan upload slots");
+  // work areas in the widest solve precision
+  const int64_t nwork = P.nypart + P.nupart + P.nrpart + P.nxpart + 64;
+  d_scratch_.alloc(static_cast<size_t>(nwork) * sizeof(double));
+  double* wk = d_scratch_.as<double>();
+  g.ypart = wk;
+  g.upart = wk + P.nypart;
+  g.rpart = wk + P.nypart + P.nupart;
+  g.xpart = wk + P.nypart + P.nupart + P.nrpart;
+  // counters: per front, per column block, then the dispensers on their own lines
+  g.cnt_bx = kNumSolveCounters * S.ns;
+  g.cnt_disp = (g.cnt_bx + P.ncb_total + kDispenserStride - 1) / kDispenserStride * kDispenserStride;
+  g.ncounters = g.cnt_disp + 2 * kDispensers * kDispenserStride;
+  d_counters_.alloc(static_cast<size_t>(g.ncounters) * sizeof(int));
+  g.counters = d_counters_.as<int>();
+  d_serr_.alloc(sizeof(int));
+  MPIR_HIP(hipMemsetAsync(d_serr_.get(), 0, sizeof(int), st));
+  g.err = d_serr_.as<int>();
   d_uvec_.alloc(std::max<int64_t>(1, static_cast<int64_t>(S.rows.size())) * sizeof(double));
 }
 
@@ -460,8 +454,8 @@ void GmresLdl::alloc_numeric() {
   d_ax_.alloc(std::max<int64_t>(1, nnz_) * sizeof(double));
   d_axs_.alloc(std::max<int64_t>(1, nnz_) * sizeof(double));
   d_ad_.alloc(n * sizeof(double));
-  d_panel_.alloc(std::max<int64_t>(1, P.panel_entries) * tf);
-  if (uf_ == kF16 || uf_ == kBF16) d_panel16_.alloc(std::max<int64_t>(1, P.panel_entries) * 2);
+  d_panel_.alloc((std::max<int64_t>(1, P.panel_entries) + kPanelSlack) * tf);
+  if (uf_ == kF16 || uf_ == kBF16) d_panel16_.alloc((std::max<int64_t>(1, P.panel_entries) + kPanelSlack) * 2);
   d_nonfinite_.alloc(sizeof(int));
   d_upool_.alloc(std::max<int64_t>(1, P.upd_pool) * tf);
   d_D_.alloc(n * tf);

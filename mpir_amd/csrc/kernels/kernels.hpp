@@ -72,6 +72,10 @@ template <class T>
 void count_positive(int n, const T* D, int* out, hipStream_t st);
 
 // ---------------------------------------------------------------- solves
+// Entries allocated past the end of a panel pool: the solves' 16-byte loads
+// may run up to 7 entries past the last entry of a tile.
+constexpr int64_t kPanelSlack = 64;
+
 // Device view of the factor for the triangular solves (factor storage TF,
 // D^{-1} in TD, arithmetic in TX).
 template <class TF, class TD>
@@ -93,51 +97,48 @@ struct SolveArgs {
 constexpr int kScheduleLevels = 0;
 constexpr int kScheduleDag = 1;
 
-// Dependency-driven schedule: task lists in topological order (kDag* kinds)
-// and per-front counts; counters: kNumSolveCounters * ns completion counts
-// followed by kDispensers task dispensers per direction, each on its own
-// 128-byte line; all zeroed before every solve.
-constexpr int kNumSolveCounters = 6;
-constexpr int kDispensers = 8;
+// Counters of the solves (int), zeroed before every application:
+//   kNumSolveCounters per supernode (completion counts), then one per
+//   column block of the big fronts (bx partial sums in), then kDispensers
+//   task dispensers per direction, each on its own 128-byte line.
+constexpr int kNumSolveCounters = 5;
+constexpr int kDispensers = 16;
 constexpr int kDispenserStride = 32;
-constexpr int dispenser_base(int ns) { return (kNumSolveCounters * ns + kDispenserStride - 1) / kDispenserStride * kDispenserStride; }
-constexpr int dag_counter_count(int ns) { return dispenser_base(ns) + 2 * kDispensers * kDispenserStride; }
-struct DagArgs {
+
+// Device view of the solve plan (task lists, gather table, offsets, work
+// areas of the solve precision).
+struct SolveDev {
   int ns = 0, nfwd = 0, nbwd = 0;
-  const Task4* fwd = nullptr;
+  const Task4* fwd = nullptr;  // dependency-driven task lists
   const Task4* bwd = nullptr;
   const int32_t* meta = nullptr;  // kMetaStride per supernode
   const int32_t* small_list = nullptr;
-  const int32_t* gather_task = nullptr;
-  const int32_t* gather_child_ptr = nullptr;
-  const int32_t* gather_child = nullptr;
-  const int32_t* xc_task = nullptr;
-  const int32_t* u_task = nullptr;
-  const int32_t* rpart_task = nullptr;
-  const int32_t* bwd_task = nullptr;
-  const int64_t* xfp_off = nullptr;
-  const int64_t* part_off = nullptr;
+  const int32_t* fx_task = nullptr;
+  const int32_t* fu_task = nullptr;
+  const int32_t* br_task = nullptr;
+  const int32_t* bx_task = nullptr;
+  const int32_t* frow_off = nullptr;  // row gathers
+  const int32_t* grow_ptr = nullptr;
+  const Task4* gent = nullptr;
+  const int32_t* cb_base = nullptr;
+  const int64_t* ypart_off = nullptr;
+  const int64_t* upart_off = nullptr;
+  const int64_t* rpart_off = nullptr;
+  const int64_t* xpart_off = nullptr;
+  void* ypart = nullptr;  // partial sums (solve precision)
+  void* upart = nullptr;
+  void* rpart = nullptr;
+  void* xpart = nullptr;
   int* counters = nullptr;
+  int cnt_bx = 0, cnt_disp = 0, ncounters = 0;
+  int* err = nullptr;  // bit 1: a dependency wait expired
 };
 
 struct DeviceSolvePlan {
   int schedule = kScheduleDag;
-  DagArgs dag;
-  int* err = nullptr;  // bit 1: a dependency wait expired
   int nlevels = 0;
-  const int32_t* small_list = nullptr;    // small fronts, grouped by level
-  const int32_t* gather_task = nullptr;   // (s, row block) pairs, grouped by level
-  const int32_t* gather_child_ptr = nullptr;  // per gather task + 1
-  const int32_t* gather_child = nullptr;      // (child, q0, q1) triples
-  const int32_t* xc_task = nullptr;       // (s, column block, row chunk) triples
-  const int32_t* u_task = nullptr;        // (s, R row block) pairs
-  const int32_t* bwd_task = nullptr;      // (s, row block) pairs
-  const int32_t* rpart_task = nullptr;    // (s, column block, row chunk) triples
-  const int64_t* part_off = nullptr;      // per supernode
-  void* part = nullptr;                   // partial sums of the solve precision
-  const int64_t* xfp_off = nullptr;       // per supernode
-  void* xfp = nullptr;                    // forward partial sums of big fronts
-  std::vector<int32_t> h_small_ptr, h_gather_ptr, h_xc_ptr, h_u_ptr, h_bwd_ptr, h_rpart_ptr;
+  SolveDev d;
+  std::vector<int32_t> h_small_ptr, h_fx_ptr, h_fu_ptr, h_br_ptr, h_bx_ptr;
 };
 
 // x (length n, permuted order) is overwritten by (L D L^T)^{-1} x.

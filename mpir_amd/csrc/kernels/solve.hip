@@ -5,28 +5,33 @@
 // factor precision TF and every product and sum done in the precision of x,
 // TX, as QDLDL_Lsolve/QDLDL_Ltsolve do with XPREC(Lx[j]).
 //
-// Forward: a front gathers b on its columns, adds its children's update
-// vectors (extend-add), solves its unit-lower diagonal block and leaves
-// u_s = v_R - L_21 y on its rows below for the parent. Backward: a front
-// gathers x on R_s (all ancestors, already final) and solves its block
-// transposed.
+// Small fronts (m <= 64, one wavefront each): gather b and the children's
+// update vectors (extend-add), solve the unit-lower block, leave
+// u_s = v_R - L_21 y for the parent; backward, gather x on R_s (all
+// ancestors, already final) and solve the block transposed.
 //
 // Big fronts hold W = L_11^{-T} (unit upper triangular) below their panel,
 // so their solves are matrix-vector products without a dependency chain:
-//   forward   y_c = W^T v_c  (= L_11^{-1} v_c),  u = v_R - L_21 y_c
-//   backward  x_c = W (D^{-1} y_c - L_21^T x_R)  (= L_11^{-T} (...))
-// Every sum runs in a fixed order, so repeated solves agree bit for bit.
-// The factor is stored in TF (double, float, half or bfloat16), D^{-1} in
-// TD (TF, or float for the 16-bit factors); products and sums run in TX.
+//   forward   y = W^T v,  u = v_R - L_21 y,  and D^{-1} y is kept
+//   backward  z = D^{-1} y - L_21^T x_R,  x = W z
+// Each product is cut into 16K-entry tiles (kXcRows rows x 64 columns for
+// W^T v and L_21^T x_R, 64 rows x kGemvCols columns for L_21 y and W z)
+// read with 16-byte loads; a tile is loaded before its task waits for its
+// inputs. Tiles leave partial sums; the last tile of a block to finish
+// (counted on a per-block counter) adds the block's partials in a fixed
+// order, so repeated solves agree bit for bit. The factor is stored in TF
+// (double, float, half or bfloat16), D^{-1} in TD (TF, or float for the
+// 16-bit factors); products and sums run in TX.
 //
-// Two schedules run the same task bodies, so their results agree bit for bit:
+// Two schedules run the same task bodies, so their results agree bit for
+// bit:
 //   levels: one launch per task kind and level of the supernodal tree;
 //   dag:    one persistent launch per direction. Workgroups take tasks in a
-//           topological order from a dispenser and wait (bounded) on
-//           per-front completion counters, so a front starts as soon as its
-//           own children (forward) or its parent (backward) are done. Values
-//           passed between workgroups of a launch use device-coherent
-//           stores and loads.
+//           topological order from a set of dispensers and wait (bounded)
+//           on per-front completion counters, so a front starts as soon as
+//           its own children (forward) or its parent (backward) are done.
+// Values passed between workgroups use device-coherent stores and loads,
+// and a task is counted only after its stores have completed.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -42,16 +47,29 @@ namespace mpir {
 namespace {
 
 constexpr int kRB = kSolveRB;
-constexpr int kChunk = 2048;  // vector entries staged in LDS at a time (row-block GEMV)
 constexpr int kSpinLimit = 1 << 20;  // polls (about a second) before a wait gives up
-// LDS of one 256-thread solve task: the GEMV stage and its wave reduction
-constexpr int kSmem = kChunk + 4 * kRB;
-static_assert(kXcRows <= kChunk && kRpartRows <= kChunk, "staging buffers share the GEMV stage");
+// LDS of a 256-thread task, in entries of the solve precision: the staged
+// vector of a tile (up to 512 rows for the column products of 16-bit
+// factors, kGemvCols entries for the row products), then the wave
+// reduction of the row products (4 x 64)
+constexpr int kVsLen = 512;
+constexpr int kSmem = kVsLen + 4 * kRB;
+static_assert(kXcRows <= kVsLen && kGemvCols <= kVsLen, "staging buffer");
+
+// Per-front completion counters (counters[k * ns + s]).
+enum : int {
+  kCDone = 0,  // children whose update vector is complete
+  kXDone,      // forward column-block tasks (partial sums of y) done
+  kUDone,      // forward R-block tasks done
+  kRDone,      // backward R-chunk tasks done
+  kBDone,      // backward column blocks with x final (small fronts: 1)
+};
+static_assert(kBDone + 1 == kNumSolveCounters, "counter layout");
 
 // ---------------------------------------------------------------- memory
 // Relaxed device-scope atomic accesses: plain loads and stores with the
-// device-coherence bit, no cache maintenance. C selects them for values
-// that another workgroup of the same launch writes or reads.
+// device-coherence bit, no cache maintenance; used for every value that one
+// task writes and another reads.
 template <class T>
 __device__ __forceinline__ T ld_dev(const T* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -60,15 +78,20 @@ template <class T>
 __device__ __forceinline__ void st_dev(T* p, T v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <bool C, class T>
-__device__ __forceinline__ T ldx(const T* p) {
-  if constexpr (C) return ld_dev(p);
-  else return *p;
+__device__ __forceinline__ int add_dev(int* c, int v) {
+  return __hip_atomic_fetch_add(c, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <bool C, class T>
-__device__ __forceinline__ void stx(T* p, T v) {
-  if constexpr (C) st_dev(p, v);
-  else *p = v;
+
+// 16 bytes of factor entries; the address needs only the element's alignment
+template <class TF>
+struct Pack {
+  TF e[16 / sizeof(TF)];
+};
+template <class TF>
+__device__ __forceinline__ Pack<TF> load_pack(const TF* p) {
+  Pack<TF> r;
+  __builtin_memcpy(&r, p, sizeof r);
+  return r;
 }
 
 // orders the LDS accesses of one wavefront (no other wave takes part)
@@ -91,9 +114,114 @@ __device__ __forceinline__ T lane_value(T v, int j) {
   }
 }
 
+// sum_{k < n} p[k * stride], added in order; the loads go out 8 at a time
+template <class TX>
+__device__ __forceinline__ TX sum_strided(const TX* p, int n, int stride) {
+  TX s = TX(0);
+  for (int k0 = 0; k0 < n; k0 += 8) {
+    TX t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) t[k] = k0 + k < n ? ld_dev(p + static_cast<int64_t>(k0 + k) * stride) : TX(0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += t[k];
+  }
+  return s;
+}
+
+// y of column c of a big front: its partial sums over the row chunks
+template <class TX>
+__device__ __forceinline__ TX forward_value(const TX* yp, int nc, int c) {
+  const int nhx = (nc + kXcRows - 1) / kXcRows;
+  const int j = c / kSolveRB;
+  const int nv = (min(nc, (j + 1) * kSolveRB) + kXcRows - 1) / kXcRows;
+  return sum_strided(yp + static_cast<int64_t>(j) * nhx * kSolveRB + (c % kSolveRB), nv, kSolveRB);
+}
+
+// The children's contributions to one front row (entries [p0, p1) of the
+// gather table, in child order): the child's update-vector entry less the
+// partial sums of L_21 y when the child is big. The first two entries are
+// loaded by pre(), before the task waits; value() loads the values.
+struct RowGather {
+  int p0 = 0, p1 = 0;
+  Task4 e0{0, 0, 0, 0}, e1{0, 0, 0, 0};
+  __device__ __forceinline__ void pre(const SolveDev& d, int frow) {
+    p0 = d.grow_ptr[frow];
+    p1 = d.grow_ptr[frow + 1];
+    e0 = d.gent[p0];
+    e1 = d.gent[p0 + 1];
+  }
+  template <class TX>
+  __device__ __forceinline__ TX value(const SolveDev& d, const TX* uvec, TX v) const {
+    const TX* up = static_cast<const TX*>(d.upart);
+    for (int p = p0; p < p1; ++p) {
+      const Task4 e = p == p0 ? e0 : p == p0 + 1 ? e1 : d.gent[p];
+      v += ld_dev(uvec + e.a) - sum_strided(up + e.b, e.c, kSolveRB);
+    }
+    return v;
+  }
+};
+
+// ---------------------------------------------------------------- waits and counts
+// Waits until *c >= target. The wait is bounded: on expiry it records bit 1
+// in *err and returns (the result is then wrong and the host reports it).
+__device__ __forceinline__ void spin_until(const int* c, int target, int* err) {
+  int it = 0;
+  while (ld_dev(c) < target) {
+    __builtin_amdgcn_s_sleep(1);
+    if (++it > kSpinLimit) {
+      atomicOr(err, 2);
+      break;
+    }
+  }
+}
 struct NoWait {
   __device__ void operator()() const {}
 };
+struct BlockWait {
+  const int* c;
+  int target;
+  int* err;
+  __device__ void operator()() const {
+    if (threadIdx.x == 0) spin_until(c, target, err);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __syncthreads();
+  }
+};
+struct WaveWait {
+  const int* c;
+  int target;
+  int* err;
+  __device__ void operator()() const {
+    if ((threadIdx.x & 63) == 0) spin_until(c, target, err);
+    wave_sync();
+  }
+};
+// Called by every thread of the block after its last store of a task: once
+// all those stores are complete, thread 0 counts the task and returns the
+// counter's previous value (other threads: -1).
+__device__ __forceinline__ int block_publish(int* c) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  return threadIdx.x == 0 ? add_dev(c, 1) : -1;
+}
+// block_publish for one of `target` tasks of a block of output: true in
+// every thread of the task that completes the block
+__device__ __forceinline__ bool publish_last(int* c, int target, int* flag) {
+  const int prev = block_publish(c);
+  if (threadIdx.x == 0) *flag = prev == target - 1;
+  __syncthreads();
+  const bool last = *flag;
+  if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  return last;
+}
+// The same for a task of one wavefront.
+__device__ __forceinline__ void wave_publish(int* c) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0);
+  wave_sync();
+  if ((threadIdx.x & 63) == 0) add_dev(c, 1);
+}
 
 // ---------------------------------------------------------------- small fronts
 // One wavefront per front (m <= 64), lane = row; v: 64 entries of LDS. L is
@@ -102,8 +230,9 @@ struct NoWait {
 // task writes is read.
 constexpr int kLC = 16;
 
-template <bool C, class TF, class TD, class TX, class Wait>
-__device__ __forceinline__ void small_fwd(const SolveArgs<TF, TD>& a, int s, TX* x, TX* uvec, TX* v, Wait wait) {
+template <class TF, class TD, class TX, class Wait>
+__device__ __forceinline__ void small_fwd(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, TX* x, TX* uvec,
+                                          Wait wait) {
   const int lane = threadIdx.x & 63;
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
@@ -118,20 +247,12 @@ __device__ __forceinline__ void small_fwd(const SolveArgs<TF, TD>& a, int s, TX*
     for (int u = 0; u < kLC; ++u) lr[u] = prow[static_cast<int64_t>(min(j0 + u, nc - 1)) * m];
   };
   load(0);
+  RowGather G;
+  if (lane < m) G.pre(d, d.frow_off[s] + lane);
   const TX b = x[f + min(lane, nc - 1)];
   const TD dinv = a.Dinv[f + min(lane, nc - 1)];
-  v[lane] = lane < nc ? b : TX(0);
   wait();
-  for (int t = a.child_ptr[s]; t < a.child_ptr[s + 1]; ++t) {
-    const int c = a.child_list[t];
-    const int nrc = a.nrow[c];
-    if (lane < nrc) {
-      const int64_t q = a.rows_off[c] + lane;
-      v[a.relidx[q]] += ldx<C>(uvec + q);
-    }
-    wave_sync();
-  }
-  TX vl = v[lane];
+  TX vl = G.value(d, uvec, lane < nc ? b : TX(0));
   for (int j0 = 0; j0 < nc; j0 += kLC) {
     if (j0 > 0) load(j0);
 #pragma unroll
@@ -144,10 +265,10 @@ __device__ __forceinline__ void small_fwd(const SolveArgs<TF, TD>& a, int s, TX*
     }
   }
   if (lane < nc) x[f + lane] = vl * static_cast<TX>(dinv);
-  else if (lane < m) stx<C>(uvec + a.rows_off[s] + (lane - nc), vl);
+  else if (lane < m) st_dev(uvec + a.rows_off[s] + (lane - nc), vl);
 }
 
-template <bool C, class TF, class TD, class TX, class Wait>
+template <class TF, class TD, class TX, class Wait>
 __device__ __forceinline__ void small_bwd(const SolveArgs<TF, TD>& a, int s, TX* x, TX* w, Wait wait) {
   const int lane = threadIdx.x & 63;
   const int f = a.first[s];
@@ -172,7 +293,7 @@ __device__ __forceinline__ void small_bwd(const SolveArgs<TF, TD>& a, int s, TX*
   const TX own = x[f + cl];
   wait();
   if (lane < nc) w[lane] = own;
-  else if (lane < m) w[lane] = ldx<C>(x + row);
+  else if (lane < m) w[lane] = ld_dev(x + row);
   wave_sync();
   // lane j < nc: z_j = w_j - sum over rows r >= nc of L[r, j] w_r, r ascending
   TX z = own;
@@ -194,436 +315,401 @@ __device__ __forceinline__ void small_bwd(const SolveArgs<TF, TD>& a, int s, TX*
       }
     }
   }
-  if (lane < nc) stx<C>(x + f + lane, z);
+  if (lane < nc) st_dev(x + f + lane, z);
 }
 
-// ---------------------------------------------------------------- big fronts
-// Forward gather, 64-row block i of front s (one wavefront): v = [b on
-// columns; 0 on R] + the children's update vectors (entries [e0, e1) of cl),
-// children in fixed order; the column part goes back into x, the R part into
-// the front's update vector.
-template <bool C, class TF, class TD, class TX>
-__device__ __forceinline__ void gather_rows(const SolveArgs<TF, TD>& a, int s, int i, const int32_t* __restrict__ cl,
-                                            int e0, int e1, TX* x, TX* uvec, TX* vsh) {
-  const int lane = threadIdx.x & 63;
-  const int f = a.first[s];
-  const int nc = a.first[s + 1] - f;
-  const int m = nc + a.nrow[s];
-  const int r0 = i * kRB;
-  const int r = r0 + lane;
-  vsh[lane] = r < nc ? x[f + r] : TX(0);
-  wave_sync();
-  for (int e = e0; e < e1; ++e) {
-    const int c = cl[3 * e], q0 = cl[3 * e + 1], q1 = cl[3 * e + 2];
-    const int q = q0 + lane;
-    if (q < q1) {
-      const int64_t o = a.rows_off[c] + q;
-      vsh[a.relidx[o] - r0] += ldx<C>(uvec + o);
-    }
-    wave_sync();
-  }
-  if (r < nc) stx<C>(x + f + r, vsh[lane]);
-  else if (r < m) stx<C>(uvec + a.rows_off[s] + (r - nc), vsh[lane]);
-}
-
-// Dot products of the bw (<= 64) columns base + t * ld, rows [0, len), with
-// vs (LDS): out[t] = sum_q base[q + t * ld] vs[q]. Wave w takes columns
-// 4w .. 4w + 3, then 16 further, lanes stride the rows (coalesced); four
-// row strides are loaded before use so each wave keeps 16 loads in flight.
-// Partial sums are combined across lanes in a fixed order.
-template <bool C, class TF, class TX>
-__device__ __forceinline__ void column_dots(const TF* __restrict__ base, int64_t ld, int bw, int len,
-                                            const TX* __restrict__ vs, TX* out) {
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  for (int t0 = w * 4; t0 < bw; t0 += 16) {
-    const TF* col[4];
+// ---------------------------------------------------------------- big-front tiles
+// Column products: up to kXcRows rows x 64 columns, entry (q, t) at
+// base[q + t * ld]; out[t] = sum_q M[q, t] vs[q]. A lane loads N = 16 /
+// sizeof(TF) consecutive rows, a wave load covers 64 N rows of one column,
+// and wave w takes columns 16 w .. 16 w + 15. Loads go in batches of at
+// most 16 per lane; batch 0 is issued before the task's wait.
+template <class TF>
+struct ColTile {
+  static constexpr int N = 16 / sizeof(TF);
+  static constexpr int RPL = 64 * N;                     // rows per wave load
+  static constexpr int NLC = (kXcRows + RPL - 1) / RPL;  // loads per column
+  static constexpr int NL = 16 * NLC;                    // loads per lane
+  static constexpr int B = NL < 16 ? NL : 16;
+  static constexpr int NB = NL / B;
+  static_assert(NL % B == 0 && NLC * RPL <= kVsLen, "tile shape");
+  Pack<TF> buf[B];
+  const TF* base;
+  int64_t ld;
+  int bw, len;
+  __device__ __forceinline__ void load(int batch) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) col[u] = base + min(t0 + u, bw - 1) * ld;
-    TX acc[4] = {TX(0), TX(0), TX(0), TX(0)};
-    int q = lane;
-    for (; q + 192 < len; q += 256) {
-      TF lv[4][4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int u = 0; u < 4; ++u) lv[r][u] = col[u][q + 64 * r];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const TX v = vs[q + 64 * r];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc[u] += static_cast<TX>(lv[r][u]) * v;
-      }
-    }
-    for (; q < len; q += 64) {
-      const TX v = vs[q];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] += static_cast<TX>(col[u][q]) * v;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      TX v = acc[u];
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0 && t0 + u < bw) stx<C>(out + t0 + u, v);
+    for (int u = 0; u < B; ++u) {
+      const int k = batch * B + u;
+      const int t = min(w * 16 + k / NLC, bw - 1);
+      const int r = (k % NLC) * RPL + N * lane;
+      // rows past the tile read the tile's first rows instead (masked)
+      buf[u] = load_pack(base + (r < len ? r : 0) + t * ld);
     }
   }
+};
+
+// The tile's column sums (vs: zeros from len to kVsLen) stored to out[t],
+// t < bw. The 16 sums of each lane are reduced across the wave by halving
+// exchanges (fixed order: 17 exchanges instead of 16 x 6).
+template <class TF, class TX>
+__device__ __forceinline__ void col_dots(ColTile<TF>& T, const TX* vs, TX* out) {
+  using CT = ColTile<TF>;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  TX v[CT::NLC][CT::N];
+  bool ok[CT::NLC][CT::N];
+#pragma unroll
+  for (int g = 0; g < CT::NLC; ++g)
+#pragma unroll
+    for (int e = 0; e < CT::N; ++e) {
+      const int r = g * CT::RPL + CT::N * lane + e;
+      v[g][e] = vs[r];
+      ok[g][e] = r < T.len;
+    }
+  TX acc[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) acc[c] = TX(0);
+#pragma unroll
+  for (int b = 0; b < CT::NB; ++b) {
+    if (b > 0) T.load(b);
+#pragma unroll
+    for (int u = 0; u < CT::B; ++u) {
+      const int k = b * CT::B + u, c = k / CT::NLC, g = k % CT::NLC;
+#pragma unroll
+      for (int e = 0; e < CT::N; ++e) acc[c] += (ok[g][e] ? static_cast<TX>(T.buf[u].e[e]) : TX(0)) * v[g][e];
+    }
+  }
+  // halving exchanges: after the step with offset o, a lane keeps the half
+  // of its columns selected by its bit o
+  TX a8[8], a4[4], a2[2];
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a8[i] = (b5 ? acc[i + 8] : acc[i]) + __shfl_xor(b5 ? acc[i] : acc[i + 8], 32);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a4[i] = (b4 ? a8[i + 4] : a8[i]) + __shfl_xor(b4 ? a8[i] : a8[i + 4], 16);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) a2[i] = (b3 ? a4[i + 2] : a4[i]) + __shfl_xor(b3 ? a4[i] : a4[i + 2], 8);
+  TX a1 = (b2 ? a2[1] : a2[0]) + __shfl_xor(b2 ? a2[0] : a2[1], 4);
+  a1 += __shfl_xor(a1, 2);
+  a1 += __shfl_xor(a1, 1);
+  const int c = (b5 ? 8 : 0) + (b4 ? 4 : 0) + (b3 ? 2 : 0) + (b2 ? 1 : 0);
+  if ((lane & 3) == 0 && w * 16 + c < T.bw) st_dev(out + w * 16 + c, a1);
 }
 
-// Forward, rows [h * kXcRows, ...) of column block j of front s: partial sums
-// of y[col] = sum_{i <= col} W[i, col] v_i over the block's columns (W is
-// zero below its diagonal, so whole 64-row tiles are summed).
-template <bool C, class TF, class TD, class TX>
-__device__ __forceinline__ void fwd_xc(const SolveArgs<TF, TD>& a, int s, int j, int h,
-                                       const int64_t* __restrict__ xfp_off, const TX* x, TX* xfp, TX* vs) {
-  const int f = a.first[s];
-  const int nc = a.first[s + 1] - f;
-  const int nr = a.nrow[s];
-  const int m = nc + nr;
-  const int64_t ld = panel_ld(nc, nr);
-  const int c0 = j * kRB;
-  const int bw = min(kRB, nc - c0);
-  const int q0 = h * kXcRows;
-  const int len = min(kXcRows, c0 + bw - q0);
-  const int nhx = (nc + kXcRows - 1) / kXcRows;
-  for (int q = threadIdx.x; q < len; q += blockDim.x) vs[q] = ldx<C>(x + f + q0 + q);
+// Row products: 64 rows x up to kGemvCols columns, entry (r, c) at
+// base[r + c * ld]; sum_c M[r, c] ys[c]. A wave load covers N columns of 64
+// rows (lane l: rows N (l % LPC) .., column l / LPC of the group); wave w
+// takes columns 64 w .. 64 w + 63. Loads go in batches of at most 16 per
+// lane; batch 0 is issued before the task's wait.
+template <class TF>
+struct RowTile {
+  static constexpr int N = 16 / sizeof(TF);
+  static constexpr int LPC = 64 / N;  // lanes per column
+  static constexpr int NL = 64 / N;   // loads per lane
+  static constexpr int B = NL < 16 ? NL : 16;
+  static constexpr int NB = NL / B;
+  static_assert(NL % B == 0 && 4 * 64 == kGemvCols, "tile shape");
+  Pack<TF> buf[B];
+  const TF* base;
+  int64_t ld;
+  int rows, cc;
+  __device__ __forceinline__ void load(int batch) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = N * (lane % LPC);
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int g = batch * B + u;
+      const int c = min(w * 64 + g * N + lane / LPC, cc - 1);
+      // rows past the tile read the tile's first rows instead (results unused)
+      buf[u] = load_pack(base + (r < rows ? r : 0) + c * ld);
+    }
+  }
+};
+
+// The tile's row sums for threads 0..63 (other threads: 0). ys: kGemvCols
+// entries. red: 4 x 64 entries of LDS.
+template <class TF, class TX>
+__device__ __forceinline__ TX row_gemv(RowTile<TF>& T, const TX* ys, TX (*red)[kRB]) {
+  using RT = RowTile<TF>;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  TX acc[RT::N];
+#pragma unroll
+  for (int e = 0; e < RT::N; ++e) acc[e] = TX(0);
+#pragma unroll
+  for (int b = 0; b < RT::NB; ++b) {
+    if (b > 0) T.load(b);
+#pragma unroll
+    for (int u = 0; u < RT::B; ++u) {
+      const int c = w * 64 + (b * RT::B + u) * RT::N + lane / RT::LPC;
+      const bool ok = c < T.cc;
+      const TX y = ys[c];
+#pragma unroll
+      for (int e = 0; e < RT::N; ++e) acc[e] += (ok ? static_cast<TX>(T.buf[u].e[e]) : TX(0)) * y;
+    }
+  }
+  // lanes l, l + LPC, ... hold the same rows
+#pragma unroll
+  for (int off = RT::LPC; off < 64; off <<= 1)
+#pragma unroll
+    for (int e = 0; e < RT::N; ++e) acc[e] += __shfl_xor(acc[e], off);
+  if (lane < RT::LPC)
+#pragma unroll
+    for (int e = 0; e < RT::N; ++e) red[w][RT::N * lane + e] = acc[e];
   __syncthreads();
-  const TF* base = a.panel + a.panel_off[s] + m + q0 + c0 * ld;
-  column_dots<C, TF, TX>(base, ld, bw, len, vs, xfp + xfp_off[s] + (static_cast<int64_t>(j) * nhx + h) * kRB);
+  const int r = threadIdx.x;
+  return r < kRB ? (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]) : TX(0);
 }
 
-// y_c of column c of a big front after the forward pass: the sum of its
-// partials.
-template <bool C, class TX>
-__device__ __forceinline__ TX forward_value(const TX* xfp_s, int nc, int c) {
-  const int nhx = (nc + kXcRows - 1) / kXcRows;
-  const int j = c / kRB;
-  const int nv = (min(nc, (j + 1) * kRB) + kXcRows - 1) / kXcRows;
-  const TX* pp = xfp_s + static_cast<int64_t>(j) * nhx * kRB + (c % kRB);
-  TX v = TX(0);
-  for (int h = 0; h < nv; ++h) v += ldx<C>(pp + static_cast<int64_t>(h) * kRB);
-  return v;
+// ---------------------------------------------------------------- big-front tasks
+struct FrontShape {
+  int f, nc, nr, m;
+  int64_t ld;
+  template <class TF, class TD>
+  __device__ __forceinline__ FrontShape(const SolveArgs<TF, TD>& a, int s) {
+    f = a.first[s];
+    nc = a.first[s + 1] - f;
+    nr = a.nrow[s];
+    m = nc + nr;
+    ld = panel_ld(nc, nr);
+  }
+};
+
+// Forward, rows [h kXcRows, ...) of column block j: gathers v on those rows
+// (b plus the children's contributions) and leaves partial sums of W^T v.
+template <class TF, class TD, class TX, class Wait>
+__device__ __forceinline__ void fx_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int j, int h, const TX* x,
+                                        const TX* uvec, TX* smem, Wait wait) {
+  static_assert(kXcRows == 256, "one row per thread");
+  const FrontShape F(a, s);
+  const int tid = threadIdx.x;
+  const int c0 = j * kRB, bw = min(kRB, F.nc - c0);
+  const int q0 = h * kXcRows, len = min(kXcRows, c0 + bw - q0);
+  ColTile<TF> T;
+  T.base = a.panel + a.panel_off[s] + F.m + q0 + c0 * F.ld;
+  T.ld = F.ld;
+  T.bw = bw;
+  T.len = len;
+  T.load(0);
+  RowGather G;
+  TX bq = TX(0);
+  if (tid < len) {
+    G.pre(d, d.frow_off[s] + q0 + tid);
+    bq = x[F.f + q0 + tid];
+  }
+  wait();
+  TX* vs = smem;
+  vs[tid] = tid < len ? G.value(d, uvec, bq) : TX(0);
+  for (int q = tid + blockDim.x; q < kVsLen; q += blockDim.x) vs[q] = TX(0);
+  __syncthreads();
+  const int nhx = (F.nc + kXcRows - 1) / kXcRows;
+  TX* yp = static_cast<TX*>(d.ypart) + d.ypart_off[s] + (static_cast<int64_t>(j) * nhx + h) * kRB;
+  col_dots<TF, TX>(T, vs, yp);
+  block_publish(d.counters + kXDone * d.ns + s);
 }
 
-// Row-block GEMV over a panel: returns sum_{c in [cb, ce)} M[r, c] y_c for the
-// row r = lane (< rows) of the 64-row block at M, with y_c = stage(c) staged
-// in LDS (smem: kSmem entries). Each lane owns a row, the four waves split
-// the columns (16 loads in flight each), and the wave results are added in
-// a fixed order.
-template <class TF, class TX, class Stage>
-__device__ __forceinline__ TX rowblock_gemv(const TF* __restrict__ M, int64_t ld, int rows, int cb, int ce,
-                                            Stage stage, TX* smem) {
+// Forward, R block b, columns [h kGemvCols, ...): partial sums of L_21 y;
+// the chunk-0 task also gathers and stores v_R. The front's last task
+// counts the front as a finished child of its parent.
+template <class TF, class TD, class TX, class Wait>
+__device__ __forceinline__ void fu_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int b, int h, TX* uvec,
+                                        TX* smem, Wait wait) {
+  const FrontShape F(a, s);
+  const int tid = threadIdx.x;
+  const int rb0 = b * kRB, rows = min(kRB, F.nr - rb0);
+  const int cs = h * kGemvCols, cc = min(kGemvCols, F.nc - cs);
+  RowTile<TF> T;
+  T.base = a.panel + a.panel_off[s] + F.nc + rb0 + cs * F.ld;
+  T.ld = F.ld;
+  T.rows = rows;
+  T.cc = cc;
+  T.load(0);
+  const bool gathers = h == 0 && tid < rows;
+  RowGather G;
+  if (gathers) G.pre(d, d.frow_off[s] + F.nc + rb0 + tid);
+  wait();
+  const TX* yp = static_cast<const TX*>(d.ypart) + d.ypart_off[s];
   TX* ys = smem;
-  TX(*red)[kRB] = reinterpret_cast<TX(*)[kRB]>(smem + kChunk);
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int rr = lane < rows ? lane : 0;
-  TX acc = TX(0);
-  for (int q0 = cb; q0 < ce; q0 += kChunk) {
-    const int len = min(kChunk, ce - q0);
-    __syncthreads();
-    for (int q = tid; q < len; q += blockDim.x) ys[q] = stage(q0 + q);
-    __syncthreads();
-    // wave w: columns [w * per, (w + 1) * per) of the chunk, 16 at a time
-    const int per = (len + 3) / 4;
-    const int cs = min(len, w * per), cend = min(len, cs + per);
-    const TF* mp = M + rr + (q0 + cs) * ld;
-    int c = cs;
-    for (; c + 16 <= cend; c += 16, mp += 16 * ld) {
-      TF lv[16];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) lv[u] = mp[u * ld];
-#pragma unroll
-      for (int u = 0; u < 16; ++u) acc += static_cast<TX>(lv[u]) * ys[c + u];
+  for (int c = tid; c < kGemvCols; c += blockDim.x) ys[c] = c < cc ? forward_value(yp, F.nc, cs + c) : TX(0);
+  const TX vr = gathers ? G.value(d, static_cast<const TX*>(uvec), TX(0)) : TX(0);
+  __syncthreads();
+  const TX sum = row_gemv<TF, TX>(T, ys, reinterpret_cast<TX(*)[kRB]>(smem + kVsLen));
+  const int nh = (F.nc + kGemvCols - 1) / kGemvCols;
+  TX* up = static_cast<TX*>(d.upart) + d.upart_off[s] + static_cast<int64_t>(b) * nh * kRB;
+  if (tid < rows) st_dev(up + h * kRB + tid, sum);
+  if (gathers) st_dev(uvec + a.rows_off[s] + rb0 + tid, vr);
+  const int prev = block_publish(d.counters + kUDone * d.ns + s);
+  const int* ms = d.meta + kMetaStride * s;
+  if (prev == ms[kMetaNFu] - 1 && ms[kMetaParent] >= 0) add_dev(d.counters + kCDone * d.ns + ms[kMetaParent], 1);
+}
+
+// Backward, R rows [h kXcRows, ...) of column block j: partial sums of
+// L_21^T x_R.
+template <class TF, class TD, class TX, class Wait>
+__device__ __forceinline__ void br_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int j, int h,
+                                        const TX* x, TX* smem, Wait wait) {
+  const FrontShape F(a, s);
+  const int tid = threadIdx.x;
+  const int c0 = j * kRB, bw = min(kRB, F.nc - c0);
+  const int q0 = h * kXcRows, len = min(kXcRows, F.nr - q0);
+  ColTile<TF> T;
+  T.base = a.panel + a.panel_off[s] + F.nc + q0 + c0 * F.ld;
+  T.ld = F.ld;
+  T.bw = bw;
+  T.len = len;
+  T.load(0);
+  wait();
+  const int32_t* R = a.rows + a.rows_off[s] + q0;
+  TX* vs = smem;
+  for (int q = tid; q < kVsLen; q += blockDim.x) vs[q] = q < len ? ld_dev(x + R[q]) : TX(0);
+  __syncthreads();
+  const int nhr = (F.nr + kXcRows - 1) / kXcRows;
+  col_dots<TF, TX>(T, vs, static_cast<TX*>(d.rpart) + d.rpart_off[s] + (static_cast<int64_t>(j) * nhr + h) * kRB);
+  block_publish(d.counters + kRDone * d.ns + s);
+}
+
+// Backward, row block i, columns [max(h kGemvCols, 64 i), ...): partial sums
+// of W z with z = D^{-1} y - L_21^T x_R; the block's last task stores x.
+template <class TF, class TD, class TX, class Wait>
+__device__ __forceinline__ void bx_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int i, int h, TX* x,
+                                        TX* smem, int* flag, Wait wait) {
+  const FrontShape F(a, s);
+  const int tid = threadIdx.x;
+  const int r0 = i * kRB, rows = min(kRB, F.nc - r0);
+  const int cs = max(h * kGemvCols, r0), cc = min(F.nc, (h + 1) * kGemvCols) - cs;
+  RowTile<TF> T;
+  T.base = a.panel + a.panel_off[s] + F.m + r0 + cs * F.ld;
+  T.ld = F.ld;
+  T.rows = rows;
+  T.cc = cc;
+  T.load(0);
+  wait();
+  const int nhr = (F.nr + kXcRows - 1) / kXcRows;
+  const TX* rp = static_cast<const TX*>(d.rpart) + (nhr > 0 ? d.rpart_off[s] : 0);
+  const TX* yp = static_cast<const TX*>(d.ypart) + d.ypart_off[s];
+  TX* ys = smem;
+  for (int c = tid; c < kGemvCols; c += blockDim.x) {
+    TX z = TX(0);
+    if (c < cc) {
+      const int col = cs + c;
+      const TX rs = sum_strided(rp + static_cast<int64_t>(col / kRB) * nhr * kRB + (col % kRB), nhr, kRB);
+      z = static_cast<TX>(a.Dinv[F.f + col]) * forward_value(yp, F.nc, col) - rs;
     }
-    for (; c < cend; ++c, mp += ld) acc += static_cast<TX>(*mp) * ys[c];
+    ys[c] = z;
   }
-  red[w][lane] = acc;
   __syncthreads();
-  return (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
-}
-
-// Forward, 64-row block b of R_s: u = v_R - L_21 y.
-template <bool C, class TF, class TD, class TX>
-__device__ __forceinline__ void fwd_u(const SolveArgs<TF, TD>& a, int s, int b, const int64_t* __restrict__ xfp_off,
-                                      const TX* xfp, TX* uvec, TX* smem) {
-  const int f = a.first[s];
-  const int nc = a.first[s + 1] - f;
-  const int nr = a.nrow[s];
-  const int64_t ld = panel_ld(nc, nr);
-  const int rb0 = b * kRB;
-  const int rows = min(kRB, nr - rb0);
-  const TF* M = a.panel + a.panel_off[s] + nc + rb0;
-  const TX* xs = xfp + xfp_off[s];
-  const TX sum =
-      rowblock_gemv<TF, TX>(M, ld, rows, 0, nc, [=](int c) { return forward_value<C>(xs, nc, c); }, smem);
-  const int lane = threadIdx.x;
-  if (lane < rows) {
-    TX* u = uvec + a.rows_off[s] + rb0 + lane;
-    stx<C>(u, ldx<C>(u) - sum);
+  const TX sum = row_gemv<TF, TX>(T, ys, reinterpret_cast<TX(*)[kRB]>(smem + kVsLen));
+  const int nh = (F.nc + kGemvCols - 1) / kGemvCols;
+  const int h0 = r0 / kGemvCols;
+  TX* xp = static_cast<TX*>(d.xpart) + d.xpart_off[s] + static_cast<int64_t>(i) * nh * kRB;
+  if (tid < rows) st_dev(xp + h * kRB + tid, sum);
+  if (publish_last(d.counters + d.cnt_bx + d.cb_base[s] + i, nh - h0, flag)) {
+    if (tid < rows) st_dev(x + F.f + r0 + tid, sum_strided(xp + h0 * kRB + tid, nh - h0, kRB));
+    block_publish(d.counters + kBDone * d.ns + s);
   }
-}
-
-// Backward, rows below the columns of a front (R rows, all final): partial
-// sums part[(j * nch + c) * 64 + t] = sum over the chunk's rows r of
-// L[nc + r, 64 j + t] x[R_r].
-template <bool C, class TF, class TD, class TX>
-__device__ __forceinline__ void bwd_rpart(const SolveArgs<TF, TD>& a, int s, int j, int ch,
-                                          const int64_t* __restrict__ part_off, const TX* x, TX* part, TX* xs) {
-  const int f = a.first[s];
-  const int nc = a.first[s + 1] - f;
-  const int nr = a.nrow[s];
-  const int64_t ld = panel_ld(nc, nr);
-  const int c0 = j * kRB;
-  const int bw = min(kRB, nc - c0);
-  const int nch = (nr + kRpartRows - 1) / kRpartRows;
-  const int32_t* R = a.rows + a.rows_off[s];
-  const int q0 = ch * kRpartRows, len = min(kRpartRows, nr - q0);
-  for (int q = threadIdx.x; q < len; q += blockDim.x) xs[q] = ldx<C>(x + R[q0 + q]);
-  __syncthreads();
-  const TF* base = a.panel + a.panel_off[s] + nc + q0 + c0 * ld;
-  column_dots<C, TF, TX>(base, ld, bw, len, xs,
-                         part + part_off[s] + static_cast<int64_t>(j) * nch * kRB + static_cast<int64_t>(ch) * kRB);
-}
-
-// Backward, 64-row block i of the columns of front s:
-// x_r = sum_{c >= 64 i} W[r, c] z_c, z = D^{-1} y - (partial sums of L_21^T x_R).
-template <bool C, class TF, class TD, class TX>
-__device__ __forceinline__ void bwd_xc(const SolveArgs<TF, TD>& a, int s, int i, const int64_t* __restrict__ part_off,
-                                       const TX* part, const int64_t* __restrict__ xfp_off, const TX* xfp, TX* x,
-                                       TX* smem) {
-  const int f = a.first[s];
-  const int nc = a.first[s + 1] - f;
-  const int nr = a.nrow[s];
-  const int m = nc + nr;
-  const int64_t ld = panel_ld(nc, nr);
-  const int r0 = i * kRB;
-  const int rows = min(kRB, nc - r0);
-  const int nch = (nr + kRpartRows - 1) / kRpartRows;
-  const TF* M = a.panel + a.panel_off[s] + m + r0;
-  const TX* xs = xfp + xfp_off[s];
-  const TX* ps = part + (nch > 0 ? part_off[s] : 0);
-  const TD* Dinv = a.Dinv + f;
-  const TX sum = rowblock_gemv<TF, TX>(
-      M, ld, rows, r0, nc,
-      [=](int c) {
-        const TX z = static_cast<TX>(Dinv[c]) * forward_value<false>(xs, nc, c);
-        const TX* pp = ps + static_cast<int64_t>(c / kRB) * nch * kRB + (c % kRB);
-        TX rs = TX(0);
-        for (int h = 0; h < nch; ++h) rs += ldx<C>(pp + static_cast<int64_t>(h) * kRB);
-        return z - rs;
-      },
-      smem);
-  const int lane = threadIdx.x;
-  if (lane < rows) stx<C>(x + f + r0 + lane, sum);
 }
 
 // ---------------------------------------------------------------- level schedule
 template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(64) k_fwd_small(SolveArgs<TF, TD> a, const int32_t* __restrict__ list, TX* x,
-                                                  TX* uvec) {
-  __shared__ TX v[kSmallFront];
-  small_fwd<false>(a, list[blockIdx.x], x, uvec, v, NoWait{});
+__global__ void __launch_bounds__(64) k_fwd_small(SolveArgs<TF, TD> a, SolveDev d, const int32_t* __restrict__ list,
+                                                  TX* x, TX* uvec) {
+  small_fwd(a, d, list[blockIdx.x], x, uvec, NoWait{});
 }
 
 template <class TF, class TD, class TX>
 __global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF, TD> a, const int32_t* __restrict__ list, TX* x) {
   __shared__ TX w[kSmallFront];
-  small_bwd<false>(a, list[blockIdx.x], x, w, NoWait{});
+  small_bwd(a, list[blockIdx.x], x, w, NoWait{});
 }
 
+// kind: kDagFx, kDagFu, kDagBr or kDagBx; tasks: the level's triples
 template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(64) k_fwd_gather(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
-                                                   const int32_t* __restrict__ cptr, const int32_t* __restrict__ cl,
-                                                   TX* x, TX* uvec) {
-  __shared__ TX vsh[kRB];
-  gather_rows<false>(a, tasks[2 * blockIdx.x], tasks[2 * blockIdx.x + 1], cl, cptr[blockIdx.x], cptr[blockIdx.x + 1],
-                     x, uvec, vsh);
-}
-
-template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(256) k_fwd_xc(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
-                                                const int64_t* __restrict__ xfp_off, const TX* x, TX* xfp) {
-  __shared__ TX vs[kXcRows];
-  fwd_xc<false>(a, tasks[3 * blockIdx.x], tasks[3 * blockIdx.x + 1], tasks[3 * blockIdx.x + 2], xfp_off, x, xfp, vs);
-}
-
-template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(256) k_fwd_u(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
-                                               const int64_t* __restrict__ xfp_off, const TX* xfp, TX* uvec) {
+__global__ void __launch_bounds__(256) k_level_tiles(SolveArgs<TF, TD> a, SolveDev d, int kind,
+                                                     const int32_t* __restrict__ tasks, TX* x, TX* uvec) {
   __shared__ TX smem[kSmem];
-  fwd_u<false>(a, tasks[2 * blockIdx.x], tasks[2 * blockIdx.x + 1], xfp_off, xfp, uvec, smem);
-}
-
-template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(256) k_bwd_rpart(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
-                                                   const int64_t* __restrict__ part_off, const TX* x, TX* part) {
-  __shared__ TX xs[kRpartRows];
-  bwd_rpart<false>(a, tasks[3 * blockIdx.x], tasks[3 * blockIdx.x + 1], tasks[3 * blockIdx.x + 2], part_off, x, part,
-                   xs);
-}
-
-template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(256) k_bwd_xc(SolveArgs<TF, TD> a, const int32_t* __restrict__ tasks,
-                                                const int64_t* __restrict__ part_off, const TX* part,
-                                                const int64_t* __restrict__ xfp_off, const TX* xfp, TX* x) {
-  __shared__ TX smem[kSmem];
-  bwd_xc<false>(a, tasks[2 * blockIdx.x], tasks[2 * blockIdx.x + 1], part_off, part, xfp_off, xfp, x, smem);
+  __shared__ int flag;
+  const int* q = tasks + 3 * blockIdx.x;
+  if (kind == kDagFx) fx_task(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), uvec, smem, NoWait{});
+  else if (kind == kDagFu) fu_task(a, d, q[0], q[1], q[2], uvec, smem, NoWait{});
+  else if (kind == kDagBr) br_task(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), smem, NoWait{});
+  else bx_task(a, d, q[0], q[1], q[2], x, smem, &flag, NoWait{});
 }
 
 // ---------------------------------------------------------------- dag schedule
-// Completion counters of front s (counters[k * ns + s]):
-enum : int {
-  kCDone = 0,  // children whose forward update vector is complete
-  kGDone,      // forward gather tasks done
-  kXDone,      // forward column-block tasks done
-  kUDone,      // forward R-block tasks done
-  kRDone,      // backward R-partial tasks done
-  kBDone,      // backward column tasks done (small fronts: 1)
-};
-static_assert(kBDone + 1 == kNumSolveCounters, "counter layout");
-
-// Waits until *c >= target. The wait is bounded: on expiry it records bit 1
-// in *err and returns (the result is then wrong and the caller reports it).
-__device__ __forceinline__ void spin_until(const int* c, int target, int* err) {
-  int it = 0;
-  while (ld_dev(c) < target) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++it > kSpinLimit) {
-      atomicOr(err, 2);
-      break;
-    }
-  }
-}
-struct BlockWait {
-  const int* c;
-  int target;
-  int* err;
-  __device__ void operator()() const {
-    if (threadIdx.x == 0) spin_until(c, target, err);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    __syncthreads();
-  }
-};
-struct WaveWait {
-  const int* c;
-  int target;
-  int* err;
-  __device__ void operator()() const {
-    if ((threadIdx.x & 63) == 0) spin_until(c, target, err);
-    wave_sync();
-  }
-};
-__device__ __forceinline__ int add_dev(int* c, int v) {
-  return __hip_atomic_fetch_add(c, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Called by every thread of the block after its last store of a task: once
-// all those stores are complete, thread 0 counts the task and returns the
-// counter's previous value (other threads: -1).
-__device__ __forceinline__ int block_publish(int* c) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  return threadIdx.x == 0 ? add_dev(c, 1) : -1;
-}
-// The same for a task of one wavefront.
-__device__ __forceinline__ void wave_publish(int* c) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_s_waitcnt(0);
-  wave_sync();
-  if ((threadIdx.x & 63) == 0) add_dev(c, 1);
-}
 // The block's next task. The list is dealt round-robin onto kDispensers
 // counters, each on its own 128-byte line (one counter serialises every
 // workgroup of the device on one address); block b draws from counter
-// b % kDispensers, so each counter hands out its share of the list in
-// list order.
+// b % kDispensers, so each counter hands out its share of the list in list
+// order.
 __device__ __forceinline__ int next_task(int* dispensers, int* tsh) {
-  const int d = blockIdx.x % kDispensers;
+  const int dn = blockIdx.x % kDispensers;
   __syncthreads();
-  if (threadIdx.x == 0) *tsh = add_dev(dispensers + kDispenserStride * d, 1) * kDispensers + d;
+  if (threadIdx.x == 0) *tsh = add_dev(dispensers + kDispenserStride * dn, 1) * kDispensers + dn;
   __syncthreads();
   return *tsh;
 }
 
 template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(256) k_fwd_dag(SolveArgs<TF, TD> a, DagArgs d, TX* x, TX* uvec, TX* xfp, int* err) {
+// four waves per SIMD (at most 128 registers): 1024 resident workgroups
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fwd_dag(SolveArgs<TF, TD> a, SolveDev d, TX* x, TX* uvec) {
   __shared__ TX smem[kSmem];
-  __shared__ int tsh;
+  __shared__ int tsh, flag;
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
   const int ns = d.ns;
   for (;;) {
-    const int t = next_task(cnt + dispenser_base(ns), &tsh);
+    const int t = next_task(cnt + d.cnt_disp, &tsh);
     if (t >= d.nfwd) break;
     const Task4 tk = d.fwd[t];
     if (tk.a == kDagSmall) {
       if (w < tk.c) {
         const int s = d.small_list[tk.b + w];
         const int* ms = d.meta + kMetaStride * s;
-        small_fwd<true>(a, s, x, uvec, smem + kRB * w, WaveWait{cnt + kCDone * ns + s, ms[kMetaNChild], err});
+        small_fwd(a, d, s, x, uvec, WaveWait{cnt + kCDone * ns + s, ms[kMetaNChild], d.err});
         if (ms[kMetaParent] >= 0) wave_publish(cnt + kCDone * ns + ms[kMetaParent]);
       }
-    } else if (tk.a == kDagGather) {
-      if (w < tk.c) {
-        const int g = tk.b + w;
-        const int s = d.gather_task[2 * g], i = d.gather_task[2 * g + 1];
-        WaveWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], err}();
-        gather_rows<true>(a, s, i, d.gather_child, d.gather_child_ptr[g], d.gather_child_ptr[g + 1], x, uvec,
-                          smem + kRB * w);
-        wave_publish(cnt + kGDone * ns + s);
-      }
-    } else if (tk.a == kDagXc) {
-      const int* q = d.xc_task + 3 * tk.b;
-      const int s = q[0];
-      BlockWait{cnt + kGDone * ns + s, d.meta[kMetaStride * s + kMetaNGather], err}();
-      fwd_xc<true>(a, s, q[1], q[2], d.xfp_off, x, xfp, smem);
-      block_publish(cnt + kXDone * ns + s);
-    } else {  // kDagU
-      const int* q = d.u_task + 2 * tk.b;
-      const int s = q[0];
-      const int* ms = d.meta + kMetaStride * s;
-      BlockWait{cnt + kXDone * ns + s, ms[kMetaNXc], err}();
-      fwd_u<true>(a, s, q[1], d.xfp_off, xfp, uvec, smem);
-      // the front's last R block completes its update vector
-      const int prev = block_publish(cnt + kUDone * ns + s);
-      if (prev == ms[kMetaNU] - 1 && ms[kMetaParent] >= 0) add_dev(cnt + kCDone * ns + ms[kMetaParent], 1);
+    } else if (tk.a == kDagFx) {
+      const int s = tk.b;
+      fx_task(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), uvec, smem,
+              BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err});
+    } else {  // kDagFu
+      const int s = tk.b;
+      fu_task(a, d, s, tk.c, tk.d, uvec, smem, BlockWait{cnt + kXDone * ns + s, d.meta[kMetaStride * s + kMetaNFx], d.err});
     }
   }
 }
 
 template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(256) k_bwd_dag(SolveArgs<TF, TD> a, DagArgs d, TX* x, TX* part, const TX* xfp,
-                                                 int* err) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_bwd_dag(SolveArgs<TF, TD> a, SolveDev d, TX* x) {
   __shared__ TX smem[kSmem];
-  __shared__ int tsh;
+  __shared__ int tsh, flag;
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
   const int ns = d.ns;
   for (;;) {
-    const int t = next_task(cnt + dispenser_base(ns) + kDispensers * kDispenserStride, &tsh);
+    const int t = next_task(cnt + d.cnt_disp + kDispensers * kDispenserStride, &tsh);
     if (t >= d.nbwd) break;
     const Task4 tk = d.bwd[t];
     if (tk.a == kDagSmall) {
       if (w < tk.c) {
         const int s = d.small_list[tk.b + w];
         const int p = d.meta[kMetaStride * s + kMetaParent];
-        // a root waits for nothing: its counter target is 0
+        // a root waits for nothing: its target is 0
         const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
         const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
-        small_bwd<true>(a, s, x, smem + kRB * w, WaveWait{pc, target, err});
+        small_bwd(a, s, x, smem + kRB * w, WaveWait{pc, target, d.err});
         wave_publish(cnt + kBDone * ns + s);
       }
-    } else if (tk.a == kDagRpart) {
-      const int* q = d.rpart_task + 3 * tk.b;
-      const int s = q[0];
+    } else if (tk.a == kDagBr) {
+      const int s = tk.b;
       const int p = d.meta[kMetaStride * s + kMetaParent];
-      if (p >= 0) BlockWait{cnt + kBDone * ns + p, d.meta[kMetaStride * p + kMetaNBFinal], err}();
-      bwd_rpart<true>(a, s, q[1], q[2], d.part_off, x, part, smem);
-      block_publish(cnt + kRDone * ns + s);
-    } else {  // kDagBxc
-      const int* q = d.bwd_task + 2 * tk.b;
-      const int s = q[0];
-      BlockWait{cnt + kRDone * ns + s, d.meta[kMetaStride * s + kMetaNRpart], err}();
-      bwd_xc<true>(a, s, q[1], d.part_off, part, d.xfp_off, xfp, x, smem);
-      block_publish(cnt + kBDone * ns + s);
+      const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
+      const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
+      br_task(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), smem, BlockWait{pc, target, d.err});
+    } else {  // kDagBx
+      const int s = tk.b;
+      bx_task(a, d, s, tk.c, tk.d, x, smem, &flag,
+              BlockWait{cnt + kRDone * ns + s, d.meta[kMetaStride * s + kMetaNBr], d.err});
     }
   }
 }
@@ -647,8 +733,8 @@ int resident_blocks(K kernel, int threads) {
 
 template <class TF, class TD, class TX>
 void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st) {
-  TX* xfp = reinterpret_cast<TX*>(dp.xfp);
-  TX* part = reinterpret_cast<TX*>(dp.part);
+  const SolveDev& d = dp.d;
+  hipLaunchKernelGGL(k_zero, dim3((d.ncounters + 255) / 256), dim3(256), 0, st, d.ncounters, d.counters);
   if (dp.schedule == kScheduleDag) {
     // Every task waits only on tasks before it in the list. The smallest
     // unfinished task is therefore either running or next on its
@@ -657,49 +743,36 @@ void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX*
     // and serve every dispenser until its share of the list runs out.
     static const int grid_f = resident_blocks(k_fwd_dag<TF, TD, TX>, 256);
     static const int grid_b = resident_blocks(k_bwd_dag<TF, TD, TX>, 256);
-    const DagArgs& d = dp.dag;
-    const int nz = dag_counter_count(d.ns);
-    hipLaunchKernelGGL(k_zero, dim3((nz + 255) / 256), dim3(256), 0, st, nz, d.counters);
     if (d.nfwd > 0)
       hipLaunchKernelGGL((k_fwd_dag<TF, TD, TX>), dim3(std::max(kDispensers, std::min(d.nfwd, grid_f))), dim3(256), 0,
-                         st, a, d, x, uvec, xfp, dp.err);
+                         st, a, d, x, uvec);
     if (d.nbwd > 0)
       hipLaunchKernelGGL((k_bwd_dag<TF, TD, TX>), dim3(std::max(kDispensers, std::min(d.nbwd, grid_b))), dim3(256), 0,
-                         st, a, d, x, part, static_cast<const TX*>(xfp), dp.err);
+                         st, a, d, x);
     MPIR_HIP(hipGetLastError());
     return;
   }
+  auto tiles = [&](int kind, const std::vector<int32_t>& ptr, const int32_t* list, int l) {
+    const int t0 = ptr[l], t1 = ptr[l + 1];
+    if (t1 > t0)
+      hipLaunchKernelGGL((k_level_tiles<TF, TD, TX>), dim3(t1 - t0), dim3(256), 0, st, a, d, kind, list + 3 * t0, x,
+                         uvec);
+  };
   // forward, leaves to root; D^{-1} fused
   for (int l = 0; l < dp.nlevels; ++l) {
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
-      hipLaunchKernelGGL((k_fwd_small<TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x, uvec);
-    const int g0 = dp.h_gather_ptr[l], g1 = dp.h_gather_ptr[l + 1];
-    if (g1 > g0)
-      hipLaunchKernelGGL((k_fwd_gather<TF, TD, TX>), dim3(g1 - g0), dim3(64), 0, st, a, dp.gather_task + 2 * g0,
-                         dp.gather_child_ptr + g0, dp.gather_child, x, uvec);
-    const int c0 = dp.h_xc_ptr[l], c1 = dp.h_xc_ptr[l + 1];
-    if (c1 > c0)
-      hipLaunchKernelGGL((k_fwd_xc<TF, TD, TX>), dim3(c1 - c0), dim3(256), 0, st, a, dp.xc_task + 3 * c0, dp.xfp_off,
-                         x, xfp);
-    const int u0 = dp.h_u_ptr[l], u1 = dp.h_u_ptr[l + 1];
-    if (u1 > u0)
-      hipLaunchKernelGGL((k_fwd_u<TF, TD, TX>), dim3(u1 - u0), dim3(256), 0, st, a, dp.u_task + 2 * u0, dp.xfp_off,
-                         xfp, uvec);
+      hipLaunchKernelGGL((k_fwd_small<TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, d, d.small_list + s0, x, uvec);
+    tiles(kDagFx, dp.h_fx_ptr, d.fx_task, l);
+    tiles(kDagFu, dp.h_fu_ptr, d.fu_task, l);
   }
   // backward, root to leaves
   for (int l = dp.nlevels - 1; l >= 0; --l) {
-    const int p0 = dp.h_rpart_ptr[l], p1 = dp.h_rpart_ptr[l + 1];
-    if (p1 > p0)
-      hipLaunchKernelGGL((k_bwd_rpart<TF, TD, TX>), dim3(p1 - p0), dim3(256), 0, st, a, dp.rpart_task + 3 * p0,
-                         dp.part_off, x, part);
-    const int t0 = dp.h_bwd_ptr[l], t1 = dp.h_bwd_ptr[l + 1];
-    if (t1 > t0)
-      hipLaunchKernelGGL((k_bwd_xc<TF, TD, TX>), dim3(t1 - t0), dim3(256), 0, st, a, dp.bwd_task + 2 * t0,
-                         dp.part_off, part, dp.xfp_off, xfp, x);
+    tiles(kDagBr, dp.h_br_ptr, d.br_task, l);
+    tiles(kDagBx, dp.h_bx_ptr, d.bx_task, l);
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
-      hipLaunchKernelGGL((k_bwd_small<TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0, x);
+      hipLaunchKernelGGL((k_bwd_small<TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, d.small_list + s0, x);
   }
   MPIR_HIP(hipGetLastError());
 }

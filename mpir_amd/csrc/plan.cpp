@@ -1,6 +1,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <map>
 #include <numeric>
 #include <set>
@@ -60,11 +61,10 @@ class PoolAlloc {
 };
 
 // Task lists of the dependency-driven solves from the per-level lists. A
-// forward task of a front waits for its children (small fronts, gathers),
-// for its gathers (column blocks) or for its column blocks (R blocks); a
-// backward task waits for its parent (R partials, small fronts) or for its
-// R partials (column blocks). Children sit on lower levels and a front's
-// own task kinds follow each other within its level, so every wait is on
+// forward task of a front waits for its children (small fronts, fx) or for
+// all its y values (fu); a backward task waits for its parent (small
+// fronts, br) or for its br tasks (bx). Children sit on lower levels and a
+// front's own kinds follow each other within its level, so every wait is on
 // tasks earlier in the list.
 void build_dag_schedule(FactorPlan& P) {
   const Supernodes& S = P.sn;
@@ -72,38 +72,39 @@ void build_dag_schedule(FactorPlan& P) {
   P.dag_meta.assign(static_cast<size_t>(ns) * kMetaStride, 0);
   for (int32_t s = 0; s < ns; ++s) {
     int32_t* m = &P.dag_meta[static_cast<size_t>(s) * kMetaStride];
+    const int32_t nc = S.first[s + 1] - S.first[s], nr = S.nrow[s];
     m[kMetaParent] = S.sparent[s];
     m[kMetaNChild] = P.child_ptr[s + 1] - P.child_ptr[s];
     m[kMetaNBFinal] = 1;  // small fronts: one backward task
+    if (nc + nr > kSmallFront) {
+      m[kMetaNH] = (nc + kGemvCols - 1) / kGemvCols;
+      m[kMetaNHR] = (nr + kXcRows - 1) / kXcRows;
+      m[kMetaNBFinal] = (nc + kSolveRB - 1) / kSolveRB;
+    }
   }
-  auto count = [&](const std::vector<int32_t>& tasks, int stride, int field) {
-    for (size_t q = 0; q < tasks.size(); q += stride) P.dag_meta[static_cast<size_t>(tasks[q]) * kMetaStride + field]++;
+  auto count = [&](const std::vector<int32_t>& tasks, int field) {
+    for (size_t q = 0; q < tasks.size(); q += 3) P.dag_meta[static_cast<size_t>(tasks[q]) * kMetaStride + field]++;
   };
-  for (int32_t q = 0; q < static_cast<int32_t>(P.big_list.size()); ++q)
-    P.dag_meta[static_cast<size_t>(P.big_list[q]) * kMetaStride + kMetaNBFinal] = 0;
-  count(P.sfwd_task, 2, kMetaNGather);
-  count(P.sxc_task, 3, kMetaNXc);
-  count(P.su_task, 2, kMetaNU);
-  count(P.rpart_task, 3, kMetaNRpart);
-  count(P.sbwd_task, 2, kMetaNBFinal);
-  auto grouped = [](std::vector<Task4>& out, int32_t kind, int32_t b, int32_t e) {
-    for (int32_t q = b; q < e; q += kDagGroup) out.push_back({kind, q, std::min(kDagGroup, e - q), 0});
+  count(P.fx_task, kMetaNFx);
+  count(P.fu_task, kMetaNFu);
+  count(P.br_task, kMetaNBr);
+  auto grouped = [](std::vector<Task4>& out, int32_t b, int32_t e) {
+    for (int32_t q = b; q < e; q += kDagGroup) out.push_back({kDagSmall, q, std::min(kDagGroup, e - q), 0});
   };
-  auto single = [](std::vector<Task4>& out, int32_t kind, int32_t b, int32_t e) {
-    for (int32_t q = b; q < e; ++q) out.push_back({kind, q, 1, 0});
+  auto tiles = [](std::vector<Task4>& out, int32_t kind, const std::vector<int32_t>& t, int32_t b, int32_t e) {
+    for (int32_t q = b; q < e; ++q) out.push_back({kind, t[3 * q], t[3 * q + 1], t[3 * q + 2]});
   };
   P.dag_fwd.clear();
   P.dag_bwd.clear();
   for (int32_t l = 0; l < S.nlevels; ++l) {
-    grouped(P.dag_fwd, kDagSmall, P.small_ptr[l], P.small_ptr[l + 1]);
-    grouped(P.dag_fwd, kDagGather, P.sfwd_ptr[l], P.sfwd_ptr[l + 1]);
-    single(P.dag_fwd, kDagXc, P.sxc_ptr[l], P.sxc_ptr[l + 1]);
-    single(P.dag_fwd, kDagU, P.su_ptr[l], P.su_ptr[l + 1]);
+    grouped(P.dag_fwd, P.small_ptr[l], P.small_ptr[l + 1]);
+    tiles(P.dag_fwd, kDagFx, P.fx_task, P.fx_ptr[l], P.fx_ptr[l + 1]);
+    tiles(P.dag_fwd, kDagFu, P.fu_task, P.fu_ptr[l], P.fu_ptr[l + 1]);
   }
   for (int32_t l = S.nlevels - 1; l >= 0; --l) {
-    single(P.dag_bwd, kDagRpart, P.rpart_ptr[l], P.rpart_ptr[l + 1]);
-    single(P.dag_bwd, kDagBxc, P.sbwd_ptr[l], P.sbwd_ptr[l + 1]);
-    grouped(P.dag_bwd, kDagSmall, P.small_ptr[l], P.small_ptr[l + 1]);
+    tiles(P.dag_bwd, kDagBr, P.br_task, P.br_ptr[l], P.br_ptr[l + 1]);
+    tiles(P.dag_bwd, kDagBx, P.bx_task, P.bx_ptr[l], P.bx_ptr[l + 1]);
+    grouped(P.dag_bwd, P.small_ptr[l], P.small_ptr[l + 1]);
   }
 }
 
@@ -255,68 +256,88 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
   }
 
   // solve tasks of the big fronts, per level
-  P.sfwd_child_ptr.assign(1, 0);
-  P.part_off.assign(S.ns, -1);
-  P.xfp_off.assign(S.ns, -1);
-  P.rpart_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
-  P.sfwd_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
-  P.sxc_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
-  P.su_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
-  P.sbwd_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.fx_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.fu_ptr = P.br_ptr = P.bx_ptr = P.fx_ptr;
+  P.cb_base.assign(ns, 0);
+  P.ypart_off.assign(ns, 0);
+  P.upart_off.assign(ns, 0);
+  P.rpart_off.assign(ns, 0);
+  P.xpart_off.assign(ns, 0);
   for (int32_t l = 0; l < S.nlevels; ++l) {
     for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
       const int32_t s = P.big_list[q];
       const int32_t nc = S.first[s + 1] - S.first[s];
       const int32_t nr = S.nrow[s];
-      const int32_t m = nc + nr;
       const int32_t ncb = (nc + kSolveRB - 1) / kSolveRB;
-      const int32_t nrb = (m + kSolveRB - 1) / kSolveRB;
-      for (int32_t i = 0; i < nrb; ++i) {
-        P.sfwd_task.push_back(s);
-        P.sfwd_task.push_back(i);
-        const int32_t r0 = i * kSolveRB, r1 = std::min(m, r0 + kSolveRB);
-        for (int32_t q2 = P.child_ptr[s]; q2 < P.child_ptr[s + 1]; ++q2) {
-          const int32_t c = P.child_list[q2];
-          const int32_t q0 = lower(c, r0), q1 = lower(c, r1);
-          if (q1 > q0) P.sfwd_child.insert(P.sfwd_child.end(), {c, q0, q1});
-        }
-        P.sfwd_child_ptr.push_back(static_cast<int32_t>(P.sfwd_child.size() / 3));
-      }
-      // forward column blocks by row chunk; backward row blocks, longest first
+      const int32_t nrb = (nr + kSolveRB - 1) / kSolveRB;
+      const int32_t nh = (nc + kGemvCols - 1) / kGemvCols;
       const int32_t nhx = (nc + kXcRows - 1) / kXcRows;
-      P.xfp_off[s] = P.nxfp;
-      P.nxfp += static_cast<int64_t>(ncb) * nhx * kSolveRB;
+      const int32_t nhr = (nr + kXcRows - 1) / kXcRows;
+      P.cb_base[s] = P.ncb_total;
+      P.ncb_total += ncb;
+      P.ypart_off[s] = P.nypart;
+      P.nypart += static_cast<int64_t>(ncb) * nhx * kSolveRB;
+      P.upart_off[s] = P.nupart;
+      P.nupart += static_cast<int64_t>(nrb) * nh * kSolveRB;
+      P.rpart_off[s] = P.nrpart;
+      P.nrpart += static_cast<int64_t>(ncb) * nhr * kSolveRB;
+      P.xpart_off[s] = P.nxpart;
+      P.nxpart += static_cast<int64_t>(ncb) * nh * kSolveRB;
+      // fx: rows up to the block's last column (W is upper triangular)
       for (int32_t j = 0; j < ncb; ++j) {
         const int32_t rend = std::min(nc, (j + 1) * kSolveRB);
-        for (int32_t h = 0; h * kXcRows < rend; ++h) {
-          P.sxc_task.push_back(s);
-          P.sxc_task.push_back(j);
-          P.sxc_task.push_back(h);
-        }
+        for (int32_t h = 0; h * kXcRows < rend; ++h) P.fx_task.insert(P.fx_task.end(), {s, j, h});
       }
-      for (int32_t b = 0; b * kSolveRB < nr; ++b) {
-        P.su_task.push_back(s);
-        P.su_task.push_back(b);
-      }
-      for (int32_t i = 0; i < ncb; ++i) {
-        P.sbwd_task.push_back(s);
-        P.sbwd_task.push_back(i);
-      }
-      const int32_t nch = (nr + kRpartRows - 1) / kRpartRows;
-      P.part_off[s] = P.npart;
-      P.npart += static_cast<int64_t>(ncb) * nch * kSolveRB;
+      for (int32_t b = 0; b < nrb; ++b)
+        for (int32_t h = 0; h < nh; ++h) P.fu_task.insert(P.fu_task.end(), {s, b, h});
       for (int32_t j = 0; j < ncb; ++j)
-        for (int32_t c = 0; c < nch; ++c) {
-          P.rpart_task.push_back(s);
-          P.rpart_task.push_back(j);
-          P.rpart_task.push_back(c);
-        }
+        for (int32_t h = 0; h < nhr; ++h) P.br_task.insert(P.br_task.end(), {s, j, h});
+      // bx: columns from the block's first row on
+      for (int32_t i = 0; i < ncb; ++i)
+        for (int32_t h = i * kSolveRB / kGemvCols; h < nh; ++h) P.bx_task.insert(P.bx_task.end(), {s, i, h});
     }
-    P.rpart_ptr[l + 1] = static_cast<int32_t>(P.rpart_task.size() / 3);
-    P.sfwd_ptr[l + 1] = static_cast<int32_t>(P.sfwd_task.size() / 2);
-    P.sxc_ptr[l + 1] = static_cast<int32_t>(P.sxc_task.size() / 3);
-    P.su_ptr[l + 1] = static_cast<int32_t>(P.su_task.size() / 2);
-    P.sbwd_ptr[l + 1] = static_cast<int32_t>(P.sbwd_task.size() / 2);
+    P.fx_ptr[l + 1] = static_cast<int32_t>(P.fx_task.size() / 3);
+    P.fu_ptr[l + 1] = static_cast<int32_t>(P.fu_task.size() / 3);
+    P.br_ptr[l + 1] = static_cast<int32_t>(P.br_task.size() / 3);
+    P.bx_ptr[l + 1] = static_cast<int32_t>(P.bx_task.size() / 3);
+  }
+  // row gathers: children in order, each child's rows ascending
+  {
+    P.frow_off.assign(static_cast<size_t>(ns) + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) {
+      const int64_t m = static_cast<int64_t>(S.first[s + 1] - S.first[s]) + S.nrow[s];
+      const int64_t next = P.frow_off[s] + m;
+      if (next > INT32_MAX) throw std::length_error("front rows exceed the 32-bit gather index");
+      P.frow_off[s + 1] = static_cast<int32_t>(next);
+    }
+    std::vector<int32_t> cnt(static_cast<size_t>(P.frow_off[ns]) + 1, 0);
+    for (int32_t s = 0; s < ns; ++s)
+      for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+        const int32_t c = P.child_list[q];
+        for (int32_t r = 0; r < S.nrow[c]; ++r) cnt[P.frow_off[s] + S.relidx[S.rows_off[c] + r] + 1]++;
+      }
+    for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
+    P.grow_ptr = cnt;
+    P.gent.assign(static_cast<size_t>(cnt.back()), Task4{0, 0, 0, 0});
+    std::vector<int32_t> next(cnt.begin(), cnt.end() - 1);
+    for (int32_t s = 0; s < ns; ++s)
+      for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+        const int32_t c = P.child_list[q];
+        const int32_t ncc = S.first[c + 1] - S.first[c];
+        const bool big = ncc + S.nrow[c] > kSmallFront;
+        const int32_t nhc = big ? (ncc + kGemvCols - 1) / kGemvCols : 0;
+        for (int32_t r = 0; r < S.nrow[c]; ++r) {
+          const int64_t src = S.rows_off[c] + r;
+          const int64_t pb = big ? P.upart_off[c] + static_cast<int64_t>(r / kSolveRB) * nhc * kSolveRB + r % kSolveRB : 0;
+          if (src > INT32_MAX || pb > INT32_MAX) throw std::length_error("update vectors exceed the 32-bit gather index");
+          P.gent[next[P.frow_off[s] + S.relidx[S.rows_off[c] + r]]++] =
+              Task4{static_cast<int32_t>(src), static_cast<int32_t>(pb), nhc, 0};
+        }
+      }
+    // two unused entries past the end: a row's first two entries are read
+    // before its count is known
+    P.gent.push_back(Task4{0, 0, 0, 0});
+    P.gent.push_back(Task4{0, 0, 0, 0});
   }
   build_dag_schedule(P);
   return P;

@@ -14,10 +14,11 @@ namespace mpir {
 constexpr int kPanelRows = 64;
 // Row / column block of the big-front triangular solves.
 constexpr int kSolveRB = 64;
-// R rows per backward partial-sum task.
-constexpr int kRpartRows = 512;
-// Rows per forward task of a big front's inverse block (partial sums).
-constexpr int kXcRows = 512;
+// Rows per task of the column-block products of the solves (W^T v, L_21^T x)
+// and of the factor's inverse post-pass.
+constexpr int kXcRows = 256;
+// Columns per task of the row-block products of the solves (L_21 y, W z).
+constexpr int kGemvCols = 256;
 // Tile edge of the trailing-update tasks.
 constexpr int kTile = 64;
 // Edge of the extend-add assembly tiles.
@@ -27,13 +28,15 @@ struct Task4 {
   int32_t a, b, c, d;
 };
 
-// Kinds of the dependency-driven solve tasks (Task4.a); b = first index into
-// the kind's own task list, c = number of entries (up to kDagGroup for the
-// one-wavefront kinds, else 1).
-enum : int32_t { kDagSmall = 0, kDagGather, kDagXc, kDagU, kDagRpart, kDagBxc };
+// Kinds of the dependency-driven solve tasks (Task4.a). Small fronts:
+// (kDagSmall, first index into small_list, count <= kDagGroup, 0); tiles of
+// a big front: (kind, s, block, chunk).
+enum : int32_t { kDagSmall = 0, kDagFx, kDagFu, kDagBr, kDagBx };
 constexpr int kDagGroup = 4;
-// Per-supernode entries of the solve dependency counts.
-enum : int32_t { kMetaParent = 0, kMetaNChild, kMetaNGather, kMetaNXc, kMetaNU, kMetaNRpart, kMetaNBFinal,
+// Per-supernode entries of the solve schedule: parent, children, column
+// blocks, R blocks, backward R-chunk tasks, backward completions (column
+// blocks, or 1 for a small front), column chunks, R chunks.
+enum : int32_t { kMetaParent = 0, kMetaNChild, kMetaNFx, kMetaNFu, kMetaNBr, kMetaNBFinal, kMetaNH, kMetaNHR,
                  kMetaStride = 8 };
 
 struct FactorPlan {
@@ -66,34 +69,39 @@ struct FactorPlan {
   // per step, the first panel / update task on the identity rows below R_s
   // (the inverse of the diagonal block); those tasks end the step's range
   std::vector<int32_t> pan_aug_ptr, upd_aug_ptr;  // nsteps_total
-  // triangular solves of big fronts, per level (pairs (s, block)):
-  //   sfwd: 64-row blocks of the front, gathering b and the children's
-  //         update vectors; sxc: triples (s, column block, row chunk) of
-  //         x_c = (L_11^{-T} D^{-1})^T v_c, partial sums over kXcRows rows at
-  //         xfp_off[s] + (block * nchunk + chunk) * kSolveRB,
-  //         nchunk = ceil(nc / kXcRows);
-  //   su:   64-row blocks of R_s, u = v_R - L_21 D x_c;
-  //   sbwd: row blocks, x_c = L_11^{-T} (x_c - L_21^T x_R)
-  std::vector<int32_t> sfwd_ptr, sxc_ptr, su_ptr, sbwd_ptr;  // nlevels+1
-  std::vector<int32_t> sfwd_task, sxc_task, su_task, sbwd_task;
-  // per forward gather task, the children whose update vectors land in its
-  // rows: (child, q0, q1) = child rows [q0, q1)
-  std::vector<int32_t> sfwd_child_ptr;        // per forward task + 1
-  std::vector<int32_t> sfwd_child;            // triples
-  // backward contribution of the rows below the columns (R rows): tasks
-  // (s, column block, row chunk) per level, partial sums at
-  // part_off[s] + (block * nchunk + chunk) * kSolveRB
-  std::vector<int32_t> rpart_ptr;             // nlevels+1
-  std::vector<int32_t> rpart_task;            // triples
-  std::vector<int64_t> part_off;              // per supernode
-  int64_t npart = 0;
-  std::vector<int64_t> xfp_off;               // per supernode
-  int64_t nxfp = 0;
+  // Triangular solves of big fronts (W = L_11^{-T} below the panel), task
+  // triples per level:
+  //   fx (s, column block j, row chunk h): partial sums of y = W^T v over
+  //      kXcRows rows of 64 columns, v gathered on those rows
+  //   fu (s, R block b, column chunk h): partial sums of L_21 y over
+  //      kGemvCols columns for 64 R rows; chunk 0 also stores v_R, so that
+  //      u = v_R - (partial sums) is formed by whoever gathers it
+  //   br (s, column block j, R chunk h): partial sums of L_21^T x_R over
+  //      kXcRows R rows
+  //   bx (s, row block i, column chunk h): partial sums of W z,
+  //      z = D^{-1} y - L_21^T x_R, over kGemvCols columns (from column
+  //      64 i on) for 64 rows; the last task of a block stores x
+  // y, L_21 y and L_21^T x_R stay as partial sums (per 64-entry block and
+  // chunk, at ypart_off / upart_off / rpart_off of the front) that their
+  // readers add in a fixed order.
+  std::vector<int32_t> fx_ptr, fu_ptr, br_ptr, bx_ptr;      // nlevels+1, in tasks
+  std::vector<int32_t> fx_task, fu_task, br_task, bx_task;  // triples
+  // row gathers of every front: the contributions to row r of front s are
+  // entries [grow_ptr[frow_off[s] + r], grow_ptr[frow_off[s] + r + 1]) of
+  // gent, in child order: (update-vector entry, first partial sum of L_21 y
+  // for that child row or 0, number of those partial sums, 0)
+  std::vector<int32_t> frow_off, grow_ptr;
+  std::vector<Task4> gent;
+  // per supernode: first column block in the bx block counters; offsets of
+  // the partial-sum areas (entries)
+  std::vector<int32_t> cb_base;
+  std::vector<int64_t> ypart_off, upart_off, rpart_off, xpart_off;
+  int64_t nypart = 0, nupart = 0, nrpart = 0, nxpart = 0;
+  int32_t ncb_total = 0;
   // dependency-driven solves: every task above, forward in level order
-  // (small groups, gathers, column blocks, R blocks per level) and backward
-  // in reverse level order (R partials, column blocks, small groups); each
-  // task depends only on tasks before it. dag_meta: kMetaStride entries per
-  // supernode (parent, children, and the task counts the waits compare with)
+  // (small groups, fx, fu per level) and backward in reverse level order
+  // (br, bx, small groups); each task depends only on tasks before it.
+  // dag_meta: kMetaStride entries per supernode
   std::vector<Task4> dag_fwd, dag_bwd;
   std::vector<int32_t> dag_meta;
   // for each entry of the ORIGINAL upper CSC: destination in the panel pool

@@ -182,7 +182,7 @@ class GmresLdl {
   DevBuf d_entry_col_, d_entry_row_, d_diag_pos_, d_perm_;
   // numeric
   DevBuf d_ax_, d_axs_, d_ad_, d_adp_, d_panel_, d_panel16_, d_upool_, d_D_, d_Dinv_, d_err_, d_cnt_, d_nonfinite_;
-  DevBuf d_uvec_, d_scratch_, d_serr_;
+  DevBuf d_uvec_, d_scratch_, d_serr_, d_counters_;
   // GMRES workspace
   DevBuf d_V_, d_w_, d_b_, d_x_, d_r_, d_d_, d_red_, d_small_;
   DevBuf d_hcol_, d_givens_;
