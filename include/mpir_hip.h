@@ -205,6 +205,41 @@ int64_t mpir_gmres_ldl_log(mpir_gmres_ldl_t h, char* buf, int64_t cap);
 /* HIP stream of the handle (hipStream_t as void*) */
 void* mpir_gmres_ldl_stream(mpir_gmres_ldl_t h);
 
+/* ------------------------------------------------------------------ batches
+ * Many independent systems on one handle (no counterpart in the reference,
+ * whose GmresLDLIR holds one system; each system here is solved exactly as
+ * GmresLDLIR::Solve, gmres_ir.hpp:534-741, would solve it). The systems are
+ * joined into one block-diagonal system: one factorization plan over the
+ * forest of their elimination trees and lock-step GMRES-IR with per-system,
+ * per-right-hand-side scalars and stopping rules. A system's result is
+ * bit-identical alone and in any batch.
+ * The symbolic phase of each pattern is kept in a process-wide cache keyed by
+ * (n, Ap, Ai, ordering), so repeated patterns are analysed once; missing
+ * patterns are analysed on `threads` host threads. The handle's setters
+ * (tolerance, iteration limits, ordering, scaling) apply to every system. */
+int mpir_batch_analyze(mpir_gmres_ldl_t h, int32_t nsys, const int32_t* n, const int32_t* const* Ap,
+                       const int32_t* const* Ai, int32_t threads);
+/* values of every system, in the entry order of its pattern */
+int mpir_batch_factor(mpir_gmres_ldl_t h, const double* const* Ax);
+int mpir_batch_factor_dev(mpir_gmres_ldl_t h, const double* const* Ax);
+/* nrhs[s] right-hand sides of system s, column-major with ld = n_s */
+int mpir_batch_solve(mpir_gmres_ldl_t h, const int32_t* nrhs, const double* const* b, double* const* x);
+int mpir_batch_solve_dev(mpir_gmres_ldl_t h, const int32_t* nrhs, const double* const* b, double* const* x);
+/* results of the last batch: positive pivots per system (nsys); per system s
+ * and right-hand side r at s * max_rhs + r: IR iterations, GMRES iterations,
+ * stop reason, final relative residual (scaled system), ms from the start of
+ * the solve until the instance stopped. Any output pointer may be NULL. */
+int mpir_batch_stats(mpir_gmres_ldl_t h, int32_t* max_rhs, int32_t* positive_pivots, int32_t* ir_iterations,
+                     int32_t* gmres_iterations, int32_t* stop_reason, double* final_rel_residual, double* done_ms);
+/* the process-wide pattern cache: entries, hits, misses */
+int mpir_pattern_cache_info(int64_t* entries, int64_t* hits, int64_t* misses);
+/* Host only: symbolic phase of a batch without a device. info[0..7] = rows,
+ * nnz(L), factor flops, supernodes, levels, panel entries, panel tasks,
+ * forward solve tasks of the joined plan; info[8 + 2 s], info[9 + 2 s] =
+ * nnz(L) and device-byte estimate of system s alone. */
+int mpir_batch_plan_info(int32_t nsys, const int32_t* n, const int32_t* const* Ap, const int32_t* const* Ai,
+                         int ordering, int64_t* info);
+
 /* ------------------------------------------------------------------ utilities
  * FP64 symmetric SpMV from the upper CSC with the reference's rounding
  * (MatrixMultiply, include/MPIR/ops.hpp:25-45), on the GPU. */
@@ -216,7 +251,9 @@ int mpir_residual_upper_f64(int32_t n, const int32_t* Ap, const int32_t* Ai, con
 /* Synthetic quasi-definite KKT systems (deterministic, SplitMix64):
  * dim 2/3 grid Laplacian of side k plus h_shift*I, m constraint rows
  * (a_mode 0: cnt_lo..cnt_hi distinct uniform columns, a_mode 1: a uniform
- * column and its +x,+y,+z grid neighbours), D = d_value*I, x_true ~ U(-1,1),
+ * column and its +x,+y,+z grid neighbours, a_mode 2: cnt_lo..cnt_hi entries,
+ * a uniform column and its nearest in-grid neighbours in a fixed offset
+ * order), D = d_value*I, x_true ~ U(-1,1),
  * b = K x_true. Query sizes with NULL arrays, then fill. */
 int mpir_gen_kkt(int dim, int k, double h_shift, int64_t m, int a_mode, int cnt_lo, int cnt_hi, double d_value,
                  uint64_t seed, int32_t* n_out, int64_t* nnz_out, int32_t* n1_out, int32_t* Ap, int32_t* Ai,

@@ -277,6 +277,137 @@ class GmresLDLIR:
         return self._h
 
 
+# ---------------------------------------------------------------------- batches
+def _ptr_array(arrs) -> C.Array:
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data if a is not None else None for a in arrs])
+
+
+class BatchLDLIR:
+    """Many independent systems, each solved as GmresLDLIR<UF,double,double>
+    (gmres_ir.hpp:534-741) solves one, on one GPU: the systems are joined
+    into one block-diagonal system (one factorization plan over the forest
+    of their elimination trees; lock-step GMRES-IR with per-system scalars).
+    A system's result is bit-identical alone and in any batch."""
+
+    def __init__(self, UF=np.float32, ordering: int = ORDER_ND, scaling: int = SCALE_DIAG, threads: int = 8):
+        self._s = GmresLDLIR(UF, ordering=ordering, scaling=scaling)
+        self.threads = int(threads)
+        self._n: list[int] = []
+        self._keep = []
+
+    # the reference's setters apply to every system
+    def SetMaxIRIterations(self, n: int) -> None:
+        self._s.SetMaxIRIterations(n)
+
+    def SetMaxGmresIterations(self, n: int) -> None:
+        self._s.SetMaxGmresIterations(n)
+
+    def SetTolerance(self, tol: float) -> None:
+        self._s.SetTolerance(tol)
+
+    def Analyze(self, patterns) -> None:
+        """patterns: list of (Ap, Ai), upper CSC of each system."""
+        pats = [(np.ascontiguousarray(a, dtype=np.int32), np.ascontiguousarray(i, dtype=np.int32)) for a, i in patterns]
+        n = np.array([len(a) - 1 for a, _ in pats], dtype=np.int32)
+        _check(load().mpir_batch_analyze(self._s.handle, len(pats), _ptr(n), _ptr_array([a for a, _ in pats]),
+                                         _ptr_array([i for _, i in pats]), self.threads), "batch Analyze")
+        self._n = [int(v) for v in n]
+
+    def Factor(self, values) -> None:
+        vals = [np.ascontiguousarray(v, dtype=np.float64) for v in values]
+        if len(vals) != len(self._n):
+            raise ValueError("one value array per system")
+        _check(load().mpir_batch_factor(self._s.handle, _ptr_array(vals)), "batch Factor")
+
+    def Compute(self, systems) -> None:
+        """systems: list of (Ap, Ai, Ax)."""
+        self.Analyze([(a, i) for a, i, _ in systems])
+        self.Factor([x for _, _, x in systems])
+
+    def Solve(self, rhs) -> list:
+        """rhs: per system an array (n,) or (n, nrhs) (or None for none);
+        returns the solutions in the same shapes."""
+        if len(rhs) != len(self._n):
+            raise ValueError("one right-hand-side array per system")
+        bs, xs, nr, shapes = [], [], [], []
+        for b, n in zip(rhs, self._n):
+            if b is None:
+                bs.append(None)
+                xs.append(None)
+                nr.append(0)
+                shapes.append(None)
+                continue
+            b = np.asarray(b, dtype=np.float64)
+            if b.shape[0] != n:
+                raise ValueError("right-hand side of the wrong length")
+            shapes.append(b.shape)
+            bb = np.ascontiguousarray(b.reshape(n, -1).T)
+            bs.append(bb)
+            xs.append(np.empty_like(bb))
+            nr.append(bb.shape[0])
+        nrhs = np.array(nr, dtype=np.int32)
+        _check(load().mpir_batch_solve(self._s.handle, _ptr(nrhs), _ptr_array(bs), _ptr_array(xs)), "batch Solve")
+        out = []
+        for x, shp in zip(xs, shapes):
+            out.append(None if x is None else x.T.reshape(shp).copy())
+        return out
+
+    def factor_dev(self, ptrs) -> None:
+        arr = (C.c_void_p * len(ptrs))(*ptrs)
+        _check(load().mpir_batch_factor_dev(self._s.handle, arr), "batch factor_dev")
+
+    def solve_dev(self, nrhs, b_ptrs, x_ptrs) -> None:
+        nr = np.ascontiguousarray(nrhs, dtype=np.int32)
+        _check(load().mpir_batch_solve_dev(self._s.handle, _ptr(nr), (C.c_void_p * len(b_ptrs))(*b_ptrs),
+                                           (C.c_void_p * len(x_ptrs))(*x_ptrs)), "batch solve_dev")
+
+    def stats(self) -> dict:
+        """positive_pivots (per system); ir_iterations, gmres_iterations,
+        stop_reason, final_rel_residual, done_ms of shape (systems, max_rhs)."""
+        S = len(self._n)
+        lib = load()
+        mr = C.c_int32()
+        _check(lib.mpir_batch_stats(self._s.handle, C.byref(mr), None, None, None, None, None, None), "batch stats")
+        R = max(mr.value, 1)
+        pos = np.zeros(S, np.int32)
+        ir, gm, stop = (np.zeros(S * R, np.int32) for _ in range(3))
+        rel, done = np.zeros(S * R), np.zeros(S * R)
+        _check(lib.mpir_batch_stats(self._s.handle, C.byref(mr), _ptr(pos), _ptr(ir), _ptr(gm), _ptr(stop), _ptr(rel),
+                                    _ptr(done)), "batch stats")
+        sh = (S, R)
+        return {"max_rhs": mr.value, "positive_pivots": pos, "ir_iterations": ir.reshape(sh),
+                "gmres_iterations": gm.reshape(sh), "stop_reason": stop.reshape(sh),
+                "final_rel_residual": rel.reshape(sh), "done_ms": done.reshape(sh), "totals": self._s.stats()}
+
+    def time_kernels(self, what: str, reps: int = 5) -> float:
+        return self._s.time_kernels(what, reps)
+
+    @property
+    def sizes(self) -> list:
+        return list(self._n)
+
+
+def pattern_cache_info() -> dict:
+    e, h, m = C.c_int64(), C.c_int64(), C.c_int64()
+    _check(load().mpir_pattern_cache_info(C.byref(e), C.byref(h), C.byref(m)), "pattern_cache_info")
+    return {"entries": e.value, "hits": h.value, "misses": m.value}
+
+
+def batch_plan_info(patterns, ordering: int = ORDER_ND) -> dict:
+    """Host-only symbolic phase of a batch (no device needed)."""
+    pats = [(np.ascontiguousarray(a, dtype=np.int32), np.ascontiguousarray(i, dtype=np.int32)) for a, i in patterns]
+    S = len(pats)
+    n = np.array([len(a) - 1 for a, _ in pats], dtype=np.int32)
+    info = np.zeros(8 + 2 * S, dtype=np.int64)
+    _check(load().mpir_batch_plan_info(S, _ptr(n), _ptr_array([a for a, _ in pats]), _ptr_array([i for _, i in pats]),
+                                       ordering, _ptr(info)), "batch_plan_info")
+    keys = ["n", "nnz_l", "factor_flops", "nsupernodes", "nlevels", "panel_entries", "panel_tasks", "fwd_tasks"]
+    out = {k: int(v) for k, v in zip(keys, info[:8])}
+    out["nnz_l_each"] = info[8::2].copy()
+    out["bytes_each"] = info[9::2].copy()
+    return out
+
+
 # ---------------------------------------------------------------------- utilities
 @dataclass
 class KKT:
@@ -317,6 +448,40 @@ def kkt_config(cfg: int, seed: int = 0, d_value: float | None = None, k: int | N
         kk = 64 if k is None else k
         return gen_kkt(3, kk, 1e-3, kk ** 3 // 4, 1, 4, 4, 1e-4 if d_value is None else d_value, seed)
     raise ValueError("unknown configuration")
+
+
+@dataclass
+class BatchSystem:
+    kkt: KKT
+    dim: int
+    k: int
+    B: np.ndarray       # right-hand sides, (n, nrhs)
+    X_true: np.ndarray  # (n, nrhs)
+
+
+def batch_config(nsys: int = 512, nrhs: int = 8, seed: int = 0, n1_lo: float = 2e3, n1_hi: float = 6e4,
+                 d_value: float = 1e-3, h_shift: float = 1e-3) -> list:
+    """The batched workload: `nsys` independent quasi-definite systems.
+    System i draws (numpy PCG64, seed) n1 log-uniform in [n1_lo, n1_hi], a 2D
+    or 3D grid by a fair coin with side round(sqrt(n1)) / round(cbrt(n1)),
+    m = floor(k^dim / 4) constraint rows of 3-5 entries (a uniform cell and
+    its nearest in-grid neighbours, values N(0,1)), D = d_value I, H shift
+    h_shift, and `nrhs` right-hand sides b = K x_true with x_true ~ U(-1,1)."""
+    import scipy.sparse as sp
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(nsys):
+        n1 = float(np.exp(rng.uniform(np.log(n1_lo), np.log(n1_hi))))
+        dim = 2 if rng.uniform() < 0.5 else 3
+        k = max(3, int(round(n1 ** (1.0 / dim))))
+        n1r = k ** dim
+        P = gen_kkt(dim, k, h_shift, n1r // 4, 2, 3, 5, d_value, seed * 1000003 + i)
+        X = rng.uniform(-1.0, 1.0, size=(P.n, nrhs))
+        U = sp.csc_matrix((P.Ax, P.Ai, P.Ap), shape=(P.n, P.n))
+        K = (U + sp.triu(U, 1).T).tocsr()
+        B = np.asarray(K @ X)
+        out.append(BatchSystem(P, dim, k, B, X))
+    return out
 
 
 def read_mtx(path):
@@ -371,4 +536,5 @@ __all__ = [
     "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "LUIR_QDLDL", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
     "amd_order", "read_mtx", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
     "ORDER_AMD", "ORDER_ND", "SCALE_NONE", "SCALE_DIAG", "SCALE_RUIZ", "SCHED_LEVELS", "SCHED_DAG",
+    "BatchLDLIR", "BatchSystem", "batch_config", "batch_plan_info", "pattern_cache_info",
 ]

@@ -84,6 +84,14 @@ SIGNATURES = {
     "mpir_gen_kkt": (C.c_int, [C.c_int, C.c_int, f64, i64, C.c_int, C.c_int, C.c_int, f64, u64,
                                P, P, P, P, P, P, P, P]),
     "mpir_amd_order": (C.c_int, [i32, P, P, P]),
+    "mpir_batch_analyze": (C.c_int, [P, i32, P, P, P, i32]),
+    "mpir_batch_factor": (C.c_int, [P, P]),
+    "mpir_batch_factor_dev": (C.c_int, [P, P]),
+    "mpir_batch_solve": (C.c_int, [P, P, P, P]),
+    "mpir_batch_solve_dev": (C.c_int, [P, P, P, P]),
+    "mpir_batch_stats": (C.c_int, [P, P, P, P, P, P, P, P]),
+    "mpir_pattern_cache_info": (C.c_int, [P, P, P]),
+    "mpir_batch_plan_info": (C.c_int, [i32, P, P, P, C.c_int, P]),
 }
 
 _lib = None

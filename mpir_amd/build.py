@@ -46,12 +46,14 @@ HOST_SOURCES = [
     CSRC / "host" / "symbolic.cpp",
     CSRC / "host" / "mtx.cpp",
     CSRC / "plan.cpp",
+    CSRC / "batch_plan.cpp",
 ]
 HIP_SOURCES = [
     CSRC / "kernels" / "factor.hip",
     CSRC / "kernels" / "solve.hip",
     CSRC / "kernels" / "spmv.hip",
     CSRC / "gmres.hip",
+    CSRC / "batch.hip",
     CSRC / "capi.cpp",
 ]
 

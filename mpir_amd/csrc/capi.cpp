@@ -259,6 +259,14 @@ struct mpir_gmres_ldl {
   std::vector<int32_t> Ap, Ai;
 };
 
+namespace {
+// symbolic phases of every pattern seen by the batch entry points
+mpir::PatternCache& pattern_cache() {
+  static mpir::PatternCache cache;
+  return cache;
+}
+}  // namespace
+
 extern "C" {
 
 const char* mpir_last_error(void) { return g_last_error.c_str(); }
@@ -504,6 +512,96 @@ int mpir_gmres_ldl_solve(mpir_gmres_ldl_t h, const double* b, double* x, int32_t
 int mpir_gmres_ldl_solve_dev(mpir_gmres_ldl_t h, const double* b, double* x, int32_t nrhs, int64_t ld) {
   return guarded([&] { MPIR_HANDLE_CHECK; MPIR_CHECK(b && x, "null vectors"); h->s->solve(b, x, nrhs, ld, true); return 0; });
 }
+// ---------------------------------------------------------------- batches
+int mpir_batch_analyze(mpir_gmres_ldl_t h, int32_t nsys, const int32_t* n, const int32_t* const* Ap,
+                       const int32_t* const* Ai, int32_t threads) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(nsys >= 1 && n && Ap && Ai, "bad batch description");
+    std::vector<int32_t> nv(n, n + nsys);
+    std::vector<const int32_t*> ap(Ap, Ap + nsys), ai(Ai, Ai + nsys);
+    std::vector<std::shared_ptr<const PatternEntry>> parts;
+    try {
+      parts = pattern_cache().get_many(nv, ap, ai, h->s->ordering(), h->s->relax(), std::max(1, threads));
+    } catch (const std::invalid_argument& e) {
+      throw Error(kErrInvalid, e.what());
+    }
+    h->s->analyze_batch(parts);
+    return 0;
+  });
+}
+int mpir_batch_factor(mpir_gmres_ldl_t h, const double* const* Ax) {
+  return guarded([&] { MPIR_HANDLE_CHECK; h->s->factor_batch(Ax, false); return 0; });
+}
+int mpir_batch_factor_dev(mpir_gmres_ldl_t h, const double* const* Ax) {
+  return guarded([&] { MPIR_HANDLE_CHECK; h->s->factor_batch(Ax, true); return 0; });
+}
+int mpir_batch_solve(mpir_gmres_ldl_t h, const int32_t* nrhs, const double* const* b, double* const* x) {
+  return guarded([&] { MPIR_HANDLE_CHECK; h->s->solve_batch(nrhs, b, x, false); return 0; });
+}
+int mpir_batch_solve_dev(mpir_gmres_ldl_t h, const int32_t* nrhs, const double* const* b, double* const* x) {
+  return guarded([&] { MPIR_HANDLE_CHECK; h->s->solve_batch(nrhs, b, x, true); return 0; });
+}
+int mpir_batch_stats(mpir_gmres_ldl_t h, int32_t* max_rhs, int32_t* positive_pivots, int32_t* ir_iterations,
+                     int32_t* gmres_iterations, int32_t* stop_reason, double* final_rel_residual, double* done_ms) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(h->s->is_batch(), "handle holds no batch");
+    const BatchStats& b = h->s->batch_stats();
+    if (max_rhs) *max_rhs = b.max_rhs;
+    auto put = [](auto* dst, const auto& v) {
+      if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    put(positive_pivots, b.positive_pivots);
+    put(ir_iterations, b.ir_iterations);
+    put(gmres_iterations, b.gmres_iterations);
+    put(stop_reason, b.stop_reason);
+    put(final_rel_residual, b.final_rel_residual);
+    put(done_ms, b.done_ms);
+    return 0;
+  });
+}
+int mpir_pattern_cache_info(int64_t* entries, int64_t* hits, int64_t* misses) {
+  return guarded([&] {
+    if (entries) *entries = static_cast<int64_t>(pattern_cache().size());
+    if (hits) *hits = pattern_cache().hits();
+    if (misses) *misses = pattern_cache().misses();
+    return 0;
+  });
+}
+int mpir_batch_plan_info(int32_t nsys, const int32_t* n, const int32_t* const* Ap, const int32_t* const* Ai,
+                         int ordering, int64_t* info) {
+  return guarded([&] {
+    MPIR_CHECK(nsys >= 1 && n && Ap && Ai && info, "bad batch description");
+    MPIR_CHECK(ordering >= 0 && ordering <= 2, "ordering must be 0, 1 or 2");
+    std::vector<PatternEntry> own;
+    own.reserve(nsys);
+    std::vector<const PatternEntry*> raw;
+    try {
+      for (int32_t s = 0; s < nsys; ++s) own.push_back(analyze_entry(n[s], Ap[s], Ai[s], ordering, true));
+    } catch (const std::invalid_argument& e) {
+      throw Error(kErrInvalid, e.what());
+    }
+    for (const auto& e : own) raw.push_back(&e);
+    const BatchSystem B = combine_patterns(raw);
+    const FactorPlan& P = B.an.plan;
+    info[0] = B.col_base.back();
+    info[1] = B.an.nnz_l;
+    info[2] = static_cast<int64_t>(B.an.flops);
+    info[3] = P.sn.ns;
+    info[4] = P.sn.nlevels;
+    info[5] = P.panel_entries;
+    info[6] = static_cast<int64_t>(P.pan_task.size());
+    info[7] = static_cast<int64_t>(P.dag_fwd.size());
+    // per system: nnz(L) and the footprint estimate
+    for (int32_t s = 0; s < nsys; ++s) {
+      info[8 + 2 * s] = own[s].an.nnz_l;
+      info[9 + 2 * s] = own[s].est_bytes;
+    }
+    return 0;
+  });
+}
+
 int mpir_gmres_ldl_stats(mpir_gmres_ldl_t h, mpir_gmres_stats* out) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;

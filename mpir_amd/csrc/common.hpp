@@ -60,32 +60,44 @@ class DevBuf {
   explicit DevBuf(size_t bytes) { alloc(bytes); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), cap_(o.cap_) { o.p_ = nullptr; o.n_ = o.cap_ = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+    if (this != &o) { release(); p_ = o.p_; n_ = o.n_; cap_ = o.cap_; o.p_ = nullptr; o.n_ = o.cap_ = 0; }
     return *this;
   }
   ~DevBuf() { release(); }
+  // Contents are not preserved. An allocation that already holds `bytes`
+  // and is at most four times larger is kept: hipFree waits for the whole
+  // device, so handles that are re-analyzed for every batch of systems
+  // reuse their memory instead.
   void alloc(size_t bytes) {
+    if (p_ && bytes <= cap_ && bytes >= cap_ / 4) {
+      n_ = bytes;
+      return;
+    }
     release();
     if (bytes == 0) return;
     MPIR_HIP(hipMalloc(&p_, bytes));
-    n_ = bytes;
+    n_ = cap_ = bytes;
   }
   // grow-only reallocation (contents not preserved)
-  void ensure(size_t bytes) { if (bytes > n_) alloc(bytes); }
+  void ensure(size_t bytes) {
+    if (bytes > cap_) alloc(bytes);
+    else if (bytes > n_) n_ = bytes;
+  }
   void release() {
     if (p_) (void)hipFree(p_);
     p_ = nullptr;
-    n_ = 0;
+    n_ = cap_ = 0;
   }
   template <class T> T* as() const { return static_cast<T*>(p_); }
   void* get() const { return p_; }
   size_t bytes() const { return n_; }
+  size_t capacity() const { return cap_; }
 
  private:
   void* p_ = nullptr;
-  size_t n_ = 0;
+  size_t n_ = 0, cap_ = 0;
 };
 
 template <class T>

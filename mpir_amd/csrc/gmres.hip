@@ -265,6 +265,7 @@ GmresLdl::GmresLdl(int uf, int uw, int ur) : uf_(uf), uw_(uw), ur_(ur) {
 GmresLdl::~GmresLdl() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   if (h_pinned_) (void)hipHostFree(h_pinned_);
+  if (b_pinned_) (void)hipHostFree(b_pinned_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -283,6 +284,7 @@ void GmresLdl::analyze(int32_t n, const int32_t* Ap, const int32_t* Ai) {
     }
   }
   analyzed_ = factored_ = false;
+  batch_.reset();
   n_ = n;
   nnz_ = Ap[n];
   Ap_.assign(Ap, Ap + n + 1);
@@ -474,6 +476,7 @@ void GmresLdl::alloc_numeric() {
 void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, const void* Lx, const void* Dinv) {
   MPIR_CHECK(n >= 1, "matrix dimension must be positive");
   analyzed_ = factored_ = false;
+  batch_.reset();
   Supernodes S;
   try {
     S = supernodes_from_l(n, Lp, Li);
@@ -530,12 +533,17 @@ void GmresLdl::round_factor16() {
 
 void GmresLdl::factor(const double* Ax, bool device_ptr) {
   if (!analyzed_) throw Error(kErrState, "factor called before analyze");
+  if (batch_) throw Error(kErrState, "handle holds a batch of systems: use the batch entry points");
   const double t0 = now_ms();
-  hipStream_t st = stream_;
   factored_ = false;
   if (nnz_ > 0)
     MPIR_HIP(hipMemcpyAsync(d_ax_.get(), Ax, nnz_ * sizeof(double),
-                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+                            device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream_));
+  factor_loaded(t0);
+}
+
+void GmresLdl::factor_loaded(double t0) {
+  hipStream_t st = stream_;
   const int64_t ncsr = 2 * nnz_ - n_;
   if (scaling_ == kScaleRuiz) {
     // Ruiz: at most kRuizSweeps sweeps, stopping once every row's max-norm is
@@ -585,12 +593,14 @@ void GmresLdl::factor(const double* Ax, bool device_ptr) {
   int nonfinite = 0;
   if (uf_ == kF16 || uf_ == kBF16)
     MPIR_HIP(hipMemcpyAsync(&h_pinned_[2], d_nonfinite_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+  if (batch_) batch_count_pivots();
   MPIR_HIP(hipStreamSynchronize(st));
   std::memcpy(&err, &h_pinned_[0], sizeof(int));
   std::memcpy(&pos, &h_pinned_[1], sizeof(int));
   if (uf_ == kF16 || uf_ == kBF16) std::memcpy(&nonfinite, &h_pinned_[2], sizeof(int));
   stats_.positive_pivots = pos;
   stats_.factor_ms = now_ms() - t0;
+  if (batch_) batch_collect_pivots(err != 0);
   if (err) throw Error(kErrNumeric, "zero pivot in the LDL^T factorization");
   if (nonfinite)
     throw Error(kErrNumeric, std::to_string(nonfinite) + " factor entries overflow the " + prec_name(uf_) + " range");
@@ -811,6 +821,7 @@ std::string GmresLdl::log_json() const {
 }
 
 void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool device_ptr) {
+  if (batch_) throw Error(kErrState, "handle holds a batch of systems: use the batch entry points");
   if (!factored_) throw Error(kErrState, "solve called before a successful factorization");
   if (precond_only_) throw Error(kErrState, "handle holds a loaded factor only; call compute first");
   MPIR_CHECK(nrhs >= 1 && ld >= n_, "bad right-hand-side layout");

@@ -8,6 +8,10 @@
 //   A  = m x n1, a_mode 0: cnt distinct uniform columns per row, values N(0,1)
 //               a_mode 1: one uniform column c plus its +x, +y, +z grid
 //                         neighbours (-x/-y/-z at the upper face), N(0,1)
+//               a_mode 2: cnt in cnt_lo..cnt_hi (uniform); one uniform
+//                         column c plus the first cnt-1 cells of the offset
+//                         list +x, +y, (+z), -x, -y, (-z), +x+y, -x-y, +x-y,
+//                         -x+y, +2x, -2x that lie inside the grid, N(0,1)
 //   D  = d_value * I
 //   x_true ~ U(-1, 1),  b = K x_true (FP64, row-wise ascending-column sums)
 // Random numbers: SplitMix64 (Steele, Lea & Flood 2014), 53-bit uniforms,
@@ -71,6 +75,9 @@ KktProblem generate_kkt(const KktSpec& spec) {
   if (spec.a_mode == 0 && (spec.cnt_lo < 1 || spec.cnt_hi < spec.cnt_lo || spec.cnt_hi > n1))
     throw std::invalid_argument("bad constraint-row count range");
   if (spec.a_mode == 1 && spec.dim != 3) throw std::invalid_argument("a_mode 1 needs a 3D grid");
+  if (spec.a_mode == 2 && (spec.cnt_lo < 1 || spec.cnt_hi < spec.cnt_lo || spec.cnt_hi > 5 || k < 3))
+    throw std::invalid_argument("a_mode 2 needs 1 <= cnt_lo <= cnt_hi <= 5 and a grid side >= 3");
+  if (spec.a_mode < 0 || spec.a_mode > 2) throw std::invalid_argument("a_mode must be 0, 1 or 2");
 
   SplitMix64 rng(spec.seed);
 
@@ -92,6 +99,24 @@ KktProblem generate_kkt(const KktSpec& spec) {
         int64_t c = rng.below(n1);
         if (std::find(cols.begin(), cols.end(), c) == cols.end()) cols.push_back(c);
       }
+    } else if (spec.a_mode == 2) {
+      int cnt = spec.cnt_lo;
+      if (spec.cnt_hi > spec.cnt_lo) cnt += static_cast<int>(rng.below(spec.cnt_hi - spec.cnt_lo + 1));
+      const int64_t c = rng.below(n1);
+      const int64_t x = c % k, y = (c / k) % k, z = spec.dim == 3 ? c / (k * k) : 0;
+      static const int off2[][3] = {{1, 0, 0}, {0, 1, 0}, {-1, 0, 0}, {0, -1, 0}, {1, 1, 0}, {-1, -1, 0},
+                                    {1, -1, 0}, {-1, 1, 0}, {2, 0, 0}, {-2, 0, 0}};
+      static const int off3[][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1},
+                                    {1, 1, 0}, {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {2, 0, 0}, {-2, 0, 0}};
+      const int (*off)[3] = spec.dim == 3 ? off3 : off2;
+      const int noff = spec.dim == 3 ? 12 : 10;
+      cols.push_back(c);
+      for (int t = 0; t < noff && static_cast<int>(cols.size()) < cnt; ++t) {
+        const int64_t xx = x + off[t][0], yy = y + off[t][1], zz = z + off[t][2];
+        if (xx < 0 || xx >= k || yy < 0 || yy >= k || zz < 0 || zz >= (spec.dim == 3 ? k : 1)) continue;
+        cols.push_back(xx + k * (yy + k * zz));
+      }
+      if (static_cast<int>(cols.size()) < cnt) throw std::logic_error("a_mode 2: too few grid neighbours");
     } else {
       const int64_t c = rng.below(n1);
       const int64_t x = c % k, y = (c / k) % k, z = c / (k * k);
@@ -163,8 +188,10 @@ CsrFull csr_from_upper(const CscUpper& A) {
   }
   for (int32_t i = 0; i < n; ++i) F.p[i + 1] += F.p[i];
   const int64_t nnz = F.p[n];
+  // a pattern-only input (empty x) gives a pattern-only CSR
+  const bool values = !A.x.empty();
   F.j.resize(nnz);
-  F.x.resize(nnz);
+  F.x.resize(values ? nnz : 0);
   F.src.resize(nnz);
   std::vector<int32_t> next(F.p.begin(), F.p.end() - 1);
   // Row r of the full matrix = column r of the upper part (columns <= r, in
@@ -175,7 +202,7 @@ CsrFull csr_from_upper(const CscUpper& A) {
     for (int32_t q = A.p[r]; q < A.p[r + 1]; ++q) {  // column r: rows <= r
       const int32_t pos = next[r]++;
       F.j[pos] = A.i[q];
-      F.x[pos] = A.x[q];
+      if (values) F.x[pos] = A.x[q];
       F.src[pos] = q;
     }
   }
@@ -185,7 +212,7 @@ CsrFull csr_from_upper(const CscUpper& A) {
       if (r == c) continue;
       const int32_t pos = next[r]++;
       F.j[pos] = c;
-      F.x[pos] = A.x[q];
+      if (values) F.x[pos] = A.x[q];
       F.src[pos] = q;
     }
   }

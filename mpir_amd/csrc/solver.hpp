@@ -10,11 +10,23 @@
 #include <string>
 #include <vector>
 
+#include "batch_plan.hpp"
 #include "common.hpp"
 #include "kernels/kernels.hpp"
 #include "plan.hpp"
 
 namespace mpir {
+
+// Outcome of a batch solve: per system s and right-hand side r, entry
+// s * max_rhs + r of the per-instance arrays (slots r >= nrhs[s] unused).
+struct BatchStats {
+  int32_t nsys = 0, max_rhs = 0;
+  std::vector<int32_t> positive_pivots, zero_pivots;  // per system
+  std::vector<int32_t> ir_iterations, gmres_iterations, stop_reason;  // per instance
+  std::vector<double> final_rel_residual, done_ms;    // per instance; done_ms from the start of the solve
+  double factor_ms = 0.0, solve_ms = 0.0;
+  int32_t arnoldi_steps = 0, precond_sweeps = 0;      // lock-step counts of the whole batch
+};
 
 enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1, kOrderNd = 2 };
 
@@ -83,6 +95,8 @@ class GmresLdl {
     scaling_ = mode;
   }
   void set_relax(bool on) { relax_ = on; }
+  int ordering() const { return ordering_; }
+  bool relax() const { return relax_; }
   // schedule of the triangular solves: kScheduleLevels or kScheduleDag
   void set_schedule(int sched) {
     MPIR_CHECK(sched == kScheduleLevels || sched == kScheduleDag, "solve schedule must be 0 (levels) or 1 (dag)");
@@ -109,6 +123,20 @@ class GmresLdl {
   int32_t n() const { return n_; }
   const std::vector<int32_t>& perm() const { return perm_; }
 
+  // ---- batches of independent systems, solved as one block-diagonal system
+  // Symbolic phase of every system from the pattern cache (analysed on the
+  // fly when missing), combined into one plan over the forest of their
+  // elimination trees.
+  void analyze_batch(const std::vector<std::shared_ptr<const PatternEntry>>& parts);
+  // values of every system (original entry order of its pattern)
+  void factor_batch(const double* const* Ax, bool device_ptr);
+  // nrhs[s] right-hand sides for system s, column-major with ld = n_s
+  void solve_batch(const int32_t* nrhs, const double* const* b, double* const* x, bool device_ptr);
+  const BatchStats& batch_stats() const { return bstats_; }
+  bool is_batch() const { return static_cast<bool>(batch_); }
+  int32_t batch_size() const;
+  int32_t batch_n(int32_t s) const;
+
   // classic iterative refinement of LUIR_QDLDL on the current factor
   // (natural ordering, no scaling); b, x host arrays
   void luir(const double* b, double* x, int maxiter, LuirResult* out);
@@ -127,6 +155,18 @@ class GmresLdl {
 
  private:
   void solve_one(const double* b_dev, double* x_dev);
+  void factor_loaded(double t0);  // numeric phase on the values in d_ax_
+  void batch_count_pivots();
+  void batch_collect_pivots(bool zero_pivot);
+  // batched GMRES on R slots of the combined vector; active[q] selects the
+  // instances that iterate, kq[q] returns their step counts
+  void precond_gmres_batch(const double* b, double* res, const std::vector<char>& active);
+  void precond_apply_slots(double* x, int R);
+  void spmv_slots(const double* x, double* y, int R);
+  void residual_slots(const double* x, const double* b, double* r, int R);
+  void bnorms(const double* v, int op, int R, std::vector<double>& out);
+  double* bpinned(size_t count);
+  void put_host(const void* src, size_t bytes, void* dst);
   void precond_gmres(const double* b, double* res);  // device vectors (permuted)
   double dnrm2(const double* v);
   double infnrm(const double* v);
@@ -188,6 +228,16 @@ class GmresLdl {
   DevBuf d_hcol_, d_givens_;
   std::vector<double> h_small_;
   double* h_pinned_ = nullptr;
+  // batch mode
+  std::unique_ptr<BatchSystem> batch_;
+  std::vector<std::shared_ptr<const PatternEntry>> batch_parts_;
+  BatchStats bstats_;
+  DevBuf d_b_sys_of_row_, d_b_col_base_, d_b_lperm_, d_b_ch_start_, d_b_ch_len_, d_b_sys_ch_;
+  DevBuf d_b_scalar_, d_b_scalar2_, d_b_mask_, d_b_kq_, d_b_off_, d_b_nrhs_, d_b_piv_;
+  DevBuf d_b_vec_;  // b, x, r, d, w of every slot
+  int32_t b_nch_ = 0, b_slots_ = 0;
+  double* b_pinned_ = nullptr;
+  size_t b_pinned_count_ = 0;
 };
 
 }  // namespace mpir

@@ -167,3 +167,51 @@ def test_mtx_driver_without_a_device():
     assert exe.exists(), "mpir_amd/mtx_solve is built by mpir_amd.build"
     proc = subprocess.run([str(exe), str(MTX)], capture_output=True, text=True, timeout=60)
     assert proc.returncode != 0 and "no gfx950 device" in proc.stderr
+
+
+def test_batch_generator_sizes_and_local_constraints():
+    systems = mpir_amd.batch_config(nsys=16, nrhs=2, seed=3, n1_lo=2e3, n1_hi=6e4)
+    assert len(systems) == 16
+    for q in systems:
+        P = q.kkt
+        assert P.n1 == q.k ** q.dim and P.n - P.n1 == P.n1 // 4
+        assert q.B.shape == (P.n, 2) and q.X_true.shape == (P.n, 2)
+        # constraint columns: 3-5 entries plus -D on the diagonal, all within
+        # two grid steps of the row's first cell
+        for j in range(P.n1, P.n, max(1, (P.n - P.n1) // 50)):
+            rows = P.Ai[P.Ap[j]:P.Ap[j + 1]]
+            assert 4 <= len(rows) <= 6 and rows[-1] == j
+            cells = rows[:-1]
+            coords = np.stack([(cells // q.k ** a) % q.k for a in range(q.dim)], axis=1)
+            assert np.abs(coords[:, None, :] - coords[None, :, :]).max() <= 4
+    again = mpir_amd.batch_config(nsys=16, nrhs=2, seed=3, n1_lo=2e3, n1_hi=6e4)
+    for a, b in zip(systems, again):
+        np.testing.assert_array_equal(a.kkt.Ax, b.kkt.Ax)
+        np.testing.assert_array_equal(a.B, b.B)
+
+
+def test_batch_plan_is_the_union_of_the_systems_plans():
+    # the joined plan over the forest has the systems' fill and flops, and
+    # as many levels as the deepest system
+    systems = mpir_amd.batch_config(nsys=5, nrhs=1, seed=7, n1_lo=300, n1_hi=4000)
+    pats = [(q.kkt.Ap, q.kkt.Ai) for q in systems]
+    joined = mpir_amd.batch_plan_info(pats)
+    alone = [mpir_amd.batch_plan_info([p]) for p in pats]
+    assert joined["n"] == sum(q.kkt.n for q in systems)
+    assert joined["nnz_l"] == sum(a["nnz_l"] for a in alone)
+    assert list(joined["nnz_l_each"]) == [a["nnz_l"] for a in alone]
+    assert abs(joined["factor_flops"] - sum(a["factor_flops"] for a in alone)) <= len(alone)
+    assert joined["nsupernodes"] == sum(a["nsupernodes"] for a in alone)
+    assert joined["nlevels"] == max(a["nlevels"] for a in alone)
+    assert joined["panel_entries"] == sum(a["panel_entries"] for a in alone)
+    assert joined["panel_tasks"] == sum(a["panel_tasks"] for a in alone)
+    for a in alone:
+        assert a["nnz_l"] > 0 and a["bytes_each"][0] > 0
+
+
+def test_batch_plan_rejects_bad_patterns():
+    P = mpir_amd.kkt_config(0)
+    Ai = P.Ai.copy()
+    Ai[P.Ap[5]] = 7  # below the diagonal of column 5
+    with pytest.raises(mpir_amd.MpirError, match="below the diagonal"):
+        mpir_amd.batch_plan_info([(P.Ap, P.Ai), (P.Ap, Ai)])
