@@ -158,6 +158,135 @@ def precision_sweep(mpir, k: int, tol: float) -> int:
     return 0
 
 
+def batched(mpir, torch, args) -> int:
+    """The batched form: `nsys` independent KKT systems (2D/3D grids, n1
+    log-uniform in [2e3, 6e4], m = n1/4 local constraint rows of 3-5 entries,
+    D = 1e-3 I), `nrhs` right-hand sides each; FP32 factor, FP64 GMRES-IR to
+    1e-10, all systems in one block-diagonal solve. One step = numeric
+    factorization of every system + GMRES-IR of every (system, RHS)."""
+    log(f"generating {args.nsys} systems x {args.nrhs} RHS")
+    systems = mpir.batch_config(nsys=args.nsys, nrhs=args.nrhs, seed=args.seed)
+    dev = torch.device("cuda:0")
+    ax = [torch.from_numpy(q.kkt.Ax).to(dev) for q in systems]
+    bt = [torch.from_numpy(np.ascontiguousarray(q.B.T)).to(dev) for q in systems]
+    xt = [torch.zeros_like(b) for b in bt]
+    s = mpir.BatchLDLIR(np.float32, ordering=mpir.ORDER_ND, threads=16)
+    s.SetTolerance(args.tol)
+    t0 = time.perf_counter()
+    s.Analyze([(q.kkt.Ap, q.kkt.Ai) for q in systems])
+    analyze_s = time.perf_counter() - t0
+    tot = s.stats()["totals"]
+    log(f"analyze {analyze_s:.1f}s: n={tot['n']} nnz_l={tot['nnz_l']} flops={tot['factor_flops']:.3e} "
+        f"levels={tot['nlevels']}")
+    nr = [args.nrhs] * len(systems)
+    axp = [a.data_ptr() for a in ax]
+    bp = [b.data_ptr() for b in bt]
+    xp = [x.data_ptr() for x in xt]
+    for _ in range(args.warmup):
+        s.factor_dev(axp)
+        s.solve_dev(nr, bp, xp)
+    torch.cuda.synchronize()
+    lat, steps_ms = [], []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        a = time.perf_counter()
+        s.factor_dev(axp)
+        fac_ms = (time.perf_counter() - a) * 1e3
+        s.solve_dev(nr, bp, xp)
+        steps_ms.append((time.perf_counter() - a) * 1e3)
+        st = s.stats()
+        # a system is done when its last right-hand side has stopped
+        lat.extend((fac_ms + st["done_ms"].max(axis=1)).tolist())
+        log(f"step {i}: {steps_ms[-1]:.1f} ms (factor {fac_ms:.1f} ms)")
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    st = s.stats()
+    pairs = len(systems) * args.nrhs
+    # backward error of every (system, RHS)
+    import scipy.sparse as sp
+    eta = []
+    for q, x in zip(systems, xt):
+        X = x.cpu().numpy().T
+        U = sp.csc_matrix((q.kkt.Ax, q.kkt.Ai, q.kkt.Ap), shape=(q.kkt.n, q.kkt.n))
+        K = (U + sp.triu(U, 1).T).tocsr()
+        R = q.B - K @ X
+        kn = abs(K).sum(axis=1).max()
+        eta.extend((np.abs(R).max(axis=0) / (kn * np.abs(X).max(axis=0) + np.abs(q.B).max(axis=0))).tolist())
+    t_fac = s.time_kernels("factor", 1)
+    t_prec = s.time_kernels("precond", 3)
+    t_spmv = s.time_kernels("spmv", 10)
+    nnz_upper = sum(int(q.kkt.Ap[-1]) for q in systems)
+    b_prec = precond_bytes(tot["n"], tot["nnz_l"], 4)
+    b_spmv = spmv_bytes(tot["n"], nnz_upper)
+    value = pairs * args.steps / total
+    out = {
+        "metric": "solves/s (system-RHS pairs) at bwd err <=1e-10, batched form; p50/p99 ms per system",
+        "value": round(value, 2), "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-factor/f64", "data": "synthetic",
+        "config": {"workload": f"{len(systems)} KKT systems (2D/3D grids, n1 log-uniform 2e3..6e4, m=n1/4 local "
+                               f"constraint rows of 3-5 entries, D=1e-3) x {args.nrhs} RHS, ND ordering, FP32 LDL^T "
+                               f"+ FP64 GMRES-IR, tol {args.tol:g}",
+                   "systems": len(systems), "rhs_per_system": args.nrhs, "n_total": tot["n"],
+                   "nnz_l": tot["nnz_l"], "factor_flops": tot["factor_flops"], "analyze_s": round(analyze_s, 2)},
+        "bwd_err_max": max(eta),
+        "p50_ms": round(pct(lat, 0.5), 3), "p99_ms": round(pct(lat, 0.99), 3),
+        "gmres_iterations_mean": float(st["gmres_iterations"].mean()),
+        "ir_iterations_max": int(st["ir_iterations"].max()),
+        "inertia_ok": bool(all(int(p) == q.kkt.n1 for p, q in zip(st["positive_pivots"], systems))),
+        "kernels": {"factor_ms": round(t_fac, 3), "factor_TFLOPs": round(tot["factor_flops"] / t_fac / 1e9, 3),
+                    "precond_slot_ms": round(t_prec, 3), "precond_GBs": round(b_prec / t_prec / 1e6, 1),
+                    "spmv_slot_ms": round(t_spmv, 4), "spmv_GBs": round(b_spmv / t_spmv / 1e6, 1)},
+    }
+    tf = tot["factor_flops"] / t_fac / 1e9
+    out["roofline"] = {"bound": "mfma", "achieved": round(tf, 3), "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
+                       "frac": round(tf / PEAK_TFLOPS["f32"], 4), "traffic": None,
+                       "kernel": "numeric LDL^T factorization of the whole batch"}
+    out["roofline_solve"] = {"bound": "hbm", "achieved": round(b_prec / t_prec / 1e6, 1), "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": round(b_prec / t_prec / 1e6 / HBM_PEAK_GBS, 4),
+                             "kernel": "M^-1 application to one right-hand side of every system"}
+    if not args.no_cpu:
+        log("cpu baseline")
+        out["cpu_baseline"] = batched_cpu_baseline(mpir, systems, tot, args)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def batched_cpu_baseline(mpir, systems, tot, args, seconds_budget: float = 20.0) -> dict:
+    """The oracle (one core) on a bounded sample of the batch: the smallest
+    systems, one RHS each, FP32 factor + FP64 GMRES-IR; its factor rate
+    (flop/s) and solve rate (nnz(L) per second per RHS) scale the sample to
+    the whole batch."""
+    from tests.oracle_lib import Oracle
+    orc = Oracle()
+    order = sorted(range(len(systems)), key=lambda i: systems[i].kkt.n)
+    fl, lz, tf, ts, used = 0.0, 0, 0.0, 0.0, 0
+    t_begin = time.perf_counter()
+    for i in order:
+        q = systems[i]
+        g = mpir.GmresLDLIR(np.float32, ordering=mpir.ORDER_ND)
+        g.Analyze(q.kkt.Ap, q.kkt.Ai)
+        st = g.stats()
+        perm = g.perm()
+        rc, x, info = orc.gmres_ir(q.kkt.Ap, q.kkt.Ai, q.kkt.Ax, q.B[:, 0], perm=perm, uf=np.float32, tol=args.tol)
+        assert rc == 0
+        fl += st["factor_flops"]
+        lz += st["nnz_l"]
+        tf += info["factor_s"]
+        ts += info["solve_s"]
+        used += 1
+        if time.perf_counter() - t_begin > seconds_budget:
+            break
+    f_rate = fl / tf
+    s_rate = lz / ts
+    est = tot["factor_flops"] / f_rate + args.nrhs * tot["nnz_l"] / s_rate
+    pairs = len(systems) * args.nrhs
+    return {"value": round(pairs / est, 3), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle FP32-factor GMRES-IR on the {used} smallest systems (1 RHS each, {tf + ts:.1f} s): "
+                       f"factor {f_rate / 1e9:.2f} GFLOP/s, solve {s_rate / 1e6:.1f} M nnz(L)/s per RHS; scaled to "
+                       f"the batch's {tot['factor_flops']:.3e} flops and {args.nrhs} x {tot['nnz_l']} nnz(L)")}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +300,11 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--sweep", action="store_true",
                     help="precision form instead of the bench line: f64/f32/f16/bf16 factors x scaling x D scale")
+    ap.add_argument("--batched", action="store_true",
+                    help="batched form instead of the bench line: many systems x several RHS in one solve")
+    ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
+    ap.add_argument("--nrhs", type=int, default=8, help="right-hand sides per system of the batched form")
+    ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve path runs on a single device "
@@ -184,6 +318,8 @@ def main(argv=None) -> int:
         return 1
     if args.sweep:
         return precision_sweep(mpir, args.k, args.tol)
+    if args.batched:
+        return batched(mpir, torch, args)
     log(f"generating the {args.k}^3 KKT workload")
     P = mpir.kkt_config(1, k=args.k)
     dev = torch.device("cuda:0")
