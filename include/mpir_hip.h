@@ -240,6 +240,49 @@ int mpir_pattern_cache_info(int64_t* entries, int64_t* hits, int64_t* misses);
 int mpir_batch_plan_info(int32_t nsys, const int32_t* n, const int32_t* const* Ap, const int32_t* const* Ai,
                          int ordering, int64_t* info);
 
+/* ------------------------------------------------------------------ serving
+ * A stream of solve requests with continuous batching (no counterpart in the
+ * reference, which solves one system per GmresLDLIR object). Each request
+ * names a registered pattern and carries fresh values and nrhs right-hand
+ * sides; its data are derived deterministically from (pattern, seed) by
+ * mpir_server_request_data: off-diagonals of H and entries of A times
+ * (1 + 0.05u), -D times (1 + 0.05u), diag(H) times (1.05 + 0.05v), u ~ U(-1,1),
+ * v ~ U(0,1); x_true ~ U(-1,1); b = K x_true. Requests are admitted in arrival
+ * order into batches while the device bytes of all batches in flight stay
+ * within budget_bytes (a request larger than the budget runs alone);
+ * `lanes` batches (host thread + HIP stream each) run concurrently, so
+ * requests arriving while a batch is on the device join the next one and
+ * overlap with it. Every request is solved as GmresLDLIR::Solve would solve
+ * it (FP32 or FP64 factor, FP64 GMRES-IR to tol); its result is
+ * bit-identical whichever batch it lands in. */
+typedef struct mpir_server* mpir_server_t;
+int mpir_server_create(int uf, int lanes, int64_t budget_bytes, int32_t max_batch, double tol, int ordering,
+                       int32_t keep_x, int32_t threads, mpir_server_t* out);
+void mpir_server_destroy(mpir_server_t s);
+/* symbolic phase of the pattern (cached); base values and the size n1 of the
+ * H block; returns the pattern id (>= 0) or a negative status */
+int mpir_server_add_pattern(mpir_server_t s, int32_t n, const int32_t* Ap, const int32_t* Ai, const double* base,
+                            int32_t n1);
+/* `count` patterns at once, analysed on `threads` host threads; returns the
+ * id of the first (the others follow in order) */
+int mpir_server_add_patterns(mpir_server_t s, int32_t count, const int32_t* n, const int32_t* const* Ap,
+                             const int32_t* const* Ai, const double* const* base, const int32_t* n1, int32_t threads);
+/* Runs nreq requests arriving at t_arrive[i] ms after the call (ascending) to
+ * completion. Outputs per request (any may be NULL): admission and
+ * completion times (ms), batch and lane, status, the largest IR / GMRES
+ * iteration count and final relative residual over its right-hand sides. */
+int mpir_server_run(mpir_server_t s, int64_t nreq, const int32_t* pattern, const uint64_t* seed, const int32_t* nrhs,
+                    const double* t_arrive, double* t_admit, double* t_done, int32_t* batch, int32_t* lane,
+                    int32_t* status, int32_t* ir_max, int32_t* gmres_max, double* relres_max);
+/* counts[0..5] = completed, failed, batches, largest backlog (arrived, not
+ * admitted), peak bytes in flight, most batches in flight at once */
+int mpir_server_summary(mpir_server_t s, int64_t* counts, double* wall_ms);
+/* solution of request i of the last run (keep_x != 0): n x nrhs, column-major */
+int mpir_server_solution(mpir_server_t s, int64_t i, double* x);
+/* the data of a request, on the host (Ax: nnz, b and x_true: n x nrhs) */
+int mpir_server_request_data(mpir_server_t s, int32_t pattern, uint64_t seed, int32_t nrhs, double* Ax, double* b,
+                             double* x_true);
+
 /* ------------------------------------------------------------------ utilities
  * FP64 symmetric SpMV from the upper CSC with the reference's rounding
  * (MatrixMultiply, include/MPIR/ops.hpp:25-45), on the GPU. */

@@ -387,6 +387,131 @@ class BatchLDLIR:
         return list(self._n)
 
 
+class Server:
+    """A stream of solve requests with continuous batching on one GPU (see
+    include/mpir_hip.h, mpir_server_*): requests are admitted in arrival
+    order into batches by device-memory footprint, `lanes` batches run
+    concurrently, and each pattern's symbolic phase is computed once."""
+
+    def __init__(self, UF=np.float32, lanes: int = 2, budget_bytes: int = 64 << 30, max_batch: int = 64,
+                 tol: float = 1e-10, ordering: int = ORDER_ND, keep_x: bool = False, threads: int = 8):
+        uf = _PREC_OF.get(np.dtype(UF))
+        if uf is None:
+            raise TypeError("UF must be float64 or float32")
+        h = C.c_void_p()
+        _check(load().mpir_server_create(uf, int(lanes), int(budget_bytes), int(max_batch), float(tol), int(ordering),
+                                         int(bool(keep_x)), int(threads), C.byref(h)), "Server()")
+        self._h = h
+        self._n: list[int] = []
+        self._nnz: list[int] = []
+        self._last = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            load().mpir_server_destroy(h)
+            self._h = None
+
+    def add_pattern(self, Ap, Ai, base_Ax, n1: int) -> int:
+        Ap = np.ascontiguousarray(Ap, dtype=np.int32)
+        Ai = np.ascontiguousarray(Ai, dtype=np.int32)
+        base = np.ascontiguousarray(base_Ax, dtype=np.float64)
+        pid = _check(load().mpir_server_add_pattern(self._h, len(Ap) - 1, _ptr(Ap), _ptr(Ai), _ptr(base), int(n1)),
+                     "add_pattern")
+        self._n.append(len(Ap) - 1)
+        self._nnz.append(int(Ap[-1]))
+        return int(pid)
+
+    def add_patterns(self, kkts, threads: int = 8) -> int:
+        """Registers many KKT systems' patterns (base values = their values),
+        analysing them on `threads` host threads; returns the first id."""
+        Ap = [np.ascontiguousarray(P.Ap, dtype=np.int32) for P in kkts]
+        Ai = [np.ascontiguousarray(P.Ai, dtype=np.int32) for P in kkts]
+        base = [np.ascontiguousarray(P.Ax, dtype=np.float64) for P in kkts]
+        n = np.array([len(a) - 1 for a in Ap], dtype=np.int32)
+        n1 = np.array([P.n1 for P in kkts], dtype=np.int32)
+        first = _check(load().mpir_server_add_patterns(self._h, len(kkts), _ptr(n), _ptr_array(Ap), _ptr_array(Ai),
+                                                       _ptr_array(base), _ptr(n1), int(threads)), "add_patterns")
+        self._n.extend(int(v) for v in n)
+        self._nnz.extend(int(a[-1]) for a in Ap)
+        return int(first)
+
+    def run(self, pattern, seed, nrhs, t_arrive_ms) -> dict:
+        """Runs the request stream to completion; returns per-request arrays."""
+        pattern = np.ascontiguousarray(pattern, dtype=np.int32)
+        seed = np.ascontiguousarray(seed, dtype=np.uint64)
+        nrhs = np.ascontiguousarray(nrhs, dtype=np.int32)
+        ta = np.ascontiguousarray(t_arrive_ms, dtype=np.float64)
+        m = len(pattern)
+        out = {k: np.zeros(m) for k in ("t_admit", "t_done", "rel_residual_max")}
+        out.update({k: np.zeros(m, np.int32) for k in ("batch", "lane", "status", "ir_max", "gmres_max")})
+        _check(load().mpir_server_run(self._h, m, _ptr(pattern), _ptr(seed), _ptr(nrhs), _ptr(ta), _ptr(out["t_admit"]),
+                                      _ptr(out["t_done"]), _ptr(out["batch"]), _ptr(out["lane"]), _ptr(out["status"]),
+                                      _ptr(out["ir_max"]), _ptr(out["gmres_max"]), _ptr(out["rel_residual_max"])),
+               "Server.run")
+        out["t_arrive"] = ta
+        out["latency_ms"] = out["t_done"] - ta
+        self._last = (pattern, nrhs)
+        return out
+
+    def summary(self) -> dict:
+        c = np.zeros(6, np.int64)
+        w = C.c_double()
+        _check(load().mpir_server_summary(self._h, _ptr(c), C.byref(w)), "summary")
+        keys = ["completed", "failed", "batches", "max_backlog", "peak_inflight_bytes", "max_concurrent"]
+        d = {k: int(v) for k, v in zip(keys, c)}
+        d["wall_ms"] = w.value
+        return d
+
+    def solution(self, i: int) -> np.ndarray:
+        pattern, nrhs = self._last
+        n, r = self._n[pattern[i]], int(nrhs[i])
+        x = np.empty(n * r)
+        _check(load().mpir_server_solution(self._h, int(i), _ptr(x)), "solution")
+        return x.reshape(r, n).T.copy()
+
+    def request_data(self, pattern: int, seed: int, nrhs: int):
+        """(Ax, B, X_true) of a request, on the host."""
+        n = self._n[pattern]
+        Ax = np.empty(self._nnz[pattern])
+        B = np.empty(n * nrhs)
+        X = np.empty(n * nrhs)
+        _check(load().mpir_server_request_data(self._h, int(pattern), C.c_uint64(int(seed)), int(nrhs), _ptr(Ax),
+                                               _ptr(B), _ptr(X)), "request_data")
+        return Ax, B.reshape(nrhs, n).T.copy(), X.reshape(nrhs, n).T.copy()
+
+
+def serving_patterns(count: int = 64, k_lo: int = 16, k_hi: int = 48, seed: int = 0, d_value: float = 1e-3,
+                     h_shift: float = 1e-3) -> list:
+    """The serving pool: pattern p is the 3D k_p^3 Laplacian, k_p = k_lo +
+    round((k_hi - k_lo) p / (count - 1)), plus m = k_p^3 / 4 constraint rows
+    of 4 entries (a uniform cell and its +x, +y, +z neighbours), D = d_value I;
+    the base values of each pattern."""
+    out = []
+    for p in range(count):
+        k = k_lo + int(round((k_hi - k_lo) * p / max(1, count - 1)))
+        out.append(gen_kkt(3, k, h_shift, k ** 3 // 4, 1, 4, 4, d_value, seed * 7919 + p))
+    return out
+
+
+def poisson_stream(rate_per_s: float, seconds: float, npatterns: int, seed: int = 0, nrhs_lo: int = 1,
+                   nrhs_hi: int = 4):
+    """Arrival times (ms) of a Poisson process, a uniform pattern and a
+    uniform right-hand-side count per request, and a seed for its data."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t, times = 0.0, []
+    while True:
+        t += rng.exponential(1000.0 / rate_per_s)
+        if t >= seconds * 1000.0:
+            break
+        times.append(t)
+    m = len(times)
+    pattern = rng.integers(0, npatterns, m).astype(np.int32)
+    nrhs = rng.integers(nrhs_lo, nrhs_hi + 1, m).astype(np.int32)
+    seeds = rng.integers(1, 2 ** 62, m, dtype=np.int64).astype(np.uint64)
+    return np.asarray(times), pattern, nrhs, seeds
+
+
 def pattern_cache_info() -> dict:
     e, h, m = C.c_int64(), C.c_int64(), C.c_int64()
     _check(load().mpir_pattern_cache_info(C.byref(e), C.byref(h), C.byref(m)), "pattern_cache_info")
@@ -536,5 +661,6 @@ __all__ = [
     "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "LUIR_QDLDL", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
     "amd_order", "read_mtx", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
     "ORDER_AMD", "ORDER_ND", "SCALE_NONE", "SCALE_DIAG", "SCALE_RUIZ", "SCHED_LEVELS", "SCHED_DAG",
-    "BatchLDLIR", "BatchSystem", "batch_config", "batch_plan_info", "pattern_cache_info",
+    "BatchLDLIR", "BatchSystem", "batch_config", "batch_plan_info", "pattern_cache_info", "Server",
+    "serving_patterns", "poisson_stream",
 ]

@@ -92,6 +92,14 @@ SIGNATURES = {
     "mpir_batch_stats": (C.c_int, [P, P, P, P, P, P, P, P]),
     "mpir_pattern_cache_info": (C.c_int, [P, P, P]),
     "mpir_batch_plan_info": (C.c_int, [i32, P, P, P, C.c_int, P]),
+    "mpir_server_create": (C.c_int, [C.c_int, C.c_int, i64, i32, f64, C.c_int, i32, i32, P]),
+    "mpir_server_destroy": (None, [P]),
+    "mpir_server_add_pattern": (C.c_int, [P, i32, P, P, P, i32]),
+    "mpir_server_add_patterns": (C.c_int, [P, i32, P, P, P, P, P, i32]),
+    "mpir_server_run": (C.c_int, [P, i64, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mpir_server_summary": (C.c_int, [P, P, P]),
+    "mpir_server_solution": (C.c_int, [P, i64, P]),
+    "mpir_server_request_data": (C.c_int, [P, i32, u64, i32, P, P, P]),
 }
 
 _lib = None

@@ -54,6 +54,7 @@ HIP_SOURCES = [
     CSRC / "kernels" / "spmv.hip",
     CSRC / "gmres.hip",
     CSRC / "batch.hip",
+    CSRC / "serve.cpp",
     CSRC / "capi.cpp",
 ]
 

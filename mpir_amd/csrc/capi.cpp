@@ -13,6 +13,7 @@
 #include "../../include/mpir_hip.h"
 #include "common.hpp"
 #include "host/sparse.hpp"
+#include "serve.hpp"
 #include "solver.hpp"
 
 namespace mpir {
@@ -257,6 +258,9 @@ using namespace mpir;
 struct mpir_gmres_ldl {
   std::unique_ptr<GmresLdl> s;
   std::vector<int32_t> Ap, Ai;
+};
+struct mpir_server {
+  std::unique_ptr<mpir::Server> s;
 };
 
 namespace {
@@ -598,6 +602,103 @@ int mpir_batch_plan_info(int32_t nsys, const int32_t* n, const int32_t* const* A
       info[8 + 2 * s] = own[s].an.nnz_l;
       info[9 + 2 * s] = own[s].est_bytes;
     }
+    return 0;
+  });
+}
+
+// ---------------------------------------------------------------- serving
+int mpir_server_create(int uf, int lanes, int64_t budget_bytes, int32_t max_batch, double tol, int ordering,
+                       int32_t keep_x, int32_t threads, mpir_server_t* out) {
+  return guarded([&] {
+    MPIR_CHECK(out, "null output handle");
+    *out = nullptr;
+    ServeOptions o;
+    o.uf = uf;
+    o.lanes = lanes;
+    o.budget_bytes = budget_bytes;
+    o.max_batch = max_batch;
+    o.tol = tol;
+    o.ordering = ordering;
+    o.keep_x = keep_x;
+    o.threads = std::max(1, threads);
+    auto h = std::make_unique<mpir_server>();
+    h->s = std::make_unique<Server>(o);
+    *out = h.release();
+    return 0;
+  });
+}
+void mpir_server_destroy(mpir_server_t h) { delete h; }
+int mpir_server_add_pattern(mpir_server_t h, int32_t n, const int32_t* Ap, const int32_t* Ai, const double* base,
+                            int32_t n1) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s, "null server handle");
+    return h->s->add_pattern(n, Ap, Ai, base, n1);
+  });
+}
+int mpir_server_add_patterns(mpir_server_t h, int32_t count, const int32_t* n, const int32_t* const* Ap,
+                             const int32_t* const* Ai, const double* const* base, const int32_t* n1, int32_t threads) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s, "null server handle");
+    return h->s->add_patterns(count, n, Ap, Ai, base, n1, std::max(1, threads));
+  });
+}
+int mpir_server_run(mpir_server_t h, int64_t nreq, const int32_t* pattern, const uint64_t* seed, const int32_t* nrhs,
+                    const double* t_arrive, double* t_admit, double* t_done, int32_t* batch, int32_t* lane,
+                    int32_t* status, int32_t* ir_max, int32_t* gmres_max, double* relres_max) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s, "null server handle");
+    MPIR_CHECK(nreq >= 0 && (nreq == 0 || (pattern && seed && nrhs && t_arrive)), "bad request arrays");
+    std::vector<ServeRecord> rs(nreq);
+    for (int64_t i = 0; i < nreq; ++i) {
+      rs[i].pattern = pattern[i];
+      rs[i].seed = seed[i];
+      rs[i].nrhs = nrhs[i];
+      rs[i].t_arrive = t_arrive[i];
+    }
+    h->s->run(rs);
+    for (int64_t i = 0; i < nreq; ++i) {
+      if (t_admit) t_admit[i] = rs[i].t_admit;
+      if (t_done) t_done[i] = rs[i].t_done;
+      if (batch) batch[i] = rs[i].batch;
+      if (lane) lane[i] = rs[i].lane;
+      if (status) status[i] = rs[i].status;
+      if (ir_max) ir_max[i] = rs[i].ir_max;
+      if (gmres_max) gmres_max[i] = rs[i].gmres_max;
+      if (relres_max) relres_max[i] = rs[i].rel_residual_max;
+    }
+    return 0;
+  });
+}
+int mpir_server_summary(mpir_server_t h, int64_t* counts, double* wall_ms) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s, "null server handle");
+    const ServeSummary& m = h->s->summary();
+    if (counts) {
+      counts[0] = m.completed;
+      counts[1] = m.failed;
+      counts[2] = m.batches;
+      counts[3] = m.max_backlog;
+      counts[4] = m.peak_inflight_bytes;
+      counts[5] = m.max_concurrent;
+    }
+    if (wall_ms) *wall_ms = m.wall_ms;
+    return 0;
+  });
+}
+int mpir_server_solution(mpir_server_t h, int64_t i, double* x) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s && x, "null argument");
+    const std::vector<double>& v = h->s->solution(i);
+    std::memcpy(x, v.data(), v.size() * sizeof(double));
+    return 0;
+  });
+}
+int mpir_server_request_data(mpir_server_t h, int32_t pattern, uint64_t seed, int32_t nrhs, double* Ax, double* b,
+                             double* x_true) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s && Ax && b && x_true, "null argument");
+    MPIR_CHECK(nrhs >= 1, "nrhs must be positive");
+    materialize(h->s->pattern(pattern), seed, nrhs, Ax, b, x_true);
     return 0;
   });
 }
