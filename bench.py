@@ -326,6 +326,8 @@ def serving(mpir, args) -> int:
                "mean_batch": round(float(sizes.mean()), 2) if len(sizes) else 0.0,
                "peak_inflight_GB": round(summ["peak_inflight_bytes"] / 1e9, 2),
                "max_concurrent_batches": summ["max_concurrent"],
+               "phase_ms_per_batch": {k: round(summ[k + "_ms"] / max(1, summ["batches"]), 2)
+                                      for k in ("prepare", "plan", "factor", "solve")},
                "rel_residual_max": float(out["rel_residual_max"][ok].max()) if ok.any() else None,
                "gmres_iterations_max": int(out["gmres_max"][ok].max()) if ok.any() else None}
         log(f"stream done: {res}")

@@ -275,8 +275,10 @@ int mpir_server_run(mpir_server_t s, int64_t nreq, const int32_t* pattern, const
                     const double* t_arrive, double* t_admit, double* t_done, int32_t* batch, int32_t* lane,
                     int32_t* status, int32_t* ir_max, int32_t* gmres_max, double* relres_max);
 /* counts[0..5] = completed, failed, batches, largest backlog (arrived, not
- * admitted), peak bytes in flight, most batches in flight at once */
-int mpir_server_summary(mpir_server_t s, int64_t* counts, double* wall_ms);
+ * admitted), peak bytes in flight, most batches in flight at once;
+ * ms[0..4] = wall time of the run, then summed over batches: request data,
+ * joined symbolic plan and upload, numeric factorization, GMRES-IR */
+int mpir_server_summary(mpir_server_t s, int64_t* counts, double* ms);
 /* solution of request i of the last run (keep_x != 0): n x nrhs, column-major */
 int mpir_server_solution(mpir_server_t s, int64_t i, double* x);
 /* the data of a request, on the host (Ax: nnz, b and x_true: n x nrhs) */

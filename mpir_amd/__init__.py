@@ -456,11 +456,12 @@ class Server:
 
     def summary(self) -> dict:
         c = np.zeros(6, np.int64)
-        w = C.c_double()
-        _check(load().mpir_server_summary(self._h, _ptr(c), C.byref(w)), "summary")
+        w = np.zeros(5)
+        _check(load().mpir_server_summary(self._h, _ptr(c), _ptr(w)), "summary")
         keys = ["completed", "failed", "batches", "max_backlog", "peak_inflight_bytes", "max_concurrent"]
         d = {k: int(v) for k, v in zip(keys, c)}
-        d["wall_ms"] = w.value
+        for k, v in zip(["wall_ms", "prepare_ms", "plan_ms", "factor_ms", "solve_ms"], w):
+            d[k] = float(v)
         return d
 
     def solution(self, i: int) -> np.ndarray:

@@ -294,30 +294,37 @@ void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntr
   }
   analyzed_ = factored_ = false;
   precond_only_ = false;
-  auto B = std::make_unique<BatchSystem>(combine_patterns(raw));
+  // the joined system and plan reuse the memory of the previous batch
+  if (!batch_) batch_ = std::make_unique<BatchSystem>();
+  if (!plan_) plan_ = std::make_unique<FactorPlan>();
+  BatchSystem* B = batch_.get();
+  try {
+    combine_patterns(raw, *B, *plan_);
+  } catch (const std::exception& e) {
+    batch_.reset();
+    throw Error(kErrInvalid, e.what());
+  }
   batch_parts_ = parts;
   hipStream_t st = stream_;
   n_ = B->col_base.back();
   nnz_ = B->ent_base.back();
-  Ap_ = B->Ap;
-  Ai_ = B->Ai;
-  Analysis& an = B->an;
-  perm_ = an.perm;
-  plan_ = std::make_unique<FactorPlan>(std::move(an.plan));
-  relaxed_ = an.relaxed;
-  Cp_ = an.C.p;
-  Ci_ = an.C.i;
-  parent_ = an.parent;
-  Lnz_ = an.Lnz;
-  true_nnz_l_ = an.nnz_l;
-  true_flops_ = an.flops;
+  Ap_.clear();
+  Ai_.clear();
+  Cp_.clear();
+  Ci_.clear();
+  parent_.clear();
+  Lnz_.clear();
+  perm_ = B->perm;
+  relaxed_ = B->relaxed;
+  true_nnz_l_ = B->nnz_l;
+  true_flops_ = B->flops;
   upload_plan();
   upload_vec(d_csr_p_, B->csr_p, st);
   upload_vec(d_csr_j_, B->csr_j, st);
   upload_vec(d_csr_src_, B->csr_src, st);
   d_csr_x_.alloc(std::max<size_t>(1, B->csr_j.size()) * sizeof(double));
   upload_vec(d_entry_col_, B->ecol, st);
-  upload_vec(d_entry_row_, B->Ai, st);
+  upload_vec(d_entry_row_, B->erow, st);
   upload_vec(d_diag_pos_, B->diag_pos, st);
   upload_vec(d_perm_, perm_, st);
   // reduction chunks: every system's rows cut into pieces of kChunk
@@ -337,8 +344,6 @@ void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntr
   upload_vec(d_b_ch_len_, ch_len, st);
   upload_vec(d_b_sys_ch_, sys_ch, st);
   d_b_piv_.alloc(2 * sizeof(int32_t) * B->nsys);
-  // the permuted pattern of the forest is kept for export_factor
-  batch_ = std::move(B);
   alloc_numeric();
   MPIR_HIP(hipStreamSynchronize(st));
 

@@ -10,6 +10,27 @@
 
 namespace mpir {
 
+namespace {
+// f(i) for i in [0, n) on up to `threads` host threads
+template <class F>
+void parallel_for(int64_t n, int threads, F&& f) {
+  const int nt = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(threads, n)));
+  if (nt == 1) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  auto work = [&] {
+    for (int64_t i; (i = next.fetch_add(1)) < n;) f(i);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+}
+int join_threads() { return static_cast<int>(std::max(1u, std::min(8u, std::thread::hardware_concurrency()))); }
+}  // namespace
+
 uint64_t pattern_key(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax) {
   // FNV-1a over the dimension, the options and the index arrays
   uint64_t h = 1469598103934665603ull;
@@ -176,119 +197,307 @@ size_t PatternCache::size() const {
   return map_.size();
 }
 
-BatchSystem combine_patterns(const std::vector<const PatternEntry*>& parts) {
-  BatchSystem B;
+void merge_plans(const std::vector<const FactorPlan*>& P, const std::vector<int32_t>& col_base, FactorPlan& M) {
+  const size_t S = P.size();
+  // every list starts empty; the vectors keep their memory
+  for (auto* v : {&M.child_ptr, &M.child_list, &M.small_ptr, &M.small_list, &M.big_ptr, &M.big_list, &M.asm_ptr,
+                  &M.asm_child_ptr, &M.asm_child, &M.level_step_ptr, &M.pan_ptr, &M.upd_ptr, &M.pan_aug_ptr,
+                  &M.upd_aug_ptr, &M.fx_ptr, &M.fu_ptr, &M.br_ptr, &M.bx_ptr, &M.fx_task, &M.fu_task, &M.br_task,
+                  &M.bx_task, &M.frow_off, &M.grow_ptr, &M.cb_base, &M.dag_meta, &M.sn.level_ptr, &M.sn.level_sn})
+    v->clear();
+  for (auto* v : {&M.upd_off, &M.ypart_off, &M.upart_off, &M.rpart_off, &M.xpart_off, &M.val_dst}) v->clear();
+  for (auto* v : {&M.asm_task, &M.pan_task, &M.upd_task, &M.gent, &M.dag_fwd, &M.dag_bwd}) v->clear();
+  Supernodes& T = M.sn;
+  // bases of every system in the joined numbering
+  struct Base {
+    int32_t sn = 0, col = 0, frow = 0, cb = 0;
+    int64_t rows = 0, panel = 0, upd = 0, gent = 0, yp = 0, up = 0, rp = 0, xp = 0;
+  };
+  std::vector<Base> b(S + 1);
+  int32_t nlev = 0;
+  for (size_t s = 0; s < S; ++s) {
+    const FactorPlan& p = *P[s];
+    const Supernodes& q = p.sn;
+    Base& n = b[s + 1];
+    n.sn = b[s].sn + q.ns;
+    n.col = col_base[s + 1];
+    n.rows = b[s].rows + static_cast<int64_t>(q.rows.size());
+    n.panel = b[s].panel + p.panel_entries;
+    n.upd = b[s].upd + p.upd_pool;
+    n.frow = b[s].frow + p.frow_off[q.ns];
+    n.gent = b[s].gent + static_cast<int64_t>(p.gent.size()) - 2;
+    n.cb = b[s].cb + p.ncb_total;
+    n.yp = b[s].yp + p.nypart;
+    n.up = b[s].up + p.nupart;
+    n.rp = b[s].rp + p.nrpart;
+    n.xp = b[s].xp + p.nxpart;
+    nlev = std::max(nlev, q.nlevels);
+    if (q.first[0] != 0 || q.n != col_base[s + 1] - col_base[s]) throw std::logic_error("plan does not match its system");
+  }
+  const Base& E = b[S];
+  if (E.rows >= INT32_MAX || E.gent >= INT32_MAX || E.up >= INT32_MAX)
+    throw std::length_error("batch exceeds the 32-bit solve gather indices");
+  // supernodes
+  T.n = E.col;
+  T.ns = E.sn;
+  T.nlevels = nlev;
+  T.first.resize(static_cast<size_t>(T.ns) + 1);
+  T.nrow.resize(T.ns);
+  T.rows_off.resize(static_cast<size_t>(T.ns) + 1);
+  T.sparent.resize(T.ns);
+  T.level.resize(T.ns);
+  T.panel_off.resize(static_cast<size_t>(T.ns) + 1);
+  T.rows.resize(E.rows);
+  T.relidx.resize(E.rows);
+  T.col2sn.resize(T.n);
+  T.nnz_l = 0;
+  T.flops = 0.0;
+  T.max_front = 0;
+  for (size_t s = 0; s < S; ++s) {
+    const Supernodes& q = P[s]->sn;
+    T.nnz_l += q.nnz_l;
+    T.flops += q.flops;
+    T.max_front = std::max(T.max_front, q.max_front);
+  }
+  parallel_for(static_cast<int64_t>(S), join_threads(), [&](int64_t s) {
+    const Supernodes& q = P[s]->sn;
+    const Base& o = b[s];
+    for (int32_t j = 0; j < q.ns; ++j) {
+      const int64_t d = o.sn + j;
+      T.first[d] = o.col + q.first[j];
+      T.nrow[d] = q.nrow[j];
+      T.rows_off[d] = o.rows + q.rows_off[j];
+      T.sparent[d] = q.sparent[j] >= 0 ? o.sn + q.sparent[j] : -1;
+      T.level[d] = q.level[j];
+      T.panel_off[d] = o.panel + q.panel_off[j];
+    }
+    for (size_t r = 0; r < q.rows.size(); ++r) {
+      T.rows[o.rows + r] = o.col + q.rows[r];
+      T.relidx[o.rows + r] = q.relidx[r];
+    }
+    for (int32_t c = 0; c < q.n; ++c) T.col2sn[o.col + c] = o.sn + q.col2sn[c];
+  });
+  T.first[T.ns] = T.n;
+  T.rows_off[T.ns] = E.rows;
+  T.panel_off[T.ns] = E.panel;
+  T.level_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
+  for (int32_t l = 0; l < nlev; ++l) {
+    for (size_t s = 0; s < S; ++s) {
+      const Supernodes& q = P[s]->sn;
+      if (l >= q.nlevels) continue;
+      for (int32_t t = q.level_ptr[l]; t < q.level_ptr[l + 1]; ++t) T.level_sn.push_back(b[s].sn + q.level_sn[t]);
+    }
+    T.level_ptr[l + 1] = static_cast<int32_t>(T.level_sn.size());
+  }
+  // children, update pool (each system keeps its own region)
+  M.child_ptr.assign(1, 0);
+  for (size_t s = 0; s < S; ++s) {
+    const FactorPlan& p = *P[s];
+    const int32_t c0 = M.child_ptr.back();
+    for (int32_t j = 0; j < p.sn.ns; ++j) M.child_ptr.push_back(c0 + p.child_ptr[j + 1]);
+    for (int32_t c : p.child_list) M.child_list.push_back(b[s].sn + c);
+    for (int64_t u : p.upd_off) M.upd_off.push_back(u >= 0 ? b[s].upd + u : -1);
+    M.cb_base.insert(M.cb_base.end(), p.cb_base.begin(), p.cb_base.end());
+    for (size_t j = M.cb_base.size() - p.cb_base.size(); j < M.cb_base.size(); ++j) M.cb_base[j] += b[s].cb;
+    for (int64_t v : p.ypart_off) M.ypart_off.push_back(b[s].yp + v);
+    for (int64_t v : p.upart_off) M.upart_off.push_back(b[s].up + v);
+    for (int64_t v : p.rpart_off) M.rpart_off.push_back(b[s].rp + v);
+    for (int64_t v : p.xpart_off) M.xpart_off.push_back(b[s].xp + v);
+  }
+  M.upd_pool = E.upd;
+  M.panel_entries = E.panel;
+  M.ncb_total = E.cb;
+  M.nypart = E.yp;
+  M.nupart = E.up;
+  M.nrpart = E.rp;
+  M.nxpart = E.xp;
+  // per-level lists
+  auto lvl_list = [&](std::vector<int32_t>& ptr, std::vector<int32_t>& list,
+                      const std::vector<int32_t> FactorPlan::*pp, const std::vector<int32_t> FactorPlan::*pl) {
+    ptr.assign(static_cast<size_t>(nlev) + 1, 0);
+    for (int32_t l = 0; l < nlev; ++l) {
+      for (size_t s = 0; s < S; ++s) {
+        const FactorPlan& p = *P[s];
+        if (l >= p.sn.nlevels) continue;
+        const std::vector<int32_t>& sp = p.*pp;
+        const std::vector<int32_t>& sl = p.*pl;
+        for (int32_t t = sp[l]; t < sp[l + 1]; ++t) list.push_back(b[s].sn + sl[t]);
+      }
+      ptr[l + 1] = static_cast<int32_t>(list.size());
+    }
+  };
+  lvl_list(M.small_ptr, M.small_list, &FactorPlan::small_ptr, &FactorPlan::small_list);
+  lvl_list(M.big_ptr, M.big_list, &FactorPlan::big_ptr, &FactorPlan::big_list);
+  auto lvl_triples = [&](std::vector<int32_t>& ptr, std::vector<int32_t>& list,
+                         const std::vector<int32_t> FactorPlan::*pp, const std::vector<int32_t> FactorPlan::*pl) {
+    ptr.assign(static_cast<size_t>(nlev) + 1, 0);
+    for (int32_t l = 0; l < nlev; ++l) {
+      for (size_t s = 0; s < S; ++s) {
+        const FactorPlan& p = *P[s];
+        if (l >= p.sn.nlevels) continue;
+        const std::vector<int32_t>& sp = p.*pp;
+        const std::vector<int32_t>& sl = p.*pl;
+        for (int32_t t = sp[l]; t < sp[l + 1]; ++t)
+          list.insert(list.end(), {b[s].sn + sl[3 * t], sl[3 * t + 1], sl[3 * t + 2]});
+      }
+      ptr[l + 1] = static_cast<int32_t>(list.size() / 3);
+    }
+  };
+  lvl_triples(M.fx_ptr, M.fx_task, &FactorPlan::fx_ptr, &FactorPlan::fx_task);
+  lvl_triples(M.fu_ptr, M.fu_task, &FactorPlan::fu_ptr, &FactorPlan::fu_task);
+  lvl_triples(M.br_ptr, M.br_task, &FactorPlan::br_ptr, &FactorPlan::br_task);
+  lvl_triples(M.bx_ptr, M.bx_task, &FactorPlan::bx_ptr, &FactorPlan::bx_task);
+  // assembly tasks with their child ranges
+  M.asm_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
+  M.asm_child_ptr.assign(1, 0);
+  for (int32_t l = 0; l < nlev; ++l) {
+    for (size_t s = 0; s < S; ++s) {
+      const FactorPlan& p = *P[s];
+      if (l >= p.sn.nlevels) continue;
+      for (int32_t t = p.asm_ptr[l]; t < p.asm_ptr[l + 1]; ++t) {
+        Task4 tk = p.asm_task[t];
+        tk.a += b[s].sn;
+        M.asm_task.push_back(tk);
+        for (int32_t c = p.asm_child_ptr[t]; c < p.asm_child_ptr[t + 1]; ++c) {
+          const int32_t* e = &p.asm_child[5 * static_cast<size_t>(c)];
+          M.asm_child.insert(M.asm_child.end(), {b[s].sn + e[0], e[1], e[2], e[3], e[4]});
+        }
+        M.asm_child_ptr.push_back(static_cast<int32_t>(M.asm_child.size() / 5));
+      }
+    }
+    M.asm_ptr[l + 1] = static_cast<int32_t>(M.asm_task.size());
+  }
+  // panel and update tasks per (level, step): the real rows of every
+  // system, then their identity rows
+  M.level_step_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
+  M.pan_ptr.assign(1, 0);
+  M.upd_ptr.assign(1, 0);
+  for (int32_t l = 0; l < nlev; ++l) {
+    int32_t steps = 0;
+    for (size_t s = 0; s < S; ++s) {
+      const FactorPlan& p = *P[s];
+      if (l < p.sn.nlevels) steps = std::max(steps, p.level_step_ptr[l + 1] - p.level_step_ptr[l]);
+    }
+    for (int32_t k = 0; k < steps; ++k) {
+      for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+          M.pan_aug_ptr.push_back(static_cast<int32_t>(M.pan_task.size()));
+          M.upd_aug_ptr.push_back(static_cast<int32_t>(M.upd_task.size()));
+        }
+        for (size_t s = 0; s < S; ++s) {
+          const FactorPlan& p = *P[s];
+          if (l >= p.sn.nlevels || k >= p.level_step_ptr[l + 1] - p.level_step_ptr[l]) continue;
+          const int32_t g = p.level_step_ptr[l] + k;
+          const int32_t pa = pass == 0 ? p.pan_ptr[g] : p.pan_aug_ptr[g];
+          const int32_t pe = pass == 0 ? p.pan_aug_ptr[g] : p.pan_ptr[g + 1];
+          for (int32_t t = pa; t < pe; ++t) {
+            Task4 tk = p.pan_task[t];
+            tk.a += b[s].sn;
+            M.pan_task.push_back(tk);
+          }
+          const int32_t ua = pass == 0 ? p.upd_ptr[g] : p.upd_aug_ptr[g];
+          const int32_t ue = pass == 0 ? p.upd_aug_ptr[g] : p.upd_ptr[g + 1];
+          for (int32_t t = ua; t < ue; ++t) {
+            Task4 tk = p.upd_task[t];
+            tk.a += b[s].sn;
+            M.upd_task.push_back(tk);
+          }
+        }
+      }
+      M.pan_ptr.push_back(static_cast<int32_t>(M.pan_task.size()));
+      M.upd_ptr.push_back(static_cast<int32_t>(M.upd_task.size()));
+    }
+    M.level_step_ptr[l + 1] = M.level_step_ptr[l] + steps;
+  }
+  // row gathers of the solves
+  M.frow_off.resize(static_cast<size_t>(T.ns) + 1);
+  M.grow_ptr.resize(static_cast<size_t>(E.frow) + 1);
+  M.gent.resize(static_cast<size_t>(E.gent) + 2);
+  parallel_for(static_cast<int64_t>(S), join_threads(), [&](int64_t s) {
+    const FactorPlan& p = *P[s];
+    const Base& o = b[s];
+    for (int32_t j = 0; j < p.sn.ns; ++j) M.frow_off[o.sn + j] = o.frow + p.frow_off[j];
+    for (int32_t r = 0; r < p.frow_off[p.sn.ns]; ++r)
+      M.grow_ptr[o.frow + r] = static_cast<int32_t>(o.gent + p.grow_ptr[r]);
+    for (size_t t = 0; t + 2 < p.gent.size(); ++t) {
+      Task4 g = p.gent[t];
+      g.a += static_cast<int32_t>(o.rows);
+      if (g.c > 0) g.b += static_cast<int32_t>(o.up);
+      M.gent[o.gent + t] = g;
+    }
+  });
+  M.frow_off[T.ns] = E.frow;
+  M.grow_ptr[E.frow] = static_cast<int32_t>(E.gent);
+  M.gent[E.gent] = Task4{0, 0, 0, 0};
+  M.gent[E.gent + 1] = Task4{0, 0, 0, 0};
+  M.nsmall = static_cast<int64_t>(M.small_list.size());
+  M.nbig = static_cast<int64_t>(M.big_list.size());
+  build_dag_schedule(M);
+}
+
+void combine_patterns(const std::vector<const PatternEntry*>& parts, BatchSystem& B, FactorPlan& plan) {
   const int32_t S = static_cast<int32_t>(parts.size());
   if (S < 1) throw std::invalid_argument("a batch needs at least one system");
   B.nsys = S;
   B.col_base.assign(S + 1, 0);
   B.ent_base.assign(S + 1, 0);
-  int64_t ncsr = 0, nrows = 0, ns_tot = 0;
-  for (int32_t s = 0; s < S; ++s) {
-    const int64_t nb = static_cast<int64_t>(B.col_base[s]) + parts[s]->n;
-    if (nb >= (int64_t(1) << 31)) throw std::invalid_argument("batch exceeds 2^31 rows");
-    B.col_base[s + 1] = static_cast<int32_t>(nb);
-    B.ent_base[s + 1] = B.ent_base[s] + parts[s]->nnz;
-    ncsr += static_cast<int64_t>(parts[s]->csr_j.size());
-    nrows += static_cast<int64_t>(parts[s]->an.plan.sn.rows.size());
-    ns_tot += parts[s]->an.plan.sn.ns;
-  }
-  const int32_t n = B.col_base[S];
-  const int64_t nnz = B.ent_base[S];
-  if (nnz >= (int64_t(1) << 31) || ncsr >= (int64_t(1) << 31))
-    throw std::invalid_argument("batch exceeds 2^31 matrix entries");
-
-  Analysis& A = B.an;
-  A.perm.resize(n);
-  B.local_perm.resize(n);
-  B.sys_of_row.resize(n);
-  A.C.n = n;
-  A.C.p.assign(static_cast<size_t>(n) + 1, 0);
-  A.C.i.resize(nnz);
-  A.val_map.resize(nnz);
-  A.parent.resize(n);
-  A.Lnz.resize(n);
-  B.Ap.assign(static_cast<size_t>(n) + 1, 0);
-  B.Ai.resize(nnz);
-  B.csr_p.assign(static_cast<size_t>(n) + 1, 0);
-  B.csr_j.resize(ncsr);
-  B.csr_src.resize(ncsr);
-  B.ecol.resize(nnz);
-  B.diag_pos.resize(n);
-  A.nnz_l = 0;
-  A.flops = 0.0;
-  A.relaxed = true;
-
-  Supernodes& T = A.plan.sn;
-  T.n = n;
-  T.ns = static_cast<int32_t>(ns_tot);
-  T.first.reserve(ns_tot + 1);
-  T.nrow.reserve(ns_tot);
-  T.rows_off.reserve(ns_tot + 1);
-  T.rows.reserve(nrows);
-  T.sparent.reserve(ns_tot);
-  T.col2sn.resize(n);
-  std::vector<int64_t> val_dst_sys(nnz);  // per original entry: panel index within its system
-  std::vector<int64_t> panel_base_sn;     // first supernode of each system
-  int64_t csr_at = 0;
-  int32_t sn_base = 0;
+  std::vector<int64_t> csr_base(S + 1, 0), panel_base(S + 1, 0);
+  B.nnz_l = 0;
+  B.flops = 0.0;
+  B.relaxed = true;
   for (int32_t s = 0; s < S; ++s) {
     const PatternEntry& e = *parts[s];
+    const int64_t nb = static_cast<int64_t>(B.col_base[s]) + e.n;
+    if (nb >= (int64_t(1) << 31)) throw std::invalid_argument("batch exceeds 2^31 rows");
+    B.col_base[s + 1] = static_cast<int32_t>(nb);
+    B.ent_base[s + 1] = B.ent_base[s] + e.nnz;
+    csr_base[s + 1] = csr_base[s] + static_cast<int64_t>(e.csr_j.size());
+    panel_base[s + 1] = panel_base[s] + e.an.plan.panel_entries;
+    B.nnz_l += e.an.nnz_l;
+    B.flops += e.an.flops;
+    B.relaxed = B.relaxed && e.an.relaxed;
+  }
+  const int32_t n = B.col_base[S];
+  const int64_t nnz = B.ent_base[S], ncsr = csr_base[S];
+  if (nnz >= (int64_t(1) << 31) || ncsr >= (int64_t(1) << 31))
+    throw std::invalid_argument("batch exceeds 2^31 matrix entries");
+  std::vector<const FactorPlan*> plans;
+  for (int32_t s = 0; s < S; ++s) plans.push_back(&parts[s]->an.plan);
+  merge_plans(plans, B.col_base, plan);
+  B.perm.resize(n);
+  B.local_perm.resize(n);
+  B.sys_of_row.resize(n);
+  B.diag_pos.resize(n);
+  B.erow.resize(nnz);
+  B.ecol.resize(nnz);
+  B.csr_p.resize(static_cast<size_t>(n) + 1);
+  B.csr_j.resize(ncsr);
+  B.csr_src.resize(ncsr);
+  plan.val_dst.resize(nnz);
+  B.csr_p[0] = 0;
+  parallel_for(S, join_threads(), [&](int64_t s) {
+    const PatternEntry& e = *parts[s];
     const Analysis& a = e.an;
-    const Supernodes& P = a.plan.sn;
     const int32_t cb = B.col_base[s];
     const int64_t eb = B.ent_base[s];
+    const int64_t csr_at = csr_base[s];
     for (int32_t k = 0; k < e.n; ++k) {
-      A.perm[cb + k] = cb + a.perm[k];
+      B.perm[cb + k] = cb + a.perm[k];
       B.local_perm[cb + k] = a.perm[k];
-      B.sys_of_row[cb + k] = s;
-      A.parent[cb + k] = a.parent[k] >= 0 ? cb + a.parent[k] : -1;
-      A.Lnz[cb + k] = a.Lnz[k];
-      A.C.p[cb + k + 1] = static_cast<int32_t>(eb + a.C.p[k + 1]);
-      B.Ap[cb + k + 1] = static_cast<int32_t>(eb + e.Ap[k + 1]);
+      B.sys_of_row[cb + k] = static_cast<int32_t>(s);
       B.csr_p[cb + k + 1] = static_cast<int32_t>(csr_at + e.csr_p[k + 1]);
       B.diag_pos[cb + k] = static_cast<int32_t>(eb + e.diag_pos[k]);
-      T.col2sn[cb + k] = sn_base + P.col2sn[k];
     }
     for (int64_t q = 0; q < e.nnz; ++q) {
-      A.C.i[eb + q] = cb + a.C.i[q];
-      A.val_map[eb + q] = static_cast<int32_t>(eb + a.val_map[q]);
-      B.Ai[eb + q] = cb + e.Ai[q];
+      B.erow[eb + q] = cb + e.Ai[q];
       B.ecol[eb + q] = cb + e.ecol[q];
-      val_dst_sys[eb + q] = a.plan.val_dst[q];
+      plan.val_dst[eb + q] = panel_base[s] + a.plan.val_dst[q];
     }
     for (size_t q = 0; q < e.csr_j.size(); ++q) {
       B.csr_j[csr_at + q] = cb + e.csr_j[q];
       B.csr_src[csr_at + q] = static_cast<int32_t>(eb + e.csr_src[q]);
     }
-    csr_at += static_cast<int64_t>(e.csr_j.size());
-    const int64_t rb = static_cast<int64_t>(T.rows.size());
-    for (int32_t j = 0; j < P.ns; ++j) {
-      T.first.push_back(cb + P.first[j]);
-      T.nrow.push_back(P.nrow[j]);
-      T.rows_off.push_back(rb + P.rows_off[j]);
-      T.sparent.push_back(P.sparent[j] >= 0 ? sn_base + P.sparent[j] : -1);
-    }
-    for (int32_t r : P.rows) T.rows.push_back(cb + r);
-    panel_base_sn.push_back(sn_base);
-    sn_base += P.ns;
-    A.nnz_l += a.nnz_l;
-    A.flops += a.flops;
-    A.relaxed = A.relaxed && a.relaxed;
-  }
-  T.first.push_back(n);
-  T.rows_off.push_back(static_cast<int64_t>(T.rows.size()));
-  // levels, panels and relative indices of the forest; per system they
-  // equal the system's own
-  finish_supernodes(T);
-  FactorPlan plan = build_plan_from_supernodes(std::move(T));
-  plan.val_dst.resize(nnz);
-  for (int32_t s = 0; s < S; ++s) {
-    const int64_t pb = plan.sn.panel_off[panel_base_sn[s]];
-    for (int64_t q = B.ent_base[s]; q < B.ent_base[s + 1]; ++q) plan.val_dst[q] = pb + val_dst_sys[q];
-  }
-  A.plan = std::move(plan);
-  return B;
+  });
 }
 
 }  // namespace mpir

@@ -71,16 +71,30 @@ uint64_t pattern_key(int32_t n, const int32_t* Ap, const int32_t* Ai, int orderi
 // The block-diagonal system of a batch.
 struct BatchSystem {
   int32_t nsys = 0;
-  std::vector<int32_t> col_base;  // nsys+1: first row of each system
-  std::vector<int64_t> ent_base;  // nsys+1: first original entry
-  std::vector<int32_t> sys_of_row;
-  std::vector<int32_t> local_perm;  // combined new row -> old row within its system
-  Analysis an;                      // of the block-diagonal matrix (plan over the forest)
-  std::vector<int32_t> Ap, Ai;      // the block-diagonal original pattern
-  std::vector<int32_t> csr_p, csr_j, csr_src;
-  std::vector<int32_t> ecol, diag_pos;
+  std::vector<int32_t> col_base;    // nsys+1: first row of each system
+  std::vector<int64_t> ent_base;    // nsys+1: first original entry
+  std::vector<int32_t> sys_of_row;  // system of each joined row
+  std::vector<int32_t> local_perm;  // joined new row -> old row within its system
+  std::vector<int32_t> perm;        // joined new row -> joined old row
+  std::vector<int32_t> erow, ecol;  // row and column of every original entry (joined numbering)
+  std::vector<int32_t> diag_pos;    // original entry of every diagonal
+  std::vector<int32_t> csr_p, csr_j, csr_src;  // full CSR of P K P^T, source entries
+  int64_t nnz_l = 0;  // nonzeros of L (without relaxation zeros)
+  double flops = 0.0;
+  bool relaxed = true;
 };
 
-BatchSystem combine_patterns(const std::vector<const PatternEntry*>& parts);
+// Joins the systems into B and their plans into plan (the forest of their
+// elimination trees, see merge_plans); both keep their vectors' memory
+// from one batch to the next.
+void combine_patterns(const std::vector<const PatternEntry*>& parts, BatchSystem& B, FactorPlan& plan);
+
+// The plan of the forest of the systems' elimination trees from the
+// systems' own plans: supernodes, schedules and storage offsets shifted by
+// the systems before each one; per level and step the systems' tasks follow
+// each other (for the panel steps: every system's real rows, then every
+// system's identity rows). Each system's update matrices keep their own
+// region of the pool. No value map.
+void merge_plans(const std::vector<const FactorPlan*>& plans, const std::vector<int32_t>& col_base, FactorPlan& M);
 
 }  // namespace mpir

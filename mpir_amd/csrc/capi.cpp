@@ -587,11 +587,12 @@ int mpir_batch_plan_info(int32_t nsys, const int32_t* n, const int32_t* const* A
       throw Error(kErrInvalid, e.what());
     }
     for (const auto& e : own) raw.push_back(&e);
-    const BatchSystem B = combine_patterns(raw);
-    const FactorPlan& P = B.an.plan;
+    BatchSystem B;
+    FactorPlan P;
+    combine_patterns(raw, B, P);
     info[0] = B.col_base.back();
-    info[1] = B.an.nnz_l;
-    info[2] = static_cast<int64_t>(B.an.flops);
+    info[1] = B.nnz_l;
+    info[2] = static_cast<int64_t>(B.flops);
     info[3] = P.sn.ns;
     info[4] = P.sn.nlevels;
     info[5] = P.panel_entries;
@@ -673,6 +674,12 @@ int mpir_server_summary(mpir_server_t h, int64_t* counts, double* wall_ms) {
   return guarded([&] {
     MPIR_CHECK(h && h->s, "null server handle");
     const ServeSummary& m = h->s->summary();
+    if (wall_ms) {
+      wall_ms[1] = m.prepare_ms;
+      wall_ms[2] = m.plan_ms;
+      wall_ms[3] = m.factor_ms;
+      wall_ms[4] = m.solve_ms;
+    }
     if (counts) {
       counts[0] = m.completed;
       counts[1] = m.failed;

@@ -67,11 +67,11 @@ class DevBuf {
   }
   ~DevBuf() { release(); }
   // Contents are not preserved. An allocation that already holds `bytes`
-  // and is at most four times larger is kept: hipFree waits for the whole
-  // device, so handles that are re-analyzed for every batch of systems
-  // reuse their memory instead.
+  // is kept (buffers only grow): hipFree waits for the whole device, so
+  // handles that are re-analyzed for every batch of systems reuse their
+  // memory instead.
   void alloc(size_t bytes) {
-    if (p_ && bytes <= cap_ && bytes >= cap_ / 4) {
+    if (p_ && bytes <= cap_) {
       n_ = bytes;
       return;
     }

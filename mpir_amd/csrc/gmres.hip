@@ -362,8 +362,9 @@ void GmresLdl::upload_plan() {
   upload_vec(d_upd_off_, P.upd_off, st);
   upload_vec(d_child_ptr_, P.child_ptr, st);
   upload_vec(d_child_list_, P.child_list, st);
-  keep_.clear();
-  keep_.resize(8);
+  // the device buffers are kept across re-analyses (a new pattern reuses
+  // their memory: freeing would wait for the whole device)
+  if (keep_.size() < 40) keep_.resize(40);
   upload_vec(keep_[0], P.small_list, st);
   upload_vec(keep_[1], P.big_list, st);
   upload_vec(keep_[2], P.asm_task, st);
@@ -388,7 +389,6 @@ void GmresLdl::upload_plan() {
   dfp_.h_pan_aug_ptr = P.pan_aug_ptr;
   dfp_.n_inv_task = static_cast<int>(P.fx_task.size() / 3);
   dfp_.h_upd_aug_ptr = P.upd_aug_ptr;
-  keep_.resize(40);
   upload_vec(keep_[15], P.asm_child_ptr, st);
   upload_vec(keep_[16], P.asm_child, st);
   dfp_.asm_child_ptr = keep_[15].as<int32_t>();
@@ -968,6 +968,7 @@ void GmresLdl::luir(const double* b, double* x, int maxiter, LuirResult* out) {
 }
 
 void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void* Dinv, int prec) {
+  if (batch_) throw Error(kErrState, "export_factor works on single-system handles");
   if (!factored_ && stats_.positive_pivots < 0) throw Error(kErrState, "no factor to export");
   const Supernodes& S = plan_->sn;
   const size_t tf = uf_ == kF64 ? 8 : 4;

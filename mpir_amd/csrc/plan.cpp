@@ -60,6 +60,8 @@ class PoolAlloc {
   int64_t top_ = 0;
 };
 
+}  // namespace
+
 // Task lists of the dependency-driven solves from the per-level lists. A
 // forward task of a front waits for its children (small fronts, fx) or for
 // all its y values (fu); a backward task waits for its parent (small
@@ -107,8 +109,6 @@ void build_dag_schedule(FactorPlan& P) {
     grouped(P.dag_bwd, P.small_ptr[l], P.small_ptr[l + 1]);
   }
 }
-
-}  // namespace
 
 FactorPlan build_plan_from_supernodes(Supernodes sn) {
   FactorPlan P;

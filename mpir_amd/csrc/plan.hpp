@@ -113,6 +113,8 @@ struct FactorPlan {
 
 // Schedule and storage layout of a supernodal partition (no value map).
 FactorPlan build_plan_from_supernodes(Supernodes sn);
+// dag_fwd, dag_bwd and dag_meta from the per-level task lists.
+void build_dag_schedule(FactorPlan& P);
 
 // perm: new -> old. C: permuted upper CSC with val_map (C entry -> original
 // entry). parent/Lnz: etree of C.

@@ -255,6 +255,7 @@ void Server::lane_main(int lane, std::vector<ServeRecord>* reqs) {
 void Server::process(int lane, std::vector<ServeRecord>* reqs, const std::vector<int64_t>& ids, int32_t batch_id) {
   GmresLdl& s = *solvers_[lane];
   const size_t B = ids.size();
+  const double t_prep = now_ms();
   std::vector<std::vector<double>> ax(B), bv(B), xv(B);
   std::vector<std::shared_ptr<const PatternEntry>> parts(B);
   std::vector<int32_t> nrhs(B);
@@ -279,16 +280,28 @@ void Server::process(int lane, std::vector<ServeRecord>* reqs, const std::vector
     xp[k] = xv[k].data();
   }
   int status = 0;
+  const double t1 = now_ms();
+  double t2 = t1, t3 = t1;
   try {
     s.analyze_batch(parts);
+    t2 = now_ms();
     s.factor_batch(axp.data(), false);
+    t3 = now_ms();
     s.solve_batch(nrhs.data(), bp.data(), xp.data(), false);
   } catch (const Error& e) {
     status = e.code;
   } catch (const std::exception&) {
     status = kErrInternal;
   }
-  const double td = now_ms() - t0_;
+  const double t4 = now_ms();
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    sum_.prepare_ms += t1 - t_prep;
+    sum_.plan_ms += t2 - t1;
+    sum_.factor_ms += t3 - t2;
+    sum_.solve_ms += t4 - t3;
+  }
+  const double td = t4 - t0_;
   const BatchStats& st = s.batch_stats();
   for (size_t k = 0; k < B; ++k) {
     ServeRecord& r = (*reqs)[ids[k]];

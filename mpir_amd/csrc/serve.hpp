@@ -74,6 +74,9 @@ struct ServeSummary {
   int64_t max_concurrent = 0;       // batches on the device at once
   double wall_ms = 0.0;
   std::vector<double> lane_busy_ms;
+  // time of every batch phase, summed over batches: request data, joined
+  // symbolic plan and its upload, numeric factorization, GMRES-IR
+  double prepare_ms = 0.0, plan_ms = 0.0, factor_ms = 0.0, solve_ms = 0.0;
 };
 
 class GmresLdl;
