@@ -191,6 +191,10 @@ int mpir_gmres_ldl_export_factor(mpir_gmres_ldl_t h, int64_t* Lp, int32_t* Li, d
  * scaled system), for parity tests and benchmarks */
 int mpir_gmres_ldl_precond_dev(mpir_gmres_ldl_t h, double* x);         /* x := M^{-1} x */
 int mpir_gmres_ldl_spmv_dev(mpir_gmres_ldl_t h, const double* x, double* y); /* y = K_s x */
+/* x := M^{-1} x for nrhs vectors of n entries stored one after the other;
+ * an FP32 factor is read once for up to 8 of them. Same results, bit for
+ * bit, as nrhs calls of mpir_gmres_ldl_precond_dev. */
+int mpir_gmres_ldl_precond_multi_dev(mpir_gmres_ldl_t h, double* x, int32_t nrhs);
 /* average ms of `reps` runs measured with HIP events on the handle's stream:
  * what = 0 SpMV, 1 M^{-1} application, 2 numeric factorization */
 int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);

@@ -77,6 +77,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_export_factor": (C.c_int, [P, P, P, P, P, P]),
     "mpir_gmres_ldl_precond_dev": (C.c_int, [P, P]),
     "mpir_gmres_ldl_spmv_dev": (C.c_int, [P, P, P]),
+    "mpir_gmres_ldl_precond_multi_dev": (C.c_int, [P, P, i32]),
     "mpir_gmres_ldl_stream": (P, [P]),
     "mpir_gmres_ldl_time": (C.c_int, [P, C.c_int, C.c_int, P]),
     "mpir_spmv_upper_f64": (C.c_int, [i32, P, P, P, P, P]),

@@ -405,8 +405,21 @@ void GmresLdl::batch_collect_pivots(bool zero_pivot) {
     throw Error(kErrNumeric, "zero pivot in the LDL^T factorization of system " + std::to_string(first_bad));
 }
 
+// M^{-1} applied to R slots of n entries: each tile of the factor is read
+// once for up to 8 right-hand sides
 void GmresLdl::precond_apply_slots(double* x, int R) {
-  for (int r = 0; r < R; ++r) precond_apply_dev(x + static_cast<int64_t>(r) * n_);
+  set_solve_slots(std::min(R, 8));
+  stats_.precond_applications += R;
+  double* u = d_uvec_.as<double>();
+  if (uf_ == kF64)
+    ldl_solve_multi<double, double, double>(solve_args<double, double>(d_panel_.get()), dsp_, x, R, u, stream_);
+  else if (uf_ == kF32)
+    ldl_solve_multi<float, float, double>(solve_args<float, float>(d_panel_.get()), dsp_, x, R, u, stream_);
+  else if (uf_ == kF16)
+    ldl_solve_multi<__half, float, double>(solve_args<__half, float>(d_panel16_.get()), dsp_, x, R, u, stream_);
+  else
+    ldl_solve_multi<__hip_bfloat16, float, double>(solve_args<__hip_bfloat16, float>(d_panel16_.get()), dsp_, x, R, u,
+                                                   stream_);
   bstats_.precond_sweeps++;
 }
 

@@ -761,6 +761,16 @@ int mpir_gmres_ldl_precond_dev(mpir_gmres_ldl_t h, double* x) {
     return 0;
   });
 }
+int mpir_gmres_ldl_precond_multi_dev(mpir_gmres_ldl_t h, double* x, int32_t nrhs) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(h->s->factored(), "no factor");
+    MPIR_CHECK(x && nrhs >= 1, "bad right-hand sides");
+    h->s->precond_apply_slots(x, nrhs);
+    h->s->sync_check();
+    return 0;
+  });
+}
 int mpir_gmres_ldl_spmv_dev(mpir_gmres_ldl_t h, const double* x, double* y) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;

@@ -429,14 +429,8 @@ void GmresLdl::upload_plan() {
   g.rpart_off = static_cast<const int64_t*>(up(P.rpart_off));
   g.xpart_off = static_cast<const int64_t*>(up(P.xpart_off));
   MPIR_CHECK(k <= 40, "plan upload slots");
-  // work areas in the widest solve precision
-  const int64_t nwork = P.nypart + P.nupart + P.nrpart + P.nxpart + 64;
-  d_scratch_.alloc(static_cast<size_t>(nwork) * sizeof(double));
-  double* wk = d_scratch_.as<double>();
-  g.ypart = wk;
-  g.upart = wk + P.nypart;
-  g.rpart = wk + P.nypart + P.nupart;
-  g.xpart = wk + P.nypart + P.nupart + P.nrpart;
+  solve_slots_ = 0;
+  set_solve_slots(1);
   // counters: per front, per column block, then the dispensers on their own lines
   g.cnt_bx = kNumSolveCounters * S.ns;
   g.cnt_disp = (g.cnt_bx + P.ncb_total + kDispenserStride - 1) / kDispenserStride * kDispenserStride;
@@ -446,7 +440,31 @@ void GmresLdl::upload_plan() {
   d_serr_.alloc(sizeof(int));
   MPIR_HIP(hipMemsetAsync(d_serr_.get(), 0, sizeof(int), st));
   g.err = d_serr_.as<int>();
-  d_uvec_.alloc(std::max<int64_t>(1, static_cast<int64_t>(S.rows.size())) * sizeof(double));
+}
+
+// Work areas of the triangular solves for `slots` right-hand sides at once
+// (in the widest solve precision): the update vectors and the partial sums
+// of every right-hand side, at fixed strides.
+void GmresLdl::set_solve_slots(int slots) {
+  if (slots <= solve_slots_) return;
+  const FactorPlan& P = *plan_;
+  SolveDev& g = dsp_.d;
+  const int64_t nwork = slots * (P.nypart + P.nupart + P.nrpart + P.nxpart) + 64;
+  d_scratch_.alloc(static_cast<size_t>(nwork) * sizeof(double));
+  double* wk = d_scratch_.as<double>();
+  g.ypart = wk;
+  g.upart = wk + slots * P.nypart;
+  g.rpart = wk + slots * (P.nypart + P.nupart);
+  g.xpart = wk + slots * (P.nypart + P.nupart + P.nrpart);
+  g.syp = P.nypart;
+  g.sup = P.nupart;
+  g.srp = P.nrpart;
+  g.sxp = P.nxpart;
+  const int64_t rows = std::max<int64_t>(1, static_cast<int64_t>(P.sn.rows.size()));
+  d_uvec_.alloc(static_cast<size_t>(slots * rows) * sizeof(double));
+  g.su = rows;
+  g.sx = n_;
+  solve_slots_ = slots;
 }
 
 void GmresLdl::alloc_numeric() {

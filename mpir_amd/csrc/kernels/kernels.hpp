@@ -132,6 +132,9 @@ struct SolveDev {
   int* counters = nullptr;
   int cnt_bx = 0, cnt_disp = 0, ncounters = 0;
   int* err = nullptr;  // bit 1: a dependency wait expired
+  // entries between right-hand side r and r + 1: in x, in uvec, and in the
+  // partial-sum areas (ldl_solve_multi)
+  int64_t sx = 0, su = 0, syp = 0, sup = 0, srp = 0, sxp = 0;
 };
 
 struct DeviceSolvePlan {
@@ -145,6 +148,12 @@ struct DeviceSolvePlan {
 // uvec: scratch of sum(nrow) entries of TX.
 template <class TF, class TD, class TX>
 void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st);
+// nrhs right-hand sides x + r dp.d.sx at once (each tile of the factor read
+// once for up to 8 of them); uvec and the partial-sum areas of dp.d hold 8
+// right-hand sides at strides dp.d.su, .syp, .sup, .srp, .sxp.
+template <class TF, class TD, class TX>
+void ldl_solve_multi(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, int nrhs, TX* uvec,
+                     hipStream_t st);
 
 // ---------------------------------------------------------------- vectors / SpMV
 struct CsrDev {

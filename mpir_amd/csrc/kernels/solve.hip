@@ -38,6 +38,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "../common.hpp"
 #include "kernels.hpp"
@@ -53,7 +54,6 @@ constexpr int kSpinLimit = 1 << 20;  // polls (about a second) before a wait giv
 // factors, kGemvCols entries for the row products), then the wave
 // reduction of the row products (4 x 64)
 constexpr int kVsLen = 512;
-constexpr int kSmem = kVsLen + 4 * kRB;
 static_assert(kXcRows <= kVsLen && kGemvCols <= kVsLen, "staging buffer");
 
 // Per-front completion counters (counters[k * ns + s]).
@@ -150,9 +150,9 @@ struct RowGather {
     e0 = d.gent[p0];
     e1 = d.gent[p0 + 1];
   }
+  // uvec, up: the update vectors and L_21 y partial sums of one right-hand side
   template <class TX>
-  __device__ __forceinline__ TX value(const SolveDev& d, const TX* uvec, TX v) const {
-    const TX* up = static_cast<const TX*>(d.upart);
+  __device__ __forceinline__ TX value(const SolveDev& d, const TX* uvec, const TX* up, TX v) const {
     for (int p = p0; p < p1; ++p) {
       const Task4 e = p == p0 ? e0 : p == p0 + 1 ? e1 : d.gent[p];
       v += ld_dev(uvec + e.a) - sum_strided(up + e.b, e.c, kSolveRB);
@@ -230,7 +230,7 @@ __device__ __forceinline__ void wave_publish(int* c) {
 // task writes is read.
 constexpr int kLC = 16;
 
-template <class TF, class TD, class TX, class Wait>
+template <int NR, class TF, class TD, class TX, class Wait>
 __device__ __forceinline__ void small_fwd(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, TX* x, TX* uvec,
                                           Wait wait) {
   const int lane = threadIdx.x & 63;
@@ -249,27 +249,40 @@ __device__ __forceinline__ void small_fwd(const SolveArgs<TF, TD>& a, const Solv
   load(0);
   RowGather G;
   if (lane < m) G.pre(d, d.frow_off[s] + lane);
-  const TX b = x[f + min(lane, nc - 1)];
+  TX b[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) b[r] = x[r * d.sx + f + min(lane, nc - 1)];
   const TD dinv = a.Dinv[f + min(lane, nc - 1)];
   wait();
-  TX vl = G.value(d, uvec, lane < nc ? b : TX(0));
+  const TX* up = static_cast<const TX*>(d.upart);
+  TX vl[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) vl[r] = G.value(d, uvec + r * d.su, up + r * d.sup, lane < nc ? b[r] : TX(0));
   for (int j0 = 0; j0 < nc; j0 += kLC) {
     if (j0 > 0) load(j0);
 #pragma unroll
     for (int u = 0; u < kLC; ++u) {
       const int j = j0 + u;
       if (j < nc) {
-        const TX yj = lane_value(vl, j);
-        if (lane > j) vl -= static_cast<TX>(lr[u]) * yj;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const TX yj = lane_value(vl[r], j);
+          if (lane > j) vl[r] -= static_cast<TX>(lr[u]) * yj;
+        }
       }
     }
   }
-  if (lane < nc) x[f + lane] = vl * static_cast<TX>(dinv);
-  else if (lane < m) st_dev(uvec + a.rows_off[s] + (lane - nc), vl);
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (lane < nc) x[r * d.sx + f + lane] = vl[r] * static_cast<TX>(dinv);
+    else if (lane < m) st_dev(uvec + r * d.su + a.rows_off[s] + (lane - nc), vl[r]);
+  }
 }
 
-template <class TF, class TD, class TX, class Wait>
-__device__ __forceinline__ void small_bwd(const SolveArgs<TF, TD>& a, int s, TX* x, TX* w, Wait wait) {
+// w: NR x 64 entries of LDS
+template <int NR, class TF, class TD, class TX, class Wait>
+__device__ __forceinline__ void small_bwd(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, TX* x, TX* w,
+                                          Wait wait) {
   const int lane = threadIdx.x & 63;
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
@@ -290,18 +303,24 @@ __device__ __forceinline__ void small_bwd(const SolveArgs<TF, TD>& a, int s, TX*
   };
   load_up(nc);
   const int32_t row = nr > 0 ? a.rows[a.rows_off[s] + min(max(lane - nc, 0), nr - 1)] : 0;
-  const TX own = x[f + cl];
+  TX z[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) z[r] = x[r * d.sx + f + cl];
   wait();
-  if (lane < nc) w[lane] = own;
-  else if (lane < m) w[lane] = ld_dev(x + row);
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (lane < nc) w[r * kRB + lane] = z[r];
+    else if (lane < m) w[r * kRB + lane] = ld_dev(x + r * d.sx + row);
+  }
   wave_sync();
   // lane j < nc: z_j = w_j - sum over rows r >= nc of L[r, j] w_r, r ascending
-  TX z = own;
   for (int r0 = nc; r0 < m; r0 += kLC) {
     if (r0 > nc) load_up(r0);
 #pragma unroll
     for (int u = 0; u < kLC; ++u)
-      if (r0 + u < m) z -= static_cast<TX>(lc[u]) * w[r0 + u];
+      if (r0 + u < m)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) z[r] -= static_cast<TX>(lc[u]) * w[r * kRB + r0 + u];
   }
   // unit upper solve with L_11^T, bottom to top
   for (int t1 = nc; t1 > 0; t1 -= kLC) {
@@ -310,12 +329,17 @@ __device__ __forceinline__ void small_bwd(const SolveArgs<TF, TD>& a, int s, TX*
     for (int u = 0; u < kLC; ++u) {
       const int t = t1 - 1 - u;
       if (t >= 0) {
-        const TX xt = lane_value(z, t);
-        if (lane < t) z -= static_cast<TX>(lc[u]) * xt;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const TX xt = lane_value(z[r], t);
+          if (lane < t) z[r] -= static_cast<TX>(lc[u]) * xt;
+        }
       }
     }
   }
-  if (lane < nc) st_dev(x + f + lane, z);
+  if (lane < nc)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) st_dev(x + r * d.sx + f + lane, z[r]);
 }
 
 // ---------------------------------------------------------------- big-front tiles
@@ -453,6 +477,7 @@ __device__ __forceinline__ TX row_gemv(RowTile<TF>& T, const TX* ys, TX (*red)[k
   for (int off = RT::LPC; off < 64; off <<= 1)
 #pragma unroll
     for (int e = 0; e < RT::N; ++e) acc[e] += __shfl_xor(acc[e], off);
+  __syncthreads();  // an earlier call's readers of red are done
   if (lane < RT::LPC)
 #pragma unroll
     for (int e = 0; e < RT::N; ++e) red[w][RT::N * lane + e] = acc[e];
@@ -462,6 +487,18 @@ __device__ __forceinline__ TX row_gemv(RowTile<TF>& T, const TX* ys, TX (*red)[k
 }
 
 // ---------------------------------------------------------------- big-front tasks
+// f(r) for the right-hand sides r < NR (one right-hand side: no loop, so the
+// single case compiles exactly as written for it)
+template <int NR, class F>
+__device__ __forceinline__ void each_rhs(F&& f) {
+  if constexpr (NR == 1) {
+    f(0);
+  } else {
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) f(r);
+  }
+}
+
 struct FrontShape {
   int f, nc, nr, m;
   int64_t ld;
@@ -476,11 +513,13 @@ struct FrontShape {
 };
 
 // Forward, rows [h kXcRows, ...) of column block j: gathers v on those rows
-// (b plus the children's contributions) and leaves partial sums of W^T v.
-template <class TF, class TD, class TX, class Wait>
+// (b plus the children's contributions) and leaves partial sums of W^T v,
+// for NR right-hand sides (the tile is read once).
+template <int NR, class TF, class TD, class TX, class Wait>
 __device__ __forceinline__ void fx_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int j, int h, const TX* x,
                                         const TX* uvec, TX* smem, Wait wait) {
   static_assert(kXcRows == 256, "one row per thread");
+  static_assert(NR == 1 || ColTile<TF>::NB == 1, "several right-hand sides need the tile in registers");
   const FrontShape F(a, s);
   const int tid = threadIdx.x;
   const int c0 = j * kRB, bw = min(kRB, F.nc - c0);
@@ -492,28 +531,37 @@ __device__ __forceinline__ void fx_task(const SolveArgs<TF, TD>& a, const SolveD
   T.len = len;
   T.load(0);
   RowGather G;
-  TX bq = TX(0);
+  TX bq[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) bq[r] = TX(0);
   if (tid < len) {
     G.pre(d, d.frow_off[s] + q0 + tid);
-    bq = x[F.f + q0 + tid];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) bq[r] = x[r * d.sx + F.f + q0 + tid];
   }
   wait();
-  TX* vs = smem;
-  vs[tid] = tid < len ? G.value(d, uvec, bq) : TX(0);
-  for (int q = tid + blockDim.x; q < kVsLen; q += blockDim.x) vs[q] = TX(0);
+  const TX* up = static_cast<const TX*>(d.upart);
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    TX* vs = smem + r * kVsLen;
+    vs[tid] = tid < len ? G.value(d, uvec + r * d.su, up + r * d.sup, bq[r]) : TX(0);
+    for (int q = tid + blockDim.x; q < kVsLen; q += blockDim.x) vs[q] = TX(0);
+  }
   __syncthreads();
   const int nhx = (F.nc + kXcRows - 1) / kXcRows;
   TX* yp = static_cast<TX*>(d.ypart) + d.ypart_off[s] + (static_cast<int64_t>(j) * nhx + h) * kRB;
-  col_dots<TF, TX>(T, vs, yp);
+  each_rhs<NR>([&](int r) { col_dots<TF, TX>(T, smem + r * kVsLen, yp + r * d.syp); });
   block_publish(d.counters + kXDone * d.ns + s);
 }
 
 // Forward, R block b, columns [h kGemvCols, ...): partial sums of L_21 y;
-// the chunk-0 task also gathers and stores v_R. The front's last task
-// counts the front as a finished child of its parent.
-template <class TF, class TD, class TX, class Wait>
+// the chunk-0 task also gathers and stores v_R, so that u = v_R - (partial
+// sums) is formed by whoever gathers it. The front's last task counts the
+// front as a finished child of its parent. NR right-hand sides.
+template <int NR, class TF, class TD, class TX, class Wait>
 __device__ __forceinline__ void fu_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int b, int h, TX* uvec,
                                         TX* smem, Wait wait) {
+  static_assert(NR == 1 || RowTile<TF>::NB == 1, "several right-hand sides need the tile in registers");
   const FrontShape F(a, s);
   const int tid = threadIdx.x;
   const int rb0 = b * kRB, rows = min(kRB, F.nr - rb0);
@@ -529,25 +577,37 @@ __device__ __forceinline__ void fu_task(const SolveArgs<TF, TD>& a, const SolveD
   if (gathers) G.pre(d, d.frow_off[s] + F.nc + rb0 + tid);
   wait();
   const TX* yp = static_cast<const TX*>(d.ypart) + d.ypart_off[s];
-  TX* ys = smem;
-  for (int c = tid; c < kGemvCols; c += blockDim.x) ys[c] = c < cc ? forward_value(yp, F.nc, cs + c) : TX(0);
-  const TX vr = gathers ? G.value(d, static_cast<const TX*>(uvec), TX(0)) : TX(0);
+  TX* upart = static_cast<TX*>(d.upart);
+  TX vr[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    TX* ys = smem + r * kVsLen;
+    for (int c = tid; c < kGemvCols; c += blockDim.x)
+      ys[c] = c < cc ? forward_value(yp + r * d.syp, F.nc, cs + c) : TX(0);
+    vr[r] = gathers ? G.value(d, static_cast<const TX*>(uvec) + r * d.su, upart + r * d.sup, TX(0)) : TX(0);
+  }
   __syncthreads();
-  const TX sum = row_gemv<TF, TX>(T, ys, reinterpret_cast<TX(*)[kRB]>(smem + kVsLen));
   const int nh = (F.nc + kGemvCols - 1) / kGemvCols;
-  TX* up = static_cast<TX*>(d.upart) + d.upart_off[s] + static_cast<int64_t>(b) * nh * kRB;
-  if (tid < rows) st_dev(up + h * kRB + tid, sum);
-  if (gathers) st_dev(uvec + a.rows_off[s] + rb0 + tid, vr);
+  TX* out = upart + d.upart_off[s] + static_cast<int64_t>(b) * nh * kRB + h * kRB;
+  TX (*red)[kRB] = reinterpret_cast<TX(*)[kRB]>(smem + NR * kVsLen);
+  each_rhs<NR>([&](int r) {
+    const TX sum = row_gemv<TF, TX>(T, smem + r * kVsLen, red);
+    if (tid < rows) st_dev(out + r * d.sup + tid, sum);
+  });
+  if (gathers)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) st_dev(uvec + r * d.su + a.rows_off[s] + rb0 + tid, vr[r]);
   const int prev = block_publish(d.counters + kUDone * d.ns + s);
   const int* ms = d.meta + kMetaStride * s;
   if (prev == ms[kMetaNFu] - 1 && ms[kMetaParent] >= 0) add_dev(d.counters + kCDone * d.ns + ms[kMetaParent], 1);
 }
 
 // Backward, R rows [h kXcRows, ...) of column block j: partial sums of
-// L_21^T x_R.
-template <class TF, class TD, class TX, class Wait>
+// L_21^T x_R for NR right-hand sides.
+template <int NR, class TF, class TD, class TX, class Wait>
 __device__ __forceinline__ void br_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int j, int h,
                                         const TX* x, TX* smem, Wait wait) {
+  static_assert(NR == 1 || ColTile<TF>::NB == 1, "several right-hand sides need the tile in registers");
   const FrontShape F(a, s);
   const int tid = threadIdx.x;
   const int c0 = j * kRB, bw = min(kRB, F.nc - c0);
@@ -560,19 +620,25 @@ __device__ __forceinline__ void br_task(const SolveArgs<TF, TD>& a, const SolveD
   T.load(0);
   wait();
   const int32_t* R = a.rows + a.rows_off[s] + q0;
-  TX* vs = smem;
-  for (int q = tid; q < kVsLen; q += blockDim.x) vs[q] = q < len ? ld_dev(x + R[q]) : TX(0);
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    TX* vs = smem + r * kVsLen;
+    for (int q = tid; q < kVsLen; q += blockDim.x) vs[q] = q < len ? ld_dev(x + r * d.sx + R[q]) : TX(0);
+  }
   __syncthreads();
   const int nhr = (F.nr + kXcRows - 1) / kXcRows;
-  col_dots<TF, TX>(T, vs, static_cast<TX*>(d.rpart) + d.rpart_off[s] + (static_cast<int64_t>(j) * nhr + h) * kRB);
+  TX* rp = static_cast<TX*>(d.rpart) + d.rpart_off[s] + (static_cast<int64_t>(j) * nhr + h) * kRB;
+  each_rhs<NR>([&](int r) { col_dots<TF, TX>(T, smem + r * kVsLen, rp + r * d.srp); });
   block_publish(d.counters + kRDone * d.ns + s);
 }
 
 // Backward, row block i, columns [max(h kGemvCols, 64 i), ...): partial sums
 // of W z with z = D^{-1} y - L_21^T x_R; the block's last task stores x.
-template <class TF, class TD, class TX, class Wait>
+// NR right-hand sides.
+template <int NR, class TF, class TD, class TX, class Wait>
 __device__ __forceinline__ void bx_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int i, int h, TX* x,
                                         TX* smem, int* flag, Wait wait) {
+  static_assert(NR == 1 || RowTile<TF>::NB == 1, "several right-hand sides need the tile in registers");
   const FrontShape F(a, s);
   const int tid = threadIdx.x;
   const int r0 = i * kRB, rows = min(kRB, F.nc - r0);
@@ -587,52 +653,63 @@ __device__ __forceinline__ void bx_task(const SolveArgs<TF, TD>& a, const SolveD
   const int nhr = (F.nr + kXcRows - 1) / kXcRows;
   const TX* rp = static_cast<const TX*>(d.rpart) + (nhr > 0 ? d.rpart_off[s] : 0);
   const TX* yp = static_cast<const TX*>(d.ypart) + d.ypart_off[s];
-  TX* ys = smem;
-  for (int c = tid; c < kGemvCols; c += blockDim.x) {
-    TX z = TX(0);
-    if (c < cc) {
-      const int col = cs + c;
-      const TX rs = sum_strided(rp + static_cast<int64_t>(col / kRB) * nhr * kRB + (col % kRB), nhr, kRB);
-      z = static_cast<TX>(a.Dinv[F.f + col]) * forward_value(yp, F.nc, col) - rs;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    TX* ys = smem + r * kVsLen;
+    for (int c = tid; c < kGemvCols; c += blockDim.x) {
+      TX z = TX(0);
+      if (c < cc) {
+        const int col = cs + c;
+        const TX rs =
+            sum_strided(rp + r * d.srp + static_cast<int64_t>(col / kRB) * nhr * kRB + (col % kRB), nhr, kRB);
+        z = static_cast<TX>(a.Dinv[F.f + col]) * forward_value(yp + r * d.syp, F.nc, col) - rs;
+      }
+      ys[c] = z;
     }
-    ys[c] = z;
   }
   __syncthreads();
-  const TX sum = row_gemv<TF, TX>(T, ys, reinterpret_cast<TX(*)[kRB]>(smem + kVsLen));
   const int nh = (F.nc + kGemvCols - 1) / kGemvCols;
   const int h0 = r0 / kGemvCols;
   TX* xp = static_cast<TX*>(d.xpart) + d.xpart_off[s] + static_cast<int64_t>(i) * nh * kRB;
-  if (tid < rows) st_dev(xp + h * kRB + tid, sum);
+  TX (*red)[kRB] = reinterpret_cast<TX(*)[kRB]>(smem + NR * kVsLen);
+  each_rhs<NR>([&](int r) {
+    const TX sum = row_gemv<TF, TX>(T, smem + r * kVsLen, red);
+    if (tid < rows) st_dev(xp + r * d.sxp + h * kRB + tid, sum);
+  });
   if (publish_last(d.counters + d.cnt_bx + d.cb_base[s] + i, nh - h0, flag)) {
-    if (tid < rows) st_dev(x + F.f + r0 + tid, sum_strided(xp + h0 * kRB + tid, nh - h0, kRB));
+    if (tid < rows)
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        st_dev(x + r * d.sx + F.f + r0 + tid, sum_strided(xp + r * d.sxp + h0 * kRB + tid, nh - h0, kRB));
     block_publish(d.counters + kBDone * d.ns + s);
   }
 }
 
 // ---------------------------------------------------------------- level schedule
-template <class TF, class TD, class TX>
+template <int NR, class TF, class TD, class TX>
 __global__ void __launch_bounds__(64) k_fwd_small(SolveArgs<TF, TD> a, SolveDev d, const int32_t* __restrict__ list,
                                                   TX* x, TX* uvec) {
-  small_fwd(a, d, list[blockIdx.x], x, uvec, NoWait{});
+  small_fwd<NR>(a, d, list[blockIdx.x], x, uvec, NoWait{});
 }
 
-template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF, TD> a, const int32_t* __restrict__ list, TX* x) {
-  __shared__ TX w[kSmallFront];
-  small_bwd(a, list[blockIdx.x], x, w, NoWait{});
+template <int NR, class TF, class TD, class TX>
+__global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF, TD> a, SolveDev d, const int32_t* __restrict__ list,
+                                                  TX* x) {
+  __shared__ TX w[NR * kSmallFront];
+  small_bwd<NR>(a, d, list[blockIdx.x], x, w, NoWait{});
 }
 
 // kind: kDagFx, kDagFu, kDagBr or kDagBx; tasks: the level's triples
-template <class TF, class TD, class TX>
+template <int NR, class TF, class TD, class TX>
 __global__ void __launch_bounds__(256) k_level_tiles(SolveArgs<TF, TD> a, SolveDev d, int kind,
                                                      const int32_t* __restrict__ tasks, TX* x, TX* uvec) {
-  __shared__ TX smem[kSmem];
+  __shared__ TX smem[NR * kVsLen + 4 * kRB];
   __shared__ int flag;
   const int* q = tasks + 3 * blockIdx.x;
-  if (kind == kDagFx) fx_task(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), uvec, smem, NoWait{});
-  else if (kind == kDagFu) fu_task(a, d, q[0], q[1], q[2], uvec, smem, NoWait{});
-  else if (kind == kDagBr) br_task(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), smem, NoWait{});
-  else bx_task(a, d, q[0], q[1], q[2], x, smem, &flag, NoWait{});
+  if (kind == kDagFx) fx_task<NR>(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), uvec, smem, NoWait{});
+  else if (kind == kDagFu) fu_task<NR>(a, d, q[0], q[1], q[2], uvec, smem, NoWait{});
+  else if (kind == kDagBr) br_task<NR>(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), smem, NoWait{});
+  else bx_task<NR>(a, d, q[0], q[1], q[2], x, smem, &flag, NoWait{});
 }
 
 // ---------------------------------------------------------------- dag schedule
@@ -649,10 +726,12 @@ __device__ __forceinline__ int next_task(int* dispensers, int* tsh) {
   return *tsh;
 }
 
-template <class TF, class TD, class TX>
-// four waves per SIMD (at most 128 registers): 1024 resident workgroups
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_fwd_dag(SolveArgs<TF, TD> a, SolveDev d, TX* x, TX* uvec) {
-  __shared__ TX smem[kSmem];
+// One right-hand side: four waves per SIMD (at most 128 registers), 1024
+// resident workgroups; several: two waves per SIMD (256 registers), so the
+// factor tile and the partial sums of one right-hand side stay in registers.
+template <int NR, class TF, class TD, class TX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 1 ? 4 : 2))) k_fwd_dag(SolveArgs<TF, TD> a, SolveDev d, TX* x, TX* uvec) {
+  __shared__ TX smem[NR * kVsLen + 4 * kRB];
   __shared__ int tsh, flag;
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
@@ -665,23 +744,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       if (w < tk.c) {
         const int s = d.small_list[tk.b + w];
         const int* ms = d.meta + kMetaStride * s;
-        small_fwd(a, d, s, x, uvec, WaveWait{cnt + kCDone * ns + s, ms[kMetaNChild], d.err});
+        small_fwd<NR>(a, d, s, x, uvec, WaveWait{cnt + kCDone * ns + s, ms[kMetaNChild], d.err});
         if (ms[kMetaParent] >= 0) wave_publish(cnt + kCDone * ns + ms[kMetaParent]);
       }
     } else if (tk.a == kDagFx) {
       const int s = tk.b;
-      fx_task(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), uvec, smem,
-              BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err});
+      fx_task<NR>(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), uvec, smem,
+                  BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err});
     } else {  // kDagFu
       const int s = tk.b;
-      fu_task(a, d, s, tk.c, tk.d, uvec, smem, BlockWait{cnt + kXDone * ns + s, d.meta[kMetaStride * s + kMetaNFx], d.err});
+      fu_task<NR>(a, d, s, tk.c, tk.d, uvec, smem,
+                  BlockWait{cnt + kXDone * ns + s, d.meta[kMetaStride * s + kMetaNFx], d.err});
     }
   }
 }
 
-template <class TF, class TD, class TX>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_bwd_dag(SolveArgs<TF, TD> a, SolveDev d, TX* x) {
-  __shared__ TX smem[kSmem];
+template <int NR, class TF, class TD, class TX>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 1 ? 4 : 2))) k_bwd_dag(SolveArgs<TF, TD> a, SolveDev d, TX* x) {
+  __shared__ TX smem[NR * kVsLen + 4 * kRB];
   __shared__ int tsh, flag;
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
@@ -697,7 +777,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         // a root waits for nothing: its target is 0
         const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
         const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
-        small_bwd(a, s, x, smem + kRB * w, WaveWait{pc, target, d.err});
+        small_bwd<NR>(a, d, s, x, smem + kRB * NR * w, WaveWait{pc, target, d.err});
         wave_publish(cnt + kBDone * ns + s);
       }
     } else if (tk.a == kDagBr) {
@@ -705,11 +785,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       const int p = d.meta[kMetaStride * s + kMetaParent];
       const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
       const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
-      br_task(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), smem, BlockWait{pc, target, d.err});
+      br_task<NR>(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), smem, BlockWait{pc, target, d.err});
     } else {  // kDagBx
       const int s = tk.b;
-      bx_task(a, d, s, tk.c, tk.d, x, smem, &flag,
-              BlockWait{cnt + kRDone * ns + s, d.meta[kMetaStride * s + kMetaNBr], d.err});
+      bx_task<NR>(a, d, s, tk.c, tk.d, x, smem, &flag,
+                  BlockWait{cnt + kRDone * ns + s, d.meta[kMetaStride * s + kMetaNBr], d.err});
     }
   }
 }
@@ -729,11 +809,10 @@ int resident_blocks(K kernel, int threads) {
   return std::max(1, per_cu) * std::max(1, cus);
 }
 
-}  // namespace
-
-template <class TF, class TD, class TX>
-void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st) {
-  const SolveDev& d = dp.d;
+// NR right-hand sides x + r sx (r < NR) in one pair of launches
+template <int NR, class TF, class TD, class TX>
+void ldl_solve_group(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, const SolveDev& d, TX* x, TX* uvec,
+                     hipStream_t st) {
   hipLaunchKernelGGL(k_zero, dim3((d.ncounters + 255) / 256), dim3(256), 0, st, d.ncounters, d.counters);
   if (dp.schedule == kScheduleDag) {
     // Every task waits only on tasks before it in the list. The smallest
@@ -741,28 +820,29 @@ void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX*
     // dispenser, whose earlier tasks are all done, so a block of that
     // dispenser takes it: blocks 0 .. kDispensers - 1 are dispatched first
     // and serve every dispenser until its share of the list runs out.
-    static const int grid_f = resident_blocks(k_fwd_dag<TF, TD, TX>, 256);
-    static const int grid_b = resident_blocks(k_bwd_dag<TF, TD, TX>, 256);
+    static const int grid_f = resident_blocks(k_fwd_dag<NR, TF, TD, TX>, 256);
+    static const int grid_b = resident_blocks(k_bwd_dag<NR, TF, TD, TX>, 256);
     if (d.nfwd > 0)
-      hipLaunchKernelGGL((k_fwd_dag<TF, TD, TX>), dim3(std::max(kDispensers, std::min(d.nfwd, grid_f))), dim3(256), 0,
-                         st, a, d, x, uvec);
+      hipLaunchKernelGGL((k_fwd_dag<NR, TF, TD, TX>), dim3(std::max(kDispensers, std::min(d.nfwd, grid_f))), dim3(256),
+                         0, st, a, d, x, uvec);
     if (d.nbwd > 0)
-      hipLaunchKernelGGL((k_bwd_dag<TF, TD, TX>), dim3(std::max(kDispensers, std::min(d.nbwd, grid_b))), dim3(256), 0,
-                         st, a, d, x);
+      hipLaunchKernelGGL((k_bwd_dag<NR, TF, TD, TX>), dim3(std::max(kDispensers, std::min(d.nbwd, grid_b))), dim3(256),
+                         0, st, a, d, x);
     MPIR_HIP(hipGetLastError());
     return;
   }
   auto tiles = [&](int kind, const std::vector<int32_t>& ptr, const int32_t* list, int l) {
     const int t0 = ptr[l], t1 = ptr[l + 1];
     if (t1 > t0)
-      hipLaunchKernelGGL((k_level_tiles<TF, TD, TX>), dim3(t1 - t0), dim3(256), 0, st, a, d, kind, list + 3 * t0, x,
-                         uvec);
+      hipLaunchKernelGGL((k_level_tiles<NR, TF, TD, TX>), dim3(t1 - t0), dim3(256), 0, st, a, d, kind, list + 3 * t0,
+                         x, uvec);
   };
   // forward, leaves to root; D^{-1} fused
   for (int l = 0; l < dp.nlevels; ++l) {
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
-      hipLaunchKernelGGL((k_fwd_small<TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, d, d.small_list + s0, x, uvec);
+      hipLaunchKernelGGL((k_fwd_small<NR, TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, d, d.small_list + s0, x,
+                         uvec);
     tiles(kDagFx, dp.h_fx_ptr, d.fx_task, l);
     tiles(kDagFu, dp.h_fu_ptr, d.fu_task, l);
   }
@@ -772,9 +852,49 @@ void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX*
     tiles(kDagBx, dp.h_bx_ptr, d.bx_task, l);
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
-      hipLaunchKernelGGL((k_bwd_small<TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, d.small_list + s0, x);
+      hipLaunchKernelGGL((k_bwd_small<NR, TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, d, d.small_list + s0, x);
   }
   MPIR_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+template <class TF, class TD, class TX>
+void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st) {
+  ldl_solve_group<1>(a, dp, dp.d, x, uvec, st);
+}
+
+template <class TF, class TD, class TX>
+void ldl_solve_multi(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, int nrhs, TX* uvec,
+                     hipStream_t st) {
+  // groups of 8, 4, 2, 1 right-hand sides for FP32 factors; FP64 tiles do
+  // not fit in registers and 16-bit tiles with their conversions spill, so
+  // those factors take the right-hand sides one at a time
+  constexpr bool multi = std::is_same<TF, float>::value;
+  static_assert(!multi || (ColTile<TF>::NB == 1 && RowTile<TF>::NB == 1), "tile in registers");
+  for (int r0 = 0; r0 < nrhs;) {
+    const int left = nrhs - r0;
+    TX* xr = x + r0 * dp.d.sx;
+    if constexpr (multi) {
+      if (left >= 8) {
+        ldl_solve_group<8>(a, dp, dp.d, xr, uvec, st);
+        r0 += 8;
+        continue;
+      }
+      if (left >= 4) {
+        ldl_solve_group<4>(a, dp, dp.d, xr, uvec, st);
+        r0 += 4;
+        continue;
+      }
+      if (left >= 2) {
+        ldl_solve_group<2>(a, dp, dp.d, xr, uvec, st);
+        r0 += 2;
+        continue;
+      }
+    }
+    ldl_solve_group<1>(a, dp, dp.d, xr, uvec, st);
+    r0 += 1;
+  }
 }
 
 template void ldl_solve<double, double, double>(const SolveArgs<double, double>&, const DeviceSolvePlan&, double*,
@@ -787,5 +907,14 @@ template void ldl_solve<__half, float, double>(const SolveArgs<__half, float>&, 
                                                double*, hipStream_t);
 template void ldl_solve<__hip_bfloat16, float, double>(const SolveArgs<__hip_bfloat16, float>&,
                                                        const DeviceSolvePlan&, double*, double*, hipStream_t);
+template void ldl_solve_multi<double, double, double>(const SolveArgs<double, double>&, const DeviceSolvePlan&,
+                                                       double*, int, double*, hipStream_t);
+template void ldl_solve_multi<float, float, double>(const SolveArgs<float, float>&, const DeviceSolvePlan&, double*,
+                                                     int, double*, hipStream_t);
+template void ldl_solve_multi<__half, float, double>(const SolveArgs<__half, float>&, const DeviceSolvePlan&, double*,
+                                                      int, double*, hipStream_t);
+template void ldl_solve_multi<__hip_bfloat16, float, double>(const SolveArgs<__hip_bfloat16, float>&,
+                                                              const DeviceSolvePlan&, double*, int, double*,
+                                                              hipStream_t);
 
 }  // namespace mpir

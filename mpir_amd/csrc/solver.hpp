@@ -142,6 +142,9 @@ class GmresLdl {
   void luir(const double* b, double* x, int maxiter, LuirResult* out);
   // building blocks exposed for tests and the kernel-level C entry points
   void precond_apply_dev(double* x);  // x := (L D L^T)^{-1} x, permuted order
+  // the same for R vectors of n entries, contiguous (the factor read once
+  // for up to 8 of them)
+  void precond_apply_slots(double* x, int R);
   void spmv_dev(const double* x, double* y);  // y = K_s x, permuted, scaled
   // factor (permuted order) copied out in QDLDL's CSC layout
   void export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* D, void* Dinv, int prec);
@@ -161,7 +164,7 @@ class GmresLdl {
   // batched GMRES on R slots of the combined vector; active[q] selects the
   // instances that iterate, kq[q] returns their step counts
   void precond_gmres_batch(const double* b, double* res, const std::vector<char>& active);
-  void precond_apply_slots(double* x, int R);
+  void set_solve_slots(int slots);
   void spmv_slots(const double* x, double* y, int R);
   void residual_slots(const double* x, const double* b, double* r, int R);
   void bnorms(const double* v, int op, int R, std::vector<double>& out);
@@ -236,6 +239,7 @@ class GmresLdl {
   DevBuf d_b_scalar_, d_b_scalar2_, d_b_mask_, d_b_kq_, d_b_off_, d_b_nrhs_, d_b_piv_;
   DevBuf d_b_vec_;  // b, x, r, d, w of every slot
   int32_t b_nch_ = 0, b_slots_ = 0;
+  int solve_slots_ = 0;  // right-hand sides the solve work areas hold
   double* b_pinned_ = nullptr;
   size_t b_pinned_count_ = 0;
 };
