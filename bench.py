@@ -386,6 +386,8 @@ def main(argv=None) -> int:
     ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
     ap.add_argument("--nrhs", type=int, default=8, help="right-hand sides per system of the batched form")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only a 1 GiB device-to-device copy (x3): the reference kernel for HBM counter scaling")
     ap.add_argument("--serve", action="store_true",
                     help="serving form instead of the bench line: Poisson request stream with continuous batching")
     ap.add_argument("--rate", type=float, default=2000.0, help="offered request rate of the serving form (1/s)")
@@ -404,6 +406,16 @@ def main(argv=None) -> int:
     if not mpir.device_ok():
         print("no gfx950 device", file=sys.stderr)
         return 1
+    if args.copy_calib:
+        # a plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for
+        # 1 GiB moved each way scales the counters of the solver's kernels
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "launches": 3}), flush=True)
+        return 0
     if args.sweep:
         return precision_sweep(mpir, args.k, args.tol)
     if args.batched:
