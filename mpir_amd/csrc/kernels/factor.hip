@@ -431,37 +431,43 @@ struct Mfma16<double> {
   static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
 
-// Trailing update tile: C[r0:r0+64, c0:c0+64] -= L_r D L_c^T over the nbk
-// columns of step k, lower part only. Columns < nc live in the panel,
-// columns >= nc in the update matrix. Tiles with r0 >= m cover identity rows
-// (rows m .. m+nc) and only panel columns. Each wave computes a 32 x 32
-// quarter with 2 x 2 MFMA blocks.
+// Trailing update tile: C[r0:r0+64, c0:c0+64] -= sum over the task's steps
+// of L_r D L_c^T (the nbk columns of each step), lower part only. A task
+// applies `steps` consecutive panel steps ending at k (field d = k | steps <<
+// kUpdStepsShift), accumulating them on the matrix cores before one
+// read-modify-write of the target. Columns < nc live in the panel, columns
+// >= nc in the update matrix. Tiles with r0 >= m cover identity rows (rows
+// m .. m+nc) and only panel columns. Each wave computes a 32 x 32 quarter
+// with 2 x 2 MFMA blocks; the next step's operands are loaded into registers
+// while the current step's products run.
 template <class T, int R>
 __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   using V = typename Mfma16<T>::V;
   __shared__ T As[kPanelNB][kTile + 1];  // As[t][i] = L[r0 + i, k0 + t]
   __shared__ T Bs[kPanelNB][kTile + 1];  // Bs[t][j] = L[c0 + j, k0 + t] D[k0 + t]
   const Task4 tk = tasks[blockIdx.x];
-  const int s = tk.a, r0 = tk.b, c0 = tk.c, k = tk.d;
+  const int s = tk.a, r0 = tk.b, c0 = tk.c;
+  const int klast = tk.d & ((1 << kUpdStepsShift) - 1);
+  const int kfirst = klast - ((tk.d >> kUpdStepsShift) & ((1 << (kUpdPanelBit - kUpdStepsShift)) - 1)) + 1;
+  const bool panel_only = (tk.d >> kUpdPanelBit) & 1;
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
   const int nr = a.nrow[s];
   const int m = nc + nr;
   const int64_t ld = panel_ld(nc, nr);
   const int rlim = r0 >= m ? m + nc : m;
-  const int k0 = k * kPanelNB;
-  const int nbk = min(kPanelNB, nc - k0);
   const T* P = a.panel + a.panel_off[s];
   T* Pw = a.panel + a.panel_off[s];
   T* U = nr > 0 ? a.upool + a.upd_off[s] : Pw;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  {
-    // thread (row i = lane, columns t = w + 4q): all loads issued before any
-    // is used (clamped addresses, values selected afterwards)
-    const int i = lane;
-    const int ra = r0 + i, rb = c0 + i;
-    const int64_t ras = ra < rlim ? ra : r0, rbs = rb < m ? rb : c0;
-    T av[16], bv[16], dv[16];
+  // thread (row i = lane, columns t = w + 4q): all loads of a step issued
+  // before any is used (clamped addresses, values selected afterwards)
+  const int ra = r0 + lane, rb = c0 + lane;
+  const int64_t ras = ra < rlim ? ra : r0, rbs = rb < m ? rb : c0;
+  T av[16], bv[16], dv[16];
+  auto load = [&](int kk) {
+    const int k0 = kk * kPanelNB;
+    const int nbk = min(kPanelNB, nc - k0);
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int t = min(w + 4 * q, nbk - 1);
@@ -469,14 +475,7 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
       bv[q] = P[rbs + (k0 + t) * ld];
       dv[q] = a.D[f + k0 + t];
     }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int t = w + 4 * q;
-      As[t][i] = (ra < rlim && t < nbk) ? av[q] : T(0);
-      Bs[t][i] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
-    }
-  }
-  __syncthreads();
+  };
   const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
   const int li = lane & 15, lk = lane >> 4;
   V acc[2][2];
@@ -486,19 +485,31 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[x][y][v] = T(0);
-  for (int kk = 0; kk < nbk; kk += 4) {
-    const int t = kk + lk;  // rows t >= nbk of As / Bs are zero
-    const T a0 = As[t][wr + li], a1 = As[t][wr + 16 + li];
-    const T b0 = Bs[t][wc + li], b1 = Bs[t][wc + 16 + li];
-    acc[0][0] = Mfma16<T>::op(a0, b0, acc[0][0]);
-    acc[0][1] = Mfma16<T>::op(a0, b1, acc[0][1]);
-    acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
-    acc[1][1] = Mfma16<T>::op(a1, b1, acc[1][1]);
+  load(kfirst);
+  for (int kk = kfirst; kk <= klast; ++kk) {
+    const int nbk = min(kPanelNB, nc - kk * kPanelNB);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int t = w + 4 * q;
+      As[t][lane] = (ra < rlim && t < nbk) ? av[q] : T(0);
+      Bs[t][lane] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
+    }
+    __syncthreads();
+    if (kk < klast) load(kk + 1);
+    for (int t0 = 0; t0 < nbk; t0 += 4) {
+      const int t = t0 + lk;  // rows t >= nbk of As / Bs are zero
+      const T a0 = As[t][wr + li], a1 = As[t][wr + 16 + li];
+      const T b0 = Bs[t][wc + li], b1 = Bs[t][wc + 16 + li];
+      acc[0][0] = Mfma16<T>::op(a0, b0, acc[0][0]);
+      acc[0][1] = Mfma16<T>::op(a0, b1, acc[0][1]);
+      acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
+      acc[1][1] = Mfma16<T>::op(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
   }
-  // the tile goes through LDS (As is free after this barrier) so that the
-  // read-modify-write of the target is coalesced: element (x, y, v) is row
-  // wr + 16x + row(lk, v), column wc + 16y + li of the tile
-  __syncthreads();
+  // the tile goes through LDS (As is free after the last barrier) so that
+  // the read-modify-write of the target is coalesced: element (x, y, v) is
+  // row wr + 16x + row(lk, v), column wc + 16y + li of the tile
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -513,7 +524,7 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int c = c0 + w + 4 * q;
-    const bool ok = c < m && r < rlim && r >= c && (c < nc || r < m);
+    const bool ok = c < m && r < rlim && r >= c && (c < nc || (r < m && !panel_only));
     T* p = !ok ? Pw : c < nc ? Pw + r + c * ld : U + (r - nc) + static_cast<int64_t>(c - nc) * nr;
     cur[q] = *p;
     dst[q] = ok ? p : nullptr;

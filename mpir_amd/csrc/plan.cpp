@@ -196,18 +196,38 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
           if (k0 >= nc) continue;
           const int32_t nbk = std::min(kPanelNB, nc - k0);
           const int32_t t0 = k0 + nbk;
+          // the step's group of kUpdGroup steps ends at column gend; its
+          // trailing columns up to gend take this step's rank-64 update,
+          // those right of it the whole group's update at its last step
+          const int32_t nk = (nc + kPanelNB - 1) / kPanelNB;
+          const int32_t g0 = k / kUpdGroup * kUpdGroup;
+          const int32_t glast = std::min(nk - 1, g0 + kUpdGroup - 1);
+          const int32_t gend = std::min(nc, (glast + 1) * kPanelNB);
+          // In a front's last group gend = nc: the last near tile may reach
+          // past nc, into the columns of the update matrix that the group's
+          // far tiles cover, so its near tiles keep to the panel columns.
+          const bool last_group = glast == nk - 1;
+          const int32_t near = k | (1 << kUpdStepsShift) | ((last_group ? 1 : 0) << kUpdPanelBit);
+          const int32_t far = k | ((k - g0 + 1) << kUpdStepsShift);
+          const bool group_end = k == glast;
           if (pass == 0) {
             // the diagonal block's own task, then rows below it in chunks of 64
             P.pan_task.push_back({s, k0, t0, k});
             for (int32_t r0 = t0; r0 < m; r0 += kPanelRows) P.pan_task.push_back({s, r0, std::min(m, r0 + kPanelRows), k});
-            for (int32_t c0 = t0; c0 < m; c0 += kTile)
-              for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, k});
+            for (int32_t c0 = t0; c0 < gend; c0 += kTile)
+              for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, near});
+            if (group_end)
+              for (int32_t c0 = gend; c0 < m; c0 += kTile)
+                for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, far});
           } else {
             // aug rows i < t0 are the only ones nonzero in this step's columns
             for (int32_t r0 = 0; r0 < t0; r0 += kPanelRows)
               P.pan_task.push_back({s, m + r0, m + std::min(t0, r0 + kPanelRows), k});
-            for (int32_t c0 = t0; c0 < nc; c0 += kTile)
-              for (int32_t r0 = 0; r0 < t0; r0 += kTile) P.upd_task.push_back({s, m + r0, c0, k});
+            for (int32_t c0 = t0; c0 < gend; c0 += kTile)
+              for (int32_t r0 = 0; r0 < t0; r0 += kTile) P.upd_task.push_back({s, m + r0, c0, near});
+            if (group_end)
+              for (int32_t c0 = gend; c0 < nc; c0 += kTile)
+                for (int32_t r0 = 0; r0 < t0; r0 += kTile) P.upd_task.push_back({s, m + r0, c0, far});
           }
         }
       }

@@ -21,6 +21,15 @@ constexpr int kXcRows = 256;
 constexpr int kGemvCols = 256;
 // Tile edge of the trailing-update tasks.
 constexpr int kTile = 64;
+// Panel steps per update group: a step updates the trailing columns of its
+// own group (rank 64); the columns right of the group are updated once, at
+// the group's last step, with all its steps (rank kUpdGroup * 64).
+constexpr int kUpdGroup = 4;
+// Update task field d: the last step k, the number of steps ending at k
+// that the task applies, and a flag restricting the task to panel columns
+// (c < nc): d = k | (steps << kUpdStepsShift) | (panel_only << kUpdPanelBit).
+constexpr int kUpdStepsShift = 20;
+constexpr int kUpdPanelBit = 30;
 // Edge of the extend-add assembly tiles.
 constexpr int kAsmTile = 64;
 
@@ -57,7 +66,8 @@ struct FactorPlan {
   std::vector<int32_t> asm_child_ptr;  // per task + 1
   std::vector<int32_t> asm_child;      // 5-tuples
   // per (level, step): panel tasks (s, row0, row1, step) and update tiles
-  // (s, tile_row0, tile_col0, step); indexed through step_ptr. A front's
+  // (s, tile_row0, tile_col0, step | steps << kUpdStepsShift); indexed
+  // through step_ptr. A front's
   // first panel task of a step has row0 = the step's first column: it owns
   // the diagonal block (writes L_kk, D, D^{-1}); the others cover at most
   // kPanelRows rows below it (or identity rows m ..)
