@@ -8,8 +8,10 @@
 // on an MI355X (gfx950) through libmpir_hip.so; link with -lmpir_hip.
 //
 // Supported precisions: UF float or double, UW = UR = double. Compute()
-// keeps the reference's signature; the FGPILU parameters (ILU_k and the
-// template sizes) are accepted and ignored, as the LDL^T form does.
+// keeps the reference's signature. With the default L D L^T preconditioner
+// ILU_k and the template sizes are accepted and ignored; after
+// SetPreconditioner(MPIR_PREC_FGPILU) Compute runs the reference's shipped
+// FGPILU path (gmres_ir.hpp:323-531) with ILU_k and MAX_SWEEPS.
 // Failures throw std::runtime_error with the library's message.
 #pragma once
 
@@ -43,7 +45,12 @@ class GmresLDLIR {
   template <typename TAx, std::size_t MAX_MATRIX_SIZE = 1 << 21, std::size_t MAX_SWEEPS = 10,
             std::size_t BATCH_SIZE = 4096>
   void Compute(std::vector<std::size_t> Ap, std::vector<std::size_t> Ai, std::vector<TAx> Ax, std::size_t ILU_k = 0) {
-    (void)ILU_k;
+    if (prec_ == MPIR_PREC_FGPILU && (ILU_k != ilu_k_ || MAX_SWEEPS != sweeps_)) {
+      check(mpir_gmres_ldl_set_preconditioner(h_, prec_, static_cast<int32_t>(ILU_k), static_cast<int32_t>(MAX_SWEEPS)),
+            "Compute");
+      ilu_k_ = ILU_k;
+      sweeps_ = MAX_SWEEPS;
+    }
     if (Ap.empty() || Ap.size() - 1 > static_cast<std::size_t>(std::numeric_limits<int32_t>::max()) ||
         Ai.size() > static_cast<std::size_t>(std::numeric_limits<int32_t>::max()) || Ax.size() != Ai.size())
       throw std::invalid_argument("GmresLDLIR::Compute: bad CSC sizes");
@@ -68,6 +75,13 @@ class GmresLDLIR {
 
   // Additions of this implementation (not in the reference's class).
   void SetOrdering(int ordering) { check(mpir_gmres_ldl_set_ordering(h_, ordering), "SetOrdering"); }
+  // MPIR_PREC_LDL (default) or MPIR_PREC_FGPILU; takes effect at the next Compute
+  void SetPreconditioner(int kind) {
+    check(mpir_gmres_ldl_set_preconditioner(h_, kind, 0, 10), "SetPreconditioner");
+    prec_ = kind;
+    ilu_k_ = 0;
+    sweeps_ = 10;
+  }
   mpir_gmres_stats Stats() const {
     mpir_gmres_stats st{};
     check(mpir_gmres_ldl_stats(h_, &st), "Stats");
@@ -82,4 +96,6 @@ class GmresLDLIR {
   }
   mpir_gmres_ldl_t h_ = nullptr;
   std::size_t n_ = 0;
+  int prec_ = MPIR_PREC_LDL;
+  std::size_t ilu_k_ = 0, sweeps_ = 10;
 };

@@ -147,6 +147,10 @@ typedef struct mpir_gmres_stats {
   double analyze_ms, factor_ms, solve_ms;
   int64_t device_bytes;
   int32_t ruiz_sweeps;         /* equilibration sweeps of the last factorization */
+  int32_t preconditioner;      /* MPIR_PREC_LDL or MPIR_PREC_FGPILU */
+  int32_t ilu_sweeps;          /* FGPILU: sweeps of the last factorization */
+  double ilu_nl_residual0;     /* FGPILU: nonlinear residual of the initial guess */
+  double ilu_nl_residual;      /* FGPILU: nonlinear residual after the last sweep */
 } mpir_gmres_stats;
 
 /* uf: MPIR_F64, MPIR_F32, or MPIR_F16 / MPIR_BF16 (factor computed with
@@ -172,6 +176,23 @@ int mpir_gmres_ldl_set_scaling(mpir_gmres_ldl_t h, int mode);
 #define MPIR_SCHED_LEVELS 0
 #define MPIR_SCHED_DAG 1
 int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule);
+/* preconditioner M of GMRES-IR, before analyze: MPIR_PREC_LDL (default),
+ * the L D L^T of the reordered matrix, or MPIR_PREC_FGPILU, the reference's
+ * shipped GmresLDLIR::Compute (include/MPIR/gmres_ir.hpp:323-531): fine-grained
+ * parallel ILU of the diagonally scaled matrix in its given order, on the
+ * ILU(ilu_k) pattern (its ILU_k argument, 0..8), sweeping until the nonlinear
+ * residual changes by less than the tolerance or max_sweeps (its MAX_SWEEPS);
+ * M^{-1} = U^{-1} L^{-1} (TriangularSolve, :743-774). FGPILU factors are f64
+ * or f32. The sweeps here are synchronous (every entry from the previous
+ * sweep), the reference's run in place, so the two agree once converged. */
+#define MPIR_PREC_LDL 0
+#define MPIR_PREC_FGPILU 1
+int mpir_gmres_ldl_set_preconditioner(mpir_gmres_ldl_t h, int kind, int32_t ilu_k, int32_t max_sweeps);
+/* FGPILU factors after factor: U by columns (Up n+1, Ui nnz, Ux nnz), L by
+ * rows on the same indices (Lx nnz, unit diagonal stored as 1), 1 / U_jj (n);
+ * nnz from mpir_gmres_ldl_ilu_nnz after analyze */
+int mpir_gmres_ldl_ilu_nnz(mpir_gmres_ldl_t h, int64_t* nnz);
+int mpir_gmres_ldl_export_ilu(mpir_gmres_ldl_t h, int32_t* Up, int32_t* Ui, double* Ux, double* Lx, double* UDinv);
 /* symbolic phase on the pattern of the upper CSC (reused by later factors) */
 int mpir_gmres_ldl_analyze(mpir_gmres_ldl_t h, int32_t n, const int32_t* Ap, const int32_t* Ai);
 /* numeric phase; Ax in the analyzed pattern's entry order (host / device) */

@@ -31,6 +31,7 @@ _PREC_OF = {np.dtype(np.float64): F64, np.dtype(np.float32): F32}
 _FACTOR_PREC_OF = {**_PREC_OF, np.dtype(np.float16): F16}
 SCALE_NONE, SCALE_DIAG, SCALE_RUIZ = 0, 1, 2
 SCHED_LEVELS, SCHED_DAG = 0, 1
+PREC_LDL, PREC_FGPILU = 0, 1  # preconditioner of GMRES-IR
 
 
 class MpirError(RuntimeError):
@@ -145,7 +146,7 @@ class GmresLDLIR:
     """
 
     def __init__(self, UF=np.float64, UW=np.float64, UR=np.float64, ordering: int = ORDER_AMD,
-                 scaling: int = SCALE_DIAG):
+                 scaling: int = SCALE_DIAG, preconditioner: int | None = None, MAX_SWEEPS: int = 10):
         # UF may also be np.float16 or "bf16": a factor computed with FP32
         # arithmetic and stored (and applied) in the 16-bit format
         uf = BF16 if isinstance(UF, str) and UF.lower() == "bf16" else _FACTOR_PREC_OF.get(np.dtype(UF))
@@ -160,6 +161,9 @@ class GmresLDLIR:
         self.UF = UF if uf == BF16 else np.dtype(UF)
         _check(load().mpir_gmres_ldl_set_ordering(self._h, ordering), "set_ordering")
         _check(load().mpir_gmres_ldl_set_scaling(self._h, scaling), "set_scaling")
+        self._prec = (PREC_LDL, 0, int(MAX_SWEEPS))
+        if preconditioner is not None:
+            self.SetPreconditioner(preconditioner, 0, MAX_SWEEPS)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -184,6 +188,14 @@ class GmresLDLIR:
         """SCALE_NONE, SCALE_DIAG (the reference's, default) or SCALE_RUIZ."""
         _check(load().mpir_gmres_ldl_set_scaling(self._h, int(mode)), "SetScaling")
 
+    def SetPreconditioner(self, kind: int, ILU_k: int = 0, MAX_SWEEPS: int = 10) -> None:
+        """PREC_LDL (default): L D L^T of the reordered matrix; PREC_FGPILU: the
+        reference's shipped Compute, FGPILU of the scaled matrix on the
+        ILU(ILU_k) pattern with at most MAX_SWEEPS sweeps (gmres_ir.hpp:323-531)."""
+        _check(load().mpir_gmres_ldl_set_preconditioner(self._h, int(kind), int(ILU_k), int(MAX_SWEEPS)),
+               "SetPreconditioner")
+        self._prec = (int(kind), int(ILU_k), int(MAX_SWEEPS))
+
     def SetSolveSchedule(self, schedule: int) -> None:
         """SCHED_DAG (default) or SCHED_LEVELS for the triangular solves of M^{-1}."""
         _check(load().mpir_gmres_ldl_set_schedule(self._h, int(schedule)), "SetSolveSchedule")
@@ -201,7 +213,12 @@ class GmresLDLIR:
         _check(load().mpir_gmres_ldl_factor(self._h, _ptr(Ax)), "Factor")
 
     def Compute(self, Ap, Ai, Ax, ILU_k: int = 0) -> None:
-        """Compute(Ap, Ai, Ax, ILU_k): diagonal scaling + ordering + LDL^T (ILU_k is ignored)."""
+        """Compute(Ap, Ai, Ax, ILU_k): diagonal scaling, then the L D L^T of the
+        reordered matrix (ILU_k unused) or, with PREC_FGPILU, FGPILU on the
+        ILU(ILU_k) pattern."""
+        kind, k, sweeps = self._prec
+        if kind == PREC_FGPILU and int(ILU_k) != k:
+            self.SetPreconditioner(kind, ILU_k, sweeps)
         Ap = np.ascontiguousarray(Ap, dtype=np.int32)
         Ai = np.ascontiguousarray(Ai, dtype=np.int32)
         Ax = np.ascontiguousarray(Ax, dtype=np.float64)
@@ -257,6 +274,21 @@ class GmresLDLIR:
         _check(load().mpir_gmres_ldl_export_factor(self._h, _ptr(Lp), _ptr(Li), _ptr(Lx), _ptr(D), _ptr(Dinv)),
                "export_factor")
         return Lp, Li[:lnz], Lx[:lnz], D, Dinv
+
+    def export_ilu(self):
+        """(Up, Ui, Ux, Lx, UDinv) of the FGPILU factors: U by columns, L by rows on
+        the same indices (unit diagonal stored), 1 / U_jj."""
+        nnz = C.c_int64()
+        _check(load().mpir_gmres_ldl_ilu_nnz(self._h, C.byref(nnz)), "ilu_nnz")
+        n, m = self._n, int(nnz.value)
+        Up = np.empty(n + 1, dtype=np.int32)
+        Ui = np.empty(max(m, 1), dtype=np.int32)
+        Ux = np.empty(max(m, 1))
+        Lx = np.empty(max(m, 1))
+        UDinv = np.empty(n)
+        _check(load().mpir_gmres_ldl_export_ilu(self._h, _ptr(Up), _ptr(Ui), _ptr(Ux), _ptr(Lx), _ptr(UDinv)),
+               "export_ilu")
+        return Up, Ui[:m], Ux[:m], Lx[:m], UDinv
 
     def time_kernels(self, what: str, reps: int = 10) -> float:
         """Average ms of one kernel group ('spmv', 'precond', 'factor') via HIP events."""
@@ -662,6 +694,6 @@ __all__ = [
     "QDLDL_etree", "QDLDL_factor", "QDLDL_solve", "LUIR_QDLDL", "GmresLDLIR", "MpirError", "KKT", "gen_kkt", "kkt_config",
     "amd_order", "read_mtx", "spmv_upper", "residual_upper", "device_ok", "F64", "F32", "F16", "BF16", "ORDER_NATURAL",
     "ORDER_AMD", "ORDER_ND", "SCALE_NONE", "SCALE_DIAG", "SCALE_RUIZ", "SCHED_LEVELS", "SCHED_DAG",
-    "BatchLDLIR", "BatchSystem", "batch_config", "batch_plan_info", "pattern_cache_info", "Server",
+    "PREC_LDL", "PREC_FGPILU", "BatchLDLIR", "BatchSystem", "batch_config", "batch_plan_info", "pattern_cache_info", "Server",
     "serving_patterns", "poisson_stream",
 ]

@@ -19,6 +19,7 @@ class GmresStats(C.Structure):
         ("final_rel_residual", f64), ("analyze_ms", f64), ("factor_ms", f64), ("solve_ms", f64),
         ("device_bytes", i64),
         ("ruiz_sweeps", i32),
+        ("preconditioner", i32), ("ilu_sweeps", i32), ("ilu_nl_residual0", f64), ("ilu_nl_residual", f64),
     ]
 
     def as_dict(self) -> dict:
@@ -64,6 +65,9 @@ SIGNATURES = {
     "mpir_gmres_ldl_set_ordering": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_scaling": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_schedule": (C.c_int, [P, C.c_int]),
+    "mpir_gmres_ldl_set_preconditioner": (C.c_int, [P, C.c_int, i32, i32]),
+    "mpir_gmres_ldl_ilu_nnz": (C.c_int, [P, P]),
+    "mpir_gmres_ldl_export_ilu": (C.c_int, [P, P, P, P, P, P]),
     "mpir_gmres_ldl_log": (i64, [P, C.c_char_p, i64]),
     "mpir_read_mtx_upper": (C.c_int, [C.c_char_p, P, P, P, P, P, P]),
     "mpir_gmres_ldl_analyze": (C.c_int, [P, i32, P, P]),

@@ -47,13 +47,16 @@ HOST_SOURCES = [
     CSRC / "host" / "mtx.cpp",
     CSRC / "plan.cpp",
     CSRC / "batch_plan.cpp",
+    CSRC / "host" / "ilu.cpp",
 ]
 HIP_SOURCES = [
     CSRC / "kernels" / "factor.hip",
     CSRC / "kernels" / "solve.hip",
     CSRC / "kernels" / "spmv.hip",
+    CSRC / "kernels" / "ilu.hip",
     CSRC / "gmres.hip",
     CSRC / "batch.hip",
+    CSRC / "ilu_solver.hip",
     CSRC / "serve.cpp",
     CSRC / "capi.cpp",
 ]

@@ -287,6 +287,7 @@ void GmresLdl::put_host(const void* src, size_t bytes, void* dst) {
 void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntry>>& parts) {
   const double t0 = now_ms();
   MPIR_CHECK(!parts.empty(), "a batch needs at least one system");
+  MPIR_CHECK(precond_ == kPrecLdl, "batches use the L D L^T preconditioner");
   std::vector<const PatternEntry*> raw;
   for (const auto& p : parts) {
     MPIR_CHECK(p != nullptr, "null pattern");
@@ -408,6 +409,10 @@ void GmresLdl::batch_collect_pivots(bool zero_pivot) {
 // M^{-1} applied to R slots of n entries: each tile of the factor is read
 // once for up to 8 right-hand sides
 void GmresLdl::precond_apply_slots(double* x, int R) {
+  if (precond_ == kPrecIlu) {
+    for (int r = 0; r < R; ++r) precond_apply_dev(x + static_cast<int64_t>(r) * n_);
+    return;
+  }
   set_solve_slots(std::min(R, 8));
   stats_.precond_applications += R;
   double* u = d_uvec_.as<double>();

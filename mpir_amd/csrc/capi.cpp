@@ -462,6 +462,29 @@ int mpir_gmres_ldl_set_scaling(mpir_gmres_ldl_t h, int mode) {
     return 0;
   });
 }
+int mpir_gmres_ldl_set_preconditioner(mpir_gmres_ldl_t h, int kind, int32_t ilu_k, int32_t max_sweeps) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->set_preconditioner(kind, ilu_k, max_sweeps);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_ilu_nnz(mpir_gmres_ldl_t h, int64_t* nnz) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(nnz, "null output");
+    MPIR_CHECK(h->s->analyzed() && h->s->preconditioner() == kPrecIlu, "no FGPILU pattern analyzed");
+    *nnz = h->s->ilu_nnz();
+    return 0;
+  });
+}
+int mpir_gmres_ldl_export_ilu(mpir_gmres_ldl_t h, int32_t* Up, int32_t* Ui, double* Ux, double* Lx, double* UDinv) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->export_ilu(Up, Ui, Ux, Lx, UDinv);
+    return 0;
+  });
+}
 int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;
@@ -733,6 +756,10 @@ int mpir_gmres_ldl_stats(mpir_gmres_ldl_t h, mpir_gmres_stats* out) {
     out->solve_ms = s.solve_ms;
     out->device_bytes = s.device_bytes;
     out->ruiz_sweeps = s.ruiz_sweeps;
+    out->preconditioner = s.preconditioner;
+    out->ilu_sweeps = s.ilu_sweeps;
+    out->ilu_nl_residual0 = s.ilu_nl_residual0;
+    out->ilu_nl_residual = s.ilu_nl_residual;
     return 0;
   });
 }

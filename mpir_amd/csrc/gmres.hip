@@ -289,6 +289,10 @@ void GmresLdl::analyze(int32_t n, const int32_t* Ap, const int32_t* Ai) {
   nnz_ = Ap[n];
   Ap_.assign(Ap, Ap + n + 1);
   Ai_.assign(Ai, Ai + nnz_);
+  if (precond_ == kPrecIlu) {
+    analyze_ilu(n, Ap, Ai, t0);
+    return;
+  }
   Analysis an;
   try {
     an = analyze_pattern(n, Ap, Ai, ordering_, relax_);
@@ -467,21 +471,14 @@ void GmresLdl::set_solve_slots(int slots) {
   solve_slots_ = slots;
 }
 
-void GmresLdl::alloc_numeric() {
-  const FactorPlan& P = *plan_;
+void GmresLdl::alloc_vectors() {
   const int32_t n = n_;
-  const size_t tf = uf_ == kF64 ? sizeof(double) : sizeof(float);
   d_ax_.alloc(std::max<int64_t>(1, nnz_) * sizeof(double));
   d_axs_.alloc(std::max<int64_t>(1, nnz_) * sizeof(double));
   d_ad_.alloc(n * sizeof(double));
-  d_panel_.alloc((std::max<int64_t>(1, P.panel_entries) + kPanelSlack) * tf);
-  if (uf_ == kF16 || uf_ == kBF16) d_panel16_.alloc((std::max<int64_t>(1, P.panel_entries) + kPanelSlack) * 2);
-  d_nonfinite_.alloc(sizeof(int));
-  d_upool_.alloc(std::max<int64_t>(1, P.upd_pool) * tf);
-  d_D_.alloc(n * tf);
-  d_Dinv_.alloc(n * tf);
   d_err_.alloc(sizeof(int));
   d_cnt_.alloc(sizeof(int));
+  d_nonfinite_.alloc(sizeof(int));
   d_b_.alloc(n * sizeof(double));
   d_x_.alloc(n * sizeof(double));
   d_r_.alloc(n * sizeof(double));
@@ -491,8 +488,21 @@ void GmresLdl::alloc_numeric() {
   d_small_.alloc(256 * sizeof(double));
 }
 
+void GmresLdl::alloc_numeric() {
+  const FactorPlan& P = *plan_;
+  const int32_t n = n_;
+  const size_t tf = uf_ == kF64 ? sizeof(double) : sizeof(float);
+  alloc_vectors();
+  d_panel_.alloc((std::max<int64_t>(1, P.panel_entries) + kPanelSlack) * tf);
+  if (uf_ == kF16 || uf_ == kBF16) d_panel16_.alloc((std::max<int64_t>(1, P.panel_entries) + kPanelSlack) * 2);
+  d_upool_.alloc(std::max<int64_t>(1, P.upd_pool) * tf);
+  d_D_.alloc(n * tf);
+  d_Dinv_.alloc(n * tf);
+}
+
 void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, const void* Lx, const void* Dinv) {
   MPIR_CHECK(n >= 1, "matrix dimension must be positive");
+  MPIR_CHECK(precond_ == kPrecLdl, "a loaded factor is an L D L^T factor");
   analyzed_ = factored_ = false;
   batch_.reset();
   Supernodes S;
@@ -596,7 +606,10 @@ void GmresLdl::factor_loaded(double t0) {
                      d_csr_x_.as<double>());
   MPIR_HIP(hipGetLastError());
   int err = 0, pos = 0;
-  if (uf_ == kF64) {
+  if (precond_ == kPrecIlu) {
+    MPIR_HIP(hipMemsetAsync(d_err_.get(), 0, sizeof(int), st));
+    factor_ilu();
+  } else if (uf_ == kF64) {
     const FactorArgs<double> a = factor_args<double>();
     factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, 0, st);
     count_positive<double>(n_, d_D_.as<double>(), d_cnt_.as<int>(), st);
@@ -609,7 +622,7 @@ void GmresLdl::factor_loaded(double t0) {
   MPIR_HIP(hipMemcpyAsync(&h_pinned_[0], d_err_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
   MPIR_HIP(hipMemcpyAsync(&h_pinned_[1], d_cnt_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
   int nonfinite = 0;
-  if (uf_ == kF16 || uf_ == kBF16)
+  if ((uf_ == kF16 || uf_ == kBF16) && precond_ != kPrecIlu)
     MPIR_HIP(hipMemcpyAsync(&h_pinned_[2], d_nonfinite_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
   if (batch_) batch_count_pivots();
   MPIR_HIP(hipStreamSynchronize(st));
@@ -647,6 +660,13 @@ double GmresLdl::infnrm(const double* v) {
 
 void GmresLdl::precond_apply_dev(double* x) {
   stats_.precond_applications++;
+  if (precond_ == kPrecIlu) {
+    if (uf_ == kF64)
+      ilu_solve<double>(ilu_g_, ilu_lv_, d_ilu_l_.as<double>(), d_ilu_u_.as<double>(), d_ilu_d_.as<double>(), x, stream_);
+    else
+      ilu_solve<float>(ilu_g_, ilu_lv_, d_ilu_l_.as<float>(), d_ilu_u_.as<float>(), d_ilu_d_.as<float>(), x, stream_);
+    return;
+  }
   double* u = d_uvec_.as<double>();
   if (uf_ == kF64) ldl_solve<double, double, double>(solve_args<double, double>(d_panel_.get()), dsp_, x, u, stream_);
   else if (uf_ == kF32) ldl_solve<float, float, double>(solve_args<float, float>(d_panel_.get()), dsp_, x, u, stream_);
@@ -888,6 +908,8 @@ double GmresLdl::time_kernels(int what, int reps) {
       spmv_dev(x, y);
     } else if (what == 1) {
       precond_apply_dev(x);
+    } else if (precond_ == kPrecIlu) {
+      factor_ilu();
     } else if (uf_ == kF64) {
       const FactorArgs<double> a = factor_args<double>();
       factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, 0, st);
@@ -978,6 +1000,7 @@ void GmresLdl::luir_impl(const double* b, double* x, int maxiter, LuirResult* ou
 
 void GmresLdl::luir(const double* b, double* x, int maxiter, LuirResult* out) {
   if (!factored_ || precond_only_) throw Error(kErrState, "LUIR needs a factorization of the matrix");
+  if (precond_ != kPrecLdl) throw Error(kErrState, "LUIR runs on an L D L^T factorization");
   MPIR_CHECK(ordering_ == kOrderNatural && scaling_ == kScaleNone, "LUIR runs on the matrix as given");
   MPIR_CHECK(maxiter >= 0, "maxiter must be non-negative");
   if (uf_ == kF64) luir_impl<double>(b, x, maxiter, out);
@@ -987,6 +1010,7 @@ void GmresLdl::luir(const double* b, double* x, int maxiter, LuirResult* out) {
 
 void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void* Dinv, int prec) {
   if (batch_) throw Error(kErrState, "export_factor works on single-system handles");
+  if (precond_ == kPrecIlu) throw Error(kErrState, "the handle holds FGPILU factors (export_ilu)");
   if (!factored_ && stats_.positive_pivots < 0) throw Error(kErrState, "no factor to export");
   const Supernodes& S = plan_->sn;
   const size_t tf = uf_ == kF64 ? 8 : 4;

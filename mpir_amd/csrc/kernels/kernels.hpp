@@ -155,6 +155,39 @@ template <class TF, class TD, class TX>
 void ldl_solve_multi(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, int nrhs, TX* uvec,
                      hipStream_t st);
 
+// ---------------------------------------------------------------- FGPILU
+// Device view of the ILU structure (host/ilu.hpp): U by columns, L by rows
+// on the same index arrays, U by rows, level sets of both solves.
+struct IluDev {
+  int n = 0;
+  int64_t nnz = 0;
+  const int32_t* Up = nullptr;
+  const int32_t* Ui = nullptr;
+  const int32_t* col = nullptr;
+  const int32_t* src = nullptr;
+  const int32_t* ur_ptr = nullptr;
+  const int32_t* ur_col = nullptr;
+  const int32_t* ur_idx = nullptr;
+  const int32_t* lo_rows = nullptr;
+  const int32_t* up_rows = nullptr;
+};
+struct IluLevels {
+  std::vector<int32_t> h_lo_ptr, h_up_ptr;
+};
+// av = A on the pattern (from the scaled values ax), U = L = av, Di = 1 / diag
+template <class T>
+void ilu_init(const IluDev& g, const double* ax, T* av, T* Ux, T* Lx, T* Di, hipStream_t st);
+// one sweep in the reference's order, run level by level over the columns:
+// the exact ILU factors of the pattern, in place
+template <class T>
+void ilu_factor(const IluDev& g, const IluLevels& lv, const T* av, T* Ux, T* Lx, T* Di, hipStream_t st);
+// *out = the nonlinear residual; part: one double per 256 entries
+template <class T>
+void ilu_nlres(const IluDev& g, const T* av, const T* Lx, const T* Ux, double* part, double* out, hipStream_t st);
+// x := (L U)^{-1} x (FP64 arithmetic)
+template <class T>
+void ilu_solve(const IluDev& g, const IluLevels& lv, const T* Lx, const T* Ux, const T* Di, double* x, hipStream_t st);
+
 // ---------------------------------------------------------------- vectors / SpMV
 struct CsrDev {
   int32_t n = 0;

@@ -30,6 +30,10 @@ struct BatchStats {
 
 enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1, kOrderNd = 2 };
 
+// preconditioner M of GMRES-IR
+constexpr int kPrecLdl = 0;  // L D L^T of the reordered matrix (the default)
+constexpr int kPrecIlu = 1;  // FGPILU of the matrix in its given order, the reference's shipped Compute
+
 // scaling of K before the factorization
 constexpr int kScaleNone = 0;
 constexpr int kScaleDiag = 1;  // the reference's s_i = 1/sqrt|k_ii| (gmres_ir.hpp:347-363)
@@ -54,6 +58,9 @@ struct SolveStats {
   double analyze_ms = 0, factor_ms = 0, solve_ms = 0;
   int64_t device_bytes = 0;
   int32_t ruiz_sweeps = 0;  // Ruiz sweeps of the last factorization
+  int32_t ilu_sweeps = 0;   // FGPILU: sweeps of the last factorization
+  double ilu_nl_residual0 = 0.0, ilu_nl_residual = 0.0;  // FGPILU: nonlinear residual before / after
+  int32_t preconditioner = kPrecLdl;
 };
 
 // Record of the last GMRES-IR solve, in the terms of the reference's
@@ -103,6 +110,15 @@ class GmresLdl {
     schedule_ = sched;
     dsp_.schedule = sched;
   }
+  // kPrecLdl (default) or kPrecIlu with the ILU level and the sweep limit of
+  // the reference's Compute (its ILU_k argument and MAX_SWEEPS parameter);
+  // takes effect at the next analyze
+  void set_preconditioner(int kind, int32_t ilu_k, int32_t max_sweeps);
+  int preconditioner() const { return precond_; }
+  // FGPILU factors in the layout of the reference (U by columns, L by rows on
+  // the same indices), converted to double: sizes n+1, nnz, nnz, nnz, n
+  void export_ilu(int32_t* Up, int32_t* Ui, double* Ux, double* Lx, double* UDinv);
+  int64_t ilu_nnz() const { return static_cast<int64_t>(ilu_Ui_.size()); }
   // Waits for the handle's stream and reports a triangular solve whose
   // dependency wait expired.
   void sync_check();
@@ -159,6 +175,12 @@ class GmresLdl {
  private:
   void solve_one(const double* b_dev, double* x_dev);
   void factor_loaded(double t0);  // numeric phase on the values in d_ax_
+  void alloc_vectors();  // the FP64 vectors, A's values and the flags every preconditioner uses
+  void analyze_ilu(int32_t n, const int32_t* Ap, const int32_t* Ai, double t0);
+  void factor_ilu();
+  template <class T>
+  void factor_ilu_t();
+  double ilu_nlres_now(const void* L, const void* U);
   void batch_count_pivots();
   void batch_collect_pivots(bool zero_pivot);
   // batched GMRES on R slots of the combined vector; active[q] selects the
@@ -231,6 +253,14 @@ class GmresLdl {
   DevBuf d_hcol_, d_givens_;
   std::vector<double> h_small_;
   double* h_pinned_ = nullptr;
+  // FGPILU
+  int precond_ = kPrecLdl;
+  int32_t ilu_k_ = 0, ilu_max_sweeps_ = 10;
+  IluDev ilu_g_;
+  IluLevels ilu_lv_;
+  std::vector<int32_t> ilu_Up_, ilu_Ui_;
+  DevBuf d_ilu_idx_[9];  // Up, Ui, col, src, ur_ptr, ur_col, ur_idx, lo_rows, up_rows
+  DevBuf d_ilu_av_, d_ilu_u_, d_ilu_l_, d_ilu_d_, d_ilu_part_, d_ilu_out_;
   // batch mode
   std::unique_ptr<BatchSystem> batch_;
   std::vector<std::shared_ptr<const PatternEntry>> batch_parts_;

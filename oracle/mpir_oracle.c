@@ -209,10 +209,14 @@ double or_infnrm(int64_t n, const double* x) {
 }
 
 /* ------------------------------------------------------------------ GMRES-IR */
+typedef struct or_ilu or_ilu;
+static void or_ilu_solve(const or_ilu* f, double* x);
+
 typedef struct {
   int32_t n;
   const int32_t *Ap, *Ai;
   const double* Ax; /* scaled values, original order (UW) */
+  const or_ilu* ilu; /* the FGPILU factors as M (instead of LDL^T), or NULL */
   /* factor of the permuted scaled matrix */
   const int32_t* perm; /* new -> old, or NULL */
   int32_t uf;          /* 0 double, 1 float */
@@ -229,6 +233,10 @@ typedef struct {
 static void or_precond(or_ctx* c, double* v) {
   const int32_t n = c->n;
   c->precond_count++;
+  if (c->ilu) {
+    or_ilu_solve(c->ilu, v);
+    return;
+  }
   if (c->perm) {
     for (int32_t k = 0; k < n; k++) c->work[k] = v[c->perm[k]];
   } else {
@@ -596,4 +604,417 @@ int32_t or_gmres_ir(int32_t n, const int32_t* Ap, const int32_t* Ai, const doubl
   free(c.Lp); free(c.Li); free(c.Lx); free(c.Dinv);
   free(Cp); free(Ci); free(Cx); free(Axs); free(AD);
   return rc;
+}
+
+/* ------------------------------------------------------------------ FGPILU
+ * GmresLDLIR::Compute's ILU path, include/MPIR/gmres_ir.hpp:323-531, with
+ * SparseLDotU (:776-799), NLResNorm (:801-824), ValueAt (:826-836) and the
+ * M^{-1} slot TriangularSolve (:743-774), run as the single-threaded
+ * OpenMP loop runs: columns ascending, entries ascending, updates in place.
+ * U is CSC (Up, Ui), L is CSR with the same index arrays (row r of L holds
+ * the entries of column r of U, mirrored). UF: the factor precision (float
+ * or double); the nonlinear residual is summed in double. */
+struct or_ilu {
+  int32_t n, uf;
+  int64_t nnz;
+  int32_t *Up, *Ui;
+  double* ax; /* A(Ui[idx], col) of each pattern entry (0 if absent) */
+  void *Ux, *Lx, *UDinv;
+  int32_t sweeps;
+  double nl_res0, nl_res;
+};
+
+/* the ILU_k pattern: Compute's loop at :369-408 (the upper pattern of A,
+ * then k times: column col collects the rows of A's columns row1 for every
+ * row1 in its current pattern, sorted) */
+static void or_ilu_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t ilu_k, int32_t** Up_out,
+                           int32_t** Ui_out) {
+  int32_t* Up = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+  memcpy(Up, Ap, sizeof(int32_t) * ((size_t)n + 1));
+  int32_t* Ui = (int32_t*)malloc(sizeof(int32_t) * (size_t)(Ap[n] > 0 ? Ap[n] : 1));
+  memcpy(Ui, Ai, sizeof(int32_t) * (size_t)Ap[n]);
+  char* marker = (char*)calloc((size_t)n, 1);
+  int32_t* rows = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+  for (int32_t it = 0; it < ilu_k; it++) {
+    int32_t* Np = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+    size_t cap = (size_t)Up[n] * 2 + 16, used = 0;
+    int32_t* Ni = (int32_t*)malloc(sizeof(int32_t) * cap);
+    Np[0] = 0;
+    for (int32_t col = 0; col < n; col++) {
+      int32_t cnt = 0;
+      for (int32_t i1 = Up[col]; i1 < Up[col + 1]; i1++) {
+        const int32_t row1 = Ui[i1];
+        if (row1 > col) continue;
+        for (int32_t i2 = Ap[row1]; i2 < Ap[row1 + 1]; i2++) {
+          const int32_t row2 = Ai[i2];
+          if (row2 > col) continue;
+          if (!marker[row2]) {
+            marker[row2] = 1;
+            rows[cnt++] = row2;
+          }
+        }
+      }
+      /* sort (insertion sort: columns are short) and clear the marks */
+      for (int32_t a = 1; a < cnt; a++) {
+        const int32_t v = rows[a];
+        int32_t b = a - 1;
+        while (b >= 0 && rows[b] > v) {
+          rows[b + 1] = rows[b];
+          b--;
+        }
+        rows[b + 1] = v;
+      }
+      if (used + (size_t)cnt > cap) {
+        cap = (used + (size_t)cnt) * 2;
+        Ni = (int32_t*)realloc(Ni, sizeof(int32_t) * cap);
+      }
+      for (int32_t a = 0; a < cnt; a++) {
+        Ni[used++] = rows[a];
+        marker[rows[a]] = 0;
+      }
+      Np[col + 1] = (int32_t)used;
+    }
+    free(Up);
+    free(Ui);
+    Up = Np;
+    Ui = Ni;
+  }
+  free(marker);
+  free(rows);
+  *Up_out = Up;
+  *Ui_out = Ui;
+}
+
+#define OR_ILU_BODY(UF)                                                                                     \
+  /* SparseLDotU: sum_k L[row, k] U[k, col] over k < cut (:776-799) */                                     \
+  UF* Ux = (UF*)f->Ux;                                                                                      \
+  UF* Lx = (UF*)f->Lx;                                                                                      \
+  UF* UDinv = (UF*)f->UDinv;                                                                                \
+  const int32_t *Up = f->Up, *Ui = f->Ui;                                                                   \
+  const int32_t n = f->n;
+
+static float or_ldotu_f32(const or_ilu* f, int32_t row, int32_t col, int32_t cut) {
+  OR_ILU_BODY(float)
+  (void)UDinv;
+  (void)n;
+  int32_t i1 = Up[row], i2 = Up[col];
+  float sum = 0.0f;
+  while (i1 < Up[row + 1] && i2 < Up[col + 1]) {
+    const int32_t kL = Ui[i1], kU = Ui[i2];
+    if (kL >= cut || kU >= cut) break;
+    if (kL == kU) {
+      sum += Lx[i1] * Ux[i2];
+      i1++;
+      i2++;
+    } else if (kL < kU) {
+      i1++;
+    } else {
+      i2++;
+    }
+  }
+  return sum;
+}
+static double or_ldotu_f64(const or_ilu* f, int32_t row, int32_t col, int32_t cut) {
+  OR_ILU_BODY(double)
+  (void)UDinv;
+  (void)n;
+  int32_t i1 = Up[row], i2 = Up[col];
+  double sum = 0.0;
+  while (i1 < Up[row + 1] && i2 < Up[col + 1]) {
+    const int32_t kL = Ui[i1], kU = Ui[i2];
+    if (kL >= cut || kU >= cut) break;
+    if (kL == kU) {
+      sum += Lx[i1] * Ux[i2];
+      i1++;
+      i2++;
+    } else if (kL < kU) {
+      i1++;
+    } else {
+      i2++;
+    }
+  }
+  return sum;
+}
+
+/* NLResNorm (:801-824): sum |A - L U| over the U entries, then over the
+ * strictly-lower L entries (row r of L holds the entries of column r of U) */
+static double or_nlres(const or_ilu* f) {
+  double res = 0.0;
+  const int32_t n = f->n;
+  for (int32_t col = 0; col < n; col++)
+    for (int32_t idx = f->Up[col]; idx < f->Up[col + 1]; idx++) {
+      const int32_t row = f->Ui[idx];
+      if (f->uf == 1) {
+        const float ax = (float)f->ax[idx];
+        const float sum = or_ldotu_f32(f, row, col, row + 1);
+        res += fabsf(ax - sum);
+      } else {
+        const double ax = f->ax[idx];
+        const double sum = or_ldotu_f64(f, row, col, row + 1);
+        res += fabs(ax - sum);
+      }
+    }
+  for (int32_t row = 0; row < n; row++)
+    for (int32_t idx = f->Up[row]; idx < f->Up[row + 1]; idx++) {
+      const int32_t col = f->Ui[idx];
+      if (col >= row) continue;
+      if (f->uf == 1) {
+        const float ax = (float)f->ax[idx];
+        const float sum = or_ldotu_f32(f, row, col, col + 1);
+        res += fabsf(ax - sum);
+      } else {
+        const double ax = f->ax[idx];
+        const double sum = or_ldotu_f64(f, row, col, col + 1);
+        res += fabs(ax - sum);
+      }
+    }
+  return res;
+}
+
+/* initial guesses (:414-441) and the FGPILU sweeps (:447-518) */
+static void or_fgpilu_run(or_ilu* f, int32_t max_sweeps, double tol) {
+  const int32_t n = f->n;
+  for (int32_t col = 0; col < n; col++)
+    for (int32_t idx = f->Up[col]; idx < f->Up[col + 1]; idx++) {
+      const int32_t row = f->Ui[idx];
+      if (f->uf == 1) {
+        ((float*)f->Ux)[idx] = (float)f->ax[idx];
+        if (row == col) ((float*)f->UDinv)[col] = 1.0f / ((float*)f->Ux)[idx];
+      } else {
+        ((double*)f->Ux)[idx] = f->ax[idx];
+        if (row == col) ((double*)f->UDinv)[col] = 1.0 / ((double*)f->Ux)[idx];
+      }
+    }
+  memcpy(f->Lx, f->Ux, (f->uf == 1 ? sizeof(float) : sizeof(double)) * (size_t)f->nnz);
+  double prev = or_nlres(f);
+  f->nl_res0 = prev;
+  f->nl_res = prev;
+  int32_t sweep;
+  for (sweep = 1; sweep <= max_sweeps; sweep++) {
+    for (int32_t ucol = 0; ucol < n; ucol++)
+      for (int32_t idx = f->Up[ucol]; idx < f->Up[ucol + 1]; idx++) {
+        const int32_t urow = f->Ui[idx];
+        if (f->uf == 1) {
+          float* Ux = (float*)f->Ux;
+          float* Lx = (float*)f->Lx;
+          float* Di = (float*)f->UDinv;
+          const float ax = (float)f->ax[idx];
+          Ux[idx] = ax - or_ldotu_f32(f, urow, ucol, urow);
+          if (urow == ucol) {
+            Di[urow] = 1.0f / Ux[idx];
+            Lx[idx] = 1.0f;
+          } else {
+            Lx[idx] = (ax - or_ldotu_f32(f, ucol, urow, urow)) * Di[urow];
+          }
+        } else {
+          double* Ux = (double*)f->Ux;
+          double* Lx = (double*)f->Lx;
+          double* Di = (double*)f->UDinv;
+          const double ax = f->ax[idx];
+          Ux[idx] = ax - or_ldotu_f64(f, urow, ucol, urow);
+          if (urow == ucol) {
+            Di[urow] = 1.0 / Ux[idx];
+            Lx[idx] = 1.0;
+          } else {
+            Lx[idx] = (ax - or_ldotu_f64(f, ucol, urow, urow)) * Di[urow];
+          }
+        }
+      }
+    const double res = or_nlres(f);
+    f->nl_res = res;
+    if (fabs(res - prev) < tol) break;
+    prev = res;
+  }
+  f->sweeps = sweep > max_sweeps ? max_sweeps : sweep;
+}
+
+/* TriangularSolve<double> (:743-774): L y = b by rows (entries left of the
+ * diagonal), then U x = y by columns, last to first, with UDinv */
+static void or_ilu_solve(const or_ilu* f, double* x) {
+  const int32_t n = f->n;
+  double* y = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int32_t row = 0; row < n; row++) {
+    double sum = 0.0;
+    for (int32_t idx = f->Up[row]; idx < f->Up[row + 1]; idx++) {
+      const int32_t col = f->Ui[idx];
+      if (col < row) sum += (f->uf == 1 ? (double)((const float*)f->Lx)[idx] : ((const double*)f->Lx)[idx]) * y[col];
+    }
+    y[row] = x[row] - sum;
+  }
+  for (int32_t col = n; col-- > 0;) {
+    y[col] *= f->uf == 1 ? (double)((const float*)f->UDinv)[col] : ((const double*)f->UDinv)[col];
+    for (int32_t idx = f->Up[col]; idx < f->Up[col + 1]; idx++) {
+      const int32_t row = f->Ui[idx];
+      if (row < col) y[row] -= (f->uf == 1 ? (double)((const float*)f->Ux)[idx] : ((const double*)f->Ux)[idx]) * y[col];
+    }
+  }
+  memcpy(x, y, sizeof(double) * (size_t)n);
+  free(y);
+}
+
+/* The FGPILU factors of the scaled matrix (values Axs, natural order). */
+static or_ilu* or_ilu_build(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Axs, int32_t uf,
+                            int32_t ilu_k, int32_t max_sweeps, double tol) {
+  or_ilu* f = (or_ilu*)calloc(1, sizeof(or_ilu));
+  f->n = n;
+  f->uf = uf;
+  or_ilu_pattern(n, Ap, Ai, ilu_k, &f->Up, &f->Ui);
+  f->nnz = f->Up[n];
+  f->ax = (double*)calloc((size_t)(f->nnz > 0 ? f->nnz : 1), sizeof(double));
+  /* ValueAt(row, col) (:826-836): A's column col searched for row */
+  for (int32_t col = 0; col < n; col++)
+    for (int32_t idx = f->Up[col]; idx < f->Up[col + 1]; idx++)
+      for (int32_t q = Ap[col]; q < Ap[col + 1]; q++)
+        if (Ai[q] == f->Ui[idx]) {
+          f->ax[idx] = Axs[q];
+          break;
+        }
+  const size_t fs = uf == 1 ? sizeof(float) : sizeof(double);
+  f->Ux = calloc((size_t)(f->nnz > 0 ? f->nnz : 1), fs);
+  f->Lx = calloc((size_t)(f->nnz > 0 ? f->nnz : 1), fs);
+  f->UDinv = calloc((size_t)n, fs);
+  or_fgpilu_run(f, max_sweeps, tol);
+  return f;
+}
+
+static void or_ilu_free(or_ilu* f) {
+  if (!f) return;
+  free(f->Up);
+  free(f->Ui);
+  free(f->ax);
+  free(f->Ux);
+  free(f->Lx);
+  free(f->UDinv);
+  free(f);
+}
+
+/* Exported: the ILU_k pattern size, and the FGPILU factors of a matrix as
+ * given (no scaling) for tests: Up/Ui (n+1 / nnz), Ux/Lx (nnz, double),
+ * UDinv (n), info = {sweeps, initial and final nonlinear residual}. Call
+ * with Up = NULL for the pattern size only. */
+int64_t or_fgpilu(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, int32_t uf, int32_t ilu_k,
+                  int32_t max_sweeps, double tol, int32_t* Up, int32_t* Ui, double* Ux, double* Lx, double* UDinv,
+                  double* info) {
+  or_ilu* f = or_ilu_build(n, Ap, Ai, Ax, uf, ilu_k, Up ? max_sweeps : 0, tol);
+  const int64_t nnz = f->nnz;
+  if (Up) {
+    memcpy(Up, f->Up, sizeof(int32_t) * ((size_t)n + 1));
+    memcpy(Ui, f->Ui, sizeof(int32_t) * (size_t)nnz);
+    for (int64_t q = 0; q < nnz; q++) {
+      Ux[q] = uf == 1 ? (double)((float*)f->Ux)[q] : ((double*)f->Ux)[q];
+      Lx[q] = uf == 1 ? (double)((float*)f->Lx)[q] : ((double*)f->Lx)[q];
+    }
+    for (int32_t i = 0; i < n; i++) UDinv[i] = uf == 1 ? (double)((float*)f->UDinv)[i] : ((double*)f->UDinv)[i];
+    if (info) {
+      info[0] = f->sweeps;
+      info[1] = f->nl_res0;
+      info[2] = f->nl_res;
+    }
+  }
+  or_ilu_free(f);
+  return nnz;
+}
+
+/* x := M^{-1} x with FGPILU factors given in the layout above (double) */
+void or_fgpilu_solve(int32_t n, const int32_t* Up, const int32_t* Ui, const double* Ux, const double* Lx,
+                     const double* UDinv, double* x) {
+  or_ilu f;
+  memset(&f, 0, sizeof f);
+  f.n = n;
+  f.uf = 0;
+  f.Up = (int32_t*)Up;
+  f.Ui = (int32_t*)Ui;
+  f.Ux = (void*)Ux;
+  f.Lx = (void*)Lx;
+  f.UDinv = (void*)UDinv;
+  or_ilu_solve(&f, x);
+}
+
+/* GmresLDLIR<UF, double, double> as shipped (:323-634): the reference's
+ * diagonal scaling, FGPILU(ILU_k, max_sweeps) of the scaled matrix in its
+ * natural order, then Solve with M = the ILU factors. info as or_gmres_ir,
+ * plus info[8] = sweeps, info[9] = final nonlinear residual. */
+int32_t or_gmres_ilu_ir(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, int32_t uf, int32_t ilu_k,
+                        int32_t max_sweeps, int64_t ir_iter, int64_t gmres_iter, double tol, const double* b,
+                        double* x, double* info) {
+  const double t0 = or_now();
+  const int32_t nnz = Ap[n];
+  double* Axs = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
+  double* AD = (double*)malloc(sizeof(double) * (size_t)n);
+  memcpy(Axs, Ax, sizeof(double) * (size_t)nnz);
+  const double eps_uf = uf == 0 ? DBL_EPSILON : (double)FLT_EPSILON;
+  for (int32_t i = 0; i < n; i++) AD[i] = 1;
+  for (int32_t col = 0; col < n; col++)
+    for (int32_t idx = Ap[col]; idx < Ap[col + 1]; idx++) {
+      const int32_t row = Ai[idx];
+      if (row == col && fabs(Axs[idx]) > eps_uf) AD[row] = 1.0 / sqrt(fabs(Axs[idx]));
+    }
+  for (int32_t col = 0; col < n; col++)
+    for (int32_t idx = Ap[col]; idx < Ap[col + 1]; idx++) Axs[idx] *= AD[Ai[idx]] * AD[col];
+  or_ilu* f = or_ilu_build(n, Ap, Ai, Axs, uf == 0 ? 0 : 1, ilu_k, max_sweeps, tol);
+  const double t1 = or_now();
+  or_ctx c;
+  memset(&c, 0, sizeof(c));
+  c.n = n;
+  c.Ap = Ap;
+  c.Ai = Ai;
+  c.Ax = Axs;
+  c.ilu = f;
+  c.gmres_iter = gmres_iter;
+  c.tol = tol;
+  double* bs = (double*)malloc(sizeof(double) * (size_t)n);
+  double* r = (double*)malloc(sizeof(double) * (size_t)n);
+  double* t = (double*)malloc(sizeof(double) * (size_t)n);
+  double* d = (double*)malloc(sizeof(double) * (size_t)n);
+  for (int32_t i = 0; i < n; i++) bs[i] = b[i] * AD[i];
+  or_precond_gmres(&c, bs, x);
+  or_matmul_upper(n, Ap, Ai, Axs, x, t);
+  for (int32_t i = 0; i < n; i++) r[i] = bs[i] - t[i];
+  const double bnorm = or_infnrm(n, bs);
+  double r_prev = DBL_MAX, r_norm = 0.0;
+  int64_t iter;
+  int32_t stop = 0;
+  for (iter = 1; iter <= ir_iter; iter++) {
+    const double scale = or_infnrm(n, r);
+    if (scale == 0.0) {
+      stop = 3;
+      r_norm = 0.0;
+      break;
+    }
+    const double inv = 1 / scale;
+    for (int32_t i = 0; i < n; i++) t[i] = r[i] * inv;
+    or_precond_gmres(&c, t, d);
+    for (int32_t i = 0; i < n; i++) d[i] = d[i] * scale;
+    const double d_norm = or_infnrm(n, d), x_norm = or_infnrm(n, x);
+    for (int32_t i = 0; i < n; i++) x[i] = x[i] + d[i];
+    or_matmul_upper(n, Ap, Ai, Axs, x, t);
+    for (int32_t i = 0; i < n; i++) r[i] = bs[i] - t[i];
+    r_norm = or_infnrm(n, r) / bnorm;
+    if (d_norm <= x_norm * (10 * DBL_EPSILON)) {
+      stop = 1;
+      break;
+    }
+    if (fabs(r_norm - r_prev) < tol) {
+      stop = 2;
+      break;
+    }
+    r_prev = r_norm;
+  }
+  for (int32_t i = 0; i < n; i++) x[i] = x[i] * AD[i];
+  const double t2 = or_now();
+  if (info) {
+    info[0] = (double)(iter > ir_iter ? ir_iter : iter);
+    info[1] = (double)c.gmres_total;
+    info[2] = stop;
+    info[3] = 0;
+    info[4] = (double)c.precond_count;
+    info[5] = r_norm;
+    info[6] = t1 - t0;
+    info[7] = t2 - t1;
+    info[8] = f->sweeps;
+    info[9] = f->nl_res;
+  }
+  or_ilu_free(f);
+  free(Axs); free(AD); free(bs); free(r); free(t); free(d);
+  return 0;
 }

@@ -185,5 +185,17 @@ int main(int argc, char** argv) {
   const double res64 = rel_residual(K, solver64.Solve(b), b);
   std::snprintf(msg, sizeof msg, "GmresLDLIR<double,double,double>: relative residual %.2e < 1e-13", res64);
   expect(res64 < 1e-13, msg);
+  // the reference's shipped path: FGPILU with ILU_K = 0 and 5 sweeps, as
+  // examples/example.cpp:144-147 calls Compute
+  GmresLDLIR<float, double, double> ilu;
+  ilu.SetPreconditioner(MPIR_PREC_FGPILU);
+  ilu.SetTolerance(1e-12);
+  ilu.SetMaxIRIterations(50);
+  ilu.Compute<double, 1 << 21, 5, 2048>(Ap, Ai, K.x, 0);
+  const double resi = rel_residual(K, ilu.Solve(b), b);
+  const mpir_gmres_stats sti = ilu.Stats();
+  std::snprintf(msg, sizeof msg, "GmresLDLIR FGPILU ILU(0): %d sweeps, %d IR iterations, relative residual %.2e < 1e-10",
+                sti.ilu_sweeps, sti.ir_iterations, resi);
+  expect(sti.preconditioner == MPIR_PREC_FGPILU && sti.ilu_sweeps >= 1 && resi < 1e-10, msg);
   return failures == 0 ? 0 : 1;
 }

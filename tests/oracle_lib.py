@@ -44,6 +44,9 @@ class Oracle:
             "or_gmres_ir": (i32, [i32, P, P, P, P, i32, i64, i64, f64, P, P, P]),
             "or_luir_f64": (i32, [i32, P, P, P, P, P, i32, P]),
             "or_luir_f32": (i32, [i32, P, P, P, P, P, i32, P]),
+            "or_fgpilu": (i64, [i32, P, P, P, i32, i32, i32, f64, P, P, P, P, P, P]),
+            "or_fgpilu_solve": (None, [i32, P, P, P, P, P, P]),
+            "or_gmres_ilu_ir": (i32, [i32, P, P, P, i32, i32, i32, i64, i64, f64, P, P, P]),
         })
 
     def etree(self, Ap, Ai):
@@ -106,6 +109,39 @@ class Oracle:
         ret = f(n, ptr(Ap), ptr(Ai), ptr(np.ascontiguousarray(Ax, np.float64)),
                 ptr(np.ascontiguousarray(b, np.float64)), ptr(x), int(maxiter), ptr(info))
         return ret, x, int(info[0]), float(info[1])
+
+    def fgpilu(self, Ap, Ai, Ax, uf=np.float64, ilu_k=0, max_sweeps=10, tol=1e-10):
+        """FGPILU factors of the matrix as given: (Up, Ui, Ux, Lx, UDinv, info)."""
+        n = len(Ap) - 1
+        Ap = np.ascontiguousarray(Ap, np.int32)
+        Ai = np.ascontiguousarray(Ai, np.int32)
+        Ax = np.ascontiguousarray(Ax, np.float64)
+        u = 0 if uf == np.float64 else 1
+        nnz = int(self.lib.or_fgpilu(n, ptr(Ap), ptr(Ai), ptr(Ax), u, ilu_k, max_sweeps, tol, None, None, None, None,
+                                     None, None))
+        Up = np.zeros(n + 1, np.int32)
+        Ui = np.zeros(max(nnz, 1), np.int32)
+        Ux, Lx, Di, info = np.zeros(max(nnz, 1)), np.zeros(max(nnz, 1)), np.zeros(n), np.zeros(3)
+        self.lib.or_fgpilu(n, ptr(Ap), ptr(Ai), ptr(Ax), u, ilu_k, max_sweeps, tol, ptr(Up), ptr(Ui), ptr(Ux), ptr(Lx),
+                           ptr(Di), ptr(info))
+        return Up, Ui[:nnz], Ux[:nnz], Lx[:nnz], Di, {"sweeps": int(info[0]), "nl_res0": info[1], "nl_res": info[2]}
+
+    def fgpilu_solve(self, Up, Ui, Ux, Lx, UDinv, x):
+        x = np.ascontiguousarray(x, np.float64).copy()
+        self.lib.or_fgpilu_solve(len(Up) - 1, ptr(Up), ptr(Ui), ptr(Ux), ptr(Lx), ptr(UDinv), ptr(x))
+        return x
+
+    def gmres_ilu_ir(self, Ap, Ai, Ax, b, uf=np.float64, ilu_k=0, max_sweeps=10, ir_iter=10, gmres_iter=10,
+                     tol=1e-10):
+        n = len(Ap) - 1
+        x = np.zeros(n)
+        info = np.zeros(10)
+        rc = self.lib.or_gmres_ilu_ir(n, ptr(Ap), ptr(Ai), ptr(np.ascontiguousarray(Ax, np.float64)),
+                                      0 if uf == np.float64 else 1, ilu_k, max_sweeps, ir_iter, gmres_iter, tol,
+                                      ptr(np.ascontiguousarray(b, np.float64)), ptr(x), ptr(info))
+        keys = ["ir_iterations", "gmres_iterations", "stop_reason", "factor_ret", "precond", "r_norm", "factor_s",
+                "solve_s", "sweeps", "nl_res"]
+        return rc, x, dict(zip(keys, info.tolist()))
 
     def gmres_ir(self, Ap, Ai, Ax, b, perm=None, uf=np.float64, ir_iter=10, gmres_iter=10, tol=1e-10):
         n = len(Ap) - 1

@@ -277,3 +277,173 @@ def test_luir_fp32_factor_refines_to_fp64(oracle):
     assert ret == P.n1
     assert 2 <= steps < 20 and rr < 1e-12
     np.testing.assert_allclose(x, y, rtol=0, atol=1e-9)
+
+
+# ---------------------------------------------------------------------- FGPILU
+def ilu_dense(n, Up, Ui, Ux, Lx):
+    """Dense L (rows of L on the indices of U's columns, mirrored) and U."""
+    L, U = np.zeros((n, n)), np.zeros((n, n))
+    for c in range(n):
+        for q in range(Up[c], Up[c + 1]):
+            U[Ui[q], c] = Ux[q]
+            L[c, Ui[q]] = Lx[q]
+    return L, U
+
+
+def ilu_pattern(n, Up, Ui):
+    pat = np.zeros((n, n), bool)
+    for c in range(n):
+        pat[Ui[Up[c]:Up[c + 1]], c] = True
+    return pat
+
+
+def upper_csc(K):
+    n = K.shape[0]
+    Ap, rows, vals = np.zeros(n + 1, np.int32), [], []
+    for j in range(n):
+        r = np.nonzero(np.triu(K)[:, j])[0]
+        rows.extend(r)
+        vals.extend(K[r, j])
+        Ap[j + 1] = len(rows)
+    return Ap, np.array(rows, np.int32), np.array(vals)
+
+
+def laplacian_2d(m):
+    T = 4 * np.eye(m) - np.eye(m, k=1) - np.eye(m, k=-1)
+    return np.kron(np.eye(m), T) - np.kron(np.eye(m, k=1) + np.eye(m, k=-1), np.eye(m))
+
+
+def test_fgpilu_pattern_is_a_boolean_power(oracle):
+    # Compute's pattern loop (gmres_ir.hpp:369-408): column col of the next
+    # pattern = rows of A's upper columns row1 for the rows row1 of column col,
+    # i.e. the level-k pattern is the structure of B^(k+1), B = triu(A) != 0
+    K, Ap, Ai, Ax = random_qd(20, 8, 0.15, 3)
+    B = (np.triu(K) != 0).astype(np.int64)
+    n = K.shape[0]
+    for k in range(4):
+        Up, Ui, Ux, Lx, Di, info = oracle.fgpilu(Ap, Ai, Ax, ilu_k=k, max_sweeps=0)
+        pat = np.zeros((n, n), bool)
+        for c in range(n):
+            rows = Ui[Up[c]:Up[c + 1]]
+            assert np.all(np.diff(rows) > 0)
+            pat[rows, c] = True
+        np.testing.assert_array_equal(pat, np.linalg.matrix_power(B, k + 1) > 0)
+        # no sweeps: the initial guesses are A on the pattern
+        assert info["sweeps"] == 0
+        Ld, Ud = ilu_dense(n, Up, Ui, Ux, Lx)
+        np.testing.assert_array_equal(Ud, np.where(pat, np.triu(K), 0))
+        np.testing.assert_array_equal(Di, 1 / np.diag(K))
+
+
+def test_fgpilu_initial_residual_closed_form(oracle):
+    # NLResNorm (:801-824) of the initial guesses: the sum over the pattern
+    # (both triangles) of |A - L U| with L = U = A there, L's diagonal included
+    K, Ap, Ai, Ax = random_qd(15, 6, 0.2, 4)
+    n = K.shape[0]
+    Up, Ui, Ux, Lx, Di, info = oracle.fgpilu(Ap, Ai, Ax, ilu_k=1, max_sweeps=0)
+    Ld, Ud = ilu_dense(n, Up, Ui, Ux, Lx)
+    pat = ilu_pattern(n, Up, Ui)
+    assert np.count_nonzero(pat) > np.count_nonzero(np.triu(K))  # fill entries count too
+    R = np.abs(K - Ld @ Ud)
+    expect = R[pat].sum() + R.T[pat & ~np.eye(n, dtype=bool)].sum()
+    assert info["nl_res0"] == pytest.approx(expect, rel=1e-13)
+
+
+def test_fgpilu_tridiagonal_is_exact_ldlt(oracle):
+    # no fill: ILU(0) = the exact factorization, from the textbook recurrence
+    # d_1 = a_1, l_{i+1} = b_i / d_i, d_{i+1} = a_{i+1} - l_{i+1} b_i
+    n = 14
+    a = 4 + 0.1 * np.arange(n)
+    b = -1 + 0.05 * np.arange(n - 1)
+    K = np.diag(a) + np.diag(b, 1) + np.diag(b, -1)
+    Ap, Ai, Ax = upper_csc(K)
+    d, l = np.zeros(n), np.zeros(n)
+    d[0] = a[0]
+    for i in range(n - 1):
+        l[i + 1] = b[i] / d[i]
+        d[i + 1] = a[i + 1] - l[i + 1] * b[i]
+    Up, Ui, Ux, Lx, Di, info = oracle.fgpilu(Ap, Ai, Ax, ilu_k=0, max_sweeps=10, tol=1e-10)
+    Ld, Ud = ilu_dense(n, Up, Ui, Ux, Lx)
+    np.testing.assert_allclose(np.diag(Ud), d, rtol=1e-15)
+    np.testing.assert_allclose(np.diag(Ud, 1), b, rtol=1e-15)
+    np.testing.assert_allclose(np.diag(Ld, -1), l[1:], rtol=1e-15)
+    np.testing.assert_array_equal(np.diag(Ld), np.ones(n))
+    np.testing.assert_allclose(Di, 1 / d, rtol=1e-15)
+    # the in-order sweep is exact at once; the second changes nothing
+    assert info["sweeps"] == 2 and info["nl_res"] < 1e-13
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_fgpilu_band_pattern_is_the_lu_factorization(oracle, seed):
+    # a full band holds all of its LU fill, so ILU(k) on it is the LU
+    # factorization without pivoting (unique; it exists for a strictly
+    # diagonally dominant matrix): L U = K, L unit lower, U upper. The
+    # diagonal is positive in the first 10 rows, negative after, as in a
+    # quasi-definite matrix.
+    rng = np.random.default_rng(20 + seed)
+    n, w = 15, 3
+    K = np.zeros((n, n))
+    for d in range(1, w + 1):
+        v = rng.uniform(-1, 1, n - d)
+        K += np.diag(v, d) + np.diag(v, -d)
+    K += np.diag(np.where(np.arange(n) < 10, 1.0, -1.0) * (np.abs(K).sum(1) + 0.5))
+    Ap, Ai, Ax = upper_csc(K)
+    for k in (0, 2):
+        Up, Ui, Ux, Lx, Di, info = oracle.fgpilu(Ap, Ai, Ax, ilu_k=k, max_sweeps=3)
+        Ld, Ud = ilu_dense(n, Up, Ui, Ux, Lx)
+        np.testing.assert_allclose(Ld @ Ud, K, rtol=0, atol=1e-14 * np.abs(K).max())
+        np.testing.assert_array_equal(np.diag(Ld), np.ones(n))
+        np.testing.assert_allclose(Di, 1 / np.diag(Ud), rtol=1e-15)
+        assert np.count_nonzero(Di > 0) == 10
+
+
+def test_fgpilu_pattern_misses_fill_below_the_diagonal_path(oracle):
+    # the pattern follows increasing paths only: an arrow matrix (row 0 full)
+    # fills completely in LU, but ILU(k) keeps its upper pattern for every k
+    n = 8
+    K = np.eye(n) * 4
+    K[0, 1:] = K[1:, 0] = 1
+    Ap, Ai, Ax = upper_csc(K)
+    for k in (0, 3, n):
+        Up, Ui, *_ = oracle.fgpilu(Ap, Ai, Ax, ilu_k=k, max_sweeps=0)
+        np.testing.assert_array_equal(ilu_pattern(n, Up, Ui), np.triu(K) != 0)
+
+
+def test_fgpilu_ilu0_matches_a_on_the_pattern_only(oracle):
+    # ILU(0) of the 2-D Laplacian: (L U)_ij = A_ij on the pattern, dropped fill elsewhere
+    K = laplacian_2d(8)
+    n = K.shape[0]
+    Ap, Ai, Ax = upper_csc(K)
+    Up, Ui, Ux, Lx, Di, info = oracle.fgpilu(Ap, Ai, Ax, ilu_k=0, max_sweeps=10)
+    Ld, Ud = ilu_dense(n, Up, Ui, Ux, Lx)
+    pat = (K != 0)
+    E = Ld @ Ud - K
+    assert np.abs(E[pat]).max() < 1e-14
+    assert np.abs(E[~pat]).max() > 1e-2
+    # M^{-1} = U^{-1} L^{-1} (TriangularSolve, :743-774) against dense solves
+    x = np.random.default_rng(5).uniform(-1, 1, n)
+    y = oracle.fgpilu_solve(Up, Ui, Ux, Lx, Di, x)
+    np.testing.assert_allclose(y, np.linalg.solve(Ud, np.linalg.solve(Ld, x)), rtol=1e-13, atol=1e-14)
+
+
+def test_fgpilu_fp32_factor_and_sweep_limit(oracle):
+    # float factors round the FP64 ones; MAX_SWEEPS = 1 stops after one sweep
+    K = laplacian_2d(6)
+    Ap, Ai, Ax = upper_csc(K)
+    r64 = oracle.fgpilu(Ap, Ai, Ax, uf=np.float64, ilu_k=1, max_sweeps=10)
+    r32 = oracle.fgpilu(Ap, Ai, Ax, uf=np.float32, ilu_k=1, max_sweeps=1)
+    assert r32[5]["sweeps"] == 1
+    np.testing.assert_allclose(r32[2], r64[2], rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(r32[3], r64[3], rtol=1e-6, atol=1e-6)
+    assert np.all(r32[2] == r32[2].astype(np.float32))
+
+
+def test_gmres_ilu_ir_cfg0(oracle):
+    # the reference's shipped path (Compute with ILU_K = 0, example.cpp:144-147)
+    # on the small KKT system: FP32 ILU(0) preconditioned GMRES-IR
+    import mpir_amd
+    P = mpir_amd.kkt_config(0)
+    rc, x, info = oracle.gmres_ilu_ir(P.Ap, P.Ai, P.Ax, P.b, uf=np.float32, ilu_k=0, max_sweeps=5, tol=1e-12)
+    assert rc == 0 and info["sweeps"] == 2
+    assert info["precond"] == info["gmres_iterations"] + info["ir_iterations"] + 1
+    assert np.abs(x - P.x_true).max() < 1e-6
