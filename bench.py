@@ -53,8 +53,10 @@ def backward_error(P, x) -> float:
     return float(np.abs(r).max() / (knorm * np.abs(x).max() + np.abs(P.b).max()))
 
 
-def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x, ordering):
+def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x, ordering, ilu=None):
     s = mpir.GmresLDLIR(uf, ordering=ordering)
+    if ilu is not None:
+        s.SetPreconditioner(mpir.PREC_FGPILU, *ilu)
     s.SetTolerance(tol)
     t0 = time.perf_counter()
     s.Analyze(P.Ap, P.Ai)
@@ -155,6 +157,51 @@ def precision_sweep(mpir, k: int, tol: float) -> int:
                             "fwd_err_fp64": float(np.abs(x - x64).max() / np.abs(x64).max()),
                             "factor_ms": round(st["factor_ms"], 3), "solve_ms": round(st["solve_ms"], 3)})
                 print(json.dumps(row), flush=True)
+    return 0
+
+
+def ilu_solve_bytes(n: int, nnz: int, s_f: int) -> int:
+    """Algorithmic bytes of one FGPILU M^{-1}: the L pass reads the column
+    pointers, row indices and L values of every pattern entry; the U pass the
+    row pointers, (column, entry) pairs and U values of the strictly-upper
+    entries and 1 / U_jj; x is read and written by both passes."""
+    strict = nnz - n
+    return (4 + s_f) * nnz + (8 + s_f) * strict + 8 * (n + 1) + s_f * n + 4 * 8 * n
+
+
+def fgpilu_form(mpir, torch, args) -> int:
+    """The reference's shipped preconditioner on the benchmark workload:
+    GmresLDLIR::Compute's FGPILU (ILU(k) pattern of the matrix in its given
+    order, MAX_SWEEPS = 5 as examples/example.cpp:144-147 calls it) under the
+    same GMRES-IR (10 IR x 10 GMRES iterations, tol 1e-10). One JSON line per
+    (k, factor precision): factor / solve time, iterations, stop reason, the
+    backward error reached, and the M^{-1} kernel against HBM bandwidth."""
+    P = mpir.kkt_config(1, k=args.k)
+    dev = torch.device("cuda:0")
+    dev_ax = torch.from_numpy(P.Ax).to(dev)
+    dev_b = torch.from_numpy(P.b).to(dev)
+    dev_x = torch.zeros(P.n, dtype=torch.float64, device=dev)
+    for ilu_k in (0, 1):
+        for uf, name, s_f in ((np.float32, "f32", 4), (np.float64, "f64", 8)):
+            s, r = run_path(mpir, torch, P, uf, args.tol, args.steps, args.warmup, dev_ax, dev_b, dev_x,
+                            mpir.ORDER_NATURAL, ilu=(ilu_k, 5))
+            st = r["stats"]
+            pre_ms = s.time_kernels("precond", 20)
+            fac_ms = s.time_kernels("factor", 3)
+            nb = ilu_solve_bytes(P.n, st["nnz_l"], s_f)
+            row = {"workload": f"KKT 3D {args.k}^3 (n={P.n}), natural order", "preconditioner": f"FGPILU ILU({ilu_k})",
+                   "factor": name, "max_sweeps": 5, "sweeps": st["ilu_sweeps"],
+                   "nl_res0": st["ilu_nl_residual0"], "nl_res": st["ilu_nl_residual"],
+                   "pattern_nnz": st["nnz_l"], "levels": st["nlevels"], "analyze_s": round(r["analyze_s"], 3),
+                   "ms_per_solve": round(1e3 * r["total_s"] / args.steps, 3),
+                   "factor_ms": round(st["factor_ms"], 3), "solve_ms": round(st["solve_ms"], 3),
+                   "ir_iterations": st["ir_iterations"], "gmres_iterations": st["gmres_iterations"],
+                   "stop_reason": st["stop_reason"], "bwd_err": backward_error(P, r["x"]),
+                   "positive_pivots": st["positive_pivots"], "n1": P.n1,
+                   "kernels": {"factor_ms": round(fac_ms, 4), "precond_ms": round(pre_ms, 4),
+                               "precond_GBs": round(nb / (pre_ms * 1e-3) / 1e9, 2),
+                               "precond_frac_hbm": round(nb / (pre_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+            print(json.dumps(row), flush=True)
     return 0
 
 
@@ -386,6 +433,8 @@ def main(argv=None) -> int:
     ap.add_argument("--nsys", type=int, default=512, help="systems of the batched form")
     ap.add_argument("--nrhs", type=int, default=8, help="right-hand sides per system of the batched form")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--fgpilu", action="store_true",
+                    help="the reference's shipped FGPILU preconditioner on the workload (one JSON line per case)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only a 1 GiB device-to-device copy (x3): the reference kernel for HBM counter scaling")
     ap.add_argument("--serve", action="store_true",
@@ -420,6 +469,8 @@ def main(argv=None) -> int:
         return precision_sweep(mpir, args.k, args.tol)
     if args.batched:
         return batched(mpir, torch, args)
+    if args.fgpilu:
+        return fgpilu_form(mpir, torch, args)
     if args.serve:
         return serving(mpir, args)
     log(f"generating the {args.k}^3 KKT workload")
