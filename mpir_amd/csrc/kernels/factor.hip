@@ -221,8 +221,12 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
                                                               bool refactor) {
   constexpr int NB = kPanelNB;
   constexpr int CW = NB / (kPanelThreads / 64);  // columns per wave (4)
-  __shared__ T colb[2][CW][NB];   // the pivot block's columns, double-buffered
-  __shared__ T gcolb[2][CW][NB];  // the same columns of the identity rows
+  // per block step J: the multipliers of every row of the block (mrs), of
+  // every identity row (gms) and d_t times the multipliers of every column
+  // (dms), double-buffered
+  __shared__ __align__(16) T mrs[2][NB][CW];
+  __shared__ __align__(16) T gms[2][NB][CW];
+  __shared__ __align__(16) T dms[2][NB][CW];
   __shared__ __align__(16) T Zs[NB][NB];   // Z, row-major
   __shared__ __align__(16) T Xs[NB][NB];   // the task's rows, transposed: Xs[t][row]
   const Task4 tk = tasks[blockIdx.x];
@@ -268,103 +272,104 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
     fa[u] = !in ? T(0) : (refactor || lane != c) ? v : dc;
     ga[u] = (lane == c && c < nbk) ? T(1) : T(0);
   }
-  // Blocked right-looking elimination, CW columns per barrier: wave J owns
-  // columns CW J .. CW J + CW - 1 and publishes them; every thread then
-  // factors the CW x CW pivot block itself, forms the multipliers of its
-  // row and of its columns by substitution, and applies the rank-CW update.
+  // Blocked right-looking elimination, CW columns per barrier. Wave J owns
+  // columns CW J .. CW J + CW - 1; when they are final it factors the CW x CW
+  // pivot block (read across its lanes), forms the multipliers of every row
+  // (its lanes are the rows) and of every identity row, and publishes them
+  // with d_t times the multipliers of every column, which by symmetry are
+  // those of the row of the same index. After the barrier the waves right of
+  // it apply the rank-CW update to their columns.
   const int nstep = (nbk + CW - 1) / CW;
   for (int J = 0; J < nstep; ++J) {
     const int b = J & 1;
     const int j0 = CW * J;
     if (w == J) {
+      bool valid[CW];
+#pragma unroll
+      for (int t = 0; t < CW; ++t) valid[t] = j0 + t < nbk;
+      // pivot block L_B D_B L_B^T; dl[t][q] = d_q l_tq, dinv = 1 / d
+      T d[CW], dinv[CW], l[CW][CW], dl[CW][CW];
 #pragma unroll
       for (int t = 0; t < CW; ++t) {
-        colb[b][t][lane] = fa[t];
-        gcolb[b][t][lane] = ga[t];
-      }
-    }
-    __syncthreads();
-    bool valid[CW];
-#pragma unroll
-    for (int t = 0; t < CW; ++t) valid[t] = j0 + t < nbk;
-    // pivot block L_B D_B L_B^T; dl[t][q] = d_q l_tq, dinv = 1 / d
-    T d[CW], dinv[CW], l[CW][CW], dl[CW][CW];
-#pragma unroll
-    for (int t = 0; t < CW; ++t) {
-      T dt = colb[b][t][j0 + t];
-      if (refactor) {
-#pragma unroll
-        for (int q = 0; q < t; ++q) dt -= l[t][q] * dl[t][q];
-      }
-      d[t] = valid[t] ? dt : T(1);
-      dinv[t] = T(1) / d[t];
-#pragma unroll
-      for (int u = t + 1; u < CW; ++u) {
-        T v = colb[b][t][j0 + u];
+        T dt = lane_value(fa[t], j0 + t);
         if (refactor) {
 #pragma unroll
-          for (int q = 0; q < t; ++q) v -= l[u][q] * dl[t][q];
-          v *= dinv[t];
+          for (int q = 0; q < t; ++q) dt -= l[t][q] * dl[t][q];
         }
-        l[u][t] = (valid[u] && valid[t]) ? v : T(0);
-        dl[u][t] = d[t] * l[u][t];
-      }
-    }
-    // multipliers of this thread's row and identity row
-    T mr[CW], gm[CW];
+        d[t] = valid[t] ? dt : T(1);
+        dinv[t] = T(1) / d[t];
 #pragma unroll
-    for (int t = 0; t < CW; ++t) {
-      T v = colb[b][t][lane];
-      T g = gcolb[b][t][lane];
-#pragma unroll
-      for (int q = 0; q < t; ++q) {
-        if (refactor) v -= mr[q] * dl[t][q];
-        g -= gm[q] * dl[t][q];
-      }
-      mr[t] = refactor ? v * dinv[t] : v;
-      gm[t] = g * dinv[t];
-    }
-    // the owner keeps its columns of L_kk (and D on the diagonal) in LDS; its
-    // stores are issued after the loop (a barrier waits for pending stores)
-    if (owner && w == 0) {
-      const int s0 = lane - j0;
-#pragma unroll
-      for (int t = 0; t < CW; ++t) {
-        T lv = T(0);
-#pragma unroll
-        for (int u = t + 1; u < CW; ++u) lv = s0 == u ? l[u][t] : lv;
-        Xs[j0 + t][lane] = s0 < t ? T(0) : s0 == t ? d[t] : s0 < CW ? lv : mr[t];
-      }
-    }
-    // rank-CW update of this thread's entries right of the block
-#pragma unroll
-    for (int u = 0; u < CW; ++u) {
-      const int c = c0w + u;
-      if (c >= j0 + CW && c < nbk) {
-        T mc[CW];
-        T fu = T(0), gu = T(0);
-#pragma unroll
-        for (int t = 0; t < CW; ++t) {
-          T v = colb[b][t][c];
+        for (int u = t + 1; u < CW; ++u) {
+          T v = lane_value(fa[t], min(j0 + u, NB - 1));
           if (refactor) {
 #pragma unroll
-            for (int q = 0; q < t; ++q) v -= mc[q] * dl[t][q];
+            for (int q = 0; q < t; ++q) v -= l[u][q] * dl[t][q];
             v *= dinv[t];
           }
-          mc[t] = v;
-          const T dm = d[t] * mc[t];
-          fu += mr[t] * dm;
-          gu += gm[t] * dm;
+          l[u][t] = (valid[u] && valid[t]) ? v : T(0);
+          dl[u][t] = d[t] * l[u][t];
         }
-        if (refactor && lane >= c) fa[u] -= fu;
-        ga[u] -= gu;
       }
-    }
-    // the identity rows' entries in the block columns are final: the multipliers
-    if (w == J) {
+      // multipliers of this lane's row and identity row
+      T mr[CW], gm[CW];
+#pragma unroll
+      for (int t = 0; t < CW; ++t) {
+        T v = fa[t];
+        T g = ga[t];
+#pragma unroll
+        for (int q = 0; q < t; ++q) {
+          if (refactor) v -= mr[q] * dl[t][q];
+          g -= gm[q] * dl[t][q];
+        }
+        mr[t] = refactor ? v * dinv[t] : v;
+        gm[t] = g * dinv[t];
+      }
+#pragma unroll
+      for (int t = 0; t < CW; ++t) {
+        mrs[b][lane][t] = mr[t];
+        gms[b][lane][t] = gm[t];
+        dms[b][lane][t] = d[t] * mr[t];
+      }
+      // the owner keeps its columns of L_kk (and D on the diagonal) in LDS;
+      // the identity rows' entries in the block columns are final
+      if (owner) {
+        const int s0 = lane - j0;
+#pragma unroll
+        for (int t = 0; t < CW; ++t) {
+          T lv = T(0);
+#pragma unroll
+          for (int u = t + 1; u < CW; ++u) lv = s0 == u ? l[u][t] : lv;
+          Xs[j0 + t][lane] = s0 < t ? T(0) : s0 == t ? d[t] : s0 < CW ? lv : mr[t];
+        }
+      }
 #pragma unroll
       for (int t = 0; t < CW; ++t)
         if (valid[t]) ga[t] = gm[t];
+    }
+    __syncthreads();
+    // rank-CW update of this thread's entries right of the block
+    if (w > J) {
+      T mr[CW], gm[CW];
+#pragma unroll
+      for (int t = 0; t < CW; ++t) {
+        mr[t] = mrs[b][lane][t];
+        gm[t] = gms[b][lane][t];
+      }
+#pragma unroll
+      for (int u = 0; u < CW; ++u) {
+        const int c = c0w + u;
+        T fu = T(0), gu = T(0);
+#pragma unroll
+        for (int t = 0; t < CW; ++t) {
+          const T dm = dms[b][c][t];
+          fu += mr[t] * dm;
+          gu += gm[t] * dm;
+        }
+        if (c < nbk) {
+          if (refactor && lane >= c) fa[u] -= fu;
+          ga[u] -= gu;
+        }
+      }
     }
   }
   if (owner) {

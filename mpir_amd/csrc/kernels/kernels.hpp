@@ -12,6 +12,19 @@
 
 namespace mpir {
 
+// value of v in lane j of the wavefront (j uniform across the wave)
+template <class T>
+__device__ __forceinline__ T lane_value(T v, int j) {
+  if constexpr (sizeof(T) == 8) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(b), j);
+    const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(b >> 32), j);
+    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(unsigned, v), j));
+  }
+}
+
 // Device view of a factorization: pattern arrays of the plan plus the
 // numeric storage in factor precision T.
 template <class T>

@@ -101,19 +101,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// value of v in lane j (j uniform across the wave)
-template <class T>
-__device__ __forceinline__ T lane_value(T v, int j) {
-  if constexpr (sizeof(T) == 8) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(b), j);
-    const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(b >> 32), j);
-    return __builtin_bit_cast(T, (static_cast<unsigned long long>(hi) << 32) | lo);
-  } else {
-    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(unsigned, v), j));
-  }
-}
-
 // sum_{k < n} p[k * stride], added in order; the loads go out 8 at a time
 template <class TX>
 __device__ __forceinline__ TX sum_strided(const TX* p, int n, int stride) {
