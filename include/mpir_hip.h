@@ -81,6 +81,25 @@ int64_t mpir_qdldl_factor_f32_i64(int64_t n, const int64_t* Ap, const int64_t* A
                                   int64_t* Li, float* Lx, float* D, float* Dinv, const int64_t* Lnz,
                                   const int64_t* etree, unsigned char* bwork, int64_t* iwork, float* fwork);
 
+/* QDLDL_factor<INT,PREC> with a 16-bit PREC: Ax, Lx, D and Dinv hold IEEE
+ * binary16 (f16) or bfloat16 (bf16) bit patterns. The factor is computed
+ * with FP32 arithmetic, every matrix value and every entry of L rounded to
+ * the format as it is stored (round to nearest even); D and Dinv are
+ * rounded on output. Returns the number of positive entries of D, or -1 on
+ * a zero pivot or a value outside the format's range. */
+int32_t mpir_qdldl_factor_f16_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const uint16_t* Ax, int32_t* Lp,
+                                  int32_t* Li, uint16_t* Lx, uint16_t* D, uint16_t* Dinv, const int32_t* Lnz,
+                                  const int32_t* etree, unsigned char* bwork, int32_t* iwork, uint16_t* fwork);
+int32_t mpir_qdldl_factor_bf16_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const uint16_t* Ax, int32_t* Lp,
+                                   int32_t* Li, uint16_t* Lx, uint16_t* D, uint16_t* Dinv, const int32_t* Lnz,
+                                   const int32_t* etree, unsigned char* bwork, int32_t* iwork, uint16_t* fwork);
+int64_t mpir_qdldl_factor_f16_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const uint16_t* Ax, int64_t* Lp,
+                                  int64_t* Li, uint16_t* Lx, uint16_t* D, uint16_t* Dinv, const int64_t* Lnz,
+                                  const int64_t* etree, unsigned char* bwork, int64_t* iwork, uint16_t* fwork);
+int64_t mpir_qdldl_factor_bf16_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const uint16_t* Ax, int64_t* Lp,
+                                   int64_t* Li, uint16_t* Lx, uint16_t* D, uint16_t* Dinv, const int64_t* Lnz,
+                                   const int64_t* etree, unsigned char* bwork, int64_t* iwork, uint16_t* fwork);
+
 /* Replaces QDLDL_solve<INT,FACTPREC,XPREC> (include/MPIR/qdldl.hpp:417-432):
  * x := (L D L^T)^{-1} x with L in FACTPREC and arithmetic in XPREC. The
  * solve runs on the GPU. Returns 0, or a negative status (the reference's
@@ -97,6 +116,25 @@ int64_t mpir_qdldl_solve_f32_f64_i64(int64_t n, const int64_t* Lp, const int64_t
                                      const float* Dinv, double* x);
 int64_t mpir_qdldl_solve_f32_f32_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const float* Lx,
                                      const float* Dinv, float* x);
+/* QDLDL_solve with a 16-bit FACTPREC (bit patterns as above), XPREC float or
+ * double: the factor's values are exact in FP32 and the solve computes in
+ * XPREC arithmetic as the FP32 forms do */
+int32_t mpir_qdldl_solve_f16_f32_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const uint16_t* Lx,
+                                     const uint16_t* Dinv, float* x);
+int32_t mpir_qdldl_solve_f16_f64_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const uint16_t* Lx,
+                                     const uint16_t* Dinv, double* x);
+int64_t mpir_qdldl_solve_f16_f32_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const uint16_t* Lx,
+                                     const uint16_t* Dinv, float* x);
+int64_t mpir_qdldl_solve_f16_f64_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const uint16_t* Lx,
+                                     const uint16_t* Dinv, double* x);
+int32_t mpir_qdldl_solve_bf16_f32_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const uint16_t* Lx,
+                                      const uint16_t* Dinv, float* x);
+int32_t mpir_qdldl_solve_bf16_f64_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const uint16_t* Lx,
+                                      const uint16_t* Dinv, double* x);
+int64_t mpir_qdldl_solve_bf16_f32_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const uint16_t* Lx,
+                                      const uint16_t* Dinv, float* x);
+int64_t mpir_qdldl_solve_bf16_f64_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const uint16_t* Lx,
+                                      const uint16_t* Dinv, double* x);
 
 /* ------------------------------------------------------------------ LUIR
  * Replaces LUIR_QDLDL<FACTPREC, double, double, double, INT> and its Stats

@@ -75,13 +75,28 @@ def QDLDL_etree(n, Ap, Ai, work, Lnz, etree) -> int:
     return _check(int(fn(int(n), *map(_ptr, arrs))), "QDLDL_etree")
 
 
-def QDLDL_factor(n, Ap, Ai, Ax, Lp, Li, Lx, D, Dinv, Lnz, etree, bwork=None, iwork=None, fwork=None) -> int:
-    """QDLDL_factor<INT,PREC> (qdldl.hpp:172-345) on the GPU; returns #D>0 or -1 on a zero pivot."""
+def _value_suffix(vt, prec):
+    """Entry-point suffix of a value array: float64 / float32 / float16, or
+    bfloat16 bit patterns in uint16 arrays with prec='bf16'."""
+    if prec is not None:
+        if str(prec).lower() != "bf16" or vt != np.uint16:
+            raise TypeError("prec='bf16' takes uint16 arrays of bfloat16 bit patterns")
+        return "bf16"
+    suf = {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32", np.dtype(np.float16): "f16"}.get(vt)
+    if suf is None:
+        raise TypeError("values must be float64, float32, float16 (or uint16 with prec='bf16')")
+    return suf
+
+
+def QDLDL_factor(n, Ap, Ai, Ax, Lp, Li, Lx, D, Dinv, Lnz, etree, bwork=None, iwork=None, fwork=None,
+                 prec=None) -> int:
+    """QDLDL_factor<INT,PREC> (qdldl.hpp:172-345) on the GPU; returns #D>0 or -1 on a zero pivot.
+    PREC is the dtype of Ax/Lx/D/Dinv: float64, float32 or float16; bfloat16 as
+    uint16 bit patterns with prec='bf16' (16-bit factors: FP32 arithmetic,
+    values rounded to the format as they are stored)."""
     it = np.asarray(Ap).dtype
     vt = np.asarray(Ax).dtype
-    if vt not in _PREC_OF:
-        raise TypeError("Ax must be float64 or float32")
-    suffix = {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32"}[vt]
+    suffix = _value_suffix(vt, prec)
     isuf = {np.dtype(np.int32): "i32", np.dtype(np.int64): "i64"}.get(it)
     if isuf is None:
         raise TypeError("index arrays must be int32 or int64")
@@ -95,12 +110,13 @@ def QDLDL_factor(n, Ap, Ai, Ax, Lp, Li, Lx, D, Dinv, Lnz, etree, bwork=None, iwo
     return _check(int(rc), "QDLDL_factor")
 
 
-def QDLDL_solve(n, Lp, Li, Lx, Dinv, x) -> None:
-    """QDLDL_solve<INT,FACTPREC,XPREC> (qdldl.hpp:417-432) on the GPU, x overwritten."""
+def QDLDL_solve(n, Lp, Li, Lx, Dinv, x, prec=None) -> None:
+    """QDLDL_solve<INT,FACTPREC,XPREC> (qdldl.hpp:417-432) on the GPU, x overwritten.
+    FACTPREC as for QDLDL_factor (prec='bf16' for bfloat16 bit patterns)."""
     it = np.asarray(Lp).dtype
     ft = np.asarray(Lx).dtype
     xt = np.asarray(x).dtype
-    key = ({np.dtype(np.float64): "f64", np.dtype(np.float32): "f32"}.get(ft),
+    key = (_value_suffix(ft, prec),
            {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32"}.get(xt),
            {np.dtype(np.int32): "i32", np.dtype(np.int64): "i64"}.get(it))
     name = f"mpir_qdldl_solve_{key[0]}_{key[1]}_{key[2]}"

@@ -53,6 +53,18 @@ SIGNATURES = {
     "mpir_qdldl_solve_f64_f64_i64": (i64, [i64, P, P, P, P, P]),
     "mpir_qdldl_solve_f32_f64_i64": (i64, [i64, P, P, P, P, P]),
     "mpir_qdldl_solve_f32_f32_i64": (i64, [i64, P, P, P, P, P]),
+    "mpir_qdldl_factor_f16_i32": (i32, [i32, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mpir_qdldl_factor_bf16_i32": (i32, [i32, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mpir_qdldl_factor_f16_i64": (i64, [i64, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mpir_qdldl_factor_bf16_i64": (i64, [i64, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mpir_qdldl_solve_f16_f32_i32": (i32, [i32, P, P, P, P, P]),
+    "mpir_qdldl_solve_f16_f64_i32": (i32, [i32, P, P, P, P, P]),
+    "mpir_qdldl_solve_f16_f32_i64": (i64, [i64, P, P, P, P, P]),
+    "mpir_qdldl_solve_f16_f64_i64": (i64, [i64, P, P, P, P, P]),
+    "mpir_qdldl_solve_bf16_f32_i32": (i32, [i32, P, P, P, P, P]),
+    "mpir_qdldl_solve_bf16_f64_i32": (i32, [i32, P, P, P, P, P]),
+    "mpir_qdldl_solve_bf16_f32_i64": (i64, [i64, P, P, P, P, P]),
+    "mpir_qdldl_solve_bf16_f64_i64": (i64, [i64, P, P, P, P, P]),
     "mpir_luir_qdldl_f64_i32": (i32, [i32, P, P, P, P, P, P, i32]),
     "mpir_luir_qdldl_f32_i32": (i32, [i32, P, P, P, P, P, P, i32]),
     "mpir_luir_qdldl_f64_i64": (i64, [i64, P, P, P, P, P, P, i32]),

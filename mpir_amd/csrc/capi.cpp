@@ -1,4 +1,6 @@
 // extern "C" entry points of libmpir_hip.so (declared in include/mpir_hip.h).
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -50,7 +52,7 @@ struct QdldlEngine {
   int32_t n = -1;
 };
 std::mutex g_engine_mu;
-QdldlEngine g_factor_engine[2];
+QdldlEngine g_factor_engine[4];
 QdldlEngine g_luir_engine[2];
 QdldlEngine g_solve_engine[2];
 
@@ -96,12 +98,42 @@ int32_t etree_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, int32_t* wor
   return 0;
 }
 
+// A 16-bit value as its IEEE binary16 (P = kF16) or bfloat16 (P = kBF16)
+// bit pattern, the storage of the 16-bit QDLDL entry points.
+template <int P>
+struct Bits16 {
+  uint16_t v;
+};
+static_assert(sizeof(Bits16<kF16>) == 2, "Bits16 is a bare uint16_t");
+template <class T>
+struct ValueIO {
+  static constexpr int prec = std::is_same<T, double>::value ? kF64 : kF32;
+  static double get(T x) { return static_cast<double>(x); }
+  static T put(double v) { return static_cast<T>(v); }
+};
+template <>
+struct ValueIO<Bits16<kF16>> {
+  static constexpr int prec = kF16;
+  static double get(Bits16<kF16> x) { return __half2float(__builtin_bit_cast(__half, x.v)); }
+  static Bits16<kF16> put(double v) { return {__builtin_bit_cast(uint16_t, __float2half(static_cast<float>(v)))}; }
+};
+template <>
+struct ValueIO<Bits16<kBF16>> {
+  static constexpr int prec = kBF16;
+  static double get(Bits16<kBF16> x) { return __bfloat162float(__builtin_bit_cast(__hip_bfloat16, x.v)); }
+  static Bits16<kBF16> put(double v) {
+    return {__builtin_bit_cast(uint16_t, __float2bfloat16(static_cast<float>(v)))};
+  }
+};
+
 // Factor the given (already ordered) upper CSC on the GPU with QDLDL's
-// semantics and write QDLDL's CSC output.
+// semantics and write QDLDL's CSC output. 16-bit factors are computed with
+// FP32 arithmetic, every matrix value and L entry rounded to the format as
+// it is stored; D and Dinv (FP32 inside) are rounded on output.
 template <class T>
 int64_t factor_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const T* Ax, int64_t* Lp64, int32_t* Li, T* Lx,
                     T* D, T* Dinv, const int32_t* Lnz, const int32_t* etree) {
-  const int prec = std::is_same<T, double>::value ? kF64 : kF32;
+  const int prec = ValueIO<T>::prec;
   std::lock_guard<std::mutex> lock(g_engine_mu);
   QdldlEngine& E = g_factor_engine[prec];
   if (!E.solver) {
@@ -123,16 +155,19 @@ int64_t factor_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const T* Ax
   etree_counts(n, Ap, Ai, work.data(), lnz.data(), par.data(), false);
   for (int32_t j = 0; j < n; ++j)
     if (lnz[j] != Lnz[j] || par[j] != etree[j]) throw Error(kErrInvalid, "Lnz/etree do not match the pattern");
-  std::vector<double> ax(Ax, Ax + nnz);
+  std::vector<double> ax(nnz);
+  for (int64_t q = 0; q < nnz; ++q) ax[q] = ValueIO<T>::get(Ax[q]);
   try {
     E.solver->factor(ax.data(), false);
   } catch (const Error& e) {
-    if (e.code == kErrNumeric) return -1;  // QDLDL: exact zero pivot
+    if (e.code == kErrNumeric) return -1;  // QDLDL: exact zero pivot (16-bit: also a value out of range)
     throw;
   }
   E.solver->export_factor(Lp64, Li, Lx, D, Dinv, prec);
   return E.solver->stats().positive_pivots;
 }
+
+
 
 template <class TF, class TX>
 int solve_impl(int32_t n, const int64_t* Lp, const int32_t* Li, const TF* Lx, const TF* Dinv, TX* x) {
@@ -271,6 +306,69 @@ mpir::PatternCache& pattern_cache() {
 }
 }  // namespace
 
+// The int32 / int64 forms of the 16-bit QDLDL entry points (the int64 form
+// narrows every index to int32, as the other int64 entry points do).
+template <class B>
+int32_t factor16_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const B* Ax, int32_t* Lp, int32_t* Li, B* Lx,
+                     B* D, B* Dinv, const int32_t* Lnz, const int32_t* etree) {
+  int64_t ret = 0;
+  const int rc = guarded([&] {
+    MPIR_CHECK(n >= 1 && Ap && Ai && Ax && Lp && Li && Lx && D && Dinv && Lnz && etree, "bad factor arguments");
+    std::vector<int64_t> lp(static_cast<size_t>(n) + 1);
+    ret = factor_impl<B>(n, Ap, Ai, Ax, lp.data(), Li, Lx, D, Dinv, Lnz, etree);
+    for (int32_t k = 0; k <= n; ++k) Lp[k] = static_cast<int32_t>(lp[k]);
+    return 0;
+  });
+  return rc != 0 ? rc : static_cast<int32_t>(ret);
+}
+template <class B>
+int64_t factor16_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const B* Ax, int64_t* Lp, int64_t* Li, B* Lx,
+                     B* D, B* Dinv, const int64_t* Lnz, const int64_t* etree) {
+  int64_t ret = 0;
+  const int rc = guarded([&] {
+    int32_t n32;
+    MPIR_CHECK(narrow(n, &n32) && n32 >= 1 && Ap && Ai && Ax && Lp && Li && Lx && D && Dinv && Lnz && etree,
+               "bad factor arguments");
+    std::vector<int32_t> p = narrow_vec(Ap, n + 1), i = narrow_vec(Ai, Ap[n]);
+    std::vector<int32_t> l = narrow_vec(Lnz, n), e = narrow_vec(etree, n);
+    int64_t lnz = 0;
+    for (int64_t k = 0; k < n; ++k) lnz += Lnz[k];
+    std::vector<int32_t> li(static_cast<size_t>(std::max<int64_t>(lnz, 1)));
+    ret = factor_impl<B>(n32, p.data(), i.data(), Ax, Lp, li.data(), Lx, D, Dinv, l.data(), e.data());
+    for (int64_t k = 0; k < lnz; ++k) Li[k] = li[k];
+    return 0;
+  });
+  return rc != 0 ? rc : ret;
+}
+// QDLDL_solve with a 16-bit factor: its values are exact in FP32, so the
+// solve runs on the FP32 engine with the arithmetic of XPREC.
+template <int P, class TX>
+int solve16(int32_t n, const int64_t* Lp, const int32_t* Li, const uint16_t* Lx, const uint16_t* Dinv, TX* x) {
+  MPIR_CHECK(n >= 1 && Lp && Li && Lx && Dinv && x, "bad solve arguments");
+  const int64_t nnz = Lp[n];
+  std::vector<float> lx(static_cast<size_t>(std::max<int64_t>(nnz, 1))), di(n);
+  for (int64_t q = 0; q < nnz; ++q) lx[q] = static_cast<float>(ValueIO<Bits16<P>>::get(Bits16<P>{Lx[q]}));
+  for (int32_t i = 0; i < n; ++i) di[i] = static_cast<float>(ValueIO<Bits16<P>>::get(Bits16<P>{Dinv[i]}));
+  return solve_impl<float, TX>(n, Lp, Li, lx.data(), di.data(), x);
+}
+template <int P, class TX>
+int32_t solve16_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const uint16_t* Lx, const uint16_t* Dinv, TX* x) {
+  return guarded([&] {
+    MPIR_CHECK(n >= 1 && Lp, "bad solve arguments");
+    std::vector<int64_t> lp(Lp, Lp + n + 1);
+    return solve16<P, TX>(n, lp.data(), Li, Lx, Dinv, x);
+  });
+}
+template <int P, class TX>
+int64_t solve16_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const uint16_t* Lx, const uint16_t* Dinv, TX* x) {
+  return guarded([&] {
+    int32_t n32;
+    MPIR_CHECK(narrow(n, &n32) && n32 >= 1 && Lp && Li, "bad solve arguments");
+    std::vector<int32_t> li = narrow_vec(Li, Lp[n]);
+    return solve16<P, TX>(n32, Lp, li.data(), Lx, Dinv, x);
+  });
+}
+
 extern "C" {
 
 const char* mpir_last_error(void) { return g_last_error.c_str(); }
@@ -385,6 +483,44 @@ MPIR_FACTOR_I32(f64, double)
 MPIR_FACTOR_I32(f32, float)
 MPIR_FACTOR_I64(f64, double)
 MPIR_FACTOR_I64(f32, float)
+
+// 16-bit factors and solves: the values are IEEE binary16 / bfloat16 bit
+// patterns
+#define MPIR_FACTOR16(SUF, P)                                                                                         \
+  int32_t mpir_qdldl_factor_##SUF##_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const uint16_t* Ax,          \
+                                        int32_t* Lp, int32_t* Li, uint16_t* Lx, uint16_t* D, uint16_t* Dinv,          \
+                                        const int32_t* Lnz, const int32_t* etree, unsigned char*, int32_t*,          \
+                                        uint16_t*) {                                                                   \
+    using B = Bits16<P>;                                                                                              \
+    return factor16_i32<B>(n, Ap, Ai, reinterpret_cast<const B*>(Ax), Lp, Li, reinterpret_cast<B*>(Lx),             \
+                           reinterpret_cast<B*>(D), reinterpret_cast<B*>(Dinv), Lnz, etree);                         \
+  }                                                                                                                    \
+  int64_t mpir_qdldl_factor_##SUF##_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, const uint16_t* Ax,          \
+                                        int64_t* Lp, int64_t* Li, uint16_t* Lx, uint16_t* D, uint16_t* Dinv,          \
+                                        const int64_t* Lnz, const int64_t* etree, unsigned char*, int64_t*,          \
+                                        uint16_t*) {                                                                   \
+    using B = Bits16<P>;                                                                                              \
+    return factor16_i64<B>(n, Ap, Ai, reinterpret_cast<const B*>(Ax), Lp, Li, reinterpret_cast<B*>(Lx),             \
+                           reinterpret_cast<B*>(D), reinterpret_cast<B*>(Dinv), Lnz, etree);                         \
+  }                                                                                                                    \
+  int32_t mpir_qdldl_solve_##SUF##_f32_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const uint16_t* Lx,       \
+                                           const uint16_t* Dinv, float* x) {                                          \
+    return solve16_i32<P, float>(n, Lp, Li, Lx, Dinv, x);                                                            \
+  }                                                                                                                    \
+  int32_t mpir_qdldl_solve_##SUF##_f64_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const uint16_t* Lx,       \
+                                           const uint16_t* Dinv, double* x) {                                         \
+    return solve16_i32<P, double>(n, Lp, Li, Lx, Dinv, x);                                                           \
+  }                                                                                                                    \
+  int64_t mpir_qdldl_solve_##SUF##_f32_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const uint16_t* Lx,       \
+                                           const uint16_t* Dinv, float* x) {                                          \
+    return solve16_i64<P, float>(n, Lp, Li, Lx, Dinv, x);                                                            \
+  }                                                                                                                    \
+  int64_t mpir_qdldl_solve_##SUF##_f64_i64(int64_t n, const int64_t* Lp, const int64_t* Li, const uint16_t* Lx,       \
+                                           const uint16_t* Dinv, double* x) {                                         \
+    return solve16_i64<P, double>(n, Lp, Li, Lx, Dinv, x);                                                           \
+  }
+MPIR_FACTOR16(f16, kF16)
+MPIR_FACTOR16(bf16, kBF16)
 
 #define MPIR_SOLVE_I32(SUF, TF, TX)                                                                                  \
   int32_t mpir_qdldl_solve_##SUF##_i32(int32_t n, const int32_t* Lp, const int32_t* Li, const TF* Lx,               \

@@ -1025,7 +1025,9 @@ void GmresLdl::export_factor(int64_t* Lp, int32_t* Li, void* Lx, void* Dv, void*
   };
   auto put = [&](void* dst, int64_t i, double v) {
     if (prec == kF64) static_cast<double*>(dst)[i] = v;
-    else static_cast<float*>(dst)[i] = static_cast<float>(v);
+    else if (prec == kF32) static_cast<float*>(dst)[i] = static_cast<float>(v);
+    else if (prec == kF16) static_cast<__half*>(dst)[i] = __float2half(static_cast<float>(v));
+    else static_cast<__hip_bfloat16*>(dst)[i] = __float2bfloat16(static_cast<float>(v));
   };
   // true pattern of each column (relaxed blocks also store explicit zeros)
   std::vector<int64_t> TLp;

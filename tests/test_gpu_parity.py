@@ -89,6 +89,62 @@ def test_qdldl_solve_dropin(gpu, case):
     np.testing.assert_allclose(x, g["f32_solve_f64"], rtol=1e-11, atol=1e-12 * np.abs(x).max())
 
 
+def to_bf16_bits(v):
+    """float64 -> bfloat16 bit patterns, round to nearest even (no NaNs here)."""
+    b = np.asarray(v, np.float32).view(np.uint32).astype(np.uint64)
+    return ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def from_bf16_bits(b):
+    return (np.asarray(b, np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("case", QDLDL_CASES)
+@pytest.mark.parametrize("prec", ["f16", "bf16"])
+def test_qdldl_factor_16bit_dropin(gpu, oracle, case, prec):
+    # QDLDL_factor<INT, 16-bit PREC>: FP32 arithmetic, values stored rounded.
+    # Pinned against the oracle's FP64 factor of the same rounded matrix
+    # (pattern and inertia exact, values to the format's accuracy), and the
+    # 16-bit solve against the oracle's FP64 solve with the same factor values.
+    g = golden(case)
+    n = len(g["Ap"]) - 1
+    lnz = int(g["etree_sum"])
+    if prec == "f16":
+        Ax = g["Ax"].astype(np.float16)
+        ax64, up, kw, vt = Ax.astype(np.float64), (lambda a: np.asarray(a, np.float64)), {}, np.float16
+    else:
+        Ax = to_bf16_bits(g["Ax"])
+        ax64, up, kw, vt = from_bf16_bits(Ax), from_bf16_bits, {"prec": "bf16"}, np.uint16
+    Lp = np.zeros(n + 1, np.int32)
+    Li = np.zeros(lnz, np.int32)
+    Lx = np.zeros(lnz, vt)
+    D = np.zeros(n, vt)
+    Dinv = np.zeros(n, vt)
+    ret = mpir_amd.QDLDL_factor(n, g["Ap"], g["Ai"], Ax, Lp, Li, Lx, D, Dinv, g["Lnz"], g["etree"], **kw)
+    ret64, F = oracle.factor(g["Ap"], g["Ai"], ax64, np.float64)
+    if prec == "f16" and max(np.abs(F["Lx"]).max(), np.abs(F["D"]).max()) > 0.9 * 65504:
+        # a pivot of this system (|D| up to 1.1e5 for the k = 6 case) is
+        # outside the FP16 range: the factorization reports failure
+        assert ret == -1
+        return
+    assert ret == ret64
+    np.testing.assert_array_equal(Lp, F["Lp"])
+    np.testing.assert_array_equal(Li, F["Li"])
+    tol = 2e-2 if prec == "f16" else 1e-1
+    np.testing.assert_allclose(up(Lx), F["Lx"], rtol=0, atol=tol * np.abs(F["Lx"]).max())
+    np.testing.assert_allclose(up(D), F["D"], rtol=tol)
+    np.testing.assert_allclose(up(Dinv), 1.0 / up(D), rtol=2 ** -7 if prec == "bf16" else 2 ** -10)
+    # QDLDL_solve with the 16-bit factor, x in FP64 and FP32
+    b = g["b"]
+    x = b.copy()
+    mpir_amd.QDLDL_solve(n, Lp, Li, Lx, Dinv, x, **kw)
+    xo = oracle.solve(F["Lp"], F["Li"], up(Lx), up(Dinv), b)
+    np.testing.assert_allclose(x, xo, rtol=1e-10, atol=1e-12 * np.abs(xo).max())
+    x32 = b.astype(np.float32)
+    mpir_amd.QDLDL_solve(n, Lp, Li, Lx, Dinv, x32, **kw)
+    np.testing.assert_allclose(x32, xo, rtol=1e-4, atol=1e-5 * np.abs(xo).max())
+
+
 def test_qdldl_zero_pivot_dropin(gpu):
     Ap = np.array([0, 1, 3], np.int32)
     Ai = np.array([0, 0, 1], np.int32)
