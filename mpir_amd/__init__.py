@@ -339,6 +339,10 @@ class BatchLDLIR:
 
     def __init__(self, UF=np.float32, ordering: int = ORDER_ND, scaling: int = SCALE_DIAG, threads: int = 8):
         self._s = GmresLDLIR(UF, ordering=ordering, scaling=scaling)
+        # the level schedule of the triangular solves: on the forest of a
+        # batch it is faster than the dependency-driven one, and it never
+        # waits across workgroups (SetSolveSchedule changes it)
+        self._s.SetSolveSchedule(SCHED_LEVELS)
         self.threads = int(threads)
         self._n: list[int] = []
         self._keep = []
@@ -352,6 +356,10 @@ class BatchLDLIR:
 
     def SetTolerance(self, tol: float) -> None:
         self._s.SetTolerance(tol)
+
+    def SetSolveSchedule(self, schedule: int) -> None:
+        """SCHED_LEVELS (the default for batches) or SCHED_DAG."""
+        self._s.SetSolveSchedule(schedule)
 
     def Analyze(self, patterns) -> None:
         """patterns: list of (Ap, Ai), upper CSC of each system."""

@@ -83,6 +83,11 @@ Server::Server(const ServeOptions& opt) : opt_(opt) {
     auto s = std::make_unique<GmresLdl>(opt.uf, kF64, kF64);
     s->set_tol(opt.tol);
     s->set_ordering(opt.ordering);
+    // Lanes run concurrently on their own streams. The level schedule of the
+    // triangular solves never waits across workgroups, so a lane's solve
+    // cannot stall behind another lane's kernels; on joined batches it is
+    // also the faster schedule.
+    s->set_schedule(kScheduleLevels);
     s->set_max_ir(opt.max_ir);
     s->set_max_gmres(opt.max_gmres);
     solvers_.push_back(std::move(s));
