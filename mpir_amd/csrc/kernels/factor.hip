@@ -436,6 +436,15 @@ struct Mfma16<double> {
   static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
 };
 
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
+// task of workgroup b when workgroup b runs on XCD b % kXcds: XCD x takes the
+// x-th contiguous run of the list (runs differ in length by at most one)
+__device__ __forceinline__ int xcd_task(int b, int n) {
+  const int x = b % kXcds, i = b / kXcds;
+  const int q = n / kXcds, rem = n % kXcds;
+  return x * q + min(x, rem) + i;
+}
+
 // Trailing update tile: C[r0:r0+64, c0:c0+64] -= sum over the task's steps
 // of L_r D L_c^T (the nbk columns of each step), lower part only. A task
 // applies `steps` consecutive panel steps ending at k (field d = k | steps <<
@@ -450,7 +459,10 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
   using V = typename Mfma16<T>::V;
   __shared__ T As[kPanelNB][kTile + 1];  // As[t][i] = L[r0 + i, k0 + t]
   __shared__ T Bs[kPanelNB][kTile + 1];  // Bs[t][j] = L[c0 + j, k0 + t] D[k0 + t]
-  const Task4 tk = tasks[blockIdx.x];
+  // Workgroups are dealt round-robin over the XCDs (blockIdx % kXcds); each
+  // XCD takes a contiguous run of the task list instead, so tiles that share
+  // operand rows and columns meet in the same L2.
+  const Task4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int s = tk.a, r0 = tk.b, c0 = tk.c;
   const int klast = tk.d & ((1 << kUpdStepsShift) - 1);
   const int kfirst = klast - ((tk.d >> kUpdStepsShift) & ((1 << (kUpdPanelBit - kUpdStepsShift)) - 1)) + 1;
