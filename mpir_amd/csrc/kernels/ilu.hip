@@ -135,7 +135,10 @@ __global__ void __launch_bounds__(kIluThreads) k_ilu_sum(const double* __restric
 }
 
 // L y = b, rows of one level; y overwrites b (row r of L: the entries of
-// column r of U above the diagonal)
+// column r of U above the diagonal, which is the column's last entry). The
+// entries are read 8 at a time, all loads of a group in flight together,
+// and summed in order.
+constexpr int kIluGroup = 8;
 template <class T>
 __global__ void k_ilu_lower(IluDev g, const int32_t* __restrict__ rows, int cnt, const T* __restrict__ Lx,
                             double* __restrict__ x) {
@@ -143,16 +146,29 @@ __global__ void k_ilu_lower(IluDev g, const int32_t* __restrict__ rows, int cnt,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnt) return;
   const int r = rows[t];
+  const int p0 = g.Up[r], p1 = g.Up[r + 1] - 1;
   double sum = 0.0;
-  for (int idx = g.Up[r]; idx < g.Up[r + 1]; ++idx) {
-    const int c = g.Ui[idx];
-    if (c < r) sum += static_cast<double>(Lx[idx]) * x[c];
+  for (int base = p0; base < p1; base += kIluGroup) {
+    int c[kIluGroup];
+    T l[kIluGroup];
+#pragma unroll
+    for (int k = 0; k < kIluGroup; ++k) {
+      const int idx = min(base + k, p1 - 1);
+      c[k] = g.Ui[idx];
+      l[k] = Lx[idx];
+    }
+    double xv[kIluGroup];
+#pragma unroll
+    for (int k = 0; k < kIluGroup; ++k) xv[k] = x[c[k]];
+#pragma unroll
+    for (int k = 0; k < kIluGroup; ++k)
+      if (base + k < p1) sum += static_cast<double>(l[k]) * xv[k];
   }
   x[r] = x[r] - sum;
 }
 
 // U x = y, rows of one level, in place: x_r = (y_r - U_rc x_c for c > r,
-// descending) / U_rr
+// descending) / U_rr, the entries read 8 at a time
 template <class T>
 __global__ void k_ilu_upper(IluDev g, const int32_t* __restrict__ rows, int cnt, const T* __restrict__ Ux,
                             const T* __restrict__ Di, double* __restrict__ x) {
@@ -160,8 +176,26 @@ __global__ void k_ilu_upper(IluDev g, const int32_t* __restrict__ rows, int cnt,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= cnt) return;
   const int r = rows[t];
+  const int p0 = g.ur_ptr[r], p1 = g.ur_ptr[r + 1];
   double v = x[r];
-  for (int q = g.ur_ptr[r + 1] - 1; q >= g.ur_ptr[r]; --q) v -= static_cast<double>(Ux[g.ur_idx[q]]) * x[g.ur_col[q]];
+  for (int top = p1; top > p0; top -= kIluGroup) {
+    int c[kIluGroup], e[kIluGroup];
+#pragma unroll
+    for (int k = 0; k < kIluGroup; ++k) {
+      const int q = max(top - 1 - k, p0);
+      c[k] = g.ur_col[q];
+      e[k] = g.ur_idx[q];
+    }
+    double uv[kIluGroup], xv[kIluGroup];
+#pragma unroll
+    for (int k = 0; k < kIluGroup; ++k) {
+      uv[k] = static_cast<double>(Ux[e[k]]);
+      xv[k] = x[c[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kIluGroup; ++k)
+      if (top - 1 - k >= p0) v -= uv[k] * xv[k];
+  }
   x[r] = v * static_cast<double>(Di[r]);
 }
 
