@@ -212,13 +212,12 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
 // r0 = k0 owns the diagonal block and writes L_kk, D and D^{-1}; every other
 // task forms its rows of L as X Z, X = its rows of the current panel
 // (identity rows of the front, r >= m, enter at their own column as e_i).
-// refactor = false: L_kk and D are already final (a factor loaded from
+// refactor (template) = false: L_kk and D are already final (a factor loaded from
 // outside); the block is rebuilt from them and only Z is formed.
 constexpr int kPanelThreads = 1024;  // 16 waves, each owning 4 block columns
 
-template <class T, int R>
-__global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, const Task4* __restrict__ tasks,
-                                                              bool refactor) {
+template <class T, int R, bool refactor>
+__global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   constexpr int NB = kPanelNB;
   constexpr int CW = NB / (kPanelThreads / 64);  // columns per wave (4)
   // per block step J: the multipliers of every row of the block (mrs), of
@@ -349,26 +348,29 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
     __syncthreads();
     // rank-CW update of this thread's entries right of the block
     if (w > J) {
-      T mr[CW], gm[CW];
+      // every operand is read before any is used (no loads under a branch)
+      T mr[CW], gm[CW], dm[CW][CW];
 #pragma unroll
       for (int t = 0; t < CW; ++t) {
         mr[t] = mrs[b][lane][t];
         gm[t] = gms[b][lane][t];
       }
 #pragma unroll
+      for (int u = 0; u < CW; ++u)
+#pragma unroll
+        for (int t = 0; t < CW; ++t) dm[u][t] = dms[b][c0w + u][t];
+#pragma unroll
       for (int u = 0; u < CW; ++u) {
         const int c = c0w + u;
         T fu = T(0), gu = T(0);
 #pragma unroll
         for (int t = 0; t < CW; ++t) {
-          const T dm = dms[b][c][t];
-          fu += mr[t] * dm;
-          gu += gm[t] * dm;
+          fu += mr[t] * dm[u][t];
+          gu += gm[t] * dm[u][t];
         }
-        if (c < nbk) {
-          if (refactor && lane >= c) fa[u] -= fu;
-          ga[u] -= gu;
-        }
+        const bool on = c < nbk;
+        fa[u] = (on && refactor && lane >= c) ? fa[u] - fu : fa[u];
+        ga[u] = on ? ga[u] - gu : ga[u];
       }
     }
   }
@@ -584,8 +586,8 @@ void factor_levels(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const dou
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_ptr[q], p1 = dp.h_pan_ptr[q + 1];
       if (p1 > p0)
-        hipLaunchKernelGGL((k_panel_step<T, R>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a, dp.pan_task + p0,
-                           true);
+        hipLaunchKernelGGL((k_panel_step<T, R, true>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a,
+                           dp.pan_task + p0);
       const int u0 = dp.h_upd_ptr[q], u1 = dp.h_upd_ptr[q + 1];
       if (u1 > u0) hipLaunchKernelGGL((k_update_tiles<T, R>), dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
     }
@@ -637,8 +639,8 @@ void invert_diag_blocks(const FactorArgs<T>& a, const DeviceFactorPlan& dp, hipS
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_aug_ptr[q], p1 = dp.h_pan_ptr[q + 1];
       if (p1 > p0)
-        hipLaunchKernelGGL((k_panel_step<T, 0>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a, dp.pan_task + p0,
-                           false);
+        hipLaunchKernelGGL((k_panel_step<T, 0, false>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a,
+                           dp.pan_task + p0);
       const int u0 = dp.h_upd_aug_ptr[q], u1 = dp.h_upd_ptr[q + 1];
       if (u1 > u0) hipLaunchKernelGGL((k_update_tiles<T, 0>), dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
     }
