@@ -447,7 +447,8 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py measures one GPU: the solve path runs on a single device "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "(a multi-GPU run would shard whole systems, or subtrees of one system's elimination "
+              "tree, over the devices); run with --gpus 1", file=sys.stderr)
         return 2
     import torch
 
