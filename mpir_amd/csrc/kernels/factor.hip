@@ -204,6 +204,41 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
   }
 }
 
+// Matrix-core product for the trailing updates: v_mfma 16x16x4 in the
+// precision of the factor. Lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; output v (0..3) of lane l is D[row(l / 16, v)][l % 16]
+// with row = 4 (l / 16) + v for fp32 and (l / 16) + 4 v for fp64 (measured
+// on gfx950).
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <class T>
+struct Mfma16;
+template <>
+struct Mfma16<float> {
+  using V = f32x4;
+  static __device__ __forceinline__ V op(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+};
+template <>
+struct Mfma16<double> {
+  using V = f64x4;
+  static __device__ __forceinline__ V op(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+};
+
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
+// task of workgroup b when workgroup b runs on XCD b % kXcds: XCD x takes the
+// x-th contiguous run of the list (runs differ in length by at most one)
+__device__ __forceinline__ int xcd_task(int b, int n) {
+  const int x = b % kXcds, i = b / kXcds;
+  const int q = n / kXcds, rem = n % kXcds;
+  return x * q + min(x, rem) + i;
+}
+
 // ---------------------------------------------------------------- big fronts
 // Panel step k of big front s, task (s, r0, r1, k). Every workgroup factors
 // the nbk x nbk diagonal block in LDS (right-looking LDL^T) and carries 64
@@ -389,62 +424,35 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
     }
     return;
   }
-  // rows of L: X Z, thread (row lane, columns c0w ..)
+  // rows of L: X Z on the matrix cores, wave w the 16 x 16 block (rows
+  // 16 (w % 4) .., columns 16 (w / 4) ..); rows and columns t >= nbk of Xs
+  // and Zs are zero. The product goes through LDS (Xs, as [column][row]) to
+  // thread (row lane, columns c0w ..).
 #pragma unroll
   for (int u = 0; u < CW; ++u) {
     Zs[lane][c0w + u] = ga[u];
     Xs[c0w + u][lane] = xv[u];
   }
   __syncthreads();
+  using V = typename Mfma16<T>::V;
+  const int li = lane & 15, lk = lane >> 4;
+  const int bx = w & 3, by = w >> 2;
+  V o;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) o[v] = T(0);
+  for (int t0 = 0; t0 < nbk; t0 += 4)
+    o = Mfma16<T>::op(Xs[t0 + lk][16 * bx + li], Zs[t0 + lk][16 * by + li], o);
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < 4; ++v) Xs[16 * by + li][16 * bx + Mfma16<T>::row(lk, v)] = o[v];
+  __syncthreads();
   if (lane >= rows) return;
-  T acc[CW];
-#pragma unroll
-  for (int u = 0; u < CW; ++u) acc[u] = T(0);
-  for (int t = 0; t < nbk; ++t) {
-    const T xt = Xs[t][lane];
-#pragma unroll
-    for (int u = 0; u < CW; ++u) acc[u] += xt * Zs[t][c0w + u];
-  }
 #pragma unroll
   for (int u = 0; u < CW; ++u) {
     const int c = c0w + u;
-    if (c < nbk) P[r + (k0 + c) * ld] = r < m ? rnd<R>(acc[u]) : acc[u];
+    const T acc = Xs[c][lane];
+    if (c < nbk) P[r + (k0 + c) * ld] = r < m ? rnd<R>(acc) : acc;
   }
-}
-
-// Matrix-core product for the trailing updates: v_mfma 16x16x4 in the
-// precision of the factor. Lane l supplies A[l % 16][l / 16] and
-// B[l / 16][l % 16]; output v (0..3) of lane l is D[row(l / 16, v)][l % 16]
-// with row = 4 (l / 16) + v for fp32 and (l / 16) + 4 v for fp64 (measured
-// on gfx950).
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-template <class T>
-struct Mfma16;
-template <>
-struct Mfma16<float> {
-  using V = f32x4;
-  static __device__ __forceinline__ V op(float a, float b, V c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
-};
-template <>
-struct Mfma16<double> {
-  using V = f64x4;
-  static __device__ __forceinline__ V op(double a, double b, V c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
-};
-
-constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
-// task of workgroup b when workgroup b runs on XCD b % kXcds: XCD x takes the
-// x-th contiguous run of the list (runs differ in length by at most one)
-__device__ __forceinline__ int xcd_task(int b, int n) {
-  const int x = b % kXcds, i = b / kXcds;
-  const int q = n / kXcds, rem = n % kXcds;
-  return x * q + min(x, rem) + i;
 }
 
 // Trailing update tile: C[r0:r0+64, c0:c0+64] -= sum over the task's steps
