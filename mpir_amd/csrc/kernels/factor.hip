@@ -261,8 +261,10 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
   __shared__ __align__(16) T mrs[2][NB][CW];
   __shared__ __align__(16) T gms[2][NB][CW];
   __shared__ __align__(16) T dms[2][NB][CW];
-  __shared__ __align__(16) T Zs[NB][NB];   // Z, row-major
-  __shared__ __align__(16) T Xs[NB][NB];   // the task's rows, transposed: Xs[t][row]
+  // row stride NB + 1: the column-wise accesses (Zs[lane][c], Xs[lane][rr])
+  // are conflict-free
+  __shared__ __align__(16) T Zs[NB][NB + 1];  // Z, row-major
+  __shared__ __align__(16) T Xs[NB][NB + 1];  // the task's rows, transposed: Xs[t][row]
   const Task4 tk = tasks[blockIdx.x];
   const int s = tk.a, r0 = tk.b, r1 = tk.c, k = tk.d;
   const int f = a.first[s];
@@ -467,8 +469,10 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
 template <class T, int R>
 __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   using V = typename Mfma16<T>::V;
-  __shared__ T As[kPanelNB][kTile + 1];  // As[t][i] = L[r0 + i, k0 + t]
-  __shared__ T Bs[kPanelNB][kTile + 1];  // Bs[t][j] = L[c0 + j, k0 + t] D[k0 + t]
+  // Row stride kTile + 16: the four 16-lane groups of an MFMA operand read
+  // (rows t0 .. t0 + 3) fall in distinct LDS banks.
+  __shared__ T As[kPanelNB][kTile + 16];  // As[t][i] = L[r0 + i, k0 + t]
+  __shared__ T Bs[kPanelNB][kTile + 16];  // Bs[t][j] = L[c0 + j, k0 + t] D[k0 + t]
   // Workgroups are dealt round-robin over the XCDs (blockIdx % kXcds); each
   // XCD takes a contiguous run of the task list instead, so tiles that share
   // operand rows and columns meet in the same L2.
@@ -534,15 +538,19 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
     }
     __syncthreads();
   }
-  // the tile goes through LDS (As is free after the last barrier) so that
-  // the read-modify-write of the target is coalesced: element (x, y, v) is
-  // row wr + 16x + row(lk, v), column wc + 16y + li of the tile
+  // the tile goes through LDS (As is free after the last barrier, viewed as
+  // St[column][row] with row stride kTile + 1, so that a wave's reads of one
+  // column are conflict-free) so that the read-modify-write of the target is
+  // coalesced: element (x, y, v) is row wr + 16x + row(lk, v), column
+  // wc + 16y + li of the tile
+  static_assert(kTile * (kTile + 1) <= kPanelNB * (kTile + 16), "staging tile exceeds As");
+  T(*St)[kTile + 1] = reinterpret_cast<T(*)[kTile + 1]>(&As[0][0]);
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
     for (int y = 0; y < 2; ++y)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) As[wc + 16 * y + li][wr + 16 * x + Mfma16<T>::row(lk, v)] = acc[x][y][v];
+      for (int v = 0; v < 4; ++v) St[wc + 16 * y + li][wr + 16 * x + Mfma16<T>::row(lk, v)] = acc[x][y][v];
   __syncthreads();
   // thread (row lane, columns w + 4q): every load first, then the stores
   const int r = r0 + lane;
@@ -558,7 +566,7 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q)
-    if (dst[q] != nullptr) *dst[q] = cur[q] - As[w + 4 * q][lane];
+    if (dst[q] != nullptr) *dst[q] = cur[q] - St[w + 4 * q][lane];
 }
 
 template <class T>
