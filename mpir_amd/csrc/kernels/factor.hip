@@ -469,10 +469,12 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
 template <class T, int R>
 __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
   using V = typename Mfma16<T>::V;
-  // Row stride kTile + 16: the four 16-lane groups of an MFMA operand read
-  // (rows t0 .. t0 + 3) fall in distinct LDS banks.
-  __shared__ T As[kPanelNB][kTile + 16];  // As[t][i] = L[r0 + i, k0 + t]
-  __shared__ T Bs[kPanelNB][kTile + 16];  // Bs[t][j] = L[c0 + j, k0 + t] D[k0 + t]
+  // Operand tiles Ls[0][t][i] = L[r0 + i, k0 + t] and Ls[1][t][j] =
+  // L[c0 + j, k0 + t] D[k0 + t], stored at column i ^ 16 (t mod 4): the four
+  // 16-lane groups of an MFMA operand read (rows t0 .. t0 + 3) fall in
+  // distinct LDS banks without padding (32 KB in FP32: five workgroups per CU).
+  __shared__ T Ls[2][kPanelNB * kTile];
+  auto at = [](int t, int i) { return t * kTile + (i ^ ((t & 3) << 4)); };
   // Workgroups are dealt round-robin over the XCDs (blockIdx % kXcds); each
   // XCD takes a contiguous run of the task list instead, so tiles that share
   // operand rows and columns meet in the same L2.
@@ -490,7 +492,8 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
   const T* P = a.panel + a.panel_off[s];
   T* Pw = a.panel + a.panel_off[s];
   T* U = nr > 0 ? a.upool + a.upd_off[s] : Pw;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // w is wave-uniform (so are the D loads: scalar loads)
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   // thread (row i = lane, columns t = w + 4q): all loads of a step issued
   // before any is used (clamped addresses, values selected afterwards)
   const int ra = r0 + lane, rb = c0 + lane;
@@ -522,15 +525,15 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int t = w + 4 * q;
-      As[t][lane] = (ra < rlim && t < nbk) ? av[q] : T(0);
-      Bs[t][lane] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
+      Ls[0][at(t, lane)] = (ra < rlim && t < nbk) ? av[q] : T(0);
+      Ls[1][at(t, lane)] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
     }
     __syncthreads();
     if (kk < klast) load(kk + 1);
     for (int t0 = 0; t0 < nbk; t0 += 4) {
-      const int t = t0 + lk;  // rows t >= nbk of As / Bs are zero
-      const T a0 = As[t][wr + li], a1 = As[t][wr + 16 + li];
-      const T b0 = Bs[t][wc + li], b1 = Bs[t][wc + 16 + li];
+      const int t = t0 + lk;  // rows t >= nbk of the operand tiles are zero
+      const T a0 = Ls[0][at(t, wr + li)], a1 = Ls[0][at(t, wr + 16 + li)];
+      const T b0 = Ls[1][at(t, wc + li)], b1 = Ls[1][at(t, wc + 16 + li)];
       acc[0][0] = Mfma16<T>::op(a0, b0, acc[0][0]);
       acc[0][1] = Mfma16<T>::op(a0, b1, acc[0][1]);
       acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
@@ -538,13 +541,13 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
     }
     __syncthreads();
   }
-  // the tile goes through LDS (As is free after the last barrier, viewed as
+  // the tile goes through LDS (Ls is free after the last barrier, viewed as
   // St[column][row] with row stride kTile + 1, so that a wave's reads of one
   // column are conflict-free) so that the read-modify-write of the target is
   // coalesced: element (x, y, v) is row wr + 16x + row(lk, v), column
   // wc + 16y + li of the tile
-  static_assert(kTile * (kTile + 1) <= kPanelNB * (kTile + 16), "staging tile exceeds As");
-  T(*St)[kTile + 1] = reinterpret_cast<T(*)[kTile + 1]>(&As[0][0]);
+  static_assert(kTile * (kTile + 1) <= 2 * kPanelNB * kTile, "staging tile exceeds Ls");
+  T(*St)[kTile + 1] = reinterpret_cast<T(*)[kTile + 1]>(&Ls[0][0]);
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
