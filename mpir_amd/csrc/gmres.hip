@@ -79,13 +79,23 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce_partial(int n, const dou
 }
 
 // out[slot] = reduction of the partials (post 1: sqrt)
+// With mirror (pinned host memory, device-mapped): mirror[0 .. nhead) =
+// head[0 .. nhead) and mirror[nhead] = the result, so the host reads them
+// after the stream's work completes without a copy.
 __global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* __restrict__ part, int np, int op, int post,
-                                                              double* __restrict__ out) {
+                                                              double* __restrict__ out, const double* __restrict__ head,
+                                                              int nhead, double* __restrict__ mirror) {
   __shared__ double sh[kRedThreads / 64];
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc = op == 2 ? fmax(acc, part[i]) : acc + part[i];
   const double r = op == 2 ? block_max(acc, sh) : block_sum(acc, sh);
-  if (threadIdx.x == 0) *out = post ? sqrt(r) : r;
+  const double v = post ? sqrt(r) : r;
+  if (threadIdx.x == 0) *out = v;
+  if (mirror != nullptr) {
+    for (int i = threadIdx.x; i < nhead; i += blockDim.x) mirror[i] = head[i];
+    if (threadIdx.x == 0) mirror[nhead] = v;
+    __threadfence_system();
+  }
 }
 
 // partial dots of w against k1 basis vectors: part[j * gridDim.x + block]
@@ -259,7 +269,18 @@ GmresLdl::GmresLdl(int uf, int uw, int ur) : uf_(uf), uw_(uw), ur_(ur) {
       throw Error(kErrNoDevice, "no gfx950 device: this library runs the solve path on the GPU only");
   }
   MPIR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  MPIR_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096 * sizeof(double), hipHostMallocDefault));
+  MPIR_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096 * sizeof(double), hipHostMallocMapped));
+  MPIR_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&h_mirror_), h_pinned_, 0));
+}
+
+// Waits for the stream by polling (the GMRES loop waits once per iteration
+// for a handful of values; polling answers sooner than a blocking wait).
+void GmresLdl::wait_stream() {
+  for (;;) {
+    const hipError_t e = hipStreamQuery(stream_);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) MPIR_HIP(e);
+  }
 }
 
 GmresLdl::~GmresLdl() {
@@ -642,9 +663,8 @@ double GmresLdl::dnrm2(const double* v) {
   hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, 1,
                      d_red_.as<double>());
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 1, 1,
-                     d_small_.as<double>() + kSlotNorm);
-  MPIR_HIP(hipMemcpyAsync(h_pinned_, d_small_.as<double>() + kSlotNorm, sizeof(double), hipMemcpyDeviceToHost, stream_));
-  MPIR_HIP(hipStreamSynchronize(stream_));
+                     d_small_.as<double>() + kSlotNorm, nullptr, 0, h_mirror_);
+  wait_stream();
   return h_pinned_[0];
 }
 
@@ -652,9 +672,8 @@ double GmresLdl::infnrm(const double* v) {
   hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, 2,
                      d_red_.as<double>());
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 2, 0,
-                     d_small_.as<double>() + kSlotNorm);
-  MPIR_HIP(hipMemcpyAsync(h_pinned_, d_small_.as<double>() + kSlotNorm, sizeof(double), hipMemcpyDeviceToHost, stream_));
-  MPIR_HIP(hipStreamSynchronize(stream_));
+                     d_small_.as<double>() + kSlotNorm, nullptr, 0, h_mirror_);
+  wait_stream();
   return h_pinned_[0];
 }
 
@@ -730,9 +749,9 @@ void GmresLdl::precond_gmres(const double* b, double* res) {
       if (pass == 1) hipLaunchKernelGGL(k_add, dim3(nblk(k1)), dim3(256), 0, st, k1, hc, hp);
     }
     hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, w, nullptr, 1, part);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, part, kRedBlocks, 1, 1, hc + k1);
-    MPIR_HIP(hipMemcpyAsync(h_pinned_, hc, (k1 + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
-    MPIR_HIP(hipStreamSynchronize(st));
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, part, kRedBlocks, 1, 1, hc + k1, hc, k1,
+                       h_mirror_);
+    wait_stream();
     for (int j = 0; j <= k; ++j) h[j][k] = h_pinned_[j];
     h[k + 1][k] = h_pinned_[k1];
     if (h[k + 1][k] != 0) hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, w, 1.0 / h[k + 1][k], w);

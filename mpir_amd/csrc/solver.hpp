@@ -253,6 +253,8 @@ class GmresLdl {
   DevBuf d_hcol_, d_givens_;
   std::vector<double> h_small_;
   double* h_pinned_ = nullptr;
+  double* h_mirror_ = nullptr;  // h_pinned_ as the device addresses it
+  void wait_stream();
   // FGPILU
   int precond_ = kPrecLdl;
   int32_t ilu_k_ = 0, ilu_max_sweeps_ = 10;
