@@ -572,13 +572,21 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
     if (dst[q] != nullptr) *dst[q] = cur[q] - St[w + 4 * q][lane];
 }
 
+// Count of D[i] > 0: one atomic per workgroup (256 threads), few workgroups,
+// so that the adds to the one counter do not queue up.
 template <class T>
-__global__ void k_count_positive(int n, const T* __restrict__ D, int* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_count_positive(int n, const T* __restrict__ D, int* __restrict__ out) {
+  __shared__ int part[4];
   int cnt = 0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     cnt += D[i] > T(0) ? 1 : 0;
   for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
-  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(out, cnt);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int c = part[0] + part[1] + part[2] + part[3];
+    if (c) atomicAdd(out, c);
+  }
 }
 
 inline unsigned blocks_for(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
@@ -714,8 +722,8 @@ void reciprocal(int n, const T* x, T* y, hipStream_t st) {
 template <class T>
 void count_positive(int n, const T* D, int* out, hipStream_t st) {
   MPIR_HIP(hipMemsetAsync(out, 0, sizeof(int), st));
-  hipLaunchKernelGGL(k_count_positive<T>, dim3(std::min<unsigned>(1024, blocks_for(n, 256))), dim3(256), 0, st, n, D,
-                     out);
+  hipLaunchKernelGGL(k_count_positive<T>, dim3(std::min<unsigned>(256, std::max(1u, blocks_for(n, 256)))), dim3(256), 0,
+                     st, n, D, out);
 }
 
 template void factor_numeric<double>(const FactorArgs<double>&, const DeviceFactorPlan&, const double*, int64_t, int,
