@@ -13,7 +13,10 @@
  *   or_factor_*       - oracle/_ref bit-exact; reconstruction ||K - LDL^T||
  *   or_solve_*        - oracle/_ref bit-exact; dense solve on tiny inputs
  *   or_matmul_upper   - oracle/_ref bit-exact; exact on integer inputs
- *   or_dnrm2/infnrm   - ops.hpp doctest known answers; oracle/_ref
+ *   or_dnrm2/infnrm   - ops.hpp doctest known answers; oracle/_ref, also on
+ *                       NaN / Inf / overflowing vectors
+ *   or_scaling        - known answers of the |a_ii| > eps(UF) threshold for
+ *                       each factor precision (tests/solve_rules.py)
  *   or_luir_*         - LUIR_QDLDL (include/qdldl-luir/luir.hpp:33-241) on the
  *                       pinned factor / solve / residual above; the loop
  *                       itself is pinned by closed forms (an exact FP64
@@ -21,11 +24,18 @@
  *                       outcome (luir.hpp does not build here: it includes
  *                       the absent QD library)
  *   or_gmres_ir       - closed forms (K = I, exact FP64 factor: one step with
- *                       happy breakdown), Arnoldi invariants, backward error.
- *                       Iteration-level parity with the reference's LDL^T
- *                       GmresLDLIR is unpinned: the shipped class is
+ *                       happy breakdown), Arnoldi invariants, backward error;
+ *                       the control layer by hand-derived traces: the
+ *                       ||b||_2 < tol early exit, the rho > tol loop test,
+ *                       the <= small-update stop, the absolute stagnation
+ *                       stop, the clamp of the GMRES limit to n
+ *                       (tests/test_oracle_solve_rules.py). Unpinned: that
+ *                       the small-update test reads ||x|| before the update
+ *                       (no input separates the two readings beyond a 10 eps
+ *                       band), and the iteration-level trajectory of the
+ *                       reference's LDL^T GmresLDLIR (the shipped class is
  *                       ILU-preconditioned and gmres_ir.hpp does not build
- *                       here (it needs <print>).
+ *                       here: it needs <print>).
  */
 #define _POSIX_C_SOURCE 199309L
 #include <float.h>
@@ -200,12 +210,17 @@ double or_dot(int64_t n, const double* a, const double* b) {
   return init;
 }
 
-/* InfNrm, include/MPIR/ops.hpp:324-329 */
+/* InfNrm, include/MPIR/ops.hpp:324-329: |*std::max_element(x, |a| < |b|)|.
+ * max_element keeps the first element and replaces it only when
+ * |kept| < |x[i]| holds, so a NaN in x[0] is returned (every comparison with
+ * it is false) and a NaN anywhere else is skipped. The reference
+ * dereferences end() for an empty vector; this returns 0 there. */
 double or_infnrm(int64_t n, const double* x) {
-  double m = 0.0;
-  for (int64_t i = 0; i < n; i++)
-    if (fabs(x[i]) > m) m = fabs(x[i]);
-  return m;
+  if (n <= 0) return 0.0;
+  double kept = x[0];
+  for (int64_t i = 1; i < n; i++)
+    if (fabs(kept) < fabs(x[i])) kept = x[i];
+  return fabs(kept);
 }
 
 /* ------------------------------------------------------------------ GMRES-IR */
@@ -381,6 +396,8 @@ static void or_permute(int32_t n, const int32_t* Ap, const int32_t* Ai, const do
  */
 int32_t or_gmres_ir(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const int32_t* perm, int32_t uf,
                     int64_t ir_iter, int64_t gmres_iter, double tol, const double* b, double* x, double* info);
+void or_scaling(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, int32_t uf, double* AD,
+                double* Axs);
 
 
 /* LUIR_QDLDL<FACTPREC, double, double, double, int32_t>,
@@ -484,22 +501,34 @@ static double or_now(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* GmresLDLIR::Compute's diagonal scaling, include/MPIR/gmres_ir.hpp:347-363:
+ * AD[i] = 1/sqrt|a_ii| where |a_ii| > epsilon of the factor precision UF
+ * (uf: 0 double, 1 float, 2 binary16, 3 bfloat16), else 1; then
+ * a_ij *= AD[i] * AD[j]. Axs may be NULL (AD only). */
+void or_scaling(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, int32_t uf, double* AD,
+                double* Axs) {
+  const double eps_uf = uf == 0   ? DBL_EPSILON
+                        : uf == 1 ? (double)FLT_EPSILON
+                        : uf == 2 ? 0.0009765625 /* 2^-10 */
+                                  : 0.0078125;   /* 2^-7 */
+  for (int32_t i = 0; i < n; i++) AD[i] = 1;
+  for (int32_t col = 0; col < n; col++)
+    for (int32_t idx = Ap[col]; idx < Ap[col + 1]; idx++) {
+      const int32_t row = Ai[idx];
+      if (row == col && fabs(Ax[idx]) > eps_uf) AD[row] = 1.0 / sqrt(fabs(Ax[idx]));
+    }
+  if (Axs)
+    for (int32_t col = 0; col < n; col++)
+      for (int32_t idx = Ap[col]; idx < Ap[col + 1]; idx++) Axs[idx] = Ax[idx] * (AD[Ai[idx]] * AD[col]);
+}
+
 int32_t or_gmres_ir(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const int32_t* perm, int32_t uf,
                     int64_t ir_iter, int64_t gmres_iter, double tol, const double* b, double* x, double* info) {
   const double t0 = or_now();
   const int32_t nnz = Ap[n];
   double* Axs = (double*)malloc(sizeof(double) * (size_t)nnz);
   double* AD = (double*)malloc(sizeof(double) * (size_t)n);
-  memcpy(Axs, Ax, sizeof(double) * (size_t)nnz);
-  const double eps_uf = uf == 0 ? DBL_EPSILON : (double)FLT_EPSILON;
-  for (int32_t i = 0; i < n; i++) AD[i] = 1;
-  for (int32_t col = 0; col < n; col++)
-    for (int32_t idx = Ap[col]; idx < Ap[col + 1]; idx++) {
-      const int32_t row = Ai[idx];
-      if (row == col && fabs(Axs[idx]) > eps_uf) AD[row] = 1.0 / sqrt(fabs(Axs[idx]));
-    }
-  for (int32_t col = 0; col < n; col++)
-    for (int32_t idx = Ap[col]; idx < Ap[col + 1]; idx++) Axs[idx] *= AD[Ai[idx]] * AD[col];
+  or_scaling(n, Ap, Ai, Ax, uf, AD, Axs);
 
   int32_t *Cp = NULL, *Ci = NULL;
   double* Cx = NULL;
@@ -941,16 +970,7 @@ int32_t or_gmres_ilu_ir(int32_t n, const int32_t* Ap, const int32_t* Ai, const d
   const int32_t nnz = Ap[n];
   double* Axs = (double*)malloc(sizeof(double) * (size_t)(nnz > 0 ? nnz : 1));
   double* AD = (double*)malloc(sizeof(double) * (size_t)n);
-  memcpy(Axs, Ax, sizeof(double) * (size_t)nnz);
-  const double eps_uf = uf == 0 ? DBL_EPSILON : (double)FLT_EPSILON;
-  for (int32_t i = 0; i < n; i++) AD[i] = 1;
-  for (int32_t col = 0; col < n; col++)
-    for (int32_t idx = Ap[col]; idx < Ap[col + 1]; idx++) {
-      const int32_t row = Ai[idx];
-      if (row == col && fabs(Axs[idx]) > eps_uf) AD[row] = 1.0 / sqrt(fabs(Axs[idx]));
-    }
-  for (int32_t col = 0; col < n; col++)
-    for (int32_t idx = Ap[col]; idx < Ap[col + 1]; idx++) Axs[idx] *= AD[Ai[idx]] * AD[col];
+  or_scaling(n, Ap, Ai, Ax, uf == 0 ? 0 : 1, AD, Axs);
   or_ilu* f = or_ilu_build(n, Ap, Ai, Axs, uf == 0 ? 0 : 1, ilu_k, max_sweeps, tol);
   const double t1 = or_now();
   or_ctx c;

@@ -42,6 +42,7 @@ class Oracle:
             "or_dot": (f64, [i64, P, P]),
             "or_infnrm": (f64, [i64, P]),
             "or_gmres_ir": (i32, [i32, P, P, P, P, i32, i64, i64, f64, P, P, P]),
+            "or_scaling": (None, [i32, P, P, P, i32, P, P]),
             "or_luir_f64": (i32, [i32, P, P, P, P, P, i32, P]),
             "or_luir_f32": (i32, [i32, P, P, P, P, P, i32, P]),
             "or_fgpilu": (i64, [i32, P, P, P, i32, i32, i32, f64, P, P, P, P, P, P]),
@@ -142,6 +143,15 @@ class Oracle:
         keys = ["ir_iterations", "gmres_iterations", "stop_reason", "factor_ret", "precond", "r_norm", "factor_s",
                 "solve_s", "sweeps", "nl_res"]
         return rc, x, dict(zip(keys, info.tolist()))
+
+    def scaling(self, Ap, Ai, Ax, uf=np.float64):
+        """GmresLDLIR::Compute's diagonal scaling: (AD, scaled values)."""
+        n = len(Ap) - 1
+        code = {np.float64: 0, np.float32: 1, np.float16: 2, "bf16": 3}[uf]
+        AD = np.zeros(n)
+        Axs = np.zeros(len(Ai))
+        self.lib.or_scaling(n, ptr(Ap), ptr(Ai), ptr(np.ascontiguousarray(Ax, np.float64)), code, ptr(AD), ptr(Axs))
+        return AD, Axs
 
     def gmres_ir(self, Ap, Ai, Ax, b, perm=None, uf=np.float64, ir_iter=10, gmres_iter=10, tol=1e-10):
         n = len(Ap) - 1
