@@ -7,15 +7,23 @@
 // (10 IR iterations, 10 GMRES iterations, tolerance 1e-10). The work runs
 // on an MI355X (gfx950) through libmpir_hip.so; link with -lmpir_hip.
 //
-// Supported precisions: UF float or double, UW = UR = double. Compute()
-// keeps the reference's signature. With the default L D L^T preconditioner
-// ILU_k and the template sizes are accepted and ignored; after
-// SetPreconditioner(MPIR_PREC_FGPILU) Compute runs the reference's shipped
-// FGPILU path (gmres_ir.hpp:323-531) with ILU_k and MAX_SWEEPS.
+// Precisions: any UF, UW, UR from {float, double} that satisfy the
+// reference's Refinable<UF, UW, UR> (fp_concepts.hpp:112-113: floating
+// point, eps(UF) >= eps(UW) >= eps(UR)), e.g. <float, float, double>.
+// What the reference stores in UW (the matrix values, the scaling vector,
+// the scaled right-hand side, the iterate x and the correction d) and in UR
+// (the residual) is rounded to that precision on the device, and the
+// small-update test uses eps(UW); the Krylov basis and its dot products stay
+// in double, at least as accurate as the reference's UW arithmetic.
+// Compute() keeps the reference's signature. With the default L D L^T
+// preconditioner ILU_k and the template sizes are accepted and ignored;
+// after SetPreconditioner(MPIR_PREC_FGPILU) Compute runs the reference's
+// shipped FGPILU path (gmres_ir.hpp:323-531) with ILU_k and MAX_SWEEPS.
 // Failures throw std::runtime_error with the library's message.
 #pragma once
 
 #include <cstddef>
+#include <algorithm>
 #include <cstdint>
 #include <limits>
 #include <stdexcept>
@@ -25,15 +33,34 @@
 
 #include "../mpir_hip.h"
 
+namespace mpir_detail {
+template <typename T>
+constexpr bool is_fp_v = std::is_same<T, float>::value || std::is_same<T, double>::value;
+// fp_concepts.hpp:112-113 for the precisions the device supports
 template <typename UF, typename UW, typename UR>
-class GmresLDLIR {
-  static_assert(std::is_same<UF, float>::value || std::is_same<UF, double>::value, "UF must be float or double");
-  static_assert(std::is_same<UW, double>::value && std::is_same<UR, double>::value,
-                "UW and UR must be double (eps(UF) >= eps(UW) >= eps(UR))");
+constexpr bool refinable_v = is_fp_v<UF> && is_fp_v<UW> && is_fp_v<UR> &&
+                             std::numeric_limits<UF>::epsilon() >= std::numeric_limits<UW>::epsilon() &&
+                             std::numeric_limits<UW>::epsilon() >= std::numeric_limits<UR>::epsilon();
+template <typename T>
+constexpr int tag_v = std::is_same<T, float>::value ? MPIR_F32 : MPIR_F64;
+}  // namespace mpir_detail
+
+#if __cplusplus >= 202002L
+template <typename UF, typename UW, typename UR>
+concept Refinable = mpir_detail::refinable_v<UF, UW, UR>;
+#define MPIR_REQUIRES_REFINABLE requires Refinable<UF, UW, UR>
+#else
+#define MPIR_REQUIRES_REFINABLE
+#endif
+
+template <typename UF, typename UW, typename UR>
+MPIR_REQUIRES_REFINABLE class GmresLDLIR {
+  static_assert(mpir_detail::refinable_v<UF, UW, UR>,
+                "GmresLDLIR<UF, UW, UR>: float or double with eps(UF) >= eps(UW) >= eps(UR) (Refinable)");
 
  public:
   GmresLDLIR() {
-    check(mpir_gmres_ldl_create(std::is_same<UF, float>::value ? MPIR_F32 : MPIR_F64, MPIR_F64, MPIR_F64, &h_),
+    check(mpir_gmres_ldl_create(mpir_detail::tag_v<UF>, mpir_detail::tag_v<UW>, mpir_detail::tag_v<UR>, &h_),
           "GmresLDLIR");
   }
   ~GmresLDLIR() { mpir_gmres_ldl_destroy(h_); }
@@ -62,9 +89,9 @@ class GmresLDLIR {
 
   std::vector<UW> Solve(const std::vector<UW>& b) {
     if (b.size() != n_) return {};  // as PrecondGmres on a size mismatch
-    std::vector<UW> x(n_);
-    check(mpir_gmres_ldl_solve(h_, b.data(), x.data(), 1, static_cast<int64_t>(n_)), "Solve");
-    return x;
+    std::vector<double> bd(b.begin(), b.end()), xd(n_);
+    check(mpir_gmres_ldl_solve(h_, bd.data(), xd.data(), 1, static_cast<int64_t>(n_)), "Solve");
+    return std::vector<UW>(xd.begin(), xd.end());
   }
 
   void SetMaxIRIterations(std::size_t n) { check(mpir_gmres_ldl_set_max_ir(h_, static_cast<int64_t>(n)), "SetMaxIRIterations"); }
@@ -72,6 +99,19 @@ class GmresLDLIR {
     check(mpir_gmres_ldl_set_max_gmres(h_, static_cast<int64_t>(n)), "SetMaxGmresIterations");
   }
   void SetTolerance(UR tol) { check(mpir_gmres_ldl_set_tol(h_, static_cast<double>(tol)), "SetTolerance"); }
+
+  // The solver log of the reference's VERBOSE build (gmres_ir.hpp:305-314)
+  // as JSON text: max_outer_iterations, max_inner_iterations, tolerance and
+  // the "gmres_ir" event of the last Solve. The reference returns an
+  // nlohmann::json; parse this string with any JSON library to get one.
+  std::string DumpLog() const {
+    const int64_t len = mpir_gmres_ldl_log(h_, nullptr, 0);
+    check(static_cast<int>(len < 0 ? len : 0), "DumpLog");
+    std::string out(static_cast<std::size_t>(len) + 1, '\0');
+    check(static_cast<int>(std::min<int64_t>(0, mpir_gmres_ldl_log(h_, &out[0], len + 1))), "DumpLog");
+    out.resize(static_cast<std::size_t>(len));
+    return out;
+  }
 
   // Additions of this implementation (not in the reference's class).
   void SetOrdering(int ordering) { check(mpir_gmres_ldl_set_ordering(h_, ordering), "SetOrdering"); }

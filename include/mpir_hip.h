@@ -19,6 +19,12 @@
 #include <stddef.h>
 #include <stdint.h>
 
+/* The library is built with hidden symbol visibility; only the functions
+ * declared here are exported. */
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -66,8 +72,12 @@ int64_t mpir_qdldl_etree_i64(int64_t n, const int64_t* Ap, const int64_t* Ai, in
 /* Replaces QDLDL_factor<INT,PREC> (include/MPIR/qdldl.hpp:172-345). Same
  * arguments and outputs (Lp, Li, Lx in CSC with rows ascending, D, Dinv);
  * returns the number of positive entries of D, or -1 on a zero pivot.
- * The factorization runs on the GPU; bwork/iwork/fwork are accepted for
- * signature compatibility and may be NULL. */
+ * Deviation: a factor with a pivot that is not finite (an overflow or a NaN
+ * in Ax) also returns -1, where the reference returns it with Inf/NaN
+ * entries. The factorization runs on the GPU; bwork/iwork/fwork are accepted
+ * for signature compatibility and may be NULL. Like the reference's, these
+ * functions are reentrant: each calling thread has its own device state, so
+ * threads may call them concurrently. */
 int32_t mpir_qdldl_factor_f64_i32(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, int32_t* Lp,
                                   int32_t* Li, double* Lx, double* D, double* Dinv, const int32_t* Lnz,
                                   const int32_t* etree, unsigned char* bwork, int32_t* iwork, double* fwork);
@@ -294,6 +304,11 @@ int mpir_batch_solve_dev(mpir_gmres_ldl_t h, const int32_t* nrhs, const double* 
  * the solve until the instance stopped. Any output pointer may be NULL. */
 int mpir_batch_stats(mpir_gmres_ldl_t h, int32_t* max_rhs, int32_t* positive_pivots, int32_t* ir_iterations,
                      int32_t* gmres_iterations, int32_t* stop_reason, double* final_rel_residual, double* done_ms);
+/* Plans the calling thread's QDLDL_solve engines have built from factor
+ * patterns. A mpir_qdldl_solve_* call whose (Lp, Li) equals the previous
+ * call's on the same thread (same factor and x precision) reuses its plan
+ * and uploads only Lx and Dinv; no counterpart in the reference. */
+int mpir_qdldl_solve_plan_builds(int64_t* builds);
 /* the process-wide pattern cache: entries, hits, misses */
 int mpir_pattern_cache_info(int64_t* entries, int64_t* hits, int64_t* misses);
 /* Host only: symbolic phase of a batch without a device. info[0..7] = rows,
@@ -379,6 +394,10 @@ int mpir_read_mtx_upper(const char* path, int32_t* n, int64_t* nnz, int32_t* sym
 
 #ifdef __cplusplus
 }
+#endif
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
 #endif
 
 #endif /* MPIR_HIP_H */

@@ -129,6 +129,14 @@ def QDLDL_solve(n, Lp, Li, Lx, Dinv, x, prec=None) -> None:
     _check(int(fn(int(n), _ptr(Lp_), _ptr(Li_), _ptr(Lx_), _ptr(Di_), _ptr(x_))), "QDLDL_solve")
 
 
+def qdldl_solve_plan_builds() -> int:
+    """Plans this thread's QDLDL_solve engines have built: a call with the
+    same factor pattern as the previous one reuses its plan."""
+    out = C.c_int64(0)
+    _check(int(load().mpir_qdldl_solve_plan_builds(C.byref(out))), "qdldl_solve_plan_builds")
+    return int(out.value)
+
+
 # ---------------------------------------------------------------------- LUIR
 def LUIR_QDLDL(Ap, Ai, Ax, b, FACTPREC=np.float32, maxiter: int = 10):
     """LUIR_QDLDL<FACTPREC, double, double, double, INT> (include/qdldl-luir/

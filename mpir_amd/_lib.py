@@ -108,6 +108,7 @@ SIGNATURES = {
     "mpir_batch_solve_dev": (C.c_int, [P, P, P, P]),
     "mpir_batch_stats": (C.c_int, [P, P, P, P, P, P, P, P]),
     "mpir_pattern_cache_info": (C.c_int, [P, P, P]),
+    "mpir_qdldl_solve_plan_builds": (C.c_int, [P]),
     "mpir_batch_plan_info": (C.c_int, [i32, P, P, P, C.c_int, P]),
     "mpir_server_create": (C.c_int, [C.c_int, C.c_int, i64, i32, f64, C.c_int, i32, i32, P]),
     "mpir_server_destroy": (None, [P]),

@@ -34,10 +34,11 @@ ARCH = "gfx950"
 HIP_FLAGS = [
     "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-g", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-result", "-munsafe-fp-atomics",
+    "-Wno-unused-result", "-munsafe-fp-atomics", "-fvisibility=hidden", "-fvisibility-inlines-hidden",
 ]
 # Host code: no contraction so host sums round exactly as written.
-CXX_FLAGS = ["-O2", "-g", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-sign-compare"]
+CXX_FLAGS = ["-O2", "-g", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-sign-compare",
+             "-fvisibility=hidden", "-fvisibility-inlines-hidden"]
 
 HOST_SOURCES = [
     CSRC / "host" / "generators.cpp",
@@ -93,10 +94,13 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
             f.result()
     tmp = LIB.with_suffix(".so.tmp")
     objs = [str(_obj(s)) for s in HOST_SOURCES + HIP_SOURCES]
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs])
+    # only the C ABI of include/mpir_hip.h is exported (hidden visibility),
+    # and the library binds its own references to itself (-Bsymbolic), so
+    # two builds loaded into one process never call into each other
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,-Bsymbolic", "-o", str(tmp), *objs])
     os.replace(tmp, LIB)
     # the C++ drop-in test program (a reference-style caller of include/mpir/*.hpp)
-    _run([CXX, "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(DROPIN_SRC), f"-L{PKG}", "-lmpir_hip",
+    _run([CXX, "-std=c++20", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(DROPIN_SRC), f"-L{PKG}", "-lmpir_hip",
           "-Wl,-rpath,$ORIGIN/../../mpir_amd", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(DROPIN_BIN)])
     # the Matrix Market driver (examples/example.cpp of the reference)
     _run([CXX, "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(DRIVER_SRC), f"-L{PKG}", "-lmpir_hip",

@@ -26,6 +26,7 @@
 #include <cstring>
 #include <limits>
 
+#include "kernels/norms.hpp"
 #include "solver.hpp"
 
 namespace mpir {
@@ -57,48 +58,71 @@ __device__ __forceinline__ double bsum(double v, double* sh) {
   return r;
 }
 
+// NaN-propagating maximum over the block (kernels/norms.hpp)
 __device__ __forceinline__ double bmax(double v, double* sh) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off));
+  for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_down(v, off));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) sh[w] = v;
   __syncthreads();
   double r = 0.0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < kBT / 64; ++i) r = fmax(r, sh[i]);
+    for (int i = 0; i < kBT / 64; ++i) r = nan_max(r, sh[i]);
   __syncthreads();
   return r;
 }
 
-// part[r * nch + ch] = reduction over chunk ch of slot r.
-// op 0: sum x y, 1: sum x^2, 2: max |x|
+// Reduction over chunk ch of slot r. op 0: sum x y and op 2: max |x|, in
+// part[r * nch + ch]; op 1: the 2-norm as a scaled sum of squares, the pair
+// (scale, ssq) in part[2 (r * nch + ch)] and the entry after it.
 __global__ void __launch_bounds__(kBT) k_bred_partial(Seg g, const double* __restrict__ x,
                                                       const double* __restrict__ y, int op, double* __restrict__ part) {
-  __shared__ double sh[kBT / 64];
+  __shared__ double sh[2 * kBT / 64];
   const int ch = blockIdx.x, r = blockIdx.y;
   const int64_t base = static_cast<int64_t>(r) * g.N + g.ch_start[ch];
   const int len = g.ch_len[ch];
+  const int64_t slot = static_cast<int64_t>(r) * g.nch + ch;
+  if (op == 1) {
+    Ssq acc = ssq_init();
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+      const int i = threadIdx.x + u * kBT;
+      acc = ssq_add(acc, i < len ? x[base + i] : 0.0);
+    }
+    const Ssq t = block_ssq(acc, sh);
+    if (threadIdx.x == 0) {
+      part[2 * slot] = t.scale;
+      part[2 * slot + 1] = t.ssq;
+    }
+    return;
+  }
   double acc = 0.0;
 #pragma unroll
   for (int u = 0; u < kPerThread; ++u) {
     const int i = threadIdx.x + u * kBT;
     const double v = i < len ? x[base + i] : 0.0;
     if (op == 0) acc += v * (i < len ? y[base + i] : 0.0);
-    else if (op == 1) acc += v * v;
-    else acc = fmax(acc, fabs(v));
+    else acc = nan_max(acc, fabs(v));
   }
   const double t = op == 2 ? bmax(acc, sh) : bsum(acc, sh);
-  if (threadIdx.x == 0) part[static_cast<int64_t>(r) * g.nch + ch] = t;
+  if (threadIdx.x == 0) part[slot] = t;
 }
 
-// out[r * S + s]: the chunks of system s in order (post 1: square root)
-__global__ void k_bred_final(Seg g, const double* __restrict__ part, int op, int post, double* __restrict__ out) {
+// out[r * S + s]: the chunks of system s in order (op 1: the 2-norm)
+__global__ void k_bred_final(Seg g, const double* __restrict__ part, int op, double* __restrict__ out) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= g.R * g.S) return;
   const int r = q / g.S, s = q % g.S;
-  const double* p = part + static_cast<int64_t>(r) * g.nch;
+  const int64_t p0 = static_cast<int64_t>(r) * g.nch;
+  if (op == 1) {
+    Ssq acc = ssq_init();
+    for (int ch = g.sys_ch[s]; ch < g.sys_ch[s + 1]; ++ch)
+      acc = ssq_merge(acc, Ssq{part[2 * (p0 + ch)], part[2 * (p0 + ch) + 1]});
+    out[q] = ssq_value(acc);
+    return;
+  }
   double acc = 0.0;
-  for (int ch = g.sys_ch[s]; ch < g.sys_ch[s + 1]; ++ch) acc = op == 2 ? fmax(acc, p[ch]) : acc + p[ch];
-  out[q] = post ? sqrt(acc) : acc;
+  for (int ch = g.sys_ch[s]; ch < g.sys_ch[s + 1]; ++ch) acc = op == 2 ? nan_max(acc, part[p0 + ch]) : acc + part[p0 + ch];
+  out[q] = acc;
 }
 
 // Partial dots of w with basis vectors 0..k1-1 (stride ldv):
@@ -288,6 +312,7 @@ void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntr
   const double t0 = now_ms();
   MPIR_CHECK(!parts.empty(), "a batch needs at least one system");
   MPIR_CHECK(precond_ == kPrecLdl, "batches use the L D L^T preconditioner");
+  MPIR_CHECK(uw_ == kF64 && ur_ == kF64, "batches refine in f64 working and residual precision");
   std::vector<const PatternEntry*> raw;
   for (const auto& p : parts) {
     MPIR_CHECK(p != nullptr, "null pattern");
@@ -446,10 +471,10 @@ void GmresLdl::bnorms(const double* v, int op, int R, std::vector<double>& out) 
   const int I = R * S;
   Seg g{n_, R, S, b_nch_, d_b_sys_of_row_.as<int32_t>(), d_b_ch_start_.as<int32_t>(), d_b_ch_len_.as<int32_t>(),
         d_b_sys_ch_.as<int32_t>()};
+  d_red_.ensure(static_cast<size_t>(2) * R * b_nch_ * sizeof(double));
   double* part = d_red_.as<double>();
   hipLaunchKernelGGL(k_bred_partial, dim3(b_nch_, R), dim3(kBT), 0, stream_, g, v, nullptr, op, part);
-  hipLaunchKernelGGL(k_bred_final, dim3(nblk(I)), dim3(256), 0, stream_, g, part, op, op == 1 ? 1 : 0,
-                     d_b_scalar_.as<double>());
+  hipLaunchKernelGGL(k_bred_final, dim3(nblk(I)), dim3(256), 0, stream_, g, part, op, d_b_scalar_.as<double>());
   double* hp = bpinned(I);
   MPIR_HIP(hipMemcpyAsync(hp, d_b_scalar_.get(), I * sizeof(double), hipMemcpyDeviceToHost, stream_));
   MPIR_HIP(hipStreamSynchronize(stream_));
@@ -478,7 +503,7 @@ void GmresLdl::precond_gmres_batch(const double* b, double* res, const std::vect
   d_V_.ensure(static_cast<size_t>(M + 1) * ldv * sizeof(double));
   d_hcol_.ensure(static_cast<size_t>(M + 2) * I * sizeof(double));
   d_small_.ensure(static_cast<size_t>(M + 2) * I * sizeof(double));
-  d_red_.ensure(static_cast<size_t>(M + 1) * R * b_nch_ * sizeof(double));
+  d_red_.ensure(static_cast<size_t>(std::max(M + 1, 2)) * R * b_nch_ * sizeof(double));
   d_b_scalar_.ensure(static_cast<size_t>(std::max(M, 1)) * I * sizeof(double));
   d_b_kq_.ensure(I * sizeof(int32_t));
   double* V = d_V_.as<double>();
@@ -535,7 +560,7 @@ void GmresLdl::precond_gmres_batch(const double* b, double* res, const std::vect
     }
     // the column is h = h1 + h2, summed on the host below
     hipLaunchKernelGGL(k_bred_partial, dim3(b_nch_, R), dim3(kBT), 0, st, g, w, nullptr, 1, part);
-    hipLaunchKernelGGL(k_bred_final, dim3(nblk(I)), dim3(256), 0, st, g, part, 1, 1, hc + static_cast<int64_t>(k1) * I);
+    hipLaunchKernelGGL(k_bred_final, dim3(nblk(I)), dim3(256), 0, st, g, part, 1, hc + static_cast<int64_t>(k1) * I);
     // both passes' coefficients and the norm to the host
     double* hs = bpinned(static_cast<size_t>(2 * k1 + 1) * I);
     MPIR_HIP(hipMemcpyAsync(hs, hc, static_cast<size_t>(k1 + 1) * I * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -557,7 +582,7 @@ void GmresLdl::precond_gmres_batch(const double* b, double* res, const std::vect
         Hq(q, j, k) = w1;
         Hq(q, j + 1, k) = w2;
       }
-      const double nu = std::hypot(Hq(q, k, k), Hq(q, k + 1, k));
+      const double nu = dnrm2_pair(Hq(q, k, k), Hq(q, k + 1, k));
       if (nu != 0) {
         cq[k] = Hq(q, k, k) / nu;
         sq[k] = -Hq(q, k + 1, k) / nu;

@@ -3,6 +3,7 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <chrono>
 #include <cstring>
@@ -45,24 +46,32 @@ int guarded(F&& f) {
 }
 
 // The QDLDL drop-ins keep one engine per factor precision, reused while the
-// pattern stays the same (hash of Ap/Ai).
+// pattern stays the same (compared entry by entry). The reference's QDLDL
+// functions are reentrant (qdldl.hpp:22-23: all state in the arguments), so
+// every calling thread has engines of its own, each with its own stream and
+// buffers: calls from different threads run concurrently and never share a
+// handle.
 struct QdldlEngine {
   std::unique_ptr<GmresLdl> solver;
-  uint64_t pattern_hash = 0;
-  int32_t n = -1;
-};
-std::mutex g_engine_mu;
-QdldlEngine g_factor_engine[4];
-QdldlEngine g_luir_engine[2];
-QdldlEngine g_solve_engine[2];
-
-uint64_t fnv1a(const void* data, size_t bytes, uint64_t h = 1469598103934665603ull) {
-  const unsigned char* p = static_cast<const unsigned char*>(data);
-  for (size_t i = 0; i < bytes; ++i) {
-    h ^= p[i];
-    h *= 1099511628211ull;
+  std::vector<int32_t> Ap, Ai;  // the pattern the solver was analysed for
+  bool same_pattern(int32_t n, const int32_t* p, const int32_t* i) const {
+    return solver && solver->analyzed() && Ap.size() == static_cast<size_t>(n) + 1 && std::equal(p, p + n + 1, Ap.begin()) &&
+           std::equal(i, i + p[n], Ai.begin());
   }
-  return h;
+  void analyze(int32_t n, const int32_t* p, const int32_t* i) {
+    Ap.clear();
+    Ai.clear();
+    solver->analyze(n, p, i);
+    Ap.assign(p, p + n + 1);
+    Ai.assign(i, i + p[n]);
+  }
+};
+struct QdldlEngines {
+  QdldlEngine factor[4], luir[2], solve[2];
+};
+QdldlEngines& engines() {
+  thread_local QdldlEngines e;
+  return e;
 }
 
 bool narrow(int64_t v, int32_t* out) {
@@ -134,22 +143,14 @@ template <class T>
 int64_t factor_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const T* Ax, int64_t* Lp64, int32_t* Li, T* Lx,
                     T* D, T* Dinv, const int32_t* Lnz, const int32_t* etree) {
   const int prec = ValueIO<T>::prec;
-  std::lock_guard<std::mutex> lock(g_engine_mu);
-  QdldlEngine& E = g_factor_engine[prec];
+  QdldlEngine& E = engines().factor[prec];
   if (!E.solver) {
     E.solver = std::make_unique<GmresLdl>(prec, kF64, kF64);
     E.solver->set_ordering(kOrderNatural);
     E.solver->set_scaling(kScaleNone);
   }
   const int64_t nnz = Ap[n];
-  uint64_t h = fnv1a(Ap, sizeof(int32_t) * (n + 1));
-  h = fnv1a(Ai, sizeof(int32_t) * nnz, h);
-  if (E.n != n || E.pattern_hash != h || !E.solver->analyzed()) {
-    E.n = -1;
-    E.solver->analyze(n, Ap, Ai);
-    E.n = n;
-    E.pattern_hash = h;
-  }
+  if (!E.same_pattern(n, Ap, Ai)) E.analyze(n, Ap, Ai);
   // the caller's etree / counts must describe this pattern
   std::vector<int32_t> work(n), lnz(n), par(n);
   etree_counts(n, Ap, Ai, work.data(), lnz.data(), par.data(), false);
@@ -173,8 +174,7 @@ template <class TF, class TX>
 int solve_impl(int32_t n, const int64_t* Lp, const int32_t* Li, const TF* Lx, const TF* Dinv, TX* x) {
   static_assert(std::is_same<TX, double>::value || std::is_same<TX, float>::value, "x precision");
   const int prec = std::is_same<TF, double>::value ? kF64 : kF32;
-  std::lock_guard<std::mutex> lock(g_engine_mu);
-  QdldlEngine& E = g_solve_engine[prec];
+  QdldlEngine& E = engines().solve[prec];
   if (!E.solver) E.solver = std::make_unique<GmresLdl>(prec, kF64, kF64);
   E.solver->load_factor_csc(n, Lp, Li, Lx, Dinv);
   DevBuf xd(sizeof(double) * n);
@@ -196,8 +196,7 @@ int luir_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax,
               mpir_luir_stats* stats, int maxiter) {
   MPIR_CHECK(n >= 1 && Ap && Ai && Ax && b && x, "bad LUIR arguments");
   const int prec = std::is_same<T, double>::value ? kF64 : kF32;
-  std::lock_guard<std::mutex> lock(g_engine_mu);
-  QdldlEngine& E = g_luir_engine[prec];
+  QdldlEngine& E = engines().luir[prec];
   if (!E.solver) {
     E.solver = std::make_unique<GmresLdl>(prec, kF64, kF64);
     E.solver->set_ordering(kOrderNatural);
@@ -216,14 +215,7 @@ int luir_impl(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax,
     return -2;
   }
   const int64_t nnz = Ap[n];
-  uint64_t h = fnv1a(Ap, sizeof(int32_t) * (n + 1));
-  h = fnv1a(Ai, sizeof(int32_t) * nnz, h);
-  if (E.n != n || E.pattern_hash != h || !E.solver->analyzed()) {
-    E.n = -1;
-    E.solver->analyze(n, Ap, Ai);
-    E.n = n;
-    E.pattern_hash = h;
-  }
+  if (!E.same_pattern(n, Ap, Ai)) E.analyze(n, Ap, Ai);
   const double t_sym = std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
   const auto tf = std::chrono::steady_clock::now();
   try {
@@ -721,6 +713,16 @@ int mpir_batch_stats(mpir_gmres_ldl_t h, int32_t* max_rhs, int32_t* positive_piv
     put(stop_reason, b.stop_reason);
     put(final_rel_residual, b.final_rel_residual);
     put(done_ms, b.done_ms);
+    return 0;
+  });
+}
+int mpir_qdldl_solve_plan_builds(int64_t* builds) {
+  return guarded([&] {
+    MPIR_CHECK(builds, "null output");
+    int64_t b = 0;
+    for (const QdldlEngine& E : engines().solve)
+      if (E.solver) b += E.solver->plan_builds();
+    *builds = b;
     return 0;
   });
 }

@@ -27,8 +27,10 @@
 #include <limits>
 #include <numeric>
 #include <string>
+#include <thread>
 
 #include "host/sparse.hpp"
+#include "kernels/norms.hpp"
 #include "solver.hpp"
 
 namespace mpir {
@@ -50,46 +52,63 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// NaN-propagating maximum over the block (kernels/norms.hpp)
 __device__ __forceinline__ double block_max(double v, double* sh) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off));
+  for (int off = 32; off > 0; off >>= 1) v = nan_max(v, __shfl_down(v, off));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) sh[w] = v;
   __syncthreads();
   double r = 0.0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) r = fmax(r, sh[i]);
+    for (int i = 0; i < static_cast<int>(blockDim.x >> 6); ++i) r = nan_max(r, sh[i]);
   __syncthreads();
   return r;
 }
 
-// op 0: sum x_i y_i, op 1: sum x_i^2, op 2: max |x_i|
+// op 0: sum x_i y_i; op 1: the 2-norm as a scaled sum of squares, one
+// (scale, ssq) pair per block in part[2b], part[2b + 1]; op 2: max |x_i|
 __global__ void __launch_bounds__(kRedThreads) k_reduce_partial(int n, const double* __restrict__ x,
                                                                 const double* __restrict__ y, int op,
                                                                 double* __restrict__ part) {
-  __shared__ double sh[kRedThreads / 64];
+  __shared__ double sh[2 * kRedThreads / 64];
+  if (op == 1) {
+    Ssq acc = ssq_init();
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc = ssq_add(acc, x[i]);
+    const Ssq r = block_ssq(acc, sh);
+    if (threadIdx.x == 0) {
+      part[2 * blockIdx.x] = r.scale;
+      part[2 * blockIdx.x + 1] = r.ssq;
+    }
+    return;
+  }
   double acc = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const double v = x[i];
     if (op == 0) acc += v * y[i];
-    else if (op == 1) acc += v * v;
-    else acc = fmax(acc, fabs(v));
+    else acc = nan_max(acc, fabs(v));
   }
   const double r = op == 2 ? block_max(acc, sh) : block_sum(acc, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
-// out[slot] = reduction of the partials (post 1: sqrt)
+// out[slot] = reduction of the partials (op 1: the 2-norm from the pairs)
 // With mirror (pinned host memory, device-mapped): mirror[0 .. nhead) =
 // head[0 .. nhead) and mirror[nhead] = the result, so the host reads them
 // after the stream's work completes without a copy.
-__global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* __restrict__ part, int np, int op, int post,
+__global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* __restrict__ part, int np, int op,
                                                               double* __restrict__ out, const double* __restrict__ head,
                                                               int nhead, double* __restrict__ mirror) {
-  __shared__ double sh[kRedThreads / 64];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) acc = op == 2 ? fmax(acc, part[i]) : acc + part[i];
-  const double r = op == 2 ? block_max(acc, sh) : block_sum(acc, sh);
-  const double v = post ? sqrt(r) : r;
+  __shared__ double sh[2 * kRedThreads / 64];
+  double v;
+  if (op == 1) {
+    Ssq acc = ssq_init();
+    for (int i = threadIdx.x; i < np; i += blockDim.x) acc = ssq_merge(acc, Ssq{part[2 * i], part[2 * i + 1]});
+    v = ssq_value(block_ssq(acc, sh));
+  } else {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) acc = op == 2 ? nan_max(acc, part[i]) : acc + part[i];
+    v = op == 2 ? block_max(acc, sh) : block_sum(acc, sh);
+  }
   if (threadIdx.x == 0) *out = v;
   if (mirror != nullptr) {
     for (int i = threadIdx.x; i < nhead; i += blockDim.x) mirror[i] = head[i];
@@ -198,6 +217,12 @@ __global__ void k_ruiz_update(int n, const int32_t* __restrict__ perm, const dou
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && rmax[i] > 0.0) ad[perm[i]] /= sqrt(rmax[i]);
 }
+// x[i] = float(x[i]): a vector the reference holds in a float working or
+// residual precision, kept in double storage
+__global__ void k_round_f32(int64_t n, double* __restrict__ x) {
+  const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (i < n) x[i] = static_cast<double>(static_cast<float>(x[i]));
+}
 __global__ void k_fill(int n, double v, double* __restrict__ y) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = v;
@@ -241,6 +266,11 @@ __global__ void k_perm_out(int n, const double* __restrict__ xp, const int32_t* 
 
 inline unsigned nblk(int64_t n, int bs = 256) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
+// rounds v to float when prec is f32 (a no-op for f64)
+void round_to(int prec, double* v, int64_t n, hipStream_t st) {
+  if (prec == kF32 && n > 0) hipLaunchKernelGGL(k_round_f32, dim3(nblk(n)), dim3(256), 0, st, n, v);
+}
+
 double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -257,9 +287,11 @@ enum Slot { kSlotNorm = 0, kSlotH = 8 };
 }  // namespace
 
 GmresLdl::GmresLdl(int uf, int uw, int ur) : uf_(uf), uw_(uw), ur_(ur) {
-  // Refinable<UF,UW,UR> (fp_concepts.hpp:112-113): eps(UF) >= eps(UW) >= eps(UR)
+  // Refinable<UF,UW,UR> (fp_concepts.hpp:112-113): eps(UF) >= eps(UW) >= eps(UR);
+  // the tags are ordered by epsilon (f64 < f32 < f16 < bf16)
   MPIR_CHECK(uf == kF64 || uf == kF32 || uf == kF16 || uf == kBF16, "factor precision must be f64, f32, f16 or bf16");
-  MPIR_CHECK(uw == kF64 && ur == kF64, "working and residual precision must be f64");
+  MPIR_CHECK((uw == kF64 || uw == kF32) && (ur == kF64 || ur == kF32), "working and residual precision must be f64 or f32");
+  MPIR_CHECK(uf >= uw && uw >= ur, "precisions must satisfy eps(UF) >= eps(UW) >= eps(UR) (Refinable)");
   // the solver runs only on a gfx950 device; there is no host fallback
   {
     int count = 0;
@@ -269,17 +301,23 @@ GmresLdl::GmresLdl(int uf, int uw, int ur) : uf_(uf), uw_(uw), ur_(ur) {
       throw Error(kErrNoDevice, "no gfx950 device: this library runs the solve path on the GPU only");
   }
   MPIR_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  MPIR_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096 * sizeof(double), hipHostMallocMapped));
+  // coherent: the device writes the norm mirror while the host polls
+  MPIR_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096 * sizeof(double),
+                         hipHostMallocMapped | hipHostMallocCoherent));
   MPIR_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&h_mirror_), h_pinned_, 0));
 }
 
-// Waits for the stream by polling (the GMRES loop waits once per iteration
-// for a handful of values; polling answers sooner than a blocking wait).
+// Waits for the stream: polls for up to 200 us (the GMRES loop waits once
+// per iteration for a handful of values, and polling answers sooner than a
+// blocking wait), then yields the core between polls, so that threads of
+// other handles and the host's symbolic work keep running.
 void GmresLdl::wait_stream() {
+  const auto t0 = std::chrono::steady_clock::now();
   for (;;) {
     const hipError_t e = hipStreamQuery(stream_);
     if (e == hipSuccess) return;
     if (e != hipErrorNotReady) MPIR_HIP(e);
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(200)) std::this_thread::yield();
   }
 }
 
@@ -459,7 +497,7 @@ void GmresLdl::upload_plan() {
   // counters: per front, per column block, then the dispensers on their own lines
   g.cnt_bx = kNumSolveCounters * S.ns;
   g.cnt_disp = (g.cnt_bx + P.ncb_total + kDispenserStride - 1) / kDispenserStride * kDispenserStride;
-  g.ncounters = g.cnt_disp + 2 * kDispensers * kDispenserStride;
+  g.ncounters = g.cnt_disp + 2 * kDispenserLines * kDispenserStride;
   d_counters_.alloc(static_cast<size_t>(g.ncounters) * sizeof(int));
   g.counters = d_counters_.as<int>();
   d_serr_.alloc(sizeof(int));
@@ -524,21 +562,35 @@ void GmresLdl::alloc_numeric() {
 void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, const void* Lx, const void* Dinv) {
   MPIR_CHECK(n >= 1, "matrix dimension must be positive");
   MPIR_CHECK(precond_ == kPrecLdl, "a loaded factor is an L D L^T factor");
-  analyzed_ = factored_ = false;
-  batch_.reset();
-  Supernodes S;
-  try {
-    S = supernodes_from_l(n, Lp, Li);
-  } catch (const std::exception& e) {
-    throw Error(kErrInvalid, std::string("factor pattern rejected: ") + e.what());
+  // the plan of the factor pattern is kept while the pattern stays the same
+  // (compared entry by entry), so a caller that solves in a loop with the
+  // same factor (luir.hpp:165-205) pays only the upload of the values
+  const bool same_pattern = precond_only_ && analyzed_ && n == n_ && loaded_Lp_.size() == static_cast<size_t>(n) + 1 &&
+                            std::equal(Lp, Lp + n + 1, loaded_Lp_.begin()) &&
+                            std::equal(Li, Li + Lp[n], loaded_Li_.begin());
+  factored_ = false;
+  if (!same_pattern) {
+    analyzed_ = false;
+    loaded_Lp_.clear();
+    loaded_Li_.clear();
+    batch_.reset();
+    Supernodes S;
+    try {
+      S = supernodes_from_l(n, Lp, Li);
+    } catch (const std::exception& e) {
+      throw Error(kErrInvalid, std::string("factor pattern rejected: ") + e.what());
+    }
+    n_ = n;
+    nnz_ = 0;
+    perm_.resize(n);
+    std::iota(perm_.begin(), perm_.end(), 0);
+    plan_ = std::make_unique<FactorPlan>(build_plan_from_supernodes(std::move(S)));
+    upload_plan();
+    alloc_numeric();
+    loaded_Lp_.assign(Lp, Lp + n + 1);
+    loaded_Li_.assign(Li, Li + Lp[n]);
+    ++plan_builds_;
   }
-  n_ = n;
-  nnz_ = 0;
-  perm_.resize(n);
-  std::iota(perm_.begin(), perm_.end(), 0);
-  plan_ = std::make_unique<FactorPlan>(build_plan_from_supernodes(std::move(S)));
-  upload_plan();
-  alloc_numeric();
   const Supernodes& SS = plan_->sn;
   const size_t tf = uf_ == kF64 ? sizeof(double) : sizeof(float);
   std::vector<char> panel(static_cast<size_t>(plan_->panel_entries) * tf, 0);
@@ -594,6 +646,8 @@ void GmresLdl::factor(const double* Ax, bool device_ptr) {
 void GmresLdl::factor_loaded(double t0) {
   hipStream_t st = stream_;
   const int64_t ncsr = 2 * nnz_ - n_;
+  // Ax_ = UW(Ax), AD_ in UW, Ax_ *= AD_ AD_ in UW (gmres_ir.hpp:96-99, 331-363)
+  round_to(uw_, d_ax_.as<double>(), nnz_, st);
   if (scaling_ == kScaleRuiz) {
     // Ruiz: at most kRuizSweeps sweeps, stopping once every row's max-norm is
     // within kRuizTol of one (the tolerance is checked before each update)
@@ -621,8 +675,10 @@ void GmresLdl::factor_loaded(double t0) {
     hipLaunchKernelGGL(k_compute_ad, dim3(nblk(n_)), dim3(256), 0, st, n_, d_ax_.as<double>(),
                        d_diag_pos_.as<int32_t>(), eps, d_ad_.as<double>());
   }
+  round_to(uw_, d_ad_.as<double>(), n_, st);
   hipLaunchKernelGGL(k_scale_entries, dim3(nblk(nnz_)), dim3(256), 0, st, nnz_, d_ax_.as<double>(),
                      d_entry_row_.as<int32_t>(), d_entry_col_.as<int32_t>(), d_ad_.as<double>(), d_axs_.as<double>());
+  round_to(uw_, d_axs_.as<double>(), nnz_, st);
   hipLaunchKernelGGL(k_gather, dim3(nblk(ncsr)), dim3(256), 0, st, ncsr, d_axs_.as<double>(), d_csr_src_.as<int32_t>(),
                      d_csr_x_.as<double>());
   MPIR_HIP(hipGetLastError());
@@ -633,36 +689,40 @@ void GmresLdl::factor_loaded(double t0) {
   } else if (uf_ == kF64) {
     const FactorArgs<double> a = factor_args<double>();
     factor_numeric<double>(a, dfp_, d_axs_.as<double>(), nnz_, 0, st);
-    count_positive<double>(n_, d_D_.as<double>(), d_cnt_.as<int>(), st);
+    count_positive<double>(n_, d_D_.as<double>(), d_Dinv_.as<double>(), d_cnt_.as<int>(), d_nonfinite_.as<int>(), st);
   } else {
     const FactorArgs<float> a = factor_args<float>();
     factor_numeric<float>(a, dfp_, d_axs_.as<double>(), nnz_, rounding(), st);
-    count_positive<float>(n_, d_D_.as<float>(), d_cnt_.as<int>(), st);
-    round_factor16();
+    count_positive<float>(n_, d_D_.as<float>(), d_Dinv_.as<float>(), d_cnt_.as<int>(), d_nonfinite_.as<int>(), st);
+    round_factor16();  // adds the 16-bit entries outside the format's range
   }
   MPIR_HIP(hipMemcpyAsync(&h_pinned_[0], d_err_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
   MPIR_HIP(hipMemcpyAsync(&h_pinned_[1], d_cnt_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
+  MPIR_HIP(hipMemcpyAsync(&h_pinned_[2], d_nonfinite_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
   int nonfinite = 0;
-  if ((uf_ == kF16 || uf_ == kBF16) && precond_ != kPrecIlu)
-    MPIR_HIP(hipMemcpyAsync(&h_pinned_[2], d_nonfinite_.get(), sizeof(int), hipMemcpyDeviceToHost, st));
   if (batch_) batch_count_pivots();
   MPIR_HIP(hipStreamSynchronize(st));
   std::memcpy(&err, &h_pinned_[0], sizeof(int));
   std::memcpy(&pos, &h_pinned_[1], sizeof(int));
-  if (uf_ == kF16 || uf_ == kBF16) std::memcpy(&nonfinite, &h_pinned_[2], sizeof(int));
+  std::memcpy(&nonfinite, &h_pinned_[2], sizeof(int));
   stats_.positive_pivots = pos;
   stats_.factor_ms = now_ms() - t0;
   if (batch_) batch_collect_pivots(err != 0);
   if (err) throw Error(kErrNumeric, "zero pivot in the LDL^T factorization");
-  if (nonfinite)
+  // QDLDL_factor has no such check (qdldl.hpp:331 tests for an exact zero
+  // only): a non-finite factor would turn every later solve into NaN
+  if (nonfinite && (uf_ == kF16 || uf_ == kBF16) && precond_ != kPrecIlu)
     throw Error(kErrNumeric, std::to_string(nonfinite) + " factor entries overflow the " + prec_name(uf_) + " range");
+  if (nonfinite)
+    throw Error(kErrNumeric, std::to_string(nonfinite) + " pivots of the " + prec_name(uf_) +
+                                 " factorization are not finite (overflow or NaN in the matrix or the factor)");
   factored_ = true;
 }
 
 double GmresLdl::dnrm2(const double* v) {
   hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, 1,
                      d_red_.as<double>());
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 1, 1,
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 1,
                      d_small_.as<double>() + kSlotNorm, nullptr, 0, h_mirror_);
   wait_stream();
   return h_pinned_[0];
@@ -671,7 +731,7 @@ double GmresLdl::dnrm2(const double* v) {
 double GmresLdl::infnrm(const double* v) {
   hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, 2,
                      d_red_.as<double>());
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 2, 0,
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 2,
                      d_small_.as<double>() + kSlotNorm, nullptr, 0, h_mirror_);
   wait_stream();
   return h_pinned_[0];
@@ -749,7 +809,7 @@ void GmresLdl::precond_gmres(const double* b, double* res) {
       if (pass == 1) hipLaunchKernelGGL(k_add, dim3(nblk(k1)), dim3(256), 0, st, k1, hc, hp);
     }
     hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, w, nullptr, 1, part);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, part, kRedBlocks, 1, 1, hc + k1, hc, k1,
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, part, kRedBlocks, 1, hc + k1, hc, k1,
                        h_mirror_);
     wait_stream();
     for (int j = 0; j <= k; ++j) h[j][k] = h_pinned_[j];
@@ -761,7 +821,7 @@ void GmresLdl::precond_gmres(const double* b, double* res) {
       h[j][k] = w1;
       h[j + 1][k] = w2;
     }
-    const double nu = std::hypot(h[k][k], h[k + 1][k]);
+    const double nu = dnrm2_pair(h[k][k], h[k + 1][k]);
     if (nu != 0) {
       c[k] = h[k][k] / nu;
       s[k] = -h[k + 1][k] / nu;
@@ -797,13 +857,24 @@ void GmresLdl::solve_one(const double* b, double* x) {
   double* d = d_d_.as<double>();
   double* tmp = d_w_.as<double>();
   const double t_start = now_ms();
-  precond_gmres(b, x);
-  residual_f64(A, x, b, r, st);
+  // A non-finite right-hand side, iterate or residual ends the solve with
+  // MPIR_ERR_NUMERIC. The reference has no such check: its InfNrm skips a
+  // NaN that is not the first entry (ops.hpp:324-329), and the loop would
+  // go on with NaN.
   const double bnorm = infnrm(b);
+  if (!std::isfinite(bnorm)) throw Error(kErrNumeric, "the right-hand side is not finite");
+  // eps of the working precision in the small-update test (gmres_ir.hpp:599)
+  const double eps_uw = uw_ == kF32 ? static_cast<double>(std::numeric_limits<float>::epsilon())
+                                    : std::numeric_limits<double>::epsilon();
+  precond_gmres(b, x);
+  round_to(uw_, x, n, st);
+  residual_f64(A, x, b, r, st);
+  round_to(ur_, r, n, st);
   double r_prev = std::numeric_limits<double>::max();
   stats_.stop_reason = 0;
   log_ = SolveLog();
   log_.initial_residual = infnrm(r);
+  if (!std::isfinite(log_.initial_residual)) throw Error(kErrNumeric, "the first iterate is not finite");
   int64_t iter;
   double r_norm = 0.0;
   for (iter = 1; iter <= max_ir_; ++iter) {
@@ -816,19 +887,25 @@ void GmresLdl::solve_one(const double* b, double* x) {
       break;
     }
     hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, r, 1.0 / scale, tmp);
+    round_to(uw_, tmp, n, st);
     const double t_inner = now_ms();
     precond_gmres(tmp, d);
     const double inner_ms = now_ms() - t_inner;
     hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, d, scale, d);
+    round_to(uw_, d, n, st);
     const double d_norm = infnrm(d);
     const double x_norm = infnrm(x);
     hipLaunchKernelGGL(k_add, dim3(nblk(n)), dim3(256), 0, st, n, x, d);
+    round_to(uw_, x, n, st);
     residual_f64(A, x, b, r, st);
+    round_to(ur_, r, n, st);
     r_norm = infnrm(r) / bnorm;
     stats_.ir_iterations = static_cast<int32_t>(iter);
+    if (!std::isfinite(r_norm) || !std::isfinite(d_norm))
+      throw Error(kErrNumeric, "the residual is not finite at refinement iteration " + std::to_string(iter));
     // the VERBOSE log's record of the iteration (gmres_ir.hpp:583-596)
     log_.iterations.push_back({static_cast<int32_t>(iter), r_norm, d_norm, now_ms() - t_start, inner_ms});
-    if (d_norm <= x_norm * (10 * std::numeric_limits<double>::epsilon())) {
+    if (d_norm <= x_norm * (10 * eps_uw)) {
       stats_.stop_reason = 1;
       break;
     }
@@ -838,7 +915,7 @@ void GmresLdl::solve_one(const double* b, double* x) {
     }
     r_prev = r_norm;
   }
-  if (max_ir_ == 0) r_norm = infnrm(r) / bnorm;
+  if (max_ir_ == 0) r_norm = bnorm > 0 ? infnrm(r) / bnorm : 0.0;
   stats_.final_rel_residual = r_norm;
   log_.stop_reason = stats_.stop_reason;
   log_.total_iterations = static_cast<int32_t>(std::min<int64_t>(iter, max_ir_));
@@ -900,10 +977,12 @@ void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool devi
     }
     hipLaunchKernelGGL(k_perm_in, dim3(nblk(n)), dim3(256), 0, st, n, bin, d_perm_.as<int32_t>(), d_ad_.as<double>(),
                        d_b_.as<double>());
+    round_to(uw_, d_b_.as<double>(), n, st);  // b_scaled = VectorMultiply<UW>(b, AD_)
     solve_one(d_b_.as<double>(), d_x_.as<double>());
     double* xout = device_ptr ? xc : d_d_.as<double>();
     hipLaunchKernelGGL(k_perm_out, dim3(nblk(n)), dim3(256), 0, st, n, d_x_.as<double>(), d_perm_.as<int32_t>(),
                        d_ad_.as<double>(), xout);
+    round_to(uw_, xout, n, st);  // VectorMultiply<UW>(x, AD_)
     if (!device_ptr) MPIR_HIP(hipMemcpyAsync(xc, xout, n * sizeof(double), hipMemcpyDeviceToHost, st));
     sync_check();
   }

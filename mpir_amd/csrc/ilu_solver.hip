@@ -158,7 +158,7 @@ void GmresLdl::factor_ilu_t() {
     else stats_.ilu_sweeps = 0.0 < tol_ ? 2 : ilu_max_sweeps_;
   }
   // sign count of the pivots U_jj (the signs of 1 / U_jj)
-  count_positive<T>(n_, Di, d_cnt_.as<int>(), st);
+  count_positive<T>(n_, Di, Di, d_cnt_.as<int>(), d_nonfinite_.as<int>(), st);
 }
 
 void GmresLdl::factor_ilu() {

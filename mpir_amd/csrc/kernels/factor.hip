@@ -572,20 +572,35 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
     if (dst[q] != nullptr) *dst[q] = cur[q] - St[w + 4 * q][lane];
 }
 
-// Count of D[i] > 0: one atomic per workgroup (256 threads), few workgroups,
-// so that the adds to the one counter do not queue up.
+// Count of D[i] > 0 (the inertia QDLDL_factor returns) and of i with D[i]
+// or Dinv[i] not finite: one atomic per counter and workgroup (256 threads),
+// few workgroups, so that the adds to one counter do not queue up. An
+// overflow or NaN anywhere in a column of L reaches the pivot of the row it
+// sits in (D_i -= L_ij^2 D_j), so the pivots tell whether the factor is finite.
 template <class T>
-__global__ void __launch_bounds__(256) k_count_positive(int n, const T* __restrict__ D, int* __restrict__ out) {
-  __shared__ int part[4];
-  int cnt = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    cnt += D[i] > T(0) ? 1 : 0;
-  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+__global__ void __launch_bounds__(256) k_count_positive(int n, const T* __restrict__ D, const T* __restrict__ Dinv,
+                                                        int* __restrict__ out, int* __restrict__ nonfinite) {
+  __shared__ int part[8];
+  int cnt = 0, bad = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const T d = D[i];
+    cnt += d > T(0) ? 1 : 0;
+    bad += (isfinite(d) && isfinite(Dinv[i])) ? 0 : 1;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    cnt += __shfl_down(cnt, off);
+    bad += __shfl_down(bad, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    part[threadIdx.x >> 6] = cnt;
+    part[4 + (threadIdx.x >> 6)] = bad;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     const int c = part[0] + part[1] + part[2] + part[3];
+    const int b = part[4] + part[5] + part[6] + part[7];
     if (c) atomicAdd(out, c);
+    if (b) atomicAdd(nonfinite, b);
   }
 }
 
@@ -699,11 +714,11 @@ __global__ void k_count_nonfinite(int64_t n, const H* __restrict__ v, int* __res
   if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(out, cnt);
 }
 
+// adds the count of non-finite rounded entries to *nonfinite
 template <class H>
 void round_panel(int64_t n, const float* src, H* dst, int* nonfinite, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_round_panel<H>, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, src, dst);
-  MPIR_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), st));
   hipLaunchKernelGGL(k_count_nonfinite<H>, dim3(std::min<unsigned>(2048, blocks_for(n, 256))), dim3(256), 0, st, n,
                      dst, nonfinite);
 }
@@ -719,11 +734,13 @@ void reciprocal(int n, const T* x, T* y, hipStream_t st) {
   hipLaunchKernelGGL(k_reciprocal<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, x, y);
 }
 
+// *out = #{D_i > 0}, *nonfinite = #{D_i or Dinv_i not finite} (both reset)
 template <class T>
-void count_positive(int n, const T* D, int* out, hipStream_t st) {
+void count_positive(int n, const T* D, const T* Dinv, int* out, int* nonfinite, hipStream_t st) {
   MPIR_HIP(hipMemsetAsync(out, 0, sizeof(int), st));
+  MPIR_HIP(hipMemsetAsync(nonfinite, 0, sizeof(int), st));
   hipLaunchKernelGGL(k_count_positive<T>, dim3(std::min<unsigned>(256, std::max(1u, blocks_for(n, 256)))), dim3(256), 0,
-                     st, n, D, out);
+                     st, n, D, Dinv, out, nonfinite);
 }
 
 template void factor_numeric<double>(const FactorArgs<double>&, const DeviceFactorPlan&, const double*, int64_t, int,
@@ -736,7 +753,7 @@ template void round_panel<__half>(int64_t, const float*, __half*, int*, hipStrea
 template void round_panel<__hip_bfloat16>(int64_t, const float*, __hip_bfloat16*, int*, hipStream_t);
 template void reciprocal<double>(int, const double*, double*, hipStream_t);
 template void reciprocal<float>(int, const float*, float*, hipStream_t);
-template void count_positive<double>(int, const double*, int*, hipStream_t);
-template void count_positive<float>(int, const float*, int*, hipStream_t);
+template void count_positive<double>(int, const double*, const double*, int*, int*, hipStream_t);
+template void count_positive<float>(int, const float*, const float*, int*, int*, hipStream_t);
 
 }  // namespace mpir

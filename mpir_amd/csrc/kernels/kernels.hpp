@@ -82,7 +82,7 @@ void round_panel(int64_t n, const float* src, H* dst, int* nonfinite, hipStream_
 template <class T>
 void reciprocal(int n, const T* x, T* y, hipStream_t st);
 template <class T>
-void count_positive(int n, const T* D, int* out, hipStream_t st);
+void count_positive(int n, const T* D, const T* Dinv, int* out, int* nonfinite, hipStream_t st);
 
 // ---------------------------------------------------------------- solves
 // Entries allocated past the end of a panel pool: the solves' 16-byte loads
@@ -112,11 +112,13 @@ constexpr int kScheduleDag = 1;
 
 // Counters of the solves (int), zeroed before every application:
 //   kNumSolveCounters per supernode (completion counts), then one per
-//   column block of the big fronts (bx partial sums in), then kDispensers
-//   task dispensers per direction, each on its own 128-byte line.
+//   column block of the big fronts (bx partial sums in), then per direction
+//   kDispensers task dispensers and one arrival counter, each on its own
+//   128-byte line.
 constexpr int kNumSolveCounters = 5;
 constexpr int kDispensers = 16;
 constexpr int kDispenserStride = 32;
+constexpr int kDispenserLines = kDispensers + 1;
 
 // Device view of the solve plan (task lists, gather table, offsets, work
 // areas of the solve precision).

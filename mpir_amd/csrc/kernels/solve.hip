@@ -700,15 +700,52 @@ __global__ void __launch_bounds__(256) k_level_tiles(SolveArgs<TF, TD> a, SolveD
 }
 
 // ---------------------------------------------------------------- dag schedule
-// The block's next task. The list is dealt round-robin onto kDispensers
-// counters, each on its own 128-byte line (one counter serialises every
-// workgroup of the device on one address); block b draws from counter
-// b % kDispensers, so each counter hands out its share of the list in list
-// order.
-__device__ __forceinline__ int next_task(int* dispensers, int* tsh) {
-  const int dn = blockIdx.x % kDispensers;
+// The task list (topological: a task waits only on tasks before it) is dealt
+// round-robin onto kDispensers counters, each on its own 128-byte line (one
+// counter would serialise every workgroup of the device on one address), so
+// class k hands out tasks k, k + 16, k + 32, ... in list order.
+//
+// Liveness. A workgroup takes the class of its arrival order: the first
+// kDispensers workgroups that start, on whatever XCDs the hardware placed
+// them, cover every class, and a workgroup leaves a class only when the
+// class is exhausted (then it moves on to the next class, and it exits when
+// all are). So while a class has tasks left, some running workgroup draws
+// from it. Tasks are drawn in list order within a class and every task waits
+// only on earlier tasks, so the earliest unfinished task that has been drawn
+// never waits on one that has not: the solve always progresses, whichever
+// workgroups other streams' kernels keep from becoming resident. (Classes
+// taken from blockIdx % kDispensers instead starve when the XCD that would
+// run, say, block 3 is full: workgroups are dealt round-robin over the XCDs,
+// so blocks 16, 17, ... can run while block 3 waits.)
+struct Dispenser {
+  int cls = 0;    // class this workgroup draws from (thread 0's copy)
+  int tried = 0;  // classes found exhausted
+};
+
+__device__ __forceinline__ Dispenser arrive(int* lines, int* tsh) {
+  Dispenser d;
+  if (threadIdx.x == 0) *tsh = add_dev(lines + kDispenserStride * kDispensers, 1) % kDispensers;
   __syncthreads();
-  if (threadIdx.x == 0) *tsh = add_dev(dispensers + kDispenserStride * dn, 1) * kDispensers + dn;
+  d.cls = *tsh;
+  return d;
+}
+
+// the workgroup's next task, or ntasks when every class is exhausted
+__device__ __forceinline__ int next_task(int* lines, Dispenser& ds, int ntasks, int* tsh) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = ntasks;
+    while (ds.tried < kDispensers) {
+      const int c = add_dev(lines + kDispenserStride * ds.cls, 1) * kDispensers + ds.cls;
+      if (c < ntasks) {
+        t = c;
+        break;
+      }
+      ++ds.tried;
+      ds.cls = ds.cls + 1 == kDispensers ? 0 : ds.cls + 1;
+    }
+    *tsh = t;
+  }
   __syncthreads();
   return *tsh;
 }
@@ -723,8 +760,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
   const int ns = d.ns;
+  int* lines = cnt + d.cnt_disp;
+  Dispenser ds = arrive(lines, &tsh);
   for (;;) {
-    const int t = next_task(cnt + d.cnt_disp, &tsh);
+    const int t = next_task(lines, ds, d.nfwd, &tsh);
     if (t >= d.nfwd) break;
     const Task4 tk = d.fwd[t];
     if (tk.a == kDagSmall) {
@@ -753,8 +792,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
   const int ns = d.ns;
+  int* lines = cnt + d.cnt_disp + kDispenserLines * kDispenserStride;
+  Dispenser ds = arrive(lines, &tsh);
   for (;;) {
-    const int t = next_task(cnt + d.cnt_disp + kDispensers * kDispenserStride, &tsh);
+    const int t = next_task(lines, ds, d.nbwd, &tsh);
     if (t >= d.nbwd) break;
     const Task4 tk = d.bwd[t];
     if (tk.a == kDagSmall) {
@@ -802,11 +843,8 @@ void ldl_solve_group(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, cons
                      hipStream_t st) {
   hipLaunchKernelGGL(k_zero, dim3((d.ncounters + 255) / 256), dim3(256), 0, st, d.ncounters, d.counters);
   if (dp.schedule == kScheduleDag) {
-    // Every task waits only on tasks before it in the list. The smallest
-    // unfinished task is therefore either running or next on its
-    // dispenser, whose earlier tasks are all done, so a block of that
-    // dispenser takes it: blocks 0 .. kDispensers - 1 are dispatched first
-    // and serve every dispenser until its share of the list runs out.
+    // at least kDispensers workgroups, so that every class can be covered
+    // (next_task states why the persistent launch always progresses)
     static const int grid_f = resident_blocks(k_fwd_dag<NR, TF, TD, TX>, 256);
     static const int grid_b = resident_blocks(k_bwd_dag<NR, TF, TD, TX>, 256);
     if (d.nfwd > 0)

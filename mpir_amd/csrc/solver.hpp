@@ -30,6 +30,26 @@ struct BatchStats {
 
 enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1, kOrderNd = 2 };
 
+// Dnrm2 of the two-vector {a, b}, as PrecondGmres forms the Givens norm
+// (include/MPIR/gmres_ir.hpp:702; ops.hpp:231-255: scaled sum of squares,
+// last entry first)
+inline double dnrm2_pair(double a, double b) {
+  double scale = 0.0, ssq = 1.0;
+  const double v[2] = {b, a};
+  for (double x : v) {
+    if (x != 0.0) {
+      const double ax = x < 0 ? -x : x;
+      if (scale < ax) {
+        ssq = 1.0 + ssq * ((scale / ax) * (scale / ax));
+        scale = ax;
+      } else {
+        ssq = ssq + ((ax / scale) * (ax / scale));
+      }
+    }
+  }
+  return scale * __builtin_sqrt(ssq);
+}
+
 // preconditioner M of GMRES-IR
 constexpr int kPrecLdl = 0;  // L D L^T of the reordered matrix (the default)
 constexpr int kPrecIlu = 1;  // FGPILU of the matrix in its given order, the reference's shipped Compute
@@ -171,6 +191,7 @@ class GmresLdl {
   // Load a factor given in QDLDL's CSC layout (host arrays, Lx/Dinv in the
   // handle's factor precision) for use as M^{-1} only.
   void load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, const void* Lx, const void* Dinv);
+  int64_t plan_builds() const { return plan_builds_; }
 
  private:
   void solve_one(const double* b_dev, double* x_dev);
@@ -230,6 +251,11 @@ class GmresLdl {
   int64_t true_nnz_l_ = 0;
   double true_flops_ = 0.0;
   bool analyzed_ = false, factored_ = false, precond_only_ = false;
+  // pattern of the factor given to load_factor_csc, and how many plans the
+  // handle has built from such patterns
+  std::vector<int64_t> loaded_Lp_;
+  std::vector<int32_t> loaded_Li_;
+  int64_t plan_builds_ = 0;
   hipStream_t stream_ = nullptr;
   int32_t n_ = 0;
   int64_t nnz_ = 0;

@@ -16,6 +16,17 @@
 #include "mpir/luir.hpp"
 #include "mpir/qdldl.hpp"
 
+#if __cplusplus >= 202002L
+// the reference's compile-time checks of Refinable (fp_concepts.hpp:116-127)
+static_assert(Refinable<double, double, double>);
+static_assert(Refinable<float, double, double>);
+static_assert(Refinable<float, float, double>);
+static_assert(!Refinable<double, double, float>);
+static_assert(!Refinable<double, float, double>);
+static_assert(!Refinable<double, float, float>);
+static_assert(!Refinable<int, double, double>);
+#endif
+
 namespace {
 
 struct Csc {
@@ -185,6 +196,35 @@ int main(int argc, char** argv) {
   const double res64 = rel_residual(K, solver64.Solve(b), b);
   std::snprintf(msg, sizeof msg, "GmresLDLIR<double,double,double>: relative residual %.2e < 1e-13", res64);
   expect(res64 < 1e-13, msg);
+  // DumpLog: the VERBOSE log of the last Solve as JSON text
+  const std::string log = solver64.DumpLog();
+  expect(log.find("\"events\"") != std::string::npos && log.find("\"gmres_ir\"") != std::string::npos &&
+             log.find("\"stop_reason\"") != std::string::npos,
+         "DumpLog returns the solver log");
+  // Refinable<UF, UW, UR> combinations with a float working (and residual)
+  // precision: x is held in UW, so it is as accurate as a float can hold
+  {
+    std::vector<float> bf(b.begin(), b.end());
+    GmresLDLIR<float, float, double> sff;
+    sff.Compute<double>(Ap, Ai, K.x, 0);
+    const std::vector<float> xf = sff.Solve(bf);
+    std::vector<double> xfd(xf.begin(), xf.end());
+    const double resf = rel_residual(K, xfd, b);
+    const mpir_gmres_stats stf = sff.Stats();
+    std::snprintf(msg, sizeof msg, "GmresLDLIR<float,float,double>: relative residual %.2e < 1e-5, stop reason %d",
+                  resf, stf.stop_reason);
+    expect(xf.size() == static_cast<std::size_t>(K.n) && resf < 1e-5 && stf.stop_reason >= 1, msg);
+    GmresLDLIR<float, float, float> s3;
+    s3.SetTolerance(1e-6f);
+    s3.Compute<double>(Ap, Ai, K.x, 0);
+    std::vector<double> x3d;
+    for (float v : s3.Solve(bf)) x3d.push_back(v);
+    const double res3 = rel_residual(K, x3d, b);
+    std::snprintf(msg, sizeof msg, "GmresLDLIR<float,float,float>: relative residual %.2e < 1e-5", res3);
+    expect(res3 < 1e-5, msg);
+    GmresLDLIR<double, double, double> sdd;
+    expect(sdd.Solve(b).empty(), "Solve before Compute (n = 0) returns an empty vector");
+  }
   // the reference's shipped path: FGPILU with ILU_K = 0 and 5 sweeps, as
   // examples/example.cpp:144-147 calls Compute
   GmresLDLIR<float, double, double> ilu;
