@@ -300,8 +300,10 @@ int mpir_batch_solve(mpir_gmres_ldl_t h, const int32_t* nrhs, const double* cons
 int mpir_batch_solve_dev(mpir_gmres_ldl_t h, const int32_t* nrhs, const double* const* b, double* const* x);
 /* results of the last batch: positive pivots per system (nsys); per system s
  * and right-hand side r at s * max_rhs + r: IR iterations, GMRES iterations,
- * stop reason, final relative residual (scaled system), ms from the start of
- * the solve until the instance stopped. Any output pointer may be NULL. */
+ * stop reason (as mpir_gmres_stats, plus 4: the right-hand side or the
+ * residual is not finite; the other instances of the batch go on), final
+ * relative residual (scaled system), ms from the start of the solve until
+ * the instance stopped. Any output pointer may be NULL. */
 int mpir_batch_stats(mpir_gmres_ldl_t h, int32_t* max_rhs, int32_t* positive_pivots, int32_t* ir_iterations,
                      int32_t* gmres_iterations, int32_t* stop_reason, double* final_rel_residual, double* done_ms);
 /* Plans the calling thread's QDLDL_solve engines have built from factor
@@ -352,6 +354,29 @@ int mpir_server_add_patterns(mpir_server_t s, int32_t count, const int32_t* n, c
 int mpir_server_run(mpir_server_t s, int64_t nreq, const int32_t* pattern, const uint64_t* seed, const int32_t* nrhs,
                     const double* t_arrive, double* t_admit, double* t_done, int32_t* batch, int32_t* lane,
                     int32_t* status, int32_t* ir_max, int32_t* gmres_max, double* relres_max);
+/* Online use: a request with the caller's values Ax (nnz of the pattern, in
+ * its upper-CSC order) and nrhs right-hand sides b (n x nrhs, column-major),
+ * both copied before the call returns; *id names it. The request joins the
+ * queue at once and a free lane admits it into its next batch (the
+ * reference's Compute + Solve of one system, gmres_ir.hpp:271-285). */
+int mpir_server_submit(mpir_server_t s, int32_t pattern, const double* Ax, const double* b, int32_t nrhs,
+                       int64_t* id);
+typedef struct mpir_server_request {
+  int32_t status;           /* 0, or the negative status of its batch */
+  int32_t batch, lane;
+  int32_t ir_max, gmres_max; /* over its right-hand sides */
+  double rel_residual_max;   /* ||r||_inf / ||b_s||_inf, scaled system */
+  double queue_ms;           /* submission to admission */
+  double latency_ms;         /* submission to completion */
+} mpir_server_request;
+/* Waits up to timeout_ms (0: just look; < 0: no limit) for request id.
+ * Returns 1 when it has completed (x, n x nrhs, and info written when
+ * non-null), 0 while it is pending, or a negative status: the request
+ * failed (info written, x not) or the call is invalid. With release != 0 a
+ * completed request is forgotten and its id becomes invalid. Threads may
+ * submit and poll concurrently. */
+int mpir_server_poll(mpir_server_t s, int64_t id, double timeout_ms, double* x, mpir_server_request* info,
+                     int32_t release);
 /* counts[0..5] = completed, failed, batches, largest backlog (arrived, not
  * admitted), peak bytes in flight, most batches in flight at once;
  * ms[0..4] = wall time of the run, then summed over batches: request data,

@@ -469,6 +469,7 @@ class Server:
         self._n: list[int] = []
         self._nnz: list[int] = []
         self._last = None
+        self._req: dict = {}
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -517,6 +518,43 @@ class Server:
         out["latency_ms"] = out["t_done"] - ta
         self._last = (pattern, nrhs)
         return out
+
+    def submit(self, pattern: int, Ax, B) -> int:
+        """A request with the caller's values Ax (the pattern's upper-CSC
+        order) and right-hand sides B (n, or n x nrhs); returns its id."""
+        if not 0 <= pattern < len(self._n):
+            raise ValueError(f"unknown pattern {pattern}")
+        n = self._n[pattern]
+        Ax = np.ascontiguousarray(Ax, dtype=np.float64)
+        if Ax.shape != (self._nnz[pattern],):
+            raise ValueError("Ax must hold the pattern's nnz values")
+        B = np.asarray(B, dtype=np.float64)
+        B2 = B.reshape(n, -1)
+        nrhs = B2.shape[1]
+        bcol = np.ascontiguousarray(B2.T)  # column-major n x nrhs
+        rid = C.c_int64(-1)
+        _check(load().mpir_server_submit(self._h, int(pattern), _ptr(Ax), _ptr(bcol), int(nrhs), C.byref(rid)),
+               "Server.submit")
+        self._req[int(rid.value)] = (n, nrhs, B.ndim == 1)
+        return int(rid.value)
+
+    def poll(self, rid: int, timeout_ms: float = 0.0, release: bool = True):
+        """(done, X, info): X has the shape of the request's B once done.
+        timeout_ms: 0 looks only, < 0 waits without limit. A failed request
+        raises MpirError."""
+        n, nrhs, vec = self._req[rid]
+        x = np.empty(n * nrhs)
+        info = _lib.ServerRequest()
+        rc = int(load().mpir_server_poll(self._h, int(rid), float(timeout_ms), _ptr(x), C.byref(info),
+                                         int(bool(release))))
+        if rc == 0:
+            return False, None, None
+        if release:
+            del self._req[rid]
+        if rc < 0:
+            raise MpirError(rc, "Server.poll")
+        X = x.reshape(nrhs, n).T.copy()
+        return True, (X[:, 0] if vec else X), info.as_dict()
 
     def summary(self) -> dict:
         c = np.zeros(6, np.int64)

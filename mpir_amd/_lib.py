@@ -36,6 +36,16 @@ class LuirStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class ServerRequest(C.Structure):
+    _fields_ = [
+        ("status", i32), ("batch", i32), ("lane", i32), ("ir_max", i32), ("gmres_max", i32),
+        ("rel_residual_max", f64), ("queue_ms", f64), ("latency_ms", f64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "mpir_last_error": (C.c_char_p, []),
@@ -116,6 +126,8 @@ SIGNATURES = {
     "mpir_server_add_patterns": (C.c_int, [P, i32, P, P, P, P, P, i32]),
     "mpir_server_run": (C.c_int, [P, i64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mpir_server_summary": (C.c_int, [P, P, P]),
+    "mpir_server_submit": (C.c_int, [P, i32, P, P, i32, P]),
+    "mpir_server_poll": (C.c_int, [P, i64, f64, P, P, i32]),
     "mpir_server_solution": (C.c_int, [P, i64, P]),
     "mpir_server_request_data": (C.c_int, [P, i32, u64, i32, P, P, P]),
 }

@@ -692,6 +692,15 @@ void GmresLdl::solve_batch(const int32_t* nrhs, const double* const* b, double* 
   bnorms(vb, 2, R, bnorm);
   std::vector<double> r_prev(I, std::numeric_limits<double>::max());
   std::vector<char> act = valid;
+  // an instance whose right-hand side or residual is not finite stops with
+  // reason 4 (a single-system handle reports MPIR_ERR_NUMERIC instead; one
+  // bad instance must not end the others of its batch)
+  for (int q = 0; q < I; ++q)
+    if (act[q] && !std::isfinite(bnorm[q])) {
+      stop[q] = 4;
+      relres[q] = std::numeric_limits<double>::quiet_NaN();
+      act[q] = 0;
+    }
   int64_t iter;
   for (iter = 1; iter <= max_ir_; ++iter) {
     bool any = false;
@@ -703,6 +712,7 @@ void GmresLdl::solve_batch(const int32_t* nrhs, const double* const* b, double* 
       if (act[q] && scale[q] == 0.0) {
         // an exactly zero residual: x is exact (the reference would scale by 1/0)
         stop[q] = 3;
+        ir_it[q] = static_cast<int32_t>(iter);
         relres[q] = 0.0;
         act[q] = 0;
         done[q] = now_ms() - t_start;
@@ -728,7 +738,9 @@ void GmresLdl::solve_batch(const int32_t* nrhs, const double* const* b, double* 
       const double r_norm = rn[q] / bnorm[q];
       ir_it[q] = static_cast<int32_t>(iter);
       relres[q] = r_norm;
-      if (dn[q] <= xn[q] * (10 * std::numeric_limits<double>::epsilon())) {
+      if (!std::isfinite(r_norm) || !std::isfinite(dn[q])) {
+        stop[q] = 4;
+      } else if (dn[q] <= xn[q] * (10 * std::numeric_limits<double>::epsilon())) {
         stop[q] = 1;
       } else if (std::fabs(r_norm - r_prev[q]) < tol_) {
         stop[q] = 2;

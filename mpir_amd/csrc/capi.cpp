@@ -831,10 +831,41 @@ int mpir_server_run(mpir_server_t h, int64_t nreq, const int32_t* pattern, const
     return 0;
   });
 }
+int mpir_server_submit(mpir_server_t h, int32_t pattern, const double* Ax, const double* b, int32_t nrhs,
+                       int64_t* id) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s && id, "null argument");
+    *id = h->s->submit(pattern, Ax, b, nrhs);
+    return 0;
+  });
+}
+int mpir_server_poll(mpir_server_t h, int64_t id, double timeout_ms, double* x, mpir_server_request* info,
+                     int32_t release) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s, "null server handle");
+    ServeRequest r;
+    if (!h->s->wait(id, timeout_ms, &r, x, release != 0)) return 0;
+    if (info) {
+      info->status = r.status;
+      info->batch = r.batch;
+      info->lane = r.lane;
+      info->ir_max = r.ir_max;
+      info->gmres_max = r.gmres_max;
+      info->rel_residual_max = r.rel_residual_max;
+      info->queue_ms = r.t_admit - r.t_arrive;
+      info->latency_ms = r.t_done - r.t_arrive;
+    }
+    if (r.status < 0) {
+      set_last_error("request " + std::to_string(id) + " failed in batch " + std::to_string(r.batch));
+      return r.status;
+    }
+    return 1;
+  });
+}
 int mpir_server_summary(mpir_server_t h, int64_t* counts, double* wall_ms) {
   return guarded([&] {
     MPIR_CHECK(h && h->s, "null server handle");
-    const ServeSummary& m = h->s->summary();
+    const ServeSummary m = h->s->summary();
     if (wall_ms) {
       wall_ms[1] = m.prepare_ms;
       wall_ms[2] = m.plan_ms;

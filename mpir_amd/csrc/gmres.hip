@@ -883,6 +883,7 @@ void GmresLdl::solve_one(const double* b, double* x) {
     // (gmres_ir.hpp:562-563) and carry NaN forward; x is exact, so stop.
     if (scale == 0.0) {
       stats_.stop_reason = 3;
+      stats_.ir_iterations = static_cast<int32_t>(iter);  // counted as the log counts it
       r_norm = 0.0;
       break;
     }

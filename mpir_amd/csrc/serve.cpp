@@ -83,18 +83,36 @@ Server::Server(const ServeOptions& opt) : opt_(opt) {
     auto s = std::make_unique<GmresLdl>(opt.uf, kF64, kF64);
     s->set_tol(opt.tol);
     s->set_ordering(opt.ordering);
-    // Lanes run concurrently on their own streams. The level schedule of the
-    // triangular solves never waits across workgroups, so a lane's solve
-    // cannot stall behind another lane's kernels; on joined batches it is
-    // also the faster schedule.
+    // on joined batches the level schedule of the triangular solves is the
+    // faster one (every level holds many fronts)
     s->set_schedule(kScheduleLevels);
     s->set_max_ir(opt.max_ir);
     s->set_max_gmres(opt.max_gmres);
     solvers_.push_back(std::move(s));
   }
+  sum_.lane_busy_ms.assign(opt_.lanes, 0.0);
+  t0_ = now_ms();
+  for (int l = 0; l < opt_.lanes; ++l) lanes_.emplace_back(&Server::lane_main, this, l);
 }
 
-Server::~Server() = default;
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stop_ = true;
+  }
+  work_cv_.notify_all();
+  for (auto& t : lanes_) t.join();
+}
+
+ServeSummary Server::summary() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  return sum_;
+}
+
+int32_t Server::npatterns() const {
+  std::lock_guard<std::mutex> lk(mu_);
+  return static_cast<int32_t>(patterns_.size());
+}
 
 int32_t Server::add_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* base, int32_t n1) {
   return add_patterns(1, &n, &Ap, &Ai, &base, &n1, 1);
@@ -114,7 +132,8 @@ int32_t Server::add_patterns(int32_t count, const int32_t* n, const int32_t* con
   } catch (const std::invalid_argument& e) {
     throw Error(kErrInvalid, e.what());
   }
-  const int32_t first = npatterns();
+  std::lock_guard<std::mutex> lk(mu_);
+  const int32_t first = static_cast<int32_t>(patterns_.size());
   for (int32_t k = 0; k < count; ++k) add_entry(entries[k], n[k], Ap[k], Ai[k], base[k], n1[k]);
   return first;
 }
@@ -139,7 +158,8 @@ void Server::add_entry(std::shared_ptr<const PatternEntry> entry, int32_t n, con
 }
 
 const ServePattern& Server::pattern(int32_t id) const {
-  MPIR_CHECK(id >= 0 && id < npatterns(), "pattern id out of range");
+  std::lock_guard<std::mutex> lk(mu_);
+  MPIR_CHECK(id >= 0 && id < static_cast<int32_t>(patterns_.size()), "pattern id out of range");
   return patterns_[id];
 }
 
@@ -148,11 +168,11 @@ const std::vector<double>& Server::solution(int64_t i) const {
   return x_[i];
 }
 
-int64_t Server::batch_bytes(const std::vector<const ServeRecord*>& rs) const {
+int64_t Server::batch_bytes(const std::vector<const ServeRequest*>& rs) const {
   const int sf = opt_.uf == kF64 ? 8 : 4;
   int64_t fixed = 0, n = 0;
   int r = 0;
-  for (const ServeRecord* q : rs) {
+  for (const ServeRequest* q : rs) {
     const PatternEntry& e = *patterns_[q->pattern].entry;
     fixed += pattern_footprint(e, sf, 0, opt_.max_gmres);
     n += e.n;
@@ -162,72 +182,149 @@ int64_t Server::batch_bytes(const std::vector<const ServeRecord*>& rs) const {
   return fixed + static_cast<int64_t>(r) * n * 8 * (opt_.max_gmres + 9);
 }
 
+int64_t Server::enqueue(std::unique_ptr<ServeRequest> r) {
+  int64_t id;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    MPIR_CHECK(!stop_, "the server is stopping");
+    r->t_arrive = now_ms() - t0_;
+    id = static_cast<int64_t>(reqs_.size());
+    reqs_.push_back(std::move(r));
+    queue_.push_back(id);
+    sum_.max_backlog = std::max<int64_t>(sum_.max_backlog, static_cast<int64_t>(queue_.size()));
+  }
+  work_cv_.notify_one();
+  return id;
+}
+
+int64_t Server::submit(int32_t pattern, const double* Ax, const double* b, int32_t nrhs) {
+  MPIR_CHECK(Ax && b, "null request data");
+  MPIR_CHECK(nrhs >= 1 && nrhs <= 64, "requests carry 1..64 right-hand sides");
+  const ServePattern& p = this->pattern(pattern);
+  auto r = std::make_unique<ServeRequest>();
+  r->pattern = pattern;
+  r->nrhs = nrhs;
+  r->ax.assign(Ax, Ax + p.nnz);
+  r->b.assign(b, b + static_cast<int64_t>(p.n) * nrhs);
+  return enqueue(std::move(r));
+}
+
+int64_t Server::submit_seeded(int32_t pattern, uint64_t seed, int32_t nrhs) {
+  MPIR_CHECK(nrhs >= 1 && nrhs <= 64, "requests carry 1..64 right-hand sides");
+  (void)this->pattern(pattern);  // range check
+  auto r = std::make_unique<ServeRequest>();
+  r->pattern = pattern;
+  r->nrhs = nrhs;
+  r->seeded = true;
+  r->seed = seed;
+  return enqueue(std::move(r));
+}
+
+bool Server::wait(int64_t id, double timeout_ms, ServeRequest* info, double* x, bool release) {
+  std::unique_lock<std::mutex> lk(mu_);
+  MPIR_CHECK(id >= 0 && id < static_cast<int64_t>(reqs_.size()) && reqs_[id], "unknown or released request id");
+  ServeRequest* r = reqs_[id].get();
+  auto ready = [&] { return r->done; };
+  if (timeout_ms < 0) done_cv_.wait(lk, ready);
+  else if (timeout_ms > 0) done_cv_.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), ready);
+  if (!r->done) return false;
+  if (x && r->status == 0) std::copy(r->x.begin(), r->x.end(), x);
+  if (info) {
+    *info = ServeRequest();
+    info->pattern = r->pattern;
+    info->nrhs = r->nrhs;
+    info->seeded = r->seeded;
+    info->seed = r->seed;
+    info->t_arrive = r->t_arrive;
+    info->t_admit = r->t_admit;
+    info->t_done = r->t_done;
+    info->batch = r->batch;
+    info->lane = r->lane;
+    info->status = r->status;
+    info->ir_max = r->ir_max;
+    info->gmres_max = r->gmres_max;
+    info->rel_residual_max = r->rel_residual_max;
+    info->bytes = r->bytes;
+    info->done = true;
+    if (!x) info->x = r->x;
+  }
+  if (release) reqs_[id].reset();
+  return true;
+}
+
 void Server::run(std::vector<ServeRecord>& reqs) {
+  const int32_t np = npatterns();
   for (size_t i = 0; i < reqs.size(); ++i) {
     const ServeRecord& r = reqs[i];
-    MPIR_CHECK(r.pattern >= 0 && r.pattern < npatterns(), "request names an unknown pattern");
+    MPIR_CHECK(r.pattern >= 0 && r.pattern < np, "request names an unknown pattern");
     MPIR_CHECK(r.nrhs >= 1 && r.nrhs <= 64, "requests carry 1..64 right-hand sides");
     MPIR_CHECK(i == 0 || r.t_arrive >= reqs[i - 1].t_arrive, "requests must be sorted by arrival time");
   }
-  sum_ = ServeSummary();
-  sum_.lane_busy_ms.assign(opt_.lanes, 0.0);
-  x_.assign(opt_.keep_x ? reqs.size() : 0, std::vector<double>());
-  queue_.clear();
-  next_ = 0;
-  outstanding_ = static_cast<int64_t>(reqs.size());
-  inflight_bytes_ = concurrent_ = 0;
-  batch_seq_ = 0;
-  t0_ = now_ms();
-  std::vector<std::thread> lanes;
-  for (int l = 0; l < opt_.lanes; ++l) lanes.emplace_back(&Server::lane_main, this, l, &reqs);
-  for (auto& t : lanes) t.join();
-  sum_.wall_ms = now_ms() - t0_;
-  for (const ServeRecord& r : reqs) {
-    if (r.status == 0) sum_.completed++;
-    else sum_.failed++;
+  double t_run;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    MPIR_CHECK(queue_.empty() && concurrent_ == 0, "run needs an idle server");
+    sum_ = ServeSummary();
+    sum_.lane_busy_ms.assign(opt_.lanes, 0.0);
+    t_run = now_ms();
   }
+  x_.assign(opt_.keep_x ? reqs.size() : 0, std::vector<double>());
+  // submit each request at its arrival time (a late submitter catches up)
+  std::vector<int64_t> ids(reqs.size());
+  for (size_t i = 0; i < reqs.size(); ++i) {
+    const double wait = t_run + reqs[i].t_arrive - now_ms();
+    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double, std::milli>(wait));
+    ids[i] = submit_seeded(reqs[i].pattern, reqs[i].seed, reqs[i].nrhs);
+  }
+  const double off = t_run - t0_;  // run clock = server clock - off
+  for (size_t i = 0; i < reqs.size(); ++i) {
+    ServeRequest info;
+    wait(ids[i], -1.0, &info, nullptr, true);
+    ServeRecord& r = reqs[i];
+    r.t_admit = info.t_admit - off;
+    r.t_done = info.t_done - off;
+    r.batch = info.batch;
+    r.lane = info.lane;
+    r.status = info.status;
+    r.ir_max = info.ir_max;
+    r.gmres_max = info.gmres_max;
+    r.rel_residual_max = info.rel_residual_max;
+    r.bytes = info.bytes;
+    if (opt_.keep_x && info.status == 0) x_[i] = std::move(info.x);
+  }
+  std::lock_guard<std::mutex> lk(mu_);
+  sum_.wall_ms = now_ms() - t_run;
 }
 
-void Server::lane_main(int lane, std::vector<ServeRecord>* reqs) {
-  const int64_t total = static_cast<int64_t>(reqs->size());
+void Server::lane_main(int lane) {
   for (;;) {
-    std::vector<int64_t> ids;
+    std::vector<ServeRequest*> batch;
     int64_t bytes = 0;
     int32_t bid = 0;
     {
       std::unique_lock<std::mutex> lk(mu_);
       for (;;) {
-        // release the requests that have arrived
-        const double t = now_ms() - t0_;
-        while (next_ < total && (*reqs)[next_].t_arrive <= t) queue_.push_back(next_++);
-        sum_.max_backlog = std::max<int64_t>(sum_.max_backlog, static_cast<int64_t>(queue_.size()));
-        if (outstanding_ == 0) return;
         if (queue_.empty()) {
-          if (next_ >= total) {
-            cv_.wait(lk);  // others still running; nothing left to admit
-            if (outstanding_ == 0) return;
-            continue;
-          }
-          const double wait = (*reqs)[next_].t_arrive - t;
-          cv_.wait_for(lk, std::chrono::duration<double, std::milli>(std::max(0.0, wait)));
+          if (stop_) return;
+          work_cv_.wait(lk);
           continue;
         }
         // admission in arrival order by device footprint
-        std::vector<const ServeRecord*> cand;
-        while (!queue_.empty() && static_cast<int>(ids.size()) < opt_.max_batch) {
-          const int64_t id = queue_.front();
-          cand.push_back(&(*reqs)[id]);
+        std::vector<const ServeRequest*> cand;
+        while (!queue_.empty() && static_cast<int>(batch.size()) < opt_.max_batch) {
+          ServeRequest* r = reqs_[queue_.front()].get();
+          cand.push_back(r);
           const int64_t b = batch_bytes(cand);
-          if (inflight_bytes_ + b > opt_.budget_bytes && !(ids.empty() && inflight_bytes_ == 0)) {
+          if (inflight_bytes_ + b > opt_.budget_bytes && !(batch.empty() && inflight_bytes_ == 0)) {
             cand.pop_back();
             break;
           }
-          ids.push_back(id);
+          batch.push_back(r);
           bytes = b;
           queue_.pop_front();
         }
-        if (ids.empty()) {
-          cv_.wait(lk);  // memory frees when a batch in flight completes
+        if (batch.empty()) {
+          work_cv_.wait(lk);  // memory frees when a batch in flight completes
           continue;
         }
         inflight_bytes_ += bytes;
@@ -237,51 +334,62 @@ void Server::lane_main(int lane, std::vector<ServeRecord>* reqs) {
         bid = batch_seq_++;
         sum_.batches++;
         const double ta = now_ms() - t0_;
-        for (int64_t id : ids) {
-          (*reqs)[id].t_admit = ta;
-          (*reqs)[id].bytes = bytes;
+        for (ServeRequest* r : batch) {
+          r->t_admit = ta;
+          r->bytes = bytes;
         }
         break;
       }
     }
     const double t_begin = now_ms();
-    process(lane, reqs, ids, bid);
+    process(lane, batch, bid);
     {
       std::lock_guard<std::mutex> lk(mu_);
       sum_.lane_busy_ms[lane] += now_ms() - t_begin;
       inflight_bytes_ -= bytes;
       concurrent_--;
-      outstanding_ -= static_cast<int64_t>(ids.size());
+      for (ServeRequest* r : batch) {
+        r->done = true;
+        if (r->status == 0) sum_.completed++;
+        else sum_.failed++;
+      }
     }
-    cv_.notify_all();
+    done_cv_.notify_all();
+    work_cv_.notify_all();
   }
 }
 
-void Server::process(int lane, std::vector<ServeRecord>* reqs, const std::vector<int64_t>& ids, int32_t batch_id) {
+void Server::process(int lane, const std::vector<ServeRequest*>& batch, int32_t batch_id) {
   GmresLdl& s = *solvers_[lane];
-  const size_t B = ids.size();
+  const size_t B = batch.size();
   const double t_prep = now_ms();
-  std::vector<std::vector<double>> ax(B), bv(B), xv(B);
+  std::vector<const ServePattern*> pats(B);
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (size_t k = 0; k < B; ++k) pats[k] = &patterns_[batch[k]->pattern];
+  }
+  std::vector<std::vector<double>> xv(B);
   std::vector<std::shared_ptr<const PatternEntry>> parts(B);
   std::vector<int32_t> nrhs(B);
-  // request data (deterministic from the pattern and the seed)
+  // request data: the caller's, or derived from the pattern and the seed
   parallel_for(static_cast<int64_t>(B), opt_.threads, [&](int64_t k) {
-    const ServeRecord& r = (*reqs)[ids[k]];
-    const ServePattern& p = patterns_[r.pattern];
-    ax[k].resize(p.nnz);
-    bv[k].resize(static_cast<size_t>(p.n) * r.nrhs);
+    ServeRequest& r = *batch[k];
+    const ServePattern& p = *pats[k];
     xv[k].resize(static_cast<size_t>(p.n) * r.nrhs);
-    std::vector<double> xt(static_cast<size_t>(p.n) * r.nrhs);
-    materialize(p, r.seed, r.nrhs, ax[k].data(), bv[k].data(), xt.data());
+    if (r.seeded) {
+      r.ax.resize(p.nnz);
+      r.b.resize(static_cast<size_t>(p.n) * r.nrhs);
+      std::vector<double> xt(static_cast<size_t>(p.n) * r.nrhs);
+      materialize(p, r.seed, r.nrhs, r.ax.data(), r.b.data(), xt.data());
+    }
   });
   std::vector<const double*> axp(B), bp(B);
   std::vector<double*> xp(B);
   for (size_t k = 0; k < B; ++k) {
-    const ServeRecord& r = (*reqs)[ids[k]];
-    parts[k] = patterns_[r.pattern].entry;
-    nrhs[k] = r.nrhs;
-    axp[k] = ax[k].data();
-    bp[k] = bv[k].data();
+    parts[k] = pats[k]->entry;
+    nrhs[k] = batch[k]->nrhs;
+    axp[k] = batch[k]->ax.data();
+    bp[k] = batch[k]->b.data();
     xp[k] = xv[k].data();
   }
   int status = 0;
@@ -299,30 +407,32 @@ void Server::process(int lane, std::vector<ServeRecord>* reqs, const std::vector
     status = kErrInternal;
   }
   const double t4 = now_ms();
-  {
-    std::lock_guard<std::mutex> lk(mu_);
-    sum_.prepare_ms += t1 - t_prep;
-    sum_.plan_ms += t2 - t1;
-    sum_.factor_ms += t3 - t2;
-    sum_.solve_ms += t4 - t3;
-  }
-  const double td = t4 - t0_;
   const BatchStats& st = s.batch_stats();
+  std::lock_guard<std::mutex> lk(mu_);
+  sum_.prepare_ms += t1 - t_prep;
+  sum_.plan_ms += t2 - t1;
+  sum_.factor_ms += t3 - t2;
+  sum_.solve_ms += t4 - t3;
+  const double td = t4 - t0_;
   for (size_t k = 0; k < B; ++k) {
-    ServeRecord& r = (*reqs)[ids[k]];
+    ServeRequest& r = *batch[k];
     r.t_done = td;
     r.batch = batch_id;
     r.lane = lane;
     r.status = status;
+    std::vector<double>().swap(r.ax);
+    std::vector<double>().swap(r.b);
     if (status == 0) {
       const int R = st.max_rhs;
       for (int c = 0; c < r.nrhs; ++c) {
         const size_t o = k * R + c;
+        // an instance whose data are not finite stops with reason 4
+        if (st.stop_reason[o] == 4) r.status = kErrNumeric;
         r.ir_max = std::max(r.ir_max, st.ir_iterations[o]);
         r.gmres_max = std::max(r.gmres_max, st.gmres_iterations[o]);
         r.rel_residual_max = std::max(r.rel_residual_max, st.final_rel_residual[o]);
       }
-      if (opt_.keep_x) x_[ids[k]] = std::move(xv[k]);
+      r.x = std::move(xv[k]);
     }
   }
 }

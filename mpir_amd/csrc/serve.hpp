@@ -1,15 +1,19 @@
-// Serving: a stream of solve requests on one device with continuous batching.
+// Serving: solve requests on one device with continuous batching.
 //
 // Each request names a registered sparsity pattern and carries fresh values
-// and 1..R right-hand sides. Requests are admitted in arrival order into
-// batches by their device-memory footprint: a batch takes queued requests
-// while the bytes of every batch in flight stay within the budget. Several
-// lanes (a host thread and a HIP stream each, with their own solver handle)
-// run batches concurrently, so requests that arrive while one batch is on
-// the device are admitted by the next free lane and overlap with it on the
-// device. The symbolic phase of a pattern is computed once (PatternCache)
-// and reused by every request of that pattern. A request's result does not
-// depend on the batch it lands in (see batch.hip).
+// and 1..R right-hand sides, supplied by the caller (submit) or derived from
+// a seed (submit_seeded, for benchmark streams). Requests are admitted in
+// arrival order into batches by their device-memory footprint: a batch takes
+// queued requests while the bytes of every batch in flight stay within the
+// budget. Several lanes (a host thread and a HIP stream each, with their own
+// solver handle) run for the server's lifetime and take batches whenever
+// they are free, so requests that arrive while one batch is on the device
+// are admitted by the next free lane and overlap with it on the device. The
+// symbolic phase of a pattern is computed once (PatternCache) and reused by
+// every request of that pattern. A request's result does not depend on the
+// batch it lands in (see batch.hip). Callers poll or wait for a request by
+// its id; the reference splits the same work into Compute (values) and
+// Solve (right-hand sides), gmres_ir.hpp:271-285.
 #pragma once
 
 #include <condition_variable>
@@ -17,6 +21,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "batch_plan.hpp"
@@ -54,6 +59,21 @@ struct ServePattern {
 // < 1.05 h_ii) and K quasi-definite.
 void materialize(const ServePattern& p, uint64_t seed, int nrhs, double* Ax, double* b, double* x_true);
 
+// One request inside the server (times in ms on the server's clock).
+struct ServeRequest {
+  int32_t pattern = 0, nrhs = 1;
+  bool seeded = false;
+  uint64_t seed = 0;
+  std::vector<double> ax, b;  // the caller's values and right-hand sides (freed once solved)
+  std::vector<double> x;      // n x nrhs, column-major, once solved
+  double t_arrive = 0.0, t_admit = 0.0, t_done = 0.0;
+  int32_t batch = -1, lane = -1, status = 0;
+  int32_t ir_max = 0, gmres_max = 0;
+  double rel_residual_max = 0.0;
+  int64_t bytes = 0;  // footprint charged at admission
+  bool done = false;
+};
+
 struct ServeRecord {
   int32_t pattern = 0;
   uint64_t seed = 0;
@@ -84,7 +104,7 @@ class GmresLdl;
 class Server {
  public:
   explicit Server(const ServeOptions& opt);
-  ~Server();
+  ~Server();  // finishes the requests in flight, then stops the lanes
   // symbolic phase now (from the cache when the pattern is known); returns
   // the pattern id
   int32_t add_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* base, int32_t n1);
@@ -92,35 +112,49 @@ class Server {
   int32_t add_patterns(int32_t count, const int32_t* n, const int32_t* const* Ap, const int32_t* const* Ai,
                        const double* const* base, const int32_t* n1, int threads);
   const ServePattern& pattern(int32_t id) const;
-  int32_t npatterns() const { return static_cast<int32_t>(patterns_.size()); }
-  // Runs a stream of requests (t_arrive ascending) to completion.
+  int32_t npatterns() const;
+  // A request with the caller's values (nnz of the pattern, upper CSC order)
+  // and nrhs right-hand sides (n x nrhs, column-major), copied; returns its id.
+  int64_t submit(int32_t pattern, const double* Ax, const double* b, int32_t nrhs);
+  // a request whose data materialize() derives from the seed in the lane
+  int64_t submit_seeded(int32_t pattern, uint64_t seed, int32_t nrhs);
+  // Waits up to timeout_ms (< 0: no limit) for request id. Returns false if
+  // it is still pending; else copies its record (and x when non-null) out
+  // and, with release, forgets the request.
+  bool wait(int64_t id, double timeout_ms, ServeRequest* info, double* x, bool release);
+  // Runs a stream of requests (t_arrive ascending, ms after the call) to
+  // completion: each is submitted (seeded) at its arrival time.
   void run(std::vector<ServeRecord>& reqs);
-  const ServeSummary& summary() const { return sum_; }
+  ServeSummary summary() const;  // a copy, taken under the server's lock
   // solution of request i of the last run (keep_x)
   const std::vector<double>& solution(int64_t i) const;
   // bytes a batch of these requests needs on the device
-  int64_t batch_bytes(const std::vector<const ServeRecord*>& rs) const;
+  int64_t batch_bytes(const std::vector<const ServeRequest*>& rs) const;
 
  private:
   void add_entry(std::shared_ptr<const PatternEntry> entry, int32_t n, const int32_t* Ap, const int32_t* Ai,
                  const double* base, int32_t n1);
-  void lane_main(int lane, std::vector<ServeRecord>* reqs);
-  void process(int lane, std::vector<ServeRecord>* reqs, const std::vector<int64_t>& ids, int32_t batch_id);
+  int64_t enqueue(std::unique_ptr<ServeRequest> r);
+  void lane_main(int lane);
+  void process(int lane, const std::vector<ServeRequest*>& batch, int32_t batch_id);
 
   ServeOptions opt_;
   PatternCache cache_;
-  std::vector<ServePattern> patterns_;
+  std::deque<ServePattern> patterns_;  // stable references; growth guarded by mu_
   std::vector<std::unique_ptr<GmresLdl>> solvers_;
-  std::vector<std::vector<double>> x_;
-  // run state
-  std::mutex mu_;
-  std::condition_variable cv_;
+  std::vector<std::vector<double>> x_;  // solutions of the last run (keep_x)
+  // server state, guarded by mu_
+  mutable std::mutex mu_;
+  std::condition_variable work_cv_;  // lanes: a request arrived, memory freed, or stop
+  std::condition_variable done_cv_;  // waiters: a batch completed
+  std::vector<std::unique_ptr<ServeRequest>> reqs_;  // by id; null once released
   std::deque<int64_t> queue_;
-  int64_t next_ = 0, outstanding_ = 0;
   int64_t inflight_bytes_ = 0, concurrent_ = 0;
   int32_t batch_seq_ = 0;
+  bool stop_ = false;
   double t0_ = 0.0;
   ServeSummary sum_;
+  std::vector<std::thread> lanes_;
 };
 
 }  // namespace mpir
