@@ -264,6 +264,20 @@ int mpir_gmres_ldl_spmv_dev(mpir_gmres_ldl_t h, const double* x, double* y); /* 
  * an FP32 factor is read once for up to 8 of them. Same results, bit for
  * bit, as nrhs calls of mpir_gmres_ldl_precond_dev. */
 int mpir_gmres_ldl_precond_multi_dev(mpir_gmres_ldl_t h, double* x, int32_t nrhs);
+/* mpir_gmres_ldl_precond_dev that allocates nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The work memory of the triangular solves (completion
+ * counters, error flag, partial sums, update vectors) comes from a static
+ * device array of 64 MiB that the library holds on each GPU it runs on until
+ * the process ends, instead of from the handle's buffers; the result is
+ * bit-identical to mpir_gmres_ldl_precond_dev's. A factor whose solves need
+ * more than 64 MiB of work memory is rejected with MPIR_ERR_INVALID (the
+ * message gives the bytes needed). The handle's device must be current.
+ * Calls on one device share the array: they run one at a time (a lock held
+ * until the result is complete), and any other use of the array overlapping
+ * them would race. Single-system L D L^T handles only. */
+int mpir_gmres_ldl_precond_dev_noalloc(mpir_gmres_ldl_t h, double* x);
 /* average ms of `reps` runs measured with HIP events on the handle's stream:
  * what = 0 SpMV, 1 M^{-1} application, 2 numeric factorization */
 int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);

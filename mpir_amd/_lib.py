@@ -102,6 +102,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_perm": (C.c_int, [P, P]),
     "mpir_gmres_ldl_export_factor": (C.c_int, [P, P, P, P, P, P]),
     "mpir_gmres_ldl_precond_dev": (C.c_int, [P, P]),
+    "mpir_gmres_ldl_precond_dev_noalloc": (C.c_int, [P, P]),
     "mpir_gmres_ldl_spmv_dev": (C.c_int, [P, P, P]),
     "mpir_gmres_ldl_precond_multi_dev": (C.c_int, [P, P, i32]),
     "mpir_gmres_ldl_stream": (P, [P]),

@@ -58,6 +58,7 @@ HIP_SOURCES = [
     CSRC / "gmres.hip",
     CSRC / "batch.hip",
     CSRC / "ilu_solver.hip",
+    CSRC / "noalloc.hip",
     CSRC / "serve.cpp",
     CSRC / "capi.cpp",
 ]

@@ -957,6 +957,14 @@ int mpir_gmres_ldl_precond_dev(mpir_gmres_ldl_t h, double* x) {
     return 0;
   });
 }
+int mpir_gmres_ldl_precond_dev_noalloc(mpir_gmres_ldl_t h, double* x) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(h->s->factored(), "no factor");
+    precond_apply_noalloc(*h->s, x);
+    return 0;
+  });
+}
 int mpir_gmres_ldl_precond_multi_dev(mpir_gmres_ldl_t h, double* x, int32_t nrhs) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;

@@ -163,6 +163,11 @@ struct DeviceSolvePlan {
 // uvec: scratch of sum(nrow) entries of TX.
 template <class TF, class TD, class TX>
 void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX* uvec, hipStream_t st);
+// ldl_solve with the work areas, counters and error flag of `d` in place of
+// dp.d's (the same task lists and offsets)
+template <class TF, class TD, class TX>
+void ldl_solve_on(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, const SolveDev& d, TX* x, TX* uvec,
+                  hipStream_t st);
 // nrhs right-hand sides x + r dp.d.sx at once (each tile of the factor read
 // once for up to 8 of them); uvec and the partial-sum areas of dp.d hold 8
 // right-hand sides at strides dp.d.su, .syp, .sup, .srp, .sxp.

@@ -890,6 +890,12 @@ void ldl_solve(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, TX*
 }
 
 template <class TF, class TD, class TX>
+void ldl_solve_on(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, const SolveDev& d, TX* x, TX* uvec,
+                  hipStream_t st) {
+  ldl_solve_group<1>(a, dp, d, x, uvec, st);
+}
+
+template <class TF, class TD, class TX>
 void ldl_solve_multi(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, int nrhs, TX* uvec,
                      hipStream_t st) {
   // groups of 8, 4, 2, 1 right-hand sides for FP32 factors; FP64 tiles do
@@ -932,6 +938,15 @@ template void ldl_solve<__half, float, double>(const SolveArgs<__half, float>&, 
                                                double*, hipStream_t);
 template void ldl_solve<__hip_bfloat16, float, double>(const SolveArgs<__hip_bfloat16, float>&,
                                                        const DeviceSolvePlan&, double*, double*, hipStream_t);
+template void ldl_solve_on<double, double, double>(const SolveArgs<double, double>&, const DeviceSolvePlan&,
+                                                    const SolveDev&, double*, double*, hipStream_t);
+template void ldl_solve_on<float, float, double>(const SolveArgs<float, float>&, const DeviceSolvePlan&,
+                                                  const SolveDev&, double*, double*, hipStream_t);
+template void ldl_solve_on<__half, float, double>(const SolveArgs<__half, float>&, const DeviceSolvePlan&,
+                                                   const SolveDev&, double*, double*, hipStream_t);
+template void ldl_solve_on<__hip_bfloat16, float, double>(const SolveArgs<__hip_bfloat16, float>&,
+                                                           const DeviceSolvePlan&, const SolveDev&, double*, double*,
+                                                           hipStream_t);
 template void ldl_solve_multi<double, double, double>(const SolveArgs<double, double>&, const DeviceSolvePlan&,
                                                        double*, int, double*, hipStream_t);
 template void ldl_solve_multi<float, float, double>(const SolveArgs<float, float>&, const DeviceSolvePlan&, double*,

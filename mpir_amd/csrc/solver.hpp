@@ -178,6 +178,12 @@ class GmresLdl {
   void luir(const double* b, double* x, int maxiter, LuirResult* out);
   // building blocks exposed for tests and the kernel-level C entry points
   void precond_apply_dev(double* x);  // x := (L D L^T)^{-1} x, permuted order
+  // Bytes of work memory one application of M^{-1} needs (counters, error
+  // flag, partial sums, update vectors), and the application with that work
+  // memory carved from `work` (device memory, 256-byte aligned) instead of
+  // the handle's buffers; waits for the result (noalloc.hip).
+  int64_t precond_work_bytes() const;
+  void precond_apply_in(double* x, void* work);
   // the same for R vectors of n entries, contiguous (the factor read once
   // for up to 8 of them)
   void precond_apply_slots(double* x, int R);
@@ -301,5 +307,9 @@ class GmresLdl {
   double* b_pinned_ = nullptr;
   size_t b_pinned_count_ = 0;
 };
+
+// x := M^{-1} x on the current device with the work memory of the solves
+// in a static device array: no device allocation (noalloc.hip)
+void precond_apply_noalloc(GmresLdl& s, double* x);
 
 }  // namespace mpir
