@@ -29,6 +29,10 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
+# HBM bytes one FP32 factorization of the 64^3 workload moves, measured with
+# rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE, the factors from a 1 GiB copy;
+# profiles/hbm_traffic_64cubed.md); a bench run cannot collect counters
+FACTOR_TRAFFIC_64 = 27.16e9
 
 
 def log(msg: str) -> None:
@@ -78,7 +82,7 @@ def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x, order
     total = time.perf_counter() - t_start
     st = s.stats()
     x = dev_x.cpu().numpy()
-    return s, dict(total_s=total, lat_ms=lat, stats=st, analyze_s=analyze_s, x=x)
+    return s, dict(total_s=total, lat_ms=lat, stats=st, analyze_s=analyze_s, x=x, perm=s.perm())
 
 
 def pct(v, q):
@@ -90,39 +94,58 @@ def pct(v, q):
     return v[lo] + (v[hi] - v[lo]) * (k - lo)
 
 
-def cpu_baseline(mpir, full_flops: float, seconds_budget: float = 20.0) -> dict:
-    """The oracle (plain C restatement of the reference path, one core) on a
-    bounded sample of the same family: the 3D KKT system on a 24^3 grid,
-    FP32 factor + FP64 GMRES-IR to 1e-10. Its measured factor rate scales the
-    sample to the 64^3 workload by factor flops."""
+def host_facts() -> dict:
+    """Core count, CPU model and frequency governor of this host."""
+    facts = {"nproc": os.cpu_count()}
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                facts["model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    gov = Path("/sys/devices/system/cpu/cpu0/cpufreq/scaling_governor")
+    facts["governor"] = gov.read_text().strip() if gov.exists() else "not exposed"
+    return facts
+
+
+def cpu_baseline(P, perm, uf=np.float32, tol: float = 1e-10) -> tuple:
+    """The oracle (the plain C restatement of the reference path: QDLDL_etree
+    + QDLDL_factor<UF> of the scaled, permuted matrix, then GmresLDLIR::Solve
+    with QDLDL_solve as M^-1) on the benchmark workload itself, with the GPU
+    run's permutation, once, on one core (the calling thread pinned with
+    sched_setaffinity; the reference path is serial). Returns the record and
+    the oracle's solution."""
     from tests.oracle_lib import Oracle
     orc = Oracle()
-    S = mpir.kkt_config(1, k=24)
-    s = mpir.GmresLDLIR(np.float32, ordering=mpir.ORDER_ND)
-    s.Analyze(S.Ap, S.Ai)
-    perm = s.perm()
-    sample_flops = s.stats()["factor_flops"]
-    runs, t_fac, t_sol = 0, 0.0, 0.0
-    t_begin = time.perf_counter()
-    while runs < 1 or (time.perf_counter() - t_begin < seconds_budget and runs < 5):
-        rc, x, info = orc.gmres_ir(S.Ap, S.Ai, S.Ax, S.b, perm=perm, uf=np.float32, tol=1e-10)
-        assert rc == 0
-        t_fac += info["factor_s"]
-        t_sol += info["solve_s"]
-        runs += 1
-    per_solve = (t_fac + t_sol) / runs
-    rate = sample_flops / (t_fac / runs)
-    est_full = full_flops / rate + (t_sol / runs) * (full_flops / sample_flops) ** 0.5
-    return {
-        "value": round(1.0 / est_full, 6),
+    facts = host_facts()
+    old = os.sched_getaffinity(0)
+    core = max(old)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        rc, xo, info = orc.gmres_ir(P.Ap, P.Ai, P.Ax, P.b, perm=perm, uf=uf, tol=tol)
+        wall = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, old)
+    assert rc == 0, "oracle GMRES-IR failed"
+    solve_s = info["factor_s"] + info["solve_s"]
+    name = "FP32" if uf == np.float32 else "FP64"
+    rec = {
+        "value": round(1.0 / solve_s, 6),
         "unit": "solves/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"oracle FP32-factor GMRES-IR on the 24^3 KKT system (n={S.n}) x{runs}: "
-                   f"{per_solve * 1e3:.0f} ms/solve ({1.0 / per_solve:.3f} solves/s), factor "
-                   f"{rate / 1e9:.2f} GFLOP/s; scaled to the 64^3 workload by factor flops "
-                   f"({full_flops:.3e} vs {sample_flops:.3e})"),
+        "sample": (f"the full workload once (n={P.n}): oracle {name}-factor GMRES-IR, same permutation as the GPU, "
+                   f"tol {tol:g}: factor {info['factor_s']:.1f} s, refinement {info['solve_s']:.2f} s "
+                   f"({int(info['ir_iterations'])} IR / {int(info['gmres_iterations'])} GMRES iterations); "
+                   f"pinned to core {core}; host {facts.get('model', '?')}, {facts['nproc']} cores, "
+                   f"governor {facts['governor']}"),
+        "factor_s": round(info["factor_s"], 3), "refine_s": round(info["solve_s"], 3), "wall_s": round(wall, 3),
+        "ir_iterations": int(info["ir_iterations"]), "gmres_iterations": int(info["gmres_iterations"]),
+        "stop_reason": int(info["stop_reason"]), "host": facts,
     }
+    return rec, xo
 
 
 def precision_sweep(mpir, k: int, tol: float) -> int:
@@ -426,6 +449,7 @@ def main(argv=None) -> int:
                     help="fill-reducing ordering (nd = nested dissection, amd = approximate minimum degree)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-factor comparison path")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-fp64", action="store_true", help="also time the oracle's FP64-factor path")
     ap.add_argument("--sweep", action="store_true",
                     help="precision form instead of the bench line: f64/f32/f16/bf16 factors x scaling x D scale")
     ap.add_argument("--batched", action="store_true",
@@ -543,7 +567,8 @@ def main(argv=None) -> int:
     # dense fronts); the M^-1 application and the SpMV (HBM-bound) alongside
     tf = res["factor_TFLOPs"]
     out["roofline"] = {"bound": "mfma", "achieved": round(tf, 3), "peak": PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
-                       "frac": round(tf / PEAK_TFLOPS["f32"], 4), "traffic": None,
+                       "frac": round(tf / PEAK_TFLOPS["f32"], 4),
+                       "traffic": FACTOR_TRAFFIC_64 if args.k == 64 and order == mpir.ORDER_ND else None,
                        "kernel": "numeric LDL^T factorization (FP32; F = sum Lnz_j (Lnz_j + 2) + n per launch sequence)"}
     out["roofline_solve"] = {"bound": "hbm", "achieved": round(res["precond_GBs"], 2), "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": round(res["precond_GBs"] / HBM_PEAK_GBS, 4),
@@ -551,9 +576,23 @@ def main(argv=None) -> int:
     out["roofline_spmv"] = {"bound": "hbm", "achieved": round(res["spmv_GBs"], 2), "peak": HBM_PEAK_GBS,
                             "unit": "GB/s", "frac": round(res["spmv_GBs"] / HBM_PEAK_GBS, 4),
                             "kernel": "FP64 CSR SpMV (35 MB working set, fits the Infinity Cache)"}
+    # forward errors: against the generator's exact solution and against the
+    # FP64-factor GPU solution (and, below, the oracle's CPU solution)
+    out["fwd_err"] = float(np.abs(r32["x"] - P.x_true).max() / np.abs(P.x_true).max())
+    if r64 is not None:
+        out["fwd_err_vs_fp64_factor"] = float(np.abs(r32["x"] - r64["x"]).max() / np.abs(r64["x"]).max())
     if not args.no_cpu:
-        log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(mpir, st["factor_flops"])
+        log("cpu baseline: the oracle on the full workload, one core")
+        rec, xo = cpu_baseline(P, r32["perm"], np.float32, args.tol)
+        out["cpu_baseline"] = rec
+        out["fwd_err_vs_oracle"] = float(np.abs(r32["x"] - xo).max() / np.abs(xo).max())
+        out["oracle_bwd_err"] = backward_error(P, xo)
+        if args.cpu_fp64:
+            rec64, _ = cpu_baseline(P, r32["perm"], np.float64, args.tol)
+            out["cpu_baseline_fp64_factor"] = rec64
+        else:
+            out["cpu_baseline_fp64_factor"] = ("not run by default (the FP64 oracle factor of this workload "
+                                               "takes as long again); bench.py --cpu-fp64 runs it")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cfloat>
 #include <cstring>
 #include <limits>
 #include <numeric>
@@ -34,6 +35,27 @@
 #include "solver.hpp"
 
 namespace mpir {
+
+enum GmDoubleSlot : int { kGsRhoB = 0, kGsRho, kGsInv, kGsHn, kGsHead = 8 };
+enum GmIntSlot : int { kGiCall = 0, kGiActive, kGiK, kGiTotalK, kGiTotalApps, kGiIrGate, kGiCount = 8 };
+enum IrSlot : int { kIrBnorm = 0, kIrScale, kIrDnorm, kIrXnorm, kIrRraw, kIrRprev, kIrR0, kIrCount = 8 };
+// mapped host words (doubles): the last finished Arnoldi step's ordinal and
+// its gate, then the refinement record
+enum MirrorSlot : int {
+  kMirStep = 4000, kMirActive, kMirIrSeq = 4004,
+  kMirIr = 4008,  // iteration, stop reason, r_norm, d_norm, gmres steps, M^-1 applications, ||b_s||_inf, ||r_0||_inf
+};
+
+struct GmDev {
+  double* gs;  // kGsHead scalars, then g (m + 1), cs (m), sn (m), y (m), H ((m + 1) m)
+  int* gi;
+  int m;
+  __host__ __device__ double* g() const { return gs + kGsHead; }
+  __host__ __device__ double* cs() const { return g() + m + 1; }
+  __host__ __device__ double* sn() const { return cs() + m; }
+  __host__ __device__ double* y() const { return sn() + m; }
+  __host__ __device__ double* H() const { return y() + m; }
+};
 
 namespace {
 
@@ -69,8 +91,10 @@ __device__ __forceinline__ double block_max(double v, double* sh) {
 // (scale, ssq) pair per block in part[2b], part[2b + 1]; op 2: max |x_i|
 __global__ void __launch_bounds__(kRedThreads) k_reduce_partial(int n, const double* __restrict__ x,
                                                                 const double* __restrict__ y, int op,
-                                                                double* __restrict__ part) {
+                                                                double* __restrict__ part,
+                                                                const int* __restrict__ gate = nullptr) {
   __shared__ double sh[2 * kRedThreads / 64];
+  if (gate != nullptr && *gate == 0) return;
   if (op == 1) {
     Ssq acc = ssq_init();
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc = ssq_add(acc, x[i]);
@@ -97,8 +121,10 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce_partial(int n, const dou
 // after the stream's work completes without a copy.
 __global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* __restrict__ part, int np, int op,
                                                               double* __restrict__ out, const double* __restrict__ head,
-                                                              int nhead, double* __restrict__ mirror) {
+                                                              int nhead, double* __restrict__ mirror,
+                                                              const int* __restrict__ gate = nullptr) {
   __shared__ double sh[2 * kRedThreads / 64];
+  if (gate != nullptr && *gate == 0) return;
   double v;
   if (op == 1) {
     Ssq acc = ssq_init();
@@ -118,9 +144,12 @@ __global__ void __launch_bounds__(kRedThreads) k_reduce_final(const double* __re
 }
 
 // partial dots of w against k1 basis vectors: part[j * gridDim.x + block]
+// (gate: the kernels of an Arnoldi step return at once when *gate == 0)
 __global__ void __launch_bounds__(kRedThreads) k_mdot_partial(int n, const double* __restrict__ V, int64_t ldv, int k1,
-                                                              const double* __restrict__ w, double* __restrict__ part) {
+                                                              const double* __restrict__ w, double* __restrict__ part,
+                                                              const int* __restrict__ gate) {
   __shared__ double sh[kRedThreads / 64];
+  if (gate != nullptr && *gate == 0) return;
   for (int j = 0; j < k1; ++j) {
     const double* vj = V + ldv * j;
     double acc = 0.0;
@@ -132,8 +161,9 @@ __global__ void __launch_bounds__(kRedThreads) k_mdot_partial(int n, const doubl
 
 // h[j] (+)= sum of the partials of vector j, one block per j
 __global__ void __launch_bounds__(kRedThreads) k_mdot_final(const double* __restrict__ part, int np, double* __restrict__ h,
-                                                            int accumulate) {
+                                                            int accumulate, const int* __restrict__ gate) {
   __shared__ double sh[kRedThreads / 64];
+  if (gate != nullptr && *gate == 0) return;
   const int j = blockIdx.x;
   double acc = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[static_cast<int64_t>(j) * np + i];
@@ -143,9 +173,9 @@ __global__ void __launch_bounds__(kRedThreads) k_mdot_final(const double* __rest
 
 // w -= sum_j V_j h_j
 __global__ void k_sub_basis(int n, const double* __restrict__ V, int64_t ldv, int k1, const double* __restrict__ h,
-                            double* __restrict__ w) {
+                            double* __restrict__ w, const int* __restrict__ gate) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (gate != nullptr && *gate == 0)) return;
   double acc = 0.0;
   for (int j = 0; j < k1; ++j) acc += V[ldv * j + i] * h[j];
   w[i] -= acc;
@@ -163,9 +193,9 @@ __global__ void k_combine(int n, const double* __restrict__ V, int64_t ldv, int 
 
 // out = x * s where s = scalars[slot] (recip: s = 1 / scalars[slot])
 __global__ void k_scale_dev(int n, const double* __restrict__ x, const double* __restrict__ sp, int recip,
-                            double* __restrict__ out) {
+                            double* __restrict__ out, const int* __restrict__ gate) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (gate != nullptr && *gate == 0)) return;
   const double s = recip ? 1.0 / *sp : *sp;
   out[i] = x[i] * s;
 }
@@ -178,6 +208,183 @@ __global__ void k_scale_host(int n, const double* __restrict__ x, double s, doub
 __global__ void k_add(int n, double* __restrict__ x, const double* __restrict__ d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = x[i] + d[i];
+}
+__global__ void k_add_gated(int n, double* __restrict__ x, const double* __restrict__ d, const int* __restrict__ gate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && *gate != 0) x[i] = x[i] + d[i];
+}
+__global__ void k_copy_gated(int n, const double* __restrict__ src, double* __restrict__ dst,
+                             const int* __restrict__ gate) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && *gate != 0) dst[i] = src[i];
+}
+
+// ------------------------------------------------------- GMRES-IR on the device
+// PrecondGmres (gmres_ir.hpp:636-741) keeps its scalars in device memory: the
+// Hessenberg matrix (column-major, ld m + 1), the Givens rotations, g, rho and
+// the step count, plus gates: a kernel of an Arnoldi step returns at once when
+// the iteration has already stopped, so the host queues step k + 1 before
+// step k has finished and never waits for a step's scalars. Solve's stopping
+// rules (gmres_ir.hpp:599-606) are evaluated by k_ir_check, which hands one
+// record per refinement iteration to the host through mapped memory.
+__device__ __forceinline__ void mirror_put(double* mirror, int slot, double v) {
+  __hip_atomic_store(mirror + slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// dnrm2 of {a, b} in the reference's order (ops.hpp:231-255)
+__device__ double dev_dnrm2_pair(double a, double b) {
+  double scale = 0.0, ssq = 1.0;
+  const double v[2] = {b, a};
+  for (int i = 0; i < 2; ++i) {
+    const double x = v[i];
+    if (x != 0.0) {
+      const double ax = fabs(x);
+      if (scale < ax) {
+        ssq = 1.0 + ssq * ((scale / ax) * (scale / ax));
+        scale = ax;
+      } else {
+        ssq = ssq + ((ax / scale) * (ax / scale));
+      }
+    }
+  }
+  return scale * sqrt(ssq);
+}
+
+// start of a call: it runs when the caller's gate is open and ||b||_2 is not
+// below tol (gmres_ir.hpp:647-650); the first M^-1 application follows
+__global__ void k_gm_begin(GmDev d, const int* __restrict__ gate, double tol) {
+  const int on = (gate == nullptr || *gate != 0) && !(d.gs[kGsRhoB] < tol);
+  d.gi[kGiCall] = on;
+  d.gi[kGiActive] = on;
+  d.gi[kGiK] = 0;
+  if (on) d.gi[kGiTotalApps] += 1;
+}
+
+// rho = ||M^-1 b||_2: g = (rho, 0, ...), V_0 scaled by 1 / rho, and the loop
+// test k < m && rho > tol for k = 0
+__global__ void k_gm_start(GmDev d, double tol) {
+  if (!d.gi[kGiActive]) return;
+  const double rho = d.gs[kGsRho];
+  d.gs[kGsInv] = 1 / rho;
+  double* g = d.g();
+  g[0] = rho;
+  for (int i = 1; i <= d.m; ++i) g[i] = 0.0;
+  d.gi[kGiActive] = (d.m > 0 && rho > tol) ? 1 : 0;
+}
+
+// Arnoldi step k after its CGS2 passes (column h1 + h2, ||w|| in kGsHn):
+// Hessenberg column, the earlier rotations, the new rotation, g, rho and
+// the gate of step k + 1 (gmres_ir.hpp:694-723). Publishes the step's
+// ordinal and gate to the host whether or not the step ran.
+__global__ void k_gm_givens(GmDev d, int k, double tol, const double* __restrict__ h1, const double* __restrict__ h2,
+                            double* mirror, double ordinal) {
+  if (d.gi[kGiActive]) {
+    const int m = d.m;
+    double* H = d.H() + static_cast<int64_t>(k) * (m + 1);
+    double* g = d.g();
+    double* c = d.cs();
+    double* s = d.sn();
+    for (int j = 0; j <= k; ++j) H[j] = h1[j] + h2[j];
+    const double hn = d.gs[kGsHn];
+    H[k + 1] = hn;
+    d.gs[kGsInv] = hn != 0 ? 1 / hn : 1.0;  // v_{k+1} /= h_{k+1,k} unless it is zero
+    for (int j = 0; j < k; ++j) {
+      const double w1 = c[j] * H[j] - s[j] * H[j + 1];
+      const double w2 = s[j] * H[j] + c[j] * H[j + 1];
+      H[j] = w1;
+      H[j + 1] = w2;
+    }
+    const double nu = dev_dnrm2_pair(H[k], H[k + 1]);
+    if (nu != 0) {
+      c[k] = H[k] / nu;
+      s[k] = -H[k + 1] / nu;
+      H[k] = c[k] * H[k] - s[k] * H[k + 1];
+      H[k + 1] = 0.0;
+      const double w1 = c[k] * g[k] - s[k] * g[k + 1];
+      const double w2 = s[k] * g[k] + c[k] * g[k + 1];
+      g[k] = w1;
+      g[k + 1] = w2;
+    }
+    const double rho = fabs(g[k + 1]);
+    d.gs[kGsRho] = rho;
+    d.gi[kGiK] = k + 1;
+    d.gi[kGiTotalK] += 1;
+    d.gi[kGiTotalApps] += 1;
+    d.gi[kGiActive] = (k + 1 < m && rho > tol) ? 1 : 0;
+  }
+  mirror_put(mirror, kMirActive, d.gi[kGiActive]);
+  __threadfence_system();
+  mirror_put(mirror, kMirStep, ordinal);
+  __threadfence_system();
+}
+
+// back substitution y = H(0:k, 0:k)^-1 g (gmres_ir.hpp:725-732)
+__global__ void k_gm_backsolve(GmDev d) {
+  const int k = d.gi[kGiK];
+  const int m = d.m;
+  const double* H = d.H();
+  const double* g = d.g();
+  double* y = d.y();
+  for (int i = k; i-- > 0;) {
+    double v = g[i];
+    for (int j = i + 1; j < k; ++j) v -= H[i + static_cast<int64_t>(j) * (m + 1)] * y[j];
+    y[i] = v / H[i + static_cast<int64_t>(i) * (m + 1)];
+  }
+}
+
+// res = sum_{j < k} V_j y_j with the device's step count (zero when k = 0)
+__global__ void k_gm_combine(int n, const double* __restrict__ V, int64_t ldv, const int* __restrict__ kp,
+                             const double* __restrict__ y, double* __restrict__ res) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k = *kp;
+  double acc = 0.0;
+  for (int j = 0; j < k; ++j) acc += V[ldv * j + i] * y[j];
+  res[i] = acc;
+}
+
+// refinement iteration: the gate opens unless the residual is exactly zero
+__global__ void k_ir_gate(const double* __restrict__ ir, int* __restrict__ gi) {
+  gi[kGiIrGate] = ir[kIrScale] != 0.0 ? 1 : 0;
+}
+
+// Solve's stopping rules after iteration `iter` (gmres_ir.hpp:575-606):
+// 1 small update (d_norm <= 10 eps(UW) x_norm, x_norm taken before the
+// update), 2 stagnation (|r_norm - r_prev| < tol, absolute), 3 zero residual
+// (the gate was closed); the record goes to the host, its sequence number
+// `seq` last. iter = 0 records the state after the first PrecondGmres only.
+__global__ void k_ir_check(double* __restrict__ ir, const int* __restrict__ gi, int iter, double tol, double eps_uw,
+                           double* mirror, double seq) {
+  double stop = 0.0, r_norm = 0.0;
+  const double bnorm = ir[kIrBnorm];
+  if (iter == 0) {
+    r_norm = ir[kIrR0] / bnorm;
+  } else if (!gi[kGiIrGate]) {
+    stop = 3.0;
+  } else {
+    r_norm = ir[kIrRraw] / bnorm;
+    if (ir[kIrDnorm] <= ir[kIrXnorm] * (10 * eps_uw)) stop = 1.0;
+    else if (fabs(r_norm - ir[kIrRprev]) < tol) stop = 2.0;
+    ir[kIrRprev] = r_norm;
+  }
+  mirror_put(mirror, kMirIr + 0, iter);
+  mirror_put(mirror, kMirIr + 1, stop);
+  mirror_put(mirror, kMirIr + 2, r_norm);
+  mirror_put(mirror, kMirIr + 3, ir[kIrDnorm]);
+  mirror_put(mirror, kMirIr + 4, gi[kGiTotalK]);
+  mirror_put(mirror, kMirIr + 5, gi[kGiTotalApps]);
+  mirror_put(mirror, kMirIr + 6, bnorm);
+  mirror_put(mirror, kMirIr + 7, ir[kIrR0]);
+  __threadfence_system();
+  mirror_put(mirror, kMirIrSeq, seq);
+  __threadfence_system();
+}
+
+__global__ void k_ir_reset(double* __restrict__ ir, int* __restrict__ gi) {
+  ir[kIrRprev] = DBL_MAX;
+  gi[kGiTotalK] = 0;
+  gi[kGiTotalApps] = 0;
+  gi[kGiIrGate] = 1;
 }
 
 __global__ void k_compute_ad(int n, const double* __restrict__ ax, const int32_t* __restrict__ diag_pos, double eps,
@@ -305,6 +512,9 @@ GmresLdl::GmresLdl(int uf, int uw, int ur) : uf_(uf), uw_(uw), ur_(ur) {
   MPIR_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pinned_), 4096 * sizeof(double),
                          hipHostMallocMapped | hipHostMallocCoherent));
   MPIR_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&h_mirror_), h_pinned_, 0));
+  // the device publishes records here with increasing sequence numbers from
+  // 1: start from zero (the allocation may hold an earlier handle's records)
+  std::memset(h_pinned_, 0, 4096 * sizeof(double));
 }
 
 // Waits for the stream: polls for up to 200 us (the GMRES loop waits once
@@ -721,18 +931,18 @@ void GmresLdl::factor_loaded(double t0) {
 
 double GmresLdl::dnrm2(const double* v) {
   hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, 1,
-                     d_red_.as<double>());
+                     d_red_.as<double>(), nullptr);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 1,
-                     d_small_.as<double>() + kSlotNorm, nullptr, 0, h_mirror_);
+                     d_small_.as<double>() + kSlotNorm, nullptr, 0, h_mirror_, nullptr);
   wait_stream();
   return h_pinned_[0];
 }
 
 double GmresLdl::infnrm(const double* v) {
   hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, 2,
-                     d_red_.as<double>());
+                     d_red_.as<double>(), nullptr);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, 2,
-                     d_small_.as<double>() + kSlotNorm, nullptr, 0, h_mirror_);
+                     d_small_.as<double>() + kSlotNorm, nullptr, 0, h_mirror_, nullptr);
   wait_stream();
   return h_pinned_[0];
 }
@@ -769,86 +979,131 @@ void GmresLdl::spmv_dev(const double* x, double* y) {
   spmv_f64(A, x, y, stream_);
 }
 
-void GmresLdl::precond_gmres(const double* b, double* res) {
+GmDev GmresLdl::gm_dev(int m) {
+  d_gm_.ensure(static_cast<size_t>(kGsHead + (m + 1) + 3 * m + static_cast<int64_t>(m + 1) * std::max(m, 1)) *
+               sizeof(double));
+  d_gmi_.ensure(kGiCount * sizeof(int));
+  d_ir_.ensure(kIrCount * sizeof(double));
+  return GmDev{d_gm_.as<double>(), d_gmi_.as<int>(), m};
+}
+
+// op 1: 2-norm, op 2: inf-norm of v into *out (device), skipped when *gate == 0
+void GmresLdl::norm_to(const double* v, int op, double* out, const int* gate) {
+  hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, stream_, n_, v, nullptr, op,
+                     d_red_.as<double>(), gate);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, stream_, d_red_.as<double>(), kRedBlocks, op, out,
+                     nullptr, 0, nullptr, gate);
+}
+
+// x := M^-1 x unless *gate == 0 (gate may be null)
+void GmresLdl::precond_apply_gated(double* x, const int* gate) {
+  if (precond_ == kPrecIlu) {
+    IluDev g = ilu_g_;
+    g.gate = gate;
+    if (uf_ == kF64)
+      ilu_solve<double>(g, ilu_lv_, d_ilu_l_.as<double>(), d_ilu_u_.as<double>(), d_ilu_d_.as<double>(), x, stream_);
+    else
+      ilu_solve<float>(g, ilu_lv_, d_ilu_l_.as<float>(), d_ilu_u_.as<float>(), d_ilu_d_.as<float>(), x, stream_);
+    return;
+  }
+  SolveDev d = dsp_.d;
+  d.gate = gate;
+  double* u = d_uvec_.as<double>();
+  if (uf_ == kF64) ldl_solve_on<double, double, double>(solve_args<double, double>(d_panel_.get()), dsp_, d, x, u, stream_);
+  else if (uf_ == kF32) ldl_solve_on<float, float, double>(solve_args<float, float>(d_panel_.get()), dsp_, d, x, u, stream_);
+  else if (uf_ == kF16)
+    ldl_solve_on<__half, float, double>(solve_args<__half, float>(d_panel16_.get()), dsp_, d, x, u, stream_);
+  else
+    ldl_solve_on<__hip_bfloat16, float, double>(solve_args<__hip_bfloat16, float>(d_panel16_.get()), dsp_, d, x, u,
+                                                stream_);
+}
+
+// Waits until the device has published record `target` in mirror word
+// `slot`: polls the mapped word (no HIP call), yields the core after 200 us,
+// and every millisecond asks the stream whether it failed or went idle.
+void GmresLdl::wait_mirror(int slot, double target) {
+  volatile double* mv = h_pinned_;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto last = t0;
+  for (;;) {
+    if (mv[slot] >= target) return;
+    const auto t = std::chrono::steady_clock::now();
+    if (t - t0 > std::chrono::microseconds(200)) std::this_thread::yield();
+    if (t - last > std::chrono::milliseconds(1)) {
+      last = t;
+      const hipError_t e = hipStreamQuery(stream_);
+      if (e == hipSuccess) {
+        if (mv[slot] >= target) return;
+        throw Error(kErrInternal, "the device did not publish its iteration record");
+      }
+      if (e != hipErrorNotReady) MPIR_HIP(e);
+    }
+  }
+}
+
+// PrecondGmres (gmres_ir.hpp:636-741) on the device: res = V y from the
+// Krylov basis of M^-1 K_s started at M^-1 b. The call is skipped when
+// *gate == 0 (res = 0). The host queues Arnoldi step k + 1 while step k
+// runs and stops queueing once a finished step reports the iteration over,
+// so at most one queued step finds its gate closed.
+void GmresLdl::precond_gmres(const double* b, double* res, const int* gate) {
   hipStream_t st = stream_;
   const int n = n_;
   max_gmres_ = std::min<int64_t>(max_gmres_, n);  // persists, as gmres_ir.hpp:643
   const int m = static_cast<int>(max_gmres_);
+  MPIR_CHECK(m + 2 <= kMirStep - 8, "GMRES iteration limit above 3990");
   const int64_t ldv = n;
-  MPIR_CHECK(m + 2 <= 4096, "GMRES iteration limit above 4094");
   d_V_.ensure(static_cast<size_t>(m + 1) * ldv * sizeof(double));
   d_hcol_.ensure(static_cast<size_t>(m + 2) * sizeof(double));
-  d_red_.ensure(static_cast<size_t>(m + 1) * kRedBlocks * sizeof(double));
+  d_red_.ensure(static_cast<size_t>(std::max(m + 1, 2)) * kRedBlocks * sizeof(double));
   d_small_.ensure(static_cast<size_t>(kSlotH + m + 2) * sizeof(double));
+  const GmDev gd = gm_dev(m);
   double* V = d_V_.as<double>();
-  MPIR_HIP(hipMemsetAsync(res, 0, n * sizeof(double), st));
-  double rho = dnrm2(b);
-  if (rho < tol_) return;
-  MPIR_HIP(hipMemcpyAsync(V, b, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-  precond_apply_dev(V);
-  rho = dnrm2(V);
-  hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, V, 1.0 / rho, V);
-  std::vector<std::vector<double>> h(m + 1, std::vector<double>(m, 0.0));
-  std::vector<double> g(m + 1, 0.0), c(m, 0.0), s(m, 0.0);
-  g[0] = rho;
+  const int* active = gd.gi + kGiActive;
+  norm_to(b, 1, gd.gs + kGsRhoB, gate);
+  hipLaunchKernelGGL(k_gm_begin, dim3(1), dim3(1), 0, st, gd, gate, tol_);
+  hipLaunchKernelGGL(k_copy_gated, dim3(nblk(n)), dim3(256), 0, st, n, b, V, active);
+  precond_apply_gated(V, active);
+  norm_to(V, 1, gd.gs + kGsRho, active);
+  hipLaunchKernelGGL(k_gm_start, dim3(1), dim3(1), 0, st, gd, tol_);
+  hipLaunchKernelGGL(k_scale_dev, dim3(nblk(n)), dim3(256), 0, st, n, V, gd.gs + kGsInv, 0, V, active);
   const unsigned red_blocks = static_cast<unsigned>(std::min<int64_t>(kRedBlocks, std::max<int64_t>(1, (n + 255) / 256)));
   double* part = d_red_.as<double>();
-  double* hc = d_hcol_.as<double>();
-  int k;
-  for (k = 0; k < m && rho > tol_; ++k) {
+  double* h1 = d_hcol_.as<double>();
+  double* h2 = d_small_.as<double>() + kSlotH;
+  CsrDev A{n_, d_csr_p_.as<int32_t>(), d_csr_j_.as<int32_t>(), d_csr_x_.as<double>()};
+  A.gate = active;
+  for (int k = 0; k < m; ++k) {
     double* w = V + ldv * (k + 1);
-    spmv_dev(V + ldv * k, w);
-    precond_apply_dev(w);
+    spmv_f64(A, V + ldv * k, w, st);
+    precond_apply_gated(w, active);
     // CGS2: two passes of h = V^T w, w -= V h
     const int k1 = k + 1;
     for (int pass = 0; pass < 2; ++pass) {
-      double* hp = pass == 0 ? hc : d_small_.as<double>() + kSlotH;
-      hipLaunchKernelGGL(k_mdot_partial, dim3(red_blocks), dim3(kRedThreads), 0, st, n, V, ldv, k1, w, part);
-      hipLaunchKernelGGL(k_mdot_final, dim3(k1), dim3(kRedThreads), 0, st, part, static_cast<int>(red_blocks), hp, 0);
-      hipLaunchKernelGGL(k_sub_basis, dim3(nblk(n)), dim3(256), 0, st, n, V, ldv, k1, hp, w);
-      if (pass == 1) hipLaunchKernelGGL(k_add, dim3(nblk(k1)), dim3(256), 0, st, k1, hc, hp);
+      double* hp = pass == 0 ? h1 : h2;
+      hipLaunchKernelGGL(k_mdot_partial, dim3(red_blocks), dim3(kRedThreads), 0, st, n, V, ldv, k1, w, part, active);
+      hipLaunchKernelGGL(k_mdot_final, dim3(k1), dim3(kRedThreads), 0, st, part, static_cast<int>(red_blocks), hp, 0,
+                         active);
+      hipLaunchKernelGGL(k_sub_basis, dim3(nblk(n)), dim3(256), 0, st, n, V, ldv, k1, hp, w, active);
     }
-    hipLaunchKernelGGL(k_reduce_partial, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, w, nullptr, 1, part);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, part, kRedBlocks, 1, hc + k1, hc, k1,
-                       h_mirror_);
-    wait_stream();
-    for (int j = 0; j <= k; ++j) h[j][k] = h_pinned_[j];
-    h[k + 1][k] = h_pinned_[k1];
-    if (h[k + 1][k] != 0) hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, w, 1.0 / h[k + 1][k], w);
-    for (int j = 0; j < k; ++j) {
-      const double w1 = c[j] * h[j][k] - s[j] * h[j + 1][k];
-      const double w2 = s[j] * h[j][k] + c[j] * h[j + 1][k];
-      h[j][k] = w1;
-      h[j + 1][k] = w2;
+    norm_to(w, 1, gd.gs + kGsHn, active);
+    const double ord = static_cast<double>(++step_seq_);
+    hipLaunchKernelGGL(k_gm_givens, dim3(1), dim3(1), 0, st, gd, k, tol_, h1, h2, h_mirror_, ord);
+    hipLaunchKernelGGL(k_scale_dev, dim3(nblk(n)), dim3(256), 0, st, n, w, gd.gs + kGsInv, 0, w, active);
+    MPIR_HIP(hipGetLastError());
+    if (k >= 1) {  // step k is queued; see whether step k - 1 ended the iteration
+      wait_mirror(kMirStep, ord - 1);
+      if (static_cast<volatile double*>(h_pinned_)[kMirActive] == 0.0) break;
     }
-    const double nu = dnrm2_pair(h[k][k], h[k + 1][k]);
-    if (nu != 0) {
-      c[k] = h[k][k] / nu;
-      s[k] = -h[k + 1][k] / nu;
-      h[k][k] = c[k] * h[k][k] - s[k] * h[k + 1][k];
-      h[k + 1][k] = 0.0;
-      const double w1 = c[k] * g[k] - s[k] * g[k + 1];
-      const double w2 = s[k] * g[k] + c[k] * g[k + 1];
-      g[k] = w1;
-      g[k + 1] = w2;
-    }
-    rho = std::fabs(g[k + 1]);
   }
-  stats_.gmres_iterations += k;
-  std::vector<double> y(std::max(k, 1), 0.0);
-  for (int i = k; i-- > 0;) {
-    y[i] = g[i];
-    for (int j = i + 1; j < k; ++j) y[i] -= h[i][j] * y[j];
-    y[i] /= h[i][i];
-  }
-  if (k > 0) {
-    std::memcpy(h_pinned_, y.data(), k * sizeof(double));
-    MPIR_HIP(hipMemcpyAsync(hc, h_pinned_, k * sizeof(double), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_combine, dim3(nblk(n)), dim3(256), 0, st, n, V, ldv, k, hc, res);
-    MPIR_HIP(hipStreamSynchronize(st));
-  }
+  hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(1), 0, st, gd);
+  hipLaunchKernelGGL(k_gm_combine, dim3(nblk(n)), dim3(256), 0, st, n, V, ldv, gd.gi + kGiK, gd.y(), res);
 }
 
+// GmresLDLIR::Solve (gmres_ir.hpp:534-634) on the device. The host queues
+// each refinement iteration and waits once per iteration for its record
+// (stop reason, residual, correction norm, counts); the norms, the scaling
+// and the stopping rules are computed on the device.
 void GmresLdl::solve_one(const double* b, double* x) {
   hipStream_t st = stream_;
   const int n = n_;
@@ -857,66 +1112,89 @@ void GmresLdl::solve_one(const double* b, double* x) {
   double* d = d_d_.as<double>();
   double* tmp = d_w_.as<double>();
   const double t_start = now_ms();
-  // A non-finite right-hand side, iterate or residual ends the solve with
-  // MPIR_ERR_NUMERIC. The reference has no such check: its InfNrm skips a
-  // NaN that is not the first entry (ops.hpp:324-329), and the loop would
-  // go on with NaN.
-  const double bnorm = infnrm(b);
-  if (!std::isfinite(bnorm)) throw Error(kErrNumeric, "the right-hand side is not finite");
   // eps of the working precision in the small-update test (gmres_ir.hpp:599)
   const double eps_uw = uw_ == kF32 ? static_cast<double>(std::numeric_limits<float>::epsilon())
                                     : std::numeric_limits<double>::epsilon();
-  precond_gmres(b, x);
+  const GmDev gd = gm_dev(static_cast<int>(std::min<int64_t>(max_gmres_, n)));
+  double* ir = d_ir_.as<double>();
+  const int* irgate = gd.gi + kGiIrGate;
+  const int32_t base_k = stats_.gmres_iterations, base_apps = stats_.precond_applications;
+  volatile double* mv = h_pinned_;
+  hipLaunchKernelGGL(k_ir_reset, dim3(1), dim3(1), 0, st, ir, gd.gi);
+  norm_to(b, 2, ir + kIrBnorm, nullptr);
+  precond_gmres(b, x, nullptr);
   round_to(uw_, x, n, st);
   residual_f64(A, x, b, r, st);
   round_to(ur_, r, n, st);
-  double r_prev = std::numeric_limits<double>::max();
+  norm_to(r, 2, ir + kIrR0, nullptr);
+  CsrDev Ag = A;
+  Ag.gate = irgate;
   stats_.stop_reason = 0;
   log_ = SolveLog();
-  log_.initial_residual = infnrm(r);
-  if (!std::isfinite(log_.initial_residual)) throw Error(kErrNumeric, "the first iterate is not finite");
-  int64_t iter;
   double r_norm = 0.0;
+  bool first = true;
+  // A non-finite right-hand side, iterate or residual ends the solve with
+  // MPIR_ERR_NUMERIC. The reference has no such check: its InfNrm skips a
+  // NaN that is not the first entry (ops.hpp:324-329), and the loop would go
+  // on with NaN.
+  auto take_record = [&](int64_t it) {
+    const double sq = static_cast<double>(++ir_seq_);
+    hipLaunchKernelGGL(k_ir_check, dim3(1), dim3(1), 0, st, ir, gd.gi, static_cast<int>(it), tol_, eps_uw, h_mirror_,
+                       sq);
+    MPIR_HIP(hipGetLastError());
+    wait_mirror(kMirIrSeq, sq);
+    stats_.gmres_iterations = base_k + static_cast<int32_t>(mv[kMirIr + 4]);
+    stats_.precond_applications = base_apps + static_cast<int32_t>(mv[kMirIr + 5]);
+    if (first) {
+      if (!std::isfinite(mv[kMirIr + 6])) throw Error(kErrNumeric, "the right-hand side is not finite");
+      if (!std::isfinite(mv[kMirIr + 7])) throw Error(kErrNumeric, "the first iterate is not finite");
+      log_.initial_residual = mv[kMirIr + 7];
+      first = false;
+    }
+  };
+  int64_t iter;
   for (iter = 1; iter <= max_ir_; ++iter) {
-    const double scale = infnrm(r);
-    // An exactly zero residual would make the reference scale by 1/0
-    // (gmres_ir.hpp:562-563) and carry NaN forward; x is exact, so stop.
-    if (scale == 0.0) {
+    norm_to(r, 2, ir + kIrScale, nullptr);
+    // an exactly zero residual would make the reference scale by 1/0
+    // (gmres_ir.hpp:562-563) and carry NaN forward; x is exact: reason 3
+    hipLaunchKernelGGL(k_ir_gate, dim3(1), dim3(1), 0, st, ir, gd.gi);
+    hipLaunchKernelGGL(k_scale_dev, dim3(nblk(n)), dim3(256), 0, st, n, r, ir + kIrScale, 1, tmp, irgate);
+    round_to(uw_, tmp, n, st);
+    const double t_inner = now_ms();
+    precond_gmres(tmp, d, irgate);
+    hipLaunchKernelGGL(k_scale_dev, dim3(nblk(n)), dim3(256), 0, st, n, d, ir + kIrScale, 0, d, irgate);
+    round_to(uw_, d, n, st);
+    norm_to(d, 2, ir + kIrDnorm, irgate);
+    norm_to(x, 2, ir + kIrXnorm, irgate);  // ||x|| before the update (gmres_ir.hpp:575-577)
+    hipLaunchKernelGGL(k_add_gated, dim3(nblk(n)), dim3(256), 0, st, n, x, d, irgate);
+    round_to(uw_, x, n, st);
+    residual_f64(Ag, x, b, r, st);
+    round_to(ur_, r, n, st);
+    norm_to(r, 2, ir + kIrRraw, irgate);
+    take_record(iter);
+    const double inner_ms = now_ms() - t_inner;  // queueing and completion of this iteration
+    stats_.ir_iterations = static_cast<int32_t>(iter);
+    const int stop = static_cast<int>(mv[kMirIr + 1]);
+    if (stop == 3) {
       stats_.stop_reason = 3;
-      stats_.ir_iterations = static_cast<int32_t>(iter);  // counted as the log counts it
       r_norm = 0.0;
       break;
     }
-    hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, r, 1.0 / scale, tmp);
-    round_to(uw_, tmp, n, st);
-    const double t_inner = now_ms();
-    precond_gmres(tmp, d);
-    const double inner_ms = now_ms() - t_inner;
-    hipLaunchKernelGGL(k_scale_host, dim3(nblk(n)), dim3(256), 0, st, n, d, scale, d);
-    round_to(uw_, d, n, st);
-    const double d_norm = infnrm(d);
-    const double x_norm = infnrm(x);
-    hipLaunchKernelGGL(k_add, dim3(nblk(n)), dim3(256), 0, st, n, x, d);
-    round_to(uw_, x, n, st);
-    residual_f64(A, x, b, r, st);
-    round_to(ur_, r, n, st);
-    r_norm = infnrm(r) / bnorm;
-    stats_.ir_iterations = static_cast<int32_t>(iter);
+    r_norm = mv[kMirIr + 2];
+    const double d_norm = mv[kMirIr + 3];
     if (!std::isfinite(r_norm) || !std::isfinite(d_norm))
       throw Error(kErrNumeric, "the residual is not finite at refinement iteration " + std::to_string(iter));
     // the VERBOSE log's record of the iteration (gmres_ir.hpp:583-596)
     log_.iterations.push_back({static_cast<int32_t>(iter), r_norm, d_norm, now_ms() - t_start, inner_ms});
-    if (d_norm <= x_norm * (10 * eps_uw)) {
-      stats_.stop_reason = 1;
+    if (stop != 0) {
+      stats_.stop_reason = stop;
       break;
     }
-    if (std::fabs(r_norm - r_prev) < tol_) {
-      stats_.stop_reason = 2;
-      break;
-    }
-    r_prev = r_norm;
   }
-  if (max_ir_ == 0) r_norm = bnorm > 0 ? infnrm(r) / bnorm : 0.0;
+  if (max_ir_ == 0) {
+    take_record(0);
+    r_norm = mv[kMirIr + 2];
+  }
   stats_.final_rel_residual = r_norm;
   log_.stop_reason = stats_.stop_reason;
   log_.total_iterations = static_cast<int32_t>(std::min<int64_t>(iter, max_ir_));

@@ -144,7 +144,7 @@ __global__ void k_ilu_lower(IluDev g, const int32_t* __restrict__ rows, int cnt,
                             double* __restrict__ x) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
+  if (t >= cnt || (g.gate != nullptr && *g.gate == 0)) return;
   const int r = rows[t];
   const int p0 = g.Up[r], p1 = g.Up[r + 1] - 1;
   double sum = 0.0;
@@ -174,7 +174,7 @@ __global__ void k_ilu_upper(IluDev g, const int32_t* __restrict__ rows, int cnt,
                             const T* __restrict__ Di, double* __restrict__ x) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= cnt) return;
+  if (t >= cnt || (g.gate != nullptr && *g.gate == 0)) return;
   const int r = rows[t];
   const int p0 = g.ur_ptr[r], p1 = g.ur_ptr[r + 1];
   double v = x[r];

@@ -150,6 +150,10 @@ struct SolveDev {
   // entries between right-hand side r and r + 1: in x, in uvec, and in the
   // partial-sum areas (ldl_solve_multi)
   int64_t sx = 0, su = 0, syp = 0, sup = 0, srp = 0, sxp = 0;
+  // when set, every kernel of an application returns at once if *gate == 0
+  // (an iteration that has already stopped; read at kernel start, written
+  // by earlier work of the same stream)
+  const int* gate = nullptr;
 };
 
 struct DeviceSolvePlan {
@@ -190,6 +194,7 @@ struct IluDev {
   const int32_t* ur_idx = nullptr;
   const int32_t* lo_rows = nullptr;
   const int32_t* up_rows = nullptr;
+  const int* gate = nullptr;  // when set: the triangular solves run only if *gate != 0
 };
 struct IluLevels {
   std::vector<int32_t> h_lo_ptr, h_up_ptr;
@@ -214,6 +219,7 @@ struct CsrDev {
   const int32_t* p = nullptr;
   const int32_t* j = nullptr;
   const double* x = nullptr;
+  const int* gate = nullptr;  // when set: the product runs only if *gate != 0
 };
 
 // y = A x (FP64, rows summed left to right, no fused multiply-add)

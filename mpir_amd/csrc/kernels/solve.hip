@@ -672,10 +672,14 @@ __device__ __forceinline__ void bx_task(const SolveArgs<TF, TD>& a, const SolveD
   }
 }
 
+// the application belongs to an iteration that has stopped (SolveDev::gate)
+__device__ __forceinline__ bool gated_off(const SolveDev& d) { return d.gate != nullptr && *d.gate == 0; }
+
 // ---------------------------------------------------------------- level schedule
 template <int NR, class TF, class TD, class TX>
 __global__ void __launch_bounds__(64) k_fwd_small(SolveArgs<TF, TD> a, SolveDev d, const int32_t* __restrict__ list,
                                                   TX* x, TX* uvec) {
+  if (gated_off(d)) return;
   small_fwd<NR>(a, d, list[blockIdx.x], x, uvec, NoWait{});
 }
 
@@ -683,6 +687,7 @@ template <int NR, class TF, class TD, class TX>
 __global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF, TD> a, SolveDev d, const int32_t* __restrict__ list,
                                                   TX* x) {
   __shared__ TX w[NR * kSmallFront];
+  if (gated_off(d)) return;
   small_bwd<NR>(a, d, list[blockIdx.x], x, w, NoWait{});
 }
 
@@ -692,6 +697,7 @@ __global__ void __launch_bounds__(256) k_level_tiles(SolveArgs<TF, TD> a, SolveD
                                                      const int32_t* __restrict__ tasks, TX* x, TX* uvec) {
   __shared__ TX smem[NR * kVsLen + 4 * kRB];
   __shared__ int flag;
+  if (gated_off(d)) return;
   const int* q = tasks + 3 * blockIdx.x;
   if (kind == kDagFx) fx_task<NR>(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), uvec, smem, NoWait{});
   else if (kind == kDagFu) fu_task<NR>(a, d, q[0], q[1], q[2], uvec, smem, NoWait{});
@@ -760,6 +766,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
   const int ns = d.ns;
+  if (gated_off(d)) return;
   int* lines = cnt + d.cnt_disp;
   Dispenser ds = arrive(lines, &tsh);
   for (;;) {
@@ -792,6 +799,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
   const int ns = d.ns;
+  if (gated_off(d)) return;
   int* lines = cnt + d.cnt_disp + kDispenserLines * kDispenserStride;
   Dispenser ds = arrive(lines, &tsh);
   for (;;) {
@@ -822,7 +830,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
   }
 }
 
-__global__ void k_zero(int n, int* __restrict__ p) {
+__global__ void k_zero(int n, int* __restrict__ p, const int* __restrict__ gate) {
+  if (gate != nullptr && *gate == 0) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
@@ -841,7 +850,7 @@ int resident_blocks(K kernel, int threads) {
 template <int NR, class TF, class TD, class TX>
 void ldl_solve_group(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, const SolveDev& d, TX* x, TX* uvec,
                      hipStream_t st) {
-  hipLaunchKernelGGL(k_zero, dim3((d.ncounters + 255) / 256), dim3(256), 0, st, d.ncounters, d.counters);
+  hipLaunchKernelGGL(k_zero, dim3((d.ncounters + 255) / 256), dim3(256), 0, st, d.ncounters, d.counters, d.gate);
   if (dp.schedule == kScheduleDag) {
     // at least kDispensers workgroups, so that every class can be covered
     // (next_task states why the persistent launch always progresses)

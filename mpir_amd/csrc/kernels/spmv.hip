@@ -28,8 +28,10 @@ constexpr int kRowsPerBlock = 256;
 // order.
 __global__ void __launch_bounds__(256) k_spmv(int n, const int32_t* __restrict__ rp, const int32_t* __restrict__ cj,
                                               const double* __restrict__ av, const double* __restrict__ x,
-                                              const double* __restrict__ b, double* __restrict__ y, int mode) {
+                                              const double* __restrict__ b, double* __restrict__ y, int mode,
+                                              const int* __restrict__ gate) {
 #pragma clang fp contract(off)
+  if (gate != nullptr && *gate == 0) return;
   constexpr int kChunk = 2048;
   __shared__ int32_t sj[kChunk];
   __shared__ double sv[kChunk];
@@ -59,14 +61,14 @@ __global__ void __launch_bounds__(256) k_spmv(int n, const int32_t* __restrict__
 void spmv_f64(const CsrDev& A, const double* x, double* y, hipStream_t st) {
   if (A.n == 0) return;
   hipLaunchKernelGGL(k_spmv, dim3((A.n + kRowsPerBlock - 1) / kRowsPerBlock), dim3(kRowsPerBlock), 0, st, A.n, A.p,
-                     A.j, A.x, x, nullptr, y, 0);
+                     A.j, A.x, x, nullptr, y, 0, A.gate);
   MPIR_HIP(hipGetLastError());
 }
 
 void residual_f64(const CsrDev& A, const double* x, const double* b, double* r, hipStream_t st) {
   if (A.n == 0) return;
   hipLaunchKernelGGL(k_spmv, dim3((A.n + kRowsPerBlock - 1) / kRowsPerBlock), dim3(kRowsPerBlock), 0, st, A.n, A.p,
-                     A.j, A.x, x, b, r, 1);
+                     A.j, A.x, x, b, r, 1, A.gate);
   MPIR_HIP(hipGetLastError());
 }
 

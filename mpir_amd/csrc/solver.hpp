@@ -17,6 +17,8 @@
 
 namespace mpir {
 
+struct GmDev;  // device state of one PrecondGmres call (gmres.hip)
+
 // Outcome of a batch solve: per system s and right-hand side r, entry
 // s * max_rhs + r of the per-instance arrays (slots r >= nrhs[s] unused).
 struct BatchStats {
@@ -219,7 +221,13 @@ class GmresLdl {
   void bnorms(const double* v, int op, int R, std::vector<double>& out);
   double* bpinned(size_t count);
   void put_host(const void* src, size_t bytes, void* dst);
-  void precond_gmres(const double* b, double* res);  // device vectors (permuted)
+  // device-resident PrecondGmres (device vectors, permuted order), skipped
+  // when *gate == 0 (gate may be null)
+  void precond_gmres(const double* b, double* res, const int* gate);
+  GmDev gm_dev(int m);
+  void norm_to(const double* v, int op, double* out, const int* gate);
+  void precond_apply_gated(double* x, const int* gate);
+  void wait_mirror(int slot, double target);
   double dnrm2(const double* v);
   double infnrm(const double* v);
   void release_numeric();
@@ -283,6 +291,8 @@ class GmresLdl {
   // GMRES workspace
   DevBuf d_V_, d_w_, d_b_, d_x_, d_r_, d_d_, d_red_, d_small_;
   DevBuf d_hcol_, d_givens_;
+  DevBuf d_gm_, d_gmi_, d_ir_;  // device state of GMRES-IR (scalars, Hessenberg, rotations, gates)
+  int64_t step_seq_ = 0, ir_seq_ = 0;  // ordinals of the records the device publishes
   std::vector<double> h_small_;
   double* h_pinned_ = nullptr;
   double* h_mirror_ = nullptr;  // h_pinned_ as the device addresses it
