@@ -572,6 +572,132 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
     if (dst[q] != nullptr) *dst[q] = cur[q] - St[w + 4 * q][lane];
 }
 
+// Trailing update tile for a factor held in a 16-bit format (R = 1 FP16,
+// R = 2 BF16): the task layout and the FP32 read-modify-write of
+// k_update_tiles, with the operand tiles in that format on the 16-bit matrix
+// cores (v_mfma_f32_16x16x32, FP32 accumulation). L entries are already
+// rounded to the format as stored; the column operand L_c D is rounded to it
+// as it is formed (the FP32 path keeps that product in FP32). Operand tiles
+// are row-major with k contiguous: lane l of an MFMA reads the 8 k-values
+// 8 (l / 16) .. + 7 of row l % 16 as one 16-byte load, and the row stride of
+// 72 entries (36 banks) puts the 16 rows of a read in distinct bank groups.
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+template <int R>
+struct Mfma16x32;
+template <>
+struct Mfma16x32<1> {
+  using E = _Float16;
+  using V8 = f16x8;
+  static __device__ __forceinline__ f32x4 op(V8 a, V8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma16x32<2> {
+  using E = __bf16;
+  using V8 = bf16x8;
+  static __device__ __forceinline__ f32x4 op(V8 a, V8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int R>
+__global__ void __launch_bounds__(256) k_update_tiles16(FactorArgs<float> a, const Task4* __restrict__ tasks) {
+  using M = Mfma16x32<R>;
+  using E = typename M::E;
+  using V8 = typename M::V8;
+  constexpr int LS = kPanelNB + 8;  // row stride (entries)
+  static_assert(kTile * (kTile + 1) * sizeof(float) <= 2 * kTile * LS * sizeof(E), "staging tile exceeds the operands");
+  __shared__ __align__(16) E Ls[2][kTile * LS];  // Ls[0][i * LS + t] = L[r0 + i, k0 + t], Ls[1]: L[c0 + j, k0 + t] D
+  const Task4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
+  const int s = tk.a, r0 = tk.b, c0 = tk.c;
+  const int klast = tk.d & ((1 << kUpdStepsShift) - 1);
+  const int kfirst = klast - ((tk.d >> kUpdStepsShift) & ((1 << (kUpdPanelBit - kUpdStepsShift)) - 1)) + 1;
+  const bool panel_only = (tk.d >> kUpdPanelBit) & 1;
+  const int f = a.first[s];
+  const int nc = a.first[s + 1] - f;
+  const int nr = a.nrow[s];
+  const int m = nc + nr;
+  const int64_t ld = panel_ld(nc, nr);
+  const int rlim = r0 >= m ? m + nc : m;
+  const float* P = a.panel + a.panel_off[s];
+  float* Pw = a.panel + a.panel_off[s];
+  float* U = nr > 0 ? a.upool + a.upd_off[s] : Pw;
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ra = r0 + lane, rb = c0 + lane;
+  const int64_t ras = ra < rlim ? ra : r0, rbs = rb < m ? rb : c0;
+  float av[16], bv[16], dv[16];
+  auto load = [&](int kk) {
+    const int k0 = kk * kPanelNB;
+    const int nbk = min(kPanelNB, nc - k0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int t = min(w + 4 * q, nbk - 1);
+      av[q] = P[ras + (k0 + t) * ld];
+      bv[q] = P[rbs + (k0 + t) * ld];
+      dv[q] = a.D[f + k0 + t];
+    }
+  };
+  const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[x][y][v] = 0.f;
+  load(kfirst);
+  for (int kk = kfirst; kk <= klast; ++kk) {
+    const int nbk = min(kPanelNB, nc - kk * kPanelNB);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int t = w + 4 * q;
+      Ls[0][lane * LS + t] = static_cast<E>((ra < rlim && t < nbk) ? av[q] : 0.f);
+      Ls[1][lane * LS + t] = static_cast<E>((rb < m && t < nbk) ? bv[q] * dv[q] : 0.f);
+    }
+    __syncthreads();
+    if (kk < klast) load(kk + 1);
+#pragma unroll
+    for (int t0 = 0; t0 < kPanelNB; t0 += 32) {  // entries t >= nbk of the operand rows are zero
+      const V8 a0 = *reinterpret_cast<const V8*>(&Ls[0][(wr + li) * LS + t0 + 8 * lk]);
+      const V8 a1 = *reinterpret_cast<const V8*>(&Ls[0][(wr + 16 + li) * LS + t0 + 8 * lk]);
+      const V8 b0 = *reinterpret_cast<const V8*>(&Ls[1][(wc + li) * LS + t0 + 8 * lk]);
+      const V8 b1 = *reinterpret_cast<const V8*>(&Ls[1][(wc + 16 + li) * LS + t0 + 8 * lk]);
+      acc[0][0] = M::op(a0, b0, acc[0][0]);
+      acc[0][1] = M::op(a0, b1, acc[0][1]);
+      acc[1][0] = M::op(a1, b0, acc[1][0]);
+      acc[1][1] = M::op(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  // staging and read-modify-write as in k_update_tiles (output layout of the
+  // 16x16x32 products: row 4 (l / 16) + v, column l % 16)
+  float(*St)[kTile + 1] = reinterpret_cast<float(*)[kTile + 1]>(&Ls[0][0]);
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) St[wc + 16 * y + li][wr + 16 * x + 4 * lk + v] = acc[x][y][v];
+  __syncthreads();
+  const int r = r0 + lane;
+  float* dst[16];
+  float cur[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = c0 + w + 4 * q;
+    const bool ok = c < m && r < rlim && r >= c && (c < nc || (r < m && !panel_only));
+    float* p = !ok ? Pw : c < nc ? Pw + r + c * ld : U + (r - nc) + static_cast<int64_t>(c - nc) * nr;
+    cur[q] = *p;
+    dst[q] = ok ? p : nullptr;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q)
+    if (dst[q] != nullptr) *dst[q] = cur[q] - St[w + 4 * q][lane];
+}
+
 // Count of D[i] > 0 (the inertia QDLDL_factor returns) and of i with D[i]
 // or Dinv[i] not finite: one atomic per counter and workgroup (256 threads),
 // few workgroups, so that the adds to one counter do not queue up. An
@@ -608,6 +734,16 @@ inline unsigned blocks_for(int64_t n, int bs) { return static_cast<unsigned>((n 
 
 }  // namespace
 
+// the trailing-update launch: 16-bit operands on the 16-bit matrix cores for
+// a factor held in FP16 / BF16, the factor's own precision otherwise
+template <class T, int R>
+void launch_update(const FactorArgs<T>& a, const Task4* tasks, int count, hipStream_t st) {
+  if constexpr (std::is_same<T, float>::value && R != 0)
+    hipLaunchKernelGGL((k_update_tiles16<R>), dim3(count), dim3(256), 0, st, a, tasks);
+  else
+    hipLaunchKernelGGL((k_update_tiles<T, R>), dim3(count), dim3(256), 0, st, a, tasks);
+}
+
 template <class T, int R>
 void factor_levels(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const double* ax_scaled, int64_t nnz,
                    hipStream_t st) {
@@ -631,7 +767,7 @@ void factor_levels(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const dou
         hipLaunchKernelGGL((k_panel_step<T, R, true>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a,
                            dp.pan_task + p0);
       const int u0 = dp.h_upd_ptr[q], u1 = dp.h_upd_ptr[q + 1];
-      if (u1 > u0) hipLaunchKernelGGL((k_update_tiles<T, R>), dim3(u1 - u0), dim3(256), 0, st, a, dp.upd_task + u0);
+      if (u1 > u0) launch_update<T, R>(a, dp.upd_task + u0, u1 - u0, st);
     }
   }
 }
