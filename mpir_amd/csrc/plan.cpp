@@ -60,6 +60,32 @@ class PoolAlloc {
   int64_t top_ = 0;
 };
 
+// Trailing-update tiles of front s: columns c0 in [c_lo, c_hi) and rows r0
+// in [c0, r_hi) (lower part, `lower`) or in [0, r_hi) (the identity rows
+// below the front, listed at r_off + r0), 64 x 64 each. They are listed in
+// super-blocks of kSuperTiles x kSuperTiles tiles, column block by column
+// block: an XCD takes a contiguous run of a launch's list (see xcd_task in
+// the factor kernels), and tiles of one super-block share their row
+// operands as well as their column operands, about 1 MB for a rank-256
+// update, which stays in the XCD's L2 while the super-block runs.
+constexpr int32_t kSuperTiles = 8;
+void push_tiles(std::vector<Task4>& out, int32_t s, int32_t c_lo, int32_t c_hi, int32_t r_hi, int32_t r_off,
+                bool lower, int32_t code) {
+  const int32_t W = kSuperTiles * kTile;
+  for (int32_t cb = c_lo; cb < c_hi; cb += W) {
+    const int32_t ce = std::min(c_hi, cb + W);
+    for (int32_t rb = lower ? cb : 0; rb < r_hi; rb += W) {
+      const int32_t re = std::min(r_hi, rb + W);
+      for (int32_t c0 = cb; c0 < ce; c0 += kTile) {
+        const int32_t base = lower ? c0 : 0;  // rows base, base + 64, ...
+        int32_t r0 = base;
+        if (r0 < rb) r0 += (rb - base + kTile - 1) / kTile * kTile;
+        for (; r0 < re; r0 += kTile) out.push_back({s, r_off + r0, c0, code});
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // Task lists of the dependency-driven solves from the per-level lists. A
@@ -214,20 +240,14 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
             // the diagonal block's own task, then rows below it in chunks of 64
             P.pan_task.push_back({s, k0, t0, k});
             for (int32_t r0 = t0; r0 < m; r0 += kPanelRows) P.pan_task.push_back({s, r0, std::min(m, r0 + kPanelRows), k});
-            for (int32_t c0 = t0; c0 < gend; c0 += kTile)
-              for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, near});
-            if (group_end)
-              for (int32_t c0 = gend; c0 < m; c0 += kTile)
-                for (int32_t r0 = c0; r0 < m; r0 += kTile) P.upd_task.push_back({s, r0, c0, far});
+            push_tiles(P.upd_task, s, t0, gend, m, 0, true, near);
+            if (group_end) push_tiles(P.upd_task, s, gend, m, m, 0, true, far);
           } else {
             // aug rows i < t0 are the only ones nonzero in this step's columns
             for (int32_t r0 = 0; r0 < t0; r0 += kPanelRows)
               P.pan_task.push_back({s, m + r0, m + std::min(t0, r0 + kPanelRows), k});
-            for (int32_t c0 = t0; c0 < gend; c0 += kTile)
-              for (int32_t r0 = 0; r0 < t0; r0 += kTile) P.upd_task.push_back({s, m + r0, c0, near});
-            if (group_end)
-              for (int32_t c0 = gend; c0 < nc; c0 += kTile)
-                for (int32_t r0 = 0; r0 < t0; r0 += kTile) P.upd_task.push_back({s, m + r0, c0, far});
+            push_tiles(P.upd_task, s, t0, gend, t0, m, false, near);
+            if (group_end) push_tiles(P.upd_task, s, gend, nc, t0, m, false, far);
           }
         }
       }
