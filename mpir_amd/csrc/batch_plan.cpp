@@ -203,7 +203,7 @@ void merge_plans(const std::vector<const FactorPlan*>& P, const std::vector<int3
   for (auto* v : {&M.child_ptr, &M.child_list, &M.small_ptr, &M.small_list, &M.big_ptr, &M.big_list, &M.asm_ptr,
                   &M.asm_child_ptr, &M.asm_child, &M.level_step_ptr, &M.pan_ptr, &M.upd_ptr, &M.pan_aug_ptr,
                   &M.upd_aug_ptr, &M.fx_ptr, &M.fu_ptr, &M.br_ptr, &M.bx_ptr, &M.fx_task, &M.fu_task, &M.br_task,
-                  &M.bx_task, &M.frow_off, &M.grow_ptr, &M.cb_base, &M.dag_meta, &M.sn.level_ptr, &M.sn.level_sn})
+                  &M.bx_task, &M.fm_ptr, &M.bm_ptr, &M.fm_task, &M.bm_task, &M.inv_task, &M.frow_off, &M.grow_ptr, &M.cb_base, &M.dag_meta, &M.sn.level_ptr, &M.sn.level_sn})
     v->clear();
   for (auto* v : {&M.upd_off, &M.ypart_off, &M.upart_off, &M.rpart_off, &M.xpart_off, &M.val_dst}) v->clear();
   for (auto* v : {&M.asm_task, &M.pan_task, &M.upd_task, &M.gent, &M.dag_fwd, &M.dag_bwd}) v->clear();
@@ -347,6 +347,13 @@ void merge_plans(const std::vector<const FactorPlan*>& P, const std::vector<int3
   lvl_triples(M.fu_ptr, M.fu_task, &FactorPlan::fu_ptr, &FactorPlan::fu_task);
   lvl_triples(M.br_ptr, M.br_task, &FactorPlan::br_ptr, &FactorPlan::br_task);
   lvl_triples(M.bx_ptr, M.bx_task, &FactorPlan::bx_ptr, &FactorPlan::bx_task);
+  lvl_triples(M.fm_ptr, M.fm_task, &FactorPlan::fm_ptr, &FactorPlan::fm_task);
+  lvl_triples(M.bm_ptr, M.bm_task, &FactorPlan::bm_ptr, &FactorPlan::bm_task);
+  for (size_t s = 0; s < S; ++s) {
+    const std::vector<int32_t>& v = P[s]->inv_task;
+    for (size_t t = 0; t < v.size(); t += 3) M.inv_task.insert(M.inv_task.end(), {b[s].sn + v[t], v[t + 1], v[t + 2]});
+  }
+  M.tuning = S > 0 ? P[0]->tuning : SolveTuning{};
   // assembly tasks with their child ranges
   M.asm_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
   M.asm_child_ptr.assign(1, 0);

@@ -660,7 +660,7 @@ void GmresLdl::upload_plan() {
   dfp_.h_pan_ptr = P.pan_ptr;
   dfp_.h_upd_ptr = P.upd_ptr;
   dfp_.h_pan_aug_ptr = P.pan_aug_ptr;
-  dfp_.n_inv_task = static_cast<int>(P.fx_task.size() / 3);
+  dfp_.n_inv_task = static_cast<int>(P.inv_task.size() / 3);
   dfp_.h_upd_aug_ptr = P.upd_aug_ptr;
   upload_vec(keep_[15], P.asm_child_ptr, st);
   upload_vec(keep_[16], P.asm_child, st);
@@ -675,6 +675,8 @@ void GmresLdl::upload_plan() {
   dsp_.h_fu_ptr = P.fu_ptr;
   dsp_.h_br_ptr = P.br_ptr;
   dsp_.h_bx_ptr = P.bx_ptr;
+  dsp_.h_fm_ptr = P.fm_ptr;
+  dsp_.h_bm_ptr = P.bm_ptr;
   SolveDev& g = dsp_.d;
   g.ns = S.ns;
   g.nfwd = static_cast<int>(P.dag_fwd.size());
@@ -689,10 +691,12 @@ void GmresLdl::upload_plan() {
   g.meta = static_cast<const int32_t*>(up(P.dag_meta));
   g.small_list = dfp_.small_list;
   g.fx_task = static_cast<const int32_t*>(up(P.fx_task));
-  dfp_.inv_task = g.fx_task;
+  dfp_.inv_task = static_cast<const int32_t*>(up(P.inv_task));
   g.fu_task = static_cast<const int32_t*>(up(P.fu_task));
   g.br_task = static_cast<const int32_t*>(up(P.br_task));
   g.bx_task = static_cast<const int32_t*>(up(P.bx_task));
+  g.fm_task = static_cast<const int32_t*>(up(P.fm_task));
+  g.bm_task = static_cast<const int32_t*>(up(P.bm_task));
   g.frow_off = static_cast<const int32_t*>(up(P.frow_off));
   g.grow_ptr = static_cast<const int32_t*>(up(P.grow_ptr));
   g.gent = static_cast<const Task4*>(up(P.gent));
@@ -764,6 +768,12 @@ void GmresLdl::alloc_numeric() {
   alloc_vectors();
   d_panel_.alloc((std::max<int64_t>(1, P.panel_entries) + kPanelSlack) * tf);
   if (uf_ == kF16 || uf_ == kBF16) d_panel16_.alloc((std::max<int64_t>(1, P.panel_entries) + kPanelSlack) * 2);
+  // the slack past the last front holds zeros: the solves' 16-byte tile
+  // loads may read into it, and they rely on every value they read being
+  // finite
+  MPIR_HIP(hipMemsetAsync(d_panel_.as<char>() + P.panel_entries * tf, 0, kPanelSlack * tf, stream_));
+  if (uf_ == kF16 || uf_ == kBF16)
+    MPIR_HIP(hipMemsetAsync(d_panel16_.as<char>() + P.panel_entries * 2, 0, kPanelSlack * 2, stream_));
   d_upool_.alloc(std::max<int64_t>(1, P.upd_pool) * tf);
   d_D_.alloc(n * tf);
   d_Dinv_.alloc(n * tf);

@@ -132,6 +132,8 @@ struct SolveDev {
   const int32_t* fu_task = nullptr;
   const int32_t* br_task = nullptr;
   const int32_t* bx_task = nullptr;
+  const int32_t* fm_task = nullptr;  // narrow fronts
+  const int32_t* bm_task = nullptr;
   const int32_t* frow_off = nullptr;  // row gathers
   const int32_t* grow_ptr = nullptr;
   const Task4* gent = nullptr;
@@ -160,7 +162,7 @@ struct DeviceSolvePlan {
   int schedule = kScheduleDag;
   int nlevels = 0;
   SolveDev d;
-  std::vector<int32_t> h_small_ptr, h_fx_ptr, h_fu_ptr, h_br_ptr, h_bx_ptr;
+  std::vector<int32_t> h_small_ptr, h_fx_ptr, h_fu_ptr, h_br_ptr, h_bx_ptr, h_fm_ptr, h_bm_ptr;
 };
 
 // x (length n, permuted order) is overwritten by (L D L^T)^{-1} x.

@@ -361,9 +361,31 @@ struct ColTile {
   }
 };
 
+// Sums of acc[k] over the 64 lanes of a wave, k < 16, by halving exchanges
+// (fixed order: 17 exchanges instead of 16 x 6): after the step with offset
+// o a lane keeps the half of its entries selected by its bit o. Returns the
+// sum of entry c (set here) in the 4 lanes 4 c', 4 c' + 1, .. of each group.
+template <class TX>
+__device__ __forceinline__ TX wave_sum16(const TX (&acc)[16], int& c) {
+  const int lane = threadIdx.x & 63;
+  TX a8[8], a4[4], a2[2];
+  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a8[i] = (b5 ? acc[i + 8] : acc[i]) + __shfl_xor(b5 ? acc[i] : acc[i + 8], 32);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) a4[i] = (b4 ? a8[i + 4] : a8[i]) + __shfl_xor(b4 ? a8[i] : a8[i + 4], 16);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) a2[i] = (b3 ? a4[i + 2] : a4[i]) + __shfl_xor(b3 ? a4[i] : a4[i + 2], 8);
+  TX a1 = (b2 ? a2[1] : a2[0]) + __shfl_xor(b2 ? a2[0] : a2[1], 4);
+  a1 += __shfl_xor(a1, 2);
+  a1 += __shfl_xor(a1, 1);
+  c = (b5 ? 8 : 0) + (b4 ? 4 : 0) + (b3 ? 2 : 0) + (b2 ? 1 : 0);
+  return a1;
+}
+
 // The tile's column sums (vs: zeros from len to kVsLen) stored to out[t],
-// t < bw. The 16 sums of each lane are reduced across the wave by halving
-// exchanges (fixed order: 17 exchanges instead of 16 x 6).
+// t < bw. The 16 sums of each lane are reduced across the wave by
+// wave_sum16.
 template <class TF, class TX>
 __device__ __forceinline__ void col_dots(ColTile<TF>& T, const TX* vs, TX* out) {
   using CT = ColTile<TF>;
@@ -391,20 +413,8 @@ __device__ __forceinline__ void col_dots(ColTile<TF>& T, const TX* vs, TX* out) 
       for (int e = 0; e < CT::N; ++e) acc[c] += (ok[g][e] ? static_cast<TX>(T.buf[u].e[e]) : TX(0)) * v[g][e];
     }
   }
-  // halving exchanges: after the step with offset o, a lane keeps the half
-  // of its columns selected by its bit o
-  TX a8[8], a4[4], a2[2];
-  const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) a8[i] = (b5 ? acc[i + 8] : acc[i]) + __shfl_xor(b5 ? acc[i] : acc[i + 8], 32);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) a4[i] = (b4 ? a8[i + 4] : a8[i]) + __shfl_xor(b4 ? a8[i] : a8[i + 4], 16);
-#pragma unroll
-  for (int i = 0; i < 2; ++i) a2[i] = (b3 ? a4[i + 2] : a4[i]) + __shfl_xor(b3 ? a4[i] : a4[i + 2], 8);
-  TX a1 = (b2 ? a2[1] : a2[0]) + __shfl_xor(b2 ? a2[0] : a2[1], 4);
-  a1 += __shfl_xor(a1, 2);
-  a1 += __shfl_xor(a1, 1);
-  const int c = (b5 ? 8 : 0) + (b4 ? 4 : 0) + (b3 ? 2 : 0) + (b2 ? 1 : 0);
+  int c = 0;
+  const TX a1 = wave_sum16(acc, c);
   if ((lane & 3) == 0 && w * 16 + c < T.bw) st_dev(out + w * 16 + c, a1);
 }
 
@@ -672,6 +682,299 @@ __device__ __forceinline__ void bx_task(const SolveArgs<TF, TD>& a, const SolveD
   }
 }
 
+// ---------------------------------------------------------------- narrow fronts
+// A big front with nc <= 64 columns and at most a few kNarrowChunk-row
+// chunks of R (the plan's narrow fronts). Its forward tasks (one per chunk)
+// each form y = W^T v1 themselves (W is at most 64 x 64) and leave the
+// final u = v_R - L_21 y of their rows, so no task waits on another task of
+// its own front; its backward task forms x1 = W (D^{-1} y - L_21^T x_R)
+// alone. Every reduction runs in a fixed order.
+//
+// Tile entries outside the front (columns past nc, rows past the chunk)
+// read clamped or neighbouring addresses inside the panel pool, whose
+// slack past the last front is zero: they are finite, and each meets a
+// zero entry of v1, y, x_R or z or lands in a sum that is not stored, so
+// no entry is masked (a factor with a non-finite entry gives a non-finite
+// result either way).
+//
+// W (nc x nc, at most 64 x 64) in 16-byte packs: lane l of wave w holds
+// N = 16 / sizeof(TF) consecutive rows N (l % 16) + 16 N p of the columns
+// 16 k + 4 w + l / 16 (k < 4, p < P = 4 / N rounded up). Columns past the
+// front read its last column, rows past it the next entries in memory.
+template <class TF>
+struct WTile {
+  static constexpr int N = 16 / sizeof(TF);
+  static constexpr int P = (64 + 16 * N - 1) / (16 * N);
+  Pack<TF> e[4][P];
+  __device__ __forceinline__ static int row(int p) { return N * (threadIdx.x & 15) + 16 * N * p; }
+  __device__ __forceinline__ static int col(int k) { return 16 * k + 4 * (threadIdx.x >> 6) + ((threadIdx.x & 63) >> 4); }
+  __device__ __forceinline__ void load(const TF* base, int64_t ld, int nc) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        const int r = row(p);
+        e[k][p] = load_pack(base + (r < nc ? r : 0) + static_cast<int64_t>(min(col(k), nc - 1)) * ld);
+      }
+  }
+};
+
+// L_21 rows of a narrow front, kNarrowChunk rows x nc <= 64 columns, entry
+// (r, c) at base[r + c * ld]: wave w takes rows 64 w .. 64 w + 63; lane l
+// loads N = 16 / sizeof(TF) consecutive rows 64 w + N (l % LPC) .. of the
+// columns N g + l / LPC (LPC = 64 / N lanes per column, load g < 64 / N).
+// Loads go in batches of at most MB per lane (batch b: columns from
+// N B b on).
+template <class TF, int MB = 16>
+struct NarrowTile {
+  static constexpr int N = 16 / sizeof(TF);
+  static constexpr int LPC = 64 / N;  // lanes per column
+  static constexpr int NL = 64 / N;   // loads per lane
+  static constexpr int B = NL < MB ? NL : MB;
+  static constexpr int NB = NL / B;
+  static_assert(NL % B == 0 && 4 * 64 == kNarrowChunk, "tile shape");
+  Pack<TF> buf[B];
+  const TF* base;
+  int64_t ld;
+  int rows, nc;
+  __device__ __forceinline__ int row0() const { return 64 * (threadIdx.x >> 6) + N * ((threadIdx.x & 63) % LPC); }
+  __device__ __forceinline__ int col(int g) const { return N * g + (threadIdx.x & 63) / LPC; }
+  __device__ __forceinline__ void load(int batch) {
+    const int r = row0();
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int c = min(col(batch * B + u), nc - 1);
+      // rows past the chunk read its first rows instead (their sums are
+      // not stored, or they meet zero entries of x_R)
+      buf[u] = load_pack(base + (r < rows ? r : 0) + static_cast<int64_t>(c) * ld);
+    }
+  }
+};
+
+// Forward, rows [h kNarrowChunk, ...) of R of narrow front s (task (s, h)):
+// gathers v on the front's columns and on those rows, y = W^T v1 (the h = 0
+// task stores it, one partial sum per column), u = v_R - L_21 y stored to
+// the update vector. The front's last task counts it as a finished child of
+// its parent. NR right-hand sides. LDS per right-hand side: v1 (64), y (64),
+// v_R (kNarrowChunk).
+template <int NR, class TF, class TD, class TX, class Wait>
+__device__ __forceinline__ void fm_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, int h, const TX* x,
+                                        TX* uvec, TX* smem, Wait wait) {
+  using NT = NarrowTile<TF>;
+  static_assert(NR == 1 || NT::NB == 1, "several right-hand sides need the tile in registers");
+  static_assert(128 + kNarrowChunk <= kVsLen, "staging buffer");
+  const FrontShape F(a, s);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int r0 = h * kNarrowChunk, rows = max(0, min(kNarrowChunk, F.nr - r0));
+  const TF* P = a.panel + a.panel_off[s];
+  WTile<TF> W;
+  W.load(P + F.m, F.ld, F.nc);
+  RowGather G1, G2;
+  TX bq[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) bq[r] = TX(0);
+  if (tid < F.nc) {
+    G1.pre(d, d.frow_off[s] + tid);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) bq[r] = x[r * d.sx + F.f + tid];
+  }
+  if (tid < rows) G2.pre(d, d.frow_off[s] + F.nc + r0 + tid);
+  wait();
+  const TX* up = static_cast<const TX*>(d.upart);
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    TX* vs = smem + r * kVsLen;
+    if (tid < 64) vs[tid] = tid < F.nc ? G1.value(d, uvec + r * d.su, up + r * d.sup, bq[r]) : TX(0);
+    vs[128 + tid] = tid < rows ? G2.value(d, uvec + r * d.su, up + r * d.sup, TX(0)) : TX(0);
+  }
+  // the L_21 tile is in flight while y is formed
+  NT T;
+  T.base = P + F.nc + r0;
+  T.ld = F.ld;
+  T.rows = rows;
+  T.nc = F.nc;
+  T.load(0);
+  __syncthreads();
+  // y = W^T v1: each lane's column sums over its rows, then over the 16
+  // lanes that hold the rest of those columns
+  using WT = WTile<TF>;
+  TX* yp = static_cast<TX*>(d.ypart) + d.ypart_off[s];
+  each_rhs<NR>([&](int r) {
+    TX* vs = smem + r * kVsLen;
+    TX acc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[k] = TX(0);
+#pragma unroll
+      for (int p = 0; p < WT::P; ++p)
+#pragma unroll
+        for (int i = 0; i < WT::N; ++i) {
+          const int q = WT::row(p) + i;  // past 64 only for 16-bit factors
+          acc[k] += static_cast<TX>(W.e[k][p].e[i]) * (q < 64 ? vs[q] : TX(0));
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[k] += __shfl_xor(acc[k], off);
+    if ((lane & 15) == 0)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int c = WT::col(k);
+        vs[64 + c] = c < F.nc ? acc[k] : TX(0);
+        if (h == 0 && c < F.nc) st_dev(yp + r * d.syp + c, acc[k]);
+      }
+  });
+  __syncthreads();
+  // u = v_R - L_21 y: lanes with the same rows (l % LPC) are summed
+  const int rr = T.row0();
+  TX acc[NR][NT::N];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int e = 0; e < NT::N; ++e) acc[r][e] = TX(0);
+#pragma unroll
+  for (int b = 0; b < NT::NB; ++b) {
+    if (b > 0) T.load(b);
+#pragma unroll
+    for (int u = 0; u < NT::B; ++u) {
+      const int c = T.col(b * NT::B + u);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const TX y = smem[r * kVsLen + 64 + c];
+#pragma unroll
+        for (int e = 0; e < NT::N; ++e) acc[r][e] += static_cast<TX>(T.buf[u].e[e]) * y;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = NT::LPC; off < 64; off <<= 1)
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int e = 0; e < NT::N; ++e) acc[r][e] += __shfl_xor(acc[r][e], off);
+  if (lane < NT::LPC)
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+      for (int e = 0; e < NT::N; ++e) {
+        const int q = rr + e;
+        if (q < rows) st_dev(uvec + r * d.su + a.rows_off[s] + r0 + q, smem[r * kVsLen + 128 + q] - acc[r][e]);
+      }
+  const int prev = block_publish(d.counters + kUDone * d.ns + s);
+  const int* ms = d.meta + kMetaStride * s;
+  if (prev == ms[kMetaNFu] - 1 && ms[kMetaParent] >= 0) add_dev(d.counters + kCDone * d.ns + ms[kMetaParent], 1);
+}
+
+// Backward, narrow front s: t = L_21^T x_R over every chunk of R, z =
+// D^{-1} y - t, x1 = W z stored to x. NR right-hand sides. LDS per
+// right-hand side: x_R of a chunk, later z (from 0); the waves' partial sums
+// of t, later of x1 (from kNarrowChunk, 4 x 64).
+template <int NR, class TF, class TD, class TX, class Wait>
+__device__ __forceinline__ void bm_task(const SolveArgs<TF, TD>& a, const SolveDev& d, int s, TX* x, TX* smem,
+                                        Wait wait) {
+  // batches of 8 loads: the tile and one right-hand side's column sums of a
+  // batch stay within the register budget
+  using NT = NarrowTile<TF, 8>;
+  static_assert(kNarrowChunk + 256 <= kVsLen, "staging buffer");
+  const FrontShape F(a, s);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int nch = max(1, (F.nr + kNarrowChunk - 1) / kNarrowChunk);
+  const TF* P = a.panel + a.panel_off[s];
+  NT T;
+  T.base = P + F.nc;
+  T.ld = F.ld;
+  T.rows = min(kNarrowChunk, F.nr);
+  T.nc = F.nc;
+  T.load(0);
+  wait();
+  const int32_t* R = a.rows + a.rows_off[s];
+  const int rr = T.row0();
+#pragma unroll
+  for (int b = 0; b < NT::NB; ++b) {
+    if (b > 0 && NT::N * NT::B * b >= F.nc) break;  // no column of the front in this batch
+    for (int hh = 0; hh < nch; ++hh) {
+      const int rows = max(0, min(kNarrowChunk, F.nr - hh * kNarrowChunk));
+      if (b > 0 || hh > 0) {
+        T.base = P + F.nc + hh * kNarrowChunk;
+        T.rows = rows;
+        T.load(b);
+      }
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        smem[r * kVsLen + tid] = tid < rows ? ld_dev(x + r * d.sx + R[hh * kNarrowChunk + tid]) : TX(0);
+      __syncthreads();
+      // the chunk's column sums of one right-hand side at a time: the lanes
+      // with the same columns (l / LPC) are summed, and lane l % LPC == 0 of
+      // wave w adds them to the wave's running sums (chunk order)
+      each_rhs<NR>([&](int r) {
+        TX acc[NT::B];
+#pragma unroll
+        for (int u = 0; u < NT::B; ++u) {
+          acc[u] = TX(0);
+#pragma unroll
+          for (int e = 0; e < NT::N; ++e) acc[u] += static_cast<TX>(T.buf[u].e[e]) * smem[r * kVsLen + rr + e];
+        }
+#pragma unroll
+        for (int off = 1; off < NT::LPC; off <<= 1)
+#pragma unroll
+          for (int u = 0; u < NT::B; ++u) acc[u] += __shfl_xor(acc[u], off);
+        if (lane % NT::LPC == 0)
+#pragma unroll
+          for (int u = 0; u < NT::B; ++u) {
+            TX* t = smem + r * kVsLen + kNarrowChunk + 64 * w + T.col(b * NT::B + u);
+            *t = hh == 0 ? acc[u] : *t + acc[u];
+          }
+      });
+      __syncthreads();  // the chunk's x_R is read by every wave
+    }
+  }
+  const TX* yp = static_cast<const TX*>(d.ypart) + d.ypart_off[s];
+  if (tid < 64)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      TX* zs = smem + r * kVsLen;
+      const TX* red = zs + kNarrowChunk;
+      const TX t = (red[tid] + red[64 + tid]) + (red[128 + tid] + red[192 + tid]);
+      zs[tid] = tid < F.nc ? static_cast<TX>(a.Dinv[F.f + tid]) * ld_dev(yp + r * d.syp + tid) - t : TX(0);
+    }
+  // W after the tile is used up: it does not hold registers during the chunks
+  using WT = WTile<TF>;
+  WT W;
+  W.load(P + F.m, F.ld, F.nc);
+  __syncthreads();
+  // x1 = W z: each lane's row sums over its 4 columns, then over the 4 lanes
+  // of the wave with the same rows, then the waves in order
+  each_rhs<NR>([&](int r) {
+    TX* zs = smem + r * kVsLen;
+    TX part[WT::P][WT::N];
+#pragma unroll
+    for (int p = 0; p < WT::P; ++p)
+#pragma unroll
+      for (int i = 0; i < WT::N; ++i) {
+        part[p][i] = TX(0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[p][i] += static_cast<TX>(W.e[k][p].e[i]) * zs[WT::col(k)];
+        part[p][i] += __shfl_xor(part[p][i], 16);
+        part[p][i] += __shfl_xor(part[p][i], 32);
+      }
+    if (lane < 16)
+#pragma unroll
+      for (int p = 0; p < WT::P; ++p)
+#pragma unroll
+        for (int i = 0; i < WT::N; ++i)
+          if (WT::row(p) + i < 64) zs[kNarrowChunk + 64 * w + WT::row(p) + i] = part[p][i];
+  });
+  __syncthreads();
+  if (tid < F.nc)
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const TX* red = smem + r * kVsLen + kNarrowChunk;
+      st_dev(x + r * d.sx + F.f + tid, (red[tid] + red[64 + tid]) + (red[128 + tid] + red[192 + tid]));
+    }
+  block_publish(d.counters + kBDone * d.ns + s);
+}
+
 // the application belongs to an iteration that has stopped (SolveDev::gate)
 __device__ __forceinline__ bool gated_off(const SolveDev& d) { return d.gate != nullptr && *d.gate == 0; }
 
@@ -702,7 +1005,9 @@ __global__ void __launch_bounds__(256) k_level_tiles(SolveArgs<TF, TD> a, SolveD
   if (kind == kDagFx) fx_task<NR>(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), uvec, smem, NoWait{});
   else if (kind == kDagFu) fu_task<NR>(a, d, q[0], q[1], q[2], uvec, smem, NoWait{});
   else if (kind == kDagBr) br_task<NR>(a, d, q[0], q[1], q[2], static_cast<const TX*>(x), smem, NoWait{});
-  else bx_task<NR>(a, d, q[0], q[1], q[2], x, smem, &flag, NoWait{});
+  else if (kind == kDagBx) bx_task<NR>(a, d, q[0], q[1], q[2], x, smem, &flag, NoWait{});
+  else if (kind == kDagFm) fm_task<NR>(a, d, q[0], q[1], static_cast<const TX*>(x), uvec, smem, NoWait{});
+  else bm_task<NR>(a, d, q[0], x, smem, NoWait{});
 }
 
 // ---------------------------------------------------------------- dag schedule
@@ -784,10 +1089,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
       const int s = tk.b;
       fx_task<NR>(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), uvec, smem,
                   BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err});
-    } else {  // kDagFu
+    } else if (tk.a == kDagFu) {
       const int s = tk.b;
       fu_task<NR>(a, d, s, tk.c, tk.d, uvec, smem,
                   BlockWait{cnt + kXDone * ns + s, d.meta[kMetaStride * s + kMetaNFx], d.err});
+    } else {  // kDagFm
+      const int s = tk.b;
+      fm_task<NR>(a, d, s, tk.c, static_cast<const TX*>(x), uvec, smem,
+                  BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err});
     }
   }
 }
@@ -822,10 +1131,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
       const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
       const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
       br_task<NR>(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), smem, BlockWait{pc, target, d.err});
-    } else {  // kDagBx
+    } else if (tk.a == kDagBx) {
       const int s = tk.b;
       bx_task<NR>(a, d, s, tk.c, tk.d, x, smem, &flag,
                   BlockWait{cnt + kRDone * ns + s, d.meta[kMetaStride * s + kMetaNBr], d.err});
+    } else {  // kDagBm
+      const int s = tk.b;
+      const int p = d.meta[kMetaStride * s + kMetaParent];
+      const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
+      const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
+      bm_task<NR>(a, d, s, x, smem, BlockWait{pc, target, d.err});
     }
   }
 }
@@ -879,11 +1194,13 @@ void ldl_solve_group(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, cons
                          uvec);
     tiles(kDagFx, dp.h_fx_ptr, d.fx_task, l);
     tiles(kDagFu, dp.h_fu_ptr, d.fu_task, l);
+    tiles(kDagFm, dp.h_fm_ptr, d.fm_task, l);
   }
   // backward, root to leaves
   for (int l = dp.nlevels - 1; l >= 0; --l) {
     tiles(kDagBr, dp.h_br_ptr, d.br_task, l);
     tiles(kDagBx, dp.h_bx_ptr, d.bx_task, l);
+    tiles(kDagBm, dp.h_bm_ptr, d.bm_task, l);
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
       hipLaunchKernelGGL((k_bwd_small<NR, TF, TD, TX>), dim3(s1 - s0), dim3(64), 0, st, a, d, d.small_list + s0, x);

@@ -89,9 +89,9 @@ void push_tiles(std::vector<Task4>& out, int32_t s, int32_t c_lo, int32_t c_hi, 
 }  // namespace
 
 // Task lists of the dependency-driven solves from the per-level lists. A
-// forward task of a front waits for its children (small fronts, fx) or for
-// all its y values (fu); a backward task waits for its parent (small
-// fronts, br) or for its br tasks (bx). Children sit on lower levels and a
+// forward task of a front waits for its children (small fronts, fx, fm) or
+// for all its y values (fu); a backward task waits for its parent (small
+// fronts, br, bm) or for its br tasks (bx). Children sit on lower levels and a
 // front's own kinds follow each other within its level, so every wait is on
 // tasks earlier in the list.
 void build_dag_schedule(FactorPlan& P) {
@@ -115,6 +115,7 @@ void build_dag_schedule(FactorPlan& P) {
   };
   count(P.fx_task, kMetaNFx);
   count(P.fu_task, kMetaNFu);
+  count(P.fm_task, kMetaNFu);
   count(P.br_task, kMetaNBr);
   auto grouped = [](std::vector<Task4>& out, int32_t b, int32_t e) {
     for (int32_t q = b; q < e; q += kDagGroup) out.push_back({kDagSmall, q, std::min(kDagGroup, e - q), 0});
@@ -128,12 +129,126 @@ void build_dag_schedule(FactorPlan& P) {
     grouped(P.dag_fwd, P.small_ptr[l], P.small_ptr[l + 1]);
     tiles(P.dag_fwd, kDagFx, P.fx_task, P.fx_ptr[l], P.fx_ptr[l + 1]);
     tiles(P.dag_fwd, kDagFu, P.fu_task, P.fu_ptr[l], P.fu_ptr[l + 1]);
+    tiles(P.dag_fwd, kDagFm, P.fm_task, P.fm_ptr[l], P.fm_ptr[l + 1]);
   }
   for (int32_t l = S.nlevels - 1; l >= 0; --l) {
     tiles(P.dag_bwd, kDagBr, P.br_task, P.br_ptr[l], P.br_ptr[l + 1]);
     tiles(P.dag_bwd, kDagBx, P.bx_task, P.bx_ptr[l], P.bx_ptr[l + 1]);
+    tiles(P.dag_bwd, kDagBm, P.bm_task, P.bm_ptr[l], P.bm_ptr[l + 1]);
     grouped(P.dag_bwd, P.small_ptr[l], P.small_ptr[l + 1]);
   }
+}
+
+void build_solve_schedule(FactorPlan& P, const SolveTuning& t) {
+  const Supernodes& S = P.sn;
+  const int32_t ns = S.ns;
+  P.tuning = t;
+  for (auto* v : {&P.fx_task, &P.fu_task, &P.br_task, &P.bx_task, &P.fm_task, &P.bm_task, &P.inv_task}) v->clear();
+  P.nypart = P.nupart = P.nrpart = P.nxpart = 0;
+  P.ncb_total = 0;
+  auto narrow = [&](int32_t s) {
+    const int32_t nc = S.first[s + 1] - S.first[s];
+    return nc + S.nrow[s] > kSmallFront && nc <= kSolveRB && S.nrow[s] <= t.narrow_rows;
+  };
+  // solve tasks of the big fronts, per level
+  P.fx_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.fu_ptr = P.br_ptr = P.bx_ptr = P.fm_ptr = P.bm_ptr = P.fx_ptr;
+  P.cb_base.assign(ns, 0);
+  P.ypart_off.assign(ns, 0);
+  P.upart_off.assign(ns, 0);
+  P.rpart_off.assign(ns, 0);
+  P.xpart_off.assign(ns, 0);
+  for (int32_t l = 0; l < S.nlevels; ++l) {
+    for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
+      const int32_t s = P.big_list[q];
+      const int32_t nc = S.first[s + 1] - S.first[s];
+      const int32_t nr = S.nrow[s];
+      const int32_t ncb = (nc + kSolveRB - 1) / kSolveRB;
+      const int32_t nrb = (nr + kSolveRB - 1) / kSolveRB;
+      const int32_t nh = (nc + kGemvCols - 1) / kGemvCols;
+      const int32_t nhx = (nc + kXcRows - 1) / kXcRows;
+      const int32_t nhr = (nr + kXcRows - 1) / kXcRows;
+      // the factor's W pass: rows up to the block's last column (W is upper
+      // triangular)
+      for (int32_t j = 0; j < ncb; ++j) {
+        const int32_t rend = std::min(nc, (j + 1) * kSolveRB);
+        for (int32_t h = 0; h * kXcRows < rend; ++h) P.inv_task.insert(P.inv_task.end(), {s, j, h});
+      }
+      P.cb_base[s] = P.ncb_total;
+      P.ncb_total += ncb;
+      P.ypart_off[s] = P.nypart;
+      P.nypart += static_cast<int64_t>(ncb) * nhx * kSolveRB;
+      if (narrow(s)) {
+        const int32_t nch = std::max<int32_t>(1, (nr + kNarrowChunk - 1) / kNarrowChunk);
+        for (int32_t h = 0; h < nch; ++h) P.fm_task.insert(P.fm_task.end(), {s, h, 0});
+        P.bm_task.insert(P.bm_task.end(), {s, 0, 0});
+        continue;
+      }
+      P.upart_off[s] = P.nupart;
+      P.nupart += static_cast<int64_t>(nrb) * nh * kSolveRB;
+      P.rpart_off[s] = P.nrpart;
+      P.nrpart += static_cast<int64_t>(ncb) * nhr * kSolveRB;
+      P.xpart_off[s] = P.nxpart;
+      P.nxpart += static_cast<int64_t>(ncb) * nh * kSolveRB;
+      for (int32_t j = 0; j < ncb; ++j) {
+        const int32_t rend = std::min(nc, (j + 1) * kSolveRB);
+        for (int32_t h = 0; h * kXcRows < rend; ++h) P.fx_task.insert(P.fx_task.end(), {s, j, h});
+      }
+      for (int32_t b = 0; b < nrb; ++b)
+        for (int32_t h = 0; h < nh; ++h) P.fu_task.insert(P.fu_task.end(), {s, b, h});
+      for (int32_t j = 0; j < ncb; ++j)
+        for (int32_t h = 0; h < nhr; ++h) P.br_task.insert(P.br_task.end(), {s, j, h});
+      // bx: columns from the block's first row on
+      for (int32_t i = 0; i < ncb; ++i)
+        for (int32_t h = i * kSolveRB / kGemvCols; h < nh; ++h) P.bx_task.insert(P.bx_task.end(), {s, i, h});
+    }
+    P.fx_ptr[l + 1] = static_cast<int32_t>(P.fx_task.size() / 3);
+    P.fu_ptr[l + 1] = static_cast<int32_t>(P.fu_task.size() / 3);
+    P.br_ptr[l + 1] = static_cast<int32_t>(P.br_task.size() / 3);
+    P.bx_ptr[l + 1] = static_cast<int32_t>(P.bx_task.size() / 3);
+    P.fm_ptr[l + 1] = static_cast<int32_t>(P.fm_task.size() / 3);
+    P.bm_ptr[l + 1] = static_cast<int32_t>(P.bm_task.size() / 3);
+  }
+  // row gathers: children in order, each child's rows ascending
+  {
+    P.frow_off.assign(static_cast<size_t>(ns) + 1, 0);
+    for (int32_t s = 0; s < ns; ++s) {
+      const int64_t m = static_cast<int64_t>(S.first[s + 1] - S.first[s]) + S.nrow[s];
+      const int64_t next = P.frow_off[s] + m;
+      if (next > INT32_MAX) throw std::length_error("front rows exceed the 32-bit gather index");
+      P.frow_off[s + 1] = static_cast<int32_t>(next);
+    }
+    std::vector<int32_t> cnt(static_cast<size_t>(P.frow_off[ns]) + 1, 0);
+    for (int32_t s = 0; s < ns; ++s)
+      for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+        const int32_t c = P.child_list[q];
+        for (int32_t r = 0; r < S.nrow[c]; ++r) cnt[P.frow_off[s] + S.relidx[S.rows_off[c] + r] + 1]++;
+      }
+    for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
+    P.grow_ptr = cnt;
+    P.gent.assign(static_cast<size_t>(cnt.back()), Task4{0, 0, 0, 0});
+    std::vector<int32_t> next(cnt.begin(), cnt.end() - 1);
+    for (int32_t s = 0; s < ns; ++s)
+      for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
+        const int32_t c = P.child_list[q];
+        const int32_t ncc = S.first[c + 1] - S.first[c];
+        // small and narrow children leave u final in the update vector
+        const bool partial = ncc + S.nrow[c] > kSmallFront && !narrow(c);
+        const int32_t nhc = partial ? (ncc + kGemvCols - 1) / kGemvCols : 0;
+        for (int32_t r = 0; r < S.nrow[c]; ++r) {
+          const int64_t src = S.rows_off[c] + r;
+          const int64_t pb = partial ? P.upart_off[c] + static_cast<int64_t>(r / kSolveRB) * nhc * kSolveRB + r % kSolveRB : 0;
+          if (src > INT32_MAX || pb > INT32_MAX) throw std::length_error("update vectors exceed the 32-bit gather index");
+          P.gent[next[P.frow_off[s] + S.relidx[S.rows_off[c] + r]]++] =
+              Task4{static_cast<int32_t>(src), static_cast<int32_t>(pb), nhc, 0};
+        }
+      }
+    // two unused entries past the end: a row's first two entries are read
+    // before its count is known
+    P.gent.push_back(Task4{0, 0, 0, 0});
+    P.gent.push_back(Task4{0, 0, 0, 0});
+  }
+  build_dag_schedule(P);
 }
 
 FactorPlan build_plan_from_supernodes(Supernodes sn) {
@@ -295,91 +410,7 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     P.asm_ptr.swap(kept_level);
   }
 
-  // solve tasks of the big fronts, per level
-  P.fx_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
-  P.fu_ptr = P.br_ptr = P.bx_ptr = P.fx_ptr;
-  P.cb_base.assign(ns, 0);
-  P.ypart_off.assign(ns, 0);
-  P.upart_off.assign(ns, 0);
-  P.rpart_off.assign(ns, 0);
-  P.xpart_off.assign(ns, 0);
-  for (int32_t l = 0; l < S.nlevels; ++l) {
-    for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
-      const int32_t s = P.big_list[q];
-      const int32_t nc = S.first[s + 1] - S.first[s];
-      const int32_t nr = S.nrow[s];
-      const int32_t ncb = (nc + kSolveRB - 1) / kSolveRB;
-      const int32_t nrb = (nr + kSolveRB - 1) / kSolveRB;
-      const int32_t nh = (nc + kGemvCols - 1) / kGemvCols;
-      const int32_t nhx = (nc + kXcRows - 1) / kXcRows;
-      const int32_t nhr = (nr + kXcRows - 1) / kXcRows;
-      P.cb_base[s] = P.ncb_total;
-      P.ncb_total += ncb;
-      P.ypart_off[s] = P.nypart;
-      P.nypart += static_cast<int64_t>(ncb) * nhx * kSolveRB;
-      P.upart_off[s] = P.nupart;
-      P.nupart += static_cast<int64_t>(nrb) * nh * kSolveRB;
-      P.rpart_off[s] = P.nrpart;
-      P.nrpart += static_cast<int64_t>(ncb) * nhr * kSolveRB;
-      P.xpart_off[s] = P.nxpart;
-      P.nxpart += static_cast<int64_t>(ncb) * nh * kSolveRB;
-      // fx: rows up to the block's last column (W is upper triangular)
-      for (int32_t j = 0; j < ncb; ++j) {
-        const int32_t rend = std::min(nc, (j + 1) * kSolveRB);
-        for (int32_t h = 0; h * kXcRows < rend; ++h) P.fx_task.insert(P.fx_task.end(), {s, j, h});
-      }
-      for (int32_t b = 0; b < nrb; ++b)
-        for (int32_t h = 0; h < nh; ++h) P.fu_task.insert(P.fu_task.end(), {s, b, h});
-      for (int32_t j = 0; j < ncb; ++j)
-        for (int32_t h = 0; h < nhr; ++h) P.br_task.insert(P.br_task.end(), {s, j, h});
-      // bx: columns from the block's first row on
-      for (int32_t i = 0; i < ncb; ++i)
-        for (int32_t h = i * kSolveRB / kGemvCols; h < nh; ++h) P.bx_task.insert(P.bx_task.end(), {s, i, h});
-    }
-    P.fx_ptr[l + 1] = static_cast<int32_t>(P.fx_task.size() / 3);
-    P.fu_ptr[l + 1] = static_cast<int32_t>(P.fu_task.size() / 3);
-    P.br_ptr[l + 1] = static_cast<int32_t>(P.br_task.size() / 3);
-    P.bx_ptr[l + 1] = static_cast<int32_t>(P.bx_task.size() / 3);
-  }
-  // row gathers: children in order, each child's rows ascending
-  {
-    P.frow_off.assign(static_cast<size_t>(ns) + 1, 0);
-    for (int32_t s = 0; s < ns; ++s) {
-      const int64_t m = static_cast<int64_t>(S.first[s + 1] - S.first[s]) + S.nrow[s];
-      const int64_t next = P.frow_off[s] + m;
-      if (next > INT32_MAX) throw std::length_error("front rows exceed the 32-bit gather index");
-      P.frow_off[s + 1] = static_cast<int32_t>(next);
-    }
-    std::vector<int32_t> cnt(static_cast<size_t>(P.frow_off[ns]) + 1, 0);
-    for (int32_t s = 0; s < ns; ++s)
-      for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
-        const int32_t c = P.child_list[q];
-        for (int32_t r = 0; r < S.nrow[c]; ++r) cnt[P.frow_off[s] + S.relidx[S.rows_off[c] + r] + 1]++;
-      }
-    for (size_t i = 1; i < cnt.size(); ++i) cnt[i] += cnt[i - 1];
-    P.grow_ptr = cnt;
-    P.gent.assign(static_cast<size_t>(cnt.back()), Task4{0, 0, 0, 0});
-    std::vector<int32_t> next(cnt.begin(), cnt.end() - 1);
-    for (int32_t s = 0; s < ns; ++s)
-      for (int32_t q = P.child_ptr[s]; q < P.child_ptr[s + 1]; ++q) {
-        const int32_t c = P.child_list[q];
-        const int32_t ncc = S.first[c + 1] - S.first[c];
-        const bool big = ncc + S.nrow[c] > kSmallFront;
-        const int32_t nhc = big ? (ncc + kGemvCols - 1) / kGemvCols : 0;
-        for (int32_t r = 0; r < S.nrow[c]; ++r) {
-          const int64_t src = S.rows_off[c] + r;
-          const int64_t pb = big ? P.upart_off[c] + static_cast<int64_t>(r / kSolveRB) * nhc * kSolveRB + r % kSolveRB : 0;
-          if (src > INT32_MAX || pb > INT32_MAX) throw std::length_error("update vectors exceed the 32-bit gather index");
-          P.gent[next[P.frow_off[s] + S.relidx[S.rows_off[c] + r]]++] =
-              Task4{static_cast<int32_t>(src), static_cast<int32_t>(pb), nhc, 0};
-        }
-      }
-    // two unused entries past the end: a row's first two entries are read
-    // before its count is known
-    P.gent.push_back(Task4{0, 0, 0, 0});
-    P.gent.push_back(Task4{0, 0, 0, 0});
-  }
-  build_dag_schedule(P);
+  build_solve_schedule(P, SolveTuning{});
   return P;
 }
 

@@ -32,6 +32,18 @@ constexpr int kUpdStepsShift = 20;
 constexpr int kUpdPanelBit = 30;
 // Edge of the extend-add assembly tiles.
 constexpr int kAsmTile = 64;
+// Narrow big fronts: at most kSolveRB columns and at most
+// SolveTuning::narrow_rows rows below the diagonal block. Their forward
+// solve is one task per kNarrowChunk rows of L_21 (each forms y = W^T v
+// itself, so no task waits on another task of the same front) and their
+// backward solve one task per front.
+constexpr int kNarrowChunk = 256;
+
+// Tunable shape of the triangular-solve schedule (it changes which task
+// kinds cover a front, never the factor's layout).
+struct SolveTuning {
+  int32_t narrow_rows = 1024;  // largest nrow of a narrow front (0: none)
+};
 
 struct Task4 {
   int32_t a, b, c, d;
@@ -40,7 +52,7 @@ struct Task4 {
 // Kinds of the dependency-driven solve tasks (Task4.a). Small fronts:
 // (kDagSmall, first index into small_list, count <= kDagGroup, 0); tiles of
 // a big front: (kind, s, block, chunk).
-enum : int32_t { kDagSmall = 0, kDagFx, kDagFu, kDagBr, kDagBx };
+enum : int32_t { kDagSmall = 0, kDagFx, kDagFu, kDagBr, kDagBx, kDagFm, kDagBm };
 constexpr int kDagGroup = 4;
 // Per-supernode entries of the solve schedule: parent, children, column
 // blocks, R blocks, backward R-chunk tasks, backward completions (column
@@ -96,6 +108,16 @@ struct FactorPlan {
   // readers add in a fixed order.
   std::vector<int32_t> fx_ptr, fu_ptr, br_ptr, bx_ptr;      // nlevels+1, in tasks
   std::vector<int32_t> fx_task, fu_task, br_task, bx_task;  // triples
+  // narrow fronts (kNarrowChunk): fm (s, R chunk h, 0) gathers v, forms
+  // y = W^T v and u = v_R - L_21 y on kNarrowChunk rows of R (the h = 0 task
+  // stores y); bm (s, 0, 0) forms x = W (D^{-1} y - L_21^T x_R). Their u is
+  // final (no partial sums) and their y has one partial sum per column.
+  std::vector<int32_t> fm_ptr, bm_ptr;    // nlevels+1, in tasks
+  std::vector<int32_t> fm_task, bm_task;  // triples
+  // the factor's W = Z D pass: (s, column block j, row chunk h) over the
+  // rows at or above the block's last column, every big front
+  std::vector<int32_t> inv_task;  // triples
+  SolveTuning tuning;
   // row gathers of every front: the contributions to row r of front s are
   // entries [grow_ptr[frow_off[s] + r], grow_ptr[frow_off[s] + r + 1]) of
   // gent, in child order: (update-vector entry, first partial sum of L_21 y
@@ -125,6 +147,9 @@ struct FactorPlan {
 FactorPlan build_plan_from_supernodes(Supernodes sn);
 // dag_fwd, dag_bwd and dag_meta from the per-level task lists.
 void build_dag_schedule(FactorPlan& P);
+// The triangular-solve schedule of P's supernodes (task lists, partial-sum
+// offsets, row gathers, dependency schedule) for tuning t; P.inv_task too.
+void build_solve_schedule(FactorPlan& P, const SolveTuning& t);
 
 // perm: new -> old. C: permuted upper CSC with val_map (C entry -> original
 // entry). parent/Lnz: etree of C.
