@@ -281,6 +281,27 @@ int mpir_gmres_ldl_precond_dev_noalloc(mpir_gmres_ldl_t h, double* x);
 /* average ms of `reps` runs measured with HIP events on the handle's stream:
  * what = 0 SpMV, 1 M^{-1} application, 2 numeric factorization */
 int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);
+/* Autotuner of the level-tile schedules (no counterpart in the reference,
+ * whose solver has no device schedule). On a factored L D L^T handle (single
+ * system or batch) it times, with HIP events over `reps` runs (1..100) each:
+ * the factor's update grouping (panel steps whose trailing update is
+ * applied at once: 2, 4, 8) x tile super-block edge (4, 8, 16 tiles), then
+ * the triangular solves' narrow-front row limit (0, 256 .. 4096 rows) x
+ * schedule (MPIR_SCHED_LEVELS / MPIR_SCHED_DAG); it keeps the fastest of
+ * each, recomputes the factor with it, and later analyses of the handle
+ * (same pattern or not) plan with the choice. Solutions stay within the
+ * solver's tolerance; bits may differ from another tuning's, since sums are
+ * taken in another order. choice (5, may be NULL): update grouping,
+ * super-block edge, narrow-front rows, schedule, candidates timed;
+ * ms (4, may be NULL): factor and M^{-1} ms with the choice, then with the
+ * tuning in place before the call. */
+int mpir_gmres_ldl_autotune(mpir_gmres_ldl_t h, int reps, int32_t* choice, double* ms);
+/* Sets a tuning without timing it (the first three entries of autotune's
+ * choice; no counterpart in the reference): update grouping 1..64,
+ * super-block edge 1..64, narrow-front rows >= 0 (0: no narrow fronts). The
+ * handle's plan is re-planned at once; an existing factor stays valid. The
+ * defaults are 4, 8, 1024. */
+int mpir_gmres_ldl_set_tuning(mpir_gmres_ldl_t h, int32_t upd_group, int32_t super_tiles, int32_t narrow_rows);
 /* The log of the last solve in the JSON schema of the reference's VERBOSE
  * build (gmres_ir.hpp:119-124, 305-314, 539-631): max_outer_iterations,
  * max_inner_iterations, tolerance and an "events" array with one

@@ -321,6 +321,22 @@ class GmresLDLIR:
         _check(load().mpir_gmres_ldl_time(self._h, code, int(reps), C.byref(ms)), "time_kernels")
         return ms.value
 
+    def SetTuning(self, upd_group: int = 4, super_tiles: int = 8, narrow_rows: int = 1024) -> None:
+        """A tuning without timing it (see mpir_gmres_ldl_set_tuning)."""
+        _check(load().mpir_gmres_ldl_set_tuning(self._h, int(upd_group), int(super_tiles), int(narrow_rows)),
+               "SetTuning")
+
+    def Autotune(self, reps: int = 3) -> dict:
+        """Times the factor's update grouping / super-block edge and the
+        solves' narrow-front limit / schedule on this factor, keeps the
+        fastest and refactors with it (see mpir_gmres_ldl_autotune)."""
+        choice = np.zeros(5, dtype=np.int32)
+        ms = np.zeros(4)
+        _check(load().mpir_gmres_ldl_autotune(self._h, int(reps), _ptr(choice), _ptr(ms)), "Autotune")
+        return {"upd_group": int(choice[0]), "super_tiles": int(choice[1]), "narrow_rows": int(choice[2]),
+                "schedule": int(choice[3]), "candidates": int(choice[4]), "factor_ms": float(ms[0]),
+                "precond_ms": float(ms[1]), "factor_ms_before": float(ms[2]), "precond_ms_before": float(ms[3])}
+
     def factor_dev(self, ax_ptr: int) -> None:
         _check(load().mpir_gmres_ldl_factor_dev(self._h, C.c_void_p(ax_ptr)), "factor_dev")
 
@@ -445,6 +461,13 @@ class BatchLDLIR:
 
     def time_kernels(self, what: str, reps: int = 5) -> float:
         return self._s.time_kernels(what, reps)
+
+    def Autotune(self, reps: int = 3) -> dict:
+        """The level-tile autotuner on the joined batch (GmresLDLIR.Autotune)."""
+        return self._s.Autotune(reps)
+
+    def SetTuning(self, upd_group: int = 4, super_tiles: int = 8, narrow_rows: int = 1024) -> None:
+        self._s.SetTuning(upd_group, super_tiles, narrow_rows)
 
     @property
     def sizes(self) -> list:

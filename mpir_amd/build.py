@@ -23,6 +23,8 @@ BUILD = ROOT / "_build" / "mpir_hip"
 LIB = PKG / "libmpir_hip.so"
 DROPIN_SRC = ROOT / "tests" / "cpp" / "dropin.cpp"
 DROPIN_BIN = ROOT / "tests" / "cpp" / "dropin_test"
+PLANCHECK_SRC = ROOT / "tests" / "cpp" / "plan_check.cpp"
+PLANCHECK_BIN = ROOT / "tests" / "cpp" / "plan_check"
 DRIVER_SRC = CSRC / "driver" / "mtx_solve.cpp"
 DRIVER_BIN = PKG / "mtx_solve"
 
@@ -103,6 +105,12 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     # the C++ drop-in test program (a reference-style caller of include/mpir/*.hpp)
     _run([CXX, "-std=c++20", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(DROPIN_SRC), f"-L{PKG}", "-lmpir_hip",
           "-Wl,-rpath,$ORIGIN/../../mpir_amd", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(DROPIN_BIN)])
+    # host-only checker of the factor and solve schedules (linked against the
+    # host objects: it needs no device and no exported symbol)
+    plan_objs = [str(_obj(CSRC / f)) for f in ("plan.cpp", "host/symbolic.cpp", "host/ordering.cpp",
+                                                 "host/ordering_nd.cpp", "host/generators.cpp")]
+    _run([CXX, *[f for f in CXX_FLAGS if f != "-fPIC"], f"-I{CSRC}", f"-I{CSRC / 'host'}", str(PLANCHECK_SRC),
+          *plan_objs, "-o", str(PLANCHECK_BIN)])
     # the Matrix Market driver (examples/example.cpp of the reference)
     _run([CXX, "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(DRIVER_SRC), f"-L{PKG}", "-lmpir_hip",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-o", str(DRIVER_BIN)])

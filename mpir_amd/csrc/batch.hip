@@ -326,6 +326,7 @@ void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntr
   BatchSystem* B = batch_.get();
   try {
     combine_patterns(raw, *B, *plan_);
+    apply_tuning();
   } catch (const std::exception& e) {
     batch_.reset();
     throw Error(kErrInvalid, e.what());

@@ -353,7 +353,10 @@ void merge_plans(const std::vector<const FactorPlan*>& P, const std::vector<int3
     const std::vector<int32_t>& v = P[s]->inv_task;
     for (size_t t = 0; t < v.size(); t += 3) M.inv_task.insert(M.inv_task.end(), {b[s].sn + v[t], v[t + 1], v[t + 2]});
   }
+  // the systems' plans come from the pattern cache, all with the default
+  // tunings; the handle re-plans the joined forest for its own
   M.tuning = S > 0 ? P[0]->tuning : SolveTuning{};
+  M.ftuning = S > 0 ? P[0]->ftuning : FactorTuning{};
   // assembly tasks with their child ranges
   M.asm_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
   M.asm_child_ptr.assign(1, 0);

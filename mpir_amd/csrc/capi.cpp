@@ -991,6 +991,33 @@ int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms) 
     return 0;
   });
 }
+int mpir_gmres_ldl_set_tuning(mpir_gmres_ldl_t h, int32_t upd_group, int32_t super_tiles, int32_t narrow_rows) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->set_tuning(FactorTuning{upd_group, super_tiles}, SolveTuning{narrow_rows});
+    return 0;
+  });
+}
+int mpir_gmres_ldl_autotune(mpir_gmres_ldl_t h, int reps, int32_t* choice, double* ms) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    const AutotuneResult r = h->s->autotune(reps);
+    if (choice) {
+      choice[0] = r.factor.upd_group;
+      choice[1] = r.factor.super_tiles;
+      choice[2] = r.solve.narrow_rows;
+      choice[3] = r.schedule;
+      choice[4] = r.candidates;
+    }
+    if (ms) {
+      ms[0] = r.factor_ms;
+      ms[1] = r.precond_ms;
+      ms[2] = r.factor_ms_before;
+      ms[3] = r.precond_ms_before;
+    }
+    return 0;
+  });
+}
 void* mpir_gmres_ldl_stream(mpir_gmres_ldl_t h) { return h && h->s ? static_cast<void*>(h->s->stream()) : nullptr; }
 
 int mpir_spmv_upper_f64(int32_t n, const int32_t* Ap, const int32_t* Ai, const double* Ax, const double* x, double* y) {

@@ -582,6 +582,7 @@ void GmresLdl::analyze(int32_t n, const int32_t* Ap, const int32_t* Ai) {
   true_nnz_l_ = an.nnz_l;
   true_flops_ = an.flops;
   hipStream_t st = stream_;
+  apply_tuning();
   upload_plan();
 
   // FP64 CSR of the permuted, scaled matrix for the SpMV: entry -> original
@@ -804,7 +805,7 @@ void GmresLdl::load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, 
     nnz_ = 0;
     perm_.resize(n);
     std::iota(perm_.begin(), perm_.end(), 0);
-    plan_ = std::make_unique<FactorPlan>(build_plan_from_supernodes(std::move(S)));
+    plan_ = std::make_unique<FactorPlan>(build_plan_from_supernodes(std::move(S), ftune_, stune_));
     upload_plan();
     alloc_numeric();
     loaded_Lp_.assign(Lp, Lp + n + 1);
@@ -1317,6 +1318,100 @@ double GmresLdl::time_kernels(int what, int reps) {
   (void)hipEventDestroy(e1);
   sync_check();
   return static_cast<double>(ms) / reps;
+}
+
+void GmresLdl::apply_tuning() {
+  FactorPlan& P = *plan_;
+  if (!same_tuning(P.ftuning, ftune_)) build_factor_steps(P, ftune_);
+  if (!same_tuning(P.tuning, stune_)) build_solve_schedule(P, stune_);
+}
+
+void GmresLdl::set_tuning(const FactorTuning& f, const SolveTuning& s) {
+  MPIR_CHECK(f.upd_group >= 1 && f.upd_group <= 64 && f.super_tiles >= 1 && f.super_tiles <= 64 && s.narrow_rows >= 0,
+             "tuning out of range");
+  ftune_ = f;
+  stune_ = s;
+  if (plan_ && analyzed_) {
+    const int slots = std::max(1, solve_slots_);
+    apply_tuning();
+    upload_plan();
+    set_solve_slots(slots);
+  }
+}
+
+AutotuneResult GmresLdl::autotune(int reps) {
+  if (!factored_) throw Error(kErrState, "autotune needs a factorization");
+  MPIR_CHECK(precond_ == kPrecLdl && !precond_only_, "autotune needs an L D L^T factor of loaded values");
+  MPIR_CHECK(reps >= 1 && reps <= 100, "reps must be in 1 .. 100");
+  FactorPlan& P = *plan_;
+  AutotuneResult res;
+  const int slots = std::max(1, solve_slots_);
+  auto install = [&]() {
+    upload_plan();
+    set_solve_slots(slots);
+  };
+  // the factor's step schedule: update grouping x super-block edge, the
+  // tuning in place first
+  std::vector<FactorTuning> fc{P.ftuning};
+  for (int g : {2, 4, 8})
+    for (int t : {4, 8, 16}) {
+      const FactorTuning c{g, t};
+      if (!same_tuning(c, P.ftuning)) fc.push_back(c);
+    }
+  double best = 0.0;
+  FactorTuning fbest = P.ftuning;
+  for (size_t i = 0; i < fc.size(); ++i) {
+    if (i > 0) {
+      build_factor_steps(P, fc[i]);
+      install();
+    }
+    const double ms = time_kernels(2, reps);
+    if (i == 0) res.factor_ms_before = ms;
+    if (i == 0 || ms < best) {
+      best = ms;
+      fbest = fc[i];
+    }
+  }
+  res.factor = fbest;
+  res.factor_ms = best;
+  build_factor_steps(P, fbest);
+  // the solves: narrow-front row limit x schedule
+  const SolveTuning s0 = P.tuning;
+  const int sched0 = schedule_;
+  struct SC {
+    SolveTuning t;
+    int sched;
+  };
+  std::vector<SC> sc{{s0, sched0}};
+  for (int rows : {0, 256, 512, 1024, 2048, 4096})
+    for (int sched : {kScheduleDag, kScheduleLevels}) {
+      const SolveTuning t{rows};
+      if (!(same_tuning(t, s0) && sched == sched0)) sc.push_back({t, sched});
+    }
+  SC sbest = sc[0];
+  for (size_t i = 0; i < sc.size(); ++i) {
+    build_solve_schedule(P, sc[i].t);
+    schedule_ = sc[i].sched;
+    install();
+    const double ms = time_kernels(1, reps);
+    if (i == 0) res.precond_ms_before = ms;
+    if (i == 0 || ms < best) {
+      best = ms;
+      sbest = sc[i];
+    }
+  }
+  res.solve = sbest.t;
+  res.schedule = sbest.sched;
+  res.precond_ms = best;
+  res.candidates = static_cast<int32_t>(fc.size() + sc.size());
+  // keep the choice, and the factor computed with it
+  ftune_ = fbest;
+  stune_ = sbest.t;
+  schedule_ = sbest.sched;
+  build_solve_schedule(P, stune_);
+  install();
+  time_kernels(2, 1);
+  return res;
 }
 
 template <class TF>

@@ -63,15 +63,14 @@ class PoolAlloc {
 // Trailing-update tiles of front s: columns c0 in [c_lo, c_hi) and rows r0
 // in [c0, r_hi) (lower part, `lower`) or in [0, r_hi) (the identity rows
 // below the front, listed at r_off + r0), 64 x 64 each. They are listed in
-// super-blocks of kSuperTiles x kSuperTiles tiles, column block by column
+// super-blocks of st x st tiles (8 by default), column block by column
 // block: an XCD takes a contiguous run of a launch's list (see xcd_task in
 // the factor kernels), and tiles of one super-block share their row
 // operands as well as their column operands, about 1 MB for a rank-256
-// update, which stays in the XCD's L2 while the super-block runs.
-constexpr int32_t kSuperTiles = 8;
+// update at st = 8, which stays in the XCD's L2 while the super-block runs.
 void push_tiles(std::vector<Task4>& out, int32_t s, int32_t c_lo, int32_t c_hi, int32_t r_hi, int32_t r_off,
-                bool lower, int32_t code) {
-  const int32_t W = kSuperTiles * kTile;
+                bool lower, int32_t code, int32_t st) {
+  const int32_t W = st * kTile;
   for (int32_t cb = c_lo; cb < c_hi; cb += W) {
     const int32_t ce = std::min(c_hi, cb + W);
     for (int32_t rb = lower ? cb : 0; rb < r_hi; rb += W) {
@@ -251,7 +250,77 @@ void build_solve_schedule(FactorPlan& P, const SolveTuning& t) {
   build_dag_schedule(P);
 }
 
-FactorPlan build_plan_from_supernodes(Supernodes sn) {
+void build_factor_steps(FactorPlan& P, const FactorTuning& t) {
+  if (t.upd_group < 1 || t.upd_group > 64 || t.super_tiles < 1 || t.super_tiles > 64)
+    throw std::invalid_argument("factor tuning out of range");
+  const Supernodes& S = P.sn;
+  P.ftuning = t;
+  P.level_step_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
+  P.pan_ptr.assign(1, 0);
+  P.upd_ptr.assign(1, 0);
+  P.pan_aug_ptr.clear();
+  P.upd_aug_ptr.clear();
+  P.pan_task.clear();
+  P.upd_task.clear();
+  for (int32_t l = 0; l < S.nlevels; ++l) {
+    int32_t max_steps = 0;
+    for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
+      const int32_t s = P.big_list[q];
+      max_steps = std::max(max_steps, (S.first[s + 1] - S.first[s] + kPanelNB - 1) / kPanelNB);
+    }
+    for (int32_t k = 0; k < max_steps; ++k) {
+      // real rows first, then the identity rows below R_s (aug rows), so a
+      // run that only rebuilds the inverses can launch the tail of a step
+      for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+          P.pan_aug_ptr.push_back(static_cast<int32_t>(P.pan_task.size()));
+          P.upd_aug_ptr.push_back(static_cast<int32_t>(P.upd_task.size()));
+        }
+        for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
+          const int32_t s = P.big_list[q];
+          const int32_t nc = S.first[s + 1] - S.first[s];
+          const int32_t m = nc + S.nrow[s];
+          const int32_t k0 = k * kPanelNB;
+          if (k0 >= nc) continue;
+          const int32_t nbk = std::min(kPanelNB, nc - k0);
+          const int32_t t0 = k0 + nbk;
+          // the step's group of t.upd_group steps ends at column gend; its
+          // trailing columns up to gend take this step's rank-64 update,
+          // those right of it the whole group's update at its last step
+          const int32_t nk = (nc + kPanelNB - 1) / kPanelNB;
+          const int32_t g0 = k / t.upd_group * t.upd_group;
+          const int32_t glast = std::min(nk - 1, g0 + t.upd_group - 1);
+          const int32_t gend = std::min(nc, (glast + 1) * kPanelNB);
+          // In a front's last group gend = nc: the last near tile may reach
+          // past nc, into the columns of the update matrix that the group's
+          // far tiles cover, so its near tiles keep to the panel columns.
+          const bool last_group = glast == nk - 1;
+          const int32_t near = k | (1 << kUpdStepsShift) | ((last_group ? 1 : 0) << kUpdPanelBit);
+          const int32_t far = k | ((k - g0 + 1) << kUpdStepsShift);
+          const bool group_end = k == glast;
+          if (pass == 0) {
+            // the diagonal block's own task, then rows below it in chunks of 64
+            P.pan_task.push_back({s, k0, t0, k});
+            for (int32_t r0 = t0; r0 < m; r0 += kPanelRows) P.pan_task.push_back({s, r0, std::min(m, r0 + kPanelRows), k});
+            push_tiles(P.upd_task, s, t0, gend, m, 0, true, near, t.super_tiles);
+            if (group_end) push_tiles(P.upd_task, s, gend, m, m, 0, true, far, t.super_tiles);
+          } else {
+            // aug rows i < t0 are the only ones nonzero in this step's columns
+            for (int32_t r0 = 0; r0 < t0; r0 += kPanelRows)
+              P.pan_task.push_back({s, m + r0, m + std::min(t0, r0 + kPanelRows), k});
+            push_tiles(P.upd_task, s, t0, gend, t0, m, false, near, t.super_tiles);
+            if (group_end) push_tiles(P.upd_task, s, gend, nc, t0, m, false, far, t.super_tiles);
+          }
+        }
+      }
+      P.pan_ptr.push_back(static_cast<int32_t>(P.pan_task.size()));
+      P.upd_ptr.push_back(static_cast<int32_t>(P.upd_task.size()));
+    }
+    P.level_step_ptr[l + 1] = P.level_step_ptr[l] + max_steps;
+  }
+}
+
+FactorPlan build_plan_from_supernodes(Supernodes sn, const FactorTuning& ft, const SolveTuning& st) {
   FactorPlan P;
   P.sn = std::move(sn);
   Supernodes& S = P.sn;
@@ -294,13 +363,7 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
   P.small_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   P.big_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
   P.asm_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
-  P.level_step_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
-  P.pan_ptr.assign(1, 0);
-  P.upd_ptr.assign(1, 0);
-  P.pan_aug_ptr.clear();
-  P.upd_aug_ptr.clear();
   for (int32_t l = 0; l < S.nlevels; ++l) {
-    int32_t max_steps = 0;
     for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
       const int32_t s = S.level_sn[q];
       const int32_t nc = S.first[s + 1] - S.first[s];
@@ -310,7 +373,6 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
         continue;
       }
       P.big_list.push_back(s);
-      max_steps = std::max(max_steps, (nc + kPanelNB - 1) / kPanelNB);
       // 64 x 64 tiles of the lower part; tiles made only of panel columns
       // are kept below only if a child's update lands in them
       for (int32_t c0 = 0; c0 < m; c0 += kAsmTile) {
@@ -321,56 +383,8 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     P.small_ptr[l + 1] = static_cast<int32_t>(P.small_list.size());
     P.big_ptr[l + 1] = static_cast<int32_t>(P.big_list.size());
     P.asm_ptr[l + 1] = static_cast<int32_t>(P.asm_task.size());
-    for (int32_t k = 0; k < max_steps; ++k) {
-      // real rows first, then the identity rows below R_s (aug rows), so a
-      // run that only rebuilds the inverses can launch the tail of a step
-      for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) {
-          P.pan_aug_ptr.push_back(static_cast<int32_t>(P.pan_task.size()));
-          P.upd_aug_ptr.push_back(static_cast<int32_t>(P.upd_task.size()));
-        }
-        for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
-          const int32_t s = P.big_list[q];
-          const int32_t nc = S.first[s + 1] - S.first[s];
-          const int32_t m = nc + S.nrow[s];
-          const int32_t k0 = k * kPanelNB;
-          if (k0 >= nc) continue;
-          const int32_t nbk = std::min(kPanelNB, nc - k0);
-          const int32_t t0 = k0 + nbk;
-          // the step's group of kUpdGroup steps ends at column gend; its
-          // trailing columns up to gend take this step's rank-64 update,
-          // those right of it the whole group's update at its last step
-          const int32_t nk = (nc + kPanelNB - 1) / kPanelNB;
-          const int32_t g0 = k / kUpdGroup * kUpdGroup;
-          const int32_t glast = std::min(nk - 1, g0 + kUpdGroup - 1);
-          const int32_t gend = std::min(nc, (glast + 1) * kPanelNB);
-          // In a front's last group gend = nc: the last near tile may reach
-          // past nc, into the columns of the update matrix that the group's
-          // far tiles cover, so its near tiles keep to the panel columns.
-          const bool last_group = glast == nk - 1;
-          const int32_t near = k | (1 << kUpdStepsShift) | ((last_group ? 1 : 0) << kUpdPanelBit);
-          const int32_t far = k | ((k - g0 + 1) << kUpdStepsShift);
-          const bool group_end = k == glast;
-          if (pass == 0) {
-            // the diagonal block's own task, then rows below it in chunks of 64
-            P.pan_task.push_back({s, k0, t0, k});
-            for (int32_t r0 = t0; r0 < m; r0 += kPanelRows) P.pan_task.push_back({s, r0, std::min(m, r0 + kPanelRows), k});
-            push_tiles(P.upd_task, s, t0, gend, m, 0, true, near);
-            if (group_end) push_tiles(P.upd_task, s, gend, m, m, 0, true, far);
-          } else {
-            // aug rows i < t0 are the only ones nonzero in this step's columns
-            for (int32_t r0 = 0; r0 < t0; r0 += kPanelRows)
-              P.pan_task.push_back({s, m + r0, m + std::min(t0, r0 + kPanelRows), k});
-            push_tiles(P.upd_task, s, t0, gend, t0, m, false, near);
-            if (group_end) push_tiles(P.upd_task, s, gend, nc, t0, m, false, far);
-          }
-        }
-      }
-      P.pan_ptr.push_back(static_cast<int32_t>(P.pan_task.size()));
-      P.upd_ptr.push_back(static_cast<int32_t>(P.upd_task.size()));
-    }
-    P.level_step_ptr[l + 1] = P.level_step_ptr[l] + max_steps;
   }
+  build_factor_steps(P, ft);
   P.nsmall = static_cast<int64_t>(P.small_list.size());
   P.nbig = static_cast<int64_t>(P.big_list.size());
 
@@ -410,7 +424,7 @@ FactorPlan build_plan_from_supernodes(Supernodes sn) {
     P.asm_ptr.swap(kept_level);
   }
 
-  build_solve_schedule(P, SolveTuning{});
+  build_solve_schedule(P, st);
   return P;
 }
 

@@ -21,10 +21,10 @@ constexpr int kXcRows = 256;
 constexpr int kGemvCols = 256;
 // Tile edge of the trailing-update tasks.
 constexpr int kTile = 64;
-// Panel steps per update group: a step updates the trailing columns of its
-// own group (rank 64); the columns right of the group are updated once, at
-// the group's last step, with all its steps (rank kUpdGroup * 64).
-constexpr int kUpdGroup = 4;
+// Panel steps per update group (FactorTuning::upd_group): a step updates
+// the trailing columns of its own group (rank 64); the columns right of the
+// group are updated once, at the group's last step, with all its steps
+// (rank upd_group * 64).
 // Update task field d: the last step k, the number of steps ending at k
 // that the task applies, and a flag restricting the task to panel columns
 // (c < nc): d = k | (steps << kUpdStepsShift) | (panel_only << kUpdPanelBit).
@@ -44,6 +44,16 @@ constexpr int kNarrowChunk = 256;
 struct SolveTuning {
   int32_t narrow_rows = 1024;  // largest nrow of a narrow front (0: none)
 };
+// Tunable shape of the factor's step schedule (the trailing-update tasks;
+// the storage layout does not change).
+struct FactorTuning {
+  int32_t upd_group = 4;    // panel steps per update group, 1 .. 64
+  int32_t super_tiles = 8;  // edge of the tile super-blocks, in tiles, 1 .. 64
+};
+inline bool same_tuning(const SolveTuning& a, const SolveTuning& b) { return a.narrow_rows == b.narrow_rows; }
+inline bool same_tuning(const FactorTuning& a, const FactorTuning& b) {
+  return a.upd_group == b.upd_group && a.super_tiles == b.super_tiles;
+}
 
 struct Task4 {
   int32_t a, b, c, d;
@@ -118,6 +128,7 @@ struct FactorPlan {
   // rows at or above the block's last column, every big front
   std::vector<int32_t> inv_task;  // triples
   SolveTuning tuning;
+  FactorTuning ftuning;
   // row gathers of every front: the contributions to row r of front s are
   // entries [grow_ptr[frow_off[s] + r], grow_ptr[frow_off[s] + r + 1]) of
   // gent, in child order: (update-vector entry, first partial sum of L_21 y
@@ -144,7 +155,10 @@ struct FactorPlan {
 };
 
 // Schedule and storage layout of a supernodal partition (no value map).
-FactorPlan build_plan_from_supernodes(Supernodes sn);
+FactorPlan build_plan_from_supernodes(Supernodes sn, const FactorTuning& ft = {}, const SolveTuning& st = {});
+// The factor's panel and update tasks per (level, step) of P's big fronts
+// for tuning t (level_step_ptr, pan_*, upd_*).
+void build_factor_steps(FactorPlan& P, const FactorTuning& t);
 // dag_fwd, dag_bwd and dag_meta from the per-level task lists.
 void build_dag_schedule(FactorPlan& P);
 // The triangular-solve schedule of P's supernodes (task lists, partial-sum

@@ -105,6 +105,17 @@ struct LuirResult {
   double time_solve = 0.0, time_ref = 0.0;  // ms
 };
 
+// What GmresLdl::autotune chose and measured (ms: HIP events on the
+// handle's stream, averages over the requested runs).
+struct AutotuneResult {
+  FactorTuning factor;
+  SolveTuning solve;
+  int schedule = 0;  // kScheduleLevels or kScheduleDag
+  double factor_ms = 0.0, precond_ms = 0.0;                  // chosen
+  double factor_ms_before = 0.0, precond_ms_before = 0.0;    // the tuning in place before the call
+  int32_t candidates = 0;
+};
+
 class GmresLdl {
  public:
   GmresLdl(int uf, int uw, int ur);
@@ -196,6 +207,16 @@ class GmresLdl {
   // handle's stream, measured with HIP events: 0 SpMV, 1 M^{-1} application,
   // 2 numeric factorization (values already loaded by factor()).
   double time_kernels(int what, int reps);
+  // Times the factor's update grouping and super-block edge and the
+  // triangular solves' narrow-front limit and schedule on this handle's
+  // factor (reps runs per candidate), keeps the fastest of each and leaves
+  // the factor recomputed with it. The choice stays with the handle: later
+  // analyses (analyze, analyze_batch) build their plans with it.
+  AutotuneResult autotune(int reps);
+  // Sets the tunings that autotune would choose; the plan in place is
+  // re-planned with them (an existing factor stays valid: the storage layout
+  // does not change).
+  void set_tuning(const FactorTuning& f, const SolveTuning& s);
   // Load a factor given in QDLDL's CSC layout (host arrays, Lx/Dinv in the
   // handle's factor precision) for use as M^{-1} only.
   void load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, const void* Lx, const void* Dinv);
@@ -232,6 +253,9 @@ class GmresLdl {
   double infnrm(const double* v);
   void release_numeric();
   void upload_plan();
+  // rebuilds plan_'s step and solve schedules for ftune_ / stune_ where they
+  // differ from the plan's
+  void apply_tuning();
   template <class T>
   FactorArgs<T> factor_args() {
     return FactorArgs<T>{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(),
@@ -259,6 +283,8 @@ class GmresLdl {
   int ordering_ = kOrderAmd;
   int scaling_ = kScaleDiag;
   int schedule_ = kScheduleDag;
+  FactorTuning ftune_;
+  SolveTuning stune_;
   bool relax_ = true;     // relaxed supernodes (AMD / ND orderings only)
   bool relaxed_ = false;  // the current plan uses them
   std::vector<int32_t> Cp_, Ci_, parent_, Lnz_;  // pattern of P K P^T, etree

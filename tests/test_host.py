@@ -133,6 +133,22 @@ def test_cpp_dropin_headers_without_a_device():
     assert "MPIR_ERR_NODEVICE" in proc.stdout
 
 
+PLAN_CHECK = Path(__file__).resolve().parent / "cpp" / "plan_check"
+
+
+def test_factor_and_solve_schedules_for_every_tuning():
+    # tests/cpp/plan_check.cpp: for several update groupings, super-block
+    # edges and narrow-front limits, every trailing-update entry of every
+    # panel step is applied exactly once and in time, the solve tasks cover
+    # each front once, the dependency lists are ordered; planted mistakes
+    # (a dropped or doubled update task, a dropped narrow task) are caught
+    import subprocess
+    assert PLAN_CHECK.exists(), "tests/cpp/plan_check is built by mpir_amd.build"
+    proc = subprocess.run([str(PLAN_CHECK)], capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert "planted mistakes caught" in proc.stdout and "plan_check ok" in proc.stdout
+
+
 MTX = Path(__file__).resolve().parent / "golden" / "kkt_cfg0.mtx"
 
 
