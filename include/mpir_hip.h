@@ -279,7 +279,9 @@ int mpir_gmres_ldl_precond_multi_dev(mpir_gmres_ldl_t h, double* x, int32_t nrhs
  * them would race. Single-system L D L^T handles only. */
 int mpir_gmres_ldl_precond_dev_noalloc(mpir_gmres_ldl_t h, double* x);
 /* average ms of `reps` runs measured with HIP events on the handle's stream:
- * what = 0 SpMV, 1 M^{-1} application, 2 numeric factorization */
+ * what = 0 SpMV, 1 M^{-1} application, 2 numeric factorization, 3 M^{-1}
+ * applied at once to as many vectors as the last batch solve had
+ * right-hand sides (up to 8; 1 for a single system) */
 int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);
 /* Autotuner of the level-tile schedules (no counterpart in the reference,
  * whose solver has no device schedule). On a factored L D L^T handle (single
@@ -287,7 +289,9 @@ int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);
  * the factor's update grouping (panel steps whose trailing update is
  * applied at once: 2, 4, 8) x tile super-block edge (4, 8, 16 tiles), then
  * the triangular solves' narrow-front row limit (0, 256 .. 4096 rows) x
- * schedule (MPIR_SCHED_LEVELS / MPIR_SCHED_DAG); it keeps the fastest of
+ * schedule (MPIR_SCHED_LEVELS / MPIR_SCHED_DAG), M^{-1} applied to as many
+ * vectors at once as the handle's last batch solve (1 for a single
+ * system); it keeps the fastest of
  * each, recomputes the factor with it, and later analyses of the handle
  * (same pattern or not) plan with the choice. Solutions stay within the
  * solver's tolerance; bits may differ from another tuning's, since sums are
@@ -341,6 +345,18 @@ int mpir_batch_solve_dev(mpir_gmres_ldl_t h, const int32_t* nrhs, const double* 
  * the instance stopped. Any output pointer may be NULL. */
 int mpir_batch_stats(mpir_gmres_ldl_t h, int32_t* max_rhs, int32_t* positive_pivots, int32_t* ir_iterations,
                      int32_t* gmres_iterations, int32_t* stop_reason, double* final_rel_residual, double* done_ms);
+/* The batched solve runs every refinement iteration and Arnoldi step on the
+ * device (the Hessenberg columns, Givens rotations, back-substitutions and
+ * stopping rules of all instances in kernels), as one fixed launch sequence
+ * up to the iteration limits in which each kernel returns at once when no
+ * instance takes part in its iteration or step. When a solve repeats the
+ * shapes, tolerance, plan and buffers of the solve before it, the sequence
+ * is captured as a HIP graph and each later solve of that kind is one graph
+ * launch (no counterpart in the reference). enable: 1 on (default), 0 off
+ * (every solve launched kernel by kernel), -1 leave as is. counts (3, may
+ * be NULL) of the last solve: graph nodes replayed (0: launched kernel by
+ * kernel), Arnoldi steps and M^{-1} sweeps of the batch. */
+int mpir_batch_graph(mpir_gmres_ldl_t h, int enable, int32_t* counts);
 /* Plans the calling thread's QDLDL_solve engines have built from factor
  * patterns. A mpir_qdldl_solve_* call whose (Lp, Li) equals the previous
  * call's on the same thread (same factor and x precision) reuses its plan

@@ -316,7 +316,7 @@ class GmresLDLIR:
 
     def time_kernels(self, what: str, reps: int = 10) -> float:
         """Average ms of one kernel group ('spmv', 'precond', 'factor') via HIP events."""
-        code = {"spmv": 0, "precond": 1, "factor": 2}[what]
+        code = {"spmv": 0, "precond": 1, "factor": 2, "precond_multi": 3}[what]
         ms = C.c_double()
         _check(load().mpir_gmres_ldl_time(self._h, code, int(reps), C.byref(ms)), "time_kernels")
         return ms.value
@@ -455,9 +455,17 @@ class BatchLDLIR:
         _check(lib.mpir_batch_stats(self._s.handle, C.byref(mr), _ptr(pos), _ptr(ir), _ptr(gm), _ptr(stop), _ptr(rel),
                                     _ptr(done)), "batch stats")
         sh = (S, R)
+        counts = np.zeros(3, np.int32)
+        _check(lib.mpir_batch_graph(self._s.handle, -1, _ptr(counts)), "batch graph")
         return {"max_rhs": mr.value, "positive_pivots": pos, "ir_iterations": ir.reshape(sh),
                 "gmres_iterations": gm.reshape(sh), "stop_reason": stop.reshape(sh),
-                "final_rel_residual": rel.reshape(sh), "done_ms": done.reshape(sh), "totals": self._s.stats()}
+                "final_rel_residual": rel.reshape(sh), "done_ms": done.reshape(sh), "graph_nodes": int(counts[0]),
+                "arnoldi_steps": int(counts[1]), "precond_sweeps": int(counts[2]), "totals": self._s.stats()}
+
+    def SetGraph(self, enable: bool) -> None:
+        """Replay repeated solves as one captured HIP graph (default) or launch
+        every kernel of every solve (see mpir_batch_graph)."""
+        _check(load().mpir_batch_graph(self._s.handle, 1 if enable else 0, None), "SetGraph")
 
     def time_kernels(self, what: str, reps: int = 5) -> float:
         return self._s.time_kernels(what, reps)

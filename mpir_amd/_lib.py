@@ -109,6 +109,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_time": (C.c_int, [P, C.c_int, C.c_int, P]),
     "mpir_gmres_ldl_autotune": (C.c_int, [P, C.c_int, P, P]),
     "mpir_gmres_ldl_set_tuning": (C.c_int, [P, i32, i32, i32]),
+    "mpir_batch_graph": (C.c_int, [P, C.c_int, P]),
     "mpir_spmv_upper_f64": (C.c_int, [i32, P, P, P, P, P]),
     "mpir_residual_upper_f64": (C.c_int, [i32, P, P, P, P, P, P]),
     "mpir_gen_kkt": (C.c_int, [C.c_int, C.c_int, f64, i64, C.c_int, C.c_int, C.c_int, f64, u64,

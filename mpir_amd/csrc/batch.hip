@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -45,6 +46,11 @@ struct Seg {
   const int32_t* ch_len;      // nch
   const int32_t* sys_ch;      // S+1: chunks of each system
 };
+
+// every kernel of a solve takes the flag of the work it belongs to (a
+// refinement iteration, a GMRES cycle, an Arnoldi step) and returns at once
+// when no instance takes part in it (flag zero); null: always runs
+__device__ __forceinline__ bool gated(const int* gate) { return gate != nullptr && *gate == 0; }
 
 __device__ __forceinline__ double bsum(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
@@ -75,8 +81,10 @@ __device__ __forceinline__ double bmax(double v, double* sh) {
 // part[r * nch + ch]; op 1: the 2-norm as a scaled sum of squares, the pair
 // (scale, ssq) in part[2 (r * nch + ch)] and the entry after it.
 __global__ void __launch_bounds__(kBT) k_bred_partial(Seg g, const double* __restrict__ x,
-                                                      const double* __restrict__ y, int op, double* __restrict__ part) {
+                                                      const double* __restrict__ y, int op, double* __restrict__ part,
+                                                      const int* __restrict__ gate) {
   __shared__ double sh[2 * kBT / 64];
+  if (gated(gate)) return;
   const int ch = blockIdx.x, r = blockIdx.y;
   const int64_t base = static_cast<int64_t>(r) * g.N + g.ch_start[ch];
   const int len = g.ch_len[ch];
@@ -108,7 +116,9 @@ __global__ void __launch_bounds__(kBT) k_bred_partial(Seg g, const double* __res
 }
 
 // out[r * S + s]: the chunks of system s in order (op 1: the 2-norm)
-__global__ void k_bred_final(Seg g, const double* __restrict__ part, int op, double* __restrict__ out) {
+__global__ void k_bred_final(Seg g, const double* __restrict__ part, int op, double* __restrict__ out,
+                             const int* __restrict__ gate) {
+  if (gated(gate)) return;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= g.R * g.S) return;
   const int r = q / g.S, s = q % g.S;
@@ -128,8 +138,10 @@ __global__ void k_bred_final(Seg g, const double* __restrict__ part, int op, dou
 // Partial dots of w with basis vectors 0..k1-1 (stride ldv):
 // part[(j * R + r) * nch + ch]
 __global__ void __launch_bounds__(kBT) k_bmdot_partial(Seg g, const double* __restrict__ V, int64_t ldv, int k1,
-                                                       const double* __restrict__ w, double* __restrict__ part) {
+                                                       const double* __restrict__ w, double* __restrict__ part,
+                                                       const int* __restrict__ gate) {
   __shared__ double sh[kBT / 64];
+  if (gated(gate)) return;
   const int ch = blockIdx.x, r = blockIdx.y;
   const int64_t base = static_cast<int64_t>(r) * g.N + g.ch_start[ch];
   const int len = g.ch_len[ch];
@@ -153,7 +165,9 @@ __global__ void __launch_bounds__(kBT) k_bmdot_partial(Seg g, const double* __re
 }
 
 // h[j * I + q] (+)= chunk sums of instance q = r * S + s, I = R * S
-__global__ void k_bmdot_final(Seg g, const double* __restrict__ part, int k1, double* __restrict__ h, int accumulate) {
+__global__ void k_bmdot_final(Seg g, const double* __restrict__ part, int k1, double* __restrict__ h, int accumulate,
+                              const int* __restrict__ gate) {
+  if (gated(gate)) return;
   const int I = g.R * g.S;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= k1 * I) return;
@@ -173,7 +187,8 @@ __device__ __forceinline__ int instance_of(const Seg& g, int64_t t, int64_t* i_o
 
 // w -= sum_j V_j h_j with the instance's h
 __global__ void k_bsub_basis(Seg g, const double* __restrict__ V, int64_t ldv, int k1, const double* __restrict__ h,
-                             double* __restrict__ w) {
+                             double* __restrict__ w, const int* __restrict__ gate) {
+  if (gated(gate)) return;
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (t >= static_cast<int64_t>(g.R) * g.N) return;
   int64_t i;
@@ -185,7 +200,9 @@ __global__ void k_bsub_basis(Seg g, const double* __restrict__ V, int64_t ldv, i
 }
 
 // out = x * f[q]
-__global__ void k_bscale(Seg g, const double* __restrict__ x, const double* __restrict__ f, double* __restrict__ out) {
+__global__ void k_bscale(Seg g, const double* __restrict__ x, const double* __restrict__ f, double* __restrict__ out,
+                         const int* __restrict__ gate) {
+  if (gated(gate)) return;
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (t >= static_cast<int64_t>(g.R) * g.N) return;
   int64_t i;
@@ -194,7 +211,9 @@ __global__ void k_bscale(Seg g, const double* __restrict__ x, const double* __re
 }
 
 // x += d for the instances with mask[q]
-__global__ void k_badd(Seg g, double* __restrict__ x, const double* __restrict__ d, const int32_t* __restrict__ mask) {
+__global__ void k_badd(Seg g, double* __restrict__ x, const double* __restrict__ d, const int32_t* __restrict__ mask,
+                       const int* __restrict__ gate) {
+  if (gated(gate)) return;
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (t >= static_cast<int64_t>(g.R) * g.N) return;
   int64_t i;
@@ -204,7 +223,8 @@ __global__ void k_badd(Seg g, double* __restrict__ x, const double* __restrict__
 
 // res = sum_{j < kq[q]} V_j y[j * I + q], j ascending (gmres_ir.hpp:735-739)
 __global__ void k_bcombine(Seg g, const double* __restrict__ V, int64_t ldv, const int32_t* __restrict__ kq,
-                           const double* __restrict__ y, double* __restrict__ res) {
+                           const double* __restrict__ y, double* __restrict__ res, const int* __restrict__ gate) {
+  if (gated(gate)) return;
   const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   if (t >= static_cast<int64_t>(g.R) * g.N) return;
   int64_t i;
@@ -265,6 +285,224 @@ __global__ void __launch_bounds__(kBT) k_bpivots(const T* __restrict__ D, const 
   }
 }
 
+// ---------------------------------------------------------------- device-resident GMRES-IR
+// Every scalar of the batched GMRES-IR lives on the device, per instance q
+// (I = R S): the refinement state (gmres_ir.hpp:534-634), the Hessenberg
+// matrix, Givens rotations and g of the current GMRES cycle
+// (gmres_ir.hpp:636-741). Flags say which refinement iterations, GMRES
+// cycles and Arnoldi steps any instance takes part in; each kernel reads
+// its flag first, so the whole solve is one fixed launch sequence up to the
+// iteration limits, which is captured once as a HIP graph.
+struct BState {
+  int I = 0, M = 0, max_ir = 0;
+  int32_t *valid, *act, *on, *run, *stop, *ir_it, *kq, *gm_it, *m_of;
+  double *bnorm, *relres, *r_prev, *scale, *rho, *dn, *xn, *rn, *fac;
+  unsigned long long* t;  // device wall clock when each instance stopped (0: not stopped), then the start
+  double *H, *g, *cs, *sn, *y;  // H: (M + 1) x M per instance, column-major; g: M + 1; cs, sn: M; y: M x I
+  int32_t* flags;
+  // refinement iteration i (1 .. max_ir; max_ir + 1 unused), cycle c (0:
+  // the first PrecondGmres, i: iteration i), step k of cycle c
+  __host__ __device__ int32_t* act_flag(int i) const { return flags + i; }
+  __host__ __device__ int32_t* cyc_flag(int c) const { return flags + max_ir + 2 + c * (M + 1); }
+  __host__ __device__ int32_t* step_flag(int c, int k) const { return cyc_flag(c) + 1 + k; }
+  __host__ __device__ int nflags() const { return max_ir + 2 + (max_ir + 1) * (M + 1); }
+};
+
+__device__ __forceinline__ void raise_flag(int32_t* f) { __hip_atomic_store(f, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// fills in place of hipMemsetAsync: captured graphs replay kernels reliably
+__global__ void k_bfill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+__global__ void k_bfill_f64(double* __restrict__ p, int64_t n, double v, const int* __restrict__ gate) {
+  if (gated(gate)) return;
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+__global__ void k_bcopy(const double* __restrict__ src, double* __restrict__ dst, int64_t n,
+                        const int* __restrict__ gate) {
+  if (gated(gate)) return;
+  const int64_t t = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+// start of a solve: every instance's refinement state, and the start time
+__global__ void k_bir_start(BState s) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q == 0) s.t[s.I] = wall_clock64();
+  if (q >= s.I) return;
+  s.act[q] = 0;
+  s.stop[q] = 0;
+  s.ir_it[q] = 0;
+  s.gm_it[q] = 0;
+  s.relres[q] = 0.0;
+  s.r_prev[q] = DBL_MAX;
+  s.t[q] = 0;
+}
+
+// after the first PrecondGmres and ||b||: an instance with a non-finite
+// right-hand side stops with reason 4, the others refine
+__global__ void k_bir_init(BState s) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.I || !s.valid[q]) return;
+  if (!isfinite(s.bnorm[q])) {
+    s.stop[q] = 4;
+    s.relres[q] = __builtin_nan("");
+    return;
+  }
+  s.act[q] = 1;
+  raise_flag(s.act_flag(1));
+}
+
+// refinement iteration i, before its cycle: an exactly zero residual means
+// x is exact (stop reason 3); fac = 1 / ||r||
+__global__ void k_bir_scale(BState s, int i) {
+  if (*s.act_flag(i) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.I) return;
+  if (!s.act[q]) {
+    s.fac[q] = 0.0;
+    return;
+  }
+  if (s.scale[q] == 0.0) {
+    s.stop[q] = 3;
+    s.ir_it[q] = i;
+    s.relres[q] = 0.0;
+    s.act[q] = 0;
+    s.t[q] = wall_clock64();
+    s.fac[q] = 0.0;
+    return;
+  }
+  s.fac[q] = 1.0 / s.scale[q];
+}
+
+__global__ void k_bir_unscale(BState s, int i) {
+  if (*s.act_flag(i) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < s.I) s.fac[q] = s.act[q] ? s.scale[q] : 0.0;
+}
+
+// the stopping rules after the update of iteration i (gmres_ir.hpp:575-614)
+__global__ void k_bir_check(BState s, int i, double tol) {
+  if (*s.act_flag(i) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.I || !s.act[q]) return;
+  const double r_norm = s.rn[q] / s.bnorm[q];
+  s.ir_it[q] = i;
+  s.relres[q] = r_norm;
+  int stop = 0;
+  if (!isfinite(r_norm) || !isfinite(s.dn[q])) stop = 4;
+  else if (s.dn[q] <= s.xn[q] * (10 * DBL_EPSILON)) stop = 1;
+  else if (fabs(r_norm - s.r_prev[q]) < tol) stop = 2;
+  if (stop == 0) {
+    s.r_prev[q] = r_norm;
+    raise_flag(s.act_flag(i + 1));
+    return;
+  }
+  s.stop[q] = stop;
+  s.act[q] = 0;
+  s.t[q] = wall_clock64();
+}
+
+// no refinement iterations: the residual of the first solve
+__global__ void k_bir_relres(BState s) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < s.I) s.relres[q] = s.rn[q] / s.bnorm[q];
+}
+
+// cycle c: the instances of the mask whose right-hand side is not below the
+// tolerance iterate (gmres_ir.hpp:647-650)
+__global__ void k_bg_begin(BState s, int c, const int32_t* __restrict__ mask, double tol, const int* __restrict__ gate) {
+  if (gated(gate)) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.I) return;
+  const int on = mask[q] && !(s.rho[q] < tol);
+  s.on[q] = on;
+  s.run[q] = on;
+  s.kq[q] = 0;
+  if (on) raise_flag(s.cyc_flag(c));
+}
+
+// after V_0 = M^{-1} b and rho = ||V_0||: V_0 / rho, g = (rho, 0, ..)
+__global__ void k_bg_start(BState s, int c) {
+  if (*s.cyc_flag(c) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.I) return;
+  s.fac[q] = s.on[q] ? 1.0 / s.rho[q] : 0.0;
+  double* g = s.g + static_cast<int64_t>(q) * (s.M + 1);
+  g[0] = s.rho[q];
+  for (int j = 1; j <= s.M; ++j) g[j] = 0.0;
+}
+
+// step k of cycle c: the instances still iterating and within their cycle
+// length (min(max_gmres, n_s), gmres_ir.hpp:643)
+__global__ void k_bg_step(BState s, int c, int k) {
+  if (*s.cyc_flag(c) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.I) return;
+  const int run = s.run[q] && k < s.m_of[q];
+  s.run[q] = run;
+  if (run) raise_flag(s.step_flag(c, k));
+}
+
+// Hessenberg column k of every iterating instance from the two CGS passes
+// and the norm, the earlier rotations applied, a new rotation and g
+// (gmres_ir.hpp:699-730); fac = 1 / H(k + 1, k) for the new basis vector
+__global__ void k_bg_givens(BState s, int c, int k, const double* __restrict__ hc, const double* __restrict__ h2,
+                            double tol) {
+  if (*s.step_flag(c, k) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.I) return;
+  if (!s.run[q]) {
+    s.fac[q] = 0.0;
+    return;
+  }
+  const int I = s.I, M = s.M, k1 = k + 1;
+  double* H = s.H + static_cast<int64_t>(q) * (M + 1) * M + static_cast<int64_t>(k) * (M + 1);  // column k
+  double* g = s.g + static_cast<int64_t>(q) * (M + 1);
+  double* cq = s.cs + static_cast<int64_t>(q) * M;
+  double* sq = s.sn + static_cast<int64_t>(q) * M;
+  for (int j = 0; j <= k; ++j) H[j] = hc[static_cast<int64_t>(j) * I + q] + h2[static_cast<int64_t>(j) * I + q];
+  H[k + 1] = hc[static_cast<int64_t>(k1) * I + q];
+  s.fac[q] = H[k + 1] != 0 ? 1.0 / H[k + 1] : 1.0;
+  for (int j = 0; j < k; ++j) {
+    const double w1 = cq[j] * H[j] - sq[j] * H[j + 1];
+    const double w2 = sq[j] * H[j] + cq[j] * H[j + 1];
+    H[j] = w1;
+    H[j + 1] = w2;
+  }
+  const double nu = dnrm2_pair(H[k], H[k + 1]);
+  if (nu != 0) {
+    cq[k] = H[k] / nu;
+    sq[k] = -H[k + 1] / nu;
+    H[k] = cq[k] * H[k] - sq[k] * H[k + 1];
+    H[k + 1] = 0.0;
+    const double w1 = cq[k] * g[k] - sq[k] * g[k + 1];
+    const double w2 = sq[k] * g[k] + cq[k] * g[k + 1];
+    g[k] = w1;
+    g[k + 1] = w2;
+  }
+  s.kq[q] = k + 1;
+  s.run[q] = fabs(g[k + 1]) > tol;
+}
+
+// back-substitution of every instance (gmres_ir.hpp:731-739): y[j I + q]
+__global__ void k_bg_backsolve(BState s, int c) {
+  if (*s.cyc_flag(c) == 0) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= s.I) return;
+  const int I = s.I, M = s.M, k = s.kq[q];
+  s.gm_it[q] += k;
+  const double* H = s.H + static_cast<int64_t>(q) * (M + 1) * M;
+  const double* g = s.g + static_cast<int64_t>(q) * (M + 1);
+  for (int i = k; i-- > 0;) {
+    double yi = g[i];
+    for (int j = i + 1; j < k; ++j) yi -= H[static_cast<int64_t>(j) * (M + 1) + i] * s.y[static_cast<int64_t>(j) * I + q];
+    s.y[static_cast<int64_t>(i) * I + q] = yi / H[static_cast<int64_t>(i) * (M + 1) + i];
+  }
+}
+
 inline unsigned nblk(int64_t n, int bs = 256) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 double now_ms() {
@@ -301,12 +539,6 @@ double* GmresLdl::bpinned(size_t count) {
 
 // Host values to device memory through the pinned buffer. Waits for the
 // stream first, so no earlier copy still reads the buffer.
-void GmresLdl::put_host(const void* src, size_t bytes, void* dst) {
-  double* hp = bpinned((bytes + sizeof(double) - 1) / sizeof(double));
-  MPIR_HIP(hipStreamSynchronize(stream_));
-  std::memcpy(hp, src, bytes);
-  MPIR_HIP(hipMemcpyAsync(dst, hp, bytes, hipMemcpyHostToDevice, stream_));
-}
 
 void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntry>>& parts) {
   const double t0 = now_ms();
@@ -432,209 +664,149 @@ void GmresLdl::batch_collect_pivots(bool zero_pivot) {
     throw Error(kErrNumeric, "zero pivot in the LDL^T factorization of system " + std::to_string(first_bad));
 }
 
-// M^{-1} applied to R slots of n entries: each tile of the factor is read
-// once for up to 8 right-hand sides
-void GmresLdl::precond_apply_slots(double* x, int R) {
-  if (precond_ == kPrecIlu) {
-    for (int r = 0; r < R; ++r) precond_apply_dev(x + static_cast<int64_t>(r) * n_);
-    return;
-  }
-  set_solve_slots(std::min(R, 8));
-  stats_.precond_applications += R;
-  double* u = d_uvec_.as<double>();
-  if (uf_ == kF64)
-    ldl_solve_multi<double, double, double>(solve_args<double, double>(d_panel_.get()), dsp_, x, R, u, stream_);
-  else if (uf_ == kF32)
-    ldl_solve_multi<float, float, double>(solve_args<float, float>(d_panel_.get()), dsp_, x, R, u, stream_);
-  else if (uf_ == kF16)
-    ldl_solve_multi<__half, float, double>(solve_args<__half, float>(d_panel16_.get()), dsp_, x, R, u, stream_);
-  else
-    ldl_solve_multi<__hip_bfloat16, float, double>(solve_args<__hip_bfloat16, float>(d_panel16_.get()), dsp_, x, R, u,
-                                                   stream_);
-  bstats_.precond_sweeps++;
+// Layout of the per-instance state in d_bs_ (I instances, cycles of M
+// steps, max_ir refinement iterations), in 256-byte aligned areas from
+// `base`; *bytes (if set) receives the total.
+BState bstate_layout(void* base, int I, int M, int max_ir, size_t* bytes = nullptr) {
+  BState s;
+  s.I = I;
+  s.M = M;
+  s.max_ir = max_ir;
+  size_t off = 0;
+  auto take = [&](size_t n) {
+    const size_t at = off;
+    off += (n + 255) / 256 * 256;
+    return static_cast<char*>(base) + at;
+  };
+  const size_t ni = static_cast<size_t>(I) * sizeof(int32_t), nd = static_cast<size_t>(I) * sizeof(double);
+  const size_t m1 = static_cast<size_t>(std::max(M, 1));
+  int32_t** ints[] = {&s.valid, &s.act, &s.on, &s.run, &s.stop, &s.ir_it, &s.kq, &s.gm_it, &s.m_of};
+  for (int32_t** v : ints) *v = reinterpret_cast<int32_t*>(take(ni));
+  double** dbl[] = {&s.bnorm, &s.relres, &s.r_prev, &s.scale, &s.rho, &s.dn, &s.xn, &s.rn, &s.fac};
+  for (double** v : dbl) *v = reinterpret_cast<double*>(take(nd));
+  s.t = reinterpret_cast<unsigned long long*>(take((static_cast<size_t>(I) + 1) * sizeof(unsigned long long)));
+  s.H = reinterpret_cast<double*>(take(nd * (M + 1) * m1));
+  s.g = reinterpret_cast<double*>(take(nd * (M + 1)));
+  s.cs = reinterpret_cast<double*>(take(nd * m1));
+  s.sn = reinterpret_cast<double*>(take(nd * m1));
+  s.y = reinterpret_cast<double*>(take(nd * m1));
+  s.flags = reinterpret_cast<int32_t*>(take(static_cast<size_t>(s.nflags()) * sizeof(int32_t)));
+  if (bytes) *bytes = off;
+  return s;
 }
 
-void GmresLdl::spmv_slots(const double* x, double* y, int R) {
-  for (int r = 0; r < R; ++r) spmv_dev(x + static_cast<int64_t>(r) * n_, y + static_cast<int64_t>(r) * n_);
-}
-
-void GmresLdl::residual_slots(const double* x, const double* b, double* r, int R) {
-  CsrDev A{n_, d_csr_p_.as<int32_t>(), d_csr_j_.as<int32_t>(), d_csr_x_.as<double>()};
-  for (int k = 0; k < R; ++k) {
-    const int64_t o = static_cast<int64_t>(k) * n_;
-    residual_f64(A, x + o, b + o, r + o, stream_);
-  }
-}
-
-// per-instance norms (op 1: 2-norm, 2: inf-norm) of R slots, copied to the host
-void GmresLdl::bnorms(const double* v, int op, int R, std::vector<double>& out) {
-  const int S = batch_->nsys;
-  const int I = R * S;
+// GmresLDLIR::Solve (gmres_ir.hpp:534-634) for every (system, right-hand
+// side) instance in lock step, with PrecondGmres (gmres_ir.hpp:636-741) as
+// the cycles: cycle 0 solves with b, cycle i (refinement iteration i) with
+// the scaled residual. Only kernels are launched (fills in place of
+// memsets), so the sequence can be captured as it stands.
+void GmresLdl::batch_enqueue(const BatchRun& br) {
+  hipStream_t st = stream_;
+  const int R = br.R, M = br.M, S = batch_->nsys, I = R * S;
+  const int64_t N = n_, RN = static_cast<int64_t>(R) * N, ldv = RN;
+  const int maxir = static_cast<int>(max_ir_);
   Seg g{n_, R, S, b_nch_, d_b_sys_of_row_.as<int32_t>(), d_b_ch_start_.as<int32_t>(), d_b_ch_len_.as<int32_t>(),
         d_b_sys_ch_.as<int32_t>()};
-  d_red_.ensure(static_cast<size_t>(2) * R * b_nch_ * sizeof(double));
-  double* part = d_red_.as<double>();
-  hipLaunchKernelGGL(k_bred_partial, dim3(b_nch_, R), dim3(kBT), 0, stream_, g, v, nullptr, op, part);
-  hipLaunchKernelGGL(k_bred_final, dim3(nblk(I)), dim3(256), 0, stream_, g, part, op, d_b_scalar_.as<double>());
-  double* hp = bpinned(I);
-  MPIR_HIP(hipMemcpyAsync(hp, d_b_scalar_.get(), I * sizeof(double), hipMemcpyDeviceToHost, stream_));
-  MPIR_HIP(hipStreamSynchronize(stream_));
-  out.assign(hp, hp + I);
-}
-
-// PrecondGmres (gmres_ir.hpp:636-741) for every active instance at once:
-// res = V y with V the CGS2 Arnoldi basis of M^{-1} K_s from M^{-1} b.
-void GmresLdl::precond_gmres_batch(const double* b, double* res, const std::vector<char>& active) {
-  hipStream_t st = stream_;
-  const int S = batch_->nsys;
-  const int R = b_slots_;
-  const int I = R * S;
-  const int64_t N = n_;
-  const int64_t RN = static_cast<int64_t>(R) * N;
-  // the reference caps the cycle at n (gmres_ir.hpp:643), per system here
-  std::vector<int> m_of(I);
-  int M = 0;
-  for (int q = 0; q < I; ++q) {
-    const int s = q % S;
-    m_of[q] = static_cast<int>(std::min<int64_t>(max_gmres_, batch_n(s)));
-    if (active[q]) M = std::max(M, m_of[q]);
-  }
-  MPIR_CHECK(M + 2 <= 4096, "GMRES iteration limit above 4094");
-  const int64_t ldv = RN;
-  d_V_.ensure(static_cast<size_t>(M + 1) * ldv * sizeof(double));
-  d_hcol_.ensure(static_cast<size_t>(M + 2) * I * sizeof(double));
-  d_small_.ensure(static_cast<size_t>(M + 2) * I * sizeof(double));
-  d_red_.ensure(static_cast<size_t>(std::max(M + 1, 2)) * R * b_nch_ * sizeof(double));
-  d_b_scalar_.ensure(static_cast<size_t>(std::max(M, 1)) * I * sizeof(double));
-  d_b_kq_.ensure(I * sizeof(int32_t));
+  const BState bs = bstate_layout(d_bs_.get(), I, M, maxir);
   double* V = d_V_.as<double>();
   double* hc = d_hcol_.as<double>();
   double* h2 = d_small_.as<double>();
   double* part = d_red_.as<double>();
-  Seg g{n_, R, S, b_nch_, d_b_sys_of_row_.as<int32_t>(), d_b_ch_start_.as<int32_t>(), d_b_ch_len_.as<int32_t>(),
-        d_b_sys_ch_.as<int32_t>()};
-  MPIR_HIP(hipMemsetAsync(res, 0, RN * sizeof(double), st));
-  std::vector<double> rho;
-  bnorms(b, 1, R, rho);
-  std::vector<char> on(I, 0);
-  bool any = false;
-  for (int q = 0; q < I; ++q) {
-    on[q] = active[q] && !(rho[q] < tol_);
-    any = any || on[q];
-  }
-  if (!any) return;
-  MPIR_HIP(hipMemcpyAsync(V, b, RN * sizeof(double), hipMemcpyDeviceToDevice, st));
-  precond_apply_slots(V, R);
-  bnorms(V, 1, R, rho);
-  // V_0 = M^{-1} b / rho (instances not iterating: zero)
-  std::vector<double> fac(I, 0.0);
-  for (int q = 0; q < I; ++q) fac[q] = on[q] ? 1.0 / rho[q] : 0.0;
-  put_host(fac.data(), I * sizeof(double), d_b_scalar_.get());
-  hipLaunchKernelGGL(k_bscale, dim3(nblk(RN)), dim3(256), 0, st, g, V, d_b_scalar_.as<double>(), V);
-  // per instance: Hessenberg (column-major (M+1) x M), Givens, g
-  std::vector<double> H(static_cast<size_t>(I) * (M + 1) * std::max(M, 1), 0.0);
-  std::vector<double> gv(static_cast<size_t>(I) * (M + 1), 0.0), cs(static_cast<size_t>(I) * M, 0.0),
-      sn(static_cast<size_t>(I) * M, 0.0);
-  std::vector<int> kq(I, 0);
-  auto Hq = [&](int q, int i, int j) -> double& { return H[(static_cast<size_t>(q) * std::max(M, 1) + j) * (M + 1) + i]; };
-  for (int q = 0; q < I; ++q) {
-    gv[static_cast<size_t>(q) * (M + 1)] = rho[q];
-  }
-  std::vector<char> run = on;  // still iterating
-  for (int k = 0; k < M; ++k) {
-    bool more = false;
-    for (int q = 0; q < I; ++q) {
-      run[q] = run[q] && k < m_of[q];
-      more = more || run[q];
+  const double tol = tol_;
+  const dim3 gi(nblk(I)), gv(nblk(RN)), gch(b_nch_, R);
+  // per-instance reductions (op 1: 2-norm, 2: inf-norm) into out[q]
+  auto bred = [&](const double* v, int op, double* out, const int* gate) {
+    hipLaunchKernelGGL(k_bred_partial, gch, dim3(kBT), 0, st, g, v, nullptr, op, part, gate);
+    hipLaunchKernelGGL(k_bred_final, gi, dim3(256), 0, st, g, part, op, out, gate);
+  };
+  auto scale = [&](const double* x, double* out, const int* gate) {
+    hipLaunchKernelGGL(k_bscale, gv, dim3(256), 0, st, g, x, bs.fac, out, gate);
+  };
+  auto precond = [&](double* x, const int* gate) {
+    double* u = d_uvec_.as<double>();
+    if (uf_ == kF64)
+      ldl_solve_multi<double, double, double>(solve_args<double, double>(d_panel_.get()), dsp_, x, R, u, st, gate);
+    else if (uf_ == kF32)
+      ldl_solve_multi<float, float, double>(solve_args<float, float>(d_panel_.get()), dsp_, x, R, u, st, gate);
+    else if (uf_ == kF16)
+      ldl_solve_multi<__half, float, double>(solve_args<__half, float>(d_panel16_.get()), dsp_, x, R, u, st, gate);
+    else
+      ldl_solve_multi<__hip_bfloat16, float, double>(solve_args<__hip_bfloat16, float>(d_panel16_.get()), dsp_, x, R,
+                                                     u, st, gate);
+  };
+  auto product = [&](const double* x, const double* b, double* y, const int* gate) {
+    CsrDev A{n_, d_csr_p_.as<int32_t>(), d_csr_j_.as<int32_t>(), d_csr_x_.as<double>(), gate};
+    for (int r = 0; r < R; ++r) {
+      const int64_t o = static_cast<int64_t>(r) * N;
+      if (b) residual_f64(A, x + o, b + o, y + o, st);
+      else spmv_f64(A, x + o, y + o, st);
     }
-    if (!more) break;
-    bstats_.arnoldi_steps++;
-    double* w = V + ldv * (k + 1);
-    spmv_slots(V + ldv * k, w, R);
-    precond_apply_slots(w, R);
-    const int k1 = k + 1;
-    for (int pass = 0; pass < 2; ++pass) {
-      double* hp = pass == 0 ? hc : h2;
-      hipLaunchKernelGGL(k_bmdot_partial, dim3(b_nch_, R), dim3(kBT), 0, st, g, V, ldv, k1, w, part);
-      hipLaunchKernelGGL(k_bmdot_final, dim3(nblk(static_cast<int64_t>(k1) * I)), dim3(256), 0, st, g, part, k1, hp, 0);
-      hipLaunchKernelGGL(k_bsub_basis, dim3(nblk(RN)), dim3(256), 0, st, g, V, ldv, k1, hp, w);
-    }
-    // the column is h = h1 + h2, summed on the host below
-    hipLaunchKernelGGL(k_bred_partial, dim3(b_nch_, R), dim3(kBT), 0, st, g, w, nullptr, 1, part);
-    hipLaunchKernelGGL(k_bred_final, dim3(nblk(I)), dim3(256), 0, st, g, part, 1, hc + static_cast<int64_t>(k1) * I);
-    // both passes' coefficients and the norm to the host
-    double* hs = bpinned(static_cast<size_t>(2 * k1 + 1) * I);
-    MPIR_HIP(hipMemcpyAsync(hs, hc, static_cast<size_t>(k1 + 1) * I * sizeof(double), hipMemcpyDeviceToHost, st));
-    double* hs2 = hs + static_cast<size_t>(k1 + 1) * I;
-    MPIR_HIP(hipMemcpyAsync(hs2, h2, static_cast<size_t>(k1) * I * sizeof(double), hipMemcpyDeviceToHost, st));
-    MPIR_HIP(hipStreamSynchronize(st));
-    std::fill(fac.begin(), fac.end(), 0.0);
-    for (int q = 0; q < I; ++q) {
-      if (!run[q]) continue;
-      double* gq = &gv[static_cast<size_t>(q) * (M + 1)];
-      double* cq = &cs[static_cast<size_t>(q) * M];
-      double* sq = &sn[static_cast<size_t>(q) * M];
-      for (int j = 0; j <= k; ++j) Hq(q, j, k) = hs[static_cast<size_t>(j) * I + q] + hs2[static_cast<size_t>(j) * I + q];
-      Hq(q, k + 1, k) = hs[static_cast<size_t>(k1) * I + q];
-      fac[q] = Hq(q, k + 1, k) != 0 ? 1.0 / Hq(q, k + 1, k) : 1.0;
-      for (int j = 0; j < k; ++j) {
-        const double w1 = cq[j] * Hq(q, j, k) - sq[j] * Hq(q, j + 1, k);
-        const double w2 = sq[j] * Hq(q, j, k) + cq[j] * Hq(q, j + 1, k);
-        Hq(q, j, k) = w1;
-        Hq(q, j + 1, k) = w2;
+  };
+  // one PrecondGmres cycle for the instances of mask: res = V y
+  auto cycle = [&](int c, const int32_t* mask, const int* gate, const double* b, double* res) {
+    hipLaunchKernelGGL(k_bfill_f64, gv, dim3(256), 0, st, res, RN, 0.0, gate);
+    bred(b, 1, bs.rho, gate);
+    hipLaunchKernelGGL(k_bg_begin, gi, dim3(256), 0, st, bs, c, mask, tol, gate);
+    const int* cg = bs.cyc_flag(c);
+    hipLaunchKernelGGL(k_bcopy, gv, dim3(256), 0, st, b, V, RN, cg);
+    precond(V, cg);
+    bred(V, 1, bs.rho, cg);
+    hipLaunchKernelGGL(k_bg_start, gi, dim3(256), 0, st, bs, c);
+    scale(V, V, cg);
+    for (int k = 0; k < M; ++k) {
+      hipLaunchKernelGGL(k_bg_step, gi, dim3(256), 0, st, bs, c, k);
+      const int* sg = bs.step_flag(c, k);
+      double* w = V + ldv * (k + 1);
+      product(V + ldv * k, nullptr, w, sg);
+      precond(w, sg);
+      const int k1 = k + 1;
+      for (int pass = 0; pass < 2; ++pass) {
+        double* hp = pass == 0 ? hc : h2;
+        hipLaunchKernelGGL(k_bmdot_partial, gch, dim3(kBT), 0, st, g, V, ldv, k1, w, part, sg);
+        hipLaunchKernelGGL(k_bmdot_final, dim3(nblk(static_cast<int64_t>(k1) * I)), dim3(256), 0, st, g, part, k1, hp, 0,
+                           sg);
+        hipLaunchKernelGGL(k_bsub_basis, gv, dim3(256), 0, st, g, V, ldv, k1, hp, w, sg);
       }
-      const double nu = dnrm2_pair(Hq(q, k, k), Hq(q, k + 1, k));
-      if (nu != 0) {
-        cq[k] = Hq(q, k, k) / nu;
-        sq[k] = -Hq(q, k + 1, k) / nu;
-        Hq(q, k, k) = cq[k] * Hq(q, k, k) - sq[k] * Hq(q, k + 1, k);
-        Hq(q, k + 1, k) = 0.0;
-        const double w1 = cq[k] * gq[k] - sq[k] * gq[k + 1];
-        const double w2 = sq[k] * gq[k] + cq[k] * gq[k + 1];
-        gq[k] = w1;
-        gq[k + 1] = w2;
-      }
-      kq[q] = k + 1;
-      run[q] = std::fabs(gq[k + 1]) > tol_;
+      bred(w, 1, hc + static_cast<int64_t>(k1) * I, sg);
+      hipLaunchKernelGGL(k_bg_givens, gi, dim3(256), 0, st, bs, c, k, hc, h2, tol);
+      scale(w, w, sg);
     }
-    put_host(fac.data(), I * sizeof(double), d_b_scalar_.get());
-    hipLaunchKernelGGL(k_bscale, dim3(nblk(RN)), dim3(256), 0, st, g, w, d_b_scalar_.as<double>(), w);
+    hipLaunchKernelGGL(k_bg_backsolve, gi, dim3(256), 0, st, bs, c);
+    hipLaunchKernelGGL(k_bcombine, gv, dim3(256), 0, st, g, V, ldv, bs.kq, bs.y, res, cg);
+  };
+  hipLaunchKernelGGL(k_bfill_i32, dim3(nblk(bs.nflags())), dim3(256), 0, st, bs.flags, static_cast<int64_t>(bs.nflags()),
+                     0);
+  hipLaunchKernelGGL(k_bir_start, gi, dim3(256), 0, st, bs);
+  hipLaunchKernelGGL(k_bperm_in, gv, dim3(256), 0, st, g, br.io_in, d_b_off_.as<int64_t>(), d_b_nrhs_.as<int32_t>(),
+                     d_b_col_base_.as<int32_t>(), d_b_lperm_.as<int32_t>(), d_ad_.as<double>(), br.vb);
+  cycle(0, bs.valid, nullptr, br.vb, br.vx);
+  product(br.vx, br.vb, br.vr, nullptr);
+  bred(br.vb, 2, bs.bnorm, nullptr);
+  hipLaunchKernelGGL(k_bir_init, gi, dim3(256), 0, st, bs);
+  for (int i = 1; i <= maxir; ++i) {
+    const int* ag = bs.act_flag(i);
+    bred(br.vr, 2, bs.scale, ag);
+    hipLaunchKernelGGL(k_bir_scale, gi, dim3(256), 0, st, bs, i);
+    scale(br.vr, br.vt, ag);
+    cycle(i, bs.act, ag, br.vt, br.vd);
+    hipLaunchKernelGGL(k_bir_unscale, gi, dim3(256), 0, st, bs, i);
+    scale(br.vd, br.vd, ag);
+    bred(br.vd, 2, bs.dn, ag);
+    bred(br.vx, 2, bs.xn, ag);
+    hipLaunchKernelGGL(k_badd, gv, dim3(256), 0, st, g, br.vx, br.vd, bs.act, ag);
+    product(br.vx, br.vb, br.vr, ag);
+    bred(br.vr, 2, bs.rn, ag);
+    hipLaunchKernelGGL(k_bir_check, gi, dim3(256), 0, st, bs, i, tol);
   }
-  // back substitution per instance, then res = V y
-  int kmax = 0;
-  std::vector<double> y(static_cast<size_t>(std::max(M, 1)) * I, 0.0);
-  for (int q = 0; q < I; ++q) {
-    const int k = kq[q];
-    kmax = std::max(kmax, k);
-    bstats_.gmres_iterations[q] += k;
-    const double* gq = &gv[static_cast<size_t>(q) * (M + 1)];
-    std::vector<double> yq(std::max(k, 1), 0.0);
-    for (int i = k; i-- > 0;) {
-      yq[i] = gq[i];
-      for (int j = i + 1; j < k; ++j) yq[i] -= Hq(q, i, j) * yq[j];
-      yq[i] /= Hq(q, i, i);
-    }
-    for (int j = 0; j < k; ++j) y[static_cast<size_t>(j) * I + q] = yq[j];
+  if (maxir == 0) {
+    bred(br.vr, 2, bs.rn, nullptr);
+    hipLaunchKernelGGL(k_bir_relres, gi, dim3(256), 0, st, bs);
   }
-  if (kmax == 0) {
-    MPIR_HIP(hipStreamSynchronize(st));
-    return;
-  }
-  // y (kmax x I) and the step counts in one upload
-  std::vector<double> yk(static_cast<size_t>(kmax) * I + (I + 1) / 2, 0.0);
-  std::memcpy(yk.data(), y.data(), static_cast<size_t>(kmax) * I * sizeof(double));
-  int32_t* hk = reinterpret_cast<int32_t*>(yk.data() + static_cast<size_t>(kmax) * I);
-  for (int q = 0; q < I; ++q) hk[q] = kq[q];
-  d_b_scalar2_.ensure(yk.size() * sizeof(double));
-  put_host(yk.data(), yk.size() * sizeof(double), d_b_scalar2_.get());
-  const double* dy = d_b_scalar2_.as<double>();
-  const int32_t* dk = reinterpret_cast<const int32_t*>(dy + static_cast<size_t>(kmax) * I);
-  hipLaunchKernelGGL(k_bcombine, dim3(nblk(RN)), dim3(256), 0, st, g, V, ldv, dk, dy, res);
-  MPIR_HIP(hipStreamSynchronize(st));
+  hipLaunchKernelGGL(k_bperm_out, gv, dim3(256), 0, st, g, br.vx, d_b_off_.as<int64_t>(), d_b_nrhs_.as<int32_t>(),
+                     d_b_col_base_.as<int32_t>(), d_b_lperm_.as<int32_t>(), d_ad_.as<double>(), br.io_out);
+  MPIR_HIP(hipGetLastError());
 }
 
-// GmresLDLIR::Solve (gmres_ir.hpp:534-634) for every (system, right-hand
-// side) instance of the batch in lock step.
 void GmresLdl::solve_batch(const int32_t* nrhs, const double* const* b, double* const* x, bool device_ptr) {
   if (!factored_ || !batch_) throw Error(kErrState, "solve_batch called before a successful factor_batch");
   MPIR_CHECK(nrhs && b && x, "null batch arguments");
@@ -652,137 +824,146 @@ void GmresLdl::solve_batch(const int32_t* nrhs, const double* const* b, double* 
   const int I = R * S;
   const int64_t N = n_;
   const int64_t RN = static_cast<int64_t>(R) * N;
+  // instances q = r S + s; the cycle length of each is min(max_gmres, n_s)
+  // (gmres_ir.hpp:643), M the longest of the valid ones
+  std::vector<int32_t> valid_m(2 * static_cast<size_t>(I), 0);
+  int M = 0;
+  for (int q = 0; q < I; ++q) {
+    const int s = q % S;
+    valid_m[q] = (q / S) < nrhs[s];
+    valid_m[I + q] = static_cast<int32_t>(std::min<int64_t>(max_gmres_, batch_n(s)));
+    if (valid_m[q]) M = std::max(M, valid_m[I + q]);
+  }
+  MPIR_CHECK(M + 2 <= 4096, "GMRES iteration limit above 4094");
   // per-system right-hand sides packed into one device array (ld = n_s)
   std::vector<int64_t> off(S + 1, 0);
   for (int s = 0; s < S; ++s) off[s + 1] = off[s] + static_cast<int64_t>(nrhs[s]) * batch_n(s);
   d_b_vec_.ensure(static_cast<size_t>(5 * RN + 2 * off[S]) * sizeof(double));
-  double* vb = d_b_vec_.as<double>();
-  double* vx = vb + RN;
-  double* vr = vx + RN;
-  double* vd = vr + RN;
-  double* vt = vd + RN;
-  double* io_in = vt + RN;
-  double* io_out = io_in + off[S];
+  BatchRun br;
+  br.R = R;
+  br.M = M;
+  br.vb = d_b_vec_.as<double>();
+  br.vx = br.vb + RN;
+  br.vr = br.vx + RN;
+  br.vd = br.vr + RN;
+  br.vt = br.vd + RN;
+  br.io_in = br.vt + RN;
+  br.io_out = br.io_in + off[S];
+  // every buffer of the sequence exists before it is launched or captured
+  d_V_.ensure(static_cast<size_t>(M + 1) * RN * sizeof(double));
+  d_hcol_.ensure(static_cast<size_t>(M + 2) * I * sizeof(double));
+  d_small_.ensure(static_cast<size_t>(M + 2) * I * sizeof(double));
+  d_red_.ensure(static_cast<size_t>(std::max(M, 2)) * R * b_nch_ * sizeof(double));
+  size_t state_bytes = 0;
+  (void)bstate_layout(nullptr, I, M, static_cast<int>(max_ir_), &state_bytes);
+  d_bs_.ensure(state_bytes);
+  set_solve_slots(std::min(R, 8));
+  const BState bs = bstate_layout(d_bs_.get(), I, M, static_cast<int>(max_ir_));
   for (int s = 0; s < S; ++s)
     if (nrhs[s] > 0)
-      MPIR_HIP(hipMemcpyAsync(io_in + off[s], b[s], (off[s + 1] - off[s]) * sizeof(double),
+      MPIR_HIP(hipMemcpyAsync(br.io_in + off[s], b[s], (off[s + 1] - off[s]) * sizeof(double),
                               device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
   std::vector<int32_t> nr(nrhs, nrhs + S);
   upload_vec(d_b_off_, off, st);
   upload_vec(d_b_nrhs_, nr, st);
-  d_b_mask_.ensure(I * sizeof(int32_t));
-  d_b_scalar_.ensure(I * sizeof(double));
-  Seg g{n_, R, S, b_nch_, d_b_sys_of_row_.as<int32_t>(), d_b_ch_start_.as<int32_t>(), d_b_ch_len_.as<int32_t>(),
-        d_b_sys_ch_.as<int32_t>()};
-  d_red_.ensure(static_cast<size_t>(R) * b_nch_ * sizeof(double) * 2);
-  hipLaunchKernelGGL(k_bperm_in, dim3(nblk(RN)), dim3(256), 0, st, g, io_in, d_b_off_.as<int64_t>(),
-                     d_b_nrhs_.as<int32_t>(), d_b_col_base_.as<int32_t>(), d_b_lperm_.as<int32_t>(), d_ad_.as<double>(),
-                     vb);
-  // statistics (instances q = r S + s internally)
-  bstats_.max_rhs = R;
-  bstats_.arnoldi_steps = bstats_.precond_sweeps = 0;
-  bstats_.gmres_iterations.assign(I, 0);
-  std::vector<int32_t> ir_it(I, 0), stop(I, 0);
-  std::vector<double> relres(I, 0.0), done(I, 0.0);
-  std::vector<char> valid(I, 0);
-  for (int q = 0; q < I; ++q) valid[q] = (q / S) < nrhs[q % S];
-
-  precond_gmres_batch(vb, vx, valid);
-  residual_slots(vx, vb, vr, R);
-  std::vector<double> bnorm, rn, dn, xn;
-  bnorms(vb, 2, R, bnorm);
-  std::vector<double> r_prev(I, std::numeric_limits<double>::max());
-  std::vector<char> act = valid;
-  // an instance whose right-hand side or residual is not finite stops with
-  // reason 4 (a single-system handle reports MPIR_ERR_NUMERIC instead; one
-  // bad instance must not end the others of its batch)
-  for (int q = 0; q < I; ++q)
-    if (act[q] && !std::isfinite(bnorm[q])) {
-      stop[q] = 4;
-      relres[q] = std::numeric_limits<double>::quiet_NaN();
-      act[q] = 0;
-    }
-  int64_t iter;
-  for (iter = 1; iter <= max_ir_; ++iter) {
-    bool any = false;
-    for (int q = 0; q < I; ++q) any = any || act[q];
-    if (!any) break;
-    std::vector<double> scale, f(I);
-    bnorms(vr, 2, R, scale);
-    for (int q = 0; q < I; ++q) {
-      if (act[q] && scale[q] == 0.0) {
-        // an exactly zero residual: x is exact (the reference would scale by 1/0)
-        stop[q] = 3;
-        ir_it[q] = static_cast<int32_t>(iter);
-        relres[q] = 0.0;
-        act[q] = 0;
-        done[q] = now_ms() - t_start;
+  MPIR_HIP(hipMemcpyAsync(bs.valid, valid_m.data(), I * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  MPIR_HIP(hipMemcpyAsync(bs.m_of, valid_m.data() + I, I * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  // the key of this solve's launch sequence: every scalar and address its
+  // kernels take
+  std::vector<uint64_t> key = {static_cast<uint64_t>(R), static_cast<uint64_t>(S), static_cast<uint64_t>(M),
+                               static_cast<uint64_t>(max_ir_), static_cast<uint64_t>(schedule_),
+                               static_cast<uint64_t>(uf_), plan_epoch_, static_cast<uint64_t>(n_)};
+  uint64_t tol_bits = 0;
+  std::memcpy(&tol_bits, &tol_, sizeof tol_bits);
+  key.push_back(tol_bits);
+  for (const DevBuf* d : {&d_b_vec_, &d_V_, &d_hcol_, &d_small_, &d_red_, &d_bs_, &d_uvec_, &d_scratch_, &d_panel_,
+                          &d_panel16_, &d_D_, &d_Dinv_, &d_counters_, &d_serr_, &d_csr_p_, &d_csr_j_, &d_csr_x_,
+                          &d_ad_, &d_b_off_, &d_b_nrhs_, &d_b_col_base_, &d_b_lperm_, &d_b_sys_of_row_,
+                          &d_b_ch_start_, &d_b_ch_len_, &d_b_sys_ch_})
+    key.push_back(reinterpret_cast<uint64_t>(d->get()));
+  key.push_back(static_cast<uint64_t>(off[S]));
+  bool replay = false;
+  const bool cached = bgraph_ != nullptr && key == bgraph_key_;
+  if (batch_graph_ && (cached || key == blast_key_)) {
+    if (!cached) {
+      if (bgraph_) {
+        (void)hipGraphExecDestroy(bgraph_);
+        bgraph_ = nullptr;
       }
-      f[q] = act[q] ? 1.0 / scale[q] : 0.0;
-    }
-    put_host(f.data(), I * sizeof(double), d_b_scalar_.get());
-    hipLaunchKernelGGL(k_bscale, dim3(nblk(RN)), dim3(256), 0, st, g, vr, d_b_scalar_.as<double>(), vt);
-    precond_gmres_batch(vt, vd, act);
-    for (int q = 0; q < I; ++q) f[q] = act[q] ? scale[q] : 0.0;
-    put_host(f.data(), I * sizeof(double), d_b_scalar_.get());
-    hipLaunchKernelGGL(k_bscale, dim3(nblk(RN)), dim3(256), 0, st, g, vd, d_b_scalar_.as<double>(), vd);
-    bnorms(vd, 2, R, dn);
-    bnorms(vx, 2, R, xn);
-    std::vector<int32_t> hm(I);
-    for (int q = 0; q < I; ++q) hm[q] = act[q] ? 1 : 0;
-    put_host(hm.data(), I * sizeof(int32_t), d_b_mask_.get());
-    hipLaunchKernelGGL(k_badd, dim3(nblk(RN)), dim3(256), 0, st, g, vx, vd, d_b_mask_.as<int32_t>());
-    residual_slots(vx, vb, vr, R);
-    bnorms(vr, 2, R, rn);
-    for (int q = 0; q < I; ++q) {
-      if (!act[q]) continue;
-      const double r_norm = rn[q] / bnorm[q];
-      ir_it[q] = static_cast<int32_t>(iter);
-      relres[q] = r_norm;
-      if (!std::isfinite(r_norm) || !std::isfinite(dn[q])) {
-        stop[q] = 4;
-      } else if (dn[q] <= xn[q] * (10 * std::numeric_limits<double>::epsilon())) {
-        stop[q] = 1;
-      } else if (std::fabs(r_norm - r_prev[q]) < tol_) {
-        stop[q] = 2;
-      } else {
-        r_prev[q] = r_norm;
-        continue;
+      hipGraph_t graph = nullptr;
+      MPIR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        batch_enqueue(br);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
       }
-      act[q] = 0;
-      done[q] = now_ms() - t_start;
+      MPIR_HIP(hipStreamEndCapture(st, &graph));
+      size_t nodes = 0;
+      MPIR_HIP(hipGraphGetNodes(graph, nullptr, &nodes));
+      const hipError_t e = hipGraphInstantiate(&bgraph_, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      MPIR_HIP(e);
+      bgraph_nodes_ = static_cast<int32_t>(nodes);
+      bgraph_key_ = key;
     }
+    MPIR_HIP(hipGraphLaunch(bgraph_, st));
+    replay = true;
+  } else {
+    batch_enqueue(br);
   }
-  if (max_ir_ == 0) {
-    bnorms(vr, 2, R, rn);
-    for (int q = 0; q < I; ++q) relres[q] = rn[q] / bnorm[q];
-  }
-  const double t_end = now_ms() - t_start;
-  for (int q = 0; q < I; ++q)
-    if (valid[q] && act[q]) done[q] = t_end;  // iteration limit
-  hipLaunchKernelGGL(k_bperm_out, dim3(nblk(RN)), dim3(256), 0, st, g, vx, d_b_off_.as<int64_t>(),
-                     d_b_nrhs_.as<int32_t>(), d_b_col_base_.as<int32_t>(), d_b_lperm_.as<int32_t>(), d_ad_.as<double>(),
-                     io_out);
+  blast_key_ = key;
   for (int s = 0; s < S; ++s)
     if (nrhs[s] > 0)
-      MPIR_HIP(hipMemcpyAsync(x[s], io_out + off[s], (off[s + 1] - off[s]) * sizeof(double),
+      MPIR_HIP(hipMemcpyAsync(x[s], br.io_out + off[s], (off[s + 1] - off[s]) * sizeof(double),
                               device_ptr ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+  // the per-instance outcome
+  std::vector<int32_t> h_stop(I), h_ir(I), h_gm(I), h_act(I), h_flags(bs.nflags());
+  std::vector<double> h_rel(I);
+  std::vector<unsigned long long> h_t(static_cast<size_t>(I) + 1);
+  auto get = [&](void* dst, const void* src, size_t bytes) {
+    MPIR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+  };
+  get(h_stop.data(), bs.stop, I * sizeof(int32_t));
+  get(h_ir.data(), bs.ir_it, I * sizeof(int32_t));
+  get(h_gm.data(), bs.gm_it, I * sizeof(int32_t));
+  get(h_act.data(), bs.act, I * sizeof(int32_t));
+  get(h_flags.data(), bs.flags, h_flags.size() * sizeof(int32_t));
+  get(h_rel.data(), bs.relres, I * sizeof(double));
+  get(h_t.data(), bs.t, h_t.size() * sizeof(unsigned long long));
   sync_check();
+  const double t_end = now_ms() - t_start;
+  int dev = 0, khz = 0;
+  MPIR_HIP(hipGetDevice(&dev));
+  MPIR_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  // lock-step counts from the flags
+  int32_t steps = 0, cycles = 0;
+  const int maxir = static_cast<int>(max_ir_);
+  for (int c = 0; c <= maxir; ++c) {
+    cycles += h_flags[maxir + 2 + c * (M + 1)] ? 1 : 0;
+    for (int k = 0; k < M; ++k) steps += h_flags[maxir + 2 + c * (M + 1) + 1 + k] ? 1 : 0;
+  }
+  bstats_.max_rhs = R;
+  bstats_.arnoldi_steps = steps;
+  bstats_.precond_sweeps = cycles + steps;
+  bstats_.graph_nodes = replay ? bgraph_nodes_ : 0;
+  stats_.precond_applications += static_cast<int64_t>(R) * (cycles + steps);
   // per-instance statistics in system-major order (s * R + r)
   bstats_.ir_iterations.assign(I, 0);
+  bstats_.gmres_iterations.assign(I, 0);
   bstats_.stop_reason.assign(I, 0);
   bstats_.final_rel_residual.assign(I, 0.0);
   bstats_.done_ms.assign(I, 0.0);
-  std::vector<int32_t> gm(I, 0);
   for (int q = 0; q < I; ++q) {
     const int r = q / S, s = q % S, o = s * R + r;
-    bstats_.ir_iterations[o] = ir_it[q];
-    bstats_.stop_reason[o] = stop[q];
-    bstats_.final_rel_residual[o] = relres[q];
-    bstats_.done_ms[o] = done[q];
-    gm[o] = bstats_.gmres_iterations[q];
+    bstats_.ir_iterations[o] = h_ir[q];
+    bstats_.gmres_iterations[o] = h_gm[q];
+    bstats_.stop_reason[o] = h_stop[q];
+    bstats_.final_rel_residual[o] = h_rel[q];
+    if (h_t[q] != 0 && khz > 0) bstats_.done_ms[o] = static_cast<double>(h_t[q] - h_t[I]) / khz;
+    else if (valid_m[q] && h_act[q]) bstats_.done_ms[o] = t_end;  // iteration limit
   }
-  bstats_.gmres_iterations.swap(gm);
   bstats_.solve_ms = now_ms() - t_start;
   stats_.solve_ms = bstats_.solve_ms;
 }

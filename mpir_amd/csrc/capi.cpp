@@ -716,6 +716,19 @@ int mpir_batch_stats(mpir_gmres_ldl_t h, int32_t* max_rhs, int32_t* positive_piv
     return 0;
   });
 }
+int mpir_batch_graph(mpir_gmres_ldl_t h, int enable, int32_t* counts) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    if (enable >= 0) h->s->set_batch_graph(enable != 0);
+    if (counts) {
+      const BatchStats& b = h->s->batch_stats();
+      counts[0] = b.graph_nodes;
+      counts[1] = b.arnoldi_steps;
+      counts[2] = b.precond_sweeps;
+    }
+    return 0;
+  });
+}
 int mpir_qdldl_solve_plan_builds(int64_t* builds) {
   return guarded([&] {
     MPIR_CHECK(builds, "null output");

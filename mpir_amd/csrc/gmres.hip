@@ -231,24 +231,6 @@ __device__ __forceinline__ void mirror_put(double* mirror, int slot, double v) {
   __hip_atomic_store(mirror + slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// dnrm2 of {a, b} in the reference's order (ops.hpp:231-255)
-__device__ double dev_dnrm2_pair(double a, double b) {
-  double scale = 0.0, ssq = 1.0;
-  const double v[2] = {b, a};
-  for (int i = 0; i < 2; ++i) {
-    const double x = v[i];
-    if (x != 0.0) {
-      const double ax = fabs(x);
-      if (scale < ax) {
-        ssq = 1.0 + ssq * ((scale / ax) * (scale / ax));
-        scale = ax;
-      } else {
-        ssq = ssq + ((ax / scale) * (ax / scale));
-      }
-    }
-  }
-  return scale * sqrt(ssq);
-}
 
 // start of a call: it runs when the caller's gate is open and ||b||_2 is not
 // below tol (gmres_ir.hpp:647-650); the first M^-1 application follows
@@ -294,7 +276,7 @@ __global__ void k_gm_givens(GmDev d, int k, double tol, const double* __restrict
       H[j] = w1;
       H[j + 1] = w2;
     }
-    const double nu = dev_dnrm2_pair(H[k], H[k + 1]);
+    const double nu = dnrm2_pair(H[k], H[k + 1]);
     if (nu != 0) {
       c[k] = H[k] / nu;
       s[k] = -H[k + 1] / nu;
@@ -533,6 +515,7 @@ void GmresLdl::wait_stream() {
 
 GmresLdl::~GmresLdl() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  if (bgraph_) (void)hipGraphExecDestroy(bgraph_);
   if (h_pinned_) (void)hipHostFree(h_pinned_);
   if (b_pinned_) (void)hipHostFree(b_pinned_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -624,6 +607,7 @@ void GmresLdl::analyze(int32_t n, const int32_t* Ap, const int32_t* Ai) {
 }
 
 void GmresLdl::upload_plan() {
+  ++plan_epoch_;
   const FactorPlan& P = *plan_;
   const Supernodes& S = P.sn;
   hipStream_t st = stream_;
@@ -958,6 +942,27 @@ double GmresLdl::infnrm(const double* v) {
   return h_pinned_[0];
 }
 
+// M^{-1} applied to R vectors of n entries: each tile of the factor is read
+// once for up to 8 of them
+void GmresLdl::precond_apply_slots(double* x, int R) {
+  if (precond_ == kPrecIlu) {
+    for (int r = 0; r < R; ++r) precond_apply_dev(x + static_cast<int64_t>(r) * n_);
+    return;
+  }
+  set_solve_slots(std::min(R, 8));
+  stats_.precond_applications += R;
+  double* u = d_uvec_.as<double>();
+  if (uf_ == kF64)
+    ldl_solve_multi<double, double, double>(solve_args<double, double>(d_panel_.get()), dsp_, x, R, u, stream_);
+  else if (uf_ == kF32)
+    ldl_solve_multi<float, float, double>(solve_args<float, float>(d_panel_.get()), dsp_, x, R, u, stream_);
+  else if (uf_ == kF16)
+    ldl_solve_multi<__half, float, double>(solve_args<__half, float>(d_panel16_.get()), dsp_, x, R, u, stream_);
+  else
+    ldl_solve_multi<__hip_bfloat16, float, double>(solve_args<__hip_bfloat16, float>(d_panel16_.get()), dsp_, x, R, u,
+                                                   stream_);
+}
+
 void GmresLdl::precond_apply_dev(double* x) {
   stats_.precond_applications++;
   if (precond_ == kPrecIlu) {
@@ -1281,7 +1286,8 @@ void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool devi
 
 double GmresLdl::time_kernels(int what, int reps) {
   if (!factored_) throw Error(kErrState, "time_kernels needs a factorization");
-  MPIR_CHECK(reps >= 1 && what >= 0 && what <= 2, "bad timing request");
+  MPIR_CHECK(reps >= 1 && what >= 0 && what <= 3, "bad timing request");
+  if (what == 3) return time_precond_slots(std::min(8, std::max(1, b_slots_)), reps);
   if (what == 2) MPIR_CHECK(!precond_only_, "no values to refactor");
   hipStream_t st = stream_;
   double* x = d_w_.as<double>();
@@ -1316,6 +1322,32 @@ double GmresLdl::time_kernels(int what, int reps) {
   MPIR_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  sync_check();
+  return static_cast<double>(ms) / reps;
+}
+
+double GmresLdl::time_precond_slots(int rhs, int reps) {
+  hipStream_t st = stream_;
+  d_V_.ensure(static_cast<size_t>(rhs) * n_ * sizeof(double));
+  double* x = d_V_.as<double>();
+  // a deterministic nonzero input: the scaling vector in every slot
+  for (int r = 0; r < rhs; ++r)
+    MPIR_HIP(hipMemcpyAsync(x + static_cast<int64_t>(r) * n_, d_ad_.get(), n_ * sizeof(double),
+                            hipMemcpyDeviceToDevice, st));
+  const int64_t apps = stats_.precond_applications;
+  precond_apply_slots(x, rhs);  // warm
+  hipEvent_t e0, e1;
+  MPIR_HIP(hipEventCreate(&e0));
+  MPIR_HIP(hipEventCreate(&e1));
+  MPIR_HIP(hipEventRecord(e0, st));
+  for (int r = 0; r < reps; ++r) precond_apply_slots(x, rhs);
+  MPIR_HIP(hipEventRecord(e1, st));
+  MPIR_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  MPIR_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  stats_.precond_applications = apps;
   sync_check();
   return static_cast<double>(ms) / reps;
 }
@@ -1388,12 +1420,15 @@ AutotuneResult GmresLdl::autotune(int reps) {
       const SolveTuning t{rows};
       if (!(same_tuning(t, s0) && sched == sched0)) sc.push_back({t, sched});
     }
+  // M^{-1} is timed on as many right-hand sides as the handle's solves apply
+  // it to at once (a batch: its last solve's, up to 8; else 1)
+  const int rhs = batch_ ? std::min(8, std::max(1, b_slots_)) : 1;
   SC sbest = sc[0];
   for (size_t i = 0; i < sc.size(); ++i) {
     build_solve_schedule(P, sc[i].t);
     schedule_ = sc[i].sched;
     install();
-    const double ms = time_kernels(1, reps);
+    const double ms = rhs == 1 ? time_kernels(1, reps) : time_precond_slots(rhs, reps);
     if (i == 0) res.precond_ms_before = ms;
     if (i == 0 || ms < best) {
       best = ms;

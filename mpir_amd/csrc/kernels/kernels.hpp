@@ -176,10 +176,11 @@ void ldl_solve_on(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, const S
                   hipStream_t st);
 // nrhs right-hand sides x + r dp.d.sx at once (each tile of the factor read
 // once for up to 8 of them); uvec and the partial-sum areas of dp.d hold 8
-// right-hand sides at strides dp.d.su, .syp, .sup, .srp, .sxp.
+// right-hand sides at strides dp.d.su, .syp, .sup, .srp, .sxp. With gate
+// set, every kernel returns at once if *gate == 0 (read at kernel start).
 template <class TF, class TD, class TX>
 void ldl_solve_multi(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, int nrhs, TX* uvec,
-                     hipStream_t st);
+                     hipStream_t st, const int* gate = nullptr);
 
 // ---------------------------------------------------------------- FGPILU
 // Device view of the ILU structure (host/ilu.hpp): U by columns, L by rows

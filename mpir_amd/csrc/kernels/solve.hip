@@ -1223,7 +1223,9 @@ void ldl_solve_on(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, const S
 
 template <class TF, class TD, class TX>
 void ldl_solve_multi(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* x, int nrhs, TX* uvec,
-                     hipStream_t st) {
+                     hipStream_t st, const int* gate) {
+  SolveDev d = dp.d;
+  d.gate = gate;
   // groups of 8, 4, 2, 1 right-hand sides for FP32 factors; FP64 tiles do
   // not fit in registers and 16-bit tiles with their conversions spill, so
   // those factors take the right-hand sides one at a time
@@ -1231,25 +1233,25 @@ void ldl_solve_multi(const SolveArgs<TF, TD>& a, const DeviceSolvePlan& dp, TX* 
   static_assert(!multi || (ColTile<TF>::NB == 1 && RowTile<TF>::NB == 1), "tile in registers");
   for (int r0 = 0; r0 < nrhs;) {
     const int left = nrhs - r0;
-    TX* xr = x + r0 * dp.d.sx;
+    TX* xr = x + r0 * d.sx;
     if constexpr (multi) {
       if (left >= 8) {
-        ldl_solve_group<8>(a, dp, dp.d, xr, uvec, st);
+        ldl_solve_group<8>(a, dp, d, xr, uvec, st);
         r0 += 8;
         continue;
       }
       if (left >= 4) {
-        ldl_solve_group<4>(a, dp, dp.d, xr, uvec, st);
+        ldl_solve_group<4>(a, dp, d, xr, uvec, st);
         r0 += 4;
         continue;
       }
       if (left >= 2) {
-        ldl_solve_group<2>(a, dp, dp.d, xr, uvec, st);
+        ldl_solve_group<2>(a, dp, d, xr, uvec, st);
         r0 += 2;
         continue;
       }
     }
-    ldl_solve_group<1>(a, dp, dp.d, xr, uvec, st);
+    ldl_solve_group<1>(a, dp, d, xr, uvec, st);
     r0 += 1;
   }
 }
@@ -1274,13 +1276,13 @@ template void ldl_solve_on<__hip_bfloat16, float, double>(const SolveArgs<__hip_
                                                            const DeviceSolvePlan&, const SolveDev&, double*, double*,
                                                            hipStream_t);
 template void ldl_solve_multi<double, double, double>(const SolveArgs<double, double>&, const DeviceSolvePlan&,
-                                                       double*, int, double*, hipStream_t);
+                                                       double*, int, double*, hipStream_t, const int*);
 template void ldl_solve_multi<float, float, double>(const SolveArgs<float, float>&, const DeviceSolvePlan&, double*,
-                                                     int, double*, hipStream_t);
+                                                     int, double*, hipStream_t, const int*);
 template void ldl_solve_multi<__half, float, double>(const SolveArgs<__half, float>&, const DeviceSolvePlan&, double*,
-                                                      int, double*, hipStream_t);
+                                                      int, double*, hipStream_t, const int*);
 template void ldl_solve_multi<__hip_bfloat16, float, double>(const SolveArgs<__hip_bfloat16, float>&,
                                                               const DeviceSolvePlan&, double*, int, double*,
-                                                              hipStream_t);
+                                                              hipStream_t, const int*);
 
 }  // namespace mpir

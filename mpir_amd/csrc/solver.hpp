@@ -28,6 +28,16 @@ struct BatchStats {
   std::vector<double> final_rel_residual, done_ms;    // per instance; done_ms from the start of the solve
   double factor_ms = 0.0, solve_ms = 0.0;
   int32_t arnoldi_steps = 0, precond_sweeps = 0;      // lock-step counts of the whole batch
+  int32_t graph_nodes = 0;  // launches of the replayed HIP graph (0: the solve was launched kernel by kernel)
+};
+
+// Device vectors of one batched solve (R right-hand-side slots of N rows;
+// io_in / io_out: the caller's right-hand sides and solutions, packed) and
+// the GMRES cycle length M.
+struct BatchRun {
+  int R = 0, M = 0;
+  double *vb = nullptr, *vx = nullptr, *vr = nullptr, *vd = nullptr, *vt = nullptr;
+  double *io_in = nullptr, *io_out = nullptr;
 };
 
 enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1, kOrderNd = 2 };
@@ -35,7 +45,7 @@ enum Ordering : int { kOrderNatural = 0, kOrderAmd = 1, kOrderNd = 2 };
 // Dnrm2 of the two-vector {a, b}, as PrecondGmres forms the Givens norm
 // (include/MPIR/gmres_ir.hpp:702; ops.hpp:231-255: scaled sum of squares,
 // last entry first)
-inline double dnrm2_pair(double a, double b) {
+__host__ __device__ inline double dnrm2_pair(double a, double b) {
   double scale = 0.0, ssq = 1.0;
   const double v[2] = {b, a};
   for (double x : v) {
@@ -183,6 +193,8 @@ class GmresLdl {
   void solve_batch(const int32_t* nrhs, const double* const* b, double* const* x, bool device_ptr);
   const BatchStats& batch_stats() const { return bstats_; }
   bool is_batch() const { return static_cast<bool>(batch_); }
+  // whether repeated batch solves replay a captured HIP graph (default on)
+  void set_batch_graph(bool on) { batch_graph_ = on; }
   int32_t batch_size() const;
   int32_t batch_n(int32_t s) const;
 
@@ -217,6 +229,8 @@ class GmresLdl {
   // re-planned with them (an existing factor stays valid: the storage layout
   // does not change).
   void set_tuning(const FactorTuning& f, const SolveTuning& s);
+  // average ms of one M^{-1} application to `rhs` vectors at once
+  double time_precond_slots(int rhs, int reps);
   // Load a factor given in QDLDL's CSC layout (host arrays, Lx/Dinv in the
   // handle's factor precision) for use as M^{-1} only.
   void load_factor_csc(int32_t n, const int64_t* Lp, const int32_t* Li, const void* Lx, const void* Dinv);
@@ -233,15 +247,11 @@ class GmresLdl {
   double ilu_nlres_now(const void* L, const void* U);
   void batch_count_pivots();
   void batch_collect_pivots(bool zero_pivot);
-  // batched GMRES on R slots of the combined vector; active[q] selects the
-  // instances that iterate, kq[q] returns their step counts
-  void precond_gmres_batch(const double* b, double* res, const std::vector<char>& active);
+  // the launch sequence of one batched GMRES-IR solve, every scalar on the
+  // device (batch.hip); the same sequence runs eagerly or is captured
+  void batch_enqueue(const BatchRun& br);
   void set_solve_slots(int slots);
-  void spmv_slots(const double* x, double* y, int R);
-  void residual_slots(const double* x, const double* b, double* r, int R);
-  void bnorms(const double* v, int op, int R, std::vector<double>& out);
   double* bpinned(size_t count);
-  void put_host(const void* src, size_t bytes, void* dst);
   // device-resident PrecondGmres (device vectors, permuted order), skipped
   // when *gate == 0 (gate may be null)
   void precond_gmres(const double* b, double* res, const int* gate);
@@ -336,9 +346,18 @@ class GmresLdl {
   std::vector<std::shared_ptr<const PatternEntry>> batch_parts_;
   BatchStats bstats_;
   DevBuf d_b_sys_of_row_, d_b_col_base_, d_b_lperm_, d_b_ch_start_, d_b_ch_len_, d_b_sys_ch_;
-  DevBuf d_b_scalar_, d_b_scalar2_, d_b_mask_, d_b_kq_, d_b_off_, d_b_nrhs_, d_b_piv_;
+  DevBuf d_b_off_, d_b_nrhs_, d_b_piv_;
   DevBuf d_b_vec_;  // b, x, r, d, w of every slot
+  DevBuf d_bs_;     // per-instance state of the device-resident batched GMRES-IR
   int32_t b_nch_ = 0, b_slots_ = 0;
+  // the captured solve: replayed whenever a solve's key (shapes,
+  // tolerances, plan, buffers) matches it; captured when a solve repeats the
+  // key of the solve before it (a one-off batch runs eagerly)
+  hipGraphExec_t bgraph_ = nullptr;
+  int32_t bgraph_nodes_ = 0;
+  std::vector<uint64_t> bgraph_key_, blast_key_;
+  bool batch_graph_ = true;
+  uint64_t plan_epoch_ = 0;  // counts upload_plan calls
   int solve_slots_ = 0;  // right-hand sides the solve work areas hold
   double* b_pinned_ = nullptr;
   size_t b_pinned_count_ = 0;

@@ -154,3 +154,59 @@ def test_multi_rhs_preconditioner_equals_single_bitwise(gpu, uf):
         torch.cuda.synchronize()
         print(f"nrhs {nrhs}: max |x| {one.abs().max().item():.3e}", flush=True)
         assert torch.equal(one, many)
+
+
+@pytest.mark.parametrize("uf,sched", [(np.float32, mpir_amd.SCHED_LEVELS), (np.float32, mpir_amd.SCHED_DAG),
+                                      (np.float64, mpir_amd.SCHED_DAG)])
+def test_batch_graph_replays_equal_eager_bitwise(gpu, oracle, uf, sched):
+    # the whole solve as one captured HIP graph: the first solve of a kind
+    # runs kernel by kernel, the next ones replay the graph; every replay
+    # (with eager work of the same handle in between) gives the eager bits,
+    # and so does a handle that never captures
+    systems = small_batch(nsys=4, nrhs=3, seed=5)
+    counts = [3, 1, 0, 2]
+    rhs = [None if c == 0 else q.B[:, :c] for q, c in zip(systems, counts)]
+    s = mpir_amd.BatchLDLIR(uf)
+    s.SetTolerance(1e-12)
+    s.SetSolveSchedule(sched)
+    s.Compute([(q.kkt.Ap, q.kkt.Ai, q.kkt.Ax) for q in systems])
+    x0 = s.Solve(rhs)
+    st0 = s.stats()
+    assert st0["graph_nodes"] == 0 and st0["arnoldi_steps"] > 0
+    for rep in range(3):
+        x = s.Solve(rhs)
+        st = s.stats()
+        assert st["graph_nodes"] > 100, st["graph_nodes"]
+        for a, b in zip(x, x0):
+            if a is not None:
+                np.testing.assert_array_equal(a, b)
+        for key in ("ir_iterations", "gmres_iterations", "stop_reason", "final_rel_residual"):
+            np.testing.assert_array_equal(st[key], st0[key])
+        assert st["arnoldi_steps"] == st0["arnoldi_steps"] and st["precond_sweeps"] == st0["precond_sweeps"]
+        # eager work of the handle between replays: another solve shape
+        s.Solve([q.B[:, :1] for q in systems])
+    e = mpir_amd.BatchLDLIR(uf)
+    e.SetTolerance(1e-12)
+    e.SetSolveSchedule(sched)
+    e.SetGraph(False)
+    e.Compute([(q.kkt.Ap, q.kkt.Ai, q.kkt.Ax) for q in systems])
+    for _ in range(2):
+        xe = e.Solve(rhs)
+        assert e.stats()["graph_nodes"] == 0
+        for a, b in zip(xe, x0):
+            if a is not None:
+                np.testing.assert_array_equal(a, b)
+    for i, (q, c) in enumerate(zip(systems, counts)):
+        for r in range(c):
+            assert backward_error(oracle, q.kkt, x0[i][:, r], q.B[:, r]) < 1e-13
+    # new values on the same batch: the graph reads them at its next replay
+    s.Compute([(q.kkt.Ap, q.kkt.Ai, q.kkt.Ax * 1.01) for q in systems])
+    s.Solve(rhs)
+    x2 = s.Solve(rhs)
+    assert s.stats()["graph_nodes"] > 0
+    for i, (q, c) in enumerate(zip(systems, counts)):
+        for r in range(c):
+            P = q.kkt
+            rr = q.B[:, r] - 1.01 * oracle.matmul(P.Ap, P.Ai, P.Ax, x2[i][:, r])
+            kn = 1.01 * np.abs(oracle.matmul(P.Ap, P.Ai, np.abs(P.Ax), np.ones(P.n))).max()
+            assert np.abs(rr).max() / (kn * np.abs(x2[i][:, r]).max() + np.abs(q.B[:, r]).max()) < 1e-13
