@@ -57,7 +57,7 @@ def backward_error(P, x) -> float:
     return float(np.abs(r).max() / (knorm * np.abs(x).max() + np.abs(P.b).max()))
 
 
-def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x, ordering, ilu=None):
+def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x, ordering, ilu=None, autotune=False):
     s = mpir.GmresLDLIR(uf, ordering=ordering)
     if ilu is not None:
         s.SetPreconditioner(mpir.PREC_FGPILU, *ilu)
@@ -66,6 +66,12 @@ def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x, order
     s.Analyze(P.Ap, P.Ai)
     analyze_s = time.perf_counter() - t0
     log(f"{np.dtype(uf).name}: analyze {analyze_s:.2f}s, warmup {warmup}")
+    tuned = None
+    if autotune:
+        # setup, not timed: the level-tile autotuner on this pattern's factor
+        s.factor_dev(dev_ax.data_ptr())
+        tuned = s.Autotune(reps=3)
+        log(f"{np.dtype(uf).name}: autotune {tuned}")
     for _ in range(warmup):
         s.factor_dev(dev_ax.data_ptr())
         s.solve_dev(dev_b.data_ptr(), dev_x.data_ptr())
@@ -82,7 +88,7 @@ def run_path(mpir, torch, P, uf, tol, steps, warmup, dev_ax, dev_b, dev_x, order
     total = time.perf_counter() - t_start
     st = s.stats()
     x = dev_x.cpu().numpy()
-    return s, dict(total_s=total, lat_ms=lat, stats=st, analyze_s=analyze_s, x=x, perm=s.perm())
+    return s, dict(total_s=total, lat_ms=lat, stats=st, analyze_s=analyze_s, x=x, perm=s.perm(), tuned=tuned)
 
 
 def pct(v, q):
@@ -525,7 +531,8 @@ def main(argv=None) -> int:
     dev_x = torch.zeros(P.n, dtype=torch.float64, device=dev)
 
     order = {"amd": mpir.ORDER_AMD, "nd": mpir.ORDER_ND}[args.ordering]
-    s32, r32 = run_path(mpir, torch, P, np.float32, args.tol, args.steps, args.warmup, dev_ax, dev_b, dev_x, order)
+    s32, r32 = run_path(mpir, torch, P, np.float32, args.tol, args.steps, args.warmup, dev_ax, dev_b, dev_x, order,
+                        autotune=args.autotune)
     eta32 = backward_error(P, r32["x"])
     st = r32["stats"]
     log(f"f32: {st}")
@@ -547,7 +554,7 @@ def main(argv=None) -> int:
     r64 = None
     if not args.no_fp64:
         s64, r64 = run_path(mpir, torch, P, np.float64, args.tol, max(1, args.steps // 2), args.warmup, dev_ax, dev_b,
-                            dev_x, order)
+                            dev_x, order, autotune=args.autotune)
         r64["eta"] = backward_error(P, r64["x"])
         del s64
 
@@ -574,6 +581,7 @@ def main(argv=None) -> int:
         "ir_iterations": st["ir_iterations"],
         "gmres_iterations": st["gmres_iterations"],
         "kernels": {k: round(v, 4) for k, v in res.items()},
+        "autotune": r32["tuned"],
     }
     if r64 is not None:
         v64 = len(r64["lat_ms"]) / r64["total_s"]
