@@ -83,9 +83,10 @@ Server::Server(const ServeOptions& opt) : opt_(opt) {
     auto s = std::make_unique<GmresLdl>(opt.uf, kF64, kF64);
     s->set_tol(opt.tol);
     s->set_ordering(opt.ordering);
-    // on joined batches the level schedule of the triangular solves is the
-    // faster one (every level holds many fronts)
-    s->set_schedule(kScheduleLevels);
+    // the dependency-driven triangular solves: lanes on other streams
+    // cannot starve them (a workgroup takes the task class of its arrival
+    // order, solve.hip)
+    s->set_schedule(kScheduleDag);
     s->set_max_ir(opt.max_ir);
     s->set_max_gmres(opt.max_gmres);
     solvers_.push_back(std::move(s));
