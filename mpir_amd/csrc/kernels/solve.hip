@@ -115,6 +115,36 @@ __device__ __forceinline__ TX sum_strided(const TX* p, int n, int stride) {
   return s;
 }
 
+// s[r] = sum_{k < n} p[r * rs + k * stride] for NR right-hand sides, each
+// added in order as sum_strided does; the loads of all right-hand sides go
+// out together, 8 at a time (k outer, r inner)
+template <int NR, class TX>
+__device__ __forceinline__ void sum_strided_multi(const TX* p, int n, int stride, int64_t rs, TX (&s)[NR]) {
+#pragma unroll
+  for (int r = 0; r < NR; ++r) s[r] = TX(0);
+  if constexpr (NR == 1) {
+    s[0] = sum_strided(p, n, stride);
+  } else {
+    const int total = n * NR;
+    for (int i0 = 0; i0 < total; i0 += 8) {
+      TX t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = i0 + u, k = i / NR, r = i % NR;
+        t[u] = i < total ? ld_dev(p + r * rs + static_cast<int64_t>(k) * stride) : TX(0);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (i0 + u < total) s[(i0 + u) % NR] += t[u];
+    }
+    // sum_strided adds the zeros of its last partial batch, which turns a
+    // -0 sum into +0: the same here, so the bits agree
+    if (n % 8 != 0)
+#pragma unroll
+      for (int r = 0; r < NR; ++r) s[r] += TX(0);
+  }
+}
+
 // y of column c of a big front: its partial sums over the row chunks
 template <class TX>
 __device__ __forceinline__ TX forward_value(const TX* yp, int nc, int c) {
@@ -122,6 +152,14 @@ __device__ __forceinline__ TX forward_value(const TX* yp, int nc, int c) {
   const int j = c / kSolveRB;
   const int nv = (min(nc, (j + 1) * kSolveRB) + kXcRows - 1) / kXcRows;
   return sum_strided(yp + static_cast<int64_t>(j) * nhx * kSolveRB + (c % kSolveRB), nv, kSolveRB);
+}
+// the same for NR right-hand sides at stride rs
+template <int NR, class TX>
+__device__ __forceinline__ void forward_values(const TX* yp, int64_t rs, int nc, int c, TX (&y)[NR]) {
+  const int nhx = (nc + kXcRows - 1) / kXcRows;
+  const int j = c / kSolveRB;
+  const int nv = (min(nc, (j + 1) * kSolveRB) + kXcRows - 1) / kXcRows;
+  sum_strided_multi<NR>(yp + static_cast<int64_t>(j) * nhx * kSolveRB + (c % kSolveRB), nv, kSolveRB, rs, y);
 }
 
 // The children's contributions to one front row (entries [p0, p1) of the
@@ -145,6 +183,25 @@ struct RowGather {
       v += ld_dev(uvec + e.a) - sum_strided(up + e.b, e.c, kSolveRB);
     }
     return v;
+  }
+  // the same for NR right-hand sides (stride d.su, d.sup), v[r] in and out:
+  // each entry's loads go out for every right-hand side at once, and each
+  // right-hand side adds them in the order value() does
+  template <int NR, class TX>
+  __device__ __forceinline__ void values(const SolveDev& d, const TX* uvec, const TX* up, TX (&v)[NR]) const {
+    if constexpr (NR == 1) {
+      v[0] = value(d, uvec, up, v[0]);
+    } else {
+      for (int p = p0; p < p1; ++p) {
+        const Task4 e = p == p0 ? e0 : p == p0 + 1 ? e1 : d.gent[p];
+        TX u[NR], ps[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) u[r] = ld_dev(uvec + r * d.su + e.a);
+        sum_strided_multi<NR>(up + e.b, e.c, kSolveRB, d.sup, ps);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) v[r] += u[r] - ps[r];
+      }
+    }
   }
 };
 
@@ -244,7 +301,8 @@ __device__ __forceinline__ void small_fwd(const SolveArgs<TF, TD>& a, const Solv
   const TX* up = static_cast<const TX*>(d.upart);
   TX vl[NR];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) vl[r] = G.value(d, uvec + r * d.su, up + r * d.sup, lane < nc ? b[r] : TX(0));
+  for (int r = 0; r < NR; ++r) vl[r] = lane < nc ? b[r] : TX(0);
+  G.values<NR>(d, uvec, up, vl);
   for (int j0 = 0; j0 < nc; j0 += kLC) {
     if (j0 > 0) load(j0);
 #pragma unroll
@@ -538,10 +596,11 @@ __device__ __forceinline__ void fx_task(const SolveArgs<TF, TD>& a, const SolveD
   }
   wait();
   const TX* up = static_cast<const TX*>(d.upart);
+  if (tid < len) G.values<NR>(d, uvec, up, bq);
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     TX* vs = smem + r * kVsLen;
-    vs[tid] = tid < len ? G.value(d, uvec + r * d.su, up + r * d.sup, bq[r]) : TX(0);
+    vs[tid] = tid < len ? bq[r] : TX(0);
     for (int q = tid + blockDim.x; q < kVsLen; q += blockDim.x) vs[q] = TX(0);
   }
   __syncthreads();
@@ -577,12 +636,14 @@ __device__ __forceinline__ void fu_task(const SolveArgs<TF, TD>& a, const SolveD
   TX* upart = static_cast<TX*>(d.upart);
   TX vr[NR];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    TX* ys = smem + r * kVsLen;
-    for (int c = tid; c < kGemvCols; c += blockDim.x)
-      ys[c] = c < cc ? forward_value(yp + r * d.syp, F.nc, cs + c) : TX(0);
-    vr[r] = gathers ? G.value(d, static_cast<const TX*>(uvec) + r * d.su, upart + r * d.sup, TX(0)) : TX(0);
+  for (int r = 0; r < NR; ++r) vr[r] = TX(0);
+  for (int c = tid; c < kGemvCols; c += blockDim.x) {
+    TX yv[NR];
+    if (c < cc) forward_values<NR>(yp, d.syp, F.nc, cs + c, yv);
+#pragma unroll
+    for (int r = 0; r < NR; ++r) smem[r * kVsLen + c] = c < cc ? yv[r] : TX(0);
   }
+  if (gathers) G.values<NR>(d, static_cast<const TX*>(uvec), upart, vr);
   __syncthreads();
   const int nh = (F.nc + kGemvCols - 1) / kGemvCols;
   TX* out = upart + d.upart_off[s] + static_cast<int64_t>(b) * nh * kRB + h * kRB;
@@ -650,19 +711,16 @@ __device__ __forceinline__ void bx_task(const SolveArgs<TF, TD>& a, const SolveD
   const int nhr = (F.nr + kXcRows - 1) / kXcRows;
   const TX* rp = static_cast<const TX*>(d.rpart) + (nhr > 0 ? d.rpart_off[s] : 0);
   const TX* yp = static_cast<const TX*>(d.ypart) + d.ypart_off[s];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    TX* ys = smem + r * kVsLen;
-    for (int c = tid; c < kGemvCols; c += blockDim.x) {
-      TX z = TX(0);
-      if (c < cc) {
-        const int col = cs + c;
-        const TX rs =
-            sum_strided(rp + r * d.srp + static_cast<int64_t>(col / kRB) * nhr * kRB + (col % kRB), nhr, kRB);
-        z = static_cast<TX>(a.Dinv[F.f + col]) * forward_value(yp + r * d.syp, F.nc, col) - rs;
-      }
-      ys[c] = z;
+  for (int c = tid; c < kGemvCols; c += blockDim.x) {
+    TX rs[NR], yv[NR];
+    const int col = cs + c;
+    if (c < cc) {
+      sum_strided_multi<NR>(rp + static_cast<int64_t>(col / kRB) * nhr * kRB + (col % kRB), nhr, kRB, d.srp, rs);
+      forward_values<NR>(yp, d.syp, F.nc, col, yv);
     }
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+      smem[r * kVsLen + c] = c < cc ? static_cast<TX>(a.Dinv[F.f + col]) * yv[r] - rs[r] : TX(0);
   }
   __syncthreads();
   const int nh = (F.nc + kGemvCols - 1) / kGemvCols;
@@ -781,11 +839,16 @@ __device__ __forceinline__ void fm_task(const SolveArgs<TF, TD>& a, const SolveD
   if (tid < rows) G2.pre(d, d.frow_off[s] + F.nc + r0 + tid);
   wait();
   const TX* up = static_cast<const TX*>(d.upart);
+  TX vr[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) vr[r] = TX(0);
+  if (tid < F.nc) G1.values<NR>(d, uvec, up, bq);
+  if (tid < rows) G2.values<NR>(d, uvec, up, vr);
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     TX* vs = smem + r * kVsLen;
-    if (tid < 64) vs[tid] = tid < F.nc ? G1.value(d, uvec + r * d.su, up + r * d.sup, bq[r]) : TX(0);
-    vs[128 + tid] = tid < rows ? G2.value(d, uvec + r * d.su, up + r * d.sup, TX(0)) : TX(0);
+    if (tid < 64) vs[tid] = tid < F.nc ? bq[r] : TX(0);
+    vs[128 + tid] = tid < rows ? vr[r] : TX(0);
   }
   // the L_21 tile is in flight while y is formed
   NT T;
