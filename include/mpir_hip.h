@@ -224,6 +224,19 @@ int mpir_gmres_ldl_set_scaling(mpir_gmres_ldl_t h, int mode);
 #define MPIR_SCHED_LEVELS 0
 #define MPIR_SCHED_DAG 1
 int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule);
+/* schedule of the numeric factorization (no counterpart in the reference,
+ * whose QDLDL_factor is a sequential loop over columns, qdldl.hpp:228-342):
+ * MPIR_SCHED_LEVELS, one launch per level of the supernodal tree and per
+ * panel step, or MPIR_SCHED_DAG (default), the levels above the last small
+ * front in one persistent launch whose tile tasks wait on per-front and
+ * per-column-block completion counters. Both give bit-identical factors. */
+int mpir_gmres_ldl_set_factor_schedule(mpir_gmres_ldl_t h, int schedule);
+/* the factorization schedule of an analyzed handle: the schedule in use,
+ * the first level of the dependency-driven launch (== nlevels: none), the
+ * number of levels and the tasks of that launch (no counterpart in the
+ * reference) */
+int mpir_gmres_ldl_factor_schedule_info(mpir_gmres_ldl_t h, int* schedule, int32_t* dag_level, int32_t* nlevels,
+                                        int64_t* dag_tasks);
 /* preconditioner M of GMRES-IR, before analyze: MPIR_PREC_LDL (default),
  * the L D L^T of the reordered matrix, or MPIR_PREC_FGPILU, the reference's
  * shipped GmresLDLIR::Compute (include/MPIR/gmres_ir.hpp:323-531): fine-grained
@@ -283,6 +296,13 @@ int mpir_gmres_ldl_precond_dev_noalloc(mpir_gmres_ldl_t h, double* x);
  * applied at once to as many vectors as the last batch solve had
  * right-hand sides (up to 8; 1 for a single system) */
 int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);
+/* One factorization of the current values with a timeline of the
+ * dependency-driven launch (no counterpart in the reference): per task, in
+ * list order, 4 entries: kind << 28 | supernode; the device wall clock
+ * (100 MHz) when a workgroup drew it, when its waits ended, and when it was
+ * counted (shifted left by 16, the workgroup index in the low 16 bits).
+ * Writes at most cap / 4 tasks; *ntask receives the number of tasks. */
+int mpir_gmres_ldl_trace_factor(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap, int64_t* ntask);
 /* Autotuner of the level-tile schedules (no counterpart in the reference,
  * whose solver has no device schedule). On a factored L D L^T handle (single
  * system or batch) it times, with HIP events over `reps` runs (1..100) each:

@@ -224,6 +224,19 @@ class GmresLDLIR:
         """SCHED_DAG (default) or SCHED_LEVELS for the triangular solves of M^{-1}."""
         _check(load().mpir_gmres_ldl_set_schedule(self._h, int(schedule)), "SetSolveSchedule")
 
+    def SetFactorSchedule(self, schedule: int) -> None:
+        """SCHED_DAG (default: the levels above the small fronts in one
+        dependency-driven launch) or SCHED_LEVELS (one launch per level and
+        panel step) for the numeric factorization; same factor bit for bit."""
+        _check(load().mpir_gmres_ldl_set_factor_schedule(self._h, int(schedule)), "SetFactorSchedule")
+
+    def factor_schedule_info(self) -> dict:
+        """schedule, first level of the dependency-driven launch, levels, tasks of that launch."""
+        sc, lv, nl, nt = C.c_int(), C.c_int32(), C.c_int32(), C.c_int64()
+        _check(load().mpir_gmres_ldl_factor_schedule_info(self._h, C.byref(sc), C.byref(lv), C.byref(nl), C.byref(nt)),
+               "factor_schedule_info")
+        return {"schedule": sc.value, "dag_level": lv.value, "nlevels": nl.value, "dag_tasks": nt.value}
+
     # phases
     def Analyze(self, Ap, Ai) -> None:
         Ap = np.ascontiguousarray(Ap, dtype=np.int32)
@@ -314,6 +327,16 @@ class GmresLDLIR:
                "export_ilu")
         return Up, Ui[:m], Ux[:m], Lx[:m], UDinv
 
+    def trace_factor(self) -> np.ndarray:
+        """One factorization with the timeline of its dependency-driven launch:
+        an (ntask, 4) uint64 array (kind << 28 | supernode, drawn, waits
+        ended, counted << 16 | workgroup; device clock at 100 MHz)."""
+        nt = C.c_int64()
+        _check(load().mpir_gmres_ldl_trace_factor(self._h, None, 0, C.byref(nt)), "trace_factor")
+        out = np.zeros((max(nt.value, 1), 4), dtype=np.uint64)
+        _check(load().mpir_gmres_ldl_trace_factor(self._h, _ptr(out), out.size, C.byref(nt)), "trace_factor")
+        return out[:nt.value]
+
     def time_kernels(self, what: str, reps: int = 10) -> float:
         """Average ms of one kernel group ('spmv', 'precond', 'factor') via HIP events."""
         code = {"spmv": 0, "precond": 1, "factor": 2, "precond_multi": 3}[what]
@@ -384,6 +407,10 @@ class BatchLDLIR:
     def SetSolveSchedule(self, schedule: int) -> None:
         """SCHED_LEVELS (the default for batches) or SCHED_DAG."""
         self._s.SetSolveSchedule(schedule)
+
+    def SetFactorSchedule(self, schedule: int) -> None:
+        """SCHED_DAG (default) or SCHED_LEVELS for the batch's factorization."""
+        self._s.SetFactorSchedule(schedule)
 
     def Analyze(self, patterns) -> None:
         """patterns: list of (Ap, Ai), upper CSC of each system."""

@@ -87,6 +87,9 @@ SIGNATURES = {
     "mpir_gmres_ldl_set_ordering": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_scaling": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_schedule": (C.c_int, [P, C.c_int]),
+    "mpir_gmres_ldl_set_factor_schedule": (C.c_int, [P, C.c_int]),
+    "mpir_gmres_ldl_factor_schedule_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                                  C.POINTER(C.c_int64)]),
     "mpir_gmres_ldl_set_preconditioner": (C.c_int, [P, C.c_int, i32, i32]),
     "mpir_gmres_ldl_ilu_nnz": (C.c_int, [P, P]),
     "mpir_gmres_ldl_export_ilu": (C.c_int, [P, P, P, P, P, P]),
@@ -107,6 +110,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_precond_multi_dev": (C.c_int, [P, P, i32]),
     "mpir_gmres_ldl_stream": (P, [P]),
     "mpir_gmres_ldl_time": (C.c_int, [P, C.c_int, C.c_int, P]),
+    "mpir_gmres_ldl_trace_factor": (C.c_int, [P, P, C.c_int64, C.POINTER(C.c_int64)]),
     "mpir_gmres_ldl_autotune": (C.c_int, [P, C.c_int, P, P]),
     "mpir_gmres_ldl_set_tuning": (C.c_int, [P, i32, i32, i32]),
     "mpir_batch_graph": (C.c_int, [P, C.c_int, P]),

@@ -93,7 +93,9 @@ int64_t pattern_footprint(const PatternEntry& e, int factor_bytes, int nrhs, int
   b += (P.nypart + P.nupart + P.nrpart + P.nxpart) * 8;                   // solve partial sums
   b += static_cast<int64_t>(nrhs) * n * 8 * (max_gmres + 1 + 6);          // basis and work vectors
   b += static_cast<int64_t>(P.pan_task.size() + P.upd_task.size() + P.asm_task.size()) * 16 +
-       static_cast<int64_t>(P.dag_fwd.size() + P.dag_bwd.size() + P.gent.size()) * 16;  // schedules
+       static_cast<int64_t>(P.dag_fwd.size() + P.dag_bwd.size() + P.gent.size()) * 16 +  // schedules
+       static_cast<int64_t>(P.fdag.size()) * static_cast<int64_t>(sizeof(FTask)) +
+       (3 * static_cast<int64_t>(P.sn.ns) + 2 * P.fdag_nsteps + P.fdag_ncb) * 4;  // factor task list, counters
   return b;
 }
 
@@ -442,6 +444,7 @@ void merge_plans(const std::vector<const FactorPlan*>& P, const std::vector<int3
   M.gent[E.gent + 1] = Task4{0, 0, 0, 0};
   M.nsmall = static_cast<int64_t>(M.small_list.size());
   M.nbig = static_cast<int64_t>(M.big_list.size());
+  build_factor_dag(M);
   build_dag_schedule(M);
 }
 

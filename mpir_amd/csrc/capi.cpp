@@ -620,6 +620,27 @@ int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule) {
     return 0;
   });
 }
+int mpir_gmres_ldl_set_factor_schedule(mpir_gmres_ldl_t h, int schedule) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->set_factor_schedule(schedule);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_factor_schedule_info(mpir_gmres_ldl_t h, int* schedule, int32_t* dag_level, int32_t* nlevels,
+                                        int64_t* dag_tasks) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    int32_t lv = 0, nl = 0;
+    int64_t nt = 0;
+    h->s->factor_schedule_info(&lv, &nl, &nt);
+    if (schedule) *schedule = h->s->factor_schedule();
+    if (dag_level) *dag_level = lv;
+    if (nlevels) *nlevels = nl;
+    if (dag_tasks) *dag_tasks = nt;
+    return 0;
+  });
+}
 
 int mpir_gmres_ldl_set_ordering(mpir_gmres_ldl_t h, int ordering) {
   return guarded([&] {
@@ -1001,6 +1022,14 @@ int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms) 
     MPIR_HANDLE_CHECK;
     MPIR_CHECK(avg_ms, "null output");
     *avg_ms = h->s->time_kernels(what, reps);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_trace_factor(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap, int64_t* ntask) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(ntask && (out || cap == 0) && cap >= 0, "bad trace buffer");
+    *ntask = h->s->trace_factor(out, cap);
     return 0;
   });
 }

@@ -651,6 +651,29 @@ void GmresLdl::upload_plan() {
   upload_vec(keep_[16], P.asm_child, st);
   dfp_.asm_child_ptr = keep_[15].as<int32_t>();
   dfp_.asm_child = keep_[16].as<int32_t>();
+  // dependency-driven factorization of the levels >= fdag_level
+  dfp_.use_dag = factor_schedule_ == kScheduleDag;
+  dfp_.dag_pool = P.dag_pool;
+  dfp_.dag_level = P.fdag_level;
+  {
+    FactorDagDev& g = dfp_.dag;
+    g = FactorDagDev();
+    upload_vec(d_fdag_, P.fdag, st);
+    upload_vec(d_fdag_meta_, P.fdag_meta, st);
+    g.task = d_fdag_.as<FTask>();
+    g.ntask = static_cast<int>(P.fdag.size());
+    g.meta = d_fdag_meta_.as<int32_t>();
+    g.asm_task = dfp_.asm_task;
+    g.asm_child_ptr = dfp_.asm_child_ptr;
+    g.asm_child = dfp_.asm_child;
+    g.ns = S.ns;
+    g.nsteps = P.fdag_nsteps;
+    const FactorDagIndex X{S.ns, P.fdag_nsteps};
+    g.cnt_disp = (X.col_done(P.fdag_ncb) + kDispenserStride - 1) / kDispenserStride * kDispenserStride;
+    g.ncounters = g.cnt_disp + kDispenserLines * kDispenserStride;
+    d_fcounters_.alloc(static_cast<size_t>(g.ncounters) * sizeof(int));
+    g.counters = d_fcounters_.as<int>();
+  }
   // triangular solves
   dsp_ = DeviceSolvePlan();
   dsp_.schedule = schedule_;
@@ -912,8 +935,11 @@ void GmresLdl::factor_loaded(double t0) {
   std::memcpy(&nonfinite, &h_pinned_[2], sizeof(int));
   stats_.positive_pivots = pos;
   stats_.factor_ms = now_ms() - t0;
-  if (batch_) batch_collect_pivots(err != 0);
-  if (err) throw Error(kErrNumeric, "zero pivot in the LDL^T factorization");
+  // bit 2: a dependency wait of the factorization expired (its factor is
+  // then incomplete); bit 1: an exact zero pivot
+  if (err & 2) throw Error(kErrInternal, "a dependency wait of the LDL^T factorization expired");
+  if (batch_) batch_collect_pivots((err & 1) != 0);
+  if (err & 1) throw Error(kErrNumeric, "zero pivot in the LDL^T factorization");
   // QDLDL_factor has no such check (qdldl.hpp:331 tests for an exact zero
   // only): a non-finite factor would turn every later solve into NaN
   if (nonfinite && (uf_ == kF16 || uf_ == kBF16) && precond_ != kPrecIlu)
@@ -1284,6 +1310,28 @@ void GmresLdl::solve(const double* b, double* x, int nrhs, int64_t ld, bool devi
   stats_.solve_ms = now_ms() - t0;
 }
 
+int64_t GmresLdl::trace_factor(uint64_t* out, int64_t cap) {
+  MPIR_CHECK(analyzed_ && !precond_only_ && precond_ == kPrecLdl, "trace_factor needs an analyzed L D L^T handle with values");
+  const int64_t nt = dfp_.dag.ntask;
+  if (cap == 0 || nt == 0) return nt;
+  hipStream_t st = stream_;
+  d_ftrace_.ensure(static_cast<size_t>(nt) * 4 * sizeof(uint64_t));
+  MPIR_HIP(hipMemsetAsync(d_ftrace_.get(), 0, static_cast<size_t>(nt) * 4 * sizeof(uint64_t), st));
+  dfp_.dag.trace = d_ftrace_.as<unsigned long long>();
+  try {
+    if (uf_ == kF64) factor_numeric<double>(factor_args<double>(), dfp_, d_axs_.as<double>(), nnz_, 0, st);
+    else factor_numeric<float>(factor_args<float>(), dfp_, d_axs_.as<double>(), nnz_, rounding(), st);
+  } catch (...) {
+    dfp_.dag.trace = nullptr;
+    throw;
+  }
+  dfp_.dag.trace = nullptr;
+  const int64_t n = std::min(nt, cap / 4) * 4;
+  MPIR_HIP(hipMemcpyAsync(out, d_ftrace_.get(), static_cast<size_t>(n) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  MPIR_HIP(hipStreamSynchronize(st));
+  return nt;
+}
+
 double GmresLdl::time_kernels(int what, int reps) {
   if (!factored_) throw Error(kErrState, "time_kernels needs a factorization");
   MPIR_CHECK(reps >= 1 && what >= 0 && what <= 3, "bad timing request");
@@ -1356,6 +1404,25 @@ void GmresLdl::apply_tuning() {
   FactorPlan& P = *plan_;
   if (!same_tuning(P.ftuning, ftune_)) build_factor_steps(P, ftune_);
   if (!same_tuning(P.tuning, stune_)) build_solve_schedule(P, stune_);
+  if (factor_schedule_ == kScheduleDag && !P.dag_pool) build_update_pool(P, true);
+}
+
+void GmresLdl::set_factor_schedule(int sched) {
+  MPIR_CHECK(sched == kScheduleLevels || sched == kScheduleDag, "factor schedule must be 0 (levels) or 1 (dag)");
+  factor_schedule_ = sched;
+  dfp_.use_dag = sched == kScheduleDag;
+  if (sched == kScheduleDag && plan_ && analyzed_ && !plan_->dag_pool) {
+    // the analyzed plan's update pool reuses blocks across levels: lay it
+    // out again for the dependency-driven launch (the factor must be
+    // computed again)
+    build_update_pool(*plan_, true);
+    upload_vec(d_upd_off_, plan_->upd_off, stream_);
+    const size_t tf = uf_ == kF64 ? sizeof(double) : sizeof(float);
+    d_upool_.alloc(std::max<int64_t>(1, plan_->upd_pool) * tf);
+    dfp_.dag_pool = true;
+    factored_ = false;
+    MPIR_HIP(hipStreamSynchronize(stream_));
+  }
 }
 
 void GmresLdl::set_tuning(const FactorTuning& f, const SolveTuning& s) {

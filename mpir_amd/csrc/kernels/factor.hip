@@ -26,18 +26,45 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
 #include "../common.hpp"
 #include "../plan.hpp"
 #include "kernels.hpp"
+#include "sync.hpp"
 
 namespace mpir {
 
 namespace {
 
 __device__ __forceinline__ void flag_zero_pivot(int* err) { atomicOr(err, 1); }
+
+// Memory accesses of the task bodies: plain for the per-level launches
+// (kernel boundaries order them), device-coherent inside the
+// dependency-driven launch, where another workgroup of the same launch
+// wrote the values.
+struct PlainMem {
+  template <class T>
+  static __device__ __forceinline__ T ld(const T* p) {
+    return *p;
+  }
+  template <class T>
+  static __device__ __forceinline__ void st(T* p, T v) {
+    *p = v;
+  }
+};
+struct CoherentMem {
+  template <class T>
+  static __device__ __forceinline__ T ld(const T* p) {
+    return ld_dev(p);
+  }
+  template <class T>
+  static __device__ __forceinline__ void st(T* p, T v) {
+    st_dev(p, v);
+  }
+};
 
 // Rounding of the factor's entries when the factor is held in a 16-bit
 // format: R = 0 none, 1 to FP16, 2 to BF16 (round to nearest even). The
@@ -144,12 +171,13 @@ __global__ void __launch_bounds__(64) k_small_fronts(FactorArgs<T> a, const int3
 // update matrices are added in fixed order (a barrier between children, no
 // two entries of one child land on the same place), then the tile is
 // written back once.
-template <class T, int R>
-__global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* __restrict__ tasks,
-                                                  const int32_t* __restrict__ cptr, const int32_t* __restrict__ cl) {
+// Children entries [e0, e1) of cl; tile: E x (E + 1) entries of LDS. A
+// child's entries are loaded KB per thread at a time (no two land on the same
+// place, so the rounds do not change any sum).
+template <class T, int R, class M, int KB = kAsmTile * kAsmTile / 256>
+__device__ __forceinline__ void asm_body(const FactorArgs<T>& a, const Task4 tk, int e0, int e1,
+                                         const int32_t* __restrict__ cl, T (*tile)[kAsmTile + 1]) {
   constexpr int E = kAsmTile;
-  __shared__ T tile[E][E + 1];  // tile[row - t0][col - c0]
-  const Task4 tk = tasks[blockIdx.x];
   const int s = tk.a, c0 = tk.b, c1 = tk.c, t0 = tk.d;
   const int f = a.first[s];
   const int nc = a.first[s + 1] - f;
@@ -167,41 +195,52 @@ __global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* 
     for (int q = 0; q < E / 4; ++q) {
       const int c = c0 + w + 4 * q;
       const bool in = c < c1 && r < t1 && r >= c && c < nc;
-      v[q] = *(in ? P + r + c * ld : P);
+      v[q] = M::ld(in ? P + r + c * ld : P);
       tile[i][w + 4 * q] = in ? v[q] : T(0);
     }
   }
   __syncthreads();
-  for (int e = cptr[blockIdx.x]; e < cptr[blockIdx.x + 1]; ++e) {
+  for (int e = e0; e < e1; ++e) {
     const int ch = cl[5 * e], b0 = cl[5 * e + 1], b1 = cl[5 * e + 2], a0 = cl[5 * e + 3], a1 = cl[5 * e + 4];
     const int nrc = a.nrow[ch];
     const T* Uc = a.upool + a.upd_off[ch];
     const int32_t* rel = a.relidx + a.rows_off[ch];
     const int na = a1 - a0, npair = (b1 - b0) * na;
-    T v[E * E / 256];
-    int dr[E * E / 256], dc[E * E / 256];
+    static_assert((E * E / 256) % KB == 0, "assembly rounds");
 #pragma unroll
-    for (int k = 0; k < E * E / 256; ++k) {
-      const int p = tid + 256 * k;
-      const int b = b0 + (p < npair ? p / na : 0);
-      const int q = a0 + (p < npair ? p % na : 0);
-      v[k] = Uc[q + static_cast<int64_t>(b) * nrc];
-      dr[k] = (p < npair && q >= b) ? rel[q] - t0 : -1;
-      dc[k] = rel[b] - c0;
+    for (int k0 = 0; k0 < E * E / 256; k0 += KB) {
+      T v[KB];
+      int dr[KB], dc[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        const int p = tid + 256 * (k0 + k);
+        const int b = b0 + (p < npair ? p / na : 0);
+        const int q = a0 + (p < npair ? p % na : 0);
+        v[k] = M::ld(Uc + q + static_cast<int64_t>(b) * nrc);
+        dr[k] = (p < npair && q >= b) ? rel[q] - t0 : -1;
+        dc[k] = rel[b] - c0;
+      }
+#pragma unroll
+      for (int k = 0; k < KB; ++k)
+        if (dr[k] >= 0) tile[dr[k]][dc[k]] += v[k];
     }
-#pragma unroll
-    for (int k = 0; k < E * E / 256; ++k)
-      if (dr[k] >= 0) tile[dr[k]][dc[k]] += v[k];
     __syncthreads();
   }
 #pragma unroll
   for (int q = 0; q < E / 4; ++q) {
     const int c = c0 + w + 4 * q;
     if (c < c1 && r < t1 && r >= c) {
-      if (c < nc) P[r + c * ld] = tile[i][w + 4 * q];
-      else U[(r - nc) + static_cast<int64_t>(c - nc) * nr] = tile[i][w + 4 * q];
+      if (c < nc) M::st(P + r + c * ld, tile[i][w + 4 * q]);
+      else M::st(U + (r - nc) + static_cast<int64_t>(c - nc) * nr, tile[i][w + 4 * q]);
     }
   }
+}
+
+template <class T, int R>
+__global__ void __launch_bounds__(256) k_assemble(FactorArgs<T> a, const Task4* __restrict__ tasks,
+                                                  const int32_t* __restrict__ cptr, const int32_t* __restrict__ cl) {
+  __shared__ T tile[kAsmTile][kAsmTile + 1];  // tile[row - t0][col - c0]
+  asm_body<T, R, PlainMem>(a, tasks[blockIdx.x], cptr[blockIdx.x], cptr[blockIdx.x + 1], cl, tile);
 }
 
 // Matrix-core product for the trailing updates: v_mfma 16x16x4 in the
@@ -466,19 +505,17 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
 // m .. m+nc) and only panel columns. Each wave computes a 32 x 32 quarter
 // with 2 x 2 MFMA blocks; the next step's operands are loaded into registers
 // while the current step's products run.
-template <class T, int R>
-__global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
+// Operand tiles Ls[0][t][i] = L[r0 + i, k0 + t] and Ls[1][t][j] =
+// L[c0 + j, k0 + t] D[k0 + t], stored at column i ^ 16 (t mod 4): the four
+// 16-lane groups of an MFMA operand read (rows t0 .. t0 + 3) fall in
+// distinct LDS banks without padding (32 KB in FP32: five workgroups per CU).
+__device__ __forceinline__ int op_at(int t, int i) { return t * kTile + (i ^ ((t & 3) << 4)); }
+
+// Ls: 2 x kPanelNB x kTile entries of LDS.
+template <class T, int R, class M>
+__device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk, T (*Ls)[kPanelNB * kTile]) {
   using V = typename Mfma16<T>::V;
-  // Operand tiles Ls[0][t][i] = L[r0 + i, k0 + t] and Ls[1][t][j] =
-  // L[c0 + j, k0 + t] D[k0 + t], stored at column i ^ 16 (t mod 4): the four
-  // 16-lane groups of an MFMA operand read (rows t0 .. t0 + 3) fall in
-  // distinct LDS banks without padding (32 KB in FP32: five workgroups per CU).
-  __shared__ T Ls[2][kPanelNB * kTile];
-  auto at = [](int t, int i) { return t * kTile + (i ^ ((t & 3) << 4)); };
-  // Workgroups are dealt round-robin over the XCDs (blockIdx % kXcds); each
-  // XCD takes a contiguous run of the task list instead, so tiles that share
-  // operand rows and columns meet in the same L2.
-  const Task4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
+  auto at = [](int t, int i) { return op_at(t, i); };
   const int s = tk.a, r0 = tk.b, c0 = tk.c;
   const int klast = tk.d & ((1 << kUpdStepsShift) - 1);
   const int kfirst = klast - ((tk.d >> kUpdStepsShift) & ((1 << (kUpdPanelBit - kUpdStepsShift)) - 1)) + 1;
@@ -505,9 +542,9 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int t = min(w + 4 * q, nbk - 1);
-      av[q] = P[ras + (k0 + t) * ld];
-      bv[q] = P[rbs + (k0 + t) * ld];
-      dv[q] = a.D[f + k0 + t];
+      av[q] = M::ld(P + ras + (k0 + t) * ld);
+      bv[q] = M::ld(P + rbs + (k0 + t) * ld);
+      dv[q] = M::ld(a.D + f + k0 + t);
     }
   };
   const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
@@ -564,12 +601,21 @@ __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Tas
     const int c = c0 + w + 4 * q;
     const bool ok = c < m && r < rlim && r >= c && (c < nc || (r < m && !panel_only));
     T* p = !ok ? Pw : c < nc ? Pw + r + c * ld : U + (r - nc) + static_cast<int64_t>(c - nc) * nr;
-    cur[q] = *p;
+    cur[q] = M::ld(p);
     dst[q] = ok ? p : nullptr;
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q)
-    if (dst[q] != nullptr) *dst[q] = cur[q] - St[w + 4 * q][lane];
+    if (dst[q] != nullptr) M::st(dst[q], cur[q] - St[w + 4 * q][lane]);
+}
+
+template <class T, int R>
+__global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
+  __shared__ T Ls[2][kPanelNB * kTile];
+  // Workgroups are dealt round-robin over the XCDs (blockIdx % kXcds); each
+  // XCD takes a contiguous run of the task list instead, so tiles that share
+  // operand rows and columns meet in the same L2.
+  upd_body<T, R, PlainMem>(a, tasks[xcd_task(blockIdx.x, gridDim.x)], Ls);
 }
 
 // Trailing update tile for a factor held in a 16-bit format (R = 1 FP16,
@@ -602,15 +648,18 @@ struct Mfma16x32<2> {
   }
 };
 
-template <int R>
-__global__ void __launch_bounds__(256) k_update_tiles16(FactorArgs<float> a, const Task4* __restrict__ tasks) {
+constexpr int kLs16 = kPanelNB + 8;  // row stride (entries) of the 16-bit operand tiles
+
+// Ls: 2 x kTile x kLs16 entries of LDS (16-byte aligned); Ls[0][i * LS + t] =
+// L[r0 + i, k0 + t], Ls[1]: L[c0 + j, k0 + t] D.
+template <int R, class Mem>
+__device__ __forceinline__ void upd16_body(const FactorArgs<float>& a, const Task4 tk,
+                                           typename Mfma16x32<R>::E (*Ls)[kTile * kLs16]) {
   using M = Mfma16x32<R>;
   using E = typename M::E;
   using V8 = typename M::V8;
-  constexpr int LS = kPanelNB + 8;  // row stride (entries)
+  constexpr int LS = kLs16;
   static_assert(kTile * (kTile + 1) * sizeof(float) <= 2 * kTile * LS * sizeof(E), "staging tile exceeds the operands");
-  __shared__ __align__(16) E Ls[2][kTile * LS];  // Ls[0][i * LS + t] = L[r0 + i, k0 + t], Ls[1]: L[c0 + j, k0 + t] D
-  const Task4 tk = tasks[xcd_task(blockIdx.x, gridDim.x)];
   const int s = tk.a, r0 = tk.b, c0 = tk.c;
   const int klast = tk.d & ((1 << kUpdStepsShift) - 1);
   const int kfirst = klast - ((tk.d >> kUpdStepsShift) & ((1 << (kUpdPanelBit - kUpdStepsShift)) - 1)) + 1;
@@ -634,9 +683,9 @@ __global__ void __launch_bounds__(256) k_update_tiles16(FactorArgs<float> a, con
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int t = min(w + 4 * q, nbk - 1);
-      av[q] = P[ras + (k0 + t) * ld];
-      bv[q] = P[rbs + (k0 + t) * ld];
-      dv[q] = a.D[f + k0 + t];
+      av[q] = Mem::ld(P + ras + (k0 + t) * ld);
+      bv[q] = Mem::ld(P + rbs + (k0 + t) * ld);
+      dv[q] = Mem::ld(a.D + f + k0 + t);
     }
   };
   const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
@@ -690,12 +739,341 @@ __global__ void __launch_bounds__(256) k_update_tiles16(FactorArgs<float> a, con
     const int c = c0 + w + 4 * q;
     const bool ok = c < m && r < rlim && r >= c && (c < nc || (r < m && !panel_only));
     float* p = !ok ? Pw : c < nc ? Pw + r + c * ld : U + (r - nc) + static_cast<int64_t>(c - nc) * nr;
-    cur[q] = *p;
+    cur[q] = Mem::ld(p);
     dst[q] = ok ? p : nullptr;
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q)
-    if (dst[q] != nullptr) *dst[q] = cur[q] - St[w + 4 * q][lane];
+    if (dst[q] != nullptr) Mem::st(dst[q], cur[q] - St[w + 4 * q][lane]);
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) k_update_tiles16(FactorArgs<float> a, const Task4* __restrict__ tasks) {
+  __shared__ __align__(16) typename Mfma16x32<R>::E Ls[2][kTile * kLs16];
+  upd16_body<R, PlainMem>(a, tasks[xcd_task(blockIdx.x, gridDim.x)], Ls);
+}
+
+// ---------------------------------------------------------------- dependency-driven factorization
+// Diagonal block of panel step k of big front s, once, by 256 threads
+// (thread (row lane, wave w) holds columns 16 w .. 16 w + 15): the
+// elimination of k_panel_step's owning task, step for step (4 columns per
+// barrier, the wave holding them factors the 4 x 4 pivot block and publishes
+// the multipliers; every entry takes the same operations in the same order,
+// so the results are those of the per-level launches bit for bit). Writes
+// L_kk (transposed, in the block's upper triangle), D, D^{-1} and
+// Z_kk = L_kk^{-T} D_k^{-1} in the block's identity rows, with its zeros as
+// the per-level launches leave them (the product I Z_kk: +0).
+// lds: 6 * 64 * 4 + 64 * 65 entries.
+template <class T, int R>
+__device__ __forceinline__ void diag_body(const FactorArgs<T>& a, int s, int k, T* __restrict__ lds) {
+  constexpr int NB = kPanelNB;
+  constexpr int CW = 4;       // columns per elimination step
+  constexpr int QW = NB / 4;  // columns per wave
+  T(*mrs)[NB][CW] = reinterpret_cast<T(*)[NB][CW]>(lds);
+  T(*gms)[NB][CW] = mrs + 2;
+  T(*dms)[NB][CW] = mrs + 4;
+  T(*Xs)[NB + 1] = reinterpret_cast<T(*)[NB + 1]>(lds + 6 * NB * CW);
+  const int f = a.first[s];
+  const int nc = a.first[s + 1] - f;
+  const int m = nc + a.nrow[s];
+  const int64_t ld = panel_ld(nc, a.nrow[s]);
+  const int k0 = k * NB;
+  const int nbk = min(NB, nc - k0);
+  T* P = a.panel + a.panel_off[s];
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int c0w = QW * w;
+  const int lsafe = min(lane, nbk - 1);
+  T fa[QW], ga[QW];
+#pragma unroll
+  for (int u = 0; u < QW; ++u) {
+    const int c = c0w + u;
+    const int cs = min(c, nbk - 1);
+    const T v = ld_dev(P + (k0 + lsafe) + (k0 + cs) * ld);
+    const bool in = lane < nbk && c < nbk && lane >= c;
+    fa[u] = in ? v : T(0);
+    ga[u] = (lane == c && c < nbk) ? T(1) : T(0);
+  }
+  for (int wo = 0; wo < 4; ++wo) {
+#pragma unroll
+    for (int jj = 0; jj < 4; ++jj) {
+      const int J = 4 * wo + jj;
+      const int b = J & 1;
+      const int j0 = CW * J;
+      if (w == wo) {
+        bool valid[CW];
+#pragma unroll
+        for (int t = 0; t < CW; ++t) valid[t] = j0 + t < nbk;
+        // pivot block L_B D_B L_B^T; dl[t][q] = d_q l_tq, dinv = 1 / d
+        T d[CW], dinv[CW], l[CW][CW], dl[CW][CW];
+#pragma unroll
+        for (int t = 0; t < CW; ++t) {
+          T dt = lane_value(fa[4 * jj + t], j0 + t);
+#pragma unroll
+          for (int q = 0; q < t; ++q) dt -= l[t][q] * dl[t][q];
+          d[t] = valid[t] ? dt : T(1);
+          dinv[t] = T(1) / d[t];
+#pragma unroll
+          for (int u = t + 1; u < CW; ++u) {
+            T v = lane_value(fa[4 * jj + t], min(j0 + u, NB - 1));
+#pragma unroll
+            for (int q = 0; q < t; ++q) v -= l[u][q] * dl[t][q];
+            v *= dinv[t];
+            l[u][t] = (valid[u] && valid[t]) ? v : T(0);
+            dl[u][t] = d[t] * l[u][t];
+          }
+        }
+        // multipliers of this lane's row and identity row
+        T mr[CW], gm[CW];
+#pragma unroll
+        for (int t = 0; t < CW; ++t) {
+          T v = fa[4 * jj + t];
+          T g = ga[4 * jj + t];
+#pragma unroll
+          for (int q = 0; q < t; ++q) {
+            v -= mr[q] * dl[t][q];
+            g -= gm[q] * dl[t][q];
+          }
+          mr[t] = v * dinv[t];
+          gm[t] = g * dinv[t];
+        }
+#pragma unroll
+        for (int t = 0; t < CW; ++t) {
+          mrs[b][lane][t] = mr[t];
+          gms[b][lane][t] = gm[t];
+          dms[b][lane][t] = d[t] * mr[t];
+        }
+        // columns of L_kk (and D on the diagonal) into LDS
+        const int s0 = lane - j0;
+#pragma unroll
+        for (int t = 0; t < CW; ++t) {
+          T lv = T(0);
+#pragma unroll
+          for (int u = t + 1; u < CW; ++u) lv = s0 == u ? l[u][t] : lv;
+          Xs[j0 + t][lane] = s0 < t ? T(0) : s0 == t ? d[t] : s0 < CW ? lv : mr[t];
+        }
+#pragma unroll
+        for (int t = 0; t < CW; ++t)
+          if (valid[t]) ga[4 * jj + t] = gm[t];
+      }
+      __syncthreads();
+      // rank-CW update of the columns right of the step's block
+      if (w >= wo) {
+        T mr[CW], gm[CW];
+#pragma unroll
+        for (int t = 0; t < CW; ++t) {
+          mr[t] = mrs[b][lane][t];
+          gm[t] = gms[b][lane][t];
+        }
+#pragma unroll
+        for (int u = 0; u < QW; ++u) {
+          if (w == wo && u < 4 * jj + 4) continue;
+          const int c = c0w + u;
+          T dm[CW];
+#pragma unroll
+          for (int t = 0; t < CW; ++t) dm[t] = dms[b][c][t];
+          T fu = T(0), gu = T(0);
+#pragma unroll
+          for (int t = 0; t < CW; ++t) {
+            fu += mr[t] * dm[t];
+            gu += gm[t] * dm[t];
+          }
+          const bool on = c < nbk;
+          fa[u] = (on && lane >= c) ? fa[u] - fu : fa[u];
+          ga[u] = on ? ga[u] - gu : ga[u];
+          // four columns at a time: the LDS reads of all sixteen in flight
+          // at once would take 64 more registers
+          if ((u & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int rr = w; rr < nbk; rr += 4)
+    if (lane < rr) st_dev(P + (k0 + lane) + (k0 + rr) * ld, rnd<R>(Xs[lane][rr]));
+  if (tid < nbk) {
+    const T d = Xs[tid][tid];
+    if (d == T(0)) flag_zero_pivot(a.err);
+    st_dev(a.D + f + k0 + tid, d);
+    st_dev(a.Dinv + f + k0 + tid, T(1) / d);
+  }
+#pragma unroll
+  for (int u = 0; u < QW; ++u) {
+    const int c = c0w + u;
+    if (lane < nbk && c < nbk) st_dev(P + (m + k0 + lane) + (k0 + c) * ld, ga[u] + T(0));
+  }
+}
+
+// Rows [r0, r1) (at most 64) of panel step k of big front s: L = X Z_kk on the
+// matrix cores, X = the rows' current entries in the step's columns (rows
+// below the diagonal block, or identity rows of earlier blocks), Z_kk from
+// the block's identity rows. Every output entry takes the MFMA sequence of
+// k_panel_step (products over t = 0, 4, 8, ... in order). Ls: as upd_body.
+template <class T, int R>
+__device__ __forceinline__ void pan_body(const FactorArgs<T>& a, int s, int r0, int r1, int k,
+                                         T (*Ls)[kPanelNB * kTile]) {
+  using V = typename Mfma16<T>::V;
+  const int f = a.first[s];
+  const int nc = a.first[s + 1] - f;
+  const int m = nc + a.nrow[s];
+  const int64_t ld = panel_ld(nc, a.nrow[s]);
+  const int k0 = k * kPanelNB;
+  const int nbk = min(kPanelNB, nc - k0);
+  T* P = a.panel + a.panel_off[s];
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int rows = r1 - r0;
+  const int r = r0 + lane;
+  const int64_t rs = lane < rows ? r : r0;
+  const int zl = min(lane, nbk - 1);
+  // thread (lane, t = w + 4 q): X[r0 + lane][t] and Z_kk[lane][t]
+  T xv[16], zv[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int ts = min(w + 4 * q, nbk - 1);
+    xv[q] = ld_dev(P + rs + (k0 + ts) * ld);
+    zv[q] = ld_dev(P + (m + k0 + zl) + (k0 + ts) * ld);
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int t = w + 4 * q;
+    Ls[0][op_at(t, lane)] = (lane < rows && t < nbk) ? xv[q] : T(0);   // X^T: [t][row]
+    Ls[1][op_at(lane, t)] = (lane < nbk && t < nbk) ? zv[q] : T(0);    // Z: [row][column]
+  }
+  __syncthreads();
+  const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  V acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[x][y][v] = T(0);
+  for (int t0 = 0; t0 < nbk; t0 += 4) {
+    const int t = t0 + lk;
+    const T a0 = Ls[0][op_at(t, wr + li)], a1 = Ls[0][op_at(t, wr + 16 + li)];
+    const T b0 = Ls[1][op_at(t, wc + li)], b1 = Ls[1][op_at(t, wc + 16 + li)];
+    acc[0][0] = Mfma16<T>::op(a0, b0, acc[0][0]);
+    acc[0][1] = Mfma16<T>::op(a0, b1, acc[0][1]);
+    acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
+    acc[1][1] = Mfma16<T>::op(a1, b1, acc[1][1]);
+  }
+  __syncthreads();
+  T(*St)[kTile + 1] = reinterpret_cast<T(*)[kTile + 1]>(&Ls[0][0]);
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) St[wc + 16 * y + li][wr + 16 * x + Mfma16<T>::row(lk, v)] = acc[x][y][v];
+  __syncthreads();
+  if (lane >= rows) return;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int c = w + 4 * q;
+    const T v = St[c][lane];
+    if (c < nbk) st_dev(P + r + (k0 + c) * ld, r < m ? rnd<R>(v) : v);
+  }
+}
+
+// A bounded wait of the dependency-driven factorization (as spin_until);
+// once any wait of the launch has expired (bit 2 of *err), the others end
+// too, so a failing launch drains in microseconds instead of one bound per
+// task. Task indices come from the plan, never from computed values, so
+// such a launch stays within its buffers.
+__device__ __forceinline__ void dag_wait(const int* c, int target, int* err) {
+  int it = 0;
+  while (ld_dev(c) < target) {
+    __builtin_amdgcn_s_sleep(1);
+    ++it;
+    if ((it & 255) == 0 && (ld_dev(err) & 2)) break;
+    if (it > kSpinLimit) {
+      atomicOr(err, 2);
+      break;
+    }
+  }
+}
+
+// The levels >= the plan's fdag_level in one persistent launch: workgroups
+// take tasks from the dispensers in list order (topological), wait on the
+// counters the task names, run it and count it (see sync.hpp for why the
+// launch always progresses). Every value another task of the launch wrote is
+// read and written with device-coherent accesses.
+template <class T, int R>
+__global__ void __launch_bounds__(256) k_factor_dag(FactorArgs<T> a, FactorDagDev g) {
+  __shared__ __align__(16) T smem[2 * kPanelNB * kTile];
+  __shared__ int tsh;
+  static_assert(6 * kPanelNB * 4 + kPanelNB * (kPanelNB + 1) <= 2 * kPanelNB * kTile, "diagonal-block LDS");
+  static_assert(kAsmTile * (kAsmTile + 1) <= 2 * kPanelNB * kTile, "assembly LDS");
+  static_assert(2 * kTile * kLs16 * 2 <= 2 * kPanelNB * kTile * sizeof(T), "16-bit operand LDS");
+  int* cnt = g.counters;
+  int* lines = cnt + g.cnt_disp;
+  const FactorDagIndex X{g.ns, g.nsteps};
+  Dispenser ds = arrive(lines, &tsh);
+  for (;;) {
+    const int t = next_task(lines, ds, g.ntask, &tsh);
+    if (t >= g.ntask) break;
+    const FTask tk = g.task[t];
+    const int kind = tk.sk >> kFKindShift;
+    const int s = tk.sk & ((1 << kFKindShift) - 1);
+    unsigned long long t_draw = 0;
+    if (threadIdx.x == 0) {
+      if (g.trace) t_draw = wall_clock64();
+      dag_wait(cnt + tk.wa, tk.ta, a.err);
+      if (tk.wb >= 0) dag_wait(cnt + tk.wb, tk.tb, a.err);
+      if (tk.wc >= 0) dag_wait(cnt + tk.wc, tk.tc, a.err);
+      if (g.trace) {
+        g.trace[4 * static_cast<int64_t>(t)] = static_cast<unsigned long long>(tk.sk);
+        g.trace[4 * static_cast<int64_t>(t) + 1] = t_draw;
+        g.trace[4 * static_cast<int64_t>(t) + 2] = wall_clock64();
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    __syncthreads();
+    const int* ms = g.meta + kFMetaStride * s;
+    if (kind == kFAsm) {
+      asm_body<T, R, CoherentMem, 8>(a, g.asm_task[tk.x], g.asm_child_ptr[tk.x], g.asm_child_ptr[tk.x + 1],
+                                     g.asm_child, reinterpret_cast<T(*)[kAsmTile + 1]>(smem));
+      block_publish(cnt + X.asm_done(s));
+    } else if (kind == kFDiag) {
+      diag_body<T, R>(a, s, tk.x, smem);
+      if (block_publish(cnt + X.diag_done(ms[kFMetaStep] + tk.x)) >= 0) add_dev(cnt + X.pan_done(ms[kFMetaStep] + tk.x), 1);
+    } else if (kind == kFPan) {
+      pan_body<T, R>(a, s, tk.x, tk.y, tk.z, reinterpret_cast<T(*)[kPanelNB * kTile]>(smem));
+      block_publish(cnt + X.pan_done(ms[kFMetaStep] + tk.z));
+    } else {
+      const Task4 ut{s, tk.x, tk.y, tk.z};
+      if constexpr (std::is_same<T, float>::value && R != 0)
+        upd16_body<R, CoherentMem>(a, ut, reinterpret_cast<typename Mfma16x32<R>::E(*)[kTile * kLs16]>(smem));
+      else
+        upd_body<T, R, CoherentMem>(a, ut, reinterpret_cast<T(*)[kPanelNB * kTile]>(smem));
+      // counted on its column block (tiles that start on one), then towards
+      // the front's completion, which the last tile reports to the parent
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        if (tk.y % kTile == 0) add_dev(cnt + X.col_done(ms[kFMetaCb] + tk.y / kTile), 1);
+        if (add_dev(cnt + X.upd_done(s), 1) == ms[kFMetaNUpd] - 1 && ms[kFMetaParent] >= 0)
+          add_dev(cnt + X.child_done(ms[kFMetaParent]), 1);
+      }
+    }
+    if (g.trace && threadIdx.x == 0) g.trace[4 * static_cast<int64_t>(t) + 3] = wall_clock64() << 16 | blockIdx.x;
+  }
+}
+
+__global__ void k_fill_zero(int n, int* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// Workgroups of a kernel resident on the whole device at once.
+template <class K>
+int resident_workgroups(K kernel, int threads) {
+  int dev = 0, cus = 0, per_cu = 0;
+  MPIR_HIP(hipGetDevice(&dev));
+  MPIR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  MPIR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+  return std::max(1, per_cu) * std::max(1, cus);
 }
 
 // Count of D[i] > 0 (the inertia QDLDL_factor returns) and of i with D[i]
@@ -752,7 +1130,11 @@ void factor_levels(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const dou
   if (nnz > 0)
     hipLaunchKernelGGL((k_scatter_values<T, R>), dim3(blocks_for(nnz, 256)), dim3(256), 0, st, nnz, ax_scaled,
                        dp.val_dst, a.panel);
-  for (int l = 0; l < dp.nlevels; ++l) {
+  // levels below the dependency-driven launch (all of them without it):
+  // one launch per level and kind, and per step of the big fronts
+  const bool dag = dp.use_dag && dp.dag_pool && dp.dag_level < dp.nlevels && dp.dag.ntask > 0;
+  const int lsplit = dag ? dp.dag_level : dp.nlevels;
+  for (int l = 0; l < lsplit; ++l) {
     const int s0 = dp.h_small_ptr[l], s1 = dp.h_small_ptr[l + 1];
     if (s1 > s0)
       hipLaunchKernelGGL((k_small_fronts<T, R>), dim3(s1 - s0), dim3(64), 0, st, a, dp.small_list + s0);
@@ -769,6 +1151,14 @@ void factor_levels(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const dou
       const int u0 = dp.h_upd_ptr[q], u1 = dp.h_upd_ptr[q + 1];
       if (u1 > u0) launch_update<T, R>(a, dp.upd_task + u0, u1 - u0, st);
     }
+  }
+  if (dag) {
+    const FactorDagDev& g = dp.dag;
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks_for(g.ncounters, 256)), dim3(256), 0, st, g.ncounters, g.counters);
+    // at least kDispensers workgroups, so that every class is covered
+    static const int grid = resident_workgroups(k_factor_dag<T, R>, 256);
+    hipLaunchKernelGGL((k_factor_dag<T, R>), dim3(std::max(kDispensers, std::min(g.ntask, grid))), dim3(256), 0, st, a,
+                       g);
   }
 }
 

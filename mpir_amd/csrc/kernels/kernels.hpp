@@ -45,6 +45,23 @@ struct FactorArgs {
   int* err;
 };
 
+// Device view of the dependency-driven factorization (plan.hpp, FTask):
+// the task list, the per-supernode counter offsets, the assembly lists it
+// indexes, and the counters (the task counters, then the dispenser lines).
+struct FactorDagDev {
+  const FTask* task = nullptr;
+  int ntask = 0;
+  const int32_t* meta = nullptr;  // kFMetaStride per supernode
+  const Task4* asm_task = nullptr;
+  const int32_t* asm_child_ptr = nullptr;
+  const int32_t* asm_child = nullptr;
+  int* counters = nullptr;
+  int ns = 0, nsteps = 0, cnt_disp = 0, ncounters = 0;
+  // when set: per task (its kind and supernode, the device wall clock when
+  // it was drawn, when its waits ended, when it was counted, the workgroup)
+  unsigned long long* trace = nullptr;
+};
+
 // Device copies of the plan's task lists, with the host-side offsets the
 // launch loop needs.
 struct DeviceFactorPlan {
@@ -63,6 +80,11 @@ struct DeviceFactorPlan {
   // (s, column block, row chunk) tasks over the inverse rows of big fronts
   const int32_t* inv_task = nullptr;
   int n_inv_task = 0;
+  // levels >= dag_level in one dependency-driven launch (when use_dag; the
+  // update-matrix pool must then be laid out for it: dag_pool)
+  bool use_dag = false, dag_pool = false;
+  int dag_level = 0;
+  FactorDagDev dag;
 };
 
 // rounding: storage format of the factor's working values (0 as T, 1 FP16,

@@ -42,13 +42,13 @@
 
 #include "../common.hpp"
 #include "kernels.hpp"
+#include "sync.hpp"
 
 namespace mpir {
 
 namespace {
 
 constexpr int kRB = kSolveRB;
-constexpr int kSpinLimit = 1 << 20;  // polls (about a second) before a wait gives up
 // LDS of a 256-thread task, in entries of the solve precision: the staged
 // vector of a tile (up to 512 rows for the column products of 16-bit
 // factors, kGemvCols entries for the row products), then the wave
@@ -66,22 +66,6 @@ enum : int {
 };
 static_assert(kBDone + 1 == kNumSolveCounters, "counter layout");
 
-// ---------------------------------------------------------------- memory
-// Relaxed device-scope atomic accesses: plain loads and stores with the
-// device-coherence bit, no cache maintenance; used for every value that one
-// task writes and another reads.
-template <class T>
-__device__ __forceinline__ T ld_dev(const T* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class T>
-__device__ __forceinline__ void st_dev(T* p, T v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ int add_dev(int* c, int v) {
-  return __hip_atomic_fetch_add(c, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // 16 bytes of factor entries; the address needs only the element's alignment
 template <class TF>
 struct Pack {
@@ -92,13 +76,6 @@ __device__ __forceinline__ Pack<TF> load_pack(const TF* p) {
   Pack<TF> r;
   __builtin_memcpy(&r, p, sizeof r);
   return r;
-}
-
-// orders the LDS accesses of one wavefront (no other wave takes part)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // sum_{k < n} p[k * stride], added in order; the loads go out 8 at a time
@@ -204,68 +181,6 @@ struct RowGather {
     }
   }
 };
-
-// ---------------------------------------------------------------- waits and counts
-// Waits until *c >= target. The wait is bounded: on expiry it records bit 1
-// in *err and returns (the result is then wrong and the host reports it).
-__device__ __forceinline__ void spin_until(const int* c, int target, int* err) {
-  int it = 0;
-  while (ld_dev(c) < target) {
-    __builtin_amdgcn_s_sleep(1);
-    if (++it > kSpinLimit) {
-      atomicOr(err, 2);
-      break;
-    }
-  }
-}
-struct NoWait {
-  __device__ void operator()() const {}
-};
-struct BlockWait {
-  const int* c;
-  int target;
-  int* err;
-  __device__ void operator()() const {
-    if (threadIdx.x == 0) spin_until(c, target, err);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    __syncthreads();
-  }
-};
-struct WaveWait {
-  const int* c;
-  int target;
-  int* err;
-  __device__ void operator()() const {
-    if ((threadIdx.x & 63) == 0) spin_until(c, target, err);
-    wave_sync();
-  }
-};
-// Called by every thread of the block after its last store of a task: once
-// all those stores are complete, thread 0 counts the task and returns the
-// counter's previous value (other threads: -1).
-__device__ __forceinline__ int block_publish(int* c) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  return threadIdx.x == 0 ? add_dev(c, 1) : -1;
-}
-// block_publish for one of `target` tasks of a block of output: true in
-// every thread of the task that completes the block
-__device__ __forceinline__ bool publish_last(int* c, int target, int* flag) {
-  const int prev = block_publish(c);
-  if (threadIdx.x == 0) *flag = prev == target - 1;
-  __syncthreads();
-  const bool last = *flag;
-  if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  return last;
-}
-// The same for a task of one wavefront.
-__device__ __forceinline__ void wave_publish(int* c) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_s_waitcnt(0);
-  wave_sync();
-  if ((threadIdx.x & 63) == 0) add_dev(c, 1);
-}
 
 // ---------------------------------------------------------------- small fronts
 // One wavefront per front (m <= 64), lane = row; v: 64 entries of LDS. L is
@@ -1074,56 +989,6 @@ __global__ void __launch_bounds__(256) k_level_tiles(SolveArgs<TF, TD> a, SolveD
 }
 
 // ---------------------------------------------------------------- dag schedule
-// The task list (topological: a task waits only on tasks before it) is dealt
-// round-robin onto kDispensers counters, each on its own 128-byte line (one
-// counter would serialise every workgroup of the device on one address), so
-// class k hands out tasks k, k + 16, k + 32, ... in list order.
-//
-// Liveness. A workgroup takes the class of its arrival order: the first
-// kDispensers workgroups that start, on whatever XCDs the hardware placed
-// them, cover every class, and a workgroup leaves a class only when the
-// class is exhausted (then it moves on to the next class, and it exits when
-// all are). So while a class has tasks left, some running workgroup draws
-// from it. Tasks are drawn in list order within a class and every task waits
-// only on earlier tasks, so the earliest unfinished task that has been drawn
-// never waits on one that has not: the solve always progresses, whichever
-// workgroups other streams' kernels keep from becoming resident. (Classes
-// taken from blockIdx % kDispensers instead starve when the XCD that would
-// run, say, block 3 is full: workgroups are dealt round-robin over the XCDs,
-// so blocks 16, 17, ... can run while block 3 waits.)
-struct Dispenser {
-  int cls = 0;    // class this workgroup draws from (thread 0's copy)
-  int tried = 0;  // classes found exhausted
-};
-
-__device__ __forceinline__ Dispenser arrive(int* lines, int* tsh) {
-  Dispenser d;
-  if (threadIdx.x == 0) *tsh = add_dev(lines + kDispenserStride * kDispensers, 1) % kDispensers;
-  __syncthreads();
-  d.cls = *tsh;
-  return d;
-}
-
-// the workgroup's next task, or ntasks when every class is exhausted
-__device__ __forceinline__ int next_task(int* lines, Dispenser& ds, int ntasks, int* tsh) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int t = ntasks;
-    while (ds.tried < kDispensers) {
-      const int c = add_dev(lines + kDispenserStride * ds.cls, 1) * kDispensers + ds.cls;
-      if (c < ntasks) {
-        t = c;
-        break;
-      }
-      ++ds.tried;
-      ds.cls = ds.cls + 1 == kDispensers ? 0 : ds.cls + 1;
-    }
-    *tsh = t;
-  }
-  __syncthreads();
-  return *tsh;
-}
-
 // One right-hand side: four waves per SIMD (at most 128 registers), 1024
 // resident workgroups; several: two waves per SIMD (256 registers), so the
 // factor tile and the partial sums of one right-hand side stay in registers.

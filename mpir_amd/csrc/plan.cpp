@@ -318,6 +318,216 @@ void build_factor_steps(FactorPlan& P, const FactorTuning& t) {
     }
     P.level_step_ptr[l + 1] = P.level_step_ptr[l] + max_steps;
   }
+  // the assembly lists are final once the plan is built (re-planning the
+  // steps of a built plan rebuilds the dependency-driven list too)
+  if (P.asm_child_ptr.size() == P.asm_task.size() + 1) build_factor_dag(P);
+}
+
+int32_t factor_dag_level(const FactorPlan& P) {
+  int32_t split = 0;
+  for (int32_t l = 0; l < P.sn.nlevels; ++l)
+    if (P.small_ptr[l + 1] > P.small_ptr[l]) split = l + 1;
+  return split;
+}
+
+// The list runs level by level. Within a level: the assembly tiles, the
+// fronts' first diagonal blocks and panel rows, then per step k: the update
+// tiles of column block k + 1, the diagonal blocks and panel rows of step
+// k + 1 (they need only those tiles), the step's other update tiles. So a
+// front's next panel step is handed out ahead of the bulk of the current
+// step's trailing update (one step of lookahead), and every wait is on
+// tasks earlier in the list.
+void build_factor_dag(FactorPlan& P) {
+  const Supernodes& S = P.sn;
+  const int32_t ns = S.ns, L = S.nlevels;
+  const int32_t split = factor_dag_level(P);
+  P.fdag_level = split;
+  P.fdag.clear();
+  P.fdag_meta.assign(static_cast<size_t>(ns) * kFMetaStride, 0);
+  for (int32_t s = 0; s < ns; ++s) P.fdag_meta[static_cast<size_t>(s) * kFMetaStride + kFMetaParent] = -1;
+  P.fdag_nsteps = P.fdag_ncb = 0;
+  if (split >= L) return;
+  auto meta = [&](int32_t s) { return &P.fdag_meta[static_cast<size_t>(s) * kFMetaStride]; };
+  std::vector<char> in_dag(ns, 0);
+  std::vector<int32_t> nk(ns, 0);
+  int32_t nsteps = 0, ncb = 0;
+  for (int32_t l = split; l < L; ++l)
+    for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) {
+      const int32_t s = P.big_list[q];
+      const int32_t nc = S.first[s + 1] - S.first[s], m = nc + S.nrow[s];
+      in_dag[s] = 1;
+      nk[s] = (nc + kPanelNB - 1) / kPanelNB;
+      meta(s)[kFMetaStep] = nsteps;
+      meta(s)[kFMetaCb] = ncb;
+      nsteps += nk[s];
+      ncb += (m + kTile - 1) / kTile;
+    }
+  P.fdag_nsteps = nsteps;
+  P.fdag_ncb = ncb;
+  const FactorDagIndex X{ns, nsteps};
+  std::vector<int32_t> n_asm(ns, 0), n_child(ns, 0), n_pan(nsteps, 0);
+  for (int32_t s = 0; s < ns; ++s) {
+    if (!in_dag[s]) continue;
+    const int32_t p = S.sparent[s];
+    if (p < 0) continue;
+    if (!in_dag[p]) throw std::logic_error("factor DAG: parent of a front in the launch is not in it");
+    meta(s)[kFMetaParent] = p;
+    n_child[p]++;
+  }
+  for (int32_t t = P.asm_ptr[split]; t < P.asm_ptr[L]; ++t) n_asm[P.asm_task[t].a]++;
+  const int32_t step_lo = P.level_step_ptr[split], step_hi = P.level_step_ptr[L];
+  // update tiles per (front, column block, last step): prior counts
+  std::vector<int64_t> cs_off(static_cast<size_t>(ncb) + 1, 0);
+  for (int32_t s = 0; s < ns; ++s) {
+    if (!in_dag[s]) continue;
+    const int32_t m = S.first[s + 1] - S.first[s] + S.nrow[s];
+    for (int32_t j = 0; j < (m + kTile - 1) / kTile; ++j) cs_off[meta(s)[kFMetaCb] + j + 1] = nk[s];
+  }
+  for (int32_t b = 0; b < ncb; ++b) cs_off[b + 1] += cs_off[b];
+  std::vector<int32_t> cs(static_cast<size_t>(cs_off[ncb]), 0);
+  auto upd_fields = [&](const Task4& tk, int32_t& kfirst, int32_t& klast) {
+    klast = tk.d & ((1 << kUpdStepsShift) - 1);
+    const int32_t steps = (tk.d >> kUpdStepsShift) & ((1 << (kUpdPanelBit - kUpdStepsShift)) - 1);
+    kfirst = klast - steps + 1;
+  };
+  for (int32_t t = P.upd_ptr[step_lo]; t < P.upd_ptr[step_hi]; ++t) {
+    const Task4 tk = P.upd_task[t];
+    int32_t kf, kl;
+    upd_fields(tk, kf, kl);
+    meta(tk.a)[kFMetaNUpd]++;
+    if (tk.c % kTile == 0) cs[cs_off[meta(tk.a)[kFMetaCb] + tk.c / kTile] + kl]++;
+  }
+  // exclusive prefix over the steps: cs[b, k] = tiles of block b with last
+  // step < k; col_total[b] = all of them
+  std::vector<int32_t> col_total(ncb, 0);
+  for (int32_t b = 0; b < ncb; ++b) {
+    int32_t run = 0;
+    for (int64_t q = cs_off[b]; q < cs_off[b + 1]; ++q) {
+      const int32_t c = cs[q];
+      cs[q] = run;
+      run += c;
+    }
+    col_total[b] = run;
+  }
+  auto prior = [&](int32_t s, int32_t b, int32_t k) -> int32_t {
+    const int32_t cb = meta(s)[kFMetaCb] + b;
+    return k >= nk[s] ? col_total[cb] : cs[cs_off[cb] + k];
+  };
+  for (int32_t s = 0; s < ns; ++s)
+    if (in_dag[s] && meta(s)[kFMetaParent] >= 0 && meta(s)[kFMetaNUpd] == 0)
+      throw std::logic_error("factor DAG: a front with a parent has no update tasks");
+  // panel tasks: the diagonal block (row0 = k0), other rows; the block's own
+  // identity rows are written by the diagonal-block task
+  auto pan_kind = [&](const Task4& tk) {
+    const int32_t s = tk.a, k0 = tk.d * kPanelNB;
+    const int32_t m = S.first[s + 1] - S.first[s] + S.nrow[s];
+    return tk.b == k0 ? kFDiag : tk.b == m + k0 ? -1 : kFPan;
+  };
+  for (int32_t t = P.pan_ptr[step_lo]; t < P.pan_ptr[step_hi]; ++t) {
+    const Task4 tk = P.pan_task[t];
+    if (pan_kind(tk) >= 0) n_pan[meta(tk.a)[kFMetaStep] + tk.d]++;
+  }
+  auto panel_wait = [&](int32_t s, int32_t k, int32_t& w, int32_t& tgt) {
+    if (k == 0) {
+      w = X.asm_done(s);
+      tgt = n_asm[s];
+    } else {
+      w = X.col_done(meta(s)[kFMetaCb] + k);
+      tgt = prior(s, k, k);
+    }
+  };
+  auto emit_panels = [&](int32_t g) {
+    for (int32_t t = P.pan_ptr[g]; t < P.pan_ptr[g + 1]; ++t) {
+      const Task4 tk = P.pan_task[t];
+      const int32_t kind = pan_kind(tk);
+      if (kind < 0) continue;
+      const int32_t s = tk.a, k = tk.d;
+      FTask f{s | (kind << kFKindShift), 0, 0, 0, 0, 0, -1, 0, -1, 0};
+      panel_wait(s, k, f.wa, f.ta);
+      if (kind == kFDiag) {
+        f.x = k;
+      } else {
+        f.x = tk.b;
+        f.y = tk.c;
+        f.z = k;
+        f.wb = X.diag_done(meta(s)[kFMetaStep] + k);
+        f.tb = 1;
+      }
+      P.fdag.push_back(f);
+    }
+  };
+  // next = true: the tiles of column block k + 1 only; false: the others
+  auto emit_updates = [&](int32_t g, bool next) {
+    for (int32_t t = P.upd_ptr[g]; t < P.upd_ptr[g + 1]; ++t) {
+      const Task4 tk = P.upd_task[t];
+      int32_t kf, kl;
+      upd_fields(tk, kf, kl);
+      const bool is_next = tk.c == (kl + 1) * kTile;
+      if (is_next != next) continue;
+      const int32_t s = tk.a;
+      const int32_t m = S.first[s + 1] - S.first[s] + S.nrow[s];
+      const int32_t st = meta(s)[kFMetaStep] + kl;
+      FTask f{s | (kFUpd << kFKindShift), tk.b, tk.c, tk.d, X.pan_done(st), n_pan[st], -1, 0, -1, 0};
+      const int32_t b0 = tk.c / kTile, b1 = (std::min(m, tk.c + kTile) - 1) / kTile;
+      f.wb = X.col_done(meta(s)[kFMetaCb] + b0);
+      f.tb = prior(s, b0, kf);
+      if (tk.c % kTile != 0 && b1 > b0) {
+        f.wc = X.col_done(meta(s)[kFMetaCb] + b1);
+        f.tc = prior(s, b1, kf);
+      }
+      P.fdag.push_back(f);
+    }
+  };
+  for (int32_t l = split; l < L; ++l) {
+    for (int32_t t = P.asm_ptr[l]; t < P.asm_ptr[l + 1]; ++t) {
+      const int32_t s = P.asm_task[t].a;
+      P.fdag.push_back(FTask{s | (kFAsm << kFKindShift), t, 0, 0, X.child_done(s), n_child[s], -1, 0, -1, 0});
+    }
+    const int32_t g0 = P.level_step_ptr[l], g1 = P.level_step_ptr[l + 1];
+    if (g1 > g0) emit_panels(g0);
+    for (int32_t g = g0; g < g1; ++g) {
+      emit_updates(g, true);
+      if (g + 1 < g1) emit_panels(g + 1);
+      emit_updates(g, false);
+    }
+  }
+}
+
+// Update-matrix pool: U_s at upd_off[s], allocated level by level. A
+// child's block is released once its parent's level has been assembled,
+// except from the first level of the dependency-driven factorization on
+// when `dag`: there fronts of different levels run at once, so a block is
+// kept (a front of a higher level that is not its ancestor could otherwise
+// reuse it before its parent has read it).
+void build_update_pool(FactorPlan& P, bool dag) {
+  const Supernodes& S = P.sn;
+  const int32_t ns = S.ns;
+  int32_t dag_level = S.nlevels;
+  if (dag) {
+    dag_level = 0;
+    for (int32_t s = 0; s < ns; ++s)
+      if (S.first[s + 1] - S.first[s] + S.nrow[s] <= kSmallFront) dag_level = std::max(dag_level, S.level[s] + 1);
+  }
+  P.upd_off.assign(ns, -1);
+  PoolAlloc pool;
+  for (int32_t l = 0; l < S.nlevels; ++l) {
+    for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int32_t s = S.level_sn[q];
+      const int64_t nr = S.nrow[s];
+      if (nr > 0) P.upd_off[s] = pool.alloc(nr * nr);
+    }
+    if (l >= dag_level) continue;
+    for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
+      const int32_t s = S.level_sn[q];
+      for (int32_t t = P.child_ptr[s]; t < P.child_ptr[s + 1]; ++t) {
+        const int32_t c = P.child_list[t];
+        const int64_t nr = S.nrow[c];
+        if (nr > 0) pool.release(P.upd_off[c], nr * nr);
+      }
+    }
+  }
+  P.upd_pool = pool.peak();
+  P.dag_pool = dag;
 }
 
 FactorPlan build_plan_from_supernodes(Supernodes sn, const FactorTuning& ft, const SolveTuning& st) {
@@ -339,25 +549,7 @@ FactorPlan build_plan_from_supernodes(Supernodes sn, const FactorTuning& ft, con
       if (S.sparent[s] >= 0) P.child_list[next[S.sparent[s]]++] = s;
   }
 
-  // update-matrix pool with level lifetimes
-  P.upd_off.assign(ns, -1);
-  PoolAlloc pool;
-  for (int32_t l = 0; l < S.nlevels; ++l) {
-    for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-      const int32_t s = S.level_sn[q];
-      const int64_t nr = S.nrow[s];
-      if (nr > 0) P.upd_off[s] = pool.alloc(nr * nr);
-    }
-    for (int32_t q = S.level_ptr[l]; q < S.level_ptr[l + 1]; ++q) {
-      const int32_t s = S.level_sn[q];
-      for (int32_t t = P.child_ptr[s]; t < P.child_ptr[s + 1]; ++t) {
-        const int32_t c = P.child_list[t];
-        const int64_t nr = S.nrow[c];
-        if (nr > 0) pool.release(P.upd_off[c], nr * nr);
-      }
-    }
-  }
-  P.upd_pool = pool.peak();
+  build_update_pool(P, false);
 
   // front classes, assembly tasks, blocked-factor tasks
   P.small_ptr.assign(static_cast<size_t>(S.nlevels) + 1, 0);
@@ -423,6 +615,7 @@ FactorPlan build_plan_from_supernodes(Supernodes sn, const FactorTuning& ft, con
     P.asm_task.swap(kept);
     P.asm_ptr.swap(kept_level);
   }
+  build_factor_dag(P);
 
   build_solve_schedule(P, st);
   return P;

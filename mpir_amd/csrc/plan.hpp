@@ -70,13 +70,46 @@ constexpr int kDagGroup = 4;
 enum : int32_t { kMetaParent = 0, kMetaNChild, kMetaNFx, kMetaNFu, kMetaNBr, kMetaNBFinal, kMetaNH, kMetaNHR,
                  kMetaStride = 8 };
 
+// Dependency-driven factorization: the big fronts of the levels at and
+// above FactorPlan::fdag_level in one persistent launch. Task kinds:
+//   asm   one extend-add tile of a front (x = index of the assembly task)
+//   diag  the diagonal block of panel step k of a front (x = k): L_kk, D,
+//         D^{-1} and Z_kk = L_kk^{-T} D_k^{-1} in the block's identity rows
+//   pan   rows [x, y) of panel step z: L = X Z_kk (rows below the block, or
+//         identity rows of earlier blocks)
+//   upd   a trailing-update tile (x = row, y = column, z = step code as in
+//         the per-level update tasks)
+// A task waits until counter wa >= ta and, for wb, wc >= 0, until counter
+// wb >= tb and counter wc >= tc.
+// Counters (index from fdag_meta): per front asm tasks done, DAG children
+// complete, update tasks done; per (front, step) diagonal blocks done and
+// panel tasks done (the diagonal block counts as one of them); per (front,
+// 64-column block) update tasks done of the tiles that start on the block.
+// Update tiles at a front's last step start at column nc, which is not on a
+// block boundary when 64 does not divide nc; they touch only the update
+// matrix, wait on the two blocks they overlap, and count only towards the
+// front's completion.
+enum : int32_t { kFAsm = 0, kFDiag = 1, kFPan = 2, kFUpd = 3 };
+constexpr int kFKindShift = 28;
+struct FTask {
+  int32_t sk;  // supernode | kind << kFKindShift
+  int32_t x, y, z;
+  int32_t wa, ta, wb, tb, wc, tc;
+};
+// Per supernode in fdag_meta: first step counter, first column-block
+// counter, update tasks, parent (or -1 when the parent is not in the launch).
+enum : int32_t { kFMetaStep = 0, kFMetaCb, kFMetaNUpd, kFMetaParent, kFMetaStride = 4 };
+
 struct FactorPlan {
   Supernodes sn;
   // children of each supernode, ascending
   std::vector<int32_t> child_ptr, child_list;
-  // update-matrix pool: U_s is nrow_s x nrow_s column-major at upd_off[s]
+  // update-matrix pool: U_s is nrow_s x nrow_s column-major at upd_off[s];
+  // dag_pool: no block is reused from fdag_level on (the layout the
+  // dependency-driven factorization needs; larger)
   std::vector<int64_t> upd_off;
   int64_t upd_pool = 0;
+  bool dag_pool = false;
   // per level: small fronts, big fronts
   std::vector<int32_t> small_ptr, small_list;
   std::vector<int32_t> big_ptr, big_list;
@@ -147,6 +180,13 @@ struct FactorPlan {
   // dag_meta: kMetaStride entries per supernode
   std::vector<Task4> dag_fwd, dag_bwd;
   std::vector<int32_t> dag_meta;
+  // dependency-driven factorization of the levels >= fdag_level (no small
+  // fronts there; nlevels when there is none): task list in a topological
+  // order, per-supernode counter offsets, counter layout
+  int32_t fdag_level = 0;
+  std::vector<FTask> fdag;
+  std::vector<int32_t> fdag_meta;  // kFMetaStride per supernode
+  int32_t fdag_nsteps = 0, fdag_ncb = 0;
   // for each entry of the ORIGINAL upper CSC: destination in the panel pool
   std::vector<int64_t> val_dst;
   // statistics
@@ -161,6 +201,28 @@ FactorPlan build_plan_from_supernodes(Supernodes sn, const FactorTuning& ft = {}
 void build_factor_steps(FactorPlan& P, const FactorTuning& t);
 // dag_fwd, dag_bwd and dag_meta from the per-level task lists.
 void build_dag_schedule(FactorPlan& P);
+// First level of the dependency-driven factorization: one above the highest
+// level that holds a small front.
+int32_t factor_dag_level(const FactorPlan& P);
+// upd_off, upd_pool and dag_pool of P's supernodes: blocks are released when
+// their parent's level has been assembled, below factor_dag_level only when
+// `dag` (the dependency-driven factorization runs fronts of several levels
+// at once).
+void build_update_pool(FactorPlan& P, bool dag);
+// fdag, fdag_meta and the counter layout from the per-level factor lists
+// (assembly, panel and update tasks).
+void build_factor_dag(FactorPlan& P);
+// Counter indices of the dependency-driven factorization (constexpr: the
+// kernels use the same functions).
+struct FactorDagIndex {
+  int32_t ns, nsteps;
+  constexpr int32_t asm_done(int32_t s) const { return s; }
+  constexpr int32_t child_done(int32_t s) const { return ns + s; }
+  constexpr int32_t upd_done(int32_t s) const { return 2 * ns + s; }
+  constexpr int32_t diag_done(int32_t step) const { return 3 * ns + step; }
+  constexpr int32_t pan_done(int32_t step) const { return 3 * ns + nsteps + step; }
+  constexpr int32_t col_done(int32_t cb) const { return 3 * ns + 2 * nsteps + cb; }
+};
 // The triangular-solve schedule of P's supernodes (task lists, partial-sum
 // offsets, row gathers, dependency schedule) for tuning t; P.inv_task too.
 void build_solve_schedule(FactorPlan& P, const SolveTuning& t);

@@ -153,6 +153,20 @@ class GmresLdl {
     schedule_ = sched;
     dsp_.schedule = sched;
   }
+  // schedule of the numeric factorization: kScheduleLevels (default; one
+  // launch per level and step) or kScheduleDag (the levels above the small
+  // fronts in one dependency-driven launch, with an update-matrix pool that
+  // reuses no block there); both give the same factor bit for bit. Selecting
+  // the DAG on an analyzed handle lays the pool out again (factor again).
+  void set_factor_schedule(int sched);
+  int factor_schedule() const { return factor_schedule_; }
+  // the dependency-driven launch of the current plan (state error before analyze)
+  void factor_schedule_info(int32_t* dag_level, int32_t* nlevels, int64_t* tasks) const {
+    if (!plan_) throw Error(kErrState, "analyze first");
+    *dag_level = plan_->fdag_level;
+    *nlevels = plan_->sn.nlevels;
+    *tasks = static_cast<int64_t>(plan_->fdag.size());
+  }
   // kPrecLdl (default) or kPrecIlu with the ILU level and the sweep limit of
   // the reference's Compute (its ILU_k argument and MAX_SWEEPS parameter);
   // takes effect at the next analyze
@@ -219,6 +233,10 @@ class GmresLdl {
   // handle's stream, measured with HIP events: 0 SpMV, 1 M^{-1} application,
   // 2 numeric factorization (values already loaded by factor()).
   double time_kernels(int what, int reps);
+  // one factorization of the current values with the dependency-driven
+  // launch's task timeline (4 entries per task) in out[0 .. cap); returns
+  // the number of tasks (cap = 0: only that)
+  int64_t trace_factor(uint64_t* out, int64_t cap);
   // Times the factor's update grouping and super-block edge and the
   // triangular solves' narrow-front limit and schedule on this handle's
   // factor (reps runs per candidate), keeps the fastest of each and leaves
@@ -293,6 +311,7 @@ class GmresLdl {
   int ordering_ = kOrderAmd;
   int scaling_ = kScaleDiag;
   int schedule_ = kScheduleDag;
+  int factor_schedule_ = kScheduleLevels;
   FactorTuning ftune_;
   SolveTuning stune_;
   bool relax_ = true;     // relaxed supernodes (AMD / ND orderings only)
@@ -324,6 +343,7 @@ class GmresLdl {
   // numeric
   DevBuf d_ax_, d_axs_, d_ad_, d_adp_, d_panel_, d_panel16_, d_upool_, d_D_, d_Dinv_, d_err_, d_cnt_, d_nonfinite_;
   DevBuf d_uvec_, d_scratch_, d_serr_, d_counters_;
+  DevBuf d_fdag_, d_fdag_meta_, d_fcounters_, d_ftrace_;  // dependency-driven factorization
   // GMRES workspace
   DevBuf d_V_, d_w_, d_b_, d_x_, d_r_, d_d_, d_red_, d_small_;
   DevBuf d_hcol_, d_givens_;

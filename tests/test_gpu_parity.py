@@ -269,6 +269,56 @@ def test_bad_schedule_rejected(gpu):
     s = mpir_amd.GmresLDLIR(np.float64)
     with pytest.raises(mpir_amd.MpirError):
         s.SetSolveSchedule(2)
+    with pytest.raises(mpir_amd.MpirError):
+        s.SetFactorSchedule(2)
+
+
+@pytest.mark.parametrize("uf,k,order", [(np.float64, 24, mpir_amd.ORDER_ND), (np.float32, 32, mpir_amd.ORDER_ND),
+                                        (np.float32, 20, mpir_amd.ORDER_AMD), (np.float32, 10, mpir_amd.ORDER_NATURAL),
+                                        (np.float16, 16, mpir_amd.ORDER_ND), ("bf16", 24, mpir_amd.ORDER_ND)])
+def test_factor_dag_matches_levels_bitwise(gpu, oracle, uf, k, order):
+    # the dependency-driven factorization computes every entry with the
+    # operations of the per-level launches in the same order, so the factors
+    # (L, D, D^{-1}) and whole GMRES-IR solves agree bit for bit; repeated
+    # DAG factorizations too (no result depends on which workgroup ran what)
+    P = mpir_amd.kkt_config(1, k=k)
+    s = mpir_amd.GmresLDLIR(uf, ordering=order)
+    if k == 32:
+        # selected after the analysis: the update pool is laid out again
+        s.Compute(P.Ap, P.Ai, P.Ax)
+        assert s.factor_schedule_info()["schedule"] == mpir_amd.SCHED_LEVELS
+        s.SetFactorSchedule(mpir_amd.SCHED_DAG)
+        s.Factor(P.Ax)
+    else:
+        s.SetFactorSchedule(mpir_amd.SCHED_DAG)
+        s.Compute(P.Ap, P.Ai, P.Ax)
+    info = s.factor_schedule_info()
+    print(f"k={k} {uf} order={order}: {info}", flush=True)
+    assert info["schedule"] == mpir_amd.SCHED_DAG
+    assert info["dag_level"] < info["nlevels"] and info["dag_tasks"] > 0
+    f_dag = s.export_factor()
+    x_dag = s.Solve(P.b)
+    st_dag = s.stats()
+    s.SetFactorSchedule(mpir_amd.SCHED_LEVELS)
+    s.Factor(P.Ax)
+    f_lev = s.export_factor()
+    x_lev = s.Solve(P.b)
+    st_lev = s.stats()
+    for a, b in zip(f_dag, f_lev):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(x_dag, x_lev)
+    assert st_dag["positive_pivots"] == st_lev["positive_pivots"]
+    if uf in (np.float64, np.float32):  # 16-bit factors lose the inertia at D = 1e-4
+        assert st_dag["positive_pivots"] == P.n1
+    assert st_dag["ir_iterations"] == st_lev["ir_iterations"]
+    assert st_dag["gmres_iterations"] == st_lev["gmres_iterations"]
+    s.SetFactorSchedule(mpir_amd.SCHED_DAG)
+    for _ in range(2):
+        s.Factor(P.Ax)
+        for a, b in zip(s.export_factor(), f_dag):
+            np.testing.assert_array_equal(a, b)
+    if uf in (np.float64, np.float32):
+        assert backward_error(oracle, P.Ap, P.Ai, P.Ax, x_dag, P.b) < 1e-13
 
 
 @pytest.mark.parametrize("k", [24, 32])
