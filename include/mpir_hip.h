@@ -224,6 +224,13 @@ int mpir_gmres_ldl_set_scaling(mpir_gmres_ldl_t h, int mode);
 #define MPIR_SCHED_LEVELS 0
 #define MPIR_SCHED_DAG 1
 int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule);
+/* supernode amalgamation of the symbolic phase, before analyze (no
+ * counterpart in the reference, whose QDLDL_factor works column by column):
+ * 0 fundamental supernodes, 1 to 3 (default) relaxed supernodes that merge a
+ * child into its parent more freely (fewer, larger fronts and a shallower
+ * elimination tree, for explicit zeros stored in the factor). Orderings AMD
+ * and ND only. */
+int mpir_gmres_ldl_set_amalgamation(mpir_gmres_ldl_t h, int level);
 /* schedule of the numeric factorization (no counterpart in the reference,
  * whose QDLDL_factor is a sequential loop over columns, qdldl.hpp:228-342):
  * MPIR_SCHED_LEVELS, one launch per level of the supernodal tree and per
@@ -303,6 +310,13 @@ int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms);
  * counted (shifted left by 16, the workgroup index in the low 16 bits).
  * Writes at most cap / 4 tasks; *ntask receives the number of tasks. */
 int mpir_gmres_ldl_trace_factor(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap, int64_t* ntask);
+/* One M^{-1} application (dependency-driven schedule) on a fixed vector with
+ * the timeline of its tasks (no counterpart in the reference): per task, the
+ * forward list then the backward list, 4 entries: kind | first supernode << 4;
+ * the device wall clock (100 MHz) when drawn, when its wait ended, when done
+ * (<< 16 | workgroup). Writes at most cap / 4 tasks; *ntask and *nfwd receive
+ * the number of tasks and of forward tasks. */
+int mpir_gmres_ldl_trace_precond(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap, int64_t* ntask, int64_t* nfwd);
 /* Autotuner of the level-tile schedules (no counterpart in the reference,
  * whose solver has no device schedule). On a factored L D L^T handle (single
  * system or batch) it times, with HIP events over `reps` runs (1..100) each:

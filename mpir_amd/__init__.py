@@ -224,10 +224,15 @@ class GmresLDLIR:
         """SCHED_DAG (default) or SCHED_LEVELS for the triangular solves of M^{-1}."""
         _check(load().mpir_gmres_ldl_set_schedule(self._h, int(schedule)), "SetSolveSchedule")
 
+    def SetAmalgamation(self, level: int) -> None:
+        """Supernode amalgamation, before Analyze: 0 none, 1 .. 3 (default;
+        fewer, larger fronts and a shallower tree for more stored zeros)."""
+        _check(load().mpir_gmres_ldl_set_amalgamation(self._h, int(level)), "SetAmalgamation")
+
     def SetFactorSchedule(self, schedule: int) -> None:
-        """SCHED_DAG (default: the levels above the small fronts in one
-        dependency-driven launch) or SCHED_LEVELS (one launch per level and
-        panel step) for the numeric factorization; same factor bit for bit."""
+        """SCHED_LEVELS (default: one launch per level and panel step) or
+        SCHED_DAG (the levels above the small fronts in one dependency-driven
+        launch) for the numeric factorization; same factor bit for bit."""
         _check(load().mpir_gmres_ldl_set_factor_schedule(self._h, int(schedule)), "SetFactorSchedule")
 
     def factor_schedule_info(self) -> dict:
@@ -336,6 +341,17 @@ class GmresLDLIR:
         out = np.zeros((max(nt.value, 1), 4), dtype=np.uint64)
         _check(load().mpir_gmres_ldl_trace_factor(self._h, _ptr(out), out.size, C.byref(nt)), "trace_factor")
         return out[:nt.value]
+
+    def trace_precond(self):
+        """One M^{-1} application with the timeline of its tasks: (an (ntask, 4)
+        uint64 array: kind | first supernode << 4, drawn, wait ended,
+        done << 16 | workgroup; device clock at 100 MHz), forward task count."""
+        nt, nf = C.c_int64(), C.c_int64()
+        _check(load().mpir_gmres_ldl_trace_precond(self._h, None, 0, C.byref(nt), C.byref(nf)), "trace_precond")
+        out = np.zeros((max(nt.value, 1), 4), dtype=np.uint64)
+        _check(load().mpir_gmres_ldl_trace_precond(self._h, _ptr(out), out.size, C.byref(nt), C.byref(nf)),
+               "trace_precond")
+        return out[:nt.value], nf.value
 
     def time_kernels(self, what: str, reps: int = 10) -> float:
         """Average ms of one kernel group ('spmv', 'precond', 'factor') via HIP events."""

@@ -88,6 +88,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_set_scaling": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_schedule": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_factor_schedule": (C.c_int, [P, C.c_int]),
+    "mpir_gmres_ldl_set_amalgamation": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_factor_schedule_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                                   C.POINTER(C.c_int64)]),
     "mpir_gmres_ldl_set_preconditioner": (C.c_int, [P, C.c_int, i32, i32]),
@@ -111,6 +112,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_stream": (P, [P]),
     "mpir_gmres_ldl_time": (C.c_int, [P, C.c_int, C.c_int, P]),
     "mpir_gmres_ldl_trace_factor": (C.c_int, [P, P, C.c_int64, C.POINTER(C.c_int64)]),
+    "mpir_gmres_ldl_trace_precond": (C.c_int, [P, P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "mpir_gmres_ldl_autotune": (C.c_int, [P, C.c_int, P, P]),
     "mpir_gmres_ldl_set_tuning": (C.c_int, [P, i32, i32, i32]),
     "mpir_batch_graph": (C.c_int, [P, C.c_int, P]),

@@ -31,7 +31,7 @@ void parallel_for(int64_t n, int threads, F&& f) {
 int join_threads() { return static_cast<int>(std::max(1u, std::min(8u, std::thread::hardware_concurrency()))); }
 }  // namespace
 
-uint64_t pattern_key(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax) {
+uint64_t pattern_key(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, int relax) {
   // FNV-1a over the dimension, the options and the index arrays
   uint64_t h = 1469598103934665603ull;
   auto mix = [&](const void* data, size_t bytes) {
@@ -41,14 +41,14 @@ uint64_t pattern_key(int32_t n, const int32_t* Ap, const int32_t* Ai, int orderi
       h *= 1099511628211ull;
     }
   };
-  const int32_t opts[3] = {n, ordering, relax ? 1 : 0};
+  const int32_t opts[3] = {n, ordering, relax};
   mix(opts, sizeof opts);
   mix(Ap, sizeof(int32_t) * (static_cast<size_t>(n) + 1));
   mix(Ai, sizeof(int32_t) * static_cast<size_t>(Ap[n]));
   return h;
 }
 
-PatternEntry analyze_entry(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax) {
+PatternEntry analyze_entry(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, int relax) {
   if (n < 1) throw std::invalid_argument("matrix dimension must be positive");
   if (Ap[0] != 0) throw std::invalid_argument("column pointers must start at 0");
   for (int32_t j = 0; j < n; ++j) {
@@ -100,7 +100,7 @@ int64_t pattern_footprint(const PatternEntry& e, int factor_bytes, int nrhs, int
 }
 
 std::shared_ptr<const PatternEntry> PatternCache::find_locked(uint64_t key, int32_t n, const int32_t* Ap,
-                                                              const int32_t* Ai, int ordering, bool relax) const {
+                                                              const int32_t* Ai, int ordering, int relax) const {
   auto range = map_.equal_range(key);
   for (auto it = range.first; it != range.second; ++it) {
     const PatternEntry& e = *it->second;
@@ -113,14 +113,14 @@ std::shared_ptr<const PatternEntry> PatternCache::find_locked(uint64_t key, int3
 }
 
 std::shared_ptr<const PatternEntry> PatternCache::get(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering,
-                                                      bool relax) {
+                                                      int relax) {
   return get_many({n}, {Ap}, {Ai}, ordering, relax, 1)[0];
 }
 
 std::vector<std::shared_ptr<const PatternEntry>> PatternCache::get_many(const std::vector<int32_t>& n,
                                                                         const std::vector<const int32_t*>& Ap,
                                                                         const std::vector<const int32_t*>& Ai,
-                                                                        int ordering, bool relax, int threads) {
+                                                                        int ordering, int relax, int threads) {
   const size_t ns = n.size();
   std::vector<std::shared_ptr<const PatternEntry>> out(ns);
   std::vector<uint64_t> key(ns);

@@ -24,7 +24,7 @@ namespace mpir {
 struct PatternEntry {
   uint64_t key = 0;
   int ordering = 0;
-  bool relax = true;
+  int relax = kDefaultAmalgamation;
   int32_t n = 0;
   int64_t nnz = 0;
   std::vector<int32_t> Ap, Ai;  // the pattern itself (upper CSC), for equality
@@ -37,7 +37,7 @@ struct PatternEntry {
   int64_t est_bytes = 0;          // device bytes of one system (see footprint)
 };
 
-PatternEntry analyze_entry(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax);
+PatternEntry analyze_entry(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, int relax);
 
 // Device bytes a solver needs for one system of this pattern with the given
 // factor element size, right-hand sides and GMRES cycle length.
@@ -47,26 +47,26 @@ int64_t pattern_footprint(const PatternEntry& e, int factor_bytes, int nrhs, int
 // ordering and relaxation; equal keys are checked entry by entry).
 class PatternCache {
  public:
-  std::shared_ptr<const PatternEntry> get(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax);
+  std::shared_ptr<const PatternEntry> get(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, int relax);
   // entries for many patterns, analysing the missing ones on `threads`
   // host threads
   std::vector<std::shared_ptr<const PatternEntry>> get_many(const std::vector<int32_t>& n,
                                                             const std::vector<const int32_t*>& Ap,
                                                             const std::vector<const int32_t*>& Ai, int ordering,
-                                                            bool relax, int threads);
+                                                            int relax, int threads);
   int64_t hits() const { return hits_; }
   int64_t misses() const { return misses_; }
   size_t size() const;
 
  private:
   std::shared_ptr<const PatternEntry> find_locked(uint64_t key, int32_t n, const int32_t* Ap, const int32_t* Ai,
-                                                  int ordering, bool relax) const;
+                                                  int ordering, int relax) const;
   mutable std::mutex mu_;
   std::unordered_multimap<uint64_t, std::shared_ptr<const PatternEntry>> map_;
   int64_t hits_ = 0, misses_ = 0;
 };
 
-uint64_t pattern_key(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax);
+uint64_t pattern_key(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, int relax);
 
 // The block-diagonal system of a batch.
 struct BatchSystem {

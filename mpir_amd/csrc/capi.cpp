@@ -620,6 +620,13 @@ int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule) {
     return 0;
   });
 }
+int mpir_gmres_ldl_set_amalgamation(mpir_gmres_ldl_t h, int level) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->set_relax(level);
+    return 0;
+  });
+}
 int mpir_gmres_ldl_set_factor_schedule(mpir_gmres_ldl_t h, int schedule) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;
@@ -777,7 +784,7 @@ int mpir_batch_plan_info(int32_t nsys, const int32_t* n, const int32_t* const* A
     own.reserve(nsys);
     std::vector<const PatternEntry*> raw;
     try {
-      for (int32_t s = 0; s < nsys; ++s) own.push_back(analyze_entry(n[s], Ap[s], Ai[s], ordering, true));
+      for (int32_t s = 0; s < nsys; ++s) own.push_back(analyze_entry(n[s], Ap[s], Ai[s], ordering, kDefaultAmalgamation));
     } catch (const std::invalid_argument& e) {
       throw Error(kErrInvalid, e.what());
     }
@@ -1022,6 +1029,14 @@ int mpir_gmres_ldl_time(mpir_gmres_ldl_t h, int what, int reps, double* avg_ms) 
     MPIR_HANDLE_CHECK;
     MPIR_CHECK(avg_ms, "null output");
     *avg_ms = h->s->time_kernels(what, reps);
+    return 0;
+  });
+}
+int mpir_gmres_ldl_trace_precond(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap, int64_t* ntask, int64_t* nfwd) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(ntask && nfwd && (out || cap == 0) && cap >= 0, "bad trace buffer");
+    *ntask = h->s->trace_precond(out, cap, nfwd);
     return 0;
   });
 }

@@ -1332,6 +1332,31 @@ int64_t GmresLdl::trace_factor(uint64_t* out, int64_t cap) {
   return nt;
 }
 
+int64_t GmresLdl::trace_precond(uint64_t* out, int64_t cap, int64_t* nfwd) {
+  MPIR_CHECK(factored_ && precond_ == kPrecLdl, "trace_precond needs a factored L D L^T handle");
+  MPIR_CHECK(dsp_.schedule == kScheduleDag, "trace_precond traces the dependency-driven schedule");
+  const int64_t nt = static_cast<int64_t>(dsp_.d.nfwd) + dsp_.d.nbwd;
+  *nfwd = dsp_.d.nfwd;
+  if (cap == 0 || nt == 0) return nt;
+  hipStream_t st = stream_;
+  d_strace_.ensure(static_cast<size_t>(nt) * 4 * sizeof(uint64_t));
+  MPIR_HIP(hipMemsetAsync(d_strace_.get(), 0, static_cast<size_t>(nt) * 4 * sizeof(uint64_t), st));
+  double* x = d_w_.as<double>();
+  MPIR_HIP(hipMemcpyAsync(x, d_ad_.get(), n_ * sizeof(double), hipMemcpyDeviceToDevice, st));
+  dsp_.d.trace = d_strace_.as<unsigned long long>();
+  try {
+    precond_apply_dev(x);
+  } catch (...) {
+    dsp_.d.trace = nullptr;
+    throw;
+  }
+  dsp_.d.trace = nullptr;
+  const int64_t n = std::min(nt, cap / 4) * 4;
+  MPIR_HIP(hipMemcpyAsync(out, d_strace_.get(), static_cast<size_t>(n) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  sync_check();
+  return nt;
+}
+
 double GmresLdl::time_kernels(int what, int reps) {
   if (!factored_) throw Error(kErrState, "time_kernels needs a factorization");
   MPIR_CHECK(reps >= 1 && what >= 0 && what <= 3, "bad timing request");

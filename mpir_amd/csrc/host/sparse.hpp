@@ -162,7 +162,9 @@ struct RelaxedPartition {
   std::vector<int32_t> first;
   std::vector<std::vector<int32_t>> pending;  // work space of the traversal
 };
-RelaxedPartition relax_supernodes(const CscUpper& C, const int32_t* parent, const int32_t* Lnz);
+// level 1 .. 3: how freely a child is merged into its parent
+// (larger fronts and a shallower tree for more explicit zeros).
+RelaxedPartition relax_supernodes(const CscUpper& C, const int32_t* parent, const int32_t* Lnz, int level = 1);
 // Supernodes for a given partition; R_s is the union structure.
 Supernodes build_supernodes_forced(const CscUpper& C, const int32_t* parent, const std::vector<int32_t>& first);
 

@@ -297,7 +297,7 @@ namespace mpir {
 // columns to just before the parent's, which is a topological reordering of
 // the elimination tree: the fill (and every L value) is unchanged.
 // Returns new -> old column order (of C's columns) and the block starts.
-RelaxedPartition relax_supernodes(const CscUpper& C, const int32_t* parent, const int32_t* Lnz) {
+RelaxedPartition relax_supernodes(const CscUpper& C, const int32_t* parent, const int32_t* Lnz, int level) {
   const int32_t n = C.n;
   // fundamental partition
   std::vector<int32_t> first{0};
@@ -338,7 +338,11 @@ RelaxedPartition relax_supernodes(const CscUpper& C, const int32_t* parent, cons
       const double ncm = static_cast<double>(nc[p] + nc[c]);
       const double z = zeros[p] + zeros[c] + static_cast<double>(nc[c]) * static_cast<double>(nc[p] + nr[p] - nr[c]);
       const double frac = z / std::max(1.0, entries(ncm, static_cast<double>(nr[p])));
-      const bool take = ncm <= 4 || (ncm <= 16 && frac < 0.8) || (ncm <= 48 && frac < 0.1) || frac < 0.05;
+      // merged front sizes up to which a zero fraction is accepted, per level
+      const double small = level >= 3 ? 64 : level == 2 ? 32 : 16;
+      const double mid = level >= 3 ? 128 : level == 2 ? 64 : 48;
+      const double mid_frac = level >= 2 ? 0.2 : 0.1;
+      const bool take = ncm <= 4 || (ncm <= small && frac < 0.8) || (ncm <= mid && frac < mid_frac) || frac < 0.05;
       if (!take) continue;
       merged[c] = 1;
       zeros[p] = z;

@@ -178,6 +178,11 @@ struct SolveDev {
   // (an iteration that has already stopped; read at kernel start, written
   // by earlier work of the same stream)
   const int* gate = nullptr;
+  // when set (dependency-driven schedule): per task of the forward list, then
+  // of the backward list, its kind | first supernode << 4 and the device wall
+  // clock when it was drawn, when its wait ended, when it was done (<< 16 |
+  // the workgroup)
+  unsigned long long* trace = nullptr;
 };
 
 struct DeviceSolvePlan {

@@ -989,6 +989,36 @@ __global__ void __launch_bounds__(256) k_level_tiles(SolveArgs<TF, TD> a, SolveD
 }
 
 // ---------------------------------------------------------------- dag schedule
+// A wait that also records, in *slot when that is set, the device clock at
+// which it ended (thread 0).
+template <class W>
+struct Traced {
+  W w;
+  unsigned long long* slot;
+  __device__ void operator()() const {
+    w();
+    if (slot != nullptr && threadIdx.x == 0) *slot = wall_clock64();
+  }
+};
+template <class W>
+__device__ __forceinline__ Traced<W> traced(W w, unsigned long long* slot) {
+  return Traced<W>{w, slot};
+}
+// the trace record of task t (nullptr: no trace): kind and supernode, the
+// clock when drawn
+__device__ __forceinline__ unsigned long long* trace_begin(const SolveDev& d, int64_t t, const Task4& tk, int s) {
+  if (d.trace == nullptr) return nullptr;
+  unsigned long long* r = d.trace + 4 * t;
+  if (threadIdx.x == 0) {
+    r[0] = static_cast<unsigned long long>(tk.a) | static_cast<unsigned long long>(s) << 4;
+    r[1] = wall_clock64();
+  }
+  return r;
+}
+__device__ __forceinline__ void trace_end(unsigned long long* r) {
+  if (r != nullptr && threadIdx.x == 0) r[3] = wall_clock64() << 16 | blockIdx.x;
+}
+
 // One right-hand side: four waves per SIMD (at most 128 registers), 1024
 // resident workgroups; several: two waves per SIMD (256 registers), so the
 // factor tile and the partial sums of one right-hand side stay in registers.
@@ -1006,26 +1036,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
     const int t = next_task(lines, ds, d.nfwd, &tsh);
     if (t >= d.nfwd) break;
     const Task4 tk = d.fwd[t];
+    unsigned long long* tr = trace_begin(d, t, tk, tk.a == kDagSmall ? d.small_list[tk.b] : tk.b);
+    unsigned long long* tw = tr != nullptr ? tr + 2 : nullptr;
     if (tk.a == kDagSmall) {
       if (w < tk.c) {
         const int s = d.small_list[tk.b + w];
         const int* ms = d.meta + kMetaStride * s;
-        small_fwd<NR>(a, d, s, x, uvec, WaveWait{cnt + kCDone * ns + s, ms[kMetaNChild], d.err});
+        small_fwd<NR>(a, d, s, x, uvec, traced(WaveWait{cnt + kCDone * ns + s, ms[kMetaNChild], d.err}, tw));
         if (ms[kMetaParent] >= 0) wave_publish(cnt + kCDone * ns + ms[kMetaParent]);
       }
     } else if (tk.a == kDagFx) {
       const int s = tk.b;
       fx_task<NR>(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), uvec, smem,
-                  BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err});
+                  traced(BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err}, tw));
     } else if (tk.a == kDagFu) {
       const int s = tk.b;
       fu_task<NR>(a, d, s, tk.c, tk.d, uvec, smem,
-                  BlockWait{cnt + kXDone * ns + s, d.meta[kMetaStride * s + kMetaNFx], d.err});
+                  traced(BlockWait{cnt + kXDone * ns + s, d.meta[kMetaStride * s + kMetaNFx], d.err}, tw));
     } else {  // kDagFm
       const int s = tk.b;
       fm_task<NR>(a, d, s, tk.c, static_cast<const TX*>(x), uvec, smem,
-                  BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err});
+                  traced(BlockWait{cnt + kCDone * ns + s, d.meta[kMetaStride * s + kMetaNChild], d.err}, tw));
     }
+    trace_end(tr);
   }
 }
 
@@ -1043,6 +1076,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
     const int t = next_task(lines, ds, d.nbwd, &tsh);
     if (t >= d.nbwd) break;
     const Task4 tk = d.bwd[t];
+    unsigned long long* tr =
+        trace_begin(d, static_cast<int64_t>(d.nfwd) + t, tk, tk.a == kDagSmall ? d.small_list[tk.b] : tk.b);
+    unsigned long long* tw = tr != nullptr ? tr + 2 : nullptr;
     if (tk.a == kDagSmall) {
       if (w < tk.c) {
         const int s = d.small_list[tk.b + w];
@@ -1050,7 +1086,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
         // a root waits for nothing: its target is 0
         const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
         const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
-        small_bwd<NR>(a, d, s, x, smem + kRB * NR * w, WaveWait{pc, target, d.err});
+        small_bwd<NR>(a, d, s, x, smem + kRB * NR * w, traced(WaveWait{pc, target, d.err}, tw));
         wave_publish(cnt + kBDone * ns + s);
       }
     } else if (tk.a == kDagBr) {
@@ -1058,18 +1094,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
       const int p = d.meta[kMetaStride * s + kMetaParent];
       const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
       const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
-      br_task<NR>(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), smem, BlockWait{pc, target, d.err});
+      br_task<NR>(a, d, s, tk.c, tk.d, static_cast<const TX*>(x), smem, traced(BlockWait{pc, target, d.err}, tw));
     } else if (tk.a == kDagBx) {
       const int s = tk.b;
       bx_task<NR>(a, d, s, tk.c, tk.d, x, smem, &flag,
-                  BlockWait{cnt + kRDone * ns + s, d.meta[kMetaStride * s + kMetaNBr], d.err});
+                  traced(BlockWait{cnt + kRDone * ns + s, d.meta[kMetaStride * s + kMetaNBr], d.err}, tw));
     } else {  // kDagBm
       const int s = tk.b;
       const int p = d.meta[kMetaStride * s + kMetaParent];
       const int* pc = cnt + kBDone * ns + (p >= 0 ? p : s);
       const int target = p >= 0 ? d.meta[kMetaStride * p + kMetaNBFinal] : 0;
-      bm_task<NR>(a, d, s, x, smem, BlockWait{pc, target, d.err});
+      bm_task<NR>(a, d, s, x, smem, traced(BlockWait{pc, target, d.err}, tw));
     }
+    trace_end(tr);
   }
 }
 

@@ -639,7 +639,7 @@ FactorPlan build_factor_plan_forced(const CscUpper& C, const std::vector<int32_t
   return P;
 }
 
-Analysis analyze_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, bool relax) {
+Analysis analyze_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai, int ordering, int relax) {
   Analysis R;
   CscUpper A;
   A.n = n;
@@ -664,8 +664,8 @@ Analysis analyze_pattern(int32_t n, const int32_t* Ap, const int32_t* Ai, int or
   int64_t lnz = etree_counts(n, R.C.p.data(), R.C.i.data(), work.data(), Lnz.data(), parent.data());
   if (lnz == -2) throw std::overflow_error("nonzeros of L overflow int32");
   if (lnz < 0) throw std::invalid_argument("etree failed");
-  if (ordering != 0 && relax) {
-    RelaxedPartition rp = relax_supernodes(R.C, parent.data(), Lnz.data());
+  if (ordering != 0 && relax > 0) {
+    RelaxedPartition rp = relax_supernodes(R.C, parent.data(), Lnz.data(), relax);
     std::vector<int32_t> perm2(n);
     for (int32_t k = 0; k < n; ++k) perm2[k] = R.perm[rp.order[k]];
     R.perm.swap(perm2);

@@ -129,7 +129,8 @@ int32_t Server::add_patterns(int32_t count, const int32_t* n, const int32_t* con
   std::vector<std::shared_ptr<const PatternEntry>> entries;
   try {
     entries = cache_.get_many(std::vector<int32_t>(n, n + count), std::vector<const int32_t*>(Ap, Ap + count),
-                              std::vector<const int32_t*>(Ai, Ai + count), opt_.ordering, true, threads);
+                              std::vector<const int32_t*>(Ai, Ai + count), opt_.ordering, kDefaultAmalgamation,
+                              threads);
   } catch (const std::invalid_argument& e) {
     throw Error(kErrInvalid, e.what());
   }

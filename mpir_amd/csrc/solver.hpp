@@ -144,9 +144,14 @@ class GmresLdl {
     MPIR_CHECK(mode >= kScaleNone && mode <= kScaleRuiz, "scaling mode must be 0, 1 or 2");
     scaling_ = mode;
   }
-  void set_relax(bool on) { relax_ = on; }
+  // supernode amalgamation: 0 none, 1 .. 3 (default; relax_supernodes);
+  // takes effect at the next analyze
+  void set_relax(int level) {
+    MPIR_CHECK(level >= 0 && level <= 3, "amalgamation level must be 0 .. 3");
+    relax_ = level;
+  }
   int ordering() const { return ordering_; }
-  bool relax() const { return relax_; }
+  int relax() const { return relax_; }
   // schedule of the triangular solves: kScheduleLevels or kScheduleDag
   void set_schedule(int sched) {
     MPIR_CHECK(sched == kScheduleLevels || sched == kScheduleDag, "solve schedule must be 0 (levels) or 1 (dag)");
@@ -237,6 +242,10 @@ class GmresLdl {
   // launch's task timeline (4 entries per task) in out[0 .. cap); returns
   // the number of tasks (cap = 0: only that)
   int64_t trace_factor(uint64_t* out, int64_t cap);
+  // one M^{-1} application (dependency-driven schedule) with the task
+  // timeline of both passes (4 entries per task, forward then backward) in
+  // out[0 .. cap); returns the number of tasks, *nfwd those of the forward pass
+  int64_t trace_precond(uint64_t* out, int64_t cap, int64_t* nfwd);
   // Times the factor's update grouping and super-block edge and the
   // triangular solves' narrow-front limit and schedule on this handle's
   // factor (reps runs per candidate), keeps the fastest of each and leaves
@@ -314,7 +323,7 @@ class GmresLdl {
   int factor_schedule_ = kScheduleLevels;
   FactorTuning ftune_;
   SolveTuning stune_;
-  bool relax_ = true;     // relaxed supernodes (AMD / ND orderings only)
+  int relax_ = kDefaultAmalgamation;  // relaxed supernodes, level 0 .. 3 (AMD / ND orderings only)
   bool relaxed_ = false;  // the current plan uses them
   std::vector<int32_t> Cp_, Ci_, parent_, Lnz_;  // pattern of P K P^T, etree
   int64_t true_nnz_l_ = 0;
@@ -343,7 +352,7 @@ class GmresLdl {
   // numeric
   DevBuf d_ax_, d_axs_, d_ad_, d_adp_, d_panel_, d_panel16_, d_upool_, d_D_, d_Dinv_, d_err_, d_cnt_, d_nonfinite_;
   DevBuf d_uvec_, d_scratch_, d_serr_, d_counters_;
-  DevBuf d_fdag_, d_fdag_meta_, d_fcounters_, d_ftrace_;  // dependency-driven factorization
+  DevBuf d_fdag_, d_fdag_meta_, d_fcounters_, d_ftrace_, d_strace_;  // dependency-driven factorization
   // GMRES workspace
   DevBuf d_V_, d_w_, d_b_, d_x_, d_r_, d_d_, d_red_, d_small_;
   DevBuf d_hcol_, d_givens_;

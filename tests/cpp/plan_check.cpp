@@ -439,6 +439,7 @@ int run() {
                         {3, 14, 686, 1, 4, 2}, {3, 22, 2662, 1, 4, 2}};
   const FactorTuning ft[] = {{4, 8}, {1, 1}, {2, 4}, {3, 16}, {8, 2}, {64, 64}};
   const SolveTuning st[] = {{1024}, {0}, {64}, {300}, {1 << 30}};
+  for (int amalg : {1, kDefaultAmalgamation})
   for (const Case& cs : cases) {
     KktSpec sp;
     sp.dim = cs.dim;
@@ -447,7 +448,7 @@ int run() {
     sp.a_mode = cs.a_mode;
     sp.cnt_lo = sp.cnt_hi = cs.cnt;
     const KktProblem kp = generate_kkt(sp);
-    Analysis an = analyze_pattern(kp.K.n, kp.K.p.data(), kp.K.i.data(), cs.ordering, true);
+    Analysis an = analyze_pattern(kp.K.n, kp.K.p.data(), kp.K.i.data(), cs.ordering, amalg);
     FactorPlan& P = an.plan;
     int32_t max_steps = 0;
     for (int32_t s : P.big_list) max_steps = std::max(max_steps, (P.sn.first[s + 1] - P.sn.first[s] + 63) / 64);
@@ -461,8 +462,9 @@ int run() {
       build_solve_schedule(P, s);
       check_solve(P);
     }
-    std::printf("n %d: %d supernodes, %lld big fronts, up to %d panel steps per front, %zu DAG tasks from level %d of %d: ok\n",
-                kp.K.n, P.sn.ns, static_cast<long long>(P.nbig), max_steps, P.fdag.size(), P.fdag_level, P.sn.nlevels);
+    std::printf("n %d, amalgamation %d: %d supernodes, %lld big fronts, up to %d panel steps per front, %zu DAG tasks from level %d of %d: ok\n",
+                kp.K.n, amalg, P.sn.ns, static_cast<long long>(P.nbig), max_steps, P.fdag.size(), P.fdag_level,
+                P.sn.nlevels);
   }
   // planted mistakes, in copies of a plan with many steps per front
   KktSpec sp;
@@ -470,7 +472,7 @@ int run() {
   sp.k = 40;
   sp.m = 400;
   const KktProblem kp = generate_kkt(sp);
-  Analysis an = analyze_pattern(kp.K.n, kp.K.p.data(), kp.K.i.data(), 2, true);
+  Analysis an = analyze_pattern(kp.K.n, kp.K.p.data(), kp.K.i.data(), 2, 1);
   build_update_pool(an.plan, true);
   build_factor_steps(an.plan, FactorTuning{4, 8});
   build_solve_schedule(an.plan, SolveTuning{1024});
