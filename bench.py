@@ -394,8 +394,9 @@ def serving(mpir, args) -> int:
     srv.add_patterns(pats, threads=16)
     analyze_s = time.perf_counter() - t0
     log(f"symbolic analysis of 64 patterns: {analyze_s:.1f} s")
-    # warm-up (allocations, code objects)
-    tw, pw, nw, sw = mpir.poisson_stream(200.0, 0.25, len(pats), seed=args.seed + 99)
+    # warm-up (code objects; every lane's device buffers grown to full
+    # batches, so that the timed streams do not wait on reallocations)
+    tw, pw, nw, sw = mpir.poisson_stream(args.rate, 0.3, len(pats), seed=args.seed + 99)
     srv.run(pw, sw, nw, tw)
 
     def one(rate, seconds, seed):
@@ -416,9 +417,9 @@ def serving(mpir, args) -> int:
                "max_backlog": summ["max_backlog"], "batches": summ["batches"],
                "mean_batch": round(float(sizes.mean()), 2) if len(sizes) else 0.0,
                "peak_inflight_GB": round(summ["peak_inflight_bytes"] / 1e9, 2),
-               "max_concurrent_batches": summ["max_concurrent"],
+               "max_concurrent_batches": summ["max_concurrent"], "device_regrowths": summ["device_regrowths"],
                "phase_ms_per_batch": {k: round(summ[k + "_ms"] / max(1, summ["batches"]), 2)
-                                      for k in ("prepare", "plan", "factor", "solve")},
+                                      for k in ("prepare", "plan", "plan_join", "plan_upload", "factor", "solve")},
                "rel_residual_max": float(out["rel_residual_max"][ok].max()) if ok.any() else None,
                "gmres_iterations_max": int(out["gmres_max"][ok].max()) if ok.any() else None}
         log(f"stream done: {res}")

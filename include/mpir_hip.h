@@ -467,6 +467,12 @@ int mpir_server_poll(mpir_server_t s, int64_t id, double timeout_ms, double* x, 
  * ms[0..4] = wall time of the run, then summed over batches: request data,
  * joined symbolic plan and upload, numeric factorization, GMRES-IR */
 int mpir_server_summary(mpir_server_t s, int64_t* counts, double* ms);
+/* the two parts of the summary's plan time, summed over batches: joining
+ * the systems' plans on the host, and the device copies, allocations and
+ * the wait for them; and how often a device buffer of the process had to
+ * grow during the run (each such reallocation waits for the whole device)
+ * (no counterpart in the reference) */
+int mpir_server_plan_phases(mpir_server_t s, double* join_ms, double* upload_ms, int64_t* regrowths);
 /* solution of request i of the last run (keep_x != 0): n x nrhs, column-major */
 int mpir_server_solution(mpir_server_t s, int64_t i, double* x);
 /* the data of a request, on the host (Ax: nnz, b and x_true: n x nrhs) */

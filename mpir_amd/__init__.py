@@ -638,6 +638,9 @@ class Server:
         d = {k: int(v) for k, v in zip(keys, c)}
         for k, v in zip(["wall_ms", "prepare_ms", "plan_ms", "factor_ms", "solve_ms"], w):
             d[k] = float(v)
+        j, u, g = C.c_double(), C.c_double(), C.c_int64()
+        _check(load().mpir_server_plan_phases(self._h, C.byref(j), C.byref(u), C.byref(g)), "summary")
+        d["plan_join_ms"], d["plan_upload_ms"], d["device_regrowths"] = j.value, u.value, g.value
         return d
 
     def solution(self, i: int) -> np.ndarray:

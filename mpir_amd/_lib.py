@@ -136,6 +136,7 @@ SIGNATURES = {
     "mpir_server_add_patterns": (C.c_int, [P, i32, P, P, P, P, P, i32]),
     "mpir_server_run": (C.c_int, [P, i64, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mpir_server_summary": (C.c_int, [P, P, P]),
+    "mpir_server_plan_phases": (C.c_int, [P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "mpir_server_submit": (C.c_int, [P, i32, P, P, i32, P]),
     "mpir_server_poll": (C.c_int, [P, i64, f64, P, P, i32]),
     "mpir_server_solution": (C.c_int, [P, i64, P]),

@@ -20,6 +20,10 @@
 // only: a system gives bit-identical results alone and in any batch.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <cstring>
+#include <thread>
+
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -524,6 +528,44 @@ int32_t GmresLdl::batch_n(int32_t s) const {
   return batch_->col_base[s + 1] - batch_->col_base[s];
 }
 
+void GmresLdl::flush_staged() {
+  if (staged_.empty()) return;
+  // 256-byte aligned places in the pinned buffer
+  std::vector<size_t> off(staged_.size() + 1, 0);
+  for (size_t i = 0; i < staged_.size(); ++i) off[i + 1] = off[i] + (staged_[i].bytes + 255) / 256 * 256;
+  const size_t total = off.back();
+  if (total > stage_cap_) {
+    MPIR_HIP(hipStreamSynchronize(stream_));  // no copy may still read the old buffer
+    if (stage_pinned_) MPIR_HIP(hipHostFree(stage_pinned_));
+    stage_pinned_ = nullptr;
+    stage_cap_ = 0;
+    const size_t cap = std::max<size_t>(total + total / 4, size_t(1) << 24);
+    MPIR_HIP(hipHostMalloc(reinterpret_cast<void**>(&stage_pinned_), cap, hipHostMallocDefault));
+    stage_cap_ = cap;
+  }
+  // host copies in 4 MB pieces on up to 8 threads
+  constexpr size_t kPiece = size_t(4) << 20;
+  std::vector<std::pair<size_t, size_t>> pieces;  // (array, byte offset)
+  for (size_t i = 0; i < staged_.size(); ++i)
+    for (size_t b = 0; b < staged_[i].bytes; b += kPiece) pieces.push_back({i, b});
+  std::atomic<size_t> next{0};
+  auto work = [&] {
+    for (size_t q; (q = next.fetch_add(1)) < pieces.size();) {
+      const Staged& e = staged_[pieces[q].first];
+      const size_t b = pieces[q].second, len = std::min(kPiece, e.bytes - b);
+      std::memcpy(stage_pinned_ + off[pieces[q].first] + b, static_cast<const char*>(e.src) + b, len);
+    }
+  };
+  const int nt = static_cast<int>(std::min<size_t>(8, std::max<size_t>(1, pieces.size())));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
+  for (size_t i = 0; i < staged_.size(); ++i)
+    MPIR_HIP(hipMemcpyAsync(staged_[i].dst, stage_pinned_ + off[i], staged_[i].bytes, hipMemcpyHostToDevice, stream_));
+  staged_.clear();
+}
+
 double* GmresLdl::bpinned(size_t count) {
   if (count > b_pinned_count_) {
     MPIR_HIP(hipStreamSynchronize(stream_));  // no copy may still use the old buffer
@@ -563,6 +605,7 @@ void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntr
     batch_.reset();
     throw Error(kErrInvalid, e.what());
   }
+  const double t_join = now_ms();
   batch_parts_ = parts;
   hipStream_t st = stream_;
   n_ = B->col_base.back();
@@ -577,15 +620,24 @@ void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntr
   relaxed_ = B->relaxed;
   true_nnz_l_ = B->nnz_l;
   true_flops_ = B->flops;
+  // every upload below goes through the pinned staging buffer
+  staging_ = true;
+  struct StageOff {
+    GmresLdl* g;
+    ~StageOff() {
+      g->staging_ = false;
+      g->staged_.clear();
+    }
+  } stage_off{this};
   upload_plan();
-  upload_vec(d_csr_p_, B->csr_p, st);
-  upload_vec(d_csr_j_, B->csr_j, st);
-  upload_vec(d_csr_src_, B->csr_src, st);
+  up(d_csr_p_, B->csr_p);
+  up(d_csr_j_, B->csr_j);
+  up(d_csr_src_, B->csr_src);
   d_csr_x_.alloc(std::max<size_t>(1, B->csr_j.size()) * sizeof(double));
-  upload_vec(d_entry_col_, B->ecol, st);
-  upload_vec(d_entry_row_, B->erow, st);
-  upload_vec(d_diag_pos_, B->diag_pos, st);
-  upload_vec(d_perm_, perm_, st);
+  up(d_entry_col_, B->ecol);
+  up(d_entry_row_, B->erow);
+  up(d_diag_pos_, B->diag_pos);
+  up(d_perm_, perm_);
   // reduction chunks: every system's rows cut into pieces of kChunk
   std::vector<int32_t> ch_start, ch_len, sys_ch(1, 0);
   for (int32_t s = 0; s < B->nsys; ++s) {
@@ -596,14 +648,16 @@ void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntr
     sys_ch.push_back(static_cast<int32_t>(ch_start.size()));
   }
   b_nch_ = static_cast<int32_t>(ch_start.size());
-  upload_vec(d_b_sys_of_row_, B->sys_of_row, st);
-  upload_vec(d_b_col_base_, B->col_base, st);
-  upload_vec(d_b_lperm_, B->local_perm, st);
-  upload_vec(d_b_ch_start_, ch_start, st);
-  upload_vec(d_b_ch_len_, ch_len, st);
-  upload_vec(d_b_sys_ch_, sys_ch, st);
+  up(d_b_sys_of_row_, B->sys_of_row);
+  up(d_b_col_base_, B->col_base);
+  up(d_b_lperm_, B->local_perm);
+  up(d_b_ch_start_, ch_start);
+  up(d_b_ch_len_, ch_len);
+  up(d_b_sys_ch_, sys_ch);
   d_b_piv_.alloc(2 * sizeof(int32_t) * B->nsys);
   alloc_numeric();
+  flush_staged();
+  staging_ = false;
   MPIR_HIP(hipStreamSynchronize(st));
 
   const Supernodes& S = plan_->sn;
@@ -620,6 +674,8 @@ void GmresLdl::analyze_batch(const std::vector<std::shared_ptr<const PatternEntr
   bstats_.nsys = batch_->nsys;
   analyzed_ = true;
   stats_.analyze_ms = now_ms() - t0;
+  bstats_.join_ms = t_join - t0;
+  bstats_.upload_ms = stats_.analyze_ms - bstats_.join_ms;
 }
 
 void GmresLdl::factor_batch(const double* const* Ax, bool device_ptr) {

@@ -201,14 +201,16 @@ size_t PatternCache::size() const {
 
 void merge_plans(const std::vector<const FactorPlan*>& P, const std::vector<int32_t>& col_base, FactorPlan& M) {
   const size_t S = P.size();
-  // every list starts empty; the vectors keep their memory
+  // these lists start empty (the vectors keep their memory)
   for (auto* v : {&M.child_ptr, &M.child_list, &M.small_ptr, &M.small_list, &M.big_ptr, &M.big_list, &M.asm_ptr,
-                  &M.asm_child_ptr, &M.asm_child, &M.level_step_ptr, &M.pan_ptr, &M.upd_ptr, &M.pan_aug_ptr,
-                  &M.upd_aug_ptr, &M.fx_ptr, &M.fu_ptr, &M.br_ptr, &M.bx_ptr, &M.fx_task, &M.fu_task, &M.br_task,
-                  &M.bx_task, &M.fm_ptr, &M.bm_ptr, &M.fm_task, &M.bm_task, &M.inv_task, &M.frow_off, &M.grow_ptr, &M.cb_base, &M.dag_meta, &M.sn.level_ptr, &M.sn.level_sn})
+                  &M.level_step_ptr, &M.fx_ptr, &M.fu_ptr, &M.br_ptr, &M.bx_ptr, &M.fx_task, &M.fu_task, &M.br_task,
+                  &M.bx_task, &M.fm_ptr, &M.bm_ptr, &M.fm_task, &M.bm_task, &M.inv_task, &M.cb_base, &M.dag_meta, &M.sn.level_ptr, &M.sn.level_sn})
     v->clear();
-  for (auto* v : {&M.upd_off, &M.ypart_off, &M.upart_off, &M.rpart_off, &M.xpart_off, &M.val_dst}) v->clear();
-  for (auto* v : {&M.asm_task, &M.pan_task, &M.upd_task, &M.gent, &M.dag_fwd, &M.dag_bwd}) v->clear();
+  for (auto* v : {&M.upd_off, &M.ypart_off, &M.upart_off, &M.rpart_off, &M.xpart_off}) v->clear();
+  for (auto* v : {&M.dag_fwd, &M.dag_bwd}) v->clear();
+  // asm_task, pan_task, upd_task, gent, frow_off, grow_ptr (and val_dst, in
+  // combine_patterns) are resized and then written entry by entry: not
+  // clearing them first means that only growth is zero-filled
   Supernodes& T = M.sn;
   // bases of every system in the joined numbering
   struct Base {
@@ -359,67 +361,120 @@ void merge_plans(const std::vector<const FactorPlan*>& P, const std::vector<int3
   // tunings; the handle re-plans the joined forest for its own
   M.tuning = S > 0 ? P[0]->tuning : SolveTuning{};
   M.ftuning = S > 0 ? P[0]->ftuning : FactorTuning{};
-  // assembly tasks with their child ranges
-  M.asm_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
-  M.asm_child_ptr.assign(1, 0);
-  for (int32_t l = 0; l < nlev; ++l) {
-    for (size_t s = 0; s < S; ++s) {
+  // assembly tasks with their child ranges, and the panel and update tasks
+  // per (level, step): the real rows of every system, then their identity
+  // rows. Positions come from counts per (level, system) and (level, step,
+  // pass, system); the copies then run in parallel per system.
+  {
+    const size_t LS = static_cast<size_t>(nlev) * S;
+    std::vector<int64_t> at(LS + 1, 0), ct(LS + 1, 0);
+    for (int32_t l = 0; l < nlev; ++l)
+      for (size_t s = 0; s < S; ++s) {
+        const FactorPlan& p = *P[s];
+        const size_t i = static_cast<size_t>(l) * S + s;
+        int64_t na = 0, nc = 0;
+        if (l < p.sn.nlevels) {
+          na = p.asm_ptr[l + 1] - p.asm_ptr[l];
+          nc = p.asm_child_ptr[p.asm_ptr[l + 1]] - p.asm_child_ptr[p.asm_ptr[l]];
+        }
+        at[i + 1] = at[i] + na;
+        ct[i + 1] = ct[i] + nc;
+      }
+    if (at[LS] >= INT32_MAX || 5 * ct[LS] >= INT32_MAX) throw std::length_error("batch exceeds the 32-bit task indices");
+    M.asm_task.resize(static_cast<size_t>(at[LS]));
+    M.asm_child.resize(static_cast<size_t>(5 * ct[LS]));
+    M.asm_child_ptr.resize(static_cast<size_t>(at[LS]) + 1);
+    M.asm_child_ptr[at[LS]] = static_cast<int32_t>(ct[LS]);
+    M.asm_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
+    for (int32_t l = 0; l < nlev; ++l) M.asm_ptr[l + 1] = static_cast<int32_t>(at[static_cast<size_t>(l + 1) * S]);
+    // steps: per level the most of any system; (level, step) pairs in order
+    M.level_step_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
+    for (int32_t l = 0; l < nlev; ++l) {
+      int32_t steps = 0;
+      for (size_t s = 0; s < S; ++s) {
+        const FactorPlan& p = *P[s];
+        if (l < p.sn.nlevels) steps = std::max(steps, p.level_step_ptr[l + 1] - p.level_step_ptr[l]);
+      }
+      M.level_step_ptr[l + 1] = M.level_step_ptr[l] + steps;
+    }
+    const int32_t G = M.level_step_ptr[nlev];
+    const size_t GS = static_cast<size_t>(G) * 2 * S;
+    std::vector<int64_t> pt(GS + 1, 0), ut(GS + 1, 0);
+    // source range of (level l, step k, pass) in system s, or empty
+    auto src = [&](const FactorPlan& p, int32_t l, int32_t k, int pass, int32_t& pa, int32_t& pe, int32_t& ua,
+                   int32_t& ue) {
+      pa = pe = ua = ue = 0;
+      if (l >= p.sn.nlevels || k >= p.level_step_ptr[l + 1] - p.level_step_ptr[l]) return;
+      const int32_t g = p.level_step_ptr[l] + k;
+      pa = pass == 0 ? p.pan_ptr[g] : p.pan_aug_ptr[g];
+      pe = pass == 0 ? p.pan_aug_ptr[g] : p.pan_ptr[g + 1];
+      ua = pass == 0 ? p.upd_ptr[g] : p.upd_aug_ptr[g];
+      ue = pass == 0 ? p.upd_aug_ptr[g] : p.upd_ptr[g + 1];
+    };
+    for (int32_t l = 0; l < nlev; ++l)
+      for (int32_t g = M.level_step_ptr[l]; g < M.level_step_ptr[l + 1]; ++g)
+        for (int pass = 0; pass < 2; ++pass)
+          for (size_t s = 0; s < S; ++s) {
+            int32_t pa, pe, ua, ue;
+            src(*P[s], l, g - M.level_step_ptr[l], pass, pa, pe, ua, ue);
+            const size_t i = (static_cast<size_t>(g) * 2 + pass) * S + s;
+            pt[i + 1] = pt[i] + (pe - pa);
+            ut[i + 1] = ut[i] + (ue - ua);
+          }
+    if (pt[GS] >= INT32_MAX || ut[GS] >= INT32_MAX) throw std::length_error("batch exceeds the 32-bit task indices");
+    M.pan_task.resize(static_cast<size_t>(pt[GS]));
+    M.upd_task.resize(static_cast<size_t>(ut[GS]));
+    M.pan_ptr.resize(static_cast<size_t>(G) + 1);
+    M.upd_ptr.resize(static_cast<size_t>(G) + 1);
+    M.pan_aug_ptr.resize(G);
+    M.upd_aug_ptr.resize(G);
+    for (int32_t g = 0; g <= G; ++g) {
+      M.pan_ptr[g] = static_cast<int32_t>(pt[static_cast<size_t>(g) * 2 * S]);
+      M.upd_ptr[g] = static_cast<int32_t>(ut[static_cast<size_t>(g) * 2 * S]);
+    }
+    for (int32_t g = 0; g < G; ++g) {
+      M.pan_aug_ptr[g] = static_cast<int32_t>(pt[(static_cast<size_t>(g) * 2 + 1) * S]);
+      M.upd_aug_ptr[g] = static_cast<int32_t>(ut[(static_cast<size_t>(g) * 2 + 1) * S]);
+    }
+    parallel_for(static_cast<int64_t>(S), join_threads(), [&](int64_t s) {
       const FactorPlan& p = *P[s];
-      if (l >= p.sn.nlevels) continue;
-      for (int32_t t = p.asm_ptr[l]; t < p.asm_ptr[l + 1]; ++t) {
-        Task4 tk = p.asm_task[t];
-        tk.a += b[s].sn;
-        M.asm_task.push_back(tk);
-        for (int32_t c = p.asm_child_ptr[t]; c < p.asm_child_ptr[t + 1]; ++c) {
+      const int32_t sn = b[s].sn;
+      for (int32_t l = 0; l < nlev && l < p.sn.nlevels; ++l) {
+        const size_t i = static_cast<size_t>(l) * S + s;
+        const int32_t t0 = p.asm_ptr[l], c0 = p.asm_child_ptr[t0];
+        for (int32_t t = t0; t < p.asm_ptr[l + 1]; ++t) {
+          Task4 tk = p.asm_task[t];
+          tk.a += sn;
+          M.asm_task[at[i] + (t - t0)] = tk;
+          M.asm_child_ptr[at[i] + (t - t0)] = static_cast<int32_t>(ct[i] + (p.asm_child_ptr[t] - c0));
+        }
+        for (int32_t c = c0; c < p.asm_child_ptr[p.asm_ptr[l + 1]]; ++c) {
           const int32_t* e = &p.asm_child[5 * static_cast<size_t>(c)];
-          M.asm_child.insert(M.asm_child.end(), {b[s].sn + e[0], e[1], e[2], e[3], e[4]});
+          int32_t* d = &M.asm_child[5 * static_cast<size_t>(ct[i] + (c - c0))];
+          d[0] = sn + e[0];
+          d[1] = e[1];
+          d[2] = e[2];
+          d[3] = e[3];
+          d[4] = e[4];
         }
-        M.asm_child_ptr.push_back(static_cast<int32_t>(M.asm_child.size() / 5));
-      }
-    }
-    M.asm_ptr[l + 1] = static_cast<int32_t>(M.asm_task.size());
-  }
-  // panel and update tasks per (level, step): the real rows of every
-  // system, then their identity rows
-  M.level_step_ptr.assign(static_cast<size_t>(nlev) + 1, 0);
-  M.pan_ptr.assign(1, 0);
-  M.upd_ptr.assign(1, 0);
-  for (int32_t l = 0; l < nlev; ++l) {
-    int32_t steps = 0;
-    for (size_t s = 0; s < S; ++s) {
-      const FactorPlan& p = *P[s];
-      if (l < p.sn.nlevels) steps = std::max(steps, p.level_step_ptr[l + 1] - p.level_step_ptr[l]);
-    }
-    for (int32_t k = 0; k < steps; ++k) {
-      for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) {
-          M.pan_aug_ptr.push_back(static_cast<int32_t>(M.pan_task.size()));
-          M.upd_aug_ptr.push_back(static_cast<int32_t>(M.upd_task.size()));
-        }
-        for (size_t s = 0; s < S; ++s) {
-          const FactorPlan& p = *P[s];
-          if (l >= p.sn.nlevels || k >= p.level_step_ptr[l + 1] - p.level_step_ptr[l]) continue;
-          const int32_t g = p.level_step_ptr[l] + k;
-          const int32_t pa = pass == 0 ? p.pan_ptr[g] : p.pan_aug_ptr[g];
-          const int32_t pe = pass == 0 ? p.pan_aug_ptr[g] : p.pan_ptr[g + 1];
-          for (int32_t t = pa; t < pe; ++t) {
-            Task4 tk = p.pan_task[t];
-            tk.a += b[s].sn;
-            M.pan_task.push_back(tk);
+        for (int32_t g = M.level_step_ptr[l]; g < M.level_step_ptr[l + 1]; ++g)
+          for (int pass = 0; pass < 2; ++pass) {
+            int32_t pa, pe, ua, ue;
+            src(p, l, g - M.level_step_ptr[l], pass, pa, pe, ua, ue);
+            const size_t j = (static_cast<size_t>(g) * 2 + pass) * S + s;
+            for (int32_t t = pa; t < pe; ++t) {
+              Task4 tk = p.pan_task[t];
+              tk.a += sn;
+              M.pan_task[pt[j] + (t - pa)] = tk;
+            }
+            for (int32_t t = ua; t < ue; ++t) {
+              Task4 tk = p.upd_task[t];
+              tk.a += sn;
+              M.upd_task[ut[j] + (t - ua)] = tk;
+            }
           }
-          const int32_t ua = pass == 0 ? p.upd_ptr[g] : p.upd_aug_ptr[g];
-          const int32_t ue = pass == 0 ? p.upd_aug_ptr[g] : p.upd_ptr[g + 1];
-          for (int32_t t = ua; t < ue; ++t) {
-            Task4 tk = p.upd_task[t];
-            tk.a += b[s].sn;
-            M.upd_task.push_back(tk);
-          }
-        }
       }
-      M.pan_ptr.push_back(static_cast<int32_t>(M.pan_task.size()));
-      M.upd_ptr.push_back(static_cast<int32_t>(M.upd_task.size()));
-    }
-    M.level_step_ptr[l + 1] = M.level_step_ptr[l] + steps;
+    });
   }
   // row gathers of the solves
   M.frow_off.resize(static_cast<size_t>(T.ns) + 1);
@@ -444,7 +499,6 @@ void merge_plans(const std::vector<const FactorPlan*>& P, const std::vector<int3
   M.gent[E.gent + 1] = Task4{0, 0, 0, 0};
   M.nsmall = static_cast<int64_t>(M.small_list.size());
   M.nbig = static_cast<int64_t>(M.big_list.size());
-  build_factor_dag(M);
   build_dag_schedule(M);
 }
 

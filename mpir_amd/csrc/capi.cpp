@@ -903,6 +903,16 @@ int mpir_server_poll(mpir_server_t h, int64_t id, double timeout_ms, double* x, 
     return 1;
   });
 }
+int mpir_server_plan_phases(mpir_server_t h, double* join_ms, double* upload_ms, int64_t* regrowths) {
+  return guarded([&] {
+    MPIR_CHECK(h && h->s, "null server handle");
+    const ServeSummary m = h->s->summary();
+    if (join_ms) *join_ms = m.plan_join_ms;
+    if (upload_ms) *upload_ms = m.plan_upload_ms;
+    if (regrowths) *regrowths = m.regrowths;
+    return 0;
+  });
+}
 int mpir_server_summary(mpir_server_t h, int64_t* counts, double* wall_ms) {
   return guarded([&] {
     MPIR_CHECK(h && h->s, "null server handle");

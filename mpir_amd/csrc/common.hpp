@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <stdexcept>
@@ -69,16 +71,25 @@ class DevBuf {
   // Contents are not preserved. An allocation that already holds `bytes`
   // is kept (buffers only grow): hipFree waits for the whole device, so
   // handles that are re-analyzed for every batch of systems reuse their
-  // memory instead.
+  // memory instead, and a buffer that has to grow at least doubles, so that
+  // growing stays rare (a serving lane's first batches are small).
   void alloc(size_t bytes) {
     if (p_ && bytes <= cap_) {
       n_ = bytes;
       return;
     }
+    const size_t want = p_ ? std::max(bytes, 2 * cap_) : bytes;
+    if (p_) regrowths().fetch_add(1, std::memory_order_relaxed);
     release();
     if (bytes == 0) return;
-    MPIR_HIP(hipMalloc(&p_, bytes));
-    n_ = cap_ = bytes;
+    MPIR_HIP(hipMalloc(&p_, want));
+    n_ = bytes;
+    cap_ = want;
+  }
+  // device buffers of the process that had to grow (each waits for the device)
+  static std::atomic<int64_t>& regrowths() {
+    static std::atomic<int64_t> n{0};
+    return n;
   }
   // grow-only reallocation (contents not preserved)
   void ensure(size_t bytes) {

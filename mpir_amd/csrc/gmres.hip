@@ -518,6 +518,7 @@ GmresLdl::~GmresLdl() {
   if (bgraph_) (void)hipGraphExecDestroy(bgraph_);
   if (h_pinned_) (void)hipHostFree(h_pinned_);
   if (b_pinned_) (void)hipHostFree(b_pinned_);
+  if (stage_pinned_) (void)hipHostFree(stage_pinned_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -606,29 +607,57 @@ void GmresLdl::analyze(int32_t n, const int32_t* Ap, const int32_t* Ai) {
   stats_.analyze_ms = now_ms() - t0;
 }
 
+// Device copies of the dependency-driven factorization's task list,
+// counter offsets and counters (empty unless the pool is laid out for it).
+void GmresLdl::upload_factor_dag() {
+  const FactorPlan& P = *plan_;
+  const Supernodes& S = P.sn;
+  hipStream_t st = stream_;
+  dfp_.use_dag = factor_schedule_ == kScheduleDag;
+  dfp_.dag_pool = P.dag_pool;
+  dfp_.dag_level = P.fdag_level;
+  FactorDagDev& g = dfp_.dag;
+  g = FactorDagDev();
+  up(d_fdag_, P.fdag);
+  up(d_fdag_meta_, P.fdag_meta);
+  g.task = d_fdag_.as<FTask>();
+  g.ntask = static_cast<int>(P.fdag.size());
+  g.meta = d_fdag_meta_.as<int32_t>();
+  g.asm_task = dfp_.asm_task;
+  g.asm_child_ptr = dfp_.asm_child_ptr;
+  g.asm_child = dfp_.asm_child;
+  g.ns = S.ns;
+  g.nsteps = P.fdag_nsteps;
+  const FactorDagIndex X{S.ns, P.fdag_nsteps};
+  g.cnt_disp = (X.col_done(P.fdag_ncb) + kDispenserStride - 1) / kDispenserStride * kDispenserStride;
+  g.ncounters = g.cnt_disp + kDispenserLines * kDispenserStride;
+  d_fcounters_.alloc(static_cast<size_t>(g.ncounters) * sizeof(int));
+  g.counters = d_fcounters_.as<int>();
+}
+
 void GmresLdl::upload_plan() {
   ++plan_epoch_;
   const FactorPlan& P = *plan_;
   const Supernodes& S = P.sn;
   hipStream_t st = stream_;
-  upload_vec(d_first_, S.first, st);
-  upload_vec(d_nrow_, S.nrow, st);
-  upload_vec(d_rows_off_, S.rows_off, st);
-  upload_vec(d_rows_, S.rows, st);
-  upload_vec(d_relidx_, S.relidx, st);
-  upload_vec(d_panel_off_, S.panel_off, st);
-  upload_vec(d_upd_off_, P.upd_off, st);
-  upload_vec(d_child_ptr_, P.child_ptr, st);
-  upload_vec(d_child_list_, P.child_list, st);
+  up(d_first_, S.first);
+  up(d_nrow_, S.nrow);
+  up(d_rows_off_, S.rows_off);
+  up(d_rows_, S.rows);
+  up(d_relidx_, S.relidx);
+  up(d_panel_off_, S.panel_off);
+  up(d_upd_off_, P.upd_off);
+  up(d_child_ptr_, P.child_ptr);
+  up(d_child_list_, P.child_list);
   // the device buffers are kept across re-analyses (a new pattern reuses
   // their memory: freeing would wait for the whole device)
   if (keep_.size() < 40) keep_.resize(40);
-  upload_vec(keep_[0], P.small_list, st);
-  upload_vec(keep_[1], P.big_list, st);
-  upload_vec(keep_[2], P.asm_task, st);
-  upload_vec(keep_[3], P.pan_task, st);
-  upload_vec(keep_[4], P.upd_task, st);
-  upload_vec(keep_[5], P.val_dst, st);
+  up(keep_[0], P.small_list);
+  up(keep_[1], P.big_list);
+  up(keep_[2], P.asm_task);
+  up(keep_[3], P.pan_task);
+  up(keep_[4], P.upd_task);
+  up(keep_[5], P.val_dst);
   dfp_ = DeviceFactorPlan();
   dfp_.nlevels = S.nlevels;
   dfp_.panel_entries = P.panel_entries;
@@ -647,33 +676,11 @@ void GmresLdl::upload_plan() {
   dfp_.h_pan_aug_ptr = P.pan_aug_ptr;
   dfp_.n_inv_task = static_cast<int>(P.inv_task.size() / 3);
   dfp_.h_upd_aug_ptr = P.upd_aug_ptr;
-  upload_vec(keep_[15], P.asm_child_ptr, st);
-  upload_vec(keep_[16], P.asm_child, st);
+  up(keep_[15], P.asm_child_ptr);
+  up(keep_[16], P.asm_child);
   dfp_.asm_child_ptr = keep_[15].as<int32_t>();
   dfp_.asm_child = keep_[16].as<int32_t>();
-  // dependency-driven factorization of the levels >= fdag_level
-  dfp_.use_dag = factor_schedule_ == kScheduleDag;
-  dfp_.dag_pool = P.dag_pool;
-  dfp_.dag_level = P.fdag_level;
-  {
-    FactorDagDev& g = dfp_.dag;
-    g = FactorDagDev();
-    upload_vec(d_fdag_, P.fdag, st);
-    upload_vec(d_fdag_meta_, P.fdag_meta, st);
-    g.task = d_fdag_.as<FTask>();
-    g.ntask = static_cast<int>(P.fdag.size());
-    g.meta = d_fdag_meta_.as<int32_t>();
-    g.asm_task = dfp_.asm_task;
-    g.asm_child_ptr = dfp_.asm_child_ptr;
-    g.asm_child = dfp_.asm_child;
-    g.ns = S.ns;
-    g.nsteps = P.fdag_nsteps;
-    const FactorDagIndex X{S.ns, P.fdag_nsteps};
-    g.cnt_disp = (X.col_done(P.fdag_ncb) + kDispenserStride - 1) / kDispenserStride * kDispenserStride;
-    g.ncounters = g.cnt_disp + kDispenserLines * kDispenserStride;
-    d_fcounters_.alloc(static_cast<size_t>(g.ncounters) * sizeof(int));
-    g.counters = d_fcounters_.as<int>();
-  }
+  upload_factor_dag();
   // triangular solves
   dsp_ = DeviceSolvePlan();
   dsp_.schedule = schedule_;
@@ -690,29 +697,29 @@ void GmresLdl::upload_plan() {
   g.nfwd = static_cast<int>(P.dag_fwd.size());
   g.nbwd = static_cast<int>(P.dag_bwd.size());
   int k = 17;
-  auto up = [&](const auto& v) {
-    upload_vec(keep_[k], v, st);
+  auto upk = [&](const auto& v) {
+    up(keep_[k], v);
     return keep_[k++].get();
   };
-  g.fwd = static_cast<const Task4*>(up(P.dag_fwd));
-  g.bwd = static_cast<const Task4*>(up(P.dag_bwd));
-  g.meta = static_cast<const int32_t*>(up(P.dag_meta));
+  g.fwd = static_cast<const Task4*>(upk(P.dag_fwd));
+  g.bwd = static_cast<const Task4*>(upk(P.dag_bwd));
+  g.meta = static_cast<const int32_t*>(upk(P.dag_meta));
   g.small_list = dfp_.small_list;
-  g.fx_task = static_cast<const int32_t*>(up(P.fx_task));
-  dfp_.inv_task = static_cast<const int32_t*>(up(P.inv_task));
-  g.fu_task = static_cast<const int32_t*>(up(P.fu_task));
-  g.br_task = static_cast<const int32_t*>(up(P.br_task));
-  g.bx_task = static_cast<const int32_t*>(up(P.bx_task));
-  g.fm_task = static_cast<const int32_t*>(up(P.fm_task));
-  g.bm_task = static_cast<const int32_t*>(up(P.bm_task));
-  g.frow_off = static_cast<const int32_t*>(up(P.frow_off));
-  g.grow_ptr = static_cast<const int32_t*>(up(P.grow_ptr));
-  g.gent = static_cast<const Task4*>(up(P.gent));
-  g.cb_base = static_cast<const int32_t*>(up(P.cb_base));
-  g.ypart_off = static_cast<const int64_t*>(up(P.ypart_off));
-  g.upart_off = static_cast<const int64_t*>(up(P.upart_off));
-  g.rpart_off = static_cast<const int64_t*>(up(P.rpart_off));
-  g.xpart_off = static_cast<const int64_t*>(up(P.xpart_off));
+  g.fx_task = static_cast<const int32_t*>(upk(P.fx_task));
+  dfp_.inv_task = static_cast<const int32_t*>(upk(P.inv_task));
+  g.fu_task = static_cast<const int32_t*>(upk(P.fu_task));
+  g.br_task = static_cast<const int32_t*>(upk(P.br_task));
+  g.bx_task = static_cast<const int32_t*>(upk(P.bx_task));
+  g.fm_task = static_cast<const int32_t*>(upk(P.fm_task));
+  g.bm_task = static_cast<const int32_t*>(upk(P.bm_task));
+  g.frow_off = static_cast<const int32_t*>(upk(P.frow_off));
+  g.grow_ptr = static_cast<const int32_t*>(upk(P.grow_ptr));
+  g.gent = static_cast<const Task4*>(upk(P.gent));
+  g.cb_base = static_cast<const int32_t*>(upk(P.cb_base));
+  g.ypart_off = static_cast<const int64_t*>(upk(P.ypart_off));
+  g.upart_off = static_cast<const int64_t*>(upk(P.upart_off));
+  g.rpart_off = static_cast<const int64_t*>(upk(P.rpart_off));
+  g.xpart_off = static_cast<const int64_t*>(upk(P.xpart_off));
   MPIR_CHECK(k <= 40, "plan upload slots");
   solve_slots_ = 0;
   set_solve_slots(1);
@@ -1444,7 +1451,7 @@ void GmresLdl::set_factor_schedule(int sched) {
     upload_vec(d_upd_off_, plan_->upd_off, stream_);
     const size_t tf = uf_ == kF64 ? sizeof(double) : sizeof(float);
     d_upool_.alloc(std::max<int64_t>(1, plan_->upd_pool) * tf);
-    dfp_.dag_pool = true;
+    upload_factor_dag();
     factored_ = false;
     MPIR_HIP(hipStreamSynchronize(stream_));
   }

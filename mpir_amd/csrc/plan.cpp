@@ -318,9 +318,9 @@ void build_factor_steps(FactorPlan& P, const FactorTuning& t) {
     }
     P.level_step_ptr[l + 1] = P.level_step_ptr[l] + max_steps;
   }
-  // the assembly lists are final once the plan is built (re-planning the
-  // steps of a built plan rebuilds the dependency-driven list too)
-  if (P.asm_child_ptr.size() == P.asm_task.size() + 1) build_factor_dag(P);
+  // the dependency-driven list follows the steps when it is in use (its
+  // pool layout is); the assembly lists are final once the plan is built
+  if (P.dag_pool && P.asm_child_ptr.size() == P.asm_task.size() + 1) build_factor_dag(P);
 }
 
 int32_t factor_dag_level(const FactorPlan& P) {
@@ -502,12 +502,11 @@ void build_factor_dag(FactorPlan& P) {
 void build_update_pool(FactorPlan& P, bool dag) {
   const Supernodes& S = P.sn;
   const int32_t ns = S.ns;
-  int32_t dag_level = S.nlevels;
-  if (dag) {
-    dag_level = 0;
-    for (int32_t s = 0; s < ns; ++s)
-      if (S.first[s + 1] - S.first[s] + S.nrow[s] <= kSmallFront) dag_level = std::max(dag_level, S.level[s] + 1);
-  }
+  int32_t dag_level = 0;
+  for (int32_t s = 0; s < ns; ++s)
+    if (S.first[s + 1] - S.first[s] + S.nrow[s] <= kSmallFront) dag_level = std::max(dag_level, S.level[s] + 1);
+  P.fdag_level = dag_level;
+  if (!dag) dag_level = S.nlevels;
   P.upd_off.assign(ns, -1);
   PoolAlloc pool;
   for (int32_t l = 0; l < S.nlevels; ++l) {
@@ -528,6 +527,15 @@ void build_update_pool(FactorPlan& P, bool dag) {
   }
   P.upd_pool = pool.peak();
   P.dag_pool = dag;
+  // the dependency-driven task list exists only for a pool laid out for it
+  // (built here once the assembly lists are final)
+  if (dag && P.asm_child_ptr.size() == P.asm_task.size() + 1) {
+    build_factor_dag(P);
+  } else if (!dag) {
+    P.fdag.clear();
+    P.fdag_meta.clear();
+    P.fdag_nsteps = P.fdag_ncb = 0;
+  }
 }
 
 FactorPlan build_plan_from_supernodes(Supernodes sn, const FactorTuning& ft, const SolveTuning& st) {
@@ -615,7 +623,6 @@ FactorPlan build_plan_from_supernodes(Supernodes sn, const FactorTuning& ft, con
     P.asm_task.swap(kept);
     P.asm_ptr.swap(kept_level);
   }
-  build_factor_dag(P);
 
   build_solve_schedule(P, st);
   return P;

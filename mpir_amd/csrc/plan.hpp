@@ -207,10 +207,11 @@ int32_t factor_dag_level(const FactorPlan& P);
 // upd_off, upd_pool and dag_pool of P's supernodes: blocks are released when
 // their parent's level has been assembled, below factor_dag_level only when
 // `dag` (the dependency-driven factorization runs fronts of several levels
-// at once).
+// at once); with `dag` also the dependency-driven task list (fdag),
+// without it that list is cleared. Sets fdag_level either way.
 void build_update_pool(FactorPlan& P, bool dag);
 // fdag, fdag_meta and the counter layout from the per-level factor lists
-// (assembly, panel and update tasks).
+// (assembly, panel and update tasks); build_update_pool(P, true) calls it.
 void build_factor_dag(FactorPlan& P);
 // Counter indices of the dependency-driven factorization (constexpr: the
 // kernels use the same functions).

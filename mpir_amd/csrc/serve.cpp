@@ -270,6 +270,7 @@ void Server::run(std::vector<ServeRecord>& reqs) {
     sum_.lane_busy_ms.assign(opt_.lanes, 0.0);
     t_run = now_ms();
   }
+  const int64_t grown = DevBuf::regrowths().load();
   x_.assign(opt_.keep_x ? reqs.size() : 0, std::vector<double>());
   // submit each request at its arrival time (a late submitter catches up)
   std::vector<int64_t> ids(reqs.size());
@@ -296,6 +297,7 @@ void Server::run(std::vector<ServeRecord>& reqs) {
   }
   std::lock_guard<std::mutex> lk(mu_);
   sum_.wall_ms = now_ms() - t_run;
+  sum_.regrowths = DevBuf::regrowths().load() - grown;
 }
 
 void Server::lane_main(int lane) {
@@ -397,9 +399,12 @@ void Server::process(int lane, const std::vector<ServeRequest*>& batch, int32_t 
   int status = 0;
   const double t1 = now_ms();
   double t2 = t1, t3 = t1;
+  double join = 0.0, upload = 0.0;
   try {
     s.analyze_batch(parts);
     t2 = now_ms();
+    join = s.batch_stats().join_ms;
+    upload = s.batch_stats().upload_ms;
     s.factor_batch(axp.data(), false);
     t3 = now_ms();
     s.solve_batch(nrhs.data(), bp.data(), xp.data(), false);
@@ -413,6 +418,8 @@ void Server::process(int lane, const std::vector<ServeRequest*>& batch, int32_t 
   std::lock_guard<std::mutex> lk(mu_);
   sum_.prepare_ms += t1 - t_prep;
   sum_.plan_ms += t2 - t1;
+  sum_.plan_join_ms += join;
+  sum_.plan_upload_ms += upload;
   sum_.factor_ms += t3 - t2;
   sum_.solve_ms += t4 - t3;
   const double td = t4 - t0_;

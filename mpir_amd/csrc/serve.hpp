@@ -97,6 +97,10 @@ struct ServeSummary {
   // time of every batch phase, summed over batches: request data, joined
   // symbolic plan and its upload, numeric factorization, GMRES-IR
   double prepare_ms = 0.0, plan_ms = 0.0, factor_ms = 0.0, solve_ms = 0.0;
+  double plan_join_ms = 0.0, plan_upload_ms = 0.0;  // the two parts of plan_ms
+  // device buffers of the process that had to grow during the run (each
+  // such reallocation waits for the whole device)
+  int64_t regrowths = 0;
 };
 
 class GmresLdl;

@@ -29,6 +29,8 @@ struct BatchStats {
   double factor_ms = 0.0, solve_ms = 0.0;
   int32_t arnoldi_steps = 0, precond_sweeps = 0;      // lock-step counts of the whole batch
   int32_t graph_nodes = 0;  // launches of the replayed HIP graph (0: the solve was launched kernel by kernel)
+  // analyze_batch: joining the systems' plans (host) / device copies, allocation and the wait for them
+  double join_ms = 0.0, upload_ms = 0.0;
 };
 
 // Device vectors of one batched solve (R right-hand-side slots of N rows;
@@ -279,6 +281,21 @@ class GmresLdl {
   void batch_enqueue(const BatchRun& br);
   void set_solve_slots(int slots);
   double* bpinned(size_t count);
+  // Host arrays to device buffers. While staging_ is set (a batch's
+  // uploads) the copies are recorded and flush_staged() runs them through
+  // one pinned buffer: host copies into it on several threads, then
+  // asynchronous DMA (the runtime stages copies from pageable memory one at
+  // a time, which serialised the serving lanes). The arrays must not change
+  // before the flush; the caller waits for the stream before the pinned
+  // buffer is reused.
+  template <class T>
+  void up(DevBuf& b, const std::vector<T>& v) {
+    b.alloc(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (v.empty()) return;
+    if (staging_) staged_.push_back({b.get(), v.data(), v.size() * sizeof(T)});
+    else MPIR_HIP(hipMemcpyAsync(b.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream_));
+  }
+  void flush_staged();
   // device-resident PrecondGmres (device vectors, permuted order), skipped
   // when *gate == 0 (gate may be null)
   void precond_gmres(const double* b, double* res, const int* gate);
@@ -290,6 +307,7 @@ class GmresLdl {
   double infnrm(const double* v);
   void release_numeric();
   void upload_plan();
+  void upload_factor_dag();
   // rebuilds plan_'s step and solve schedules for ftune_ / stune_ where they
   // differ from the plan's
   void apply_tuning();
@@ -390,6 +408,15 @@ class GmresLdl {
   int solve_slots_ = 0;  // right-hand sides the solve work areas hold
   double* b_pinned_ = nullptr;
   size_t b_pinned_count_ = 0;
+  struct Staged {
+    void* dst;
+    const void* src;
+    size_t bytes;
+  };
+  bool staging_ = false;
+  std::vector<Staged> staged_;
+  char* stage_pinned_ = nullptr;
+  size_t stage_cap_ = 0;
 };
 
 // x := M^{-1} x on the current device with the work memory of the solves

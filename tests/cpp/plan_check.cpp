@@ -501,8 +501,11 @@ int run() {
   if (base.fdag.size() < 10) fail("planted-mistake plan has no DAG levels");
   {
     // the level-lifetime pool reuses blocks inside the DAG levels
+    FactorPlan lev = base;
+    build_update_pool(lev, false);
+    if (!lev.fdag.empty()) fail("a level-lifetime pool kept the DAG list");
     FactorPlan bad = base;
-    build_update_pool(bad, false);
+    bad.upd_off = lev.upd_off;
     if (!caught(check_factor_dag, bad)) fail("a level-lifetime update pool was not caught");
   }
   {
