@@ -32,7 +32,7 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # HBM bytes one FP32 factorization of the 64^3 workload moves, measured with
 # rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE, the factors from a 1 GiB copy;
 # profiles/hbm_traffic_64cubed.md); a bench run cannot collect counters
-FACTOR_TRAFFIC_64 = 27.16e9
+FACTOR_TRAFFIC_64 = 23.88e9
 
 
 def log(msg: str) -> None:
@@ -589,7 +589,9 @@ def main(argv=None) -> int:
         out["fp64_factor"] = {"value": round(v64, 4), "bwd_err": r64["eta"],
                               "p50_ms": round(pct(r64["lat_ms"], 0.5), 3),
                               "ir_iterations": r64["stats"]["ir_iterations"],
-                              "gmres_iterations": r64["stats"]["gmres_iterations"]}
+                              "gmres_iterations": r64["stats"]["gmres_iterations"],
+                              "factor_ms": round(r64["stats"]["factor_ms"], 3),
+                              "solve_ms": round(r64["stats"]["solve_ms"], 3)}
         out["mixed_vs_fp64"] = round(value / v64, 4)
     # dominant kernel group of a solve: the numeric factorization (MFMA-bound
     # dense fronts); the M^-1 application and the SpMV (HBM-bound) alongside

@@ -425,7 +425,8 @@ class BatchLDLIR:
         self._s.SetSolveSchedule(schedule)
 
     def SetFactorSchedule(self, schedule: int) -> None:
-        """SCHED_DAG (default) or SCHED_LEVELS for the batch's factorization."""
+        """SCHED_LEVELS (default) or SCHED_DAG for the batch's factorization;
+        same factor bit for bit."""
         self._s.SetFactorSchedule(schedule)
 
     def Analyze(self, patterns) -> None:
