@@ -233,8 +233,8 @@ int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule);
 int mpir_gmres_ldl_set_amalgamation(mpir_gmres_ldl_t h, int level);
 /* schedule of the numeric factorization (no counterpart in the reference,
  * whose QDLDL_factor is a sequential loop over columns, qdldl.hpp:228-342):
- * MPIR_SCHED_LEVELS, one launch per level of the supernodal tree and per
- * panel step, or MPIR_SCHED_DAG (default), the levels above the last small
+ * MPIR_SCHED_LEVELS (default), one launch per level of the supernodal tree
+ * and per panel step, or MPIR_SCHED_DAG, the levels above the last small
  * front in one persistent launch whose tile tasks wait on per-front and
  * per-column-block completion counters. Both give bit-identical factors. */
 int mpir_gmres_ldl_set_factor_schedule(mpir_gmres_ldl_t h, int schedule);
