@@ -55,6 +55,20 @@ struct PlainMem {
     *p = v;
   }
 };
+// 16 bytes of consecutive entries: loaded from any address with the entry's
+// own alignment, stored to LDS at 16-byte aligned places
+template <class T>
+struct alignas(16) Pack16 {
+  static constexpr int N = 16 / sizeof(T);
+  T e[N];
+};
+template <class T>
+__device__ __forceinline__ Pack16<T> load_pack16(const T* p) {
+  Pack16<T> r;
+  __builtin_memcpy(&r, p, sizeof r);
+  return r;
+}
+
 struct CoherentMem {
   template <class T>
   static __device__ __forceinline__ T ld(const T* p) {
@@ -529,22 +543,51 @@ __device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk,
   const T* P = a.panel + a.panel_off[s];
   T* Pw = a.panel + a.panel_off[s];
   T* U = nr > 0 ? a.upool + a.upd_off[s] : Pw;
-  // w is wave-uniform (so are the D loads: scalar loads)
+  // w is wave-uniform (so are the D loads of the scalar path: scalar loads)
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // thread (row i = lane, columns t = w + 4q): all loads of a step issued
-  // before any is used (clamped addresses, values selected afterwards)
+  // Operand staging: all loads of a step issued before any is used (clamped
+  // addresses, values selected afterwards). Per-level launches (PlainMem)
+  // move 16-byte packs of PN consecutive rows of one column: thread (pack
+  // row i = PN (tid % PL), columns t = tid / PL + PC j), which fills the
+  // operand tiles with the same values as the entry-wise path of the
+  // dependency-driven launch, whose loads must be device-coherent. A pack
+  // starting inside the tile may run up to PN - 1 rows past its last row:
+  // those rows lie inside the column or in the pool's zeroed slack and are
+  // stored as zeros.
+  constexpr bool packed = std::is_same<M, PlainMem>::value;
+  using Pk = Pack16<T>;
+  constexpr int PN = Pk::N;            // rows per pack
+  constexpr int PL = kTile / PN;       // threads per column
+  constexpr int PC = 256 / PL;         // columns per pass of the workgroup
+  constexpr int NP = kPanelNB / PC;    // packs per thread and operand
+  const int pi = PN * (tid % PL), pt = tid / PL;
+  const int rap = r0 + pi, rbp = c0 + pi;
+  const int64_t rapS = rap < rlim ? rap : r0, rbpS = rbp < m ? rbp : c0;
+  Pk ap[packed ? NP : 1], bp[packed ? NP : 1];
+  T dp[packed ? NP : 1];
+  // entry-wise path: thread (row i = lane, columns t = w + 4q)
   const int ra = r0 + lane, rb = c0 + lane;
   const int64_t ras = ra < rlim ? ra : r0, rbs = rb < m ? rb : c0;
-  T av[16], bv[16], dv[16];
+  T av[packed ? 1 : 16], bv[packed ? 1 : 16], dv[packed ? 1 : 16];
   auto load = [&](int kk) {
     const int k0 = kk * kPanelNB;
     const int nbk = min(kPanelNB, nc - k0);
+    if constexpr (packed) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int t = min(w + 4 * q, nbk - 1);
-      av[q] = M::ld(P + ras + (k0 + t) * ld);
-      bv[q] = M::ld(P + rbs + (k0 + t) * ld);
-      dv[q] = M::ld(a.D + f + k0 + t);
+      for (int j = 0; j < NP; ++j) {
+        const int t = min(pt + PC * j, nbk - 1);
+        ap[j] = load_pack16(P + rapS + (k0 + t) * ld);
+        bp[j] = load_pack16(P + rbpS + (k0 + t) * ld);
+        dp[j] = a.D[f + k0 + t];
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int t = min(w + 4 * q, nbk - 1);
+        av[q] = M::ld(P + ras + (k0 + t) * ld);
+        bv[q] = M::ld(P + rbs + (k0 + t) * ld);
+        dv[q] = M::ld(a.D + f + k0 + t);
+      }
     }
   };
   const int wr = (w & 1) * 32, wc = (w >> 1) * 32;
@@ -559,11 +602,28 @@ __device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk,
   load(kfirst);
   for (int kk = kfirst; kk <= klast; ++kk) {
     const int nbk = min(kPanelNB, nc - kk * kPanelNB);
+    if constexpr (packed) {
+      // PN consecutive rows stay consecutive under the swizzle (it flips
+      // bits 4 and 5 of the row index only): one 16-byte LDS store each
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int t = w + 4 * q;
-      Ls[0][at(t, lane)] = (ra < rlim && t < nbk) ? av[q] : T(0);
-      Ls[1][at(t, lane)] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
+      for (int j = 0; j < NP; ++j) {
+        const int t = pt + PC * j;
+        Pk x, y;
+#pragma unroll
+        for (int e = 0; e < PN; ++e) {
+          x.e[e] = (rap + e < rlim && t < nbk) ? ap[j].e[e] : T(0);
+          y.e[e] = (rbp + e < m && t < nbk) ? bp[j].e[e] * dp[j] : T(0);
+        }
+        *reinterpret_cast<Pk*>(&Ls[0][at(t, pi)]) = x;
+        *reinterpret_cast<Pk*>(&Ls[1][at(t, pi)]) = y;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int t = w + 4 * q;
+        Ls[0][at(t, lane)] = (ra < rlim && t < nbk) ? av[q] : T(0);
+        Ls[1][at(t, lane)] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
+      }
     }
     __syncthreads();
     if (kk < klast) load(kk + 1);
@@ -611,7 +671,7 @@ __device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk,
 
 template <class T, int R>
 __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
-  __shared__ T Ls[2][kPanelNB * kTile];
+  __shared__ __align__(16) T Ls[2][kPanelNB * kTile];
   // Workgroups are dealt round-robin over the XCDs (blockIdx % kXcds); each
   // XCD takes a contiguous run of the task list instead, so tiles that share
   // operand rows and columns meet in the same L2.
