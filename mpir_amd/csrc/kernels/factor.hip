@@ -203,7 +203,30 @@ __device__ __forceinline__ void asm_body(const FactorArgs<T>& a, const Task4 tk,
   T* U = nr > 0 ? a.upool + a.upd_off[s] : P;
   const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
   const int r = t0 + i;
-  {
+  if constexpr (std::is_same<M, PlainMem>::value) {
+    // 16-byte packs of PN rows of one panel column: thread (rows pi ..,
+    // columns pt + PC j); a pack may run up to PN - 1 rows past the tile,
+    // inside the column or the pool's zeroed slack (those rows are not kept)
+    using Pk = Pack16<T>;
+    constexpr int PN = Pk::N, PL = E / PN, PC = 256 / PL, NP = E / PC;
+    const int pi = PN * (tid % PL), pt = tid / PL;
+    Pk v[NP];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int c = c0 + pt + PC * j;
+      v[j] = load_pack16(c < c1 && c < nc ? P + t0 + pi + c * ld : P);
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      const int c = c0 + pt + PC * j;
+#pragma unroll
+      for (int e = 0; e < PN; ++e) {
+        const int re = t0 + pi + e;
+        const bool in = c < c1 && re < t1 && re >= c && c < nc;
+        tile[pi + e][pt + PC * j] = in ? v[j].e[e] : T(0);
+      }
+    }
+  } else {
     T v[E / 4];
 #pragma unroll
     for (int q = 0; q < E / 4; ++q) {
