@@ -789,7 +789,9 @@ void GmresLdl::alloc_numeric() {
   MPIR_HIP(hipMemsetAsync(d_panel_.as<char>() + P.panel_entries * tf, 0, kPanelSlack * tf, stream_));
   if (uf_ == kF16 || uf_ == kBF16)
     MPIR_HIP(hipMemsetAsync(d_panel16_.as<char>() + P.panel_entries * 2, 0, kPanelSlack * 2, stream_));
-  d_upool_.alloc(std::max<int64_t>(1, P.upd_pool) * tf);
+  // the assembly's 16-byte loads of a child's update matrix may read up to
+  // 3 entries past the last one (those values are not used)
+  d_upool_.alloc((std::max<int64_t>(1, P.upd_pool) + kPanelSlack) * tf);
   d_D_.alloc(n * tf);
   d_Dinv_.alloc(n * tf);
 }
@@ -1450,7 +1452,7 @@ void GmresLdl::set_factor_schedule(int sched) {
     build_update_pool(*plan_, true);
     upload_vec(d_upd_off_, plan_->upd_off, stream_);
     const size_t tf = uf_ == kF64 ? sizeof(double) : sizeof(float);
-    d_upool_.alloc(std::max<int64_t>(1, plan_->upd_pool) * tf);
+    d_upool_.alloc((std::max<int64_t>(1, plan_->upd_pool) + kPanelSlack) * tf);
     upload_factor_dag();
     factored_ = false;
     MPIR_HIP(hipStreamSynchronize(stream_));

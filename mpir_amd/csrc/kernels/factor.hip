@@ -237,31 +237,68 @@ __device__ __forceinline__ void asm_body(const FactorArgs<T>& a, const Task4 tk,
     }
   }
   __syncthreads();
-  for (int e = e0; e < e1; ++e) {
-    const int ch = cl[5 * e], b0 = cl[5 * e + 1], b1 = cl[5 * e + 2], a0 = cl[5 * e + 3], a1 = cl[5 * e + 4];
-    const int nrc = a.nrow[ch];
-    const T* Uc = a.upool + a.upd_off[ch];
-    const int32_t* rel = a.relidx + a.rows_off[ch];
-    const int na = a1 - a0, npair = (b1 - b0) * na;
-    static_assert((E * E / 256) % KB == 0, "assembly rounds");
+  if constexpr (std::is_same<M, PlainMem>::value) {
+    // A child's block is at most E rows x E columns (its rows and columns
+    // map into the tile's, strictly increasing): thread (rows a0 + pi ..,
+    // columns b0 + pt + PC j) reads each column's PN rows as one 16-byte
+    // pack (up to PN - 1 rows past the block: inside the pool, which ends in
+    // slack; not used).
+    using Pk = Pack16<T>;
+    constexpr int PN = Pk::N, PL = E / PN, PC = 256 / PL, NP = E / PC;
+    const int pi = PN * (tid % PL), pt = tid / PL;
+    for (int e = e0; e < e1; ++e) {
+      const int ch = cl[5 * e], b0 = cl[5 * e + 1], b1 = cl[5 * e + 2], a0 = cl[5 * e + 3], a1 = cl[5 * e + 4];
+      const int nrc = a.nrow[ch];
+      const T* Uc = a.upool + a.upd_off[ch];
+      const int32_t* rel = a.relidx + a.rows_off[ch];
+      const int q0 = a0 + pi;
+      Pk v[NP];
+      int dc[NP], bq[NP];
 #pragma unroll
-    for (int k0 = 0; k0 < E * E / 256; k0 += KB) {
-      T v[KB];
-      int dr[KB], dc[KB];
-#pragma unroll
-      for (int k = 0; k < KB; ++k) {
-        const int p = tid + 256 * (k0 + k);
-        const int b = b0 + (p < npair ? p / na : 0);
-        const int q = a0 + (p < npair ? p % na : 0);
-        v[k] = M::ld(Uc + q + static_cast<int64_t>(b) * nrc);
-        dr[k] = (p < npair && q >= b) ? rel[q] - t0 : -1;
-        dc[k] = rel[b] - c0;
+      for (int j = 0; j < NP; ++j) {
+        const int b = b0 + pt + PC * j;
+        const int bs = b < b1 ? b : b0;
+        v[j] = load_pack16(Uc + (q0 < a1 ? q0 : a0) + static_cast<int64_t>(bs) * nrc);
+        dc[j] = rel[bs] - c0;
+        bq[j] = b < b1 ? b : nrc;  // a column past the block takes no row
       }
+      int dr[PN];
 #pragma unroll
-      for (int k = 0; k < KB; ++k)
-        if (dr[k] >= 0) tile[dr[k]][dc[k]] += v[k];
+      for (int u = 0; u < PN; ++u) dr[u] = q0 + u < a1 ? rel[min(q0 + u, a1 - 1)] - t0 : -1;
+#pragma unroll
+      for (int j = 0; j < NP; ++j)
+#pragma unroll
+        for (int u = 0; u < PN; ++u)
+          if (dr[u] >= 0 && q0 + u >= bq[j]) tile[dr[u]][dc[j]] += v[j].e[u];
+      __syncthreads();
     }
-    __syncthreads();
+  } else {
+    for (int e = e0; e < e1; ++e) {
+      const int ch = cl[5 * e], b0 = cl[5 * e + 1], b1 = cl[5 * e + 2], a0 = cl[5 * e + 3], a1 = cl[5 * e + 4];
+      const int nrc = a.nrow[ch];
+      const T* Uc = a.upool + a.upd_off[ch];
+      const int32_t* rel = a.relidx + a.rows_off[ch];
+      const int na = a1 - a0, npair = (b1 - b0) * na;
+      static_assert((E * E / 256) % KB == 0, "assembly rounds");
+#pragma unroll
+      for (int k0 = 0; k0 < E * E / 256; k0 += KB) {
+        T v[KB];
+        int dr[KB], dc[KB];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          const int p = tid + 256 * (k0 + k);
+          const int b = b0 + (p < npair ? p / na : 0);
+          const int q = a0 + (p < npair ? p % na : 0);
+          v[k] = M::ld(Uc + q + static_cast<int64_t>(b) * nrc);
+          dr[k] = (p < npair && q >= b) ? rel[q] - t0 : -1;
+          dc[k] = rel[b] - c0;
+        }
+#pragma unroll
+        for (int k = 0; k < KB; ++k)
+          if (dr[k] >= 0) tile[dr[k]][dc[k]] += v[k];
+      }
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int q = 0; q < E / 4; ++q) {
