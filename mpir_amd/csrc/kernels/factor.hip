@@ -585,9 +585,23 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
 // distinct LDS banks without padding (32 KB in FP32: five workgroups per CU).
 __device__ __forceinline__ int op_at(int t, int i) { return t * kTile + (i ^ ((t & 3) << 4)); }
 
-// Ls: 2 x kPanelNB x kTile entries of LDS.
+// Operand chunk width (k entries staged at a time) of the update tiles: 32
+// for the per-level launches (16.6 KB of LDS, the staging tile included,
+// so that more workgroups fit on a CU), 64 for the dependency-driven launch.
+template <class M>
+constexpr int upd_chunk() {
+  return std::is_same<M, PlainMem>::value ? 32 : kPanelNB;
+}
+// LDS of the update tiles: two operand chunks, or the kTile x (kTile + 1)
+// staging tile, whichever is larger
+template <class M>
+constexpr int upd_lds() {
+  return 2 * upd_chunk<M>() * kTile > kTile * (kTile + 1) ? 2 * upd_chunk<M>() * kTile : kTile * (kTile + 1);
+}
+
+// Ls: upd_lds<M>() entries of LDS (16-byte aligned).
 template <class T, int R, class M>
-__device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk, T (*Ls)[kPanelNB * kTile]) {
+__device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk, T* Ls) {
   using V = typename Mfma16<T>::V;
   auto at = [](int t, int i) { return op_at(t, i); };
   const int s = tk.a, r0 = tk.b, c0 = tk.c;
@@ -605,48 +619,55 @@ __device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk,
   T* U = nr > 0 ? a.upool + a.upd_off[s] : Pw;
   // w is wave-uniform (so are the D loads of the scalar path: scalar loads)
   const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // Operand staging: all loads of a step issued before any is used (clamped
-  // addresses, values selected afterwards). Per-level launches (PlainMem)
-  // move 16-byte packs of PN consecutive rows of one column: thread (pack
-  // row i = PN (tid % PL), columns t = tid / PL + PC j), which fills the
-  // operand tiles with the same values as the entry-wise path of the
-  // dependency-driven launch, whose loads must be device-coherent. A pack
-  // starting inside the tile may run up to PN - 1 rows past its last row:
-  // those rows lie inside the column or in the pool's zeroed slack and are
-  // stored as zeros.
+  // The steps' k range is staged KC columns at a time (chunk q: panel
+  // columns q KC ..), in order, so the products run over k in the same
+  // order for either chunk width. Operand staging: all loads of a chunk
+  // issued before any is used (clamped addresses, values selected
+  // afterwards). Per-level launches (PlainMem) move 16-byte packs of PN
+  // consecutive rows of one column: thread (pack row i = PN (tid % PL),
+  // columns t = tid / PL + PC j), which fills the operand tiles with the
+  // same values as the entry-wise path of the dependency-driven launch,
+  // whose loads must be device-coherent. A pack starting inside the tile
+  // may run up to PN - 1 rows past its last row: those rows lie inside the
+  // column or in the pool's zeroed slack and are stored as zeros.
   constexpr bool packed = std::is_same<M, PlainMem>::value;
+  constexpr int KC = upd_chunk<M>();
+  static_assert(kPanelNB % KC == 0 && KC % 4 == 0, "chunk width");
+  T* L0 = Ls;
+  T* L1 = Ls + KC * kTile;
   using Pk = Pack16<T>;
-  constexpr int PN = Pk::N;            // rows per pack
-  constexpr int PL = kTile / PN;       // threads per column
-  constexpr int PC = 256 / PL;         // columns per pass of the workgroup
-  constexpr int NP = kPanelNB / PC;    // packs per thread and operand
+  constexpr int PN = Pk::N;         // rows per pack
+  constexpr int PL = kTile / PN;    // threads per column
+  constexpr int PC = 256 / PL;      // columns per pass of the workgroup
+  constexpr int NP = KC / PC;       // packs per thread and operand
+  static_assert(!packed || KC % PC == 0, "pack passes");
   const int pi = PN * (tid % PL), pt = tid / PL;
   const int rap = r0 + pi, rbp = c0 + pi;
   const int64_t rapS = rap < rlim ? rap : r0, rbpS = rbp < m ? rbp : c0;
   Pk ap[packed ? NP : 1], bp[packed ? NP : 1];
   T dp[packed ? NP : 1];
-  // entry-wise path: thread (row i = lane, columns t = w + 4q)
+  // entry-wise path (KC = 64): thread (row i = lane, columns t = w + 4q)
   const int ra = r0 + lane, rb = c0 + lane;
   const int64_t ras = ra < rlim ? ra : r0, rbs = rb < m ? rb : c0;
   T av[packed ? 1 : 16], bv[packed ? 1 : 16], dv[packed ? 1 : 16];
-  auto load = [&](int kk) {
-    const int k0 = kk * kPanelNB;
-    const int nbk = min(kPanelNB, nc - k0);
+  auto load = [&](int q) {
+    const int k0 = q * KC;
+    const int nbq = min(KC, nc - k0);
     if constexpr (packed) {
 #pragma unroll
       for (int j = 0; j < NP; ++j) {
-        const int t = min(pt + PC * j, nbk - 1);
+        const int t = min(pt + PC * j, nbq - 1);
         ap[j] = load_pack16(P + rapS + (k0 + t) * ld);
         bp[j] = load_pack16(P + rbpS + (k0 + t) * ld);
         dp[j] = a.D[f + k0 + t];
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int t = min(w + 4 * q, nbk - 1);
-        av[q] = M::ld(P + ras + (k0 + t) * ld);
-        bv[q] = M::ld(P + rbs + (k0 + t) * ld);
-        dv[q] = M::ld(a.D + f + k0 + t);
+      for (int u = 0; u < 16; ++u) {
+        const int t = min(w + 4 * u, nbq - 1);
+        av[u] = M::ld(P + ras + (k0 + t) * ld);
+        bv[u] = M::ld(P + rbs + (k0 + t) * ld);
+        dv[u] = M::ld(a.D + f + k0 + t);
       }
     }
   };
@@ -659,9 +680,13 @@ __device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk,
     for (int y = 0; y < 2; ++y)
 #pragma unroll
       for (int v = 0; v < 4; ++v) acc[x][y][v] = T(0);
-  load(kfirst);
-  for (int kk = kfirst; kk <= klast; ++kk) {
-    const int nbk = min(kPanelNB, nc - kk * kPanelNB);
+  // chunks of the steps kfirst .. klast that hold columns of the front
+  constexpr int CPS = kPanelNB / KC;
+  const int qfirst = kfirst * CPS;
+  const int qlast = min(klast * CPS + CPS - 1, (nc - 1) / KC);
+  load(qfirst);
+  for (int q = qfirst; q <= qlast; ++q) {
+    const int nbq = min(KC, nc - q * KC);
     if constexpr (packed) {
       // PN consecutive rows stay consecutive under the swizzle (it flips
       // bits 4 and 5 of the row index only): one 16-byte LDS store each
@@ -671,26 +696,26 @@ __device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk,
         Pk x, y;
 #pragma unroll
         for (int e = 0; e < PN; ++e) {
-          x.e[e] = (rap + e < rlim && t < nbk) ? ap[j].e[e] : T(0);
-          y.e[e] = (rbp + e < m && t < nbk) ? bp[j].e[e] * dp[j] : T(0);
+          x.e[e] = (rap + e < rlim && t < nbq) ? ap[j].e[e] : T(0);
+          y.e[e] = (rbp + e < m && t < nbq) ? bp[j].e[e] * dp[j] : T(0);
         }
-        *reinterpret_cast<Pk*>(&Ls[0][at(t, pi)]) = x;
-        *reinterpret_cast<Pk*>(&Ls[1][at(t, pi)]) = y;
+        *reinterpret_cast<Pk*>(&L0[at(t, pi)]) = x;
+        *reinterpret_cast<Pk*>(&L1[at(t, pi)]) = y;
       }
     } else {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int t = w + 4 * q;
-        Ls[0][at(t, lane)] = (ra < rlim && t < nbk) ? av[q] : T(0);
-        Ls[1][at(t, lane)] = (rb < m && t < nbk) ? bv[q] * dv[q] : T(0);
+      for (int u = 0; u < 16; ++u) {
+        const int t = w + 4 * u;
+        L0[at(t, lane)] = (ra < rlim && t < nbq) ? av[u] : T(0);
+        L1[at(t, lane)] = (rb < m && t < nbq) ? bv[u] * dv[u] : T(0);
       }
     }
     __syncthreads();
-    if (kk < klast) load(kk + 1);
-    for (int t0 = 0; t0 < nbk; t0 += 4) {
-      const int t = t0 + lk;  // rows t >= nbk of the operand tiles are zero
-      const T a0 = Ls[0][at(t, wr + li)], a1 = Ls[0][at(t, wr + 16 + li)];
-      const T b0 = Ls[1][at(t, wc + li)], b1 = Ls[1][at(t, wc + 16 + li)];
+    if (q < qlast) load(q + 1);
+    for (int t0 = 0; t0 < nbq; t0 += 4) {
+      const int t = t0 + lk;  // rows t >= nbq of the operand tiles are zero
+      const T a0 = L0[at(t, wr + li)], a1 = L0[at(t, wr + 16 + li)];
+      const T b0 = L1[at(t, wc + li)], b1 = L1[at(t, wc + 16 + li)];
       acc[0][0] = Mfma16<T>::op(a0, b0, acc[0][0]);
       acc[0][1] = Mfma16<T>::op(a0, b1, acc[0][1]);
       acc[1][0] = Mfma16<T>::op(a1, b0, acc[1][0]);
@@ -703,8 +728,7 @@ __device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk,
   // column are conflict-free) so that the read-modify-write of the target is
   // coalesced: element (x, y, v) is row wr + 16x + row(lk, v), column
   // wc + 16y + li of the tile
-  static_assert(kTile * (kTile + 1) <= 2 * kPanelNB * kTile, "staging tile exceeds Ls");
-  T(*St)[kTile + 1] = reinterpret_cast<T(*)[kTile + 1]>(&Ls[0][0]);
+  T(*St)[kTile + 1] = reinterpret_cast<T(*)[kTile + 1]>(Ls);
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -731,7 +755,7 @@ __device__ __forceinline__ void upd_body(const FactorArgs<T>& a, const Task4 tk,
 
 template <class T, int R>
 __global__ void __launch_bounds__(256) k_update_tiles(FactorArgs<T> a, const Task4* __restrict__ tasks) {
-  __shared__ __align__(16) T Ls[2][kPanelNB * kTile];
+  __shared__ __align__(16) T Ls[upd_lds<PlainMem>()];
   // Workgroups are dealt round-robin over the XCDs (blockIdx % kXcds); each
   // XCD takes a contiguous run of the task list instead, so tiles that share
   // operand rows and columns meet in the same L2.
@@ -1027,7 +1051,8 @@ __device__ __forceinline__ void diag_body(const FactorArgs<T>& a, int s, int k, 
 // matrix cores, X = the rows' current entries in the step's columns (rows
 // below the diagonal block, or identity rows of earlier blocks), Z_kk from
 // the block's identity rows. Every output entry takes the MFMA sequence of
-// k_panel_step (products over t = 0, 4, 8, ... in order). Ls: as upd_body.
+// k_panel_step (products over t = 0, 4, 8, ... in order). Ls: 2 x kPanelNB x
+// kTile entries of LDS.
 template <class T, int R>
 __device__ __forceinline__ void pan_body(const FactorArgs<T>& a, int s, int r0, int r1, int k,
                                          T (*Ls)[kPanelNB * kTile]) {
@@ -1165,7 +1190,7 @@ __global__ void __launch_bounds__(256) k_factor_dag(FactorArgs<T> a, FactorDagDe
       if constexpr (std::is_same<T, float>::value && R != 0)
         upd16_body<R, CoherentMem>(a, ut, reinterpret_cast<typename Mfma16x32<R>::E(*)[kTile * kLs16]>(smem));
       else
-        upd_body<T, R, CoherentMem>(a, ut, reinterpret_cast<T(*)[kPanelNB * kTile]>(smem));
+        upd_body<T, R, CoherentMem>(a, ut, reinterpret_cast<T*>(smem));
       // counted on its column block (tiles that start on one), then towards
       // the front's completion, which the last tile reports to the parent
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
