@@ -1,11 +1,15 @@
 // Multifrontal LDL^T for quasi-definite matrices on gfx950.
 //
 // Numerically this is QDLDL_factor (include/MPIR/qdldl.hpp:172-345 of
-// yex33/MPIR): no pivoting, L unit lower triangular, D diagonal, each L entry
-// formed as y * Dinv with Dinv = 1 / D (qdldl.hpp:315, :339), and a zero
-// pivot reported as failure (qdldl.hpp:331-333). The loop order is different:
-// columns are grouped into supernodes and eliminated front by front, level by
-// level of the supernodal tree, which is what exposes parallelism.
+// yex33/MPIR): no pivoting, L unit lower triangular, D diagonal with
+// Dinv = 1 / D (qdldl.hpp:315), each L entry a scaled Schur complement entry
+// (qdldl.hpp:339: y * Dinv; small fronts do exactly that, big fronts form the
+// rows below a diagonal block as products with its explicit inverse and the
+// block's own multipliers with a Newton-refined reciprocal, recip_nr), and a
+// zero pivot reported as failure (qdldl.hpp:331-333). The loop order is
+// different: columns are grouped into supernodes and eliminated front by
+// front, level by level of the supernodal tree, which is what exposes
+// parallelism.
 //
 // Storage (per supernode s with nc columns and nr rows below the block,
 // m = nc + nr, ld = panel_ld(nc, nr)):
@@ -91,6 +95,24 @@ __device__ __forceinline__ T rnd(T v) {
   if constexpr (R == 1) return static_cast<T>(__half2float(__float2half_rn(static_cast<float>(v))));
   else if constexpr (R == 2) return static_cast<T>(__bfloat162float(__float2bfloat16(static_cast<float>(v))));
   else return v;
+}
+
+// 1 / d from the hardware reciprocal estimate refined by Newton-Raphson
+// steps x <- x + x (1 - d x) (one in FP32, two in FP64): within an ulp or so
+// of the correctly rounded quotient, at a fraction of a division's
+// dependent latency. The diagonal-block eliminations of big fronts form
+// their multipliers with it (the pivots of a block are a serial chain);
+// D^{-1} itself is stored as the quotient 1 / d.
+template <class T>
+__device__ __forceinline__ T recip_nr(T d) {
+  if constexpr (sizeof(T) == 4) {
+    const float x = __builtin_amdgcn_rcpf(d);
+    return __builtin_fmaf(__builtin_fmaf(-d, x, 1.f), x, x);
+  } else {
+    double x = __builtin_amdgcn_rcp(d);
+    x = __builtin_fma(__builtin_fma(-d, x, 1.0), x, x);
+    return __builtin_fma(__builtin_fma(-d, x, 1.0), x, x);
+  }
 }
 
 // ---------------------------------------------------------------- scatter
@@ -446,7 +468,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
           for (int q = 0; q < t; ++q) dt -= l[t][q] * dl[t][q];
         }
         d[t] = valid[t] ? dt : T(1);
-        dinv[t] = T(1) / d[t];
+        dinv[t] = recip_nr(d[t]);
 #pragma unroll
         for (int u = t + 1; u < CW; ++u) {
           T v = lane_value(fa[t], min(j0 + u, NB - 1));
@@ -955,7 +977,7 @@ __device__ __forceinline__ void diag_body(const FactorArgs<T>& a, int s, int k, 
 #pragma unroll
           for (int q = 0; q < t; ++q) dt -= l[t][q] * dl[t][q];
           d[t] = valid[t] ? dt : T(1);
-          dinv[t] = T(1) / d[t];
+          dinv[t] = recip_nr(d[t]);
 #pragma unroll
           for (int u = t + 1; u < CW; ++u) {
             T v = lane_value(fa[4 * jj + t], min(j0 + u, NB - 1));
