@@ -32,7 +32,7 @@ PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}
 # HBM bytes one FP32 factorization of the 64^3 workload moves, measured with
 # rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE, the factors from a 1 GiB copy;
 # profiles/hbm_traffic_64cubed.md); a bench run cannot collect counters
-FACTOR_TRAFFIC_64 = 23.88e9
+FACTOR_TRAFFIC_64 = 23.97e9
 
 
 def log(msg: str) -> None:
