@@ -421,10 +421,12 @@ struct RowTile {
   }
 };
 
-// The tile's row sums for threads 0..63 (other threads: 0). ys: kGemvCols
-// entries. red: 4 x 64 entries of LDS.
+// The tile's row sums, one right-hand side's part: each wave's sums of its
+// columns go to red[w][row] (4 x 64 entries of LDS, one such slot per
+// right-hand side, so that the parts of several right-hand sides need one
+// barrier between them and row_sum). ys: kGemvCols entries.
 template <class TF, class TX>
-__device__ __forceinline__ TX row_gemv(RowTile<TF>& T, const TX* ys, TX (*red)[kRB]) {
+__device__ __forceinline__ void row_gemv_part(RowTile<TF>& T, const TX* ys, TX (*red)[kRB]) {
   using RT = RowTile<TF>;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   TX acc[RT::N];
@@ -447,11 +449,14 @@ __device__ __forceinline__ TX row_gemv(RowTile<TF>& T, const TX* ys, TX (*red)[k
   for (int off = RT::LPC; off < 64; off <<= 1)
 #pragma unroll
     for (int e = 0; e < RT::N; ++e) acc[e] += __shfl_xor(acc[e], off);
-  __syncthreads();  // an earlier call's readers of red are done
   if (lane < RT::LPC)
 #pragma unroll
     for (int e = 0; e < RT::N; ++e) red[w][RT::N * lane + e] = acc[e];
-  __syncthreads();
+}
+// the row sums for threads 0..63 (other threads: 0), after a barrier that
+// follows row_gemv_part
+template <class TX>
+__device__ __forceinline__ TX row_sum(const TX (*red)[kRB]) {
   const int r = threadIdx.x;
   return r < kRB ? (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]) : TX(0);
 }
@@ -562,9 +567,11 @@ __device__ __forceinline__ void fu_task(const SolveArgs<TF, TD>& a, const SolveD
   __syncthreads();
   const int nh = (F.nc + kGemvCols - 1) / kGemvCols;
   TX* out = upart + d.upart_off[s] + static_cast<int64_t>(b) * nh * kRB + h * kRB;
-  TX (*red)[kRB] = reinterpret_cast<TX(*)[kRB]>(smem + NR * kVsLen);
+  TX (*red)[4][kRB] = reinterpret_cast<TX(*)[4][kRB]>(smem + NR * kVsLen);
+  each_rhs<NR>([&](int r) { row_gemv_part<TF, TX>(T, smem + r * kVsLen, red[r]); });
+  __syncthreads();
   each_rhs<NR>([&](int r) {
-    const TX sum = row_gemv<TF, TX>(T, smem + r * kVsLen, red);
+    const TX sum = row_sum<TX>(red[r]);
     if (tid < rows) st_dev(out + r * d.sup + tid, sum);
   });
   if (gathers)
@@ -641,9 +648,11 @@ __device__ __forceinline__ void bx_task(const SolveArgs<TF, TD>& a, const SolveD
   const int nh = (F.nc + kGemvCols - 1) / kGemvCols;
   const int h0 = r0 / kGemvCols;
   TX* xp = static_cast<TX*>(d.xpart) + d.xpart_off[s] + static_cast<int64_t>(i) * nh * kRB;
-  TX (*red)[kRB] = reinterpret_cast<TX(*)[kRB]>(smem + NR * kVsLen);
+  TX (*red)[4][kRB] = reinterpret_cast<TX(*)[4][kRB]>(smem + NR * kVsLen);
+  each_rhs<NR>([&](int r) { row_gemv_part<TF, TX>(T, smem + r * kVsLen, red[r]); });
+  __syncthreads();
   each_rhs<NR>([&](int r) {
-    const TX sum = row_gemv<TF, TX>(T, smem + r * kVsLen, red);
+    const TX sum = row_sum<TX>(red[r]);
     if (tid < rows) st_dev(xp + r * d.sxp + h * kRB + tid, sum);
   });
   if (publish_last(d.counters + d.cnt_bx + d.cb_base[s] + i, nh - h0, flag)) {
@@ -976,7 +985,7 @@ __global__ void __launch_bounds__(64) k_bwd_small(SolveArgs<TF, TD> a, SolveDev 
 template <int NR, class TF, class TD, class TX>
 __global__ void __launch_bounds__(256) k_level_tiles(SolveArgs<TF, TD> a, SolveDev d, int kind,
                                                      const int32_t* __restrict__ tasks, TX* x, TX* uvec) {
-  __shared__ TX smem[NR * kVsLen + 4 * kRB];
+  __shared__ TX smem[NR * (kVsLen + 4 * kRB)];
   __shared__ int flag;
   if (gated_off(d)) return;
   const int* q = tasks + 3 * blockIdx.x;
@@ -1024,7 +1033,7 @@ __device__ __forceinline__ void trace_end(unsigned long long* r) {
 // factor tile and the partial sums of one right-hand side stay in registers.
 template <int NR, class TF, class TD, class TX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 1 ? 4 : 2))) k_fwd_dag(SolveArgs<TF, TD> a, SolveDev d, TX* x, TX* uvec) {
-  __shared__ TX smem[NR * kVsLen + 4 * kRB];
+  __shared__ TX smem[NR * (kVsLen + 4 * kRB)];
   __shared__ int tsh, flag;
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
@@ -1064,7 +1073,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 
 
 template <int NR, class TF, class TD, class TX>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 1 ? 4 : 2))) k_bwd_dag(SolveArgs<TF, TD> a, SolveDev d, TX* x) {
-  __shared__ TX smem[NR * kVsLen + 4 * kRB];
+  __shared__ TX smem[NR * (kVsLen + 4 * kRB)];
   __shared__ int tsh, flag;
   const int w = threadIdx.x >> 6;
   int* cnt = d.counters;
