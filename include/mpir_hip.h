@@ -231,6 +231,13 @@ int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule);
  * elimination tree, for explicit zeros stored in the factor). Orderings AMD
  * and ND only. */
 int mpir_gmres_ldl_set_amalgamation(mpir_gmres_ldl_t h, int level);
+/* panel-step launches of the per-level factorization with at least
+ * min_tasks tasks (default 1024; 0: never) run as two launches: each
+ * front's diagonal block is eliminated once and its explicit inverse
+ * published, then the rows below read it back instead of eliminating the
+ * block again. The factor is the same bit for bit (no counterpart in the
+ * reference, whose QDLDL_factor works column by column). */
+int mpir_gmres_ldl_set_panel_split(mpir_gmres_ldl_t h, int min_tasks);
 /* schedule of the numeric factorization (no counterpart in the reference,
  * whose QDLDL_factor is a sequential loop over columns, qdldl.hpp:228-342):
  * MPIR_SCHED_LEVELS (default), one launch per level of the supernodal tree

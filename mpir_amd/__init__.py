@@ -229,6 +229,12 @@ class GmresLDLIR:
         fewer, larger fronts and a shallower tree for more stored zeros)."""
         _check(load().mpir_gmres_ldl_set_amalgamation(self._h, int(level)), "SetAmalgamation")
 
+    def SetPanelSplit(self, min_tasks: int) -> None:
+        """Panel-step launches of at least min_tasks tasks (default 1024; 0
+        never) eliminate each front's diagonal block once, in a launch of
+        their own; same factor bit for bit."""
+        _check(load().mpir_gmres_ldl_set_panel_split(self._h, int(min_tasks)), "SetPanelSplit")
+
     def SetFactorSchedule(self, schedule: int) -> None:
         """SCHED_LEVELS (default: one launch per level and panel step) or
         SCHED_DAG (the levels above the small fronts in one dependency-driven
@@ -428,6 +434,10 @@ class BatchLDLIR:
         """SCHED_LEVELS (default) or SCHED_DAG for the batch's factorization;
         same factor bit for bit."""
         self._s.SetFactorSchedule(schedule)
+
+    def SetPanelSplit(self, min_tasks: int) -> None:
+        """As GmresLDLIR.SetPanelSplit, for the batch's factorization."""
+        self._s.SetPanelSplit(min_tasks)
 
     def Analyze(self, patterns) -> None:
         """patterns: list of (Ap, Ai), upper CSC of each system."""

@@ -89,6 +89,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_set_schedule": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_factor_schedule": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_set_amalgamation": (C.c_int, [P, C.c_int]),
+    "mpir_gmres_ldl_set_panel_split": (C.c_int, [P, C.c_int]),
     "mpir_gmres_ldl_factor_schedule_info": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                                   C.POINTER(C.c_int64)]),
     "mpir_gmres_ldl_set_preconditioner": (C.c_int, [P, C.c_int, i32, i32]),

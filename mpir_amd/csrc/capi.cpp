@@ -627,6 +627,13 @@ int mpir_gmres_ldl_set_amalgamation(mpir_gmres_ldl_t h, int level) {
     return 0;
   });
 }
+int mpir_gmres_ldl_set_panel_split(mpir_gmres_ldl_t h, int min_tasks) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    h->s->set_panel_split(min_tasks);
+    return 0;
+  });
+}
 int mpir_gmres_ldl_set_factor_schedule(mpir_gmres_ldl_t h, int schedule) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;

@@ -680,6 +680,21 @@ void GmresLdl::upload_plan() {
   up(keep_[16], P.asm_child);
   dfp_.asm_child_ptr = keep_[15].as<int32_t>();
   dfp_.asm_child = keep_[16].as<int32_t>();
+  // split panel steps: a front's Z_kk slot is its place in its level's list
+  // of big fronts (levels run one after the other)
+  {
+    h_zslot_.assign(static_cast<size_t>(std::max(1, S.ns)), -1);
+    int32_t max_big = 0;
+    for (int32_t l = 0; l < S.nlevels; ++l) {
+      max_big = std::max(max_big, P.big_ptr[l + 1] - P.big_ptr[l]);
+      for (int32_t q = P.big_ptr[l]; q < P.big_ptr[l + 1]; ++q) h_zslot_[P.big_list[q]] = q - P.big_ptr[l];
+    }
+    const int64_t slot_bytes = static_cast<int64_t>(kPanelNB) * kPanelNB * (uf_ == kF64 ? 8 : 4);
+    dfp_.zbuf_fronts = static_cast<int>(std::min<int64_t>(max_big, kZbufMaxBytes / slot_bytes));
+    dfp_.pan_split_min = pan_split_min_;
+    up(d_zslot_, h_zslot_);
+    if (dfp_.zbuf_fronts > 0) d_zbuf_.ensure(static_cast<size_t>(dfp_.zbuf_fronts * slot_bytes));
+  }
   upload_factor_dag();
   // triangular solves
   dsp_ = DeviceSolvePlan();
@@ -947,6 +962,7 @@ void GmresLdl::factor_loaded(double t0) {
   // bit 2: a dependency wait of the factorization expired (its factor is
   // then incomplete); bit 1: an exact zero pivot
   if (err & 2) throw Error(kErrInternal, "a dependency wait of the LDL^T factorization expired");
+  if (err & 4) throw Error(kErrInternal, "a panel step found no slot for its diagonal block's inverse");
   if (batch_) batch_collect_pivots((err & 1) != 0);
   if (err & 1) throw Error(kErrNumeric, "zero pivot in the LDL^T factorization");
   // QDLDL_factor has no such check (qdldl.hpp:331 tests for an exact zero

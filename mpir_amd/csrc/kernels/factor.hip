@@ -384,10 +384,16 @@ __device__ __forceinline__ int xcd_task(int b, int n) {
 // (identity rows of the front, r >= m, enter at their own column as e_i).
 // refactor (template) = false: L_kk and D are already final (a factor loaded from
 // outside); the block is rebuilt from them and only Z is formed.
+// PM (launch split, refactor only): 0 every task as above; 1 only the tasks
+// that own a diagonal block, which also store Z_kk to the front's slot of
+// a.zbuf; 2 only the other tasks, with Z_kk read from that slot instead of
+// eliminated again (the same values: every task of a front eliminates the
+// same block with the same operations).
 constexpr int kPanelThreads = 1024;  // 16 waves, each owning 4 block columns
 
-template <class T, int R, bool refactor>
+template <class T, int R, bool refactor, int PM = 0>
 __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, const Task4* __restrict__ tasks) {
+  static_assert(PM == 0 || refactor, "split launches refactor");
   constexpr int NB = kPanelNB;
   constexpr int CW = NB / (kPanelThreads / 64);  // columns per wave (4)
   // per block step J: the multipliers of every row of the block (mrs), of
@@ -412,6 +418,8 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int c0w = CW * w;  // this thread's columns: c0w .. c0w + CW - 1
   const bool owner = r0 == k0;
+  if (PM == 1 && !owner) return;
+  if (PM == 2 && owner) return;
   const int rows = r1 - r0;
   // Loads are unconditional from clamped addresses, then selected: a load
   // under a branch is waited for before the branch joins.
@@ -433,15 +441,28 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
   const T* Dk = a.D + f + k0;
   const int lsafe = min(lane, nbk - 1);
   T fa[CW], ga[CW];
+  // the front's Z_kk slot (split launches): entry (row i, column c) at
+  // i + kPanelNB c
+  const int zi = PM != 0 ? a.zslot[s] : 0;
+  if (PM != 0 && (zi < 0 || zi >= a.zcap)) {  // no slot: reported, nothing stored
+    if (tid == 0) atomicOr(a.err, 4);
+    return;
+  }
+  T* zs = PM != 0 ? a.zbuf + static_cast<int64_t>(zi) * (NB * NB) : nullptr;
+  if constexpr (PM == 2) {
 #pragma unroll
-  for (int u = 0; u < CW; ++u) {
-    const int c = c0w + u;
-    const int cs = min(c, nbk - 1);
-    const T v = refactor ? P[(k0 + lsafe) + (k0 + cs) * ld] : P[(k0 + cs) + (k0 + lsafe) * ld];
-    const T dc = Dk[cs];
-    const bool in = lane < nbk && c < nbk && lane >= c;
-    fa[u] = !in ? T(0) : (refactor || lane != c) ? v : dc;
-    ga[u] = (lane == c && c < nbk) ? T(1) : T(0);
+    for (int u = 0; u < CW; ++u) ga[u] = zs[lane + (c0w + u) * NB];
+  } else {
+#pragma unroll
+    for (int u = 0; u < CW; ++u) {
+      const int c = c0w + u;
+      const int cs = min(c, nbk - 1);
+      const T v = refactor ? P[(k0 + lsafe) + (k0 + cs) * ld] : P[(k0 + cs) + (k0 + lsafe) * ld];
+      const T dc = Dk[cs];
+      const bool in = lane < nbk && c < nbk && lane >= c;
+      fa[u] = !in ? T(0) : (refactor || lane != c) ? v : dc;
+      ga[u] = (lane == c && c < nbk) ? T(1) : T(0);
+    }
   }
   // Blocked right-looking elimination, CW columns per barrier. Wave J owns
   // columns CW J .. CW J + CW - 1; when they are final it factors the CW x CW
@@ -450,7 +471,7 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
   // with d_t times the multipliers of every column, which by symmetry are
   // those of the row of the same index. After the barrier the waves right of
   // it apply the rank-CW update to their columns.
-  const int nstep = (nbk + CW - 1) / CW;
+  const int nstep = PM == 2 ? 0 : (nbk + CW - 1) / CW;
   for (int J = 0; J < nstep; ++J) {
     const int b = J & 1;
     const int j0 = CW * J;
@@ -548,6 +569,10 @@ __global__ void __launch_bounds__(kPanelThreads) k_panel_step(FactorArgs<T> a, c
   }
   if (owner) {
     if (!refactor) return;
+    if constexpr (PM == 1) {
+#pragma unroll
+      for (int u = 0; u < CW; ++u) zs[lane + (c0w + u) * NB] = ga[u];
+    }
     __syncthreads();
     // L_kk goes to the block's upper triangle, transposed (entry (j, r) holds
     // L[r, j]): the other workgroups of this launch read the lower triangle
@@ -1310,11 +1335,22 @@ void factor_levels(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const dou
       hipLaunchKernelGGL((k_assemble<T, R>), dim3(a1 - a0), dim3(256), 0, st, a, dp.asm_task + a0,
                          dp.asm_child_ptr + a0, dp.asm_child);
     const int st0 = dp.h_level_step_ptr[l], st1 = dp.h_level_step_ptr[l + 1];
+    const bool split_level = a.zbuf != nullptr && dp.h_big_ptr[l + 1] - dp.h_big_ptr[l] <= dp.zbuf_fronts;
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_ptr[q], p1 = dp.h_pan_ptr[q + 1];
-      if (p1 > p0)
+      if (split_level && dp.pan_split_min > 0 && p1 - p0 >= dp.pan_split_min) {
+        // the fronts' diagonal blocks (they are among the real rows, before
+        // the identity rows), then every other task with Z_kk read back
+        const int pa = dp.h_pan_aug_ptr[q];
+        if (pa > p0)
+          hipLaunchKernelGGL((k_panel_step<T, R, true, 1>), dim3(pa - p0), dim3(kPanelThreads), 0, st, a,
+                             dp.pan_task + p0);
+        hipLaunchKernelGGL((k_panel_step<T, R, true, 2>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a,
+                           dp.pan_task + p0);
+      } else if (p1 > p0) {
         hipLaunchKernelGGL((k_panel_step<T, R, true>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a,
                            dp.pan_task + p0);
+      }
       const int u0 = dp.h_upd_ptr[q], u1 = dp.h_upd_ptr[q + 1];
       if (u1 > u0) launch_update<T, R>(a, dp.upd_task + u0, u1 - u0, st);
     }

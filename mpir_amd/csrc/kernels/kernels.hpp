@@ -43,6 +43,11 @@ struct FactorArgs {
   T* D;
   T* Dinv;
   int* err;
+  // split panel steps: Z_kk of each front of a level (kPanelNB x kPanelNB,
+  // column-major), slot zslot[s] (the front's place in its level's list)
+  T* zbuf = nullptr;
+  const int32_t* zslot = nullptr;
+  int zcap = 0;  // slots in zbuf
 };
 
 // Device view of the dependency-driven factorization (plan.hpp, FTask):
@@ -62,6 +67,12 @@ struct FactorDagDev {
   unsigned long long* trace = nullptr;
 };
 
+// Panel-step launches of at least this many tasks are split (the fronts'
+// diagonal blocks eliminated once, DeviceFactorPlan::pan_split_min), when
+// the Z_kk blocks of the level fit in this many bytes.
+constexpr int kPanelSplitMin = 1024;
+constexpr int64_t kZbufMaxBytes = int64_t(1) << 30;
+
 // Device copies of the plan's task lists, with the host-side offsets the
 // launch loop needs.
 struct DeviceFactorPlan {
@@ -80,6 +91,11 @@ struct DeviceFactorPlan {
   // (s, column block, row chunk) tasks over the inverse rows of big fronts
   const int32_t* inv_task = nullptr;
   int n_inv_task = 0;
+  // panel-step launches of at least pan_split_min tasks (0: none) run as
+  // two launches, the fronts' diagonal blocks first (their Z_kk to
+  // FactorArgs::zbuf), then the other rows with Z_kk read back, when the
+  // level has at most zbuf_fronts big fronts
+  int pan_split_min = 0, zbuf_fronts = 0;
   // levels >= dag_level in one dependency-driven launch (when use_dag; the
   // update-matrix pool must then be laid out for it: dag_pool)
   bool use_dag = false, dag_pool = false;

@@ -148,6 +148,13 @@ class GmresLdl {
   }
   // supernode amalgamation: 0 none, 1 .. 3 (default; relax_supernodes);
   // takes effect at the next analyze
+  // panel-step launches of at least min_tasks tasks are split (0: never);
+  // the factor does not change
+  void set_panel_split(int min_tasks) {
+    MPIR_CHECK(min_tasks >= 0, "panel split threshold must be >= 0");
+    pan_split_min_ = min_tasks;
+    dfp_.pan_split_min = min_tasks;
+  }
   void set_relax(int level) {
     MPIR_CHECK(level >= 0 && level <= 3, "amalgamation level must be 0 .. 3");
     relax_ = level;
@@ -289,6 +296,8 @@ class GmresLdl {
   // before the flush; the caller waits for the stream before the pinned
   // buffer is reused.
   template <class T>
+  // While staging_ is set (a batch's analysis), the copy is made when the
+  // stage is flushed, from v's storage then: v must outlive flush_staged().
   void up(DevBuf& b, const std::vector<T>& v) {
     b.alloc(std::max<size_t>(1, v.size()) * sizeof(T));
     if (v.empty()) return;
@@ -316,7 +325,9 @@ class GmresLdl {
     return FactorArgs<T>{d_first_.as<int32_t>(), d_nrow_.as<int32_t>(), d_rows_off_.as<int64_t>(),
                          d_rows_.as<int32_t>(), d_relidx_.as<int32_t>(), d_panel_off_.as<int64_t>(),
                          d_upd_off_.as<int64_t>(), d_child_ptr_.as<int32_t>(), d_child_list_.as<int32_t>(),
-                         d_panel_.as<T>(), d_upool_.as<T>(), d_D_.as<T>(), d_Dinv_.as<T>(), d_err_.as<int>()};
+                         d_panel_.as<T>(), d_upool_.as<T>(), d_D_.as<T>(), d_Dinv_.as<T>(), d_err_.as<int>(),
+                         dfp_.zbuf_fronts > 0 ? d_zbuf_.as<T>() : nullptr, d_zslot_.as<int32_t>(),
+                         dfp_.zbuf_fronts};
   }
   template <class TF, class TD>
   SolveArgs<TF, TD> solve_args(void* panel) {
@@ -342,6 +353,7 @@ class GmresLdl {
   FactorTuning ftune_;
   SolveTuning stune_;
   int relax_ = kDefaultAmalgamation;  // relaxed supernodes, level 0 .. 3 (AMD / ND orderings only)
+  int pan_split_min_ = kPanelSplitMin;
   bool relaxed_ = false;  // the current plan uses them
   std::vector<int32_t> Cp_, Ci_, parent_, Lnz_;  // pattern of P K P^T, etree
   int64_t true_nnz_l_ = 0;
@@ -365,6 +377,10 @@ class GmresLdl {
   DeviceSolvePlan dsp_;
   std::vector<DevBuf> keep_;  // device copies of plan arrays
   DevBuf d_first_, d_nrow_, d_rows_off_, d_rows_, d_relidx_, d_panel_off_, d_upd_off_, d_child_ptr_, d_child_list_;
+  DevBuf d_zbuf_, d_zslot_;  // split panel steps: Z_kk slots, a front's slot
+  // host copy of the slot map: a staged upload reads it when the stage is
+  // flushed, so it must outlive upload_plan
+  std::vector<int32_t> h_zslot_;
   DevBuf d_csr_p_, d_csr_j_, d_csr_src_, d_csr_x_;
   DevBuf d_entry_col_, d_entry_row_, d_diag_pos_, d_perm_;
   // numeric

@@ -73,6 +73,24 @@ def test_batch_member_equals_alone_bitwise(gpu):
     assert xs[4] is None
 
 
+def test_batch_split_panel_steps_match_bitwise(gpu):
+    # the batch's factorization with every panel-step launch split gives the
+    # solutions of single launches bit for bit
+    systems = small_batch(nsys=6, seed=7)
+    counts = [2, 1, 3, 1, 2, 1]
+    rhs = [q.B[:, :c] for q, c in zip(systems, counts)]
+    s = mpir_amd.BatchLDLIR(np.float32)
+    s.SetTolerance(1e-12)
+    s.SetPanelSplit(0)
+    s.Compute([(q.kkt.Ap, q.kkt.Ai, q.kkt.Ax) for q in systems])
+    x0 = s.Solve(rhs)
+    s.SetPanelSplit(1)
+    s.Factor([q.kkt.Ax for q in systems])
+    x1 = s.Solve(rhs)
+    for a, b in zip(x0, x1):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_batch_matches_single_system_solver(gpu, oracle):
     systems = small_batch(nsys=3, nrhs=1, seed=3)
     _, xs, st = solve(systems, [1, 1, 1], tol=1e-10)

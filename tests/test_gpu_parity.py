@@ -265,6 +265,27 @@ def test_dag_schedule_matches_level_schedule_bitwise(gpu, oracle, uf, k, order):
         assert backward_error(oracle, P.Ap, P.Ai, P.Ax, x_dag, P.b) < 1e-13
 
 
+@pytest.mark.parametrize("uf,k", [(np.float64, 24), (np.float32, 32), (np.float16, 16)])
+def test_split_panel_steps_match_bitwise(gpu, uf, k):
+    # panel-step launches split into the fronts' diagonal blocks and the rows
+    # that read the blocks' inverses back give the factor of single launches,
+    # bit for bit (threshold 1: every launch split; 0: none)
+    P = mpir_amd.kkt_config(1, k=k)
+    s = mpir_amd.GmresLDLIR(uf, ordering=mpir_amd.ORDER_ND)
+    s.SetPanelSplit(0)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    f0 = s.export_factor()
+    x0 = s.Solve(P.b)
+    s.SetPanelSplit(1)
+    for _ in range(2):
+        s.Factor(P.Ax)
+        for a, b in zip(s.export_factor(), f0):
+            np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(s.Solve(P.b), x0)
+    with pytest.raises(mpir_amd.MpirError):
+        s.SetPanelSplit(-1)
+
+
 def test_bad_schedule_rejected(gpu):
     s = mpir_amd.GmresLDLIR(np.float64)
     with pytest.raises(mpir_amd.MpirError):
