@@ -232,7 +232,7 @@ int mpir_gmres_ldl_set_schedule(mpir_gmres_ldl_t h, int schedule);
  * and ND only. */
 int mpir_gmres_ldl_set_amalgamation(mpir_gmres_ldl_t h, int level);
 /* panel-step launches of the per-level factorization with at least
- * min_tasks tasks (default 1024; 0: never) run as two launches: each
+ * min_tasks tasks (default 512; 0: never) run as two launches: each
  * front's diagonal block is eliminated once and its explicit inverse
  * published, then the rows below read it back instead of eliminating the
  * block again. The factor is the same bit for bit (no counterpart in the

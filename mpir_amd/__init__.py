@@ -230,7 +230,7 @@ class GmresLDLIR:
         _check(load().mpir_gmres_ldl_set_amalgamation(self._h, int(level)), "SetAmalgamation")
 
     def SetPanelSplit(self, min_tasks: int) -> None:
-        """Panel-step launches of at least min_tasks tasks (default 1024; 0
+        """Panel-step launches of at least min_tasks tasks (default 512; 0
         never) eliminate each front's diagonal block once, in a launch of
         their own; same factor bit for bit."""
         _check(load().mpir_gmres_ldl_set_panel_split(self._h, int(min_tasks)), "SetPanelSplit")

@@ -70,7 +70,7 @@ struct FactorDagDev {
 // Panel-step launches of at least this many tasks are split (the fronts'
 // diagonal blocks eliminated once, DeviceFactorPlan::pan_split_min), when
 // the Z_kk blocks of the level fit in this many bytes.
-constexpr int kPanelSplitMin = 1024;
+constexpr int kPanelSplitMin = 512;
 constexpr int64_t kZbufMaxBytes = int64_t(1) << 30;
 
 // Device copies of the plan's task lists, with the host-side offsets the
