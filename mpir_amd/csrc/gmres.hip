@@ -693,6 +693,26 @@ void GmresLdl::upload_plan() {
     dfp_.zbuf_fronts = static_cast<int>(std::min<int64_t>(max_big, kZbufMaxBytes / slot_bytes));
     dfp_.pan_split_min = pan_split_min_;
     up(d_zslot_, h_zslot_);
+    // each step's tasks owning a diagonal block (among its real rows: r0 = k0)
+    // and the others, as lists of their own for the two launches of a split step
+    h_pan_own_.clear();
+    h_pan_rest_.clear();
+    dfp_.h_pan_own_ptr.assign(1, 0);
+    dfp_.h_pan_rest_ptr.assign(1, 0);
+    const int32_t nsteps = static_cast<int32_t>(P.pan_ptr.size()) - 1;
+    for (int32_t q = 0; q < nsteps; ++q) {
+      for (int32_t t = P.pan_ptr[q]; t < P.pan_ptr[q + 1]; ++t) {
+        const Task4 tk = P.pan_task[t];
+        if (t < P.pan_aug_ptr[q] && tk.b == tk.d * kPanelNB) h_pan_own_.push_back(tk);
+        else h_pan_rest_.push_back(tk);
+      }
+      dfp_.h_pan_own_ptr.push_back(static_cast<int32_t>(h_pan_own_.size()));
+      dfp_.h_pan_rest_ptr.push_back(static_cast<int32_t>(h_pan_rest_.size()));
+    }
+    up(d_pan_own_, h_pan_own_);
+    up(d_pan_rest_, h_pan_rest_);
+    dfp_.pan_own = d_pan_own_.as<Task4>();
+    dfp_.pan_rest = d_pan_rest_.as<Task4>();
     if (dfp_.zbuf_fronts > 0) d_zbuf_.ensure(static_cast<size_t>(dfp_.zbuf_fronts * slot_bytes));
   }
   upload_factor_dag();

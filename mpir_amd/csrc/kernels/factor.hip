@@ -1339,14 +1339,16 @@ void factor_levels(const FactorArgs<T>& a, const DeviceFactorPlan& dp, const dou
     for (int q = st0; q < st1; ++q) {
       const int p0 = dp.h_pan_ptr[q], p1 = dp.h_pan_ptr[q + 1];
       if (split_level && dp.pan_split_min > 0 && p1 - p0 >= dp.pan_split_min) {
-        // the fronts' diagonal blocks (they are among the real rows, before
-        // the identity rows), then every other task with Z_kk read back
-        const int pa = dp.h_pan_aug_ptr[q];
-        if (pa > p0)
-          hipLaunchKernelGGL((k_panel_step<T, R, true, 1>), dim3(pa - p0), dim3(kPanelThreads), 0, st, a,
-                             dp.pan_task + p0);
-        hipLaunchKernelGGL((k_panel_step<T, R, true, 2>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a,
-                           dp.pan_task + p0);
+        // the fronts' diagonal blocks, then every other task with Z_kk read
+        // back (each from the step's own list)
+        const int o0 = dp.h_pan_own_ptr[q], o1 = dp.h_pan_own_ptr[q + 1];
+        const int e0 = dp.h_pan_rest_ptr[q], e1 = dp.h_pan_rest_ptr[q + 1];
+        if (o1 > o0)
+          hipLaunchKernelGGL((k_panel_step<T, R, true, 1>), dim3(o1 - o0), dim3(kPanelThreads), 0, st, a,
+                             dp.pan_own + o0);
+        if (e1 > e0)
+          hipLaunchKernelGGL((k_panel_step<T, R, true, 2>), dim3(e1 - e0), dim3(kPanelThreads), 0, st, a,
+                             dp.pan_rest + e0);
       } else if (p1 > p0) {
         hipLaunchKernelGGL((k_panel_step<T, R, true>), dim3(p1 - p0), dim3(kPanelThreads), 0, st, a,
                            dp.pan_task + p0);

@@ -96,6 +96,11 @@ struct DeviceFactorPlan {
   // FactorArgs::zbuf), then the other rows with Z_kk read back, when the
   // level has at most zbuf_fronts big fronts
   int pan_split_min = 0, zbuf_fronts = 0;
+  // per step, the tasks that own a diagonal block and all others, for the
+  // two launches of a split step (offsets per step into each list)
+  const Task4* pan_own = nullptr;
+  const Task4* pan_rest = nullptr;
+  std::vector<int32_t> h_pan_own_ptr, h_pan_rest_ptr;
   // levels >= dag_level in one dependency-driven launch (when use_dag; the
   // update-matrix pool must then be laid out for it: dag_pool)
   bool use_dag = false, dag_pool = false;

@@ -381,6 +381,10 @@ class GmresLdl {
   // host copy of the slot map: a staged upload reads it when the stage is
   // flushed, so it must outlive upload_plan
   std::vector<int32_t> h_zslot_;
+  // split panel steps: each step's owner tasks and other tasks (uploaded
+  // from these members, which outlive a staged flush)
+  std::vector<Task4> h_pan_own_, h_pan_rest_;
+  DevBuf d_pan_own_, d_pan_rest_;
   DevBuf d_csr_p_, d_csr_j_, d_csr_src_, d_csr_x_;
   DevBuf d_entry_col_, d_entry_row_, d_diag_pos_, d_perm_;
   // numeric
