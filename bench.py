@@ -460,6 +460,57 @@ def serving(mpir, args) -> int:
     return 0
 
 
+TASK_KINDS = {0: "small", 1: "fx", 2: "fu", 3: "br", 4: "bx", 5: "fm", 6: "bm"}
+
+
+def precond_timeline(mpir, args) -> int:
+    """Where one M^-1 application's time goes, per task kind and per level of
+    the supernodal tree: the device task timeline of the dependency-driven
+    solves (trace_precond) joined with the plan's supernode levels. Prints a
+    table per direction, then one JSON line with the spans."""
+    P = mpir.kkt_config(1, k=args.k)
+    order = {"amd": mpir.ORDER_AMD, "nd": mpir.ORDER_ND}[args.ordering]
+    s = mpir.GmresLDLIR(np.float32, ordering=order)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    ms = s.time_kernels("precond", 20)
+    s.trace_precond()  # the first traced application warms the trace buffer
+    tr, nf = s.trace_precond()
+    sn = s.supernode_info()
+    kind = (tr[:, 0] & 15).astype(int)
+    sup = (tr[:, 0] >> 4).astype(int)
+    drawn = tr[:, 1].astype(np.int64)
+    waited = tr[:, 2].astype(np.int64)
+    done = (tr[:, 3] >> 16).astype(np.int64)
+    wg = (tr[:, 3] & 0xFFFF).astype(int)
+    out = {"precond_ms_events": round(ms, 4), "tasks": int(len(tr)), "forward_tasks": int(nf)}
+    for part, sl in (("forward", slice(0, nf)), ("backward", slice(nf, len(tr)))):
+        k, sp, dr, wt, dn = kind[sl], sup[sl], drawn[sl], waited[sl], done[sl]
+        t0 = dr.min()
+        span = (dn.max() - t0) / 100.0
+        run = (dn - wt) / 100.0
+        busy = run.sum() / (span * (wg[sl].max() + 1))
+        print(f"## {part}: span {span:.1f} us, {len(k)} tasks, {wg[sl].max() + 1} workgroups, "
+              f"workgroups running a task {100 * busy:.0f} % of the span", flush=True)
+        print("| kind | tasks | wait mean us | run mean us | run p90 us | run sum us |\n|---|---:|---:|---:|---:|---:|")
+        for kk in sorted(set(k)):
+            m = k == kk
+            print(f"| {TASK_KINDS[kk]} | {m.sum()} | {((wt[m] - dr[m]) / 100).mean():.2f} | {run[m].mean():.2f} | "
+                  f"{np.percentile(run[m], 90):.2f} | {run[m].sum():.1f} |")
+        print("\n| level | tasks | first start us | last end us | level span us | run sum us | max cols | max rows |\n"
+              "|---:|---:|---:|---:|---:|---:|---:|---:|")
+        lev = sn["level"][sp]
+        for lv in sorted(set(lev)):
+            m = lev == lv
+            a, b = (wt[m].min() - t0) / 100.0, (dn[m].max() - t0) / 100.0
+            print(f"| {lv} | {m.sum()} | {a:.1f} | {b:.1f} | {b - a:.1f} | {run[m].sum():.1f} | "
+                  f"{sn['ncols'][sp[m]].max()} | {sn['nrows'][sp[m]].max()} |")
+        print(flush=True)
+        out[part + "_span_us"] = round(span, 1)
+        out[part + "_busy"] = round(float(busy), 3)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -487,6 +538,8 @@ def main(argv=None) -> int:
                     help="the reference's shipped FGPILU preconditioner on the workload (one JSON line per case)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only a 1 GiB device-to-device copy (x3): the reference kernel for HBM counter scaling")
+    ap.add_argument("--precond-timeline", action="store_true",
+                    help="task timeline of one M^-1 application per kind and tree level (FP32 factor)")
     ap.add_argument("--serve", action="store_true",
                     help="serving form instead of the bench line: Poisson request stream with continuous batching")
     ap.add_argument("--rate", type=float, default=2000.0, help="offered request rate of the serving form (1/s)")
@@ -524,6 +577,8 @@ def main(argv=None) -> int:
         return fgpilu_form(mpir, torch, args)
     if args.serve:
         return serving(mpir, args)
+    if args.precond_timeline:
+        return precond_timeline(mpir, args)
     log(f"generating the {args.k}^3 KKT workload")
     P = mpir.kkt_config(1, k=args.k)
     dev = torch.device("cuda:0")

@@ -324,6 +324,12 @@ int mpir_gmres_ldl_trace_factor(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap, 
  * (<< 16 | workgroup). Writes at most cap / 4 tasks; *ntask and *nfwd receive
  * the number of tasks and of forward tasks. */
 int mpir_gmres_ldl_trace_precond(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap, int64_t* ntask, int64_t* nfwd);
+/* Per supernode of the analyzed plan, in the indices the two traces record
+ * (no counterpart in the reference): its level in the supernodal tree (height
+ * above the leaves), its columns and its rows below the diagonal block.
+ * Writes them when cap >= the number of supernodes, which *ns receives. */
+int mpir_gmres_ldl_supernode_info(mpir_gmres_ldl_t h, int32_t* level, int32_t* ncols, int32_t* nrows, int32_t cap,
+                                  int32_t* ns);
 /* Autotuner of the level-tile schedules (no counterpart in the reference,
  * whose solver has no device schedule). On a factored L D L^T handle (single
  * system or batch) it times, with HIP events over `reps` runs (1..100) each:

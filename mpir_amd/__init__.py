@@ -359,6 +359,17 @@ class GmresLDLIR:
                "trace_precond")
         return out[:nt.value], nf.value
 
+    def supernode_info(self) -> dict:
+        """Per supernode of the analyzed plan, in the indices the traces record:
+        'level' (height above the leaves in the supernodal tree), 'ncols',
+        'nrows' (rows below the diagonal block); int32 arrays."""
+        ns = C.c_int32()
+        _check(load().mpir_gmres_ldl_supernode_info(self._h, None, None, None, 0, C.byref(ns)), "supernode_info")
+        out = {k: np.zeros(max(ns.value, 1), dtype=np.int32) for k in ("level", "ncols", "nrows")}
+        _check(load().mpir_gmres_ldl_supernode_info(self._h, _ptr(out["level"]), _ptr(out["ncols"]),
+                                                    _ptr(out["nrows"]), ns.value, C.byref(ns)), "supernode_info")
+        return {k: v[:ns.value] for k, v in out.items()}
+
     def time_kernels(self, what: str, reps: int = 10) -> float:
         """Average ms of one kernel group ('spmv', 'precond', 'factor') via HIP events."""
         code = {"spmv": 0, "precond": 1, "factor": 2, "precond_multi": 3}[what]

@@ -114,6 +114,7 @@ SIGNATURES = {
     "mpir_gmres_ldl_time": (C.c_int, [P, C.c_int, C.c_int, P]),
     "mpir_gmres_ldl_trace_factor": (C.c_int, [P, P, C.c_int64, C.POINTER(C.c_int64)]),
     "mpir_gmres_ldl_trace_precond": (C.c_int, [P, P, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "mpir_gmres_ldl_supernode_info": (C.c_int, [P, P, P, P, C.c_int32, C.POINTER(C.c_int32)]),
     "mpir_gmres_ldl_autotune": (C.c_int, [P, C.c_int, P, P]),
     "mpir_gmres_ldl_set_tuning": (C.c_int, [P, i32, i32, i32]),
     "mpir_batch_graph": (C.c_int, [P, C.c_int, P]),

@@ -1057,6 +1057,15 @@ int mpir_gmres_ldl_trace_precond(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap,
     return 0;
   });
 }
+int mpir_gmres_ldl_supernode_info(mpir_gmres_ldl_t h, int32_t* level, int32_t* ncols, int32_t* nrows, int32_t cap,
+                                  int32_t* ns) {
+  return guarded([&] {
+    MPIR_HANDLE_CHECK;
+    MPIR_CHECK(ns && cap >= 0 && (cap == 0 || (level && ncols && nrows)), "bad supernode buffers");
+    *ns = h->s->supernode_info(level, ncols, nrows, cap);
+    return 0;
+  });
+}
 int mpir_gmres_ldl_trace_factor(mpir_gmres_ldl_t h, uint64_t* out, int64_t cap, int64_t* ntask) {
   return guarded([&] {
     MPIR_HANDLE_CHECK;

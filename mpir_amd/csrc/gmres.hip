@@ -1377,6 +1377,19 @@ int64_t GmresLdl::trace_factor(uint64_t* out, int64_t cap) {
   return nt;
 }
 
+int32_t GmresLdl::supernode_info(int32_t* level, int32_t* ncols, int32_t* nrows, int32_t cap) const {
+  MPIR_CHECK(analyzed_ && plan_ != nullptr && precond_ == kPrecLdl, "supernode_info needs an analyzed L D L^T handle");
+  const Supernodes& S = plan_->sn;
+  if (cap >= S.ns) {
+    for (int32_t s = 0; s < S.ns; ++s) {
+      level[s] = S.level[s];
+      ncols[s] = S.first[s + 1] - S.first[s];
+      nrows[s] = S.nrow[s];
+    }
+  }
+  return S.ns;
+}
+
 int64_t GmresLdl::trace_precond(uint64_t* out, int64_t cap, int64_t* nfwd) {
   MPIR_CHECK(factored_ && precond_ == kPrecLdl, "trace_precond needs a factored L D L^T handle");
   MPIR_CHECK(dsp_.schedule == kScheduleDag, "trace_precond traces the dependency-driven schedule");

@@ -255,6 +255,10 @@ class GmresLdl {
   // timeline of both passes (4 entries per task, forward then backward) in
   // out[0 .. cap); returns the number of tasks, *nfwd those of the forward pass
   int64_t trace_precond(uint64_t* out, int64_t cap, int64_t* nfwd);
+  // per supernode of the analyzed plan (the indices the traces record): its
+  // level in the supernodal tree, columns and rows below its diagonal block,
+  // written when cap >= the number of supernodes; returns that number
+  int32_t supernode_info(int32_t* level, int32_t* ncols, int32_t* nrows, int32_t cap) const;
   // Times the factor's update grouping and super-block edge and the
   // triangular solves' narrow-front limit and schedule on this handle's
   // factor (reps runs per candidate), keeps the fastest of each and leaves

@@ -286,6 +286,28 @@ def test_split_panel_steps_match_bitwise(gpu, uf, k):
         s.SetPanelSplit(-1)
 
 
+def test_supernode_info_and_precond_trace(gpu):
+    """The plan's supernodes (level, columns, rows) cover the matrix, and the
+    M^-1 task trace names only those supernodes, in time order per task."""
+    s = mpir_amd.GmresLDLIR(np.float32, ordering=mpir_amd.ORDER_ND)
+    with pytest.raises(mpir_amd.MpirError):
+        s.supernode_info()  # nothing analyzed yet
+    P = mpir_amd.kkt_config(1, k=16)
+    s.Compute(P.Ap, P.Ai, P.Ax)
+    st = s.stats()
+    sn = s.supernode_info()
+    assert len(sn["level"]) == st["nsupernodes"] > 1
+    assert int(sn["ncols"].sum()) == P.n and (sn["ncols"] >= 1).all() and (sn["nrows"] >= 0).all()
+    assert sn["level"].min() == 0 and sn["level"].max() == st["nlevels"] - 1
+    assert (sn["nrows"][sn["level"] == st["nlevels"] - 1] == 0).any()  # a root
+    tr, nf = s.trace_precond()
+    assert 0 < nf < len(tr)
+    sup = (tr[:, 0] >> 4).astype(np.int64)
+    drawn, waited, done = tr[:, 1].astype(np.int64), tr[:, 2].astype(np.int64), (tr[:, 3] >> 16).astype(np.int64)
+    assert (sup < st["nsupernodes"]).all()
+    assert (drawn <= waited).all() and (waited <= done).all()
+
+
 def test_bad_schedule_rejected(gpu):
     s = mpir_amd.GmresLDLIR(np.float64)
     with pytest.raises(mpir_amd.MpirError):
